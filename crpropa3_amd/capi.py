@@ -1,0 +1,240 @@
+"""ctypes binding of the C ABI in include/crpropa3_amd.h.
+
+This is the thin layer a Python caller (or the reference's steering code)
+uses to reach the HIP library.  It loads crpropa3_amd/libcrpropa3_amd.so and
+raises if the library is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcrpropa3_amd.so")
+
+CRP_OK = 0
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int32)
+_vp = C.c_void_p
+
+
+class RunStats(C.Structure):
+    _fields_ = [("accepted_steps", C.c_ulonglong), ("rejected_trials", C.c_ulonglong),
+                ("field_evals", C.c_ulonglong), ("finished", C.c_ulonglong),
+                ("kernel_ms", C.c_double), ("launches", C.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class HostState(C.Structure):
+    _fields_ = [("pos", _dp), ("dir", _dp), ("energy", _dp), ("charge", _dp), ("current_step", _dp),
+                ("next_step", _dp), ("trajectory_length", _dp), ("time", _dp), ("active", _ip),
+                ("prev_pos", _dp), ("prev_dir", _dp)]
+
+
+class CrpError(RuntimeError):
+    """Error reported by the C ABI (the reference raises std::runtime_error)."""
+
+
+_lib = None
+
+
+def lib():
+    """The loaded HIP library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("crpropa3_amd: %s is missing; build it with __graft_entry__.build() "
+                              "or `make -C crpropa3_amd`" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        L.crp_last_error.restype = C.c_char_p
+        L.crp_field_create_uniform.argtypes = [_dp, C.POINTER(_vp)]
+        L.crp_field_create_grid3f.argtypes = [_fp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int,
+                                              C.POINTER(_vp)]
+        L.crp_field_create_grid3f_device.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int,
+                                                     C.POINTER(_vp)]
+        L.crp_field_create_planewave.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, C.POINTER(_vp)]
+        L.crp_field_destroy.argtypes = [_vp]
+        L.crp_field_eval.argtypes = [_vp, _dp, C.c_size_t, _dp]
+        for nm in ("crp_propagator_create_bp", "crp_propagator_create_ck"):
+            getattr(L, nm).argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.POINTER(_vp)]
+        L.crp_propagator_set_max_trajectory_length.argtypes = [_vp, C.c_double]
+        L.crp_propagator_set_precision.argtypes = [_vp, C.c_int]
+        L.crp_propagator_destroy.argtypes = [_vp]
+        L.crp_batch_create.argtypes = [C.c_size_t, C.POINTER(_vp)]
+        L.crp_batch_destroy.argtypes = [_vp]
+        L.crp_batch_size.argtypes = [_vp, C.POINTER(C.c_size_t)]
+        L.crp_batch_upload.argtypes = [_vp, C.c_size_t, C.POINTER(HostState)]
+        L.crp_batch_download.argtypes = [_vp, C.POINTER(HostState)]
+        L.crp_batch_copy.argtypes = [_vp, _vp]
+        L.crp_propagate.argtypes = [_vp, _vp, C.c_longlong, C.POINTER(RunStats)]
+        L.crp_set_device.argtypes = [C.c_int]
+        L.crp_get_device_count.argtypes = [C.POINTER(C.c_int)]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != CRP_OK:
+        msg = lib().crp_last_error().decode("utf-8", "replace")
+        raise CrpError(msg or ("crpropa3_amd error %d" % rc))
+    return rc
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+class Field:
+    """A device-side magnetic field (owns its device buffers)."""
+
+    def __init__(self, handle, keep=()):
+        self.handle = handle
+        self._keep = keep
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.crp_field_destroy(self.handle)
+            self.handle = None
+
+    @staticmethod
+    def uniform(B):
+        B = _f64(B, (3,))
+        h = _vp()
+        check(lib().crp_field_create_uniform(_p(B), C.byref(h)))
+        return Field(h)
+
+    @staticmethod
+    def grid3f(values, origin, spacing, reflective=False, clip_volume=False):
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        if v.ndim != 4 or v.shape[3] != 3:
+            raise ValueError("grid values must have shape (Nx, Ny, Nz, 3)")
+        o = _f64(origin, (3,))
+        s = _f64(np.broadcast_to(np.asarray(spacing, dtype=np.float64), (3,)))
+        h = _vp()
+        check(lib().crp_field_create_grid3f(_p(v, _fp), v.shape[0], v.shape[1], v.shape[2], _p(o), _p(s),
+                                            int(reflective), int(clip_volume), C.byref(h)))
+        return Field(h)
+
+    @staticmethod
+    def grid3f_device(ptr, shape, origin, spacing, reflective=False, clip_volume=False, keep=None):
+        """Borrow grid values already in device memory (e.g. a torch tensor's data_ptr())."""
+        o = _f64(origin, (3,))
+        s = _f64(np.broadcast_to(np.asarray(spacing, dtype=np.float64), (3,)))
+        h = _vp()
+        check(lib().crp_field_create_grid3f_device(_vp(ptr), shape[0], shape[1], shape[2], _p(o), _p(s),
+                                                   int(reflective), int(clip_volume), C.byref(h)))
+        return Field(h, keep=(keep,))
+
+    @staticmethod
+    def planewave(k, Ak, xi, kappa, beta):
+        k, Ak, beta = _f64(k), _f64(Ak), _f64(beta)
+        xi, kappa = _f64(xi, (-1, 3)), _f64(kappa, (-1, 3))
+        h = _vp()
+        check(lib().crp_field_create_planewave(len(k), _p(k), _p(Ak), _p(xi), _p(kappa), _p(beta), C.byref(h)))
+        return Field(h)
+
+    def eval(self, pos):
+        pos = _f64(pos, (-1, 3))
+        B = np.zeros_like(pos)
+        check(lib().crp_field_eval(self.handle, _p(pos), pos.shape[0], _p(B)))
+        return B
+
+
+class Propagator:
+    """PropagationBP / PropagationCK with a fused MaximumTrajectoryLength."""
+
+    def __init__(self, kind, field, tolerance, min_step, max_step, max_length=0.0, precision=PRECISION_FP64):
+        h = _vp()
+        fn = lib().crp_propagator_create_bp if kind == "BP" else lib().crp_propagator_create_ck
+        check(fn(field.handle if field is not None else None, tolerance, min_step, max_step, C.byref(h)))
+        self.handle = h
+        self.field = field
+        self.kind = kind
+        if max_length:
+            check(lib().crp_propagator_set_max_trajectory_length(h, max_length))
+        if precision != PRECISION_FP64:
+            check(lib().crp_propagator_set_precision(h, precision))
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.crp_propagator_destroy(self.handle)
+            self.handle = None
+
+    def propagate(self, batch, max_steps=-1):
+        st = RunStats()
+        check(lib().crp_propagate(self.handle, batch.handle, max_steps, C.byref(st)))
+        return st
+
+
+class Batch:
+    """Device-resident particle batch (structure of arrays)."""
+
+    def __init__(self, capacity):
+        h = _vp()
+        check(lib().crp_batch_create(capacity, C.byref(h)))
+        self.handle = h
+        self.capacity = capacity
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.crp_batch_destroy(self.handle)
+            self.handle = None
+
+    def __len__(self):
+        n = C.c_size_t(0)
+        check(lib().crp_batch_size(self.handle, C.byref(n)))
+        return n.value
+
+    def upload(self, pos, dir, energy, charge, current_step=None, next_step=None, trajectory_length=None,
+               time=None, active=None):
+        pos = _f64(pos, (-1, 3))
+        n = pos.shape[0]
+        arrs = {
+            "pos": pos, "dir": _f64(dir, (-1, 3)), "energy": _f64(energy), "charge": _f64(charge),
+            "current_step": _f64(current_step if current_step is not None else np.zeros(n)),
+            "next_step": _f64(next_step if next_step is not None else np.zeros(n)),
+            "trajectory_length": _f64(trajectory_length if trajectory_length is not None else np.zeros(n)),
+            "time": _f64(time if time is not None else np.zeros(n)),
+        }
+        act = np.ascontiguousarray(active if active is not None else np.ones(n), dtype=np.int32)
+        s = HostState()
+        for k, a in arrs.items():
+            setattr(s, k, _p(a))
+        s.active = _p(act, _ip)
+        check(lib().crp_batch_upload(self.handle, n, C.byref(s)))
+
+    def download(self):
+        n = len(self)
+        out = {"pos": np.zeros((n, 3)), "dir": np.zeros((n, 3)), "energy": np.zeros(n), "charge": np.zeros(n),
+               "current_step": np.zeros(n), "next_step": np.zeros(n), "trajectory_length": np.zeros(n),
+               "time": np.zeros(n), "prev_pos": np.zeros((n, 3)), "prev_dir": np.zeros((n, 3))}
+        act = np.zeros(n, dtype=np.int32)
+        s = HostState()
+        for k, a in out.items():
+            setattr(s, k, _p(a))
+        s.active = _p(act, _ip)
+        check(lib().crp_batch_download(self.handle, C.byref(s)))
+        out["active"] = act
+        return out
+
+    def copy_from(self, other):
+        check(lib().crp_batch_copy(self.handle, other.handle))
+
+
+def device_count():
+    n = C.c_int(0)
+    check(lib().crp_get_device_count(C.byref(n)))
+    return n.value

@@ -1,0 +1,317 @@
+/*
+ * Correctly rounded cos() and pow() for the FP64 propagation path.
+ *
+ * The reference calls glibc's cos() in the plane-wave mode sum
+ * (src/magneticField/turbulentField/PlaneWaveTurbulence.cpp:237) and pow(r,-0.2)
+ * in the adaptive step control (src/module/PropagationBP.cpp:104,111,
+ * src/module/PropagationCK.cpp:116,123).  The GPU cannot call glibc, and its own
+ * libm is not correctly rounded, so both are evaluated here in double-double
+ * arithmetic (about 100 correct bits) and rounded once at the end.
+ *
+ * Method (textbook; Muller, "Elementary Functions", ch. 11, and the
+ * Dekker/Knuth error-free transformations):
+ *   cos:  Cody-Waite reduction by pi/2 split in 30-bit chunks, then a table of
+ *         sin/cos(j/32) combined with Taylor series of the remainder.
+ *   pow:  x^y = exp(y log x); log via a 256-entry table of log(1+(j+.5)/256)
+ *         and a log1p series, exp via 2^(j/256) and an expm1 series.
+ * Tables come from gen_crmath_tables.py (Python decimal, 90 digits).
+ *
+ * The same source compiles as C11 (host; the oracle's "cr" mode) and as HIP
+ * device code.  Products are split exactly with fma on the device and with
+ * Dekker's splitting on the host; both are exact, so the two builds give the
+ * same bits.  Build with -ffp-contract=off.
+ */
+#ifndef CRP_CRMATH_H
+#define CRP_CRMATH_H
+
+#include <stdint.h>
+#include <string.h>
+
+#if defined(__HIPCC__)
+#define CRP_MATH_FN static inline __device__
+#define CRP_TABLE static __device__ __constant__ const
+#else
+#define CRP_MATH_FN static inline
+#define CRP_TABLE static const
+#endif
+
+#include "crp_crmath_tables.h"
+
+typedef struct {
+	double hi, lo;
+} crp_dd;
+
+CRP_MATH_FN crp_dd crp_mkdd(double hi, double lo) {
+	crp_dd r;
+	r.hi = hi;
+	r.lo = lo;
+	return r;
+}
+
+CRP_MATH_FN uint64_t crp_as_u64(double x) {
+	uint64_t u;
+	memcpy(&u, &x, 8);
+	return u;
+}
+
+CRP_MATH_FN double crp_from_u64(uint64_t u) {
+	double x;
+	memcpy(&x, &u, 8);
+	return x;
+}
+
+/* round to nearest integer (ties away from zero is irrelevant here: only used
+ * to pick reduction indices; any nearby integer keeps the method exact) */
+CRP_MATH_FN double crp_rint(double x) {
+#if defined(__HIPCC__)
+	return __builtin_rint(x);
+#else
+	const double big = 6755399441055744.0; /* 1.5 * 2^52 */
+	if (x > 4503599627370496.0 || x < -4503599627370496.0)
+		return x;
+	return (x + big) - big;
+#endif
+}
+
+/* 2^e for integer e in the normal range */
+CRP_MATH_FN double crp_pow2i(int e) {
+	return crp_from_u64((uint64_t)(e + 1023) << 52);
+}
+
+/* ---------------------------------------------------------- error-free ops */
+CRP_MATH_FN crp_dd crp_two_sum(double a, double b) {
+	double s = a + b;
+	double bb = s - a;
+	double e = (a - (s - bb)) + (b - bb);
+	return crp_mkdd(s, e);
+}
+
+CRP_MATH_FN crp_dd crp_fast_two_sum(double a, double b) {
+	double s = a + b;
+	double e = b - (s - a);
+	return crp_mkdd(s, e);
+}
+
+CRP_MATH_FN crp_dd crp_two_prod(double a, double b) {
+	double p = a * b;
+#if defined(__HIP_DEVICE_COMPILE__)
+	double e = __builtin_fma(a, b, -p);
+#else
+	const double split = 134217729.0; /* 2^27 + 1 */
+	double ca = split * a, cb = split * b;
+	double ah = ca - (ca - a), al = a - ah;
+	double bh = cb - (cb - b), bl = b - bh;
+	double e = ((ah * bh - p) + ah * bl + al * bh) + al * bl;
+#endif
+	return crp_mkdd(p, e);
+}
+
+CRP_MATH_FN crp_dd crp_dd_add(crp_dd a, crp_dd b) {
+	crp_dd s = crp_two_sum(a.hi, b.hi);
+	crp_dd t = crp_two_sum(a.lo, b.lo);
+	s.lo += t.hi;
+	s = crp_fast_two_sum(s.hi, s.lo);
+	s.lo += t.lo;
+	return crp_fast_two_sum(s.hi, s.lo);
+}
+
+CRP_MATH_FN crp_dd crp_dd_add_d(crp_dd a, double b) {
+	crp_dd s = crp_two_sum(a.hi, b);
+	s.lo += a.lo;
+	return crp_fast_two_sum(s.hi, s.lo);
+}
+
+CRP_MATH_FN crp_dd crp_dd_mul(crp_dd a, crp_dd b) {
+	crp_dd p = crp_two_prod(a.hi, b.hi);
+	p.lo += a.hi * b.lo + a.lo * b.hi;
+	return crp_fast_two_sum(p.hi, p.lo);
+}
+
+CRP_MATH_FN crp_dd crp_dd_mul_d(crp_dd a, double b) {
+	crp_dd p = crp_two_prod(a.hi, b);
+	p.lo += a.lo * b;
+	return crp_fast_two_sum(p.hi, p.lo);
+}
+
+CRP_MATH_FN crp_dd crp_dd_neg(crp_dd a) {
+	return crp_mkdd(-a.hi, -a.lo);
+}
+
+CRP_MATH_FN crp_dd crp_tab_dd(const double *t, int i) {
+	return crp_mkdd(t[2 * i], t[2 * i + 1]);
+}
+
+/* ---------------------------------------------------------- log (dd) */
+/* log(x) for finite x > 0, absolute error about 2^-104. */
+CRP_MATH_FN crp_dd crp_log_dd(double x) {
+	int e = 0;
+	uint64_t u = crp_as_u64(x);
+	if ((u >> 52) == 0) { /* subnormal */
+		x *= 18014398509481984.0; /* 2^54 */
+		u = crp_as_u64(x);
+		e = -54;
+	}
+	e += (int)(u >> 52) - 1023;
+	double m = crp_from_u64((u & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL); /* [1,2) */
+	int j = (int)((m - 1.0) * 256.0);
+	double c = 1.0 + ((double)j + 0.5) * (1.0 / 256.0);
+	double d = m - c; /* exact */
+	/* t = d / c as a double-double */
+	double th = d / c;
+	crp_dd pc = crp_two_prod(th, c);
+	double tl = ((d - pc.hi) - pc.lo) / c;
+	crp_dd t = crp_fast_two_sum(th, tl);
+	/* log1p(t) = t - t^2/2 + t^3/3 - ... , |t| <= 2^-9; terms from t^6 in double */
+	double tt = t.hi;
+	double tail = -1.0 / 6 + tt * (1.0 / 7 + tt * (-1.0 / 8 + tt * (1.0 / 9 + tt * (-1.0 / 10 + tt * (1.0 / 11)))));
+	/* Horner in dd: q = 1/5 + t*tail */
+	crp_dd q = crp_dd_add(crp_tab_dd(crp_recip_tab, 5), crp_dd_mul_d(t, tail));
+	q = crp_dd_add(crp_mkdd(-0.25, 0.0), crp_dd_mul(t, q));
+	q = crp_dd_add(crp_tab_dd(crp_recip_tab, 3), crp_dd_mul(t, q));
+	q = crp_dd_add(crp_mkdd(-0.5, 0.0), crp_dd_mul(t, q));
+	q = crp_dd_add(crp_mkdd(1.0, 0.0), crp_dd_mul(t, q));
+	crp_dd l1p = crp_dd_mul(t, q);
+	/* e*ln2 + log(c_j) + log1p(t) */
+	double fe = (double)e;
+	crp_dd acc = crp_two_sum(fe * CRP_LN2_1, crp_log_tab[2 * j]);
+	acc = crp_dd_add_d(acc, fe * CRP_LN2_2);
+	acc = crp_dd_add_d(acc, crp_log_tab[2 * j + 1]);
+	acc = crp_dd_add_d(acc, fe * CRP_LN2_3);
+	acc = crp_dd_add(acc, l1p);
+	return acc;
+}
+
+/* ---------------------------------------------------------- exp (dd) */
+/* exp(z) for |z| < 700 as an unscaled double-double (value = r * 2^*scale). */
+CRP_MATH_FN crp_dd crp_exp_dd(crp_dd z, int *scale) {
+	double k = crp_rint(z.hi * CRP_INV_LN2_256);
+	/* r = z - k*ln2/256; k*M1, k*M2 are exact (|k| < 2^20) */
+	double r0 = z.hi - k * CRP_LN2_256_1; /* exact (Sterbenz) */
+	crp_dd r = crp_two_sum(r0, -k * CRP_LN2_256_2);
+	r = crp_dd_add_d(r, z.lo);
+	r = crp_dd_add_d(r, -k * CRP_LN2_256_3);
+	/* expm1(r), |r| <= 2^-9.5: r + r^2/2 + ... ; r^5.. in double */
+	double rr = r.hi;
+	double tail = 1.0 / 120 + rr * (1.0 / 720 + rr * (1.0 / 5040 + rr * (1.0 / 40320 + rr * (1.0 / 362880))));
+	crp_dd q = crp_dd_add(crp_tab_dd(crp_invfact_tab, 4), crp_dd_mul_d(r, tail)); /* 1/24 + r*tail */
+	q = crp_dd_add(crp_tab_dd(crp_invfact_tab, 3), crp_dd_mul(r, q));              /* 1/6 */
+	q = crp_dd_add(crp_mkdd(0.5, 0.0), crp_dd_mul(r, q));                           /* 1/2 */
+	q = crp_dd_add(crp_mkdd(1.0, 0.0), crp_dd_mul(r, q));
+	crp_dd em1 = crp_dd_mul(r, q);
+	long long ki = (long long)k;
+	int jj = (int)(ki & 255);
+	long long kk = (ki - jj) / 256;
+	crp_dd T = crp_tab_dd(crp_exp2_tab, jj);
+	crp_dd res = crp_dd_add(T, crp_dd_mul(T, em1));
+	*scale = (int)kk;
+	return res;
+}
+
+/*
+ * pow(x, y), correctly rounded for finite x > 0 and |y log2 x| < 1000
+ * (the step-control use has y = -0.2).  Special values follow C99 pow for
+ * non-integer y: pow(0,y<0)=+inf, pow(0,y>0)=0, pow(inf,y<0)=0, pow(x<0,y)=NaN.
+ */
+CRP_MATH_FN double crp_pow_cr(double x, double y) {
+	if (x != x || y != y)
+		return x + y;
+	if (x == 1.0 || y == 0.0)
+		return 1.0;
+	if (x < 0.0)
+		return (x - x) / (x - x); /* NaN */
+	if (x == 0.0)
+		return y < 0 ? 1.0 / 0.0 : 0.0;
+	if (x == 1.0 / 0.0)
+		return y < 0 ? 0.0 : x;
+	crp_dd l = crp_log_dd(x);
+	crp_dd z = crp_dd_mul_d(l, y);
+	int sc;
+	crp_dd r = crp_exp_dd(z, &sc);
+	double v = r.hi + r.lo; /* the one rounding */
+	/* 2^sc in two halves keeps the intermediate normal */
+	int s1 = sc / 2, s2 = sc - s1;
+	return v * crp_pow2i(s1) * crp_pow2i(s2);
+}
+
+/* ---------------------------------------------------------- cos */
+/* sin and cos of a double-double a with 0 <= a <= pi/4 + tiny */
+CRP_MATH_FN void crp_sincos_dd_reduced(crp_dd a, crp_dd *s_out, crp_dd *c_out) {
+	int j = (int)crp_rint(a.hi * 32.0);
+	double dh = a.hi - (double)j * (1.0 / 32.0); /* exact */
+	crp_dd d = crp_two_sum(dh, a.lo);
+	crp_dd z = crp_dd_mul(d, d);
+	double zz = z.hi;
+	/* sin(d) = d + d*z*(-1/6 + z*(1/120 + z*(-1/5040 + ...))) */
+	double ts = -1.0 / 5040 + zz * (1.0 / 362880 + zz * (-1.0 / 39916800 + zz * (1.0 / 6227020800.0)));
+	crp_dd ps = crp_dd_add(crp_tab_dd(crp_invfact_tab, 5), crp_dd_mul_d(z, ts)); /* 1/120 */
+	ps = crp_dd_add(crp_dd_neg(crp_tab_dd(crp_invfact_tab, 3)), crp_dd_mul(z, ps)); /* -1/6 */
+	crp_dd sd_m = crp_dd_mul(crp_dd_mul(d, z), ps);                                /* sin(d) - d */
+	crp_dd sd = crp_dd_add(d, sd_m);
+	/* cos(d) - 1 = z*(-1/2 + z*(1/24 + z*(-1/720 + ...))) */
+	double tc = 1.0 / 40320 + zz * (-1.0 / 3628800 + zz * (1.0 / 479001600.0));
+	crp_dd pcq = crp_dd_add(crp_dd_neg(crp_tab_dd(crp_invfact_tab, 6)), crp_dd_mul_d(z, tc)); /* -1/720 */
+	pcq = crp_dd_add(crp_tab_dd(crp_invfact_tab, 4), crp_dd_mul(z, pcq));                    /* 1/24 */
+	pcq = crp_dd_add(crp_mkdd(-0.5, 0.0), crp_dd_mul(z, pcq));
+	crp_dd cdm1 = crp_dd_mul(z, pcq); /* cos(d) - 1 */
+	if (j == 0) {
+		*s_out = sd;
+		*c_out = crp_dd_add_d(cdm1, 1.0);
+		return;
+	}
+	crp_dd S = crp_mkdd(crp_sincos_tab[4 * j], crp_sincos_tab[4 * j + 1]);
+	crp_dd C = crp_mkdd(crp_sincos_tab[4 * j + 2], crp_sincos_tab[4 * j + 3]);
+	/* sin(t+d) = S + (S*(cos d - 1) + C*sin d);  cos(t+d) = C + (C*(cos d - 1) - S*sin d) */
+	crp_dd s_corr = crp_dd_add(crp_dd_mul(S, cdm1), crp_dd_mul(C, sd));
+	crp_dd c_corr = crp_dd_add(crp_dd_mul(C, cdm1), crp_dd_neg(crp_dd_mul(S, sd)));
+	*s_out = crp_dd_add(S, s_corr);
+	*c_out = crp_dd_add(C, c_corr);
+}
+
+/*
+ * cos(x), correctly rounded.  The reduction keeps about 2^-130 absolute
+ * accuracy for |x| < 2^23 * pi/2 (1.3e7); beyond it products q*P_k stay exact
+ * (two_prod) but cancellation is only tracked to double-double precision.
+ */
+CRP_MATH_FN double crp_cos_cr(double x) {
+	double ax = x < 0 ? -x : x;
+	if (!(ax < 1.0 / 0.0))
+		return x - x; /* NaN for inf and NaN */
+	if (ax < 0x1p-27)
+		return 1.0;
+	double q = crp_rint(x * CRP_TWO_OVER_PI);
+	crp_dd r;
+	if (q == 0.0) {
+		r = crp_mkdd(x, 0.0);
+	} else if (q < 8388608.0 && q > -8388608.0) {
+		double r0 = x - q * CRP_PIO2_1; /* exact */
+		r = crp_two_sum(r0, -q * CRP_PIO2_2);
+		r = crp_dd_add_d(r, -q * CRP_PIO2_3);
+		r = crp_dd_add_d(r, -q * CRP_PIO2_4);
+		r = crp_dd_add_d(r, -q * CRP_PIO2_5);
+	} else {
+		crp_dd p1 = crp_two_prod(q, CRP_PIO2_1);
+		r = crp_two_sum(x - p1.hi, -p1.lo);
+		r = crp_dd_add(r, crp_dd_neg(crp_two_prod(q, CRP_PIO2_2)));
+		r = crp_dd_add(r, crp_dd_neg(crp_two_prod(q, CRP_PIO2_3)));
+		r = crp_dd_add(r, crp_dd_neg(crp_two_prod(q, CRP_PIO2_4)));
+		r = crp_dd_add_d(r, -q * CRP_PIO2_5);
+	}
+	int neg = r.hi < 0;
+	crp_dd a = neg ? crp_dd_neg(r) : r;
+	crp_dd s, c;
+	crp_sincos_dd_reduced(a, &s, &c);
+	if (neg)
+		s = crp_dd_neg(s);
+	long long qi = (long long)q;
+	int n = (int)(qi & 3);
+	crp_dd res;
+	switch (n) {
+	case 0: res = c; break;
+	case 1: res = crp_dd_neg(s); break;
+	case 2: res = crp_dd_neg(c); break;
+	default: res = s; break;
+	}
+	return res.hi + res.lo;
+}
+
+#endif

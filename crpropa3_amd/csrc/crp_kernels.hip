@@ -1,0 +1,794 @@
+// HIP kernels of the batched propagator and the C ABI declared in
+// include/crpropa3_amd.h.
+//
+// Execution model (one particle per lane, persistent waves):
+//   * particle state lives in registers for the whole run; HBM holds it as a
+//     structure of arrays (one array per component, coalesced per wave);
+//   * each loop iteration performs one trial of PropagationBP/CK::process()
+//     for every busy lane (the 1 / 3 / 6 field evaluations stay converged
+//     across the wave); accepted trials commit the step, the fused
+//     MaximumTrajectoryLength condition, and the step counter;
+//   * lanes whose particle finished (inactive, or its step budget spent)
+//     store it and are refilled from a device queue: one atomicAdd per wave
+//     reserves popcount(ballot) indices and each lane takes its mbcnt slot.
+// Reference semantics: src/module/PropagationBP.cpp:62-124,
+// src/module/PropagationCK.cpp:77-136, src/module/BreakCondition.cpp:42-64,
+// src/Candidate.cpp:92-104, src/ParticleState.cpp:30-32.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "crp_device.hpp"
+#include "crpropa3_amd.h"
+
+using namespace crp;
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+	g_err = msg;
+	return code;
+}
+
+#define CRP_HIP_TRY(call)                                                                                      \
+	do {                                                                                                       \
+		hipError_t e_ = (call);                                                                                \
+		if (e_ != hipSuccess)                                                                                  \
+			return fail(CRP_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));                        \
+	} while (0)
+
+// ------------------------------------------------------------------ objects
+enum FieldKind { FK_NONE = 0, FK_UNIFORM = 1, FK_GRID = 2, FK_PLANEWAVE = 3 };
+enum PropKind { PK_BP_FIXED = 0, PK_BP_ADAPTIVE = 1, PK_CK_FIXED = 2, PK_CK_ADAPTIVE = 3 };
+
+struct crp_field {
+	int kind;
+	int device;
+	FieldUniform uniform;
+	GridDesc grid;
+	PlaneWaveDesc pw;
+	void *owned[4];
+};
+
+struct crp_propagator {
+	int type; // 0 BP, 1 CK
+	const crp_field *field;
+	double tolerance, minStep, maxStep;
+	double maxLength;
+	int precision;
+};
+
+enum { CTR_QUEUE = 0, CTR_ACCEPTED, CTR_REJECTED, CTR_EVALS, CTR_FINISHED, CTR_COUNT };
+
+struct crp_batch {
+	size_t cap, n;
+	int device;
+	hipStream_t stream;
+	double *base;
+	double *px, *py, *pz, *dx, *dy, *dz, *E, *q, *cur, *next, *len, *time;
+	double *ppx, *ppy, *ppz, *pdx, *pdy, *pdz;
+	int32_t *active;
+	unsigned long long *ctr;
+	hipEvent_t ev0, ev1;
+};
+
+struct Soa {
+	double *px, *py, *pz, *dx, *dy, *dz, *E, *q, *cur, *next, *len, *time;
+	double *ppx, *ppy, *ppz, *pdx, *pdy, *pdz;
+	int32_t *active;
+};
+
+struct Params {
+	double tol, minStep, maxStep, maxLen;
+	long long maxSteps; // < 0: unlimited
+};
+
+static Soa soa_of(const crp_batch *b) {
+	Soa s;
+	s.px = b->px; s.py = b->py; s.pz = b->pz;
+	s.dx = b->dx; s.dy = b->dy; s.dz = b->dz;
+	s.E = b->E; s.q = b->q; s.cur = b->cur; s.next = b->next; s.len = b->len; s.time = b->time;
+	s.ppx = b->ppx; s.ppy = b->ppy; s.ppz = b->ppz;
+	s.pdx = b->pdx; s.pdy = b->pdy; s.pdz = b->pdz;
+	s.active = b->active;
+	return s;
+}
+
+// ------------------------------------------------------------------ kernels
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+	int i = threadIdx.x;
+	if (i < n)
+		p[i] = 0ull;
+}
+
+template <class F>
+__global__ void k_field_eval(F field, const double *__restrict__ pos, double *__restrict__ B, size_t n) {
+	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	V3 p = v3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+	V3 b = field.eval(p);
+	B[3 * i] = b.x;
+	B[3 * i + 1] = b.y;
+	B[3 * i + 2] = b.z;
+}
+
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; } // std::max
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; } // std::min
+__device__ __forceinline__ double dclip(double x, double lo, double hi) { return dmax(lo, dmin(x, hi)); }
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1)
+		v += __shfl_xor(v, o);
+	return v;
+}
+
+template <class F, int KIND>
+__global__ void __launch_bounds__(256) k_propagate(F field, Params P, Soa S, unsigned long long n,
+                                                   unsigned long long *__restrict__ ctr) {
+	constexpr bool kCK = (KIND == PK_CK_FIXED || KIND == PK_CK_ADAPTIVE);
+	constexpr bool kAdaptive = (KIND == PK_BP_ADAPTIVE || KIND == PK_CK_ADAPTIVE);
+	constexpr unsigned long long kEvalsPerTrial = kCK ? 6 : (kAdaptive ? 3 : 1);
+	const int lane = __lane_id();
+
+	bool have = false, exhausted = false, inProc = false;
+	unsigned long long idx = 0;
+	long long steps = 0;
+	V3 pos = v3(0, 0, 0), dir = v3(0, 0, 0), ppos = pos, pdir = dir;
+	double E = 0, q = 0, cur = 0, next = 0, len = 0, tim = 0;
+	double step = 0, newStep = 0;
+	unsigned long long nAcc = 0, nRej = 0, nEval = 0, nFin = 0;
+
+	for (;;) {
+		// ---- refill lanes without a particle from the queue
+		const bool need = !have && !exhausted;
+		const unsigned long long m = __ballot(need);
+		if (m) {
+			const int leader = __ffsll((long long)m) - 1;
+			unsigned long long base = 0;
+			if (lane == leader)
+				base = atomicAdd(&ctr[CTR_QUEUE], (unsigned long long)__popcll(m));
+			base = __shfl(base, leader);
+			if (need) {
+				const unsigned long long my = base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
+				if (my < n) {
+					idx = my;
+					have = S.active[idx] != 0;
+					if (have) {
+						pos = v3(S.px[idx], S.py[idx], S.pz[idx]);
+						dir = v3(S.dx[idx], S.dy[idx], S.dz[idx]);
+						E = S.E[idx];
+						q = S.q[idx];
+						cur = S.cur[idx];
+						next = S.next[idx];
+						len = S.len[idx];
+						tim = S.time[idx];
+						steps = 0;
+						inProc = false;
+					}
+				} else {
+					exhausted = true;
+				}
+			}
+		}
+		if (__ballot(have) == 0) {
+			if (__ballot(!exhausted) == 0)
+				break;
+			continue;
+		}
+		if (!have)
+			continue;
+
+		// ---- one trial of process() for this lane
+		bool commit = false;
+		Y yOut;
+		if (!inProc) {
+			ppos = pos; // candidate->previous = current
+			pdir = dir;
+			if (q == 0) {
+				// rectilinear propagation for neutral particles (PropagationBP.cpp:74-80)
+				step = dclip(next, P.minStep, P.maxStep);
+				yOut.x = v3(pos.x + dir.x * step, pos.y + dir.y * step, pos.z + dir.z * step);
+				yOut.u = dir;
+				newStep = P.maxStep;
+				commit = true;
+			} else if (!kAdaptive) {
+				step = P.maxStep;
+				newStep = step;
+				if (kCK) {
+					double eU;
+					ck_trystep(field, Y{pos, dir}, step / kCLight, (q * kCLight) / E, yOut, eU);
+				} else {
+					yOut = bp_dy(field, pos, dir, step, q, E / (kCLight * kCLight));
+				}
+				nEval += kEvalsPerTrial;
+				commit = true;
+			} else {
+				step = dclip(next, P.minStep, P.maxStep);
+				newStep = step;
+				inProc = true;
+			}
+		}
+		if (kAdaptive && inProc) {
+			double r;
+			if (kCK) {
+				double eU;
+				ck_trystep(field, Y{pos, dir}, step / kCLight, (q * kCLight) / E, yOut, eU);
+				r = eU / P.tol;
+			} else {
+				const double m_ = E / (kCLight * kCLight);
+				yOut = bp_dy(field, pos, dir, step, q, m_);
+				Y half = bp_dy(field, pos, dir, step / 2, q, m_);
+				Y cmp = bp_dy(field, half.x, half.u, step / 2, q, m_);
+				V3 d = v3(yOut.x.x - cmp.x.x, yOut.x.y - cmp.x.y, yOut.x.z - cmp.x.z);
+				double Sv = norm(d) / (step * (1 - 1 / 4.));
+				r = __builtin_sqrt((Sv * Sv + Sv * Sv) + Sv * Sv) / P.tol;
+			}
+			nEval += kEvalsPerTrial;
+			if (r > 1) {
+				if (step == P.minStep) {
+					commit = true;
+				} else {
+					nRej++;
+					newStep = step * 0.95 * crp_pow_cr(r, -0.2);
+					newStep = dmax(newStep, 0.1 * step);
+					newStep = dmax(newStep, P.minStep);
+					step = newStep;
+				}
+			} else {
+				if (step != P.maxStep) {
+					newStep = step * 0.95 * crp_pow_cr(r, -0.2);
+					newStep = dmin(newStep, 5 * step);
+					newStep = dmin(newStep, P.maxStep);
+				}
+				commit = true;
+			}
+		}
+		if (!commit)
+			continue;
+
+		// ---- commit: setPosition, setDirection(getUnitVector()), setCurrentStep, setNextStep
+		inProc = false;
+		pos = yOut.x;
+		if (q != 0) {
+			const double n1 = norm(yOut.u);
+			V3 u1 = v3(yOut.u.x / n1, yOut.u.y / n1, yOut.u.z / n1);
+			const double n2 = norm(u1);
+			dir = v3(u1.x / n2, u1.y / n2, u1.z / n2);
+		}
+		cur = step;
+		len += step;
+		tim += step / kCLight;
+		next = newStep;
+		nAcc++;
+		steps++;
+		bool alive = true;
+		if (P.maxLen > 0) { // MaximumTrajectoryLength::process
+			if (len >= P.maxLen) {
+				alive = false;
+				nFin++;
+			} else {
+				next = dmin(next, P.maxLen - len);
+			}
+		}
+		if (!alive || (P.maxSteps >= 0 && steps >= P.maxSteps)) {
+			S.px[idx] = pos.x; S.py[idx] = pos.y; S.pz[idx] = pos.z;
+			S.dx[idx] = dir.x; S.dy[idx] = dir.y; S.dz[idx] = dir.z;
+			S.cur[idx] = cur; S.next[idx] = next; S.len[idx] = len; S.time[idx] = tim;
+			S.active[idx] = alive ? 1 : 0;
+			S.ppx[idx] = ppos.x; S.ppy[idx] = ppos.y; S.ppz[idx] = ppos.z;
+			S.pdx[idx] = pdir.x; S.pdy[idx] = pdir.y; S.pdz[idx] = pdir.z;
+			have = false;
+		}
+	}
+	nAcc = wave_sum(nAcc);
+	nRej = wave_sum(nRej);
+	nEval = wave_sum(nEval);
+	nFin = wave_sum(nFin);
+	if (lane == 0) {
+		atomicAdd(&ctr[CTR_ACCEPTED], nAcc);
+		atomicAdd(&ctr[CTR_REJECTED], nRej);
+		atomicAdd(&ctr[CTR_EVALS], nEval);
+		atomicAdd(&ctr[CTR_FINISHED], nFin);
+	}
+}
+
+// ------------------------------------------------------------------ launch helpers
+template <class F, int KIND>
+static int launch_propagate(const F &f, const Params &P, crp_batch *b, int *launches) {
+	static int blocksPerCU = -1, numCU = -1;
+	if (numCU < 0) {
+		int dev;
+		CRP_HIP_TRY(hipGetDevice(&dev));
+		CRP_HIP_TRY(hipDeviceGetAttribute(&numCU, hipDeviceAttributeMultiprocessorCount, dev));
+	}
+	int bpc = 0;
+	CRP_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_propagate<F, KIND>, 256, 0));
+	if (bpc < 1)
+		bpc = 1;
+	(void)blocksPerCU;
+	size_t want = (b->n + 255) / 256;
+	size_t grid = std::min<size_t>(want, (size_t)numCU * bpc);
+	if (grid < 1)
+		grid = 1;
+	hipLaunchKernelGGL((k_propagate<F, KIND>), dim3((unsigned)grid), dim3(256), 0, b->stream, f, P, soa_of(b),
+	                   (unsigned long long)b->n, b->ctr);
+	CRP_HIP_TRY(hipGetLastError());
+	(*launches)++;
+	return CRP_OK;
+}
+
+template <class F>
+static int dispatch_kind(const F &f, const crp_propagator *p, const Params &P, crp_batch *b, int *launches) {
+	const bool adaptive = !(p->minStep == p->maxStep);
+	if (p->type == 0)
+		return adaptive ? launch_propagate<F, PK_BP_ADAPTIVE>(f, P, b, launches)
+		                : launch_propagate<F, PK_BP_FIXED>(f, P, b, launches);
+	return adaptive ? launch_propagate<F, PK_CK_ADAPTIVE>(f, P, b, launches)
+	                : launch_propagate<F, PK_CK_FIXED>(f, P, b, launches);
+}
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+const char *crp_last_error(void) { return g_err.c_str(); }
+
+int crp_abi_version(void) { return CRP_ABI_VERSION; }
+
+int crp_set_device(int device) {
+	CRP_HIP_TRY(hipSetDevice(device));
+	return CRP_OK;
+}
+
+int crp_get_device_count(int *count) {
+	if (!count)
+		return fail(CRP_E_INVALID, "crp_get_device_count: count is NULL");
+	CRP_HIP_TRY(hipGetDeviceCount(count));
+	return CRP_OK;
+}
+
+int crp_device_synchronize(void) {
+	CRP_HIP_TRY(hipDeviceSynchronize());
+	return CRP_OK;
+}
+
+static crp_field *new_field(int kind) {
+	crp_field *f = new (std::nothrow) crp_field;
+	if (!f)
+		return nullptr;
+	std::memset(f, 0, sizeof(*f));
+	f->kind = kind;
+	hipGetDevice(&f->device);
+	return f;
+}
+
+int crp_field_create_uniform(const double B[3], crp_field_t **out) {
+	if (!B || !out)
+		return fail(CRP_E_INVALID, "crp_field_create_uniform: NULL argument");
+	crp_field *f = new_field(FK_UNIFORM);
+	if (!f)
+		return fail(CRP_E_NOMEM, "crp_field_create_uniform: out of host memory");
+	for (int a = 0; a < 3; a++)
+		f->uniform.b[a] = B[a];
+	*out = f;
+	return CRP_OK;
+}
+
+static int fill_grid_desc(GridDesc &g, const float *d_values, int nx, int ny, int nz, const double origin[3],
+                          const double spacing[3], int reflective, int clip) {
+	if (nx < 1 || ny < 1 || nz < 1)
+		return fail(CRP_E_INVALID, "crp_field_create_grid3f: grid size must be >= 1 in every dimension");
+	if (!origin || !spacing)
+		return fail(CRP_E_INVALID, "crp_field_create_grid3f: NULL origin or spacing");
+	for (int a = 0; a < 3; a++)
+		if (!(spacing[a] > 0))
+			return fail(CRP_E_INVALID, "crp_field_create_grid3f: spacing must be > 0");
+	if ((double)nx * ny * nz * 3.0 > 9.0e18)
+		return fail(CRP_E_INVALID, "crp_field_create_grid3f: grid too large");
+	g.values = d_values;
+	g.nx = nx;
+	g.ny = ny;
+	g.nz = nz;
+	g.reflective = reflective ? 1 : 0;
+	g.clip = clip ? 1 : 0;
+	const int N[3] = {nx, ny, nz};
+	for (int a = 0; a < 3; a++) {
+		g.spacing[a] = spacing[a];
+		g.origin[a] = origin[a];
+		g.gridOrigin[a] = origin[a] + spacing[a] / 2;      // Grid::setOrigin
+		g.edge[a] = origin[a] + (double)N[a] * spacing[a]; // Grid::interpolate clip box
+	}
+	return CRP_OK;
+}
+
+int crp_field_create_grid3f(const float *values, int nx, int ny, int nz, const double origin[3],
+                            const double spacing[3], int reflective, int clip_volume, crp_field_t **out) {
+	if (!values || !out)
+		return fail(CRP_E_INVALID, "crp_field_create_grid3f: NULL argument");
+	crp_field *f = new_field(FK_GRID);
+	if (!f)
+		return fail(CRP_E_NOMEM, "crp_field_create_grid3f: out of host memory");
+	int rc = fill_grid_desc(f->grid, nullptr, nx, ny, nz, origin, spacing, reflective, clip_volume);
+	if (rc != CRP_OK) {
+		delete f;
+		return rc;
+	}
+	size_t bytes = (size_t)nx * ny * nz * 3 * sizeof(float);
+	void *d = nullptr;
+	hipError_t e = hipMalloc(&d, bytes);
+	if (e != hipSuccess) {
+		delete f;
+		return fail(CRP_E_NOMEM, std::string("crp_field_create_grid3f: hipMalloc: ") + hipGetErrorString(e));
+	}
+	e = hipMemcpy(d, values, bytes, hipMemcpyHostToDevice);
+	if (e != hipSuccess) {
+		hipFree(d);
+		delete f;
+		return fail(CRP_E_HIP, std::string("crp_field_create_grid3f: hipMemcpy: ") + hipGetErrorString(e));
+	}
+	f->grid.values = (const float *)d;
+	f->owned[0] = d;
+	*out = f;
+	return CRP_OK;
+}
+
+int crp_field_create_grid3f_device(const float *d_values, int nx, int ny, int nz, const double origin[3],
+                                   const double spacing[3], int reflective, int clip_volume, crp_field_t **out) {
+	if (!d_values || !out)
+		return fail(CRP_E_INVALID, "crp_field_create_grid3f_device: NULL argument");
+	crp_field *f = new_field(FK_GRID);
+	if (!f)
+		return fail(CRP_E_NOMEM, "crp_field_create_grid3f_device: out of host memory");
+	int rc = fill_grid_desc(f->grid, d_values, nx, ny, nz, origin, spacing, reflective, clip_volume);
+	if (rc != CRP_OK) {
+		delete f;
+		return rc;
+	}
+	*out = f;
+	return CRP_OK;
+}
+
+int crp_field_create_planewave(int nm, const double *k, const double *Ak, const double *xi, const double *kappa,
+                               const double *beta, crp_field_t **out) {
+	if (!k || !Ak || !xi || !kappa || !beta || !out)
+		return fail(CRP_E_INVALID, "crp_field_create_planewave: NULL argument");
+	if (nm <= 1) // PlaneWaveTurbulence.cpp:93-97
+		return fail(CRP_E_INVALID, "crp_field_create_planewave: Nm <= 1. Specify at least two wavemodes");
+	crp_field *f = new_field(FK_PLANEWAVE);
+	if (!f)
+		return fail(CRP_E_NOMEM, "crp_field_create_planewave: out of host memory");
+	// xi*Ak is the first product of B += xi[i] * Ak[i] * cos(...): formed once here, same IEEE op.
+	std::vector<double> xiA(3 * (size_t)nm);
+	for (int i = 0; i < nm; i++)
+		for (int a = 0; a < 3; a++)
+			xiA[3 * i + a] = xi[3 * i + a] * Ak[i];
+	const size_t nb = (size_t)nm * sizeof(double);
+	const void *src[4] = {k, beta, kappa, xiA.data()};
+	const size_t sz[4] = {nb, nb, 3 * nb, 3 * nb};
+	for (int j = 0; j < 4; j++) {
+		void *d = nullptr;
+		hipError_t e = hipMalloc(&d, sz[j]);
+		if (e == hipSuccess)
+			e = hipMemcpy(d, src[j], sz[j], hipMemcpyHostToDevice);
+		if (e != hipSuccess) {
+			if (d)
+				hipFree(d);
+			for (int i = 0; i < j; i++)
+				hipFree(f->owned[i]);
+			delete f;
+			return fail(CRP_E_HIP, std::string("crp_field_create_planewave: ") + hipGetErrorString(e));
+		}
+		f->owned[j] = d;
+	}
+	f->pw.nm = nm;
+	f->pw.k = (const double *)f->owned[0];
+	f->pw.beta = (const double *)f->owned[1];
+	f->pw.kappa = (const double *)f->owned[2];
+	f->pw.xiA = (const double *)f->owned[3];
+	*out = f;
+	return CRP_OK;
+}
+
+int crp_field_destroy(crp_field_t *f) {
+	if (!f)
+		return CRP_OK;
+	for (int i = 0; i < 4; i++)
+		if (f->owned[i])
+			hipFree(f->owned[i]);
+	delete f;
+	return CRP_OK;
+}
+
+int crp_field_eval(const crp_field_t *f, const double *pos, size_t n, double *B) {
+	if (!pos || !B)
+		return fail(CRP_E_INVALID, "crp_field_eval: NULL argument");
+	if (n == 0)
+		return CRP_OK;
+	double *dpos = nullptr, *dB = nullptr;
+	CRP_HIP_TRY(hipMalloc(&dpos, 3 * n * sizeof(double)));
+	hipError_t e = hipMalloc(&dB, 3 * n * sizeof(double));
+	if (e != hipSuccess) {
+		hipFree(dpos);
+		return fail(CRP_E_NOMEM, "crp_field_eval: hipMalloc failed");
+	}
+	e = hipMemcpy(dpos, pos, 3 * n * sizeof(double), hipMemcpyHostToDevice);
+	const unsigned blocks = (unsigned)((n + 255) / 256);
+	if (e == hipSuccess) {
+		const int kind = f ? f->kind : FK_NONE;
+		switch (kind) {
+		case FK_UNIFORM: hipLaunchKernelGGL(k_field_eval<FieldUniform>, blocks, 256, 0, 0, f->uniform, dpos, dB, n); break;
+		case FK_GRID: hipLaunchKernelGGL(k_field_eval<FieldGrid>, blocks, 256, 0, 0, FieldGrid{f->grid}, dpos, dB, n); break;
+		case FK_PLANEWAVE: hipLaunchKernelGGL(k_field_eval<FieldPlaneWave>, blocks, 256, 0, 0, FieldPlaneWave{f->pw}, dpos, dB, n); break;
+		default: hipLaunchKernelGGL(k_field_eval<FieldNone>, blocks, 256, 0, 0, FieldNone{}, dpos, dB, n); break;
+		}
+		e = hipGetLastError();
+	}
+	if (e == hipSuccess)
+		e = hipMemcpy(B, dB, 3 * n * sizeof(double), hipMemcpyDeviceToHost);
+	hipFree(dpos);
+	hipFree(dB);
+	if (e != hipSuccess)
+		return fail(CRP_E_HIP, std::string("crp_field_eval: ") + hipGetErrorString(e));
+	return CRP_OK;
+}
+
+static int validate_prop(const char *who, double tol, double minStep, double maxStep) {
+	// setTolerance / setMaximumStep / setMinimumStep (PropagationBP.cpp:162-183, PropagationCK.cpp:160-179)
+	if ((tol > 1) || (tol < 0))
+		return fail(CRP_E_INVALID, std::string(who) + ": target error not in range 0-1");
+	if (minStep < 0)
+		return fail(CRP_E_INVALID, std::string(who) + ": minStep < 0 ");
+	if (minStep > maxStep)
+		return fail(CRP_E_INVALID, std::string(who) + ": minStep > maxStep");
+	return CRP_OK;
+}
+
+static int create_prop(int type, const char *who, const crp_field_t *field, double tol, double minStep,
+                       double maxStep, crp_propagator_t **out) {
+	if (!out)
+		return fail(CRP_E_INVALID, std::string(who) + ": out is NULL");
+	int rc = validate_prop(who, tol, minStep, maxStep);
+	if (rc != CRP_OK)
+		return rc;
+	crp_propagator *p = new (std::nothrow) crp_propagator;
+	if (!p)
+		return fail(CRP_E_NOMEM, std::string(who) + ": out of host memory");
+	p->type = type;
+	p->field = field;
+	p->tolerance = tol;
+	p->minStep = minStep;
+	p->maxStep = maxStep;
+	p->maxLength = 0;
+	p->precision = CRP_PRECISION_FP64;
+	*out = p;
+	return CRP_OK;
+}
+
+int crp_propagator_create_bp(const crp_field_t *field, double tolerance, double min_step, double max_step,
+                             crp_propagator_t **out) {
+	return create_prop(0, "PropagationBP", field, tolerance, min_step, max_step, out);
+}
+
+int crp_propagator_create_ck(const crp_field_t *field, double tolerance, double min_step, double max_step,
+                             crp_propagator_t **out) {
+	return create_prop(1, "PropagationCK", field, tolerance, min_step, max_step, out);
+}
+
+int crp_propagator_set_max_trajectory_length(crp_propagator_t *p, double max_length) {
+	if (!p)
+		return fail(CRP_E_INVALID, "crp_propagator_set_max_trajectory_length: NULL propagator");
+	p->maxLength = max_length > 0 ? max_length : 0;
+	return CRP_OK;
+}
+
+int crp_propagator_set_precision(crp_propagator_t *p, int precision) {
+	if (!p)
+		return fail(CRP_E_INVALID, "crp_propagator_set_precision: NULL propagator");
+	if (precision != CRP_PRECISION_FP64)
+		return fail(CRP_E_INVALID, "crp_propagator_set_precision: only CRP_PRECISION_FP64 is built");
+	p->precision = precision;
+	return CRP_OK;
+}
+
+int crp_propagator_destroy(crp_propagator_t *p) {
+	delete p;
+	return CRP_OK;
+}
+
+int crp_batch_create(size_t capacity, crp_batch_t **out) {
+	if (!out || capacity == 0)
+		return fail(CRP_E_INVALID, "crp_batch_create: capacity must be > 0");
+	crp_batch *b = new (std::nothrow) crp_batch;
+	if (!b)
+		return fail(CRP_E_NOMEM, "crp_batch_create: out of host memory");
+	std::memset(b, 0, sizeof(*b));
+	b->cap = capacity;
+	hipGetDevice(&b->device);
+	// one allocation: 18 double arrays, the active flags, then a 256-byte aligned counter block
+	const size_t nd = 18;
+	size_t off = nd * capacity * sizeof(double) + capacity * sizeof(int32_t);
+	off = (off + 255) & ~(size_t)255;
+	const size_t bytes = off + 64 * sizeof(unsigned long long);
+	hipError_t e = hipMalloc((void **)&b->base, bytes);
+	if (e != hipSuccess) {
+		delete b;
+		return fail(CRP_E_NOMEM, std::string("crp_batch_create: hipMalloc: ") + hipGetErrorString(e));
+	}
+	double **arr[18] = {&b->px, &b->py, &b->pz, &b->dx, &b->dy, &b->dz, &b->E, &b->q, &b->cur,
+	                    &b->next, &b->len, &b->time, &b->ppx, &b->ppy, &b->ppz, &b->pdx, &b->pdy, &b->pdz};
+	for (size_t i = 0; i < nd; i++)
+		*arr[i] = b->base + i * capacity;
+	b->active = (int32_t *)(b->base + nd * capacity);
+	b->ctr = (unsigned long long *)((char *)b->base + off);
+	e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+	if (e == hipSuccess)
+		e = hipEventCreate(&b->ev0);
+	if (e == hipSuccess)
+		e = hipEventCreate(&b->ev1);
+	if (e != hipSuccess) {
+		hipFree(b->base);
+		delete b;
+		return fail(CRP_E_HIP, std::string("crp_batch_create: ") + hipGetErrorString(e));
+	}
+	*out = b;
+	return CRP_OK;
+}
+
+int crp_batch_destroy(crp_batch_t *b) {
+	if (!b)
+		return CRP_OK;
+	hipStreamSynchronize(b->stream);
+	hipEventDestroy(b->ev0);
+	hipEventDestroy(b->ev1);
+	hipStreamDestroy(b->stream);
+	hipFree(b->base);
+	delete b;
+	return CRP_OK;
+}
+
+int crp_batch_size(const crp_batch_t *b, size_t *n) {
+	if (!b || !n)
+		return fail(CRP_E_INVALID, "crp_batch_size: NULL argument");
+	*n = b->n;
+	return CRP_OK;
+}
+
+int crp_batch_upload(crp_batch_t *b, size_t n, const crp_host_state *s) {
+	if (!b || !s)
+		return fail(CRP_E_INVALID, "crp_batch_upload: NULL argument");
+	if (n > b->cap)
+		return fail(CRP_E_INVALID, "crp_batch_upload: n exceeds the batch capacity");
+	if (!s->pos || !s->dir || !s->energy || !s->charge || !s->active)
+		return fail(CRP_E_INVALID, "crp_batch_upload: pos, dir, energy, charge and active are required");
+	b->n = n;
+	if (n == 0)
+		return CRP_OK;
+	std::vector<double> tmp(n);
+	double *dst3[6] = {b->px, b->py, b->pz, b->dx, b->dy, b->dz};
+	for (int c = 0; c < 6; c++) {
+		const double *src = c < 3 ? s->pos : s->dir;
+		for (size_t i = 0; i < n; i++)
+			tmp[i] = src[3 * i + (c % 3)];
+		CRP_HIP_TRY(hipMemcpy(dst3[c], tmp.data(), n * sizeof(double), hipMemcpyHostToDevice));
+	}
+	const double *src1[6] = {s->energy, s->charge, s->current_step, s->next_step, s->trajectory_length, s->time};
+	double *dst1[6] = {b->E, b->q, b->cur, b->next, b->len, b->time};
+	for (int c = 0; c < 6; c++) {
+		if (src1[c]) {
+			CRP_HIP_TRY(hipMemcpy(dst1[c], src1[c], n * sizeof(double), hipMemcpyHostToDevice));
+		} else {
+			std::fill(tmp.begin(), tmp.end(), 0.0);
+			CRP_HIP_TRY(hipMemcpy(dst1[c], tmp.data(), n * sizeof(double), hipMemcpyHostToDevice));
+		}
+	}
+	CRP_HIP_TRY(hipMemcpy(b->active, s->active, n * sizeof(int32_t), hipMemcpyHostToDevice));
+	// previous := current until the first step
+	CRP_HIP_TRY(hipMemcpy(b->ppx, b->px, 3 * b->cap * sizeof(double), hipMemcpyDeviceToDevice));
+	CRP_HIP_TRY(hipMemcpy(b->pdx, b->dx, 3 * b->cap * sizeof(double), hipMemcpyDeviceToDevice));
+	return CRP_OK;
+}
+
+int crp_batch_download(const crp_batch_t *b, crp_host_state *s) {
+	if (!b || !s)
+		return fail(CRP_E_INVALID, "crp_batch_download: NULL argument");
+	const size_t n = b->n;
+	if (n == 0)
+		return CRP_OK;
+	CRP_HIP_TRY(hipStreamSynchronize(b->stream));
+	std::vector<double> tmp(n);
+	const double *src3[12] = {b->px, b->py, b->pz, b->dx, b->dy, b->dz, b->ppx, b->ppy, b->ppz, b->pdx, b->pdy, b->pdz};
+	double *dstv[4] = {s->pos, s->dir, s->prev_pos, s->prev_dir};
+	for (int c = 0; c < 12; c++) {
+		double *dst = dstv[c / 3];
+		if (!dst)
+			continue;
+		CRP_HIP_TRY(hipMemcpy(tmp.data(), src3[c], n * sizeof(double), hipMemcpyDeviceToHost));
+		for (size_t i = 0; i < n; i++)
+			dst[3 * i + (c % 3)] = tmp[i];
+	}
+	const double *src1[6] = {b->E, b->q, b->cur, b->next, b->len, b->time};
+	double *dst1[6] = {s->energy, s->charge, s->current_step, s->next_step, s->trajectory_length, s->time};
+	for (int c = 0; c < 6; c++)
+		if (dst1[c])
+			CRP_HIP_TRY(hipMemcpy(dst1[c], src1[c], n * sizeof(double), hipMemcpyDeviceToHost));
+	if (s->active)
+		CRP_HIP_TRY(hipMemcpy(s->active, b->active, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+	return CRP_OK;
+}
+
+int crp_batch_copy(crp_batch_t *dst, const crp_batch_t *src) {
+	if (!dst || !src)
+		return fail(CRP_E_INVALID, "crp_batch_copy: NULL argument");
+	if (src->n > dst->cap)
+		return fail(CRP_E_INVALID, "crp_batch_copy: destination capacity too small");
+	const size_t n = src->n;
+	dst->n = n;
+	if (n == 0)
+		return CRP_OK;
+	CRP_HIP_TRY(hipStreamSynchronize(src->stream));
+	const double *s[18] = {src->px, src->py, src->pz, src->dx, src->dy, src->dz, src->E, src->q, src->cur,
+	                       src->next, src->len, src->time, src->ppx, src->ppy, src->ppz, src->pdx, src->pdy, src->pdz};
+	double *d[18] = {dst->px, dst->py, dst->pz, dst->dx, dst->dy, dst->dz, dst->E, dst->q, dst->cur,
+	                 dst->next, dst->len, dst->time, dst->ppx, dst->ppy, dst->ppz, dst->pdx, dst->pdy, dst->pdz};
+	for (int c = 0; c < 18; c++)
+		CRP_HIP_TRY(hipMemcpyAsync(d[c], s[c], n * sizeof(double), hipMemcpyDeviceToDevice, dst->stream));
+	CRP_HIP_TRY(hipMemcpyAsync(dst->active, src->active, n * sizeof(int32_t), hipMemcpyDeviceToDevice, dst->stream));
+	CRP_HIP_TRY(hipStreamSynchronize(dst->stream));
+	return CRP_OK;
+}
+
+int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps, crp_run_stats *stats) {
+	if (!p || !b)
+		return fail(CRP_E_INVALID, "crp_propagate: NULL argument");
+	if (max_steps < 0 && !(p->maxLength > 0 && p->minStep > 0))
+		return fail(CRP_E_UNBOUNDED, "crp_propagate: max_steps < 0 needs a maximum trajectory length and minStep > 0");
+	crp_run_stats st;
+	std::memset(&st, 0, sizeof(st));
+	if (b->n == 0 || max_steps == 0) {
+		if (stats)
+			*stats = st;
+		return CRP_OK;
+	}
+	Params P;
+	P.tol = p->tolerance;
+	P.minStep = p->minStep;
+	P.maxStep = p->maxStep;
+	P.maxLen = p->maxLength;
+	P.maxSteps = max_steps;
+	hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, b->stream, b->ctr, (int)CTR_COUNT);
+	CRP_HIP_TRY(hipGetLastError());
+	CRP_HIP_TRY(hipEventRecord(b->ev0, b->stream));
+	int rc;
+	const crp_field *f = p->field;
+	switch (f ? f->kind : FK_NONE) {
+	case FK_UNIFORM: rc = dispatch_kind(f->uniform, p, P, b, &st.launches); break;
+	case FK_GRID: rc = dispatch_kind(FieldGrid{f->grid}, p, P, b, &st.launches); break;
+	case FK_PLANEWAVE: rc = dispatch_kind(FieldPlaneWave{f->pw}, p, P, b, &st.launches); break;
+	default: rc = dispatch_kind(FieldNone{}, p, P, b, &st.launches); break;
+	}
+	if (rc != CRP_OK)
+		return rc;
+	CRP_HIP_TRY(hipEventRecord(b->ev1, b->stream));
+	unsigned long long h[CTR_COUNT];
+	CRP_HIP_TRY(hipMemcpyAsync(h, b->ctr, sizeof(h), hipMemcpyDeviceToHost, b->stream));
+	CRP_HIP_TRY(hipStreamSynchronize(b->stream));
+	float ms = 0;
+	CRP_HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+	st.accepted_steps = h[CTR_ACCEPTED];
+	st.rejected_trials = h[CTR_REJECTED];
+	st.field_evals = h[CTR_EVALS];
+	st.finished = h[CTR_FINISHED];
+	st.kernel_ms = ms;
+	if (stats)
+		*stats = st;
+	return CRP_OK;
+}
+
+} // extern "C"

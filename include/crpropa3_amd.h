@@ -1,0 +1,144 @@
+/*
+ * crpropa3_amd -- C ABI of the MI355X batched charged-particle propagator.
+ *
+ * Plain C types only (pointers, sizes, doubles); no C++ exception crosses this
+ * boundary.  Every function returns CRP_OK (0) or a negative error code; the
+ * message of the last error on the calling thread is crp_last_error().
+ *
+ * Units are SI, as in the reference (include/crpropa/Units.h): metres, joules,
+ * tesla, coulomb, seconds.  Host pointers are read or written synchronously
+ * unless a function says it takes device pointers.
+ *
+ * Each entry point names the reference interface it stands in for.
+ */
+#ifndef CRPROPA3_AMD_H
+#define CRPROPA3_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CRP_ABI_VERSION 1
+
+enum crp_status {
+	CRP_OK = 0,
+	CRP_E_INVALID = -1,  /* bad argument (the reference throws std::runtime_error) */
+	CRP_E_HIP = -2,      /* HIP runtime error */
+	CRP_E_NOMEM = -3,    /* allocation failed */
+	CRP_E_STATE = -4,    /* object in the wrong state for the call */
+	CRP_E_UNBOUNDED = -5 /* run would never end (no step budget, no length limit) */
+};
+
+/* Arithmetic of a propagation run. */
+enum crp_precision {
+	CRP_PRECISION_FP64 = 0,  /* reference arithmetic, bit for bit (FP32 trilinear as Grid3f does) */
+	CRP_PRECISION_MIXED = 1  /* FP32 field + rotation, FP64 position and step control */
+};
+
+typedef struct crp_field crp_field_t;
+typedef struct crp_propagator crp_propagator_t;
+typedef struct crp_batch crp_batch_t;
+
+/* Counters of one crp_propagate call. */
+typedef struct {
+	unsigned long long accepted_steps; /* process() calls that moved a particle */
+	unsigned long long rejected_trials; /* adaptive trials thrown away (step shrunk) */
+	unsigned long long field_evals;     /* getField evaluations */
+	unsigned long long finished;        /* particles made inactive by the length limit */
+	double kernel_ms;                   /* device time of the propagation kernels (HIP events) */
+	int launches;                       /* kernel launches */
+} crp_run_stats;
+
+/* ---- runtime ---------------------------------------------------------- */
+const char *crp_last_error(void);
+int crp_abi_version(void);
+/* Select the HIP device for objects created afterwards on this thread. */
+int crp_set_device(int device);
+int crp_get_device_count(int *count);
+/* Wait for all work on the current device (hipDeviceSynchronize). */
+int crp_device_synchronize(void);
+
+/* ---- fields (MagneticField::getField, include/crpropa/magneticField/MagneticField.h:22-32) */
+
+/* UniformMagneticField(B) -- include/crpropa/magneticField/MagneticField.h (UniformMagneticField) */
+int crp_field_create_uniform(const double B[3], crp_field_t **out);
+
+/* MagneticFieldGrid(ref_ptr<Grid3f>) -- src/magneticField/MagneticFieldGrid.cpp:5-19 with
+ * Grid<Vector3f> (include/crpropa/Grid.h:158-582).  values: host float32, AoS (x,y,z) per point,
+ * index (ix*ny + iy)*nz + iz (Grid.h:338-340).  origin/spacing as GridProperties; reflective,
+ * clip_volume as Grid::setReflective / setClipVolume.  Trilinear interpolation only. */
+int crp_field_create_grid3f(const float *values, int nx, int ny, int nz, const double origin[3],
+                            const double spacing[3], int reflective, int clip_volume, crp_field_t **out);
+
+/* Same, with values already on the device (borrowed: the caller keeps them alive). */
+int crp_field_create_grid3f_device(const float *d_values, int nx, int ny, int nz, const double origin[3],
+                                   const double spacing[3], int reflective, int clip_volume, crp_field_t **out);
+
+/* PlaneWaveTurbulence::getField -- src/magneticField/turbulentField/PlaneWaveTurbulence.cpp:231-240.
+ * The mode table as built by the PlaneWaveTurbulence constructor (:99-178): k[nm], Ak[nm],
+ * xi[3*nm], kappa[3*nm] (AoS), beta[nm]; host pointers. */
+int crp_field_create_planewave(int nm, const double *k, const double *Ak, const double *xi, const double *kappa,
+                               const double *beta, crp_field_t **out);
+
+int crp_field_destroy(crp_field_t *f);
+
+/* B at n host positions (AoS xyz), evaluated on the device with the FP64 path. */
+int crp_field_eval(const crp_field_t *f, const double *pos, size_t n, double *B);
+
+/* ---- propagators ------------------------------------------------------ */
+
+/* PropagationBP(field, tolerance, minStep, maxStep) -- src/module/PropagationBP.cpp:43-59,162-183.
+ * Fixed-step form PropagationBP(field, step) = (field, 0.42, step, step).  field may be NULL
+ * (B = 0, src/module/PropagationBP.cpp:137-149).  Validation as setTolerance / setMinimumStep /
+ * setMaximumStep: tolerance in [0,1], 0 <= minStep <= maxStep. */
+int crp_propagator_create_bp(const crp_field_t *field, double tolerance, double min_step, double max_step,
+                             crp_propagator_t **out);
+
+/* PropagationCK(field, tolerance, minStep, maxStep) -- src/module/PropagationCK.cpp:63-75,160-179. */
+int crp_propagator_create_ck(const crp_field_t *field, double tolerance, double min_step, double max_step,
+                             crp_propagator_t **out);
+
+/* MaximumTrajectoryLength(maxLength) fused after each step -- src/module/BreakCondition.cpp:42-64
+ * (no observer positions).  max_length <= 0 removes it. */
+int crp_propagator_set_max_trajectory_length(crp_propagator_t *p, double max_length);
+int crp_propagator_set_precision(crp_propagator_t *p, int precision);
+int crp_propagator_destroy(crp_propagator_t *p);
+
+/* ---- particle batches: device-resident structure of arrays ------------ */
+/* The fields of Candidate/ParticleState that the path reads or writes
+ * (include/crpropa/Candidate.h, include/crpropa/ParticleState.h). */
+typedef struct {
+	double *pos;               /* [3n] current.position (AoS on the host side) */
+	double *dir;               /* [3n] current.direction */
+	double *energy;            /* [n] current.energy */
+	double *charge;            /* [n] current.charge (Z * eplus) */
+	double *current_step;      /* [n] */
+	double *next_step;         /* [n] */
+	double *trajectory_length; /* [n] */
+	double *time;              /* [n] */
+	int32_t *active;           /* [n] */
+	double *prev_pos;          /* [3n] previous.position, written on download (may be NULL) */
+	double *prev_dir;          /* [3n] previous.direction (may be NULL) */
+} crp_host_state;
+
+int crp_batch_create(size_t capacity, crp_batch_t **out);
+int crp_batch_destroy(crp_batch_t *b);
+int crp_batch_size(const crp_batch_t *b, size_t *n);
+/* Copy n particles from host arrays to the device (pos/dir AoS are transposed to SoA). */
+int crp_batch_upload(crp_batch_t *b, size_t n, const crp_host_state *s);
+int crp_batch_download(const crp_batch_t *b, crp_host_state *s);
+/* Device-to-device copy of the whole state (dst gets src's size). */
+int crp_batch_copy(crp_batch_t *dst, const crp_batch_t *src);
+
+/* ModuleList::run(candidates) for [propagator, MaximumTrajectoryLength] (src/ModuleList.cpp.in:97-151):
+ * at most max_steps process() calls per active particle (max_steps < 0: until every particle is
+ * inactive, which needs a trajectory-length limit and minStep > 0).  Synchronous. */
+int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps, crp_run_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,547 @@
+/*
+ * ORACLE -- test infrastructure, not product code (see crp_oracle.h).
+ *
+ * CPU restatement of the reference hot path.  Vector operations are written
+ * out component by component in the reference's order of evaluation:
+ *   Vector3<T>::getR   sqrt((x*x + y*y) + z*z)           include/crpropa/Vector3.h:126-128
+ *   Vector3<T>::dot    (x*v.x + y*v.y) + z*v.z           include/crpropa/Vector3.h:263-265
+ *   Vector3<T>::cross  (y*v.z - v.y*z, z*v.x - v.z*x, x*v.y - v.x*y)  include/crpropa/Vector3.h:268-271
+ *   v * s, v / s       per component                     include/crpropa/Vector3.h:305-316
+ * Build: oracle/Makefile (gcc -O3 -ffp-contract=off, no -march: no FMA).
+ */
+#include "crp_oracle.h"
+#include "crp_crmath.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* c_light, include/crpropa/Units.h:64 */
+static const double C_LIGHT = 2.99792458e8;
+
+/* ------------------------------------------------------------------ libm */
+double orc_cos(double x, int libm) { return libm == ORC_LIBM_CR ? crp_cos_cr(x) : cos(x); }
+double orc_pow(double x, double y, int libm) { return libm == ORC_LIBM_CR ? crp_pow_cr(x, y) : pow(x, y); }
+
+void orc_cos_batch(const double *x, size_t n, double *out, int libm) {
+	for (size_t i = 0; i < n; i++)
+		out[i] = orc_cos(x[i], libm);
+}
+
+void orc_pow_batch(const double *x, double y, size_t n, double *out, int libm) {
+	for (size_t i = 0; i < n; i++)
+		out[i] = orc_pow(x[i], y, libm);
+}
+
+/* ------------------------------------------------------------------ MT19937 (src/Random.cpp) */
+#define ORC_N 624
+#define ORC_M 397
+
+/* Random::initialize, src/Random.cpp:370-379 */
+static void orc_rng_initialize(orc_rng *r, uint32_t seed) {
+	r->state[0] = seed;
+	for (int i = 1; i < ORC_N; i++)
+		r->state[i] = 1812433253U * (r->state[i - 1] ^ (r->state[i - 1] >> 30)) + (uint32_t)i;
+}
+
+/* Random::twist, include/crpropa/Random.h:214-225 */
+static inline uint32_t orc_twist(uint32_t m, uint32_t s0, uint32_t s1) {
+	uint32_t mix = (s0 & 0x80000000U) | (s1 & 0x7fffffffU);
+	return m ^ (mix >> 1) ^ ((0U - (s1 & 1U)) & 0x9908b0dfU);
+}
+
+/* Random::reload, src/Random.cpp:381-391 */
+static void orc_rng_reload(orc_rng *r) {
+	uint32_t *p = r->state;
+	int i;
+	for (i = ORC_N - ORC_M; i--; ++p)
+		*p = orc_twist(p[ORC_M], p[0], p[1]);
+	for (i = ORC_M; --i; ++p)
+		*p = orc_twist(p[ORC_M - ORC_N], p[0], p[1]);
+	*p = orc_twist(p[ORC_M - ORC_N], p[0], r->state[0]);
+	r->left = ORC_N;
+	r->pNext = r->state;
+}
+
+/* Random::seed(uint32_t), src/Random.cpp:309-314 */
+void orc_rng_seed(orc_rng *r, uint32_t seed) {
+	orc_rng_initialize(r, seed);
+	orc_rng_reload(r);
+}
+
+/* Random::randInt, src/Random.cpp:249-260 */
+uint32_t orc_rng_randint(orc_rng *r) {
+	if (r->left == 0)
+		orc_rng_reload(r);
+	--r->left;
+	uint32_t s1 = *r->pNext++;
+	s1 ^= (s1 >> 11);
+	s1 ^= (s1 << 7) & 0x9d2c5680U;
+	s1 ^= (s1 << 15) & 0xefc60000U;
+	return s1 ^ (s1 >> 18);
+}
+
+/* Random::rand, src/Random.cpp:80-82 */
+double orc_rng_rand(orc_rng *r) { return (double)orc_rng_randint(r) * (1.0 / 4294967295.0); }
+
+/* Random::randExc / randDblExc, src/Random.cpp:88-98 */
+static double orc_rng_randexc(orc_rng *r) { return (double)orc_rng_randint(r) * (1.0 / 4294967296.0); }
+static double orc_rng_randdblexc(orc_rng *r) { return ((double)orc_rng_randint(r) + 0.5) * (1.0 / 4294967296.0); }
+
+/* Random::randUniform, src/Random.cpp:118-120 */
+double orc_rng_uniform(orc_rng *r, double min, double max) { return min + (max - min) * orc_rng_rand(r); }
+
+/* Random::randNorm, src/Random.cpp:111-116 */
+double orc_rng_norm(orc_rng *r, double mean, double variance) {
+	double rr = sqrt(-2.0 * log(1.0 - orc_rng_randdblexc(r))) * variance;
+	double phi = 2.0 * 3.14159265358979323846264338328 * orc_rng_randexc(r);
+	return mean + rr * cos(phi);
+}
+
+/* Random::randVector, src/Random.cpp:141-146 */
+void orc_rng_vector(orc_rng *r, double out[3]) {
+	double z = orc_rng_uniform(r, -1.0, 1.0);
+	double t = orc_rng_uniform(r, -1.0 * M_PI, M_PI);
+	double rr = sqrt(1 - z * z);
+	out[0] = rr * cos(t);
+	out[1] = rr * sin(t);
+	out[2] = z;
+}
+
+/* Random::randPowerLaw, src/Random.cpp:189-205 */
+double orc_rng_powerlaw(orc_rng *r, double index, double min, double max) {
+	if (fabs(index + 1.0) < 2.220446049250313e-16) {
+		double part1 = log(max);
+		double part2 = log(min);
+		return exp((part1 - part2) * orc_rng_rand(r) + part2);
+	}
+	double part1 = pow(max, index + 1);
+	double part2 = pow(min, index + 1);
+	double ex = 1 / (index + 1);
+	return pow((part1 - part2) * orc_rng_rand(r) + part2, ex);
+}
+
+/* Random::randBin(vector<double>), src/Random.cpp:135-139 (std::lower_bound) */
+size_t orc_rng_bin(orc_rng *r, const double *cdf, size_t n) {
+	double v = orc_rng_rand(r) * cdf[n - 1];
+	size_t lo = 0, hi = n;
+	while (lo < hi) {
+		size_t mid = lo + (hi - lo) / 2;
+		if (cdf[mid] < v)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	return lo;
+}
+
+void orc_rng_randint_batch(uint32_t seed, size_t n, uint32_t *out) {
+	orc_rng r;
+	orc_rng_seed(&r, seed);
+	for (size_t i = 0; i < n; i++)
+		out[i] = orc_rng_randint(&r);
+}
+
+/* Noise grid used by the golden fixtures: per point (z fastest), components
+ * x, y, z each randUniform(-1, 1) * amp, stored as float. */
+void orc_noise_grid(uint32_t seed, int nx, int ny, int nz, double amp, float *out) {
+	orc_rng r;
+	orc_rng_seed(&r, seed);
+	size_t n = (size_t)nx * ny * nz;
+	for (size_t i = 0; i < 3 * n; i++)
+		out[i] = (float)(orc_rng_uniform(&r, -1, 1) * amp);
+}
+
+/* Per particle: species (randBin over cumulative weights, when nw > 1),
+ * energy randPowerLaw(-1, emin, emax), direction randVector -- the draw order
+ * of the golden fixtures. */
+void orc_sample_particles(uint32_t seed, size_t n, double emin, double emax, const double *weights, size_t nw,
+                          int32_t *species, double *energy, double *dir) {
+	orc_rng r;
+	orc_rng_seed(&r, seed);
+	double cdf[64];
+	double acc = 0;
+	for (size_t i = 0; i < nw && i < 64; i++) {
+		acc += weights[i];
+		cdf[i] = acc;
+	}
+	for (size_t i = 0; i < n; i++) {
+		species[i] = nw > 1 ? (int32_t)orc_rng_bin(&r, cdf, nw) : 0;
+		energy[i] = orc_rng_powerlaw(&r, -1, emin, emax);
+		orc_rng_vector(&r, dir + 3 * i);
+	}
+}
+
+/* ------------------------------------------------------------------ grid */
+/* periodicClamp, include/crpropa/Grid.h:31-34 */
+static inline void orc_periodic_clamp(double x, int n, int *lo, int *hi) {
+	*lo = (((int)floor(x) % n) + n) % n;
+	*hi = (*lo + 1) % n;
+}
+
+/* reflectiveClamp, include/crpropa/Grid.h:49-59 */
+static inline void orc_reflective_clamp(double x, int n, int *lo, int *hi, double *res) {
+	while ((x < -0.5) || (x > (n - 0.5)))
+		x = 2 * n * (x > (n - 0.5)) - x - 1;
+	*res = x;
+	*lo = (int)floor(x);
+	*hi = *lo + (*lo < n - 1);
+	if (x < 0) {
+		*lo = 0;
+		*hi = 0;
+	}
+}
+
+/* Grid<Vector3f>::interpolate + trilinearInterpolate, include/crpropa/Grid.h:318-335,540-582.
+ * Weights are narrowed to float (Vector3<float>::operator*(float)), products
+ * and the 8-term sum run in float, the result widens to double
+ * (src/magneticField/MagneticFieldGrid.cpp:17-19). */
+static void orc_grid_eval1(const orc_field *f, const double p[3], double B[3]) {
+	if (f->clip) {
+		double ex = f->origin[0] + (double)f->nx * f->spacing[0];
+		double ey = f->origin[1] + (double)f->ny * f->spacing[1];
+		double ez = f->origin[2] + (double)f->nz * f->spacing[2];
+		int in = (p[0] >= f->origin[0]) && (p[0] <= ex);
+		in &= (p[1] >= f->origin[1]) && (p[1] <= ey);
+		in &= (p[2] >= f->origin[2]) && (p[2] <= ez);
+		if (!in) {
+			B[0] = B[1] = B[2] = 0;
+			return;
+		}
+	}
+	/* gridOrigin = origin + spacing/2 (Grid.h:225-228); r = (pos - gridOrigin)/spacing */
+	double r[3];
+	for (int a = 0; a < 3; a++) {
+		double go = f->origin[a] + f->spacing[a] / 2;
+		r[a] = (p[a] - go) / f->spacing[a];
+	}
+	int n[3] = {f->nx, f->ny, f->nz};
+	int lo[3], hi[3];
+	double f0[3];
+	for (int a = 0; a < 3; a++) {
+		if (f->reflective) {
+			double res;
+			orc_reflective_clamp(r[a], n[a], &lo[a], &hi[a], &res);
+			f0[a] = res - floor(res);
+		} else {
+			orc_periodic_clamp(r[a], n[a], &lo[a], &hi[a]);
+			f0[a] = r[a] - floor(r[a]);
+		}
+	}
+	float wx0 = (float)f0[0], wy0 = (float)f0[1], wz0 = (float)f0[2];
+	float wx1 = (float)(1 - f0[0]), wy1 = (float)(1 - f0[1]), wz1 = (float)(1 - f0[2]);
+	const int X[8] = {lo[0], hi[0], lo[0], lo[0], hi[0], lo[0], hi[0], hi[0]};
+	const int Y[8] = {lo[1], lo[1], hi[1], lo[1], lo[1], hi[1], hi[1], hi[1]};
+	const int Z[8] = {lo[2], lo[2], lo[2], hi[2], hi[2], hi[2], lo[2], hi[2]};
+	const float WX[8] = {wx1, wx0, wx1, wx1, wx0, wx1, wx0, wx0};
+	const float WY[8] = {wy1, wy1, wy0, wy1, wy1, wy0, wy0, wy0};
+	const float WZ[8] = {wz1, wz1, wz1, wz0, wz0, wz0, wz1, wz0};
+	float b[3] = {0.f, 0.f, 0.f};
+	for (int c = 0; c < 8; c++) {
+		size_t idx = ((size_t)X[c] * f->ny + Y[c]) * f->nz + Z[c];
+		const float *v = f->grid + 3 * idx;
+		for (int a = 0; a < 3; a++) {
+			float t = ((v[a] * WX[c]) * WY[c]) * WZ[c];
+			b[a] += t;
+		}
+	}
+	B[0] = b[0];
+	B[1] = b[1];
+	B[2] = b[2];
+}
+
+/* PlaneWaveTurbulence::getField (default build), src/magneticField/turbulentField/PlaneWaveTurbulence.cpp:231-240 */
+static void orc_planewave_eval1(const orc_field *f, const double p[3], double B[3], int libm) {
+	double bx = 0, by = 0, bz = 0;
+	for (int i = 0; i < f->nm; i++) {
+		const double *kap = f->kappa + 3 * i;
+		const double *xi = f->xi + 3 * i;
+		double z_ = (p[0] * kap[0] + p[1] * kap[1]) + p[2] * kap[2];
+		double c = orc_cos(f->k[i] * z_ + f->beta[i], libm);
+		double A = f->Ak[i];
+		bx += (xi[0] * A) * c;
+		by += (xi[1] * A) * c;
+		bz += (xi[2] * A) * c;
+	}
+	B[0] = bx;
+	B[1] = by;
+	B[2] = bz;
+}
+
+/* MagneticField::getField dispatch + PropagationBP/CK::getFieldAtPosition
+ * (src/module/PropagationBP.cpp:137-149): a missing field gives B = 0. */
+static inline void orc_field_eval1(const orc_field *f, const double p[3], double B[3], int libm) {
+	switch (f ? f->type : ORC_FIELD_NONE) {
+	case ORC_FIELD_UNIFORM: B[0] = f->B0[0]; B[1] = f->B0[1]; B[2] = f->B0[2]; break;
+	case ORC_FIELD_GRID: orc_grid_eval1(f, p, B); break;
+	case ORC_FIELD_PLANEWAVE: orc_planewave_eval1(f, p, B, libm); break;
+	default: B[0] = B[1] = B[2] = 0; break;
+	}
+}
+
+void orc_field_eval(const orc_field *f, const double *pos, size_t m, double *B, int libm) {
+	for (size_t i = 0; i < m; i++)
+		orc_field_eval1(f, pos + 3 * i, B + 3 * i, libm);
+}
+
+/* PlaneWaveTurbulence ctor, src/magneticField/turbulentField/PlaneWaveTurbulence.cpp:99-178 (seed != 0).
+ * simple_spectrum selects E(k) = k^(-s-2) (SimpleTurbulenceSpectrum,
+ * include/crpropa/magneticField/turbulentField/SimpleGridTurbulence.h:36-38; lBendover is
+ * then 1000*lMax) instead of TurbulenceSpectrum::energySpectrum
+ * (include/crpropa/magneticField/turbulentField/TurbulentField.h:72-77). */
+void orc_planewave_init(double brms, double lmin, double lmax, double lbendover, double sindex, double qindex,
+                        int simple_spectrum, int nm, int seed, double *k, double *Ak, double *xi, double *kappa,
+                        double *beta) {
+	orc_rng r;
+	orc_rng_seed(&r, (uint32_t)seed);
+	double kmax = 2 * M_PI / lmin;
+	double kmin = 2 * M_PI / lmax;
+	double delta = log10(kmax / kmin);
+	for (int i = 0; i < nm; i++)
+		k[i] = pow(10, log10(kmin) + ((double)i) / ((double)(nm - 1)) * delta);
+	double delta_k0 = (k[1] - k[0]) / k[1];
+	double Ak2_sum = 0;
+	for (int i = 0; i < nm; i++) {
+		double kk = k[i];
+		double kHat = kk * lbendover;
+		double E;
+		if (simple_spectrum) {
+			E = pow(kk, -sindex - 2);
+		} else {
+			double kh = kk * lbendover;
+			E = pow(kh, qindex) / pow(1.0 + kh * kh, (sindex + qindex) / 2.0 + 1.0);
+		}
+		double Gk = E * (1 + kHat * kHat);
+		Ak[i] = Gk * delta_k0 * kk;
+		Ak2_sum += Ak[i];
+		double phi = orc_rng_uniform(&r, -M_PI, M_PI);
+		double costheta = orc_rng_uniform(&r, -1., 1.);
+		double sintheta = sqrt(1 - costheta * costheta);
+		double alpha = orc_rng_uniform(&r, 0, 2 * M_PI);
+		double b = orc_rng_uniform(&r, 0, 2 * M_PI);
+		kappa[3 * i] = sintheta * cos(phi);
+		kappa[3 * i + 1] = sintheta * sin(phi);
+		kappa[3 * i + 2] = costheta;
+		xi[3 * i] = costheta * cos(phi) * cos(alpha) + sin(phi) * sin(alpha);
+		xi[3 * i + 1] = costheta * sin(phi) * cos(alpha) - cos(phi) * sin(alpha);
+		xi[3 * i + 2] = -sintheta * cos(alpha);
+		beta[i] = b;
+	}
+	for (int i = 0; i < nm; i++)
+		Ak[i] = sqrt(2 * Ak[i] / Ak2_sum) * brms;
+}
+
+/* ------------------------------------------------------------------ propagators */
+typedef struct {
+	double x[3], u[3];
+} orc_y;
+
+/* PropagationBP::dY, src/module/PropagationBP.cpp:19-39 */
+static inline orc_y orc_bp_dy(const orc_field *f, const double pos0[3], const double dir0[3], double step, double q,
+                              double m, int libm) {
+	double pos[3], dir[3], B[3], t[3], s[3], v[3];
+	for (int a = 0; a < 3; a++)
+		pos[a] = pos0[a] + (dir0[a] * step) / 2.;
+	orc_field_eval1(f, pos, B, libm);
+	for (int a = 0; a < 3; a++)
+		t[a] = ((((B[a] * q) / 2) / m) * step) / C_LIGHT;
+	double tt = (t[0] * t[0] + t[1] * t[1]) + t[2] * t[2];
+	for (int a = 0; a < 3; a++)
+		s[a] = (t[a] * 2) / (1 + tt);
+	/* v_help = dir + dir x t */
+	v[0] = dir0[0] + (dir0[1] * t[2] - t[1] * dir0[2]);
+	v[1] = dir0[1] + (dir0[2] * t[0] - t[2] * dir0[0]);
+	v[2] = dir0[2] + (dir0[0] * t[1] - t[0] * dir0[1]);
+	/* dir = dir + v_help x s */
+	dir[0] = dir0[0] + (v[1] * s[2] - s[1] * v[2]);
+	dir[1] = dir0[1] + (v[2] * s[0] - s[2] * v[0]);
+	dir[2] = dir0[2] + (v[0] * s[1] - s[0] * v[1]);
+	orc_y y;
+	for (int a = 0; a < 3; a++) {
+		y.x[a] = pos[a] + (dir[a] * step) / 2.;
+		y.u[a] = dir[a];
+	}
+	return y;
+}
+
+static inline double orc_norm(const double v[3]) { return sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]); }
+
+/* PropagationCK::dYdt, src/module/PropagationCK.cpp:51-61 */
+static inline orc_y orc_ck_dydt(const orc_field *f, const orc_y *y, double q, double E, int libm) {
+	orc_y d;
+	double n = orc_norm(y->u);
+	double B[3];
+	for (int a = 0; a < 3; a++)
+		d.x[a] = (y->u[a] / n) * C_LIGHT;
+	orc_field_eval1(f, y->x, B, libm);
+	double fac = (q * C_LIGHT) / E;
+	d.u[0] = (d.x[1] * B[2] - B[1] * d.x[2]) * fac;
+	d.u[1] = (d.x[2] * B[0] - B[2] * d.x[0]) * fac;
+	d.u[2] = (d.x[0] * B[1] - B[0] * d.x[1]) * fac;
+	return d;
+}
+
+/* Cash-Karp tableau, src/module/PropagationCK.cpp:11-26 (textbook constants) */
+static const double CK_A[36] = {0., 0., 0., 0., 0., 0.,
+                                1. / 5., 0., 0., 0., 0., 0.,
+                                3. / 40., 9. / 40., 0., 0., 0., 0.,
+                                3. / 10., -9. / 10., 6. / 5., 0., 0., 0.,
+                                -11. / 54., 5. / 2., -70. / 27., 35. / 27., 0., 0.,
+                                1631. / 55296., 175. / 512., 575. / 13824., 44275. / 110592., 253. / 4096., 0.};
+static const double CK_B[6] = {37. / 378., 0, 250. / 621., 125. / 594., 0., 512. / 1771.};
+static const double CK_BS[6] = {2825. / 27648., 0., 18575. / 48384., 13525. / 55296., 277. / 14336., 1. / 4.};
+
+/* PropagationCK::tryStep, src/module/PropagationCK.cpp:28-49 */
+static inline void orc_ck_trystep(const orc_field *f, const orc_y *y, orc_y *out, orc_y *err, double h, double q,
+                                  double E, int libm) {
+	orc_y k[6];
+	*out = *y;
+	for (int a = 0; a < 3; a++)
+		err->x[a] = err->u[a] = 0;
+	for (int i = 0; i < 6; i++) {
+		orc_y yn = *y;
+		for (int j = 0; j < i; j++) {
+			double aij = CK_A[i * 6 + j];
+			for (int a = 0; a < 3; a++) {
+				yn.x[a] += (k[j].x[a] * aij) * h;
+				yn.u[a] += (k[j].u[a] * aij) * h;
+			}
+		}
+		k[i] = orc_ck_dydt(f, &yn, q, E, libm);
+		double bi = CK_B[i], ei = CK_B[i] - CK_BS[i];
+		for (int a = 0; a < 3; a++) {
+			out->x[a] += (k[i].x[a] * bi) * h;
+			out->u[a] += (k[i].u[a] * bi) * h;
+			err->x[a] += (k[i].x[a] * ei) * h;
+			err->u[a] += (k[i].u[a] * ei) * h;
+		}
+	}
+}
+
+static inline double orc_clip(double x, double lo, double hi) {
+	/* std::max(lower, std::min(x, upper)), include/crpropa/Common.h:29-32 */
+	double m = (hi < x) ? hi : x;
+	return (m < lo) ? lo : m;
+}
+static inline double orc_max(double a, double b) { return (a < b) ? b : a; } /* std::max */
+static inline double orc_min(double a, double b) { return (b < a) ? b : a; } /* std::min */
+
+/* One process() call of PropagationBP (src/module/PropagationBP.cpp:62-124) or
+ * PropagationCK (src/module/PropagationCK.cpp:77-136), followed by
+ * MaximumTrajectoryLength::process (src/module/BreakCondition.cpp:42-64). */
+static inline void orc_process(const orc_field *f, const orc_propagator *p, orc_state *s, size_t i, int libm,
+                               long long *trials) {
+	double *pos = s->pos + 3 * i, *dir = s->dir + 3 * i;
+	if (s->prev_pos) {
+		for (int a = 0; a < 3; a++) {
+			s->prev_pos[3 * i + a] = pos[a];
+			s->prev_dir[3 * i + a] = dir[a];
+		}
+	}
+	double q = s->charge[i];
+	double step = p->maxStep;
+	double newStep;
+	orc_y yIn, yOut, yErr;
+	for (int a = 0; a < 3; a++) {
+		yIn.x[a] = pos[a];
+		yIn.u[a] = dir[a];
+	}
+	if (q == 0) {
+		step = orc_clip(s->next_step[i], p->minStep, p->maxStep);
+		for (int a = 0; a < 3; a++)
+			pos[a] = yIn.x[a] + yIn.u[a] * step;
+		newStep = p->maxStep;
+	} else {
+		newStep = step;
+		double E = s->energy[i];
+		if (p->minStep == p->maxStep) {
+			if (p->type == ORC_PROP_BP) {
+				double m = E / (C_LIGHT * C_LIGHT);
+				yOut = orc_bp_dy(f, yIn.x, yIn.u, step, q, m, libm);
+			} else {
+				orc_ck_trystep(f, &yIn, &yOut, &yErr, step / C_LIGHT, q, E, libm);
+			}
+		} else {
+			step = orc_clip(s->next_step[i], p->minStep, p->maxStep);
+			newStep = step;
+			double r;
+			double m = E / (C_LIGHT * C_LIGHT);
+			for (;;) {
+				if (p->type == ORC_PROP_BP) {
+					/* tryStep + errorEstimation, src/module/PropagationBP.cpp:8-16,152-159 */
+					yOut = orc_bp_dy(f, yIn.x, yIn.u, step, q, m, libm);
+					orc_y half = orc_bp_dy(f, yIn.x, yIn.u, step / 2, q, m, libm);
+					orc_y cmp = orc_bp_dy(f, half.x, half.u, step / 2, q, m, libm);
+					double diff[3] = {yOut.x[0] - cmp.x[0], yOut.x[1] - cmp.x[1], yOut.x[2] - cmp.x[2]};
+					double S = orc_norm(diff) / (step * (1 - 1 / 4.));
+					/* Y(double) sets all three u components to S (PropagationBP.h:40-42) */
+					r = sqrt((S * S + S * S) + S * S) / p->tolerance;
+				} else {
+					orc_ck_trystep(f, &yIn, &yOut, &yErr, step / C_LIGHT, q, E, libm);
+					r = orc_norm(yErr.u) / p->tolerance;
+				}
+				if (r > 1) {
+					if (step == p->minStep)
+						break;
+					(*trials)++;
+					newStep = step * 0.95 * orc_pow(r, -0.2, libm);
+					newStep = orc_max(newStep, 0.1 * step);
+					newStep = orc_max(newStep, p->minStep);
+					step = newStep;
+				} else {
+					if (step != p->maxStep) {
+						newStep = step * 0.95 * orc_pow(r, -0.2, libm);
+						newStep = orc_min(newStep, 5 * step);
+						newStep = orc_min(newStep, p->maxStep);
+					}
+					break;
+				}
+			}
+		}
+		/* setPosition; setDirection(getUnitVector()) normalises twice
+		 * (src/module/PropagationBP.cpp:120-121, src/ParticleState.cpp:30-32) */
+		double n1 = orc_norm(yOut.u);
+		double u1[3] = {yOut.u[0] / n1, yOut.u[1] / n1, yOut.u[2] / n1};
+		double n2 = orc_norm(u1);
+		for (int a = 0; a < 3; a++) {
+			pos[a] = yOut.x[a];
+			dir[a] = u1[a] / n2;
+		}
+	}
+	/* Candidate::setCurrentStep / setNextStep, src/Candidate.cpp:92-100 */
+	s->current_step[i] = step;
+	s->trajectory_length[i] += step;
+	s->time[i] += step / C_LIGHT;
+	s->next_step[i] = newStep;
+	if (p->maxTrajectoryLength > 0) {
+		double len = s->trajectory_length[i];
+		if (len >= p->maxTrajectoryLength)
+			s->active[i] = 0;
+		else
+			s->next_step[i] = orc_min(s->next_step[i], p->maxTrajectoryLength - len);
+	}
+}
+
+long long orc_propagate(const orc_field *f, const orc_propagator *p, orc_state *s, size_t n, long long maxSteps,
+                        int libm, int nthreads, long long *trials) {
+	long long total = 0, rej = 0;
+#ifdef _OPENMP
+	if (nthreads <= 0)
+		nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : total, rej) num_threads(nthreads)
+#endif
+	for (size_t i = 0; i < n; i++) {
+		long long local = 0;
+		for (long long k = 0; k < maxSteps && s->active[i]; k++) {
+			orc_process(f, p, s, i, libm, &rej);
+			local++;
+		}
+		total += local;
+	}
+	if (trials)
+		*trials = rej;
+	return total;
+}

@@ -1,0 +1,101 @@
+/*
+ * ORACLE -- test infrastructure, not product code.
+ *
+ * A plain-C restatement of the reference's charged-particle propagation hot
+ * path (CRPropa3 PropagationBP / PropagationCK over Grid3f trilinear fields,
+ * PlaneWaveTurbulence and uniform fields, MaximumTrajectoryLength, and the
+ * MT19937 Random used to draw inputs).  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg load it, as the checker.
+ *
+ * Every function cites the reference file:line whose arithmetic it restates,
+ * in the same operation order (see crp_oracle.c).  libm selects the
+ * transcendental functions: ORC_LIBM_GLIBC calls the host's cos()/pow()
+ * exactly as the reference does; ORC_LIBM_CR uses the correctly rounded
+ * implementations that the GPU kernels use.
+ */
+#ifndef CRP_ORACLE_H
+#define CRP_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_LIBM_GLIBC = 0, ORC_LIBM_CR = 1 };
+enum { ORC_FIELD_NONE = 0, ORC_FIELD_UNIFORM = 1, ORC_FIELD_GRID = 2, ORC_FIELD_PLANEWAVE = 3 };
+enum { ORC_PROP_BP = 0, ORC_PROP_CK = 1 };
+
+typedef struct {
+	int type;
+	double B0[3]; /* uniform */
+	/* Grid3f: float32 values, AoS (x,y,z) per point, index (ix*Ny + iy)*Nz + iz */
+	const float *grid;
+	int nx, ny, nz;
+	double origin[3], spacing[3];
+	int reflective, clip;
+	/* plane waves: Nm modes, xi/kappa AoS 3*Nm */
+	int nm;
+	const double *k, *Ak, *xi, *kappa, *beta;
+} orc_field;
+
+typedef struct {
+	int type; /* ORC_PROP_BP / ORC_PROP_CK */
+	double tolerance, minStep, maxStep;
+	double maxTrajectoryLength; /* <= 0: no MaximumTrajectoryLength module */
+} orc_propagator;
+
+/* Particle state, structure of arrays.  pos/dir are AoS (3 per particle). */
+typedef struct {
+	double *pos, *dir;
+	double *energy, *charge, *redshift;
+	double *current_step, *next_step, *trajectory_length, *time;
+	int32_t *active;
+	double *prev_pos, *prev_dir; /* may be NULL */
+} orc_state;
+
+/* MT19937 as in src/Random.cpp */
+typedef struct {
+	uint32_t state[624];
+	uint32_t *pNext;
+	int left;
+} orc_rng;
+
+void orc_rng_seed(orc_rng *r, uint32_t seed);
+uint32_t orc_rng_randint(orc_rng *r);
+double orc_rng_rand(orc_rng *r);
+double orc_rng_uniform(orc_rng *r, double min, double max);
+double orc_rng_norm(orc_rng *r, double mean, double variance);
+void orc_rng_vector(orc_rng *r, double out[3]);
+double orc_rng_powerlaw(orc_rng *r, double index, double min, double max);
+size_t orc_rng_bin(orc_rng *r, const double *cdf, size_t n);
+
+/* batch helpers for the tests */
+void orc_rng_randint_batch(uint32_t seed, size_t n, uint32_t *out);
+void orc_noise_grid(uint32_t seed, int nx, int ny, int nz, double amp, float *out);
+void orc_sample_particles(uint32_t seed, size_t n, double emin, double emax, const double *weights, size_t nw,
+                          int32_t *species, double *energy, double *dir);
+
+/* fields */
+void orc_field_eval(const orc_field *f, const double *pos, size_t m, double *B, int libm);
+void orc_planewave_init(double brms, double lmin, double lmax, double lbendover, double sindex, double qindex,
+                        int simple_spectrum, int nm, int seed, double *k, double *Ak, double *xi, double *kappa,
+                        double *beta);
+
+/* propagation: maxSteps process() calls per particle (stopping early when
+ * inactive), OpenMP over particles with nthreads (0: default). Returns the
+ * total number of accepted steps and writes rejected trial counts to *trials. */
+long long orc_propagate(const orc_field *f, const orc_propagator *p, orc_state *s, size_t n, long long maxSteps,
+                        int libm, int nthreads, long long *trials);
+
+/* elementary functions, for the math tests */
+double orc_cos(double x, int libm);
+double orc_pow(double x, double y, int libm);
+void orc_cos_batch(const double *x, size_t n, double *out, int libm);
+void orc_pow_batch(const double *x, double y, size_t n, double *out, int libm);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

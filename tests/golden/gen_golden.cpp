@@ -1,0 +1,458 @@
+// Golden-vector generator for the propagation hot path.
+//
+// Links against the reference CRPropa3 C++ (built from its own sources by
+// oracle/Makefile, target `ref`) and writes small .npy fixtures under
+// tests/golden/<case>/.  Only this container runs it; the fixtures (data)
+// are what the tests read.
+//
+// What it exercises (reference file:line):
+//   Grid<Vector3f>::interpolate / trilinearInterpolate   include/crpropa/Grid.h:318-335,540-582
+//   PlaneWaveTurbulence ctor + getField                  src/magneticField/turbulentField/PlaneWaveTurbulence.cpp:79-240
+//   PropagationBP::process (fixed and adaptive)          src/module/PropagationBP.cpp:62-124
+//   PropagationCK::process                               src/module/PropagationCK.cpp:77-136
+//   MaximumTrajectoryLength::process                     src/module/BreakCondition.cpp:42-64
+//   Random (MT19937, randUniform, randVector, randPowerLaw, randBin) src/Random.cpp:80-205
+//
+// Particle ids: every species uses nucleusId(60, Z) so that nuclearMass()
+// takes its analytic branch (A > 56, src/ParticleMass.cpp:70-78) and the
+// downloaded mass table is never read; the charge is Z*eplus as for any
+// nucleus (src/ParticleState.cpp:50-56).  Rest mass is not used by BP
+// (m = E/c^2, src/module/PropagationBP.cpp:85) or CK (q/E,
+// src/module/PropagationCK.cpp:59), so trajectories do not depend on A.
+
+#include "crpropa/Candidate.h"
+#include "crpropa/Grid.h"
+#include "crpropa/ParticleID.h"
+#include "crpropa/Random.h"
+#include "crpropa/Units.h"
+#include "crpropa/magneticField/MagneticField.h"
+#include "crpropa/magneticField/MagneticFieldGrid.h"
+#include "crpropa/magneticField/turbulentField/PlaneWaveTurbulence.h"
+#include "crpropa/module/BreakCondition.h"
+#include "crpropa/module/PropagationBP.h"
+#include "crpropa/module/PropagationCK.h"
+
+#include <cstdio>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <sys/stat.h>
+#include <gnu/libc-version.h>
+
+using namespace crpropa;
+
+static std::string g_root;
+
+// ---------------------------------------------------------------- npy writer
+static void write_npy(const std::string &path, const char *descr, const std::vector<size_t> &shape,
+                      const void *data, size_t nbytes) {
+	std::string sh = "(";
+	for (size_t i = 0; i < shape.size(); i++) {
+		sh += std::to_string(shape[i]);
+		sh += (shape.size() == 1 || i + 1 < shape.size()) ? "," : "";
+		if (i + 1 < shape.size()) sh += " ";
+	}
+	sh += ")";
+	std::string header = std::string("{'descr': '") + descr + "', 'fortran_order': False, 'shape': " + sh + ", }";
+	size_t total = 10 + header.size() + 1;
+	size_t pad = (64 - total % 64) % 64;
+	header.append(pad, ' ');
+	header += '\n';
+	FILE *f = fopen(path.c_str(), "wb");
+	if (!f) { perror(path.c_str()); exit(1); }
+	const unsigned char magic[8] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0};
+	fwrite(magic, 1, 8, f);
+	uint16_t hl = (uint16_t)header.size();
+	fwrite(&hl, 2, 1, f);
+	fwrite(header.data(), 1, header.size(), f);
+	fwrite(data, 1, nbytes, f);
+	fclose(f);
+}
+
+static std::string casedir(const std::string &name) {
+	std::string d = g_root + "/" + name;
+	mkdir(d.c_str(), 0755);
+	return d;
+}
+
+static void save_f64(const std::string &dir, const std::string &name, const std::vector<double> &v,
+                     std::vector<size_t> shape) {
+	write_npy(dir + "/" + name + ".npy", "<f8", shape, v.data(), v.size() * 8);
+}
+static void save_f32(const std::string &dir, const std::string &name, const std::vector<float> &v,
+                     std::vector<size_t> shape) {
+	write_npy(dir + "/" + name + ".npy", "<f4", shape, v.data(), v.size() * 4);
+}
+static void save_i32(const std::string &dir, const std::string &name, const std::vector<int32_t> &v,
+                     std::vector<size_t> shape) {
+	write_npy(dir + "/" + name + ".npy", "<i4", shape, v.data(), v.size() * 4);
+}
+
+static void save_meta(const std::string &dir, const std::string &json) {
+	FILE *f = fopen((dir + "/meta.json").c_str(), "w");
+	fprintf(f, "%s\n", json.c_str());
+	fclose(f);
+}
+
+static std::string fmt(double x) {
+	char b[64];
+	snprintf(b, sizeof b, "%.17g", x);
+	return b;
+}
+
+// ---------------------------------------------------------------- inputs
+// Spectrum equivalent to SimpleTurbulenceSpectrum (FFTW-gated in the
+// reference, include/crpropa/magneticField/turbulentField/SimpleGridTurbulence.h:21-41):
+// E(k) = k^(-s-2), lBendover = 1000 lMax, q = 0.
+class SimpleSpectrum : public TurbulenceSpectrum {
+public:
+	SimpleSpectrum(double Brms, double lMin, double lMax, double s = 5. / 3)
+	    : TurbulenceSpectrum(Brms, lMin, lMax, 1000 * lMax, s, 0) {}
+	double energySpectrum(double k) const { return std::pow(k, -getSindex() - 2); }
+};
+
+// Grid filled with uniform white noise: each component randUniform(-1,1)*amp,
+// drawn x,y,z per point in storage order (ix, iy, iz), z fastest.
+static ref_ptr<Grid3f> make_noise_grid(int n, double spacing, Vector3d origin, uint32_t seed, double amp) {
+	ref_ptr<Grid3f> g = new Grid3f(origin, n, spacing);
+	Random rng(seed);
+	for (int ix = 0; ix < n; ix++)
+		for (int iy = 0; iy < n; iy++)
+			for (int iz = 0; iz < n; iz++) {
+				Vector3f &b = g->get(ix, iy, iz);
+				b.x = rng.randUniform(-1, 1) * amp;
+				b.y = rng.randUniform(-1, 1) * amp;
+				b.z = rng.randUniform(-1, 1) * amp;
+			}
+	return g;
+}
+
+static void save_grid(const std::string &dir, ref_ptr<Grid3f> g) {
+	size_t n = g->getNx() * g->getNy() * g->getNz();
+	std::vector<float> v(n * 3);
+	for (size_t i = 0; i < n; i++) {
+		const Vector3f &b = g->getGrid()[i];
+		v[3 * i] = b.x; v[3 * i + 1] = b.y; v[3 * i + 2] = b.z;
+	}
+	save_f32(dir, "grid", v, {g->getNx(), g->getNy(), g->getNz(), 3});
+}
+
+struct Particle {
+	int id;
+	double E;
+	Vector3d pos, dir;
+};
+
+// Draw order per particle: species (if weights given), energy, direction.
+static std::vector<Particle> make_particles(size_t N, uint32_t seed, double Emin, double Emax, Vector3d start,
+                                            const std::vector<int> &ids, const std::vector<double> &weights) {
+	Random rng(seed);
+	std::vector<double> cdf;
+	double acc = 0;
+	for (double w : weights) { acc += w; cdf.push_back(acc); }
+	std::vector<Particle> ps(N);
+	for (size_t i = 0; i < N; i++) {
+		int id = ids[0];
+		if (ids.size() > 1) id = ids[rng.randBin(cdf)];
+		ps[i].id = id;
+		ps[i].E = rng.randPowerLaw(-1, Emin, Emax);
+		ps[i].dir = rng.randVector();
+		ps[i].pos = start;
+	}
+	return ps;
+}
+
+static void save_particles(const std::string &dir, const std::vector<Particle> &ps) {
+	size_t N = ps.size();
+	std::vector<double> pos(3 * N), d(3 * N), E(N), q(N);
+	std::vector<int32_t> id(N);
+	for (size_t i = 0; i < N; i++) {
+		pos[3 * i] = ps[i].pos.x; pos[3 * i + 1] = ps[i].pos.y; pos[3 * i + 2] = ps[i].pos.z;
+		d[3 * i] = ps[i].dir.x; d[3 * i + 1] = ps[i].dir.y; d[3 * i + 2] = ps[i].dir.z;
+		E[i] = ps[i].E;
+		id[i] = ps[i].id;
+		ParticleState s(ps[i].id, ps[i].E, ps[i].pos, ps[i].dir);
+		q[i] = s.getCharge();
+	}
+	save_f64(dir, "init_pos", pos, {N, 3});
+	save_f64(dir, "init_dir", d, {N, 3});
+	save_f64(dir, "energy", E, {N});
+	save_f64(dir, "charge", q, {N});
+	save_i32(dir, "id", id, {N});
+}
+
+// ---------------------------------------------------------------- trajectories
+// Runs `prop` (then the optional length limit) on each particle, the way
+// ModuleList::run(Candidate*) loops (src/ModuleList.cpp.in:64-67), for at most
+// maxSteps process() calls, recording the state after the listed step counts.
+static void run_traj(const std::string &name, const Module &prop, const MaximumTrajectoryLength *maxlen,
+                     const std::vector<Particle> &ps, size_t maxSteps, const std::vector<size_t> &snaps,
+                     const std::string &meta) {
+	std::string dir = casedir(name);
+	save_particles(dir, ps);
+	size_t N = ps.size(), S = snaps.size();
+	std::vector<double> pos(S * N * 3), d(S * N * 3), cur(S * N), nxt(S * N), len(S * N), tim(S * N);
+	std::vector<int32_t> act(S * N), nsteps(N);
+	std::vector<double> prevpos(N * 3);
+#pragma omp parallel for schedule(dynamic, 4)
+	for (size_t i = 0; i < N; i++) {
+		ParticleState st(ps[i].id, ps[i].E, ps[i].pos, ps[i].dir);
+		ref_ptr<Candidate> c = new Candidate(st);
+		size_t si = 0, step = 0;
+		while (step < maxSteps && c->isActive()) {
+			prop.process(c);
+			if (maxlen) maxlen->process(c);
+			step++;
+			while (si < S && snaps[si] == step) {
+				size_t k = si * N + i;
+				Vector3d p = c->current.getPosition(), u = c->current.getDirection();
+				pos[3 * k] = p.x; pos[3 * k + 1] = p.y; pos[3 * k + 2] = p.z;
+				d[3 * k] = u.x; d[3 * k + 1] = u.y; d[3 * k + 2] = u.z;
+				cur[k] = c->getCurrentStep(); nxt[k] = c->getNextStep();
+				len[k] = c->getTrajectoryLength(); tim[k] = c->getTime();
+				act[k] = c->isActive() ? 1 : 0;
+				si++;
+			}
+		}
+		nsteps[i] = (int32_t)step;
+		// snapshots past the particle's end hold its final state
+		for (; si < S; si++) {
+			size_t k = si * N + i;
+			Vector3d p = c->current.getPosition(), u = c->current.getDirection();
+			pos[3 * k] = p.x; pos[3 * k + 1] = p.y; pos[3 * k + 2] = p.z;
+			d[3 * k] = u.x; d[3 * k + 1] = u.y; d[3 * k + 2] = u.z;
+			cur[k] = c->getCurrentStep(); nxt[k] = c->getNextStep();
+			len[k] = c->getTrajectoryLength(); tim[k] = c->getTime();
+			act[k] = c->isActive() ? 1 : 0;
+		}
+		Vector3d pp = c->previous.getPosition();
+		prevpos[3 * i] = pp.x; prevpos[3 * i + 1] = pp.y; prevpos[3 * i + 2] = pp.z;
+	}
+	std::vector<int32_t> sn(snaps.begin(), snaps.end());
+	save_i32(dir, "snap_steps", sn, {S});
+	save_f64(dir, "pos", pos, {S, N, 3});
+	save_f64(dir, "dir", d, {S, N, 3});
+	save_f64(dir, "current_step", cur, {S, N});
+	save_f64(dir, "next_step", nxt, {S, N});
+	save_f64(dir, "trajectory_length", len, {S, N});
+	save_f64(dir, "time", tim, {S, N});
+	save_i32(dir, "active", act, {S, N});
+	save_i32(dir, "nsteps", nsteps, {N});
+	save_f64(dir, "final_previous_pos", prevpos, {N, 3});
+	save_meta(dir, meta);
+	printf("  %-28s N=%zu steps<=%zu\n", name.c_str(), N, maxSteps);
+}
+
+// ---------------------------------------------------------------- cases
+static void gen_trilinear() {
+	// 16^3 grid with a non-trivial origin; spacing 50 kpc; amplitude 1 nG.
+	const int n = 16;
+	const double sp = 50 * kpc;
+	Vector3d origin(-3.1 * kpc, 7.25 * kpc, 0.0);
+	ref_ptr<Grid3f> g = make_noise_grid(n, sp, origin, 1234, nG);
+	Random rng(99);
+	const size_t M = 4096;
+	std::vector<double> pos(M * 3);
+	double L = n * sp;
+	for (size_t i = 0; i < M; i++) {
+		for (int a = 0; a < 3; a++) {
+			double o = origin.data[a];
+			double x;
+			switch (i % 8) {
+			case 0: x = o + rng.randUniform(0, L); break;                          // inside
+			case 1: x = o + rng.randUniform(-3 * L, 4 * L); break;                 // wrapped
+			case 2: x = o + sp / 2 + sp * (int)rng.randUniform(-n, 2 * n); break;  // on grid points
+			case 3: x = o + sp * (int)rng.randUniform(-n, 2 * n); break;           // on cell faces
+			case 4: x = o + rng.randUniform(-sp, sp); break;                       // near the lower edge
+			case 5: x = o + L + rng.randUniform(-sp, sp); break;                   // near the upper edge
+			case 6: x = o + rng.randUniform(-1e4 * L, 1e4 * L); break;             // far away
+			default: x = rng.randUniform(-1, 1) * sp; break;                       // around 0
+			}
+			pos[3 * i + a] = x;
+		}
+	}
+	std::string dir = casedir("trilinear16");
+	save_grid(dir, g);
+	save_f64(dir, "pos", pos, {M, 3});
+	const char *names[3] = {"B_periodic", "B_reflective", "B_clip"};
+	for (int mode = 0; mode < 3; mode++) {
+		g->setReflective(mode == 1);
+		g->setClipVolume(mode == 2);
+		std::vector<double> B(M * 3);
+		for (size_t i = 0; i < M; i++) {
+			Vector3d b = g->interpolate(Vector3d(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]));
+			B[3 * i] = b.x; B[3 * i + 1] = b.y; B[3 * i + 2] = b.z;
+		}
+		save_f64(dir, names[mode], B, {M, 3});
+	}
+	save_meta(dir, "{\"n\": 16, \"spacing\": " + fmt(sp) + ", \"origin\": [" + fmt(origin.x) + ", " + fmt(origin.y) +
+	                   ", " + fmt(origin.z) + "], \"grid_seed\": 1234, \"amp\": " + fmt(nG) + "}");
+	printf("  trilinear16 M=%zu\n", M);
+}
+
+static void gen_planewave_case(const std::string &name, const TurbulenceSpectrum &spec, int Nm, int seed,
+                               double posScale, const std::string &meta) {
+	PlaneWaveTurbulence pw(spec, Nm, seed);
+	Random rng(7);
+	const size_t M = 2048;
+	std::vector<double> pos(M * 3), B(M * 3);
+	for (size_t i = 0; i < M; i++) {
+		Vector3d p = rng.randVector() * rng.randUniform(0, posScale);
+		Vector3d b = pw.getField(p);
+		pos[3 * i] = p.x; pos[3 * i + 1] = p.y; pos[3 * i + 2] = p.z;
+		B[3 * i] = b.x; B[3 * i + 1] = b.y; B[3 * i + 2] = b.z;
+	}
+	std::string dir = casedir(name);
+	save_f64(dir, "pos", pos, {M, 3});
+	save_f64(dir, "B", B, {M, 3});
+	save_meta(dir, meta);
+	printf("  %-28s Nm=%d\n", name.c_str(), Nm);
+}
+
+static std::string grid_meta(const std::string &extra, int n, double sp, uint32_t seed, double amp) {
+	return "{\"grid\": {\"n\": " + std::to_string(n) + ", \"spacing\": " + fmt(sp) +
+	       ", \"origin\": [0, 0, 0], \"seed\": " + std::to_string(seed) + ", \"amp\": " + fmt(amp) + "}, " + extra +
+	       "}";
+}
+
+int main(int argc, char **argv) {
+	g_root = argc > 1 ? argv[1] : "tests/golden";
+	mkdir(g_root.c_str(), 0755);
+	printf("gen_golden: glibc %s, g++ %s\n", gnu_get_libc_version(), __VERSION__);
+	{
+		FILE *f = fopen((g_root + "/PROVENANCE.txt").c_str(), "w");
+		fprintf(f, "Generated by tests/golden/gen_golden.cpp linked against the reference C++ (oracle/Makefile target ref)\n");
+		fprintf(f, "compiler: g++ %s\nflags: -O3 -DNDEBUG -std=c++11 -fopenmp (no -march: no FMA)\nglibc: %s\n",
+		        __VERSION__, gnu_get_libc_version());
+		fclose(f);
+	}
+
+	gen_trilinear();
+
+	// Plane waves: the C3 mode table (Simple spectrum), the reference test's
+	// general spectrum (test/testPropagation.cpp:492), and a 2-mode table.
+	SimpleSpectrum specC3(1 * muG, 1 * pc, 100 * pc);
+	gen_planewave_case("planewave_c3", specC3, 256, 11, 500 * pc,
+	                   "{\"spectrum\": \"simple\", \"Brms\": " + fmt(muG) + ", \"lMin\": " + fmt(pc) +
+	                       ", \"lMax\": " + fmt(100 * pc) + ", \"sIndex\": " + fmt(5. / 3) +
+	                       ", \"Nm\": 256, \"seed\": 11}");
+	TurbulenceSpectrum specGen(gauss, pc, 100 * pc);
+	gen_planewave_case("planewave_general10", specGen, 10, 1, 50 * pc,
+	                   "{\"spectrum\": \"general\", \"Brms\": " + fmt(gauss) + ", \"lMin\": " + fmt(pc) +
+	                       ", \"lMax\": " + fmt(100 * pc) + ", \"lBendover\": 1, \"sIndex\": " + fmt(5. / 3) +
+	                       ", \"qIndex\": 4, \"Nm\": 10, \"seed\": 1}");
+	SimpleSpectrum spec2(3 * nG, 10 * kpc, 2 * Mpc, 11. / 3);
+	gen_planewave_case("planewave_simple2", spec2, 2, 5, 5 * Mpc,
+	                   "{\"spectrum\": \"simple\", \"Brms\": " + fmt(3 * nG) + ", \"lMin\": " + fmt(10 * kpc) +
+	                       ", \"lMax\": " + fmt(2 * Mpc) + ", \"sIndex\": " + fmt(11. / 3) +
+	                       ", \"Nm\": 2, \"seed\": 5}");
+
+	// ---- grid trajectories (32^3, spacing 50 kpc, white noise 1 nG, periodic)
+	const int n = 32;
+	const double sp = 50 * kpc;
+	const uint32_t gseed = 2024;
+	ref_ptr<Grid3f> g32 = make_noise_grid(n, sp, Vector3d(0.), gseed, nG);
+	{
+		std::string d = casedir("grid32");
+		save_grid(d, g32);
+		save_meta(d, grid_meta("\"note\": \"shared by the *_grid32 trajectory cases\"", n, sp, gseed, nG));
+	}
+	ref_ptr<MagneticField> fgrid = new MagneticFieldGrid(g32);
+	Vector3d centre(n * sp / 2, n * sp / 2, n * sp / 2);
+	std::vector<int> proton = {nucleusId(60, 1)};
+	std::vector<int> species = {nucleusId(60, 1), nucleusId(60, 2), nucleusId(60, 7), nucleusId(60, 14), nucleusId(60, 26)};
+	std::vector<double> one = {1.0};
+	std::vector<double> wts = {0.5, 0.2, 0.15, 0.1, 0.05};
+
+	{
+		std::vector<Particle> ps = make_particles(256, 42, 1 * EeV, 100 * EeV, centre, proton, one);
+		PropagationBP bp(fgrid, 0.1 * Mpc);
+		run_traj("bp_fixed_grid32", bp, nullptr, ps, 1000, {1, 2, 10, 100, 1000},
+		         "{\"propagator\": \"BP\", \"fixedStep\": " + fmt(0.1 * Mpc) +
+		             ", \"field\": \"grid32\", \"particle_seed\": 42, \"Emin\": " + fmt(EeV) +
+		             ", \"Emax\": " + fmt(100 * EeV) + "}");
+	}
+	{
+		std::vector<Particle> ps = make_particles(256, 43, 1 * EeV, 100 * EeV, centre, proton, one);
+		PropagationBP bp(fgrid, 1e-3, 1 * kpc, 1 * Mpc);
+		run_traj("bp_adaptive_grid32", bp, nullptr, ps, 1000, {1, 2, 10, 100, 1000},
+		         "{\"propagator\": \"BP\", \"tolerance\": 1e-3, \"minStep\": " + fmt(kpc) +
+		             ", \"maxStep\": " + fmt(Mpc) + ", \"field\": \"grid32\", \"particle_seed\": 43, \"Emin\": " +
+		             fmt(EeV) + ", \"Emax\": " + fmt(100 * EeV) + "}");
+	}
+	{
+		std::vector<Particle> ps = make_particles(256, 44, 1 * EeV, 300 * EeV, centre, species, wts);
+		PropagationCK ck(fgrid, 1e-4, 1 * kpc, 1 * Mpc);
+		MaximumTrajectoryLength ml(20 * Mpc);
+		run_traj("ck_grid32_maxlen", ck, &ml, ps, 4000, {1, 2, 10, 100, 1000, 4000},
+		         "{\"propagator\": \"CK\", \"tolerance\": 1e-4, \"minStep\": " + fmt(kpc) +
+		             ", \"maxStep\": " + fmt(Mpc) + ", \"maxTrajectoryLength\": " + fmt(20 * Mpc) +
+		             ", \"field\": \"grid32\", \"particle_seed\": 44, \"Emin\": " + fmt(EeV) + ", \"Emax\": " +
+		             fmt(300 * EeV) + ", \"species_weights\": [0.5, 0.2, 0.15, 0.1, 0.05]}");
+	}
+	{
+		std::vector<Particle> ps = make_particles(128, 45, 1 * EeV, 100 * EeV, centre, proton, one);
+		PropagationCK ck(fgrid, 1e-4, 10 * kpc, 10 * kpc);
+		run_traj("ck_fixed_grid32", ck, nullptr, ps, 500, {1, 10, 100, 500},
+		         "{\"propagator\": \"CK\", \"tolerance\": 1e-4, \"minStep\": " + fmt(10 * kpc) +
+		             ", \"maxStep\": " + fmt(10 * kpc) + ", \"field\": \"grid32\", \"particle_seed\": 45, \"Emin\": " +
+		             fmt(EeV) + ", \"Emax\": " + fmt(100 * EeV) + "}");
+	}
+
+	// ---- C1-shaped: 64^3 grid (50 kpc), 4096 protons, BP fixed 0.1 Mpc x 2000
+	{
+		const int n64 = 64;
+		ref_ptr<Grid3f> g64 = make_noise_grid(n64, sp, Vector3d(0.), 4242, nG);
+		ref_ptr<MagneticField> f64 = new MagneticFieldGrid(g64);
+		Vector3d c64(n64 * sp / 2, n64 * sp / 2, n64 * sp / 2);
+		std::vector<Particle> ps = make_particles(4096, 42, 1 * EeV, 100 * EeV, c64, proton, one);
+		PropagationBP bp(f64, 0.1 * Mpc);
+		run_traj("bp_fixed_grid64_c1", bp, nullptr, ps, 2000, {1, 1000, 2000},
+		         "{\"propagator\": \"BP\", \"fixedStep\": " + fmt(0.1 * Mpc) +
+		             ", \"grid\": {\"n\": 64, \"spacing\": " + fmt(sp) + ", \"origin\": [0, 0, 0], \"seed\": 4242, \"amp\": " +
+		             fmt(nG) + ", \"generated\": \"noise grid from the same recipe; not stored\"}, \"particle_seed\": 42, \"Emin\": " +
+		             fmt(EeV) + ", \"Emax\": " + fmt(100 * EeV) + "}");
+	}
+
+	// ---- plane-wave trajectories
+	{
+		SimpleSpectrum spec(1 * muG, 1 * pc, 100 * pc);
+		ref_ptr<MagneticField> pw = new PlaneWaveTurbulence(spec, 32, 11);
+		std::vector<Particle> ps = make_particles(128, 46, 1e15 * eV, 1e18 * eV, Vector3d(0.), proton, one);
+		PropagationBP bp(pw, 0.05 * pc);
+		run_traj("bp_fixed_planewave32", bp, nullptr, ps, 1000, {1, 10, 100, 1000},
+		         "{\"propagator\": \"BP\", \"fixedStep\": " + fmt(0.05 * pc) +
+		             ", \"field\": {\"type\": \"planewave\", \"spectrum\": \"simple\", \"Brms\": " + fmt(muG) +
+		             ", \"lMin\": " + fmt(pc) + ", \"lMax\": " + fmt(100 * pc) + ", \"sIndex\": " + fmt(5. / 3) +
+		             ", \"Nm\": 32, \"seed\": 11}, \"particle_seed\": 46, \"Emin\": " + fmt(1e15 * eV) +
+		             ", \"Emax\": " + fmt(1e18 * eV) + "}");
+		PropagationCK ck(pw, 1e-4, 0.001 * pc, 1 * pc);
+		std::vector<Particle> ps2 = make_particles(128, 47, 1e15 * eV, 1e18 * eV, Vector3d(0.), proton, one);
+		run_traj("ck_planewave32", ck, nullptr, ps2, 300, {1, 10, 100, 300},
+		         "{\"propagator\": \"CK\", \"tolerance\": 1e-4, \"minStep\": " + fmt(0.001 * pc) +
+		             ", \"maxStep\": " + fmt(pc) +
+		             ", \"field\": {\"type\": \"planewave\", \"spectrum\": \"simple\", \"Brms\": " + fmt(muG) +
+		             ", \"lMin\": " + fmt(pc) + ", \"lMax\": " + fmt(100 * pc) + ", \"sIndex\": " + fmt(5. / 3) +
+		             ", \"Nm\": 32, \"seed\": 11}, \"particle_seed\": 47, \"Emin\": " + fmt(1e15 * eV) +
+		             ", \"Emax\": " + fmt(1e18 * eV) + "}");
+	}
+
+	// ---- uniform field: the reference tests' gyration setup, many directions
+	{
+		ref_ptr<MagneticField> uf = new UniformMagneticField(Vector3d(0, 0, 1 * nG));
+		std::vector<Particle> ps = make_particles(64, 48, 1 * EeV, 100 * EeV, Vector3d(0.), species, wts);
+		PropagationBP bp(uf, 10 * Mpc);
+		run_traj("bp_fixed_uniform", bp, nullptr, ps, 100, {1, 10, 100},
+		         "{\"propagator\": \"BP\", \"fixedStep\": " + fmt(10 * Mpc) +
+		             ", \"field\": {\"type\": \"uniform\", \"B\": [0, 0, " + fmt(nG) +
+		             "]}, \"particle_seed\": 48, \"Emin\": " + fmt(EeV) + ", \"Emax\": " + fmt(100 * EeV) + "}");
+		PropagationCK ck(uf, 1e-4, 0.1 * kpc, 1 * Gpc);
+		run_traj("ck_uniform", ck, nullptr, ps, 100, {1, 10, 100},
+		         "{\"propagator\": \"CK\", \"tolerance\": 1e-4, \"minStep\": " + fmt(0.1 * kpc) +
+		             ", \"maxStep\": " + fmt(Gpc) + ", \"field\": {\"type\": \"uniform\", \"B\": [0, 0, " + fmt(nG) +
+		             "]}, \"particle_seed\": 48, \"Emin\": " + fmt(EeV) + ", \"Emax\": " + fmt(100 * EeV) + "}");
+	}
+	printf("done\n");
+	return 0;
+}

@@ -1,0 +1,271 @@
+"""ctypes bindings to the oracle (oracle/liboracle.so): the CPU restatement of
+the reference hot path.  Test infrastructure only -- the product never loads it.
+"""
+import ctypes as C
+import json
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+LIBM_GLIBC = 0
+LIBM_CR = 1
+FIELD_NONE, FIELD_UNIFORM, FIELD_GRID, FIELD_PLANEWAVE = 0, 1, 2, 3
+PROP_BP, PROP_CK = 0, 1
+
+# SI units (include/crpropa/Units.h)
+c_light = 2.99792458e8
+eplus = 1.602176487e-19
+pc = 648000 / np.pi * 149597870700.0
+kpc = 1e3 * pc
+Mpc = 1e6 * pc
+Gpc = 1e9 * pc
+eV = eplus * 1.0
+EeV = 1e18 * eV
+gauss = 1e-4
+nG = 1e-9 * gauss
+muG = 1e-6 * gauss
+
+_P = C.POINTER
+_dp = _P(C.c_double)
+_fp = _P(C.c_float)
+_ip = _P(C.c_int32)
+
+
+class Field(C.Structure):
+    _fields_ = [
+        ("type", C.c_int),
+        ("B0", C.c_double * 3),
+        ("grid", _fp),
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("origin", C.c_double * 3),
+        ("spacing", C.c_double * 3),
+        ("reflective", C.c_int), ("clip", C.c_int),
+        ("nm", C.c_int),
+        ("k", _dp), ("Ak", _dp), ("xi", _dp), ("kappa", _dp), ("beta", _dp),
+    ]
+
+
+class Propagator(C.Structure):
+    _fields_ = [("type", C.c_int), ("tolerance", C.c_double), ("minStep", C.c_double),
+                ("maxStep", C.c_double), ("maxTrajectoryLength", C.c_double)]
+
+
+class State(C.Structure):
+    _fields_ = [("pos", _dp), ("dir", _dp), ("energy", _dp), ("charge", _dp), ("redshift", _dp),
+                ("current_step", _dp), ("next_step", _dp), ("trajectory_length", _dp), ("time", _dp),
+                ("active", _ip), ("prev_pos", _dp), ("prev_dir", _dp)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(ORACLE_DIR, "liboracle.so")
+        if not os.path.exists(path):
+            subprocess.check_call(["make", "-s", "-C", ORACLE_DIR, "port"])
+        L = C.CDLL(path)
+        L.orc_propagate.restype = C.c_longlong
+        L.orc_propagate.argtypes = [_P(Field), _P(Propagator), _P(State), C.c_size_t, C.c_longlong, C.c_int,
+                                    C.c_int, _P(C.c_longlong)]
+        L.orc_field_eval.argtypes = [_P(Field), _dp, C.c_size_t, _dp, C.c_int]
+        L.orc_planewave_init.argtypes = [C.c_double] * 6 + [C.c_int, C.c_int, C.c_int] + [_dp] * 5
+        L.orc_cos_batch.argtypes = [_dp, C.c_size_t, _dp, C.c_int]
+        L.orc_pow_batch.argtypes = [_dp, C.c_double, C.c_size_t, _dp, C.c_int]
+        L.orc_noise_grid.argtypes = [C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_double, _fp]
+        L.orc_rng_randint_batch.argtypes = [C.c_uint32, C.c_size_t, _P(C.c_uint32)]
+        L.orc_sample_particles.argtypes = [C.c_uint32, C.c_size_t, C.c_double, C.c_double, _dp, C.c_size_t,
+                                           _ip, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def ptr(a, t=_dp):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+# ---------------------------------------------------------------- fixtures
+def load_case(name):
+    d = os.path.join(GOLDEN, name)
+    out = {}
+    for f in os.listdir(d):
+        if f.endswith(".npy"):
+            out[f[:-4]] = np.load(os.path.join(d, f), allow_pickle=False)
+    mp = os.path.join(d, "meta.json")
+    if os.path.exists(mp):
+        with open(mp) as fh:
+            out["meta"] = json.load(fh)
+    return out
+
+
+# ---------------------------------------------------------------- fields
+class OracleField:
+    """Holds numpy buffers alive for an orc_field."""
+
+    def __init__(self):
+        self.s = Field()
+        self._keep = []
+
+    @staticmethod
+    def uniform(B):
+        f = OracleField()
+        f.s.type = FIELD_UNIFORM
+        for a in range(3):
+            f.s.B0[a] = B[a]
+        return f
+
+    @staticmethod
+    def grid(values, origin, spacing, reflective=False, clip=False):
+        f = OracleField()
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        f._keep.append(v)
+        f.s.type = FIELD_GRID
+        f.s.grid = ptr(v, _fp)
+        f.s.nx, f.s.ny, f.s.nz = v.shape[:3]
+        sp = np.broadcast_to(np.asarray(spacing, dtype=np.float64), (3,))
+        for a in range(3):
+            f.s.origin[a] = origin[a]
+            f.s.spacing[a] = sp[a]
+        f.s.reflective = int(reflective)
+        f.s.clip = int(clip)
+        return f
+
+    @staticmethod
+    def planewave(tables):
+        f = OracleField()
+        k, Ak, xi, kappa, beta = [np.ascontiguousarray(tables[n], dtype=np.float64)
+                                  for n in ("k", "Ak", "xi", "kappa", "beta")]
+        f._keep += [k, Ak, xi, kappa, beta]
+        f.s.type = FIELD_PLANEWAVE
+        f.s.nm = len(k)
+        f.s.k, f.s.Ak, f.s.xi, f.s.kappa, f.s.beta = [ptr(a) for a in (k, Ak, xi, kappa, beta)]
+        return f
+
+    def eval(self, pos, libm=LIBM_GLIBC):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        B = np.zeros_like(pos)
+        lib().orc_field_eval(C.byref(self.s), ptr(pos), pos.shape[0], ptr(B), libm)
+        return B
+
+
+def planewave_tables(meta):
+    """Mode table as built by the PlaneWaveTurbulence constructor (host glibc)."""
+    nm = int(meta["Nm"])
+    k = np.zeros(nm)
+    Ak = np.zeros(nm)
+    xi = np.zeros((nm, 3))
+    kappa = np.zeros((nm, 3))
+    beta = np.zeros(nm)
+    simple = meta["spectrum"] == "simple"
+    lb = 1000 * meta["lMax"] if simple else meta.get("lBendover", 1.0)
+    q = 0.0 if simple else meta.get("qIndex", 4.0)
+    lib().orc_planewave_init(meta["Brms"], meta["lMin"], meta["lMax"], lb, meta["sIndex"], q, int(simple), nm,
+                             int(meta["seed"]), ptr(k), ptr(Ak), ptr(xi), ptr(kappa), ptr(beta))
+    return {"k": k, "Ak": Ak, "xi": xi, "kappa": kappa, "beta": beta}
+
+
+def noise_grid(seed, n, amp):
+    g = np.zeros((n, n, n, 3), dtype=np.float32)
+    lib().orc_noise_grid(seed, n, n, n, amp, ptr(g, _fp))
+    return g
+
+
+def sample_particles(seed, n, emin, emax, weights=(1.0,)):
+    w = np.asarray(weights, dtype=np.float64)
+    sp = np.zeros(n, dtype=np.int32)
+    E = np.zeros(n)
+    d = np.zeros((n, 3))
+    lib().orc_sample_particles(seed, n, emin, emax, ptr(w), len(w), ptr(sp, _ip), ptr(E), ptr(d))
+    return sp, E, d
+
+
+# ---------------------------------------------------------------- propagation
+def normalise(d):
+    """ParticleState::setDirection: dir / dir.getR() (src/ParticleState.cpp:30-32)."""
+    d = np.asarray(d, dtype=np.float64)
+    n = np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+    return d / n[:, None]
+
+
+class OracleState:
+    def __init__(self, pos, dir, energy, charge, next_step=None):
+        n = len(energy)
+        self.n = n
+        self.pos = np.ascontiguousarray(pos, dtype=np.float64).copy()
+        self.dir = normalise(dir)
+        self.energy = np.ascontiguousarray(energy, dtype=np.float64).copy()
+        self.charge = np.ascontiguousarray(charge, dtype=np.float64).copy()
+        self.redshift = np.zeros(n)
+        self.current_step = np.zeros(n)
+        self.next_step = np.zeros(n) if next_step is None else np.array(next_step, dtype=np.float64)
+        self.trajectory_length = np.zeros(n)
+        self.time = np.zeros(n)
+        self.active = np.ones(n, dtype=np.int32)
+        self.prev_pos = self.pos.copy()
+        self.prev_dir = self.dir.copy()
+
+    def struct(self):
+        s = State()
+        for name in ("pos", "dir", "energy", "charge", "redshift", "current_step", "next_step",
+                     "trajectory_length", "time", "prev_pos", "prev_dir"):
+            setattr(s, name, ptr(getattr(self, name)))
+        s.active = ptr(self.active, _ip)
+        return s
+
+
+def propagate(field, prop_type, state, steps, tolerance=0.42, min_step=0.0, max_step=0.0, max_len=0.0,
+              libm=LIBM_GLIBC, nthreads=0):
+    p = Propagator(prop_type, tolerance, min_step, max_step, max_len)
+    s = state.struct()
+    trials = C.c_longlong(0)
+    f = C.byref(field.s) if field is not None else None
+    n = lib().orc_propagate(f, C.byref(p), C.byref(s), state.n, steps, libm, nthreads, C.byref(trials))
+    return n, trials.value
+
+
+def cos_batch(x, libm):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros_like(x)
+    lib().orc_cos_batch(ptr(x), x.size, ptr(out), libm)
+    return out
+
+
+def pow_batch(x, y, libm):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros_like(x)
+    lib().orc_pow_batch(ptr(x), y, x.size, ptr(out), libm)
+    return out
+
+
+def field_for_case(case):
+    """Build the oracle field a golden trajectory case ran in."""
+    meta = case["meta"]
+    f = meta["field"] if "field" in meta else None
+    if f == "grid32":
+        g = load_case("grid32")
+        gm = g["meta"]["grid"]
+        return OracleField.grid(g["grid"], gm["origin"], gm["spacing"])
+    if isinstance(f, dict) and f["type"] == "planewave":
+        return OracleField.planewave(planewave_tables(f))
+    if isinstance(f, dict) and f["type"] == "uniform":
+        return OracleField.uniform(f["B"])
+    if "grid" in meta:  # generated noise grid (C1-shaped case)
+        gm = meta["grid"]
+        vals = noise_grid(gm["seed"], gm["n"], gm["amp"])
+        return OracleField.grid(vals, gm["origin"], gm["spacing"])
+    raise ValueError("unknown field in case")
+
+
+def propagator_args(meta):
+    if meta["propagator"] == "BP":
+        if "fixedStep" in meta:
+            return dict(prop_type=PROP_BP, tolerance=0.42, min_step=meta["fixedStep"], max_step=meta["fixedStep"])
+        return dict(prop_type=PROP_BP, tolerance=meta["tolerance"], min_step=meta["minStep"],
+                    max_step=meta["maxStep"])
+    return dict(prop_type=PROP_CK, tolerance=meta["tolerance"], min_step=meta["minStep"], max_step=meta["maxStep"])
