@@ -107,6 +107,15 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
 		p[i] = 0ull;
 }
 
+// Plain streaming copy (batch reset); also the known-size reference kernel
+// against which HBM traffic counters are calibrated.
+__global__ void __launch_bounds__(256) k_copy_f64(const double *__restrict__ src, double *__restrict__ dst, size_t n) {
+	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const size_t stride = (size_t)gridDim.x * blockDim.x;
+	for (; i < n; i += stride)
+		dst[i] = src[i];
+}
+
 template <class F>
 __global__ void k_field_eval(F field, const double *__restrict__ pos, double *__restrict__ B, size_t n) {
 	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -737,8 +746,10 @@ int crp_batch_copy(crp_batch_t *dst, const crp_batch_t *src) {
 	                       src->next, src->len, src->time, src->ppx, src->ppy, src->ppz, src->pdx, src->pdy, src->pdz};
 	double *d[18] = {dst->px, dst->py, dst->pz, dst->dx, dst->dy, dst->dz, dst->E, dst->q, dst->cur,
 	                 dst->next, dst->len, dst->time, dst->ppx, dst->ppy, dst->ppz, dst->pdx, dst->pdy, dst->pdz};
+	const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
 	for (int c = 0; c < 18; c++)
-		CRP_HIP_TRY(hipMemcpyAsync(d[c], s[c], n * sizeof(double), hipMemcpyDeviceToDevice, dst->stream));
+		hipLaunchKernelGGL(k_copy_f64, dim3(blocks), dim3(256), 0, dst->stream, s[c], d[c], n);
+	CRP_HIP_TRY(hipGetLastError());
 	CRP_HIP_TRY(hipMemcpyAsync(dst->active, src->active, n * sizeof(int32_t), hipMemcpyDeviceToDevice, dst->stream));
 	CRP_HIP_TRY(hipStreamSynchronize(dst->stream));
 	return CRP_OK;

@@ -1,0 +1,102 @@
+// Grid3f storage with the reference's interface (include/crpropa/Grid.h:75-582):
+// Nx*Ny*Nz Vector3f values, z fastest (index ix*Ny*Nz + iy*Nz + iz), the volume
+// starting at `origin`, sample points at origin + (i + 1/2) * spacing,
+// periodic (default) or reflective continuation, optional clipVolume.
+// Interpolation runs on the GPU (trilinear, the reference's FP32 arithmetic).
+#ifndef CRPROPA_AMD_GRID_H
+#define CRPROPA_AMD_GRID_H
+
+#include "crpropa/Referenced.h"
+#include "crpropa/Vector3.h"
+
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+struct crp_field;
+
+namespace crpropa {
+
+enum interpolationType { TRILINEAR = 0, TRICUBIC, NEAREST_NEIGHBOUR };
+
+class GridProperties : public Referenced {
+public:
+	size_t Nx, Ny, Nz;
+	Vector3d origin;
+	Vector3d spacing;
+	bool reflective;
+	interpolationType ipol;
+	bool clipVolume;
+
+	GridProperties(Vector3d origin, size_t N, double spacing)
+	    : Nx(N), Ny(N), Nz(N), origin(origin), spacing(Vector3d(spacing)), reflective(false), ipol(TRILINEAR),
+	      clipVolume(false) {}
+	GridProperties(Vector3d origin, size_t Nx, size_t Ny, size_t Nz, double spacing)
+	    : Nx(Nx), Ny(Ny), Nz(Nz), origin(origin), spacing(Vector3d(spacing)), reflective(false), ipol(TRILINEAR),
+	      clipVolume(false) {}
+	GridProperties(Vector3d origin, size_t Nx, size_t Ny, size_t Nz, Vector3d spacing)
+	    : Nx(Nx), Ny(Ny), Nz(Nz), origin(origin), spacing(spacing), reflective(false), ipol(TRILINEAR),
+	      clipVolume(false) {}
+	void setReflective(bool b) { reflective = b; }
+	void setInterpolationType(interpolationType i) { ipol = i; }
+	void setClipVolume(bool b) { clipVolume = b; }
+};
+
+class Grid3f : public Referenced {
+public:
+	Grid3f(Vector3d origin, size_t N, double spacing);
+	Grid3f(Vector3d origin, size_t Nx, size_t Ny, size_t Nz, double spacing);
+	Grid3f(Vector3d origin, size_t Nx, size_t Ny, size_t Nz, Vector3d spacing);
+	Grid3f(const GridProperties &p);
+	~Grid3f();
+
+	void setOrigin(Vector3d o) { origin = o; changed(); }
+	void setGridSize(size_t nx, size_t ny, size_t nz);
+	void setSpacing(Vector3d s) { spacing = s; changed(); }
+	void setReflective(bool b) { reflective = b; changed(); }
+	void setClipVolume(bool b) { clipVolume = b; changed(); }
+	/* Only TRILINEAR runs on the GPU; the others throw. */
+	void setInterpolationType(interpolationType t);
+	interpolationType getInterpolationType() const { return ipolType; }
+
+	Vector3d getOrigin() const { return origin; }
+	Vector3d getSpacing() const { return spacing; }
+	size_t getNx() const { return Nx; }
+	size_t getNy() const { return Ny; }
+	size_t getNz() const { return Nz; }
+	bool isReflective() const { return reflective; }
+	bool getClipVolume() const { return clipVolume; }
+	size_t getSizeOf() const { return sizeof(grid) + sizeof(Vector3f) * grid.size(); }
+
+	/* Mutators mark the device copy stale; it is re-uploaded on next use. */
+	Vector3f &get(size_t ix, size_t iy, size_t iz) { changed(); return grid[ix * Ny * Nz + iy * Nz + iz]; }
+	const Vector3f &get(size_t ix, size_t iy, size_t iz) const { return grid[ix * Ny * Nz + iy * Nz + iz]; }
+	Vector3f getValue(size_t ix, size_t iy, size_t iz) const { return grid[ix * Ny * Nz + iy * Nz + iz]; }
+	void setValue(size_t ix, size_t iy, size_t iz, Vector3f v) { changed(); grid[ix * Ny * Nz + iy * Nz + iz] = v; }
+	std::vector<Vector3f> &getGrid() { changed(); return grid; }
+	const std::vector<Vector3f> &getGrid() const { return grid; }
+	Vector3d positionFromIndex(int index) const;
+
+	/* Trilinear value at a position, evaluated on the GPU. */
+	Vector3f interpolate(const Vector3d &position) const;
+
+	/* Device field for this grid (created or refreshed on demand). */
+	crp_field *deviceField() const;
+	/* Marks host values as modified after writing through getGrid()/data(). */
+	void changed() { dirty = true; }
+
+private:
+	std::vector<Vector3f> grid;
+	size_t Nx, Ny, Nz;
+	Vector3d origin, spacing;
+	bool clipVolume, reflective;
+	interpolationType ipolType;
+	mutable crp_field *dev;
+	mutable bool dirty;
+	mutable std::mutex mtx;
+};
+
+} // namespace crpropa
+
+#endif
