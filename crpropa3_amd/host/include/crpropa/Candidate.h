@@ -48,6 +48,13 @@ public:
 	double getNextStep() const { return nextStep; }
 	/* nextStep = min(nextStep, step) (src/Candidate.cpp:102-104) */
 	void limitNextStep(double step) { nextStep = step < nextStep ? step : nextStep; }
+	/* Restores the step bookkeeping computed on the device. */
+	void setStepState(double current, double next, double length, double t) {
+		currentStep = current;
+		nextStep = next;
+		trajectoryLength = length;
+		time = t;
+	}
 	void setTime(double t) { time = t; }
 	double getTime() const { return time; }
 	void setTagOrigin(const std::string &t) { tagOrigin = t; }

@@ -40,6 +40,9 @@ public:
 
 	/* Sets the charge directly (for particles defined by charge only). */
 	void setCharge(double q) { charge = q; }
+	/* Stores a direction that is already a unit vector (a device result) without
+	 * normalising it again. */
+	void setDirectionExact(const Vector3d &dir) { direction = dir; }
 
 private:
 	int id;

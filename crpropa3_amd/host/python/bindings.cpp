@@ -1,0 +1,369 @@
+// Python steering for the propagation path, with the reference's class names
+// and call patterns (the SWIG module of python/2_headers.i; SWIG is replaced by
+// pybind11).  Python subclasses of Module and MagneticField work as in the
+// reference's directors: Python modules run on the host between GPU steps;
+// Python fields cannot run on the GPU and are rejected by the propagators.
+#include <pybind11/numpy.h>
+#include <pybind11/operators.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "crpropa/Candidate.h"
+#include "crpropa/Grid.h"
+#include "crpropa/Module.h"
+#include "crpropa/ModuleList.h"
+#include "crpropa/ParticleState.h"
+#include "crpropa/Random.h"
+#include "crpropa/Units.h"
+#include "crpropa/Vector3.h"
+#include "crpropa/magneticField/MagneticField.h"
+#include "crpropa/magneticField/MagneticFieldGrid.h"
+#include "crpropa/magneticField/turbulentField/PlaneWaveTurbulence.h"
+#include "crpropa/module/BreakCondition.h"
+#include "crpropa/module/PropagationBP.h"
+#include "crpropa/module/PropagationCK.h"
+
+namespace py = pybind11;
+using namespace crpropa;
+
+PYBIND11_DECLARE_HOLDER_TYPE(T, crpropa::ref_ptr<T>, true);
+
+namespace {
+
+class PyModule : public Module {
+public:
+	void process(Candidate *c) const override { PYBIND11_OVERRIDE_PURE(void, Module, process, c); }
+	std::string getDescription() const override { PYBIND11_OVERRIDE(std::string, Module, getDescription); }
+};
+
+class PyMagneticField : public MagneticField {
+public:
+	Vector3d getField(const Vector3d &p) const override { PYBIND11_OVERRIDE(Vector3d, MagneticField, getField, p); }
+	Vector3d getField(const Vector3d &p, double z) const override {
+		PYBIND11_OVERRIDE(Vector3d, MagneticField, getField, p, z);
+	}
+};
+
+template <typename T>
+void bind_vector3(py::module_ &m, const char *name) {
+	typedef Vector3<T> V;
+	py::class_<V>(m, name)
+	    .def(py::init<>())
+	    .def(py::init<T>())
+	    .def(py::init<T, T, T>())
+	    .def(py::init([](py::sequence s) {
+		    if (py::len(s) != 3)
+			    throw std::runtime_error("Expected a sequence of length 3");
+		    return V(s[0].cast<T>(), s[1].cast<T>(), s[2].cast<T>());
+	    }))
+	    .def_readwrite("x", &V::x)
+	    .def_readwrite("y", &V::y)
+	    .def_readwrite("z", &V::z)
+	    .def("getX", &V::getX)
+	    .def("getY", &V::getY)
+	    .def("getZ", &V::getZ)
+	    .def("getR", &V::getR)
+	    .def("getR2", &V::getR2)
+	    .def("getUnitVector", &V::getUnitVector)
+	    .def("getDistanceTo", &V::getDistanceTo)
+	    .def("getAngleTo", &V::getAngleTo)
+	    .def("dot", &V::dot)
+	    .def("cross", &V::cross)
+	    .def(py::self + py::self)
+	    .def(py::self - py::self)
+	    .def(py::self * py::self)
+	    .def(py::self / py::self)
+	    .def(py::self * T())
+	    .def(T() * py::self)
+	    .def(py::self / T())
+	    .def(py::self == py::self)
+	    .def(-py::self)
+	    .def("__len__", [](const V &) { return 3; })
+	    .def("__getitem__",
+	         [](const V &v, int i) {
+		         if (i < 0 || i > 2)
+			         throw py::index_error();
+		         return i == 0 ? v.x : (i == 1 ? v.y : v.z);
+	         })
+	    .def("__array__", [](const V &v, py::args, py::kwargs) {
+		    py::array_t<T> a(3);
+		    auto r = a.template mutable_unchecked<1>();
+		    r(0) = v.x;
+		    r(1) = v.y;
+		    r(2) = v.z;
+		    return a;
+	    })
+	    .def("__repr__", [name](const V &v) {
+		    char buf[160];
+		    snprintf(buf, sizeof buf, "Vector(%.6G, %.6G, %.6G)", (double)v.x, (double)v.y, (double)v.z);
+		    return std::string(buf);
+	    });
+}
+
+} // namespace
+
+PYBIND11_MODULE(_crpropa, m) {
+	m.doc() = "crpropa3_amd: GPU (MI355X) drop-in for CRPropa3's PropagationBP/CK hot path";
+
+	// ---- units (include/crpropa/Units.h)
+#define UNIT(n) m.attr(#n) = n
+	UNIT(meter); UNIT(second); UNIT(kilogram); UNIT(ampere); UNIT(newton); UNIT(joule); UNIT(tesla); UNIT(volt);
+	UNIT(eplus); UNIT(c_light); UNIT(c_squared); UNIT(amu); UNIT(mass_proton); UNIT(mass_electron);
+	UNIT(gauss); UNIT(microgauss); UNIT(nanogauss); UNIT(muG); UNIT(nG);
+	UNIT(electronvolt); UNIT(eV); UNIT(keV); UNIT(MeV); UNIT(GeV); UNIT(TeV); UNIT(PeV); UNIT(EeV);
+	UNIT(au); UNIT(parsec); UNIT(kiloparsec); UNIT(megaparsec); UNIT(gigaparsec); UNIT(pc); UNIT(kpc); UNIT(Mpc);
+	UNIT(Gpc); UNIT(kilometer); UNIT(km);
+#undef UNIT
+
+	bind_vector3<double>(m, "Vector3d");
+	bind_vector3<float>(m, "Vector3f");
+	py::implicitly_convertible<py::sequence, Vector3d>();
+
+	m.def("nucleusId", &nucleusId);
+	m.def("chargeNumber", &chargeNumber);
+	m.def("massNumber", &massNumber);
+	m.def("isNucleus", &isNucleus);
+
+	py::class_<ParticleState>(m, "ParticleState")
+	    .def(py::init<int, double, Vector3d, Vector3d>(), py::arg("id") = 0, py::arg("energy") = 0.,
+	         py::arg("position") = Vector3d(0, 0, 0), py::arg("direction") = Vector3d(-1, 0, 0))
+	    .def("setPosition", &ParticleState::setPosition)
+	    .def("getPosition", &ParticleState::getPosition)
+	    .def("setDirection", &ParticleState::setDirection)
+	    .def("getDirection", &ParticleState::getDirection)
+	    .def("setEnergy", &ParticleState::setEnergy)
+	    .def("getEnergy", &ParticleState::getEnergy)
+	    .def("getRigidity", &ParticleState::getRigidity)
+	    .def("setId", &ParticleState::setId)
+	    .def("getId", &ParticleState::getId)
+	    .def("getCharge", &ParticleState::getCharge)
+	    .def("setCharge", &ParticleState::setCharge)
+	    .def("getMass", &ParticleState::getMass)
+	    .def("getVelocity", &ParticleState::getVelocity)
+	    .def("getMomentum", &ParticleState::getMomentum)
+	    .def("getDescription", &ParticleState::getDescription)
+	    .def("__repr__", &ParticleState::getDescription);
+
+	py::class_<Candidate, ref_ptr<Candidate>>(m, "Candidate")
+	    .def(py::init<int, double, Vector3d, Vector3d, double, double, std::string>(), py::arg("id") = 0,
+	         py::arg("energy") = 0., py::arg("position") = Vector3d(0, 0, 0),
+	         py::arg("direction") = Vector3d(-1, 0, 0), py::arg("z") = 0., py::arg("weight") = 1.,
+	         py::arg("tagOrigin") = "PRIM")
+	    .def(py::init<const ParticleState &>())
+	    .def_readwrite("source", &Candidate::source)
+	    .def_readwrite("created", &Candidate::created)
+	    .def_readwrite("current", &Candidate::current)
+	    .def_readwrite("previous", &Candidate::previous)
+	    .def("isActive", &Candidate::isActive)
+	    .def("setActive", &Candidate::setActive)
+	    .def("getTrajectoryLength", &Candidate::getTrajectoryLength)
+	    .def("setTrajectoryLength", &Candidate::setTrajectoryLength)
+	    .def("getRedshift", &Candidate::getRedshift)
+	    .def("setRedshift", &Candidate::setRedshift)
+	    .def("getWeight", &Candidate::getWeight)
+	    .def("setWeight", &Candidate::setWeight)
+	    .def("getCurrentStep", &Candidate::getCurrentStep)
+	    .def("setCurrentStep", &Candidate::setCurrentStep)
+	    .def("getNextStep", &Candidate::getNextStep)
+	    .def("setNextStep", &Candidate::setNextStep)
+	    .def("limitNextStep", &Candidate::limitNextStep)
+	    .def("getTime", &Candidate::getTime)
+	    .def("setTime", &Candidate::setTime)
+	    .def("getVelocity", &Candidate::getVelocity)
+	    .def("getTagOrigin", &Candidate::getTagOrigin)
+	    .def("setTagOrigin", &Candidate::setTagOrigin)
+	    .def("setProperty", &Candidate::setProperty)
+	    .def("getProperty", &Candidate::getProperty)
+	    .def("hasProperty", &Candidate::hasProperty)
+	    .def("removeProperty", &Candidate::removeProperty)
+	    .def("getSerialNumber", &Candidate::getSerialNumber)
+	    .def("getDescription", &Candidate::getDescription);
+
+	py::class_<Module, PyModule, ref_ptr<Module>>(m, "Module")
+	    .def(py::init<>())
+	    .def("process", (void (Module::*)(Candidate *) const) & Module::process)
+	    .def("getDescription", &Module::getDescription)
+	    .def("setDescription", &Module::setDescription);
+
+	py::class_<AbstractCondition, Module, ref_ptr<AbstractCondition>>(m, "AbstractCondition")
+	    .def("setMakeRejectedInactive", &AbstractCondition::setMakeRejectedInactive)
+	    .def("setRejectFlag", &AbstractCondition::setRejectFlag)
+	    .def("onReject", &AbstractCondition::onReject);
+
+	py::class_<MagneticField, PyMagneticField, ref_ptr<MagneticField>>(m, "MagneticField")
+	    .def(py::init<>())
+	    .def("getField", (Vector3d(MagneticField::*)(const Vector3d &) const) & MagneticField::getField)
+	    .def("getField", (Vector3d(MagneticField::*)(const Vector3d &, double) const) & MagneticField::getField);
+
+	py::class_<UniformMagneticField, MagneticField, ref_ptr<UniformMagneticField>>(m, "UniformMagneticField")
+	    .def(py::init<const Vector3d &>());
+
+	py::enum_<interpolationType>(m, "interpolationType")
+	    .value("TRILINEAR", TRILINEAR)
+	    .value("TRICUBIC", TRICUBIC)
+	    .value("NEAREST_NEIGHBOUR", NEAREST_NEIGHBOUR)
+	    .export_values();
+
+	py::class_<GridProperties, ref_ptr<GridProperties>>(m, "GridProperties")
+	    .def(py::init<Vector3d, size_t, double>())
+	    .def(py::init<Vector3d, size_t, size_t, size_t, double>())
+	    .def(py::init<Vector3d, size_t, size_t, size_t, Vector3d>())
+	    .def("setReflective", &GridProperties::setReflective)
+	    .def("setClipVolume", &GridProperties::setClipVolume)
+	    .def("setInterpolationType", &GridProperties::setInterpolationType)
+	    .def_readwrite("Nx", &GridProperties::Nx)
+	    .def_readwrite("Ny", &GridProperties::Ny)
+	    .def_readwrite("Nz", &GridProperties::Nz)
+	    .def_readwrite("origin", &GridProperties::origin)
+	    .def_readwrite("spacing", &GridProperties::spacing);
+
+	py::class_<Grid3f, ref_ptr<Grid3f>>(m, "Grid3f", py::buffer_protocol())
+	    .def(py::init<Vector3d, size_t, double>())
+	    .def(py::init<Vector3d, size_t, size_t, size_t, double>())
+	    .def(py::init<Vector3d, size_t, size_t, size_t, Vector3d>())
+	    .def(py::init<const GridProperties &>())
+	    .def("getNx", &Grid3f::getNx)
+	    .def("getNy", &Grid3f::getNy)
+	    .def("getNz", &Grid3f::getNz)
+	    .def("getOrigin", &Grid3f::getOrigin)
+	    .def("getSpacing", &Grid3f::getSpacing)
+	    .def("setReflective", &Grid3f::setReflective)
+	    .def("isReflective", &Grid3f::isReflective)
+	    .def("setClipVolume", &Grid3f::setClipVolume)
+	    .def("getClipVolume", &Grid3f::getClipVolume)
+	    .def("setInterpolationType", &Grid3f::setInterpolationType)
+	    .def("getValue", &Grid3f::getValue)
+	    .def("setValue", &Grid3f::setValue)
+	    .def("interpolate", &Grid3f::interpolate)
+	    .def("positionFromIndex", &Grid3f::positionFromIndex)
+	    .def("getSizeOf", &Grid3f::getSizeOf)
+	    .def("changed", &Grid3f::changed)
+	    .def_buffer([](Grid3f &g) -> py::buffer_info {
+		    std::vector<Vector3f> &v = g.getGrid();
+		    return py::buffer_info(v.data(), sizeof(float), py::format_descriptor<float>::format(), 4,
+		                           {g.getNx(), g.getNy(), g.getNz(), (size_t)3},
+		                           {sizeof(float) * 3 * g.getNy() * g.getNz(), sizeof(float) * 3 * g.getNz(),
+		                            sizeof(float) * 3, sizeof(float)});
+	    })
+	    .def("array",
+	         [](py::object self) {
+		         Grid3f &g = self.cast<Grid3f &>();
+		         g.changed();
+		         return py::array_t<float>(py::buffer_info(
+		                                       g.getGrid().data(), sizeof(float),
+		                                       py::format_descriptor<float>::format(), 4,
+		                                       {g.getNx(), g.getNy(), g.getNz(), (size_t)3},
+		                                       {sizeof(float) * 3 * g.getNy() * g.getNz(),
+		                                        sizeof(float) * 3 * g.getNz(), sizeof(float) * 3, sizeof(float)}),
+		                                   self);
+	         },
+	         "Writable numpy view (Nx, Ny, Nz, 3) of the grid values.");
+
+	py::class_<MagneticFieldGrid, MagneticField, ref_ptr<MagneticFieldGrid>>(m, "MagneticFieldGrid")
+	    .def(py::init<ref_ptr<Grid3f>>())
+	    .def("setGrid", &MagneticFieldGrid::setGrid)
+	    .def("getGrid", &MagneticFieldGrid::getGrid);
+
+	py::class_<TurbulenceSpectrum, ref_ptr<TurbulenceSpectrum>>(m, "TurbulenceSpectrum")
+	    .def(py::init<double, double, double, double, double, double>(), py::arg("Brms"), py::arg("lMin"),
+	         py::arg("lMax"), py::arg("lBendover") = 1., py::arg("sIndex") = 5. / 3., py::arg("qIndex") = 4.)
+	    .def("getBrms", &TurbulenceSpectrum::getBrms)
+	    .def("getLmin", &TurbulenceSpectrum::getLmin)
+	    .def("getLmax", &TurbulenceSpectrum::getLmax)
+	    .def("getLbendover", &TurbulenceSpectrum::getLbendover)
+	    .def("getSindex", &TurbulenceSpectrum::getSindex)
+	    .def("getQindex", &TurbulenceSpectrum::getQindex)
+	    .def("energySpectrum", &TurbulenceSpectrum::energySpectrum);
+
+	py::class_<SimpleTurbulenceSpectrum, TurbulenceSpectrum, ref_ptr<SimpleTurbulenceSpectrum>>(
+	    m, "SimpleTurbulenceSpectrum")
+	    .def(py::init<double, double, double, double>(), py::arg("Brms"), py::arg("lMin"), py::arg("lMax"),
+	         py::arg("sIndex") = 5. / 3.);
+
+	py::class_<TurbulentField, MagneticField, ref_ptr<TurbulentField>>(m, "TurbulentField")
+	    .def("getBrms", &TurbulentField::getBrms);
+
+	py::class_<PlaneWaveTurbulence, TurbulentField, ref_ptr<PlaneWaveTurbulence>>(m, "PlaneWaveTurbulence")
+	    .def(py::init<const TurbulenceSpectrum &, int, int>(), py::arg("spectrum"), py::arg("Nm") = 64,
+	         py::arg("seed") = 0)
+	    .def("getNm", &PlaneWaveTurbulence::getNm)
+	    .def("getK", &PlaneWaveTurbulence::getK)
+	    .def("getAk", &PlaneWaveTurbulence::getAk)
+	    .def("getBeta", &PlaneWaveTurbulence::getBeta)
+	    .def("getXi", &PlaneWaveTurbulence::getXi)
+	    .def("getKappa", &PlaneWaveTurbulence::getKappa);
+
+	py::class_<DevicePropagator, Module, ref_ptr<DevicePropagator>> dp(m, "DevicePropagator");
+	py::enum_<DevicePropagator::Precision>(dp, "Precision")
+	    .value("FP64", DevicePropagator::FP64)
+	    .value("MIXED", DevicePropagator::MIXED)
+	    .export_values();
+	dp.def("setField", &DevicePropagator::setField)
+	    .def("getField", &DevicePropagator::getField)
+	    .def("setTolerance", &DevicePropagator::setTolerance)
+	    .def("setMinimumStep", &DevicePropagator::setMinimumStep)
+	    .def("setMaximumStep", &DevicePropagator::setMaximumStep)
+	    .def("getTolerance", &DevicePropagator::getTolerance)
+	    .def("getMinimumStep", &DevicePropagator::getMinimumStep)
+	    .def("getMaximumStep", &DevicePropagator::getMaximumStep)
+	    .def("setPrecision", &DevicePropagator::setPrecision)
+	    .def("getPrecision", &DevicePropagator::getPrecision);
+
+	py::class_<PropagationBP, DevicePropagator, ref_ptr<PropagationBP>>(m, "PropagationBP")
+	    .def(py::init<ref_ptr<MagneticField>, double>(), py::arg("field") = ref_ptr<MagneticField>(),
+	         py::arg("fixedStep") = 1. * kpc)
+	    .def(py::init<ref_ptr<MagneticField>, double, double, double>(), py::arg("field"), py::arg("tolerance"),
+	         py::arg("minStep"), py::arg("maxStep"));
+
+	py::class_<PropagationCK, DevicePropagator, ref_ptr<PropagationCK>>(m, "PropagationCK")
+	    .def(py::init<ref_ptr<MagneticField>, double, double, double>(), py::arg("field") = ref_ptr<MagneticField>(),
+	         py::arg("tolerance") = 1e-4, py::arg("minStep") = 0.1 * kpc, py::arg("maxStep") = 1 * Gpc);
+
+	py::class_<MaximumTrajectoryLength, AbstractCondition, ref_ptr<MaximumTrajectoryLength>>(
+	    m, "MaximumTrajectoryLength")
+	    .def(py::init<double>(), py::arg("maxLength") = 0.)
+	    .def("setMaximumTrajectoryLength", &MaximumTrajectoryLength::setMaximumTrajectoryLength)
+	    .def("getMaximumTrajectoryLength", &MaximumTrajectoryLength::getMaximumTrajectoryLength)
+	    .def("addObserverPosition", &MaximumTrajectoryLength::addObserverPosition)
+	    .def("getObserverPositions", &MaximumTrajectoryLength::getObserverPositions);
+
+	py::class_<ModuleList, Module, ref_ptr<ModuleList>>(m, "ModuleList")
+	    .def(py::init<>())
+	    .def("add", &ModuleList::add, py::keep_alive<1, 2>())
+	    .def("remove", &ModuleList::remove)
+	    .def("size", &ModuleList::size)
+	    .def("__len__", &ModuleList::size)
+	    .def("__getitem__", &ModuleList::operator[])
+	    .def("setShowProgress", &ModuleList::setShowProgress, py::arg("show") = true)
+	    .def("setMaximumSteps", &ModuleList::setMaximumSteps)
+	    .def("process", (void (ModuleList::*)(Candidate *) const) & ModuleList::process)
+	    .def("run",
+	         [](ModuleList &ml, ref_ptr<Candidate> c, bool recursive, bool secondariesFirst) {
+		         py::gil_scoped_release nogil;
+		         ml.run(c, recursive, secondariesFirst);
+	         },
+	         py::arg("candidate"), py::arg("recursive") = true, py::arg("secondariesFirst") = false)
+	    .def("run",
+	         [](ModuleList &ml, std::vector<ref_ptr<Candidate>> v, bool recursive, bool secondariesFirst) {
+		         py::gil_scoped_release nogil;
+		         ml.run(&v, recursive, secondariesFirst);
+	         },
+	         py::arg("candidates"), py::arg("recursive") = true, py::arg("secondariesFirst") = false)
+	    .def("lastAcceptedSteps", &ModuleList::lastAcceptedSteps)
+	    .def("lastKernelMs", &ModuleList::lastKernelMs)
+	    .def("getDescription", &ModuleList::getDescription)
+	    .def("showModules", &ModuleList::showModules);
+
+	py::class_<Random>(m, "Random")
+	    .def(py::init<>())
+	    .def(py::init<uint32_t>())
+	    .def("seed", (void (Random::*)(uint32_t)) & Random::seed)
+	    .def("randInt", &Random::randInt)
+	    .def("rand", &Random::rand)
+	    .def("randUniform", &Random::randUniform)
+	    .def("randNorm", &Random::randNorm, py::arg("mean") = 0.0, py::arg("variance") = 1.0)
+	    .def("randVector", &Random::randVector)
+	    .def("randPowerLaw", &Random::randPowerLaw)
+	    .def("randBin", &Random::randBin)
+	    .def_static("instance", &Random::instance, py::return_value_policy::reference);
+}

@@ -1,0 +1,205 @@
+// Host side of the fields: Grid3f storage, the reference's field classes, and
+// their device descriptors (include/crpropa3_amd.h).
+#include "crpropa/Grid.h"
+#include "crpropa/Random.h"
+#include "crpropa/magneticField/MagneticField.h"
+#include "crpropa/magneticField/MagneticFieldGrid.h"
+#include "crpropa/magneticField/turbulentField/PlaneWaveTurbulence.h"
+
+#include "crpropa3_amd.h"
+
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace crpropa {
+
+static void throwIf(int rc) {
+	if (rc != CRP_OK)
+		throw std::runtime_error(crp_last_error());
+}
+
+void evaluateOnDevice(const MagneticField &f, const double *pos, size_t n, double *B) {
+	crp_field *d = f.deviceField();
+	if (!d)
+		throw std::runtime_error("crpropa3_amd: this magnetic field has no device implementation");
+	throwIf(crp_field_eval(d, pos, n, B));
+}
+
+// ------------------------------------------------------------------ Grid3f
+Grid3f::Grid3f(Vector3d origin, size_t N, double spacing)
+    : Nx(0), Ny(0), Nz(0), origin(origin), spacing(Vector3d(spacing)), clipVolume(false), reflective(false),
+      ipolType(TRILINEAR), dev(nullptr), dirty(true) {
+	setGridSize(N, N, N);
+}
+Grid3f::Grid3f(Vector3d origin, size_t nx, size_t ny, size_t nz, double spacing)
+    : Nx(0), Ny(0), Nz(0), origin(origin), spacing(Vector3d(spacing)), clipVolume(false), reflective(false),
+      ipolType(TRILINEAR), dev(nullptr), dirty(true) {
+	setGridSize(nx, ny, nz);
+}
+Grid3f::Grid3f(Vector3d origin, size_t nx, size_t ny, size_t nz, Vector3d spacing)
+    : Nx(0), Ny(0), Nz(0), origin(origin), spacing(spacing), clipVolume(false), reflective(false),
+      ipolType(TRILINEAR), dev(nullptr), dirty(true) {
+	setGridSize(nx, ny, nz);
+}
+Grid3f::Grid3f(const GridProperties &p)
+    : Nx(0), Ny(0), Nz(0), origin(p.origin), spacing(p.spacing), clipVolume(p.clipVolume),
+      reflective(p.reflective), ipolType(TRILINEAR), dev(nullptr), dirty(true) {
+	setGridSize(p.Nx, p.Ny, p.Nz);
+	setInterpolationType(p.ipol);
+}
+
+Grid3f::~Grid3f() {
+	if (dev)
+		crp_field_destroy(dev);
+}
+
+void Grid3f::setGridSize(size_t nx, size_t ny, size_t nz) {
+	Nx = nx;
+	Ny = ny;
+	Nz = nz;
+	grid.resize(nx * ny * nz);
+	changed();
+}
+
+void Grid3f::setInterpolationType(interpolationType t) {
+	if (t != TRILINEAR && t != TRICUBIC && t != NEAREST_NEIGHBOUR)
+		throw std::runtime_error("InterpolationType: unknown interpolation type");
+	if (t != TRILINEAR)
+		throw std::runtime_error("crpropa3_amd: only TRILINEAR interpolation runs on the GPU");
+	ipolType = t;
+}
+
+Vector3d Grid3f::positionFromIndex(int index) const {
+	int ix = index / (Ny * Nz);
+	int iy = (index / Nz) % Ny;
+	int iz = index % Nz;
+	return Vector3d(ix, iy, iz) * spacing + (origin + spacing / 2.);
+}
+
+crp_field *Grid3f::deviceField() const {
+	std::lock_guard<std::mutex> lock(mtx);
+	if (dev && !dirty)
+		return dev;
+	if (dev) {
+		crp_field_destroy(dev);
+		dev = nullptr;
+	}
+	const double o[3] = {origin.x, origin.y, origin.z};
+	const double s[3] = {spacing.x, spacing.y, spacing.z};
+	crp_field *f = nullptr;
+	throwIf(crp_field_create_grid3f(reinterpret_cast<const float *>(grid.data()), (int)Nx, (int)Ny, (int)Nz, o, s,
+	                                reflective ? 1 : 0, clipVolume ? 1 : 0, &f));
+	dev = f;
+	dirty = false;
+	return dev;
+}
+
+Vector3f Grid3f::interpolate(const Vector3d &position) const {
+	const double p[3] = {position.x, position.y, position.z};
+	double B[3];
+	throwIf(crp_field_eval(deviceField(), p, 1, B));
+	return Vector3f((float)B[0], (float)B[1], (float)B[2]);
+}
+
+static_assert(sizeof(Vector3f) == 3 * sizeof(float), "Vector3f must be three packed floats");
+
+// ------------------------------------------------------------------ simple fields
+UniformMagneticField::~UniformMagneticField() {
+	if (dev)
+		crp_field_destroy(dev);
+}
+
+crp_field *UniformMagneticField::deviceField() const {
+	std::lock_guard<std::mutex> lock(mtx);
+	if (!dev) {
+		const double b[3] = {value.x, value.y, value.z};
+		throwIf(crp_field_create_uniform(b, &dev));
+	}
+	return dev;
+}
+
+Vector3d MagneticFieldGrid::getField(const Vector3d &position) const {
+	if (!grid.valid())
+		return Vector3d(0.);
+	const double p[3] = {position.x, position.y, position.z};
+	double B[3];
+	throwIf(crp_field_eval(grid->deviceField(), p, 1, B));
+	return Vector3d(B[0], B[1], B[2]);
+}
+
+// ------------------------------------------------------------------ plane waves
+// Mode table exactly as PlaneWaveTurbulence::PlaneWaveTurbulence builds it
+// (src/magneticField/turbulentField/PlaneWaveTurbulence.cpp:99-178), with the
+// host's libm, so the table equals the reference's bit for bit.
+PlaneWaveTurbulence::PlaneWaveTurbulence(const TurbulenceSpectrum &spectrum, int Nm, int seed)
+    : TurbulentField(spectrum), Nm(Nm), dev(nullptr) {
+	if (Nm <= 1)
+		throw std::runtime_error("PlaneWaveTurbulence: Nm <= 1. Specify at least two wavemodes in "
+		                         "order to generate the k distribution properly.");
+	Random random;
+	if (seed != 0)
+		random.seed((uint32_t)seed);
+	double kmax = 2 * M_PI / spectrum.getLmin();
+	double kmin = 2 * M_PI / spectrum.getLmax();
+	xi.assign(Nm, Vector3d(0.));
+	kappa.assign(Nm, Vector3d(0.));
+	phi.assign(Nm, 0.);
+	costheta.assign(Nm, 0.);
+	beta.assign(Nm, 0.);
+	Ak.assign(Nm, 0.);
+	k.assign(Nm, 0.);
+	double delta = std::log10(kmax / kmin);
+	for (int i = 0; i < Nm; i++)
+		k[i] = std::pow(10, std::log10(kmin) + ((double)i) / ((double)(Nm - 1)) * delta);
+	double delta_k0 = (k[1] - k[0]) / k[1];
+	double Ak2_sum = 0;
+	for (int i = 0; i < Nm; i++) {
+		double kk = k[i];
+		double kHat = kk * spectrum.getLbendover();
+		double Gk = spectrum.energySpectrum(kk) * (1 + kHat * kHat);
+		Ak[i] = Gk * delta_k0 * kk;
+		Ak2_sum += Ak[i];
+		double ph = random.randUniform(-M_PI, M_PI);
+		double ct = random.randUniform(-1., 1.);
+		double st = std::sqrt(1 - ct * ct);
+		double alpha = random.randUniform(0, 2 * M_PI);
+		double b = random.randUniform(0, 2 * M_PI);
+		kappa[i] = Vector3d(st * std::cos(ph), st * std::sin(ph), ct);
+		xi[i] = Vector3d(ct * std::cos(ph) * std::cos(alpha) + std::sin(ph) * std::sin(alpha),
+		                 ct * std::sin(ph) * std::cos(alpha) - std::cos(ph) * std::sin(alpha), -st * std::cos(alpha));
+		phi[i] = ph;
+		costheta[i] = ct;
+		beta[i] = b;
+	}
+	for (int i = 0; i < Nm; i++)
+		Ak[i] = std::sqrt(2 * Ak[i] / Ak2_sum) * spectrum.getBrms();
+}
+
+PlaneWaveTurbulence::~PlaneWaveTurbulence() {
+	if (dev)
+		crp_field_destroy(dev);
+}
+
+crp_field *PlaneWaveTurbulence::deviceField() const {
+	std::lock_guard<std::mutex> lock(mtx);
+	if (!dev) {
+		std::vector<double> x(3 * Nm), kp(3 * Nm);
+		for (int i = 0; i < Nm; i++) {
+			x[3 * i] = xi[i].x; x[3 * i + 1] = xi[i].y; x[3 * i + 2] = xi[i].z;
+			kp[3 * i] = kappa[i].x; kp[3 * i + 1] = kappa[i].y; kp[3 * i + 2] = kappa[i].z;
+		}
+		throwIf(crp_field_create_planewave(Nm, k.data(), Ak.data(), x.data(), kp.data(), beta.data(), &dev));
+	}
+	return dev;
+}
+
+Vector3d PlaneWaveTurbulence::getField(const Vector3d &pos) const {
+	const double p[3] = {pos.x, pos.y, pos.z};
+	double B[3];
+	throwIf(crp_field_eval(deviceField(), p, 1, B));
+	return Vector3d(B[0], B[1], B[2]);
+}
+
+} // namespace crpropa

@@ -1,0 +1,367 @@
+// Host side of the propagators, the length limit and ModuleList: parameter
+// handling with the reference's validation, and the packing of Candidates into
+// device batches (include/crpropa3_amd.h).
+#include "crpropa/ModuleList.h"
+#include "crpropa/module/BreakCondition.h"
+#include "crpropa/module/PropagationBP.h"
+#include "crpropa/module/PropagationCK.h"
+
+#include "crpropa3_amd.h"
+
+#include <iostream>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <vector>
+
+namespace crpropa {
+
+static void throwIf(int rc) {
+	if (rc != CRP_OK)
+		throw std::runtime_error(crp_last_error());
+}
+
+struct PropagatorDeleter {
+	void operator()(crp_propagator *p) const { crp_propagator_destroy(p); }
+};
+struct BatchDeleter {
+	void operator()(crp_batch *b) const { crp_batch_destroy(b); }
+};
+typedef std::unique_ptr<crp_propagator, PropagatorDeleter> PropHandle;
+typedef std::unique_ptr<crp_batch, BatchDeleter> BatchHandle;
+
+// ------------------------------------------------------------------ packing
+/* Host mirror of a candidate vector in the layout crp_batch_upload takes. */
+struct Packed {
+	std::vector<double> pos, dir, E, q, cur, next, len, time, ppos, pdir;
+	std::vector<int32_t> active;
+	void resize(size_t n) {
+		pos.resize(3 * n); dir.resize(3 * n); ppos.resize(3 * n); pdir.resize(3 * n);
+		E.resize(n); q.resize(n); cur.resize(n); next.resize(n); len.resize(n); time.resize(n);
+		active.resize(n);
+	}
+	crp_host_state view() {
+		crp_host_state s;
+		s.pos = pos.data(); s.dir = dir.data(); s.energy = E.data(); s.charge = q.data();
+		s.current_step = cur.data(); s.next_step = next.data(); s.trajectory_length = len.data();
+		s.time = time.data(); s.active = active.data(); s.prev_pos = ppos.data(); s.prev_dir = pdir.data();
+		return s;
+	}
+};
+
+static void pack(const std::vector<Candidate *> &c, Packed &p) {
+	p.resize(c.size());
+	for (size_t i = 0; i < c.size(); i++) {
+		const ParticleState &s = c[i]->current;
+		const Vector3d &x = s.getPosition(), &u = s.getDirection();
+		p.pos[3 * i] = x.x; p.pos[3 * i + 1] = x.y; p.pos[3 * i + 2] = x.z;
+		p.dir[3 * i] = u.x; p.dir[3 * i + 1] = u.y; p.dir[3 * i + 2] = u.z;
+		p.E[i] = s.getEnergy();
+		p.q[i] = s.getCharge();
+		p.cur[i] = c[i]->getCurrentStep();
+		p.next[i] = c[i]->getNextStep();
+		p.len[i] = c[i]->getTrajectoryLength();
+		p.time[i] = c[i]->getTime();
+		p.active[i] = c[i]->isActive() ? 1 : 0;
+	}
+}
+
+/* Writes device results back; `moved[i]` tells whether candidate i took a step
+ * (its previous state is then the state before its last step). */
+static void unpack(const std::vector<Candidate *> &c, const Packed &p, const std::vector<char> &moved,
+                   const MaximumTrajectoryLength *limit) {
+	for (size_t i = 0; i < c.size(); i++) {
+		if (!moved[i])
+			continue;
+		Candidate *cand = c[i];
+		// previous = state before the last step; directions are device unit vectors, kept as they are
+		cand->previous = cand->current;
+		cand->previous.setPosition(Vector3d(p.ppos[3 * i], p.ppos[3 * i + 1], p.ppos[3 * i + 2]));
+		cand->previous.setDirectionExact(Vector3d(p.pdir[3 * i], p.pdir[3 * i + 1], p.pdir[3 * i + 2]));
+		cand->current.setPosition(Vector3d(p.pos[3 * i], p.pos[3 * i + 1], p.pos[3 * i + 2]));
+		cand->current.setDirectionExact(Vector3d(p.dir[3 * i], p.dir[3 * i + 1], p.dir[3 * i + 2]));
+		cand->setStepState(p.cur[i], p.next[i], p.len[i], p.time[i]);
+		if (!p.active[i] && cand->isActive()) {
+			if (limit)
+				limit->reject(cand);
+			else
+				cand->setActive(false);
+		}
+	}
+}
+
+// ------------------------------------------------------------------ DevicePropagator
+void DevicePropagator::setTolerance(double tol) {
+	if ((tol > 1) || (tol < 0))
+		throw std::runtime_error(std::string(name()) + ": target error not in range 0-1");
+	tolerance = tol;
+}
+
+void DevicePropagator::setMinimumStep(double min) {
+	if (min < 0)
+		throw std::runtime_error(std::string(name()) + ": minStep < 0 ");
+	if (min > maxStep)
+		throw std::runtime_error(std::string(name()) + ": minStep > maxStep");
+	minStep = min;
+}
+
+void DevicePropagator::setMaximumStep(double max) {
+	if (max < minStep)
+		throw std::runtime_error(std::string(name()) + ": maxStep < minStep");
+	maxStep = max;
+}
+
+crp_propagator *DevicePropagator::makeDevicePropagator(double maxLength) const {
+	crp_field *f = nullptr;
+	if (field.valid()) {
+		f = field->deviceField();
+		if (!f)
+			throw std::runtime_error(std::string(name()) +
+			                         ": the magnetic field has no GPU implementation (only uniform, "
+			                         "MagneticFieldGrid and PlaneWaveTurbulence fields run on the device)");
+	}
+	crp_propagator *p = nullptr;
+	if (kind == BP)
+		throwIf(crp_propagator_create_bp(f, tolerance, minStep, maxStep, &p));
+	else
+		throwIf(crp_propagator_create_ck(f, tolerance, minStep, maxStep, &p));
+	PropHandle h(p);
+	if (maxLength > 0)
+		throwIf(crp_propagator_set_max_trajectory_length(p, maxLength));
+	if (precision != FP64)
+		throwIf(crp_propagator_set_precision(p, CRP_PRECISION_MIXED));
+	return h.release();
+}
+
+/* One step of one candidate on the GPU (correct, not fast: batch of one). */
+void DevicePropagator::process(Candidate *candidate) const {
+	PropHandle p(makeDevicePropagator(0));
+	crp_batch *b = nullptr;
+	throwIf(crp_batch_create(1, &b));
+	BatchHandle bh(b);
+	std::vector<Candidate *> one(1, candidate);
+	Packed pk;
+	pack(one, pk);
+	pk.active[0] = 1;
+	crp_host_state s = pk.view();
+	throwIf(crp_batch_upload(b, 1, &s));
+	throwIf(crp_propagate(p.get(), b, 1, nullptr));
+	throwIf(crp_batch_download(b, &s));
+	unpack(one, pk, std::vector<char>(1, 1), nullptr);
+}
+
+PropagationBP::PropagationBP(ref_ptr<MagneticField> f, double fixedStep) : DevicePropagator(BP) {
+	setField(f);
+	setTolerance(0.42);
+	setMaximumStep(fixedStep);
+	setMinimumStep(fixedStep);
+}
+
+PropagationBP::PropagationBP(ref_ptr<MagneticField> f, double tol, double minS, double maxS) : DevicePropagator(BP) {
+	setField(f);
+	setTolerance(tol);
+	setMaximumStep(maxS);
+	setMinimumStep(minS);
+}
+
+std::string PropagationBP::getDescription() const {
+	std::stringstream s;
+	s << "Propagation in magnetic fields using the adaptive Boris push method.";
+	s << " Target error: " << tolerance;
+	s << ", Minimum Step: " << minStep / kpc << " kpc";
+	s << ", Maximum Step: " << maxStep / kpc << " kpc";
+	return s.str();
+}
+
+PropagationCK::PropagationCK(ref_ptr<MagneticField> f, double tol, double minS, double maxS) : DevicePropagator(CK) {
+	setField(f);
+	setTolerance(tol);
+	setMaximumStep(maxS);
+	setMinimumStep(minS);
+}
+
+std::string PropagationCK::getDescription() const {
+	std::stringstream s;
+	s << "Propagation in magnetic fields using the Cash-Karp method.";
+	s << " Target error: " << tolerance;
+	s << ", Minimum Step: " << minStep / kpc << " kpc";
+	s << ", Maximum Step: " << maxStep / kpc << " kpc";
+	return s.str();
+}
+
+// ------------------------------------------------------------------ MaximumTrajectoryLength
+MaximumTrajectoryLength::MaximumTrajectoryLength(double length)
+    : AbstractCondition("MaximumTrajectoryLength"), maxLength(length) {}
+
+void MaximumTrajectoryLength::process(Candidate *c) const {
+	double length = c->getTrajectoryLength();
+	Vector3d position = c->current.getPosition();
+	if (observerPositions.size()) {
+		bool inRange = false;
+		for (size_t i = 0; i < observerPositions.size(); i++)
+			if (position.getDistanceTo(observerPositions[i]) + length < maxLength)
+				inRange = true;
+		if (!inRange) {
+			reject(c);
+			return;
+		}
+	}
+	if (length >= maxLength)
+		reject(c);
+	else
+		c->limitNextStep(maxLength - length);
+}
+
+std::string MaximumTrajectoryLength::getDescription() const {
+	std::stringstream s;
+	s << "Maximum trajectory length: " << maxLength / Mpc << " Mpc, ";
+	s << "Flag: '" << rejectFlagKey << "' -> '" << rejectFlagValue << "', ";
+	s << "MakeInactive: " << (makeRejectedInactive ? "yes" : "no");
+	return s.str();
+}
+
+// ------------------------------------------------------------------ ModuleList
+void ModuleList::remove(std::size_t i) {
+	if (i >= modules.size())
+		throw std::runtime_error("ModuleList: index out of range");
+	auto it = modules.begin();
+	std::advance(it, i);
+	modules.erase(it);
+}
+
+ref_ptr<Module> ModuleList::operator[](const std::size_t i) {
+	if (i >= modules.size())
+		throw std::runtime_error("ModuleList: index out of range");
+	auto it = modules.begin();
+	std::advance(it, i);
+	return *it;
+}
+
+void ModuleList::process(Candidate *candidate) const {
+	for (auto &m : modules)
+		m->process(candidate);
+}
+
+void ModuleList::run(Candidate *candidate, bool recursive, bool secondariesFirst) {
+	candidate_vector_t v(1, candidate);
+	run(&v, recursive, secondariesFirst);
+}
+
+/* The fused form: [DevicePropagator] or [DevicePropagator, MaximumTrajectoryLength
+ * without observers (default flags)]. */
+static bool fusable(const ModuleList::module_list_t &mods, const DevicePropagator **prop,
+                    const MaximumTrajectoryLength **limit) {
+	*prop = nullptr;
+	*limit = nullptr;
+	if (mods.empty() || mods.size() > 2)
+		return false;
+	auto it = mods.begin();
+	*prop = dynamic_cast<const DevicePropagator *>(it->get());
+	if (!*prop)
+		return false;
+	if (mods.size() == 2) {
+		++it;
+		*limit = dynamic_cast<const MaximumTrajectoryLength *>(it->get());
+		if (!*limit || !(*limit)->getObserverPositions().empty() || !(*limit)->getMakeRejectedInactive() ||
+		    (*limit)->hasRejectAction() || !((*limit)->getMaximumTrajectoryLength() > 0))
+			return false;
+	}
+	return true;
+}
+
+bool ModuleList::runBatched(const candidate_vector_t &cands) {
+	const DevicePropagator *prop;
+	const MaximumTrajectoryLength *limit;
+	if (!fusable(modules, &prop, &limit))
+		return false;
+	std::vector<Candidate *> act;
+	for (auto &c : cands)
+		if (c.valid() && c->isActive())
+			act.push_back(c.get());
+	lastAccepted = 0;
+	lastKernel = 0;
+	if (act.empty())
+		return true;
+	const double maxLen = limit ? limit->getMaximumTrajectoryLength() : 0;
+	long long steps = maxStepsPerRun;
+	if (steps < 0 && !(maxLen > 0 && prop->getMinimumStep() > 0))
+		throw std::runtime_error("ModuleList::run: the module list never deactivates candidates "
+		                         "(add MaximumTrajectoryLength and use minStep > 0, or setMaximumSteps)");
+	PropHandle p(prop->makeDevicePropagator(maxLen));
+	crp_batch *b = nullptr;
+	throwIf(crp_batch_create(act.size(), &b));
+	BatchHandle bh(b);
+	Packed pk;
+	pack(act, pk);
+	crp_host_state s = pk.view();
+	throwIf(crp_batch_upload(b, act.size(), &s));
+	crp_run_stats st;
+	throwIf(crp_propagate(p.get(), b, steps, &st));
+	throwIf(crp_batch_download(b, &s));
+	std::vector<char> moved(act.size());
+	for (size_t i = 0; i < act.size(); i++)
+		moved[i] = pk.len[i] != act[i]->getTrajectoryLength() || pk.cur[i] != act[i]->getCurrentStep() ||
+		           !pk.active[i];
+	unpack(act, pk, moved, limit);
+	lastAccepted = st.accepted_steps;
+	lastKernel = st.kernel_ms;
+	return true;
+}
+
+void ModuleList::runStepwise(const candidate_vector_t &cands) {
+	// the first module must be the GPU propagator; the others run on the host
+	auto it = modules.begin();
+	const DevicePropagator *prop = dynamic_cast<const DevicePropagator *>(it->get());
+	if (!prop) {
+		for (auto &c : cands)
+			while (c.valid() && c->isActive())
+				process(c.get());
+		return;
+	}
+	PropHandle p(prop->makeDevicePropagator(0));
+	crp_batch *b = nullptr;
+	throwIf(crp_batch_create(cands.size() ? cands.size() : 1, &b));
+	BatchHandle bh(b);
+	for (;;) {
+		std::vector<Candidate *> act;
+		for (auto &c : cands)
+			if (c.valid() && c->isActive())
+				act.push_back(c.get());
+		if (act.empty())
+			break;
+		Packed pk;
+		pack(act, pk);
+		crp_host_state s = pk.view();
+		throwIf(crp_batch_upload(b, act.size(), &s));
+		throwIf(crp_propagate(p.get(), b, 1, nullptr));
+		throwIf(crp_batch_download(b, &s));
+		unpack(act, pk, std::vector<char>(act.size(), 1), nullptr);
+		for (Candidate *c : act)
+			for (auto m = std::next(modules.begin()); m != modules.end(); ++m)
+				(*m)->process(c);
+	}
+}
+
+void ModuleList::run(const candidate_vector_t *candidates, bool recursive, bool secondariesFirst) {
+	if (!candidates || modules.empty())
+		return;
+	if (!runBatched(*candidates))
+		runStepwise(*candidates);
+	if (recursive) {
+		for (auto &c : *candidates)
+			if (c.valid() && !c->secondaries.empty())
+				run(&c->secondaries, recursive, secondariesFirst);
+	}
+}
+
+std::string ModuleList::getDescription() const {
+	std::stringstream ss;
+	ss << "ModuleList\n";
+	for (auto &m : modules)
+		ss << "  " << m->getDescription() << "\n";
+	return ss.str();
+}
+
+void ModuleList::showModules() const { std::cout << getDescription(); }
+
+} // namespace crpropa
