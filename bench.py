@@ -107,6 +107,8 @@ def main():
     ap.add_argument("--workload", default="ck_grid256", choices=sorted(WORKLOADS))
     ap.add_argument("--particles", type=int, default=0, help="override the particle count (tests only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--grid-layout", default="cells", choices=["cells", "points"],
+                    help="device storage of the grid (results are identical)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it (got --gpus %d)"
@@ -118,7 +120,7 @@ def main():
     grid, pos, d, E, q = make_workload(w, args.particles or None)
     n = len(E)
     capi.check(capi.lib().crp_set_device(0))
-    field = capi.Field.grid3f(grid, [0, 0, 0], [w["spacing"]] * 3)
+    field = capi.Field.grid3f(grid, [0, 0, 0], [w["spacing"]] * 3).set_layout(args.grid_layout)
     prop = capi.Propagator(w["kind"], field, w["tolerance"], w["min_step"], w["max_step"],
                            max_length=w["max_length"])
     init = capi.Batch(n)
@@ -162,7 +164,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic: seeded Kolmogorov Grid3f and seeded particles",
-        "config": {"workload": args.workload, "propagator": w["kind"], "precision": "fp64 (bit-exact path)",
+        "config": {"workload": args.workload, "propagator": w["kind"], "precision": "fp64 (bit-exact path)", "grid_layout": args.grid_layout,
                    "particles": n, "grid": "%d^3 FP32" % w["grid_n"], "tolerance": w["tolerance"],
                    "min_step_kpc": w["min_step"] / kpc, "max_step_kpc": w["max_step"] / kpc,
                    "max_length_Mpc": w["max_length"] / Mpc,

@@ -60,6 +60,8 @@ def lib():
                                                      C.POINTER(_vp)]
         L.crp_field_create_planewave.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, C.POINTER(_vp)]
         L.crp_field_destroy.argtypes = [_vp]
+        L.crp_field_set_grid_layout.argtypes = [_vp, C.c_int]
+        L.crp_field_get_grid_layout.argtypes = [_vp, C.POINTER(C.c_int)]
         L.crp_field_eval.argtypes = [_vp, _dp, C.c_size_t, _dp]
         for nm in ("crp_propagator_create_bp", "crp_propagator_create_ck"):
             getattr(L, nm).argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.POINTER(_vp)]
@@ -145,6 +147,18 @@ class Field:
         h = _vp()
         check(lib().crp_field_create_planewave(len(k), _p(k), _p(Ak), _p(xi), _p(kappa), _p(beta), C.byref(h)))
         return Field(h)
+
+    LAYOUTS = {"points": 0, "cells": 1}
+
+    def set_layout(self, layout):
+        """Grid storage on the device: "points" (8 gathers) or "cells" (corner table)."""
+        check(lib().crp_field_set_grid_layout(self.handle, self.LAYOUTS[layout]))
+        return self
+
+    def layout(self):
+        v = C.c_int(-1)
+        check(lib().crp_field_get_grid_layout(self.handle, C.byref(v)))
+        return {0: "points", 1: "cells"}[v.value]
 
     def eval(self, pos):
         pos = _f64(pos, (-1, 3))

@@ -53,6 +53,7 @@ struct crp_field {
 	FieldUniform uniform;
 	GridDesc grid;
 	PlaneWaveDesc pw;
+	float *cells; // cell-major corner table (periodic grids), or nullptr
 	void *owned[4];
 };
 
@@ -74,6 +75,7 @@ struct crp_batch {
 	double *px, *py, *pz, *dx, *dy, *dz, *E, *q, *cur, *next, *len, *time;
 	double *ppx, *ppy, *ppz, *pdx, *pdy, *pdz;
 	int32_t *active;
+	uint32_t *order; // queue order: particle indices, longest expected work first
 	unsigned long long *ctr;
 	hipEvent_t ev0, ev1;
 };
@@ -82,6 +84,7 @@ struct Soa {
 	double *px, *py, *pz, *dx, *dy, *dz, *E, *q, *cur, *next, *len, *time;
 	double *ppx, *ppy, *ppz, *pdx, *pdy, *pdz;
 	int32_t *active;
+	const uint32_t *order;
 };
 
 struct Params {
@@ -97,6 +100,7 @@ static Soa soa_of(const crp_batch *b) {
 	s.ppx = b->ppx; s.ppy = b->ppy; s.ppz = b->ppz;
 	s.pdx = b->pdx; s.pdy = b->pdy; s.pdz = b->pdz;
 	s.active = b->active;
+	s.order = b->order;
 	return s;
 }
 
@@ -114,6 +118,39 @@ __global__ void __launch_bounds__(256) k_copy_f64(const double *__restrict__ src
 	const size_t stride = (size_t)gridDim.x * blockDim.x;
 	for (; i < n; i += stride)
 		dst[i] = src[i];
+}
+
+// Cell-major corner table of a periodic grid: cell (x0,y0,z0) stores the values
+// of its 8 corners, in the order of Grid.h:572-579, as 24 floats + 8 zeros.
+__global__ void __launch_bounds__(256) k_build_cells(const float *__restrict__ g, float *__restrict__ cells, int nx,
+                                                     int ny, int nz) {
+	const size_t total = (size_t)nx * ny * nz;
+	const size_t stride = (size_t)gridDim.x * blockDim.x;
+	for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += stride) {
+		const int z0 = (int)(c % nz);
+		const size_t t = c / nz;
+		const int y0 = (int)(t % ny);
+		const int x0 = (int)(t / ny);
+		const int x1 = (x0 + 1) % nx, y1 = (y0 + 1) % ny, z1 = (z0 + 1) % nz;
+		const int X[8] = {x0, x1, x0, x0, x1, x0, x1, x1};
+		const int Yc[8] = {y0, y0, y1, y0, y0, y1, y1, y1};
+		const int Z[8] = {z0, z0, z0, z1, z1, z1, z0, z1};
+		float v[32];
+#pragma unroll
+		for (int k = 0; k < 8; k++) {
+			const float *s = g + 3 * (((size_t)X[k] * ny + Yc[k]) * nz + Z[k]);
+			v[3 * k] = s[0];
+			v[3 * k + 1] = s[1];
+			v[3 * k + 2] = s[2];
+		}
+#pragma unroll
+		for (int k = 24; k < 32; k++)
+			v[k] = 0.0f;
+		float4 *o = reinterpret_cast<float4 *>(cells + (size_t)kCellFloats * c);
+#pragma unroll
+		for (int k = 0; k < 8; k++)
+			o[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+	}
 }
 
 template <class F>
@@ -168,7 +205,7 @@ __global__ void __launch_bounds__(256) k_propagate(F field, Params P, Soa S, uns
 			if (need) {
 				const unsigned long long my = base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
 				if (my < n) {
-					idx = my;
+					idx = S.order[my];
 					have = S.active[idx] != 0;
 					if (have) {
 						pos = v3(S.px[idx], S.py[idx], S.pz[idx]);
@@ -417,6 +454,38 @@ static int fill_grid_desc(GridDesc &g, const float *d_values, int nx, int ny, in
 	return CRP_OK;
 }
 
+static int build_cells(crp_field *f) {
+	const GridDesc &g = f->grid;
+	const size_t ncell = (size_t)g.nx * g.ny * g.nz;
+	float *cells = nullptr;
+	hipError_t e = hipMalloc((void **)&cells, ncell * kCellFloats * sizeof(float));
+	if (e != hipSuccess)
+		return fail(CRP_E_NOMEM, std::string("crp_field: cell table hipMalloc: ") + hipGetErrorString(e));
+	const unsigned blocks = (unsigned)std::min<size_t>((ncell + 255) / 256, 65536);
+	hipLaunchKernelGGL(k_build_cells, dim3(blocks), dim3(256), 0, 0, g.values, cells, g.nx, g.ny, g.nz);
+	e = hipGetLastError();
+	if (e == hipSuccess)
+		e = hipDeviceSynchronize();
+	if (e != hipSuccess) {
+		hipFree(cells);
+		return fail(CRP_E_HIP, std::string("crp_field: building the cell table: ") + hipGetErrorString(e));
+	}
+	if (f->cells)
+		hipFree(f->cells);
+	f->cells = cells;
+	return CRP_OK;
+}
+
+/* Periodic grids get the cell table by default; if it does not fit in device
+ * memory the point layout is kept (same results, more traffic). */
+static int build_cells_default(crp_field *f) {
+	if (f->grid.reflective)
+		return CRP_OK;
+	if (build_cells(f) != CRP_OK)
+		(void)hipGetLastError();
+	return CRP_OK;
+}
+
 int crp_field_create_grid3f(const float *values, int nx, int ny, int nz, const double origin[3],
                             const double spacing[3], int reflective, int clip_volume, crp_field_t **out) {
 	if (!values || !out)
@@ -445,7 +514,7 @@ int crp_field_create_grid3f(const float *values, int nx, int ny, int nz, const d
 	f->grid.values = (const float *)d;
 	f->owned[0] = d;
 	*out = f;
-	return CRP_OK;
+	return build_cells_default(f);
 }
 
 int crp_field_create_grid3f_device(const float *d_values, int nx, int ny, int nz, const double origin[3],
@@ -461,6 +530,30 @@ int crp_field_create_grid3f_device(const float *d_values, int nx, int ny, int nz
 		return rc;
 	}
 	*out = f;
+	return build_cells_default(f);
+}
+
+int crp_field_set_grid_layout(crp_field_t *f, int layout) {
+	if (!f || f->kind != FK_GRID)
+		return fail(CRP_E_INVALID, "crp_field_set_grid_layout: not a grid field");
+	if (layout == CRP_GRID_LAYOUT_POINTS) {
+		if (f->cells) {
+			CRP_HIP_TRY(hipFree(f->cells));
+			f->cells = nullptr;
+		}
+		return CRP_OK;
+	}
+	if (layout != CRP_GRID_LAYOUT_CELLS)
+		return fail(CRP_E_INVALID, "crp_field_set_grid_layout: unknown layout");
+	if (f->grid.reflective)
+		return fail(CRP_E_INVALID, "crp_field_set_grid_layout: the cell table needs a periodic grid");
+	return build_cells(f);
+}
+
+int crp_field_get_grid_layout(const crp_field_t *f, int *layout) {
+	if (!f || !layout || f->kind != FK_GRID)
+		return fail(CRP_E_INVALID, "crp_field_get_grid_layout: not a grid field");
+	*layout = f->cells ? CRP_GRID_LAYOUT_CELLS : CRP_GRID_LAYOUT_POINTS;
 	return CRP_OK;
 }
 
@@ -511,6 +604,8 @@ int crp_field_destroy(crp_field_t *f) {
 	for (int i = 0; i < 4; i++)
 		if (f->owned[i])
 			hipFree(f->owned[i]);
+	if (f->cells)
+		hipFree(f->cells);
 	delete f;
 	return CRP_OK;
 }
@@ -533,7 +628,13 @@ int crp_field_eval(const crp_field_t *f, const double *pos, size_t n, double *B)
 		const int kind = f ? f->kind : FK_NONE;
 		switch (kind) {
 		case FK_UNIFORM: hipLaunchKernelGGL(k_field_eval<FieldUniform>, blocks, 256, 0, 0, f->uniform, dpos, dB, n); break;
-		case FK_GRID: hipLaunchKernelGGL(k_field_eval<FieldGrid>, blocks, 256, 0, 0, FieldGrid{f->grid}, dpos, dB, n); break;
+		case FK_GRID:
+			if (f->cells)
+				hipLaunchKernelGGL(k_field_eval<FieldGridCells>, blocks, 256, 0, 0, FieldGridCells{f->grid, f->cells},
+				                   dpos, dB, n);
+			else
+				hipLaunchKernelGGL(k_field_eval<FieldGrid>, blocks, 256, 0, 0, FieldGrid{f->grid}, dpos, dB, n);
+			break;
 		case FK_PLANEWAVE: hipLaunchKernelGGL(k_field_eval<FieldPlaneWave>, blocks, 256, 0, 0, FieldPlaneWave{f->pw}, dpos, dB, n); break;
 		default: hipLaunchKernelGGL(k_field_eval<FieldNone>, blocks, 256, 0, 0, FieldNone{}, dpos, dB, n); break;
 		}
@@ -620,9 +721,10 @@ int crp_batch_create(size_t capacity, crp_batch_t **out) {
 	std::memset(b, 0, sizeof(*b));
 	b->cap = capacity;
 	hipGetDevice(&b->device);
-	// one allocation: 18 double arrays, the active flags, then a 256-byte aligned counter block
+	// one allocation: 18 double arrays, the active flags, the queue order, then a
+	// 256-byte aligned counter block
 	const size_t nd = 18;
-	size_t off = nd * capacity * sizeof(double) + capacity * sizeof(int32_t);
+	size_t off = nd * capacity * sizeof(double) + capacity * sizeof(int32_t) + capacity * sizeof(uint32_t);
 	off = (off + 255) & ~(size_t)255;
 	const size_t bytes = off + 64 * sizeof(unsigned long long);
 	hipError_t e = hipMalloc((void **)&b->base, bytes);
@@ -635,6 +737,7 @@ int crp_batch_create(size_t capacity, crp_batch_t **out) {
 	for (size_t i = 0; i < nd; i++)
 		*arr[i] = b->base + i * capacity;
 	b->active = (int32_t *)(b->base + nd * capacity);
+	b->order = (uint32_t *)(b->active + capacity);
 	b->ctr = (unsigned long long *)((char *)b->base + off);
 	e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
 	if (e == hipSuccess)
@@ -698,6 +801,22 @@ int crp_batch_upload(crp_batch_t *b, size_t n, const crp_host_state *s) {
 		}
 	}
 	CRP_HIP_TRY(hipMemcpy(b->active, s->active, n * sizeof(int32_t), hipMemcpyHostToDevice));
+	// Queue order: ascending magnetic rigidity E/|q| (neutral particles last).  Low-rigidity
+	// particles take the most adaptive steps, so starting them first shortens the tail of a
+	// run (longest-processing-time-first scheduling); results do not depend on the order.
+	if (n > 0xffffffffull)
+		return fail(CRP_E_INVALID, "crp_batch_upload: at most 2^32-1 particles per batch");
+	{
+		std::vector<uint32_t> ord(n);
+		std::vector<double> key(n);
+		for (size_t i = 0; i < n; i++) {
+			ord[i] = (uint32_t)i;
+			const double aq = std::fabs(s->charge[i]);
+			key[i] = aq > 0 ? s->energy[i] / aq : HUGE_VAL;
+		}
+		std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t c) { return key[a] < key[c]; });
+		CRP_HIP_TRY(hipMemcpy(b->order, ord.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+	}
 	// previous := current until the first step
 	CRP_HIP_TRY(hipMemcpy(b->ppx, b->px, 3 * b->cap * sizeof(double), hipMemcpyDeviceToDevice));
 	CRP_HIP_TRY(hipMemcpy(b->pdx, b->dx, 3 * b->cap * sizeof(double), hipMemcpyDeviceToDevice));
@@ -751,6 +870,7 @@ int crp_batch_copy(crp_batch_t *dst, const crp_batch_t *src) {
 		hipLaunchKernelGGL(k_copy_f64, dim3(blocks), dim3(256), 0, dst->stream, s[c], d[c], n);
 	CRP_HIP_TRY(hipGetLastError());
 	CRP_HIP_TRY(hipMemcpyAsync(dst->active, src->active, n * sizeof(int32_t), hipMemcpyDeviceToDevice, dst->stream));
+	CRP_HIP_TRY(hipMemcpyAsync(dst->order, src->order, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, dst->stream));
 	CRP_HIP_TRY(hipStreamSynchronize(dst->stream));
 	return CRP_OK;
 }
@@ -780,7 +900,10 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 	const crp_field *f = p->field;
 	switch (f ? f->kind : FK_NONE) {
 	case FK_UNIFORM: rc = dispatch_kind(f->uniform, p, P, b, &st.launches); break;
-	case FK_GRID: rc = dispatch_kind(FieldGrid{f->grid}, p, P, b, &st.launches); break;
+	case FK_GRID:
+		rc = f->cells ? dispatch_kind(FieldGridCells{f->grid, f->cells}, p, P, b, &st.launches)
+		              : dispatch_kind(FieldGrid{f->grid}, p, P, b, &st.launches);
+		break;
 	case FK_PLANEWAVE: rc = dispatch_kind(FieldPlaneWave{f->pw}, p, P, b, &st.launches); break;
 	default: rc = dispatch_kind(FieldNone{}, p, P, b, &st.launches); break;
 	}

@@ -83,6 +83,14 @@ int crp_field_create_grid3f_device(const float *d_values, int nx, int ny, int nz
 int crp_field_create_planewave(int nm, const double *k, const double *Ak, const double *xi, const double *kappa,
                                const double *beta, crp_field_t **out);
 
+/* Device layout of a grid field.  POINTS reads the Grid3f values where they are (8 gathers of
+ * 12 bytes per evaluation); CELLS (the default for periodic grids) adds a cell-major table with
+ * the 8 corner values of every cell in one 128-byte line (32 floats per cell of device memory).
+ * Both give identical results.  No reference counterpart: a device storage choice. */
+enum crp_grid_layout { CRP_GRID_LAYOUT_POINTS = 0, CRP_GRID_LAYOUT_CELLS = 1 };
+int crp_field_set_grid_layout(crp_field_t *f, int layout);
+int crp_field_get_grid_layout(const crp_field_t *f, int *layout);
+
 int crp_field_destroy(crp_field_t *f);
 
 /* B at n host positions (AoS xyz), evaluated on the device with the FP64 path. */

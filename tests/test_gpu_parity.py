@@ -55,14 +55,39 @@ def make_prop(meta, field):
 
 
 # ------------------------------------------------------------------ fields
-@pytest.mark.parametrize("mode", ["B_periodic", "B_reflective", "B_clip"])
-def test_trilinear_bit_exact(mode):
+@pytest.mark.parametrize("mode,layout", [("B_periodic", "cells"), ("B_periodic", "points"),
+                                         ("B_reflective", "points"), ("B_clip", "cells"), ("B_clip", "points")])
+def test_trilinear_bit_exact(mode, layout):
     c = O.load_case("trilinear16")
     m = c["meta"]
     f = capi.Field.grid3f(c["grid"], m["origin"], m["spacing"], reflective=(mode == "B_reflective"),
                           clip_volume=(mode == "B_clip"))
+    f.set_layout(layout)
+    assert f.layout() == layout
     B = f.eval(c["pos"])
     assert bits_equal(B, c[mode])
+
+
+def test_grid_layout_defaults():
+    c = O.load_case("trilinear16")
+    m = c["meta"]
+    assert capi.Field.grid3f(c["grid"], m["origin"], m["spacing"]).layout() == "cells"
+    fr = capi.Field.grid3f(c["grid"], m["origin"], m["spacing"], reflective=True)
+    assert fr.layout() == "points"
+    with pytest.raises(capi.CrpError):
+        fr.set_layout("cells")
+
+
+def test_nonuniform_grid_shape_cells():
+    """Non-cubic grid with per-axis spacing: both layouts equal the oracle bit for bit."""
+    rng = np.random.default_rng(4)
+    g = (rng.normal(size=(5, 7, 3, 3)) * 1e-13).astype(np.float32)
+    origin, sp = [-1e19, 2e19, 0.0], [3e19, 1e19, 4e19]
+    pos = rng.uniform(-5e20, 5e20, size=(3000, 3))
+    ref = O.OracleField.grid(g, origin, sp).eval(pos)
+    for layout in ("cells", "points"):
+        f = capi.Field.grid3f(g, origin, sp).set_layout(layout)
+        assert bits_equal(f.eval(pos), ref), layout
 
 
 @pytest.mark.parametrize("name", ["planewave_c3", "planewave_general10", "planewave_simple2"])
@@ -85,11 +110,14 @@ def test_uniform_and_null_field():
 
 
 # ------------------------------------------------------------------ trajectories
-@pytest.mark.parametrize("name", TRAJ_CASES)
+@pytest.mark.parametrize("name", TRAJ_CASES + ["bp_fixed_grid32:points", "ck_grid32_maxlen:points"])
 def test_trajectory_parity(name):
+    name, _, layout = name.partition(":")
     c = O.load_case(name)
     m = c["meta"]
     field = device_field(c)
+    if layout:
+        field.set_layout(layout)
     prop = make_prop(m, field)
     n = len(c["energy"])
     b = capi.Batch(n)

@@ -1,0 +1,211 @@
+"""The reference's propagation tests (test/testPropagation.cpp,
+test/testFunctionalGroups.cpp) through the drop-in Python API, on the GPU."""
+import numpy as np
+import pytest
+
+import oracle_ctypes as O
+from crpropa3_amd import (Candidate, EeV, Grid3f, MagneticField, MagneticFieldGrid, MaximumTrajectoryLength,
+                          Module, ModuleList, Mpc, ParticleState, PlaneWaveTurbulence, PropagationBP, PropagationCK,
+                          TeV, TurbulenceSpectrum, UniformMagneticField, Vector3d, gauss, kpc, nG, nucleusId, pc)
+
+pytestmark = pytest.mark.gpu
+
+DEQ = 4 * 2.220446049250313e-16  # gtest EXPECT_DOUBLE_EQ: within 4 ulp
+
+
+def proton(E=100 * EeV, d=(0, 1, 0)):
+    p = ParticleState()
+    p.setId(nucleusId(1, 1))
+    p.setEnergy(E)
+    p.setPosition(Vector3d(0, 0, 0))
+    p.setDirection(Vector3d(*d))
+    return p
+
+
+@pytest.mark.parametrize("cls", [PropagationCK, PropagationBP])
+def test_zero_field(cls):
+    propa = cls(UniformMagneticField(Vector3d(0, 0, 0))) if cls is PropagationCK else \
+        PropagationBP(UniformMagneticField(Vector3d(0, 0, 0)), 1 * kpc)
+    minStep = 0.1 * kpc
+    propa.setMinimumStep(minStep)
+    propa.setTolerance(0.42)
+    c = Candidate(proton())
+    c.setNextStep(0)
+    propa.process(c)
+    assert c.getCurrentStep() == pytest.approx(minStep, rel=DEQ)
+    assert c.getNextStep() == pytest.approx(5 * minStep, rel=DEQ)
+
+
+@pytest.mark.parametrize("cls", [PropagationCK, PropagationBP])
+def test_reduce_step(cls):
+    propa = cls(UniformMagneticField(Vector3d(0, 0, 100 * nG)))
+    minStep, maxStep = 0.1 * kpc, 1e9 * pc
+    propa.setMinimumStep(minStep)
+    propa.setMaximumStep(maxStep)
+    propa.setTolerance(1e-15)
+    c = Candidate(proton(100 * TeV))
+    c.setNextStep(maxStep)
+    propa.process(c)
+    assert c.getCurrentStep() == pytest.approx(minStep, rel=DEQ)
+    assert c.getNextStep() == pytest.approx(minStep, rel=DEQ)
+
+
+@pytest.mark.parametrize("cls", [PropagationCK, PropagationBP])
+def test_increase_step(cls):
+    propa = cls(UniformMagneticField(Vector3d(0, 0, 1 * nG)))
+    minStep, maxStep = 0.001 * pc, 3.125 * pc
+    propa.setMinimumStep(minStep)
+    propa.setMaximumStep(maxStep)
+    propa.setTolerance(0.9)
+    c = Candidate(proton())
+    for i in range(1, 6):
+        propa.process(c)
+        assert c.getNextStep() / pc == pytest.approx(minStep * 5 ** i / pc, rel=DEQ)
+    assert c.getCurrentStep() / pc == pytest.approx(maxStep / pc / 5., rel=DEQ)
+    assert c.getNextStep() / pc == pytest.approx(maxStep / pc, rel=DEQ)
+
+
+def test_bp_proton():
+    propa = PropagationBP(UniformMagneticField(Vector3d(0, 0, 1 * nG)))
+    step = 0.01 * kpc
+    propa.setMinimumStep(step)
+    propa.setMaximumStep(10 * step)
+    propa.setTolerance(0.00001)
+    c = Candidate(proton())
+    c.setNextStep(0)
+    propa.process(c)
+    assert c.getCurrentStep() == pytest.approx(step, rel=DEQ)
+    assert c.getNextStep() == pytest.approx(5 * step, rel=DEQ)
+
+
+@pytest.mark.parametrize("cls,prec", [(PropagationBP, DEQ), (PropagationCK, 1e-7)])
+def test_gyration(cls, prec):
+    propa = cls(UniformMagneticField(Vector3d(0, 0, 1 * nG)))
+    step = 10. * Mpc
+    propa.setMaximumStep(step)
+    propa.setMinimumStep(step)
+    c = Candidate(proton(d=(1, 1, 1)))
+    c.setNextStep(0)
+    propa.process(c)
+    d, p = c.current.getDirection(), c.current.getPosition()
+    assert d.x * d.x + d.y * d.y == pytest.approx(2 / 3., rel=prec)
+    assert d.z * d.z == pytest.approx(1 / 3., rel=prec)
+    assert p.z * p.z == pytest.approx(step * step / 3., rel=prec)
+    for _ in range(9):
+        propa.process(c)
+    d, p = c.current.getDirection(), c.current.getPosition()
+    assert d.x * d.x + d.y * d.y == pytest.approx(2 / 3., rel=prec)
+    assert d.z * d.z == pytest.approx(1 / 3., rel=prec)
+    assert p.z * p.z == pytest.approx(100 * step * step / 3., rel=prec)
+
+
+def test_bp_fixed_step_optimization():
+    """test/testPropagation.cpp:488-526."""
+    fixed = pc
+    p1 = PropagationBP(PlaneWaveTurbulence(TurbulenceSpectrum(gauss, pc, 100 * pc), 10, 1), fixed)
+    c1 = Candidate(proton(d=(1, 1, 1)))
+    for _ in range(9):
+        p1.process(c1)
+    p2 = PropagationBP(PlaneWaveTurbulence(TurbulenceSpectrum(gauss, pc, 100 * pc), 10, 1), 1, fixed, 1.1 * fixed)
+    c2 = Candidate(proton(d=(1, 1, 1)))
+    for _ in range(9):
+        p2.process(c2)
+    for a, b in ((c1.current.getDirection(), c2.current.getDirection()),
+                 (c1.current.getPosition(), c2.current.getPosition())):
+        assert np.allclose([a.x, a.y, a.z], [b.x, b.y, b.z], rtol=DEQ, atol=0)
+
+
+@pytest.mark.parametrize("cls,maxs,nxt", [(PropagationCK, 42 * Mpc, 42 * Mpc), (PropagationBP, 1 * kpc, 1 * kpc)])
+def test_neutron(cls, maxs, nxt):
+    propa = cls(UniformMagneticField(Vector3d(0, 0, 1 * nG)))
+    propa.setMinimumStep(1 * kpc)
+    propa.setMaximumStep(maxs)
+    p = ParticleState()
+    p.setId(nucleusId(1, 0))
+    p.setEnergy(100 * EeV)
+    p.setPosition(Vector3d(0, 0, 0))
+    p.setDirection(Vector3d(0, 1, 0))
+    c = Candidate(p)
+    propa.process(c)
+    assert c.getCurrentStep() == pytest.approx(1 * kpc, rel=DEQ)
+    assert c.getNextStep() == pytest.approx(nxt, rel=DEQ)
+    pos = c.current.getPosition()
+    assert (pos.x, pos.y, pos.z) == (0, 1 * kpc, 0)
+
+
+def test_functional_gyroradius():
+    """test/testFunctionalGroups.cpp:10-43: CK through a ModuleList, helix radius E/(qBc)."""
+    B = 1 * nG
+    E = 100 * EeV
+    sim = ModuleList()
+    sim.add(PropagationCK(UniformMagneticField(Vector3d(0, 0, B)), 1e-4, 0.1 * kpc, 5 * Mpc))
+    sim.add(MaximumTrajectoryLength(2 * np.pi * E / (O.eplus * B * O.c_light)))
+    c = Candidate(proton(E, (1, 0, 0)))
+    sim.run(c)
+    rg = E / (O.eplus * B * O.c_light)
+    p = c.current.getPosition()
+    assert not c.isActive()
+    assert c.getProperty("Rejected") == "MaximumTrajectoryLength"
+    assert np.hypot(p.x, p.y + rg) == pytest.approx(rg, rel=1e-3)
+
+
+def _grid_candidates(n, seed):
+    g = O.load_case("grid32")
+    gm = g["meta"]["grid"]
+    grid = Grid3f(Vector3d(0.), gm["n"], gm["spacing"])
+    grid.array()[:] = g["grid"]
+    sp, E, d = O.sample_particles(seed, n, 1 * EeV, 300 * EeV, [0.5, 0.2, 0.15, 0.1, 0.05])
+    Z = np.array([1, 2, 7, 14, 26])[sp]
+    L = gm["n"] * gm["spacing"]
+    cands = [Candidate(nucleusId(60, int(Z[i])), float(E[i]), Vector3d(L / 2, L / 2, L / 2),
+                       Vector3d(*d[i])) for i in range(n)]
+    return grid, g["grid"], gm, cands, E, Z, d
+
+
+def test_modulelist_batched_equals_oracle_and_per_candidate():
+    grid, gvals, gm, cands, E, Z, d = _grid_candidates(512, 44)
+    field = MagneticFieldGrid(grid)
+    sim = ModuleList()
+    sim.add(PropagationCK(field, 1e-4, 1 * kpc, 1 * Mpc))
+    sim.add(MaximumTrajectoryLength(2 * Mpc))
+    sim.run(cands)
+    assert all(not c.isActive() for c in cands)
+    pos = np.array([[c.current.getPosition().x, c.current.getPosition().y, c.current.getPosition().z] for c in cands])
+    L = gm["n"] * gm["spacing"]
+    st = O.OracleState(np.tile([L / 2] * 3, (512, 1)), d, E, Z * O.eplus)
+    O.propagate(O.OracleField.grid(gvals, gm["origin"], gm["spacing"]), O.PROP_CK, st, 10 ** 9, tolerance=1e-4,
+                min_step=1 * kpc, max_step=1 * Mpc, max_len=2 * Mpc, libm=O.LIBM_CR)
+    assert np.array_equal(pos.view(np.uint64), st.pos.view(np.uint64))
+    # the step-wise path (a host module after the propagator) gives the same result
+    class Noop(Module):
+        def process(self, c):
+            pass
+    _, _, _, cands2, _, _, _ = _grid_candidates(64, 44)
+    sim2 = ModuleList()
+    sim2.add(PropagationCK(field, 1e-4, 1 * kpc, 1 * Mpc))
+    sim2.add(MaximumTrajectoryLength(2 * Mpc))
+    sim2.add(Noop())
+    sim2.run(cands2)
+    pos2 = np.array([[c.current.getPosition().x, c.current.getPosition().y, c.current.getPosition().z]
+                     for c in cands2])
+    assert np.array_equal(pos2.view(np.uint64), st.pos[:64].view(np.uint64))
+
+
+def test_python_field_rejected():
+    class MyField(MagneticField):
+        def getField(self, pos):
+            return Vector3d(0, 0, 1e-13)
+    propa = PropagationBP(MyField(), 1 * kpc)
+    with pytest.raises(RuntimeError, match="no GPU implementation"):
+        propa.process(Candidate(proton()))
+
+
+def test_grid_interpolate_on_device():
+    c = O.load_case("trilinear16")
+    m = c["meta"]
+    grid = Grid3f(Vector3d(*m["origin"]), 16, m["spacing"])
+    grid.array()[:] = c["grid"]
+    f = MagneticFieldGrid(grid)
+    for i in range(0, 200, 7):
+        b = f.getField(Vector3d(*c["pos"][i]))
+        assert (b.x, b.y, b.z) == tuple(c["B_periodic"][i])
