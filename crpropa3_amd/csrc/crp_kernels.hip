@@ -547,7 +547,7 @@ int crp_field_set_grid_layout(crp_field_t *f, int layout) {
 		return fail(CRP_E_INVALID, "crp_field_set_grid_layout: unknown layout");
 	if (f->grid.reflective)
 		return fail(CRP_E_INVALID, "crp_field_set_grid_layout: the cell table needs a periodic grid");
-	return build_cells(f);
+	return f->cells ? CRP_OK : build_cells(f);
 }
 
 int crp_field_get_grid_layout(const crp_field_t *f, int *layout) {
