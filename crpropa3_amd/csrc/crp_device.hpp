@@ -32,6 +32,20 @@ __device__ __forceinline__ V3 v3(double x, double y, double z) {
 	return r;
 }
 
+// Cash-Karp tableau of src/module/PropagationCK.cpp:11-26 (textbook constants).
+struct CK {
+	static constexpr double a10 = 1. / 5.;
+	static constexpr double a20 = 3. / 40., a21 = 9. / 40.;
+	static constexpr double a30 = 3. / 10., a31 = -9. / 10., a32 = 6. / 5.;
+	static constexpr double a40 = -11. / 54., a41 = 5. / 2., a42 = -70. / 27., a43 = 35. / 27.;
+	static constexpr double a50 = 1631. / 55296., a51 = 175. / 512., a52 = 575. / 13824., a53 = 44275. / 110592.,
+	                        a54 = 253. / 4096.;
+	static constexpr double b0 = 37. / 378., b1 = 0., b2 = 250. / 621., b3 = 125. / 594., b4 = 0.,
+	                        b5 = 512. / 1771.;
+	static constexpr double s0 = 2825. / 27648., s1 = 0., s2 = 18575. / 48384., s3 = 13525. / 55296.,
+	                        s4 = 277. / 14336., s5 = 1. / 4.;
+};
+
 // Vector3<T>::getR: sqrt((x*x + y*y) + z*z)
 __device__ __forceinline__ double norm(const V3 &v) { return __builtin_sqrt((v.x * v.x + v.y * v.y) + v.z * v.z); }
 
@@ -44,6 +58,7 @@ struct GridDesc {
 	double gridOrigin[3]; // origin + spacing/2 (Grid.h:225-228)
 	double spacing[3];
 	double origin[3], edge[3]; // clipVolume box: origin, origin + N*spacing (Grid.h:320-327)
+	double invSpacing[3];      // 1/spacing (mixed-precision path only)
 };
 
 struct PlaneWaveDesc {
@@ -52,16 +67,56 @@ struct PlaneWaveDesc {
 	const double *beta;  // [nm]
 	const double *kappa; // [3*nm]
 	const double *xiA;   // [3*nm]: xi[i] * Ak[i], per component (the first product of getField)
+	// mixed-precision path: phase in turns = p . kt + bt, amplitude vectors in FP32
+	const double *kt;  // [3*nm]: k[i] * kappa[i] / (2 pi)
+	const double *bt;  // [nm]: beta[i] / (2 pi)
+	const float *xiAf; // [3*nm]
 };
 
 struct FieldUniform {
 	double b[3];
 	__device__ __forceinline__ V3 eval(const V3 &) const { return v3(b[0], b[1], b[2]); }
+	__device__ __forceinline__ float3 evalf(const V3 &) const { return make_float3((float)b[0], (float)b[1], (float)b[2]); }
 };
 
 struct FieldNone {
 	__device__ __forceinline__ V3 eval(const V3 &) const { return v3(0.0, 0.0, 0.0); }
+	__device__ __forceinline__ float3 evalf(const V3 &) const { return make_float3(0.f, 0.f, 0.f); }
 };
+
+// ---------------------------------------------------------------- mixed-precision grid helpers
+// Cell index and fraction of a periodic axis without integer division: the
+// position in cells is formed with a multiply by 1/spacing (FP64), wrapped
+// with floor() in FP64, then the fraction goes to FP32.
+__device__ __forceinline__ void periodic_cell_fast(double x, int n, double invn, int &lo, float &frac) {
+	double fl = __builtin_floor(x);
+	frac = (float)(x - fl);
+	double w = fl - (double)n * __builtin_floor(fl * invn);
+	int i = (int)w;
+	i = i >= n ? i - n : (i < 0 ? i + n : i);
+	lo = i;
+}
+
+__device__ __forceinline__ float3 trilinear_f32(const float *v, float fx, float fy, float fz) {
+	// v: 24 floats, corners in the order of Grid.h:572-579
+	const float gx = 1.f - fx, gy = 1.f - fy, gz = 1.f - fz;
+	const float w0 = gx * gy * gz, w1 = fx * gy * gz, w2 = gx * fy * gz, w3 = gx * gy * fz;
+	const float w4 = fx * gy * fz, w5 = gx * fy * fz, w6 = fx * fy * gz, w7 = fx * fy * fz;
+	float b[3];
+#pragma unroll
+	for (int a = 0; a < 3; a++) {
+		float s = v[a] * w0;
+		s = __builtin_fmaf(v[3 + a], w1, s);
+		s = __builtin_fmaf(v[6 + a], w2, s);
+		s = __builtin_fmaf(v[9 + a], w3, s);
+		s = __builtin_fmaf(v[12 + a], w4, s);
+		s = __builtin_fmaf(v[15 + a], w5, s);
+		s = __builtin_fmaf(v[18 + a], w6, s);
+		s = __builtin_fmaf(v[21 + a], w7, s);
+		b[a] = s;
+	}
+	return make_float3(b[0], b[1], b[2]);
+}
 
 __device__ __forceinline__ void periodic_clamp(double x, int n, int &lo, int &hi, double &frac) {
 	double fl = __builtin_floor(x);
@@ -140,6 +195,44 @@ struct FieldGrid {
 		}
 		return v3((double)b[0], (double)b[1], (double)b[2]);
 	}
+
+	// mixed precision: reference index arithmetic, FP32 weights and fused sum
+	__device__ __forceinline__ float3 evalf(const V3 &p) const {
+		if (g.clip) {
+			bool in = (p.x >= g.origin[0]) && (p.x <= g.edge[0]);
+			in &= (p.y >= g.origin[1]) && (p.y <= g.edge[1]);
+			in &= (p.z >= g.origin[2]) && (p.z <= g.edge[2]);
+			if (!in)
+				return make_float3(0.f, 0.f, 0.f);
+		}
+		double rx = (p.x - g.gridOrigin[0]) * g.invSpacing[0];
+		double ry = (p.y - g.gridOrigin[1]) * g.invSpacing[1];
+		double rz = (p.z - g.gridOrigin[2]) * g.invSpacing[2];
+		int x0, x1, y0, y1, z0, z1;
+		double fx, fy, fz;
+		if (g.reflective) {
+			reflective_clamp(rx, g.nx, x0, x1, fx);
+			reflective_clamp(ry, g.ny, y0, y1, fy);
+			reflective_clamp(rz, g.nz, z0, z1, fz);
+		} else {
+			periodic_clamp(rx, g.nx, x0, x1, fx);
+			periodic_clamp(ry, g.ny, y0, y1, fy);
+			periodic_clamp(rz, g.nz, z0, z1, fz);
+		}
+		const size_t sy = (size_t)g.nz, sx = (size_t)g.ny * g.nz;
+		const size_t X[2] = {(size_t)x0 * sx, (size_t)x1 * sx}, Yo[2] = {(size_t)y0 * sy, (size_t)y1 * sy};
+		const int Z[2] = {z0, z1};
+		const int cx[8] = {0, 1, 0, 0, 1, 0, 1, 1}, cy[8] = {0, 0, 1, 0, 0, 1, 1, 1}, cz[8] = {0, 0, 0, 1, 1, 1, 0, 1};
+		float v[24];
+#pragma unroll
+		for (int k = 0; k < 8; k++) {
+			const float *c = g.values + 3 * (X[cx[k]] + Yo[cy[k]] + Z[cz[k]]);
+			v[3 * k] = c[0];
+			v[3 * k + 1] = c[1];
+			v[3 * k + 2] = c[2];
+		}
+		return trilinear_f32(v, (float)fx, (float)fy, (float)fz);
+	}
 };
 
 // Periodic Grid3f read through a cell-major corner table: cell (i,j,k) holds
@@ -192,6 +285,28 @@ struct FieldGridCells {
 		}
 		return v3((double)b[0], (double)b[1], (double)b[2]);
 	}
+
+	// mixed precision: reciprocal spacing, division-free periodic wrap, fused FP32 sum
+	__device__ __forceinline__ float3 evalf(const V3 &p) const {
+		if (g.clip) {
+			bool in = (p.x >= g.origin[0]) && (p.x <= g.edge[0]);
+			in &= (p.y >= g.origin[1]) && (p.y <= g.edge[1]);
+			in &= (p.z >= g.origin[2]) && (p.z <= g.edge[2]);
+			if (!in)
+				return make_float3(0.f, 0.f, 0.f);
+		}
+		int x0, y0, z0;
+		float fx, fy, fz;
+		periodic_cell_fast((p.x - g.gridOrigin[0]) * g.invSpacing[0], g.nx, 1.0 / g.nx, x0, fx);
+		periodic_cell_fast((p.y - g.gridOrigin[1]) * g.invSpacing[1], g.ny, 1.0 / g.ny, y0, fy);
+		periodic_cell_fast((p.z - g.gridOrigin[2]) * g.invSpacing[2], g.nz, 1.0 / g.nz, z0, fz);
+		const size_t cidx = ((size_t)x0 * g.ny + y0) * g.nz + z0;
+		const float4 *c = reinterpret_cast<const float4 *>(cells + kCellFloats * cidx);
+		const float4 q0 = c[0], q1 = c[1], q2 = c[2], q3 = c[3], q4 = c[4], q5 = c[5];
+		const float v[24] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w,
+		                     q3.x, q3.y, q3.z, q3.w, q4.x, q4.y, q4.z, q4.w, q5.x, q5.y, q5.z, q5.w};
+		return trilinear_f32(v, fx, fy, fz);
+	}
 };
 
 struct FieldPlaneWave {
@@ -209,7 +324,109 @@ struct FieldPlaneWave {
 		}
 		return v3(bx, by, bz);
 	}
+
+	// mixed precision: phase in turns in FP64 (positions span many wavelengths),
+	// reduced to [-1/2, 1/2] and evaluated with the hardware cosine
+	// (v_cos_f32 computes cos(2 pi x)); amplitudes and the sum in FP32.
+	__device__ __forceinline__ float3 evalf(const V3 &p) const {
+		float bx = 0.f, by = 0.f, bz = 0.f;
+		for (int i = 0; i < w.nm; i++) {
+			double t = __builtin_fma(p.x, w.kt[3 * i], __builtin_fma(p.y, w.kt[3 * i + 1], __builtin_fma(p.z, w.kt[3 * i + 2], w.bt[i])));
+			const float fr = (float)(t - __builtin_rint(t));
+			const float c = __builtin_amdgcn_cosf(fr);
+			bx = __builtin_fmaf(w.xiAf[3 * i], c, bx);
+			by = __builtin_fmaf(w.xiAf[3 * i + 1], c, by);
+			bz = __builtin_fmaf(w.xiAf[3 * i + 2], c, bz);
+		}
+		return make_float3(bx, by, bz);
+	}
 };
+
+// ---------------------------------------------------------------- mixed-precision steps
+__device__ __forceinline__ float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
+__device__ __forceinline__ float3 cross3(const float3 &a, const float3 &b) {
+	return f3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+__device__ __forceinline__ float dot3(const float3 &a, const float3 &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 unit3(const float3 &a) {
+	const float r = __builtin_amdgcn_rsqf(dot3(a, a));
+	return f3(a.x * r, a.y * r, a.z * r);
+}
+
+// Boris step with FP32 field and rotation, FP64 position drift.  coef =
+// (q/2/m) * step / c formed in FP64 by the caller (t = B * coef).
+template <class F>
+__device__ __forceinline__ void bp_dy_mixed(const F &field, V3 &pos, float3 &dir, double step, float coef) {
+	const double hs = step * 0.5;
+	pos = v3(pos.x + (double)dir.x * hs, pos.y + (double)dir.y * hs, pos.z + (double)dir.z * hs);
+	const float3 B = field.evalf(pos);
+	const float3 t = f3(B.x * coef, B.y * coef, B.z * coef);
+	const float sf = 2.f / (1.f + dot3(t, t));
+	const float3 s = f3(t.x * sf, t.y * sf, t.z * sf);
+	const float3 c1 = cross3(dir, t);
+	const float3 vh = f3(dir.x + c1.x, dir.y + c1.y, dir.z + c1.z);
+	const float3 c2 = cross3(vh, s);
+	dir = f3(dir.x + c2.x, dir.y + c2.y, dir.z + c2.z);
+	pos = v3(pos.x + (double)dir.x * hs, pos.y + (double)dir.y * hs, pos.z + (double)dir.z * hs);
+}
+
+// Cash-Karp trial with FP32 direction stages and field, FP64 positions and
+// FP64 error accumulation.  Q = q c^2 / E (per particle); h = step / c.
+struct YM {
+	V3 x;
+	float3 u;
+};
+
+template <class F>
+__device__ __forceinline__ void ck_dydt_mixed(const F &field, const YM &y, float Q, float3 &n, float3 &du) {
+	n = unit3(y.u);                 // velocity / c
+	const float3 B = field.evalf(y.x);
+	const float3 c = cross3(n, B);
+	du = f3(c.x * Q, c.y * Q, c.z * Q);
+}
+
+template <class F>
+__device__ __forceinline__ void ck_trystep_mixed(const F &field, const YM &y, double h, float Q, YM &out, double &errU) {
+	constexpr double cA[6][5] = {{0, 0, 0, 0, 0},
+	                             {CK::a10, 0, 0, 0, 0},
+	                             {CK::a20, CK::a21, 0, 0, 0},
+	                             {CK::a30, CK::a31, CK::a32, 0, 0},
+	                             {CK::a40, CK::a41, CK::a42, CK::a43, 0},
+	                             {CK::a50, CK::a51, CK::a52, CK::a53, CK::a54}};
+	constexpr double cB[6] = {CK::b0, CK::b1, CK::b2, CK::b3, CK::b4, CK::b5};
+	constexpr double cE[6] = {CK::b0 - CK::s0, CK::b1 - CK::s1, CK::b2 - CK::s2,
+	                          CK::b3 - CK::s3, CK::b4 - CK::s4, CK::b5 - CK::s5};
+	const double ch = kCLight * h; // position step per unit of velocity/c
+	const float hf = (float)h;
+	float3 kn[6], ku[6];
+	double ex = 0, ey = 0, ez = 0;
+	out = y;
+#pragma unroll
+	for (int i = 0; i < 6; i++) {
+		YM yn = y;
+#pragma unroll
+		for (int j = 0; j < i; j++) {
+			const double ax = cA[i][j] * ch;
+			const float au = (float)cA[i][j] * hf;
+			yn.x = v3(yn.x.x + (double)kn[j].x * ax, yn.x.y + (double)kn[j].y * ax, yn.x.z + (double)kn[j].z * ax);
+			yn.u = f3(__builtin_fmaf(ku[j].x, au, yn.u.x), __builtin_fmaf(ku[j].y, au, yn.u.y),
+			          __builtin_fmaf(ku[j].z, au, yn.u.z));
+		}
+		ck_dydt_mixed(field, yn, Q, kn[i], ku[i]);
+		if (cB[i] != 0.0) {
+			const double bx = cB[i] * ch;
+			const float bu = (float)cB[i] * hf;
+			out.x = v3(out.x.x + (double)kn[i].x * bx, out.x.y + (double)kn[i].y * bx, out.x.z + (double)kn[i].z * bx);
+			out.u = f3(__builtin_fmaf(ku[i].x, bu, out.u.x), __builtin_fmaf(ku[i].y, bu, out.u.y),
+			           __builtin_fmaf(ku[i].z, bu, out.u.z));
+		}
+		const double eh = cE[i] * h;
+		ex += (double)ku[i].x * eh;
+		ey += (double)ku[i].y * eh;
+		ez += (double)ku[i].z * eh;
+	}
+	errU = __builtin_sqrt((ex * ex + ey * ey) + ez * ez);
+}
 
 // ---------------------------------------------------------------- Boris push
 struct Y {
@@ -237,20 +454,6 @@ __device__ __forceinline__ Y bp_dy(const F &field, const V3 &pos0, const V3 &dir
 }
 
 // ---------------------------------------------------------------- Cash-Karp
-// Tableau of src/module/PropagationCK.cpp:11-26 (textbook Cash-Karp constants).
-struct CK {
-	static constexpr double a10 = 1. / 5.;
-	static constexpr double a20 = 3. / 40., a21 = 9. / 40.;
-	static constexpr double a30 = 3. / 10., a31 = -9. / 10., a32 = 6. / 5.;
-	static constexpr double a40 = -11. / 54., a41 = 5. / 2., a42 = -70. / 27., a43 = 35. / 27.;
-	static constexpr double a50 = 1631. / 55296., a51 = 175. / 512., a52 = 575. / 13824., a53 = 44275. / 110592.,
-	                        a54 = 253. / 4096.;
-	static constexpr double b0 = 37. / 378., b1 = 0., b2 = 250. / 621., b3 = 125. / 594., b4 = 0.,
-	                        b5 = 512. / 1771.;
-	static constexpr double s0 = 2825. / 27648., s1 = 0., s2 = 18575. / 48384., s3 = 13525. / 55296.,
-	                        s4 = 277. / 14336., s5 = 1. / 4.;
-};
-
 // PropagationCK::dYdt (src/module/PropagationCK.cpp:51-61); qcE = (q*c)/E
 template <class F>
 __device__ __forceinline__ Y ck_dydt(const F &field, const Y &y, double qcE) {

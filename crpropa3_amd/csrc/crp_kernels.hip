@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "crp_device.hpp"
+#include "crp_propagate.hpp"
 #include "crpropa3_amd.h"
 
 using namespace crp;
@@ -45,7 +46,6 @@ static int fail(int code, const std::string &msg) {
 
 // ------------------------------------------------------------------ objects
 enum FieldKind { FK_NONE = 0, FK_UNIFORM = 1, FK_GRID = 2, FK_PLANEWAVE = 3 };
-enum PropKind { PK_BP_FIXED = 0, PK_BP_ADAPTIVE = 1, PK_CK_FIXED = 2, PK_CK_ADAPTIVE = 3 };
 
 struct crp_field {
 	int kind;
@@ -54,7 +54,7 @@ struct crp_field {
 	GridDesc grid;
 	PlaneWaveDesc pw;
 	float *cells; // cell-major corner table (periodic grids), or nullptr
-	void *owned[4];
+	void *owned[8];
 };
 
 struct crp_propagator {
@@ -65,7 +65,6 @@ struct crp_propagator {
 	int precision;
 };
 
-enum { CTR_QUEUE = 0, CTR_ACCEPTED, CTR_REJECTED, CTR_EVALS, CTR_FINISHED, CTR_COUNT };
 
 struct crp_batch {
 	size_t cap, n;
@@ -80,17 +79,6 @@ struct crp_batch {
 	hipEvent_t ev0, ev1;
 };
 
-struct Soa {
-	double *px, *py, *pz, *dx, *dy, *dz, *E, *q, *cur, *next, *len, *time;
-	double *ppx, *ppy, *ppz, *pdx, *pdy, *pdz;
-	int32_t *active;
-	const uint32_t *order;
-};
-
-struct Params {
-	double tol, minStep, maxStep, maxLen;
-	long long maxSteps; // < 0: unlimited
-};
 
 static Soa soa_of(const crp_batch *b) {
 	Soa s;
@@ -165,220 +153,47 @@ __global__ void k_field_eval(F field, const double *__restrict__ pos, double *__
 	B[3 * i + 2] = b.z;
 }
 
-__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; } // std::max
-__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; } // std::min
-__device__ __forceinline__ double dclip(double x, double lo, double hi) { return dmax(lo, dmin(x, hi)); }
 
-__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
-#pragma unroll
-	for (int o = 32; o > 0; o >>= 1)
-		v += __shfl_xor(v, o);
-	return v;
-}
-
-template <class F, int KIND>
-__global__ void __launch_bounds__(256) k_propagate(F field, Params P, Soa S, unsigned long long n,
-                                                   unsigned long long *__restrict__ ctr) {
-	constexpr bool kCK = (KIND == PK_CK_FIXED || KIND == PK_CK_ADAPTIVE);
-	constexpr bool kAdaptive = (KIND == PK_BP_ADAPTIVE || KIND == PK_CK_ADAPTIVE);
-	constexpr unsigned long long kEvalsPerTrial = kCK ? 6 : (kAdaptive ? 3 : 1);
-	const int lane = __lane_id();
-
-	bool have = false, exhausted = false, inProc = false;
-	unsigned long long idx = 0;
-	long long steps = 0;
-	V3 pos = v3(0, 0, 0), dir = v3(0, 0, 0), ppos = pos, pdir = dir;
-	double E = 0, q = 0, cur = 0, next = 0, len = 0, tim = 0;
-	double step = 0, newStep = 0;
-	unsigned long long nAcc = 0, nRej = 0, nEval = 0, nFin = 0;
-
-	for (;;) {
-		// ---- refill lanes without a particle from the queue
-		const bool need = !have && !exhausted;
-		const unsigned long long m = __ballot(need);
-		if (m) {
-			const int leader = __ffsll((long long)m) - 1;
-			unsigned long long base = 0;
-			if (lane == leader)
-				base = atomicAdd(&ctr[CTR_QUEUE], (unsigned long long)__popcll(m));
-			base = __shfl(base, leader);
-			if (need) {
-				const unsigned long long my = base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
-				if (my < n) {
-					idx = S.order[my];
-					have = S.active[idx] != 0;
-					if (have) {
-						pos = v3(S.px[idx], S.py[idx], S.pz[idx]);
-						dir = v3(S.dx[idx], S.dy[idx], S.dz[idx]);
-						E = S.E[idx];
-						q = S.q[idx];
-						cur = S.cur[idx];
-						next = S.next[idx];
-						len = S.len[idx];
-						tim = S.time[idx];
-						steps = 0;
-						inProc = false;
-					}
-				} else {
-					exhausted = true;
-				}
-			}
-		}
-		if (__ballot(have) == 0) {
-			if (__ballot(!exhausted) == 0)
-				break;
-			continue;
-		}
-		if (!have)
-			continue;
-
-		// ---- one trial of process() for this lane
-		bool commit = false;
-		Y yOut;
-		if (!inProc) {
-			ppos = pos; // candidate->previous = current
-			pdir = dir;
-			if (q == 0) {
-				// rectilinear propagation for neutral particles (PropagationBP.cpp:74-80)
-				step = dclip(next, P.minStep, P.maxStep);
-				yOut.x = v3(pos.x + dir.x * step, pos.y + dir.y * step, pos.z + dir.z * step);
-				yOut.u = dir;
-				newStep = P.maxStep;
-				commit = true;
-			} else if (!kAdaptive) {
-				step = P.maxStep;
-				newStep = step;
-				if (kCK) {
-					double eU;
-					ck_trystep(field, Y{pos, dir}, step / kCLight, (q * kCLight) / E, yOut, eU);
-				} else {
-					yOut = bp_dy(field, pos, dir, step, q, E / (kCLight * kCLight));
-				}
-				nEval += kEvalsPerTrial;
-				commit = true;
-			} else {
-				step = dclip(next, P.minStep, P.maxStep);
-				newStep = step;
-				inProc = true;
-			}
-		}
-		if (kAdaptive && inProc) {
-			double r;
-			if (kCK) {
-				double eU;
-				ck_trystep(field, Y{pos, dir}, step / kCLight, (q * kCLight) / E, yOut, eU);
-				r = eU / P.tol;
-			} else {
-				const double m_ = E / (kCLight * kCLight);
-				yOut = bp_dy(field, pos, dir, step, q, m_);
-				Y half = bp_dy(field, pos, dir, step / 2, q, m_);
-				Y cmp = bp_dy(field, half.x, half.u, step / 2, q, m_);
-				V3 d = v3(yOut.x.x - cmp.x.x, yOut.x.y - cmp.x.y, yOut.x.z - cmp.x.z);
-				double Sv = norm(d) / (step * (1 - 1 / 4.));
-				r = __builtin_sqrt((Sv * Sv + Sv * Sv) + Sv * Sv) / P.tol;
-			}
-			nEval += kEvalsPerTrial;
-			if (r > 1) {
-				if (step == P.minStep) {
-					commit = true;
-				} else {
-					nRej++;
-					newStep = step * 0.95 * crp_pow_cr(r, -0.2);
-					newStep = dmax(newStep, 0.1 * step);
-					newStep = dmax(newStep, P.minStep);
-					step = newStep;
-				}
-			} else {
-				if (step != P.maxStep) {
-					newStep = step * 0.95 * crp_pow_cr(r, -0.2);
-					newStep = dmin(newStep, 5 * step);
-					newStep = dmin(newStep, P.maxStep);
-				}
-				commit = true;
-			}
-		}
-		if (!commit)
-			continue;
-
-		// ---- commit: setPosition, setDirection(getUnitVector()), setCurrentStep, setNextStep
-		inProc = false;
-		pos = yOut.x;
-		if (q != 0) {
-			const double n1 = norm(yOut.u);
-			V3 u1 = v3(yOut.u.x / n1, yOut.u.y / n1, yOut.u.z / n1);
-			const double n2 = norm(u1);
-			dir = v3(u1.x / n2, u1.y / n2, u1.z / n2);
-		}
-		cur = step;
-		len += step;
-		tim += step / kCLight;
-		next = newStep;
-		nAcc++;
-		steps++;
-		bool alive = true;
-		if (P.maxLen > 0) { // MaximumTrajectoryLength::process
-			if (len >= P.maxLen) {
-				alive = false;
-				nFin++;
-			} else {
-				next = dmin(next, P.maxLen - len);
-			}
-		}
-		if (!alive || (P.maxSteps >= 0 && steps >= P.maxSteps)) {
-			S.px[idx] = pos.x; S.py[idx] = pos.y; S.pz[idx] = pos.z;
-			S.dx[idx] = dir.x; S.dy[idx] = dir.y; S.dz[idx] = dir.z;
-			S.cur[idx] = cur; S.next[idx] = next; S.len[idx] = len; S.time[idx] = tim;
-			S.active[idx] = alive ? 1 : 0;
-			S.ppx[idx] = ppos.x; S.ppy[idx] = ppos.y; S.ppz[idx] = ppos.z;
-			S.pdx[idx] = pdir.x; S.pdy[idx] = pdir.y; S.pdz[idx] = pdir.z;
-			have = false;
-		}
-	}
-	nAcc = wave_sum(nAcc);
-	nRej = wave_sum(nRej);
-	nEval = wave_sum(nEval);
-	nFin = wave_sum(nFin);
-	if (lane == 0) {
-		atomicAdd(&ctr[CTR_ACCEPTED], nAcc);
-		atomicAdd(&ctr[CTR_REJECTED], nRej);
-		atomicAdd(&ctr[CTR_EVALS], nEval);
-		atomicAdd(&ctr[CTR_FINISHED], nFin);
-	}
-}
 
 // ------------------------------------------------------------------ launch helpers
-template <class F, int KIND>
+template <class F, int KIND, bool MIX>
 static int launch_propagate(const F &f, const Params &P, crp_batch *b, int *launches) {
-	static int blocksPerCU = -1, numCU = -1;
+	static int numCU = -1;
 	if (numCU < 0) {
 		int dev;
 		CRP_HIP_TRY(hipGetDevice(&dev));
 		CRP_HIP_TRY(hipDeviceGetAttribute(&numCU, hipDeviceAttributeMultiprocessorCount, dev));
 	}
 	int bpc = 0;
-	CRP_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_propagate<F, KIND>, 256, 0));
+	CRP_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_propagate<F, KIND, MIX>, 256, 0));
 	if (bpc < 1)
 		bpc = 1;
-	(void)blocksPerCU;
 	size_t want = (b->n + 255) / 256;
 	size_t grid = std::min<size_t>(want, (size_t)numCU * bpc);
 	if (grid < 1)
 		grid = 1;
-	hipLaunchKernelGGL((k_propagate<F, KIND>), dim3((unsigned)grid), dim3(256), 0, b->stream, f, P, soa_of(b),
+	hipLaunchKernelGGL((k_propagate<F, KIND, MIX>), dim3((unsigned)grid), dim3(256), 0, b->stream, f, P, soa_of(b),
 	                   (unsigned long long)b->n, b->ctr);
 	CRP_HIP_TRY(hipGetLastError());
 	(*launches)++;
 	return CRP_OK;
 }
 
-template <class F>
-static int dispatch_kind(const F &f, const crp_propagator *p, const Params &P, crp_batch *b, int *launches) {
+template <class F, bool MIX>
+static int dispatch_kind_p(const F &f, const crp_propagator *p, const Params &P, crp_batch *b, int *launches) {
 	const bool adaptive = !(p->minStep == p->maxStep);
 	if (p->type == 0)
-		return adaptive ? launch_propagate<F, PK_BP_ADAPTIVE>(f, P, b, launches)
-		                : launch_propagate<F, PK_BP_FIXED>(f, P, b, launches);
-	return adaptive ? launch_propagate<F, PK_CK_ADAPTIVE>(f, P, b, launches)
-	                : launch_propagate<F, PK_CK_FIXED>(f, P, b, launches);
+		return adaptive ? launch_propagate<F, PK_BP_ADAPTIVE, MIX>(f, P, b, launches)
+		                : launch_propagate<F, PK_BP_FIXED, MIX>(f, P, b, launches);
+	return adaptive ? launch_propagate<F, PK_CK_ADAPTIVE, MIX>(f, P, b, launches)
+	                : launch_propagate<F, PK_CK_FIXED, MIX>(f, P, b, launches);
+}
+
+template <class F>
+static int dispatch_kind(const F &f, const crp_propagator *p, const Params &P, crp_batch *b, int *launches) {
+	if (p->precision == CRP_PRECISION_MIXED)
+		return dispatch_kind_p<F, true>(f, p, P, b, launches);
+	return dispatch_kind_p<F, false>(f, p, P, b, launches);
 }
 
 // ------------------------------------------------------------------ C ABI
@@ -450,6 +265,7 @@ static int fill_grid_desc(GridDesc &g, const float *d_values, int nx, int ny, in
 		g.origin[a] = origin[a];
 		g.gridOrigin[a] = origin[a] + spacing[a] / 2;      // Grid::setOrigin
 		g.edge[a] = origin[a] + (double)N[a] * spacing[a]; // Grid::interpolate clip box
+		g.invSpacing[a] = 1.0 / spacing[a];
 	}
 	return CRP_OK;
 }
@@ -571,10 +387,21 @@ int crp_field_create_planewave(int nm, const double *k, const double *Ak, const 
 	for (int i = 0; i < nm; i++)
 		for (int a = 0; a < 3; a++)
 			xiA[3 * i + a] = xi[3 * i + a] * Ak[i];
+	// mixed-precision tables: phase in turns = p . (k kappa / 2pi) + beta / 2pi; amplitudes in FP32
+	const double inv2pi = 1.0 / (2.0 * M_PI);
+	std::vector<double> kt(3 * (size_t)nm), bt(nm);
+	std::vector<float> xiAf(3 * (size_t)nm);
+	for (int i = 0; i < nm; i++) {
+		for (int a = 0; a < 3; a++) {
+			kt[3 * i + a] = k[i] * kappa[3 * i + a] * inv2pi;
+			xiAf[3 * i + a] = (float)xiA[3 * i + a];
+		}
+		bt[i] = beta[i] * inv2pi;
+	}
 	const size_t nb = (size_t)nm * sizeof(double);
-	const void *src[4] = {k, beta, kappa, xiA.data()};
-	const size_t sz[4] = {nb, nb, 3 * nb, 3 * nb};
-	for (int j = 0; j < 4; j++) {
+	const void *src[7] = {k, beta, kappa, xiA.data(), kt.data(), bt.data(), xiAf.data()};
+	const size_t sz[7] = {nb, nb, 3 * nb, 3 * nb, 3 * nb, nb, 3 * (size_t)nm * sizeof(float)};
+	for (int j = 0; j < 7; j++) {
 		void *d = nullptr;
 		hipError_t e = hipMalloc(&d, sz[j]);
 		if (e == hipSuccess)
@@ -594,6 +421,9 @@ int crp_field_create_planewave(int nm, const double *k, const double *Ak, const 
 	f->pw.beta = (const double *)f->owned[1];
 	f->pw.kappa = (const double *)f->owned[2];
 	f->pw.xiA = (const double *)f->owned[3];
+	f->pw.kt = (const double *)f->owned[4];
+	f->pw.bt = (const double *)f->owned[5];
+	f->pw.xiAf = (const float *)f->owned[6];
 	*out = f;
 	return CRP_OK;
 }
@@ -601,7 +431,7 @@ int crp_field_create_planewave(int nm, const double *k, const double *Ak, const 
 int crp_field_destroy(crp_field_t *f) {
 	if (!f)
 		return CRP_OK;
-	for (int i = 0; i < 4; i++)
+	for (int i = 0; i < 8; i++)
 		if (f->owned[i])
 			hipFree(f->owned[i]);
 	if (f->cells)
@@ -701,8 +531,8 @@ int crp_propagator_set_max_trajectory_length(crp_propagator_t *p, double max_len
 int crp_propagator_set_precision(crp_propagator_t *p, int precision) {
 	if (!p)
 		return fail(CRP_E_INVALID, "crp_propagator_set_precision: NULL propagator");
-	if (precision != CRP_PRECISION_FP64)
-		return fail(CRP_E_INVALID, "crp_propagator_set_precision: only CRP_PRECISION_FP64 is built");
+	if (precision != CRP_PRECISION_FP64 && precision != CRP_PRECISION_MIXED)
+		return fail(CRP_E_INVALID, "crp_propagator_set_precision: unknown precision");
 	p->precision = precision;
 	return CRP_OK;
 }

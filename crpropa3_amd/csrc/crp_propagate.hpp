@@ -1,0 +1,282 @@
+// The persistent propagation kernel (included by crp_kernels.hip).
+//
+// One particle per lane.  Each loop iteration performs one trial of
+// PropagationBP/CK::process() for every busy lane, so the 1 / 3 / 6 field
+// evaluations of a trial stay converged across the wave; an accepted trial
+// commits the step (setPosition, setDirection, setCurrentStep, setNextStep),
+// the fused MaximumTrajectoryLength condition and the step counter.  Lanes
+// whose particle finished (inactive, or its step budget spent) store it and
+// take the next queue entry: one atomicAdd per wave reserves popcount(ballot)
+// entries, each lane takes its prefix-count slot (wave64 ballot/mbcnt).
+//
+// MIX = false: the reference's FP64 arithmetic bit for bit
+//   (src/module/PropagationBP.cpp:62-124, src/module/PropagationCK.cpp:77-136,
+//    src/module/BreakCondition.cpp:42-64, src/Candidate.cpp:92-104).
+// MIX = true: FP32 field evaluation, rotation and direction stages; FP64
+//   positions, CK error norm, step acceptance and step-size update.
+#pragma once
+
+#include "crp_device.hpp"
+
+namespace crp {
+
+enum PropKind { PK_BP_FIXED = 0, PK_BP_ADAPTIVE = 1, PK_CK_FIXED = 2, PK_CK_ADAPTIVE = 3 };
+enum { CTR_QUEUE = 0, CTR_ACCEPTED, CTR_REJECTED, CTR_EVALS, CTR_FINISHED, CTR_COUNT };
+
+struct Soa {
+	double *px, *py, *pz, *dx, *dy, *dz, *E, *q, *cur, *next, *len, *time;
+	double *ppx, *ppy, *ppz, *pdx, *pdy, *pdz;
+	int32_t *active;
+	const uint32_t *order;
+};
+
+struct Params {
+	double tol, minStep, maxStep, maxLen;
+	long long maxSteps; // < 0: unlimited
+};
+
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; } // std::max
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; } // std::min
+__device__ __forceinline__ double dclip(double x, double lo, double hi) { return dmax(lo, dmin(x, hi)); }
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1)
+		v += __shfl_xor(v, o);
+	return v;
+}
+
+// step-size factor 0.95 r^-0.2: correctly rounded on the FP64 path (the
+// reference calls glibc pow), the device libm on the mixed path
+template <bool MIX>
+__device__ __forceinline__ double step_factor(double r) {
+	if (MIX)
+		return 0.95 * ::pow(r, -0.2);
+	return 0.95 * crp_pow_cr(r, -0.2);
+}
+
+template <class F, int KIND, bool MIX>
+__global__ void __launch_bounds__(256) k_propagate(F field, Params P, Soa S, unsigned long long n,
+                                                   unsigned long long *__restrict__ ctr) {
+	constexpr bool kCK = (KIND == PK_CK_FIXED || KIND == PK_CK_ADAPTIVE);
+	constexpr bool kAdaptive = (KIND == PK_BP_ADAPTIVE || KIND == PK_CK_ADAPTIVE);
+	constexpr unsigned long long kEvalsPerTrial = kCK ? 6 : (kAdaptive ? 3 : 1);
+	const int lane = __lane_id();
+
+	bool have = false, exhausted = false, inProc = false;
+	unsigned long long idx = 0;
+	long long steps = 0;
+	V3 pos = v3(0, 0, 0), dir = v3(0, 0, 0), ppos = pos, pdir = dir;
+	float3 dirf = make_float3(0.f, 0.f, 0.f);
+	double E = 0, q = 0, cur = 0, next = 0, len = 0, tim = 0;
+	double step = 0, newStep = 0;
+	unsigned long long nAcc = 0, nRej = 0, nEval = 0, nFin = 0;
+
+	for (;;) {
+		// ---- refill lanes without a particle from the queue
+		const bool need = !have && !exhausted;
+		const unsigned long long m = __ballot(need);
+		if (m) {
+			const int leader = __ffsll((long long)m) - 1;
+			unsigned long long base = 0;
+			if (lane == leader)
+				base = atomicAdd(&ctr[CTR_QUEUE], (unsigned long long)__popcll(m));
+			base = __shfl(base, leader);
+			if (need) {
+				const unsigned long long my = base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
+				if (my < n) {
+					idx = S.order[my];
+					have = S.active[idx] != 0;
+					if (have) {
+						pos = v3(S.px[idx], S.py[idx], S.pz[idx]);
+						dir = v3(S.dx[idx], S.dy[idx], S.dz[idx]);
+						if (MIX)
+							dirf = make_float3((float)dir.x, (float)dir.y, (float)dir.z);
+						E = S.E[idx];
+						q = S.q[idx];
+						cur = S.cur[idx];
+						next = S.next[idx];
+						len = S.len[idx];
+						tim = S.time[idx];
+						steps = 0;
+						inProc = false;
+					}
+				} else {
+					exhausted = true;
+				}
+			}
+		}
+		if (__ballot(have) == 0) {
+			if (__ballot(!exhausted) == 0)
+				break;
+			continue;
+		}
+		if (!have)
+			continue;
+
+		// ---- one trial of process() for this lane
+		bool commit = false;
+		bool neutral = false;
+		Y yOut;          // FP64 path result
+		V3 oPos;         // mixed path result
+		float3 oDir;
+		if (!inProc) {
+			ppos = pos; // candidate->previous = current
+			pdir = MIX ? v3(dirf.x, dirf.y, dirf.z) : dir;
+			if (q == 0) {
+				// rectilinear propagation for neutral particles (PropagationBP.cpp:74-80)
+				neutral = true;
+				step = dclip(next, P.minStep, P.maxStep);
+				if (MIX)
+					oPos = v3(pos.x + (double)dirf.x * step, pos.y + (double)dirf.y * step, pos.z + (double)dirf.z * step);
+				else
+					yOut.x = v3(pos.x + dir.x * step, pos.y + dir.y * step, pos.z + dir.z * step);
+				newStep = P.maxStep;
+				commit = true;
+			} else if (!kAdaptive) {
+				step = P.maxStep;
+				newStep = step;
+				if (MIX) {
+					if (kCK) {
+						double eU;
+						YM o;
+						ck_trystep_mixed(field, YM{pos, dirf}, step / kCLight, (float)((q * kCLight * kCLight) / E), o, eU);
+						oPos = o.x;
+						oDir = o.u;
+					} else {
+						oPos = pos;
+						oDir = dirf;
+						bp_dy_mixed(field, oPos, oDir, step, (float)((q * kCLight * kCLight) / (2 * E) * step / kCLight));
+					}
+				} else {
+					if (kCK) {
+						double eU;
+						ck_trystep(field, Y{pos, dir}, step / kCLight, (q * kCLight) / E, yOut, eU);
+					} else {
+						yOut = bp_dy(field, pos, dir, step, q, E / (kCLight * kCLight));
+					}
+				}
+				nEval += kEvalsPerTrial;
+				commit = true;
+			} else {
+				step = dclip(next, P.minStep, P.maxStep);
+				newStep = step;
+				inProc = true;
+			}
+		}
+		if (kAdaptive && inProc) {
+			double r;
+			if (MIX) {
+				if (kCK) {
+					double eU;
+					YM o;
+					ck_trystep_mixed(field, YM{pos, dirf}, step / kCLight, (float)((q * kCLight * kCLight) / E), o, eU);
+					oPos = o.x;
+					oDir = o.u;
+					r = eU / P.tol;
+				} else {
+					const double k0 = (q * kCLight * kCLight) / (2 * E) / kCLight;
+					oPos = pos;
+					oDir = dirf;
+					bp_dy_mixed(field, oPos, oDir, step, (float)(k0 * step));
+					V3 hPos = pos;
+					float3 hDir = dirf;
+					bp_dy_mixed(field, hPos, hDir, step / 2, (float)(k0 * (step / 2)));
+					bp_dy_mixed(field, hPos, hDir, step / 2, (float)(k0 * (step / 2)));
+					V3 d = v3(oPos.x - hPos.x, oPos.y - hPos.y, oPos.z - hPos.z);
+					double Sv = norm(d) / (step * 0.75);
+					r = __builtin_sqrt(3.0 * Sv * Sv) / P.tol;
+				}
+			} else {
+				if (kCK) {
+					double eU;
+					ck_trystep(field, Y{pos, dir}, step / kCLight, (q * kCLight) / E, yOut, eU);
+					r = eU / P.tol;
+				} else {
+					const double m_ = E / (kCLight * kCLight);
+					yOut = bp_dy(field, pos, dir, step, q, m_);
+					Y half = bp_dy(field, pos, dir, step / 2, q, m_);
+					Y cmp = bp_dy(field, half.x, half.u, step / 2, q, m_);
+					V3 d = v3(yOut.x.x - cmp.x.x, yOut.x.y - cmp.x.y, yOut.x.z - cmp.x.z);
+					double Sv = norm(d) / (step * (1 - 1 / 4.));
+					r = __builtin_sqrt((Sv * Sv + Sv * Sv) + Sv * Sv) / P.tol;
+				}
+			}
+			nEval += kEvalsPerTrial;
+			if (r > 1) {
+				if (step == P.minStep) {
+					commit = true;
+				} else {
+					nRej++;
+					newStep = step * step_factor<MIX>(r);
+					newStep = dmax(newStep, 0.1 * step);
+					newStep = dmax(newStep, P.minStep);
+					step = newStep;
+				}
+			} else {
+				if (step != P.maxStep) {
+					newStep = step * step_factor<MIX>(r);
+					newStep = dmin(newStep, 5 * step);
+					newStep = dmin(newStep, P.maxStep);
+				}
+				commit = true;
+			}
+		}
+		if (!commit)
+			continue;
+
+		// ---- commit: setPosition, setDirection(getUnitVector()), setCurrentStep, setNextStep
+		inProc = false;
+		if (MIX) {
+			pos = oPos;
+			if (!neutral)
+				dirf = unit3(oDir);
+		} else {
+			pos = yOut.x;
+			if (!neutral) {
+				const double n1 = norm(yOut.u);
+				V3 u1 = v3(yOut.u.x / n1, yOut.u.y / n1, yOut.u.z / n1);
+				const double n2 = norm(u1);
+				dir = v3(u1.x / n2, u1.y / n2, u1.z / n2);
+			}
+		}
+		cur = step;
+		len += step;
+		tim += step / kCLight;
+		next = newStep;
+		nAcc++;
+		steps++;
+		bool alive = true;
+		if (P.maxLen > 0) { // MaximumTrajectoryLength::process
+			if (len >= P.maxLen) {
+				alive = false;
+				nFin++;
+			} else {
+				next = dmin(next, P.maxLen - len);
+			}
+		}
+		if (!alive || (P.maxSteps >= 0 && steps >= P.maxSteps)) {
+			if (MIX)
+				dir = v3(dirf.x, dirf.y, dirf.z);
+			S.px[idx] = pos.x; S.py[idx] = pos.y; S.pz[idx] = pos.z;
+			S.dx[idx] = dir.x; S.dy[idx] = dir.y; S.dz[idx] = dir.z;
+			S.cur[idx] = cur; S.next[idx] = next; S.len[idx] = len; S.time[idx] = tim;
+			S.active[idx] = alive ? 1 : 0;
+			S.ppx[idx] = ppos.x; S.ppy[idx] = ppos.y; S.ppz[idx] = ppos.z;
+			S.pdx[idx] = pdir.x; S.pdy[idx] = pdir.y; S.pdz[idx] = pdir.z;
+			have = false;
+		}
+	}
+	nAcc = wave_sum(nAcc);
+	nRej = wave_sum(nRej);
+	nEval = wave_sum(nEval);
+	nFin = wave_sum(nFin);
+	if (lane == 0) {
+		atomicAdd(&ctr[CTR_ACCEPTED], nAcc);
+		atomicAdd(&ctr[CTR_REJECTED], nRej);
+		atomicAdd(&ctr[CTR_EVALS], nEval);
+		atomicAdd(&ctr[CTR_FINISHED], nFin);
+	}
+}
+
+} // namespace crp
