@@ -43,6 +43,7 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 BYTES_PER_EVAL = 96  # 8 corners x 3 x FP32 of one trilinear Grid3f evaluation
+BYTES_PER_EVAL_FP16 = 52  # 8 corners x 3 x FP16 + one FP32 brick scale
 FLOP_PER_MODE = 16   # dot 5, phase 2, xi*A 3, *cos 3, accumulate 3 (cos counted separately)
 FLOP_BORIS = 60
 
@@ -154,6 +155,8 @@ def main():
                     help="fp64: the reference's arithmetic bit for bit; mixed: FP32 field and rotation")
     ap.add_argument("--grid-layout", default="cells", choices=["cells", "points"],
                     help="device storage of the grid (results are identical)")
+    ap.add_argument("--grid-storage", default="fp32", choices=["fp32", "fp16"],
+                    help="grid values read by the mixed path: FP32, or FP16 with a per-8^3-brick FP32 scale")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -171,6 +174,10 @@ def main():
     capi.check(capi.lib().crp_set_device(0))
     if w["field"] == "grid":
         field = capi.Field.grid3f(fdata, [0, 0, 0], [w["spacing"]] * 3).set_layout(args.grid_layout)
+        if args.grid_storage == "fp16":
+            if args.precision != "mixed":
+                sys.exit("bench.py: --grid-storage fp16 is read by the mixed path only (use --precision mixed)")
+            field.set_storage("fp16")
     else:
         field = capi.Field.planewave(fdata["k"], fdata["Ak"], fdata["xi"], fdata["kappa"], fdata["beta"])
     prec = capi.PRECISION_MIXED if args.precision == "mixed" else capi.PRECISION_FP64
@@ -206,14 +213,16 @@ def main():
               "accepted_steps_per_pass": acc // args.steps,
               "trials_per_pass": (evals // per_trial) // args.steps}
     if w["field"] == "grid":
-        config.update({"grid": "%d^3 FP32, spacing %.0f kpc" % (w["grid_n"], w["spacing"] / kpc),
-                       "grid_layout": args.grid_layout})
-        bytes_per_launch = evals * BYTES_PER_EVAL / launches
+        fp16 = args.grid_storage == "fp16"
+        config.update({"grid": "%d^3 %s, spacing %.0f kpc" % (w["grid_n"], "FP16+brick scale" if fp16 else "FP32",
+                                                              w["spacing"] / kpc),
+                       "grid_layout": "cells16" if fp16 else args.grid_layout})
+        bytes_per_launch = evals * (BYTES_PER_EVAL_FP16 if fp16 else BYTES_PER_EVAL) / launches
         achieved = bytes_per_launch / kern_s / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(args.workload, args.precision),
-                    "kernel": "k_propagate<%s, %s>" % ("FieldGridCells" if args.grid_layout == "cells" else "FieldGrid",
-                                                     w["kind"]),
+                    "kernel": "k_propagate<%s, %s>" % ("FieldGridCells16" if fp16 else (
+                        "FieldGridCells" if args.grid_layout == "cells" else "FieldGrid"), w["kind"]),
                     "kernel_ms_avg": kern_s * 1e3, "algorithmic_bytes_per_launch": bytes_per_launch}
     else:
         config.update({"field": "PlaneWaveTurbulence Nm=%d" % w["nm"]})

@@ -62,6 +62,8 @@ def lib():
         L.crp_field_destroy.argtypes = [_vp]
         L.crp_field_set_grid_layout.argtypes = [_vp, C.c_int]
         L.crp_field_get_grid_layout.argtypes = [_vp, C.POINTER(C.c_int)]
+        L.crp_field_set_grid_storage.argtypes = [_vp, C.c_int]
+        L.crp_field_eval_ex.argtypes = [_vp, _dp, C.c_size_t, _dp, C.c_int]
         L.crp_field_eval.argtypes = [_vp, _dp, C.c_size_t, _dp]
         for nm in ("crp_propagator_create_bp", "crp_propagator_create_ck"):
             getattr(L, nm).argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.POINTER(_vp)]
@@ -160,10 +162,20 @@ class Field:
         check(lib().crp_field_get_grid_layout(self.handle, C.byref(v)))
         return {0: "points", 1: "cells"}[v.value]
 
-    def eval(self, pos):
+    STORAGE = {"fp32": 0, "fp16": 1}
+
+    def set_storage(self, storage):
+        """Grid storage read by mixed-precision runs: "fp32" or "fp16" (FP16 + per-brick scale)."""
+        check(lib().crp_field_set_grid_storage(self.handle, self.STORAGE[storage]))
+        return self
+
+    def eval(self, pos, precision=PRECISION_FP64):
         pos = _f64(pos, (-1, 3))
         B = np.zeros_like(pos)
-        check(lib().crp_field_eval(self.handle, _p(pos), pos.shape[0], _p(B)))
+        if precision == PRECISION_FP64:
+            check(lib().crp_field_eval(self.handle, _p(pos), pos.shape[0], _p(B)))
+        else:
+            check(lib().crp_field_eval_ex(self.handle, _p(pos), pos.shape[0], _p(B), precision))
         return B
 
 

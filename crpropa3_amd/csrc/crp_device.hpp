@@ -11,6 +11,7 @@
 //   PropagationCK::dYdt / tryStep           src/module/PropagationCK.cpp:28-61
 #pragma once
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -306,6 +307,51 @@ struct FieldGridCells {
 		const float v[24] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w,
 		                     q3.x, q3.y, q3.z, q3.w, q4.x, q4.y, q4.z, q4.w, q5.x, q5.y, q5.z, q5.w};
 		return trilinear_f32(v, fx, fy, fz);
+	}
+};
+
+// Reduced storage for the mixed path: each periodic cell holds its 8 corner
+// vectors as FP16 values normalised by the FP32 scale of the 8^3-point brick
+// that contains the cell (max |component| over the brick and its +1 faces),
+// followed by that scale: 48 + 4 bytes, padded to one 64-byte sector.
+constexpr int kCell16Bytes = 64;
+
+struct FieldGridCells16 {
+	GridDesc g;
+	const unsigned char *cells16; // [nx*ny*nz][64]
+
+	__device__ __forceinline__ V3 eval(const V3 &p) const {
+		float3 b = evalf(p);
+		return v3(b.x, b.y, b.z);
+	}
+
+	__device__ __forceinline__ float3 evalf(const V3 &p) const {
+		if (g.clip) {
+			bool in = (p.x >= g.origin[0]) && (p.x <= g.edge[0]);
+			in &= (p.y >= g.origin[1]) && (p.y <= g.edge[1]);
+			in &= (p.z >= g.origin[2]) && (p.z <= g.edge[2]);
+			if (!in)
+				return make_float3(0.f, 0.f, 0.f);
+		}
+		int x0, y0, z0;
+		float fx, fy, fz;
+		periodic_cell_fast((p.x - g.gridOrigin[0]) * g.invSpacing[0], g.nx, 1.0 / g.nx, x0, fx);
+		periodic_cell_fast((p.y - g.gridOrigin[1]) * g.invSpacing[1], g.ny, 1.0 / g.ny, y0, fy);
+		periodic_cell_fast((p.z - g.gridOrigin[2]) * g.invSpacing[2], g.nz, 1.0 / g.nz, z0, fz);
+		const size_t cidx = ((size_t)x0 * g.ny + y0) * g.nz + z0;
+		const uint4 *c = reinterpret_cast<const uint4 *>(cells16 + (size_t)kCell16Bytes * cidx);
+		const uint4 a0 = c[0], a1 = c[1], a2 = c[2], a3 = c[3];
+		const unsigned int w[13] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w, a2.x, a2.y, a2.z, a2.w, a3.x};
+		float v[24];
+#pragma unroll
+		for (int k = 0; k < 12; k++) {
+			const unsigned int u = w[k];
+			v[2 * k] = __half2float(__ushort_as_half((unsigned short)(u & 0xffffu)));
+			v[2 * k + 1] = __half2float(__ushort_as_half((unsigned short)(u >> 16)));
+		}
+		const float scale = __uint_as_float(w[12]);
+		float3 b = trilinear_f32(v, fx, fy, fz);
+		return make_float3(b.x * scale, b.y * scale, b.z * scale);
 	}
 };
 

@@ -54,6 +54,7 @@ struct crp_field {
 	GridDesc grid;
 	PlaneWaveDesc pw;
 	float *cells; // cell-major corner table (periodic grids), or nullptr
+	unsigned char *cells16; // FP16 corner table with brick scales (mixed path), or nullptr
 	void *owned[8];
 };
 
@@ -139,6 +140,85 @@ __global__ void __launch_bounds__(256) k_build_cells(const float *__restrict__ g
 		for (int k = 0; k < 8; k++)
 			o[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
 	}
+}
+
+// Per 8^3-point brick: the largest |component| over the brick's points and its
+// +1 faces (periodic), i.e. over every corner of every cell in the brick.
+__global__ void __launch_bounds__(256) k_brick_scales(const float *__restrict__ g, float *__restrict__ scales, int nx,
+                                                      int ny, int nz, int bx, int by, int bz) {
+	const size_t nb = (size_t)bx * by * bz;
+	const size_t b = (size_t)blockIdx.x;
+	if (b >= nb)
+		return;
+	const int kz = (int)(b % bz), ky = (int)((b / bz) % by), kx = (int)(b / ((size_t)bz * by));
+	float m = 0.f;
+	for (int t = threadIdx.x; t < 9 * 9 * 9; t += blockDim.x) {
+		const int dz = t % 9, dy = (t / 9) % 9, dx = t / 81;
+		const int x = (kx * 8 + dx) % nx, y = (ky * 8 + dy) % ny, z = (kz * 8 + dz) % nz;
+		const float *s = g + 3 * (((size_t)x * ny + y) * nz + z);
+		m = fmaxf(m, fmaxf(fabsf(s[0]), fmaxf(fabsf(s[1]), fabsf(s[2]))));
+	}
+	__shared__ float red[256];
+	red[threadIdx.x] = m;
+	__syncthreads();
+	for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+		if (threadIdx.x < o)
+			red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		scales[b] = red[0] > 0.f ? red[0] : 1.f;
+}
+
+// FP16 cell table: corners of cell (x0,y0,z0) divided by its brick's scale,
+// rounded to FP16, then the scale (see FieldGridCells16).
+__global__ void __launch_bounds__(256) k_build_cells16(const float *__restrict__ g, const float *__restrict__ scales,
+                                                       unsigned char *__restrict__ cells, int nx, int ny, int nz,
+                                                       int by, int bz) {
+	const size_t total = (size_t)nx * ny * nz;
+	const size_t stride = (size_t)gridDim.x * blockDim.x;
+	for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += stride) {
+		const int z0 = (int)(c % nz);
+		const size_t t = c / nz;
+		const int y0 = (int)(t % ny);
+		const int x0 = (int)(t / ny);
+		const int x1 = (x0 + 1) % nx, y1 = (y0 + 1) % ny, z1 = (z0 + 1) % nz;
+		const int X[8] = {x0, x1, x0, x0, x1, x0, x1, x1};
+		const int Yc[8] = {y0, y0, y1, y0, y0, y1, y1, y1};
+		const int Z[8] = {z0, z0, z0, z1, z1, z1, z0, z1};
+		const float scale = scales[((size_t)(x0 >> 3) * by + (y0 >> 3)) * bz + (z0 >> 3)];
+		const float inv = 1.f / scale;
+		unsigned short h[24];
+#pragma unroll
+		for (int k = 0; k < 8; k++) {
+			const float *s = g + 3 * (((size_t)X[k] * ny + Yc[k]) * nz + Z[k]);
+#pragma unroll
+			for (int a = 0; a < 3; a++)
+				h[3 * k + a] = __half_as_ushort(__float2half_rn(s[a] * inv));
+		}
+		unsigned int wv[16];
+#pragma unroll
+		for (int k = 0; k < 12; k++)
+			wv[k] = (unsigned int)h[2 * k] | ((unsigned int)h[2 * k + 1] << 16);
+		wv[12] = __float_as_uint(scale);
+		wv[13] = wv[14] = wv[15] = 0u;
+		uint4 *o = reinterpret_cast<uint4 *>(cells + (size_t)kCell16Bytes * c);
+#pragma unroll
+		for (int k = 0; k < 4; k++)
+			o[k] = make_uint4(wv[4 * k], wv[4 * k + 1], wv[4 * k + 2], wv[4 * k + 3]);
+	}
+}
+
+template <class F>
+__global__ void k_field_evalf(F field, const double *__restrict__ pos, double *__restrict__ B, size_t n) {
+	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	V3 p = v3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+	float3 b = field.evalf(p);
+	B[3 * i] = b.x;
+	B[3 * i + 1] = b.y;
+	B[3 * i + 2] = b.z;
 }
 
 template <class F>
@@ -366,6 +446,94 @@ int crp_field_set_grid_layout(crp_field_t *f, int layout) {
 	return f->cells ? CRP_OK : build_cells(f);
 }
 
+int crp_field_set_grid_storage(crp_field_t *f, int storage) {
+	if (!f || f->kind != FK_GRID)
+		return fail(CRP_E_INVALID, "crp_field_set_grid_storage: not a grid field");
+	if (storage == CRP_GRID_STORAGE_FP32) {
+		if (f->cells16) {
+			CRP_HIP_TRY(hipFree(f->cells16));
+			f->cells16 = nullptr;
+		}
+		return CRP_OK;
+	}
+	if (storage != CRP_GRID_STORAGE_FP16_BRICK)
+		return fail(CRP_E_INVALID, "crp_field_set_grid_storage: unknown storage");
+	if (f->grid.reflective)
+		return fail(CRP_E_INVALID, "crp_field_set_grid_storage: FP16 storage needs a periodic grid");
+	if (f->cells16)
+		return CRP_OK;
+	const GridDesc &g = f->grid;
+	const int bx = (g.nx + 7) / 8, by = (g.ny + 7) / 8, bz = (g.nz + 7) / 8;
+	const size_t nb = (size_t)bx * by * bz, ncell = (size_t)g.nx * g.ny * g.nz;
+	float *scales = nullptr;
+	unsigned char *cells = nullptr;
+	hipError_t e = hipMalloc((void **)&scales, nb * sizeof(float));
+	if (e == hipSuccess)
+		e = hipMalloc((void **)&cells, ncell * kCell16Bytes);
+	if (e != hipSuccess) {
+		if (scales)
+			hipFree(scales);
+		return fail(CRP_E_NOMEM, std::string("crp_field_set_grid_storage: hipMalloc: ") + hipGetErrorString(e));
+	}
+	hipLaunchKernelGGL(k_brick_scales, dim3((unsigned)nb), dim3(256), 0, 0, g.values, scales, g.nx, g.ny, g.nz, bx, by, bz);
+	const unsigned blocks = (unsigned)std::min<size_t>((ncell + 255) / 256, 65536);
+	hipLaunchKernelGGL(k_build_cells16, dim3(blocks), dim3(256), 0, 0, g.values, scales, cells, g.nx, g.ny, g.nz, by, bz);
+	e = hipGetLastError();
+	if (e == hipSuccess)
+		e = hipDeviceSynchronize();
+	hipFree(scales);
+	if (e != hipSuccess) {
+		hipFree(cells);
+		return fail(CRP_E_HIP, std::string("crp_field_set_grid_storage: ") + hipGetErrorString(e));
+	}
+	f->cells16 = cells;
+	return CRP_OK;
+}
+
+int crp_field_eval_ex(const crp_field_t *f, const double *pos, size_t n, double *B, int precision) {
+	if (precision == CRP_PRECISION_FP64)
+		return crp_field_eval(f, pos, n, B);
+	if (precision != CRP_PRECISION_MIXED)
+		return fail(CRP_E_INVALID, "crp_field_eval_ex: unknown precision");
+	if (!pos || !B)
+		return fail(CRP_E_INVALID, "crp_field_eval_ex: NULL argument");
+	if (n == 0)
+		return CRP_OK;
+	double *dpos = nullptr, *dB = nullptr;
+	CRP_HIP_TRY(hipMalloc(&dpos, 3 * n * sizeof(double)));
+	hipError_t e = hipMalloc(&dB, 3 * n * sizeof(double));
+	if (e == hipSuccess)
+		e = hipMemcpy(dpos, pos, 3 * n * sizeof(double), hipMemcpyHostToDevice);
+	const unsigned blocks = (unsigned)((n + 255) / 256);
+	if (e == hipSuccess) {
+		switch (f ? f->kind : FK_NONE) {
+		case FK_UNIFORM: hipLaunchKernelGGL(k_field_evalf<FieldUniform>, blocks, 256, 0, 0, f->uniform, dpos, dB, n); break;
+		case FK_GRID:
+			if (f->cells16)
+				hipLaunchKernelGGL(k_field_evalf<FieldGridCells16>, blocks, 256, 0, 0,
+				                   FieldGridCells16{f->grid, f->cells16}, dpos, dB, n);
+			else if (f->cells)
+				hipLaunchKernelGGL(k_field_evalf<FieldGridCells>, blocks, 256, 0, 0, FieldGridCells{f->grid, f->cells},
+				                   dpos, dB, n);
+			else
+				hipLaunchKernelGGL(k_field_evalf<FieldGrid>, blocks, 256, 0, 0, FieldGrid{f->grid}, dpos, dB, n);
+			break;
+		case FK_PLANEWAVE: hipLaunchKernelGGL(k_field_evalf<FieldPlaneWave>, blocks, 256, 0, 0, FieldPlaneWave{f->pw}, dpos, dB, n); break;
+		default: hipLaunchKernelGGL(k_field_evalf<FieldNone>, blocks, 256, 0, 0, FieldNone{}, dpos, dB, n); break;
+		}
+		e = hipGetLastError();
+	}
+	if (e == hipSuccess)
+		e = hipMemcpy(B, dB, 3 * n * sizeof(double), hipMemcpyDeviceToHost);
+	if (dpos)
+		hipFree(dpos);
+	if (dB)
+		hipFree(dB);
+	if (e != hipSuccess)
+		return fail(CRP_E_HIP, std::string("crp_field_eval_ex: ") + hipGetErrorString(e));
+	return CRP_OK;
+}
+
 int crp_field_get_grid_layout(const crp_field_t *f, int *layout) {
 	if (!f || !layout || f->kind != FK_GRID)
 		return fail(CRP_E_INVALID, "crp_field_get_grid_layout: not a grid field");
@@ -436,6 +604,8 @@ int crp_field_destroy(crp_field_t *f) {
 			hipFree(f->owned[i]);
 	if (f->cells)
 		hipFree(f->cells);
+	if (f->cells16)
+		hipFree(f->cells16);
 	delete f;
 	return CRP_OK;
 }
@@ -731,8 +901,11 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 	switch (f ? f->kind : FK_NONE) {
 	case FK_UNIFORM: rc = dispatch_kind(f->uniform, p, P, b, &st.launches); break;
 	case FK_GRID:
-		rc = f->cells ? dispatch_kind(FieldGridCells{f->grid, f->cells}, p, P, b, &st.launches)
-		              : dispatch_kind(FieldGrid{f->grid}, p, P, b, &st.launches);
+		if (p->precision == CRP_PRECISION_MIXED && f->cells16)
+			rc = dispatch_kind_p<FieldGridCells16, true>(FieldGridCells16{f->grid, f->cells16}, p, P, b, &st.launches);
+		else
+			rc = f->cells ? dispatch_kind(FieldGridCells{f->grid, f->cells}, p, P, b, &st.launches)
+			              : dispatch_kind(FieldGrid{f->grid}, p, P, b, &st.launches);
 		break;
 	case FK_PLANEWAVE: rc = dispatch_kind(FieldPlaneWave{f->pw}, p, P, b, &st.launches); break;
 	default: rc = dispatch_kind(FieldNone{}, p, P, b, &st.launches); break;

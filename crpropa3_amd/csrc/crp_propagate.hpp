@@ -46,13 +46,14 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 	return v;
 }
 
-// step-size factor 0.95 r^-0.2: correctly rounded on the FP64 path (the
-// reference calls glibc pow), the device libm on the mixed path
+// r^-0.2 of the step-size update (used as (step * 0.95) * r^-0.2, the
+// reference's order): correctly rounded on the FP64 path (the reference calls
+// glibc pow), the device libm on the mixed path
 template <bool MIX>
-__device__ __forceinline__ double step_factor(double r) {
+__device__ __forceinline__ double pow_m02(double r) {
 	if (MIX)
-		return 0.95 * ::pow(r, -0.2);
-	return 0.95 * crp_pow_cr(r, -0.2);
+		return ::pow(r, -0.2);
+	return crp_pow_cr(r, -0.2);
 }
 
 template <class F, int KIND, bool MIX>
@@ -208,14 +209,14 @@ __global__ void __launch_bounds__(256) k_propagate(F field, Params P, Soa S, uns
 					commit = true;
 				} else {
 					nRej++;
-					newStep = step * step_factor<MIX>(r);
+					newStep = step * 0.95 * pow_m02<MIX>(r);
 					newStep = dmax(newStep, 0.1 * step);
 					newStep = dmax(newStep, P.minStep);
 					step = newStep;
 				}
 			} else {
 				if (step != P.maxStep) {
-					newStep = step * step_factor<MIX>(r);
+					newStep = step * 0.95 * pow_m02<MIX>(r);
 					newStep = dmin(newStep, 5 * step);
 					newStep = dmin(newStep, P.maxStep);
 				}

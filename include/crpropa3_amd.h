@@ -91,6 +91,17 @@ enum crp_grid_layout { CRP_GRID_LAYOUT_POINTS = 0, CRP_GRID_LAYOUT_CELLS = 1 };
 int crp_field_set_grid_layout(crp_field_t *f, int layout);
 int crp_field_get_grid_layout(const crp_field_t *f, int *layout);
 
+/* Reduced grid storage for the mixed-precision path: FP16_BRICK adds a cell table of FP16 corner
+ * values normalised by the FP32 scale of each 8^3-point brick (64 bytes per cell: 48 B values +
+ * 4 B scale); mixed-precision propagators then read it instead of the FP32 values.  The FP64 path
+ * never uses it.  FP32 frees it.  No reference counterpart (the reference stores FP32 only). */
+enum crp_grid_storage { CRP_GRID_STORAGE_FP32 = 0, CRP_GRID_STORAGE_FP16_BRICK = 1 };
+int crp_field_set_grid_storage(crp_field_t *f, int storage);
+
+/* B at n host positions with the given precision's field evaluation (MIXED: the FP32 evaluation
+ * of the mixed path, FP16 storage if set). */
+int crp_field_eval_ex(const crp_field_t *f, const double *pos, size_t n, double *B, int precision);
+
 int crp_field_destroy(crp_field_t *f);
 
 /* B at n host positions (AoS xyz), evaluated on the device with the FP64 path. */

@@ -132,3 +132,45 @@ def test_mixed_ck_to_max_length_completes():
     assert np.all(mix["active"] == 0)
     assert st.finished == len(E)
     assert np.all(mix["trajectory_length"] >= 3 * O.Mpc)
+
+
+def test_fp16_brick_storage_field_error():
+    """FP16 corner values with a per-8^3-brick FP32 scale: field error below 2^-11 of the brick scale."""
+    sp = 50 * O.kpc
+    grid = simple_grid_turbulence(64, sp, 1 * O.nG, 2 * sp, 32 * sp, 5)
+    f32 = capi.Field.grid3f(grid, [0, 0, 0], [sp] * 3)
+    f16 = capi.Field.grid3f(grid, [0, 0, 0], [sp] * 3).set_storage("fp16")
+    pos = np.random.default_rng(2).uniform(-64 * sp, 128 * sp, size=(20000, 3))
+    B32 = f32.eval(pos, capi.PRECISION_MIXED)
+    B16 = f16.eval(pos, capi.PRECISION_MIXED)
+    Bex = f32.eval(pos)
+    scale = np.abs(grid).max()
+    err16 = np.abs(B16 - Bex).max() / scale
+    err32 = np.abs(B32 - Bex).max() / scale
+    print("field error / max|B|: FP32 mixed %.2e, FP16 bricks %.2e" % (err32, err16))
+    assert err32 < 1e-6
+    assert err16 < 2 ** -11
+
+
+def test_fp16_brick_storage_propagation():
+    n = 4096
+    sp = 50 * O.kpc
+    grid = simple_grid_turbulence(64, sp, 1 * O.nG, 2 * sp, 32 * sp, 5)
+    _, E, d = O.sample_particles(9, n, 1 * O.EeV, 100 * O.EeV)
+    pos = np.tile([32 * sp] * 3, (n, 1))
+    q = np.full(n, O.eplus)
+    f = capi.Field.grid3f(grid, [0, 0, 0], [sp] * 3)
+    ref, _ = run(f, "BP", 0.42, 0.05 * O.Mpc, 0.05 * O.Mpc, pos, O.normalise(d), E, q, 1000, capi.PRECISION_FP64)
+    f.set_storage("fp16")
+    for steps in (10, 1000):
+        r, _ = run(f, "BP", 0.42, 0.05 * O.Mpc, 0.05 * O.Mpc, pos, O.normalise(d), E, q, steps, capi.PRECISION_FP64)
+        m, _ = run(f, "BP", 0.42, 0.05 * O.Mpc, 0.05 * O.Mpc, pos, O.normalise(d), E, q, steps, capi.PRECISION_MIXED)
+        e = rel_err(m["pos"], r["pos"])
+        print("FP16-brick BP %d steps: relative position error %s" % (steps, pct(e)))
+        if steps == 10:
+            assert np.median(e) < 1e-5
+    # the FP64 path ignores the FP16 table: still the exact result
+    assert np.array_equal(r["pos"].view(np.uint64), ref["pos"].view(np.uint64))
+    dr = np.linalg.norm(ref["pos"] - pos, axis=1)
+    dm = np.linalg.norm(m["pos"] - pos, axis=1)
+    assert _ks_equivalent(dm, dr, "FP16 displacement") > 1e-3
