@@ -25,6 +25,7 @@
 
 #include "crp_device.hpp"
 #include "crp_propagate.hpp"
+#include "crp_adjoint.hpp"
 #include "crpropa3_amd.h"
 
 using namespace crp;
@@ -925,6 +926,90 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 	st.kernel_ms = ms;
 	if (stats)
 		*stats = st;
+	return CRP_OK;
+}
+
+static int adj_args(const crp_adjoint_config *cfg, size_t n, AdjArgs &A) {
+	if (!cfg || !cfg->grid)
+		return fail(CRP_E_INVALID, "crp_adjoint: NULL config or grid");
+	if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1 || cfg->nsteps < 0 || cfg->checkpoint_every < 1)
+		return fail(CRP_E_INVALID, "crp_adjoint: bad grid size, step count or checkpoint interval");
+	for (int a = 0; a < 3; a++)
+		if (!(cfg->spacing[a] > 0))
+			return fail(CRP_E_INVALID, "crp_adjoint: spacing must be > 0");
+	std::memset(&A, 0, sizeof(A));
+	A.G.g = cfg->grid;
+	A.G.nx = cfg->nx;
+	A.G.ny = cfg->ny;
+	A.G.nz = cfg->nz;
+	for (int a = 0; a < 3; a++) {
+		A.G.gridOrigin[a] = cfg->origin[a] + cfg->spacing[a] / 2;
+		A.G.invSpacing[a] = 1.0 / cfg->spacing[a];
+	}
+	A.G.scale = cfg->scale;
+	A.h = cfg->step;
+	A.nsteps = cfg->nsteps;
+	A.K = cfg->checkpoint_every;
+	A.n = n;
+	return CRP_OK;
+}
+
+int crp_adjoint_workspace(const crp_adjoint_config *cfg, size_t n, size_t *checkpoint_bytes, size_t *scratch_bytes) {
+	AdjArgs A;
+	int rc = adj_args(cfg, n, A);
+	if (rc != CRP_OK)
+		return rc;
+	const size_t nseg = ((size_t)cfg->nsteps + cfg->checkpoint_every - 1) / cfg->checkpoint_every;
+	if (checkpoint_bytes)
+		*checkpoint_bytes = (nseg > 0 ? nseg : 1) * 6 * n * sizeof(double);
+	if (scratch_bytes)
+		*scratch_bytes = (size_t)cfg->checkpoint_every * 6 * n * sizeof(double);
+	return CRP_OK;
+}
+
+int crp_adjoint_forward(const crp_adjoint_config *cfg, size_t n, const double *x0, const double *u0,
+                        const double *kq, double *checkpoints, double *x_out, double *u_out, void *stream) {
+	AdjArgs A;
+	int rc = adj_args(cfg, n, A);
+	if (rc != CRP_OK)
+		return rc;
+	if (n == 0)
+		return CRP_OK;
+	if (!x0 || !u0 || !kq || !checkpoints || !x_out || !u_out)
+		return fail(CRP_E_INVALID, "crp_adjoint_forward: NULL pointer");
+	A.x0 = x0;
+	A.u0 = u0;
+	A.kq = kq;
+	A.ckpt = checkpoints;
+	A.xout = x_out;
+	A.uout = u_out;
+	hipLaunchKernelGGL(k_adjoint_forward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, A);
+	CRP_HIP_TRY(hipGetLastError());
+	return CRP_OK;
+}
+
+int crp_adjoint_backward(const crp_adjoint_config *cfg, size_t n, const double *kq, const double *checkpoints,
+                         const double *grad_x_out, const double *grad_u_out, double *scratch, double *grad_grid,
+                         double *grad_x0, double *grad_u0, double *grad_scale, void *stream) {
+	AdjArgs A;
+	int rc = adj_args(cfg, n, A);
+	if (rc != CRP_OK)
+		return rc;
+	if (n == 0)
+		return CRP_OK;
+	if (!kq || !checkpoints || !grad_u_out || !scratch || !grad_grid || !grad_x0 || !grad_u0 || !grad_scale)
+		return fail(CRP_E_INVALID, "crp_adjoint_backward: NULL pointer");
+	A.kq = kq;
+	A.ckpt = const_cast<double *>(checkpoints);
+	A.gxout = grad_x_out;
+	A.guout = grad_u_out;
+	A.scratch = scratch;
+	A.gradG = grad_grid;
+	A.gradX0 = grad_x0;
+	A.gradU0 = grad_u0;
+	A.gradScale = grad_scale;
+	hipLaunchKernelGGL(k_adjoint_backward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, A);
+	CRP_HIP_TRY(hipGetLastError());
 	return CRP_OK;
 }
 

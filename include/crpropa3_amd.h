@@ -157,6 +157,33 @@ int crp_batch_copy(crp_batch_t *dst, const crp_batch_t *src);
  * inactive, which needs a trajectory-length limit and minStep > 0).  Synchronous. */
 int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps, crp_run_stats *stats);
 
+/* ---- adjoint of the fixed-step Boris push through a periodic Grid3f -------
+ * No reference counterpart: the reference has no derivatives.  The forward model is
+ * PropagationBP::dY (src/module/PropagationBP.cpp:19-39) with both direction normalisations,
+ * B = scale * trilinear(grid) evaluated in FP64 (a differentiable model of Grid.h:540-582).
+ * All pointers are device pointers; work is queued on `stream` (a hipStream_t, NULL = default)
+ * and the calls do not synchronise.  Positions/directions are AoS (n,3) FP64; kq[i] = q c h / (2 E). */
+typedef struct {
+	const float *grid;  /* device, (ix*ny + iy)*nz + iz, 3 floats per point */
+	int nx, ny, nz;
+	double origin[3], spacing[3];
+	double scale;       /* field = scale * grid */
+	double step;        /* h */
+	int nsteps;
+	int checkpoint_every; /* K */
+} crp_adjoint_config;
+
+/* Device workspace sizes: checkpoints ((ceil(nsteps/K)) * 6 * n doubles) and scratch (K * 6 * n). */
+int crp_adjoint_workspace(const crp_adjoint_config *cfg, size_t n, size_t *checkpoint_bytes, size_t *scratch_bytes);
+/* Forward: final positions/directions and checkpoints every K steps. */
+int crp_adjoint_forward(const crp_adjoint_config *cfg, size_t n, const double *x0, const double *u0,
+                        const double *kq, double *checkpoints, double *x_out, double *u_out, void *stream);
+/* Backward: given dL/dx_out (may be NULL) and dL/du_out, ADD dL/dgrid (3*nx*ny*nz doubles) and
+ * dL/dscale (one double) into the outputs, and WRITE dL/dx0, dL/du0. */
+int crp_adjoint_backward(const crp_adjoint_config *cfg, size_t n, const double *kq, const double *checkpoints,
+                         const double *grad_x_out, const double *grad_u_out, double *scratch, double *grad_grid,
+                         double *grad_x0, double *grad_u0, double *grad_scale, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

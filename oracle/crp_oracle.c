@@ -545,3 +545,180 @@ long long orc_propagate(const orc_field *f, const orc_propagator *p, orc_state *
 		*trials = rej;
 	return total;
 }
+
+/* ------------------------------------------------------------------ differentiable Boris model */
+typedef struct {
+	const float *g;
+	int n[3];
+	double go[3], sp[3], scale;
+} orc_agrid;
+
+/* corner c of the cell containing p: flat point index, weight, d(weight)/d(p) */
+static void orc_acell(const orc_agrid *G, const double p[3], size_t idx[8], double w[8], double dw[8][3]) {
+	int lo[3], hi[3];
+	double f[3];
+	for (int a = 0; a < 3; a++) {
+		double r = (p[a] - G->go[a]) / G->sp[a];
+		double fl = floor(r);
+		f[a] = r - fl;
+		long long i = (long long)fl % G->n[a];
+		if (i < 0)
+			i += G->n[a];
+		lo[a] = (int)i;
+		hi[a] = (int)((i + 1) % G->n[a]);
+	}
+	for (int c = 0; c < 8; c++) {
+		/* corner order of Grid.h:572-579 */
+		static const int bx[8] = {0, 1, 0, 0, 1, 0, 1, 1}, by[8] = {0, 0, 1, 0, 0, 1, 1, 1}, bz[8] = {0, 0, 0, 1, 1, 1, 0, 1};
+		int b[3] = {bx[c], by[c], bz[c]};
+		double ww[3], sg[3];
+		for (int a = 0; a < 3; a++) {
+			ww[a] = b[a] ? f[a] : 1.0 - f[a];
+			sg[a] = b[a] ? 1.0 : -1.0;
+		}
+		w[c] = ww[0] * ww[1] * ww[2];
+		dw[c][0] = sg[0] * ww[1] * ww[2] / G->sp[0];
+		dw[c][1] = ww[0] * sg[1] * ww[2] / G->sp[1];
+		dw[c][2] = ww[0] * ww[1] * sg[2] / G->sp[2];
+		int X = b[0] ? hi[0] : lo[0], Yy = b[1] ? hi[1] : lo[1], Z = b[2] ? hi[2] : lo[2];
+		idx[c] = ((size_t)X * G->n[1] + Yy) * G->n[2] + Z;
+	}
+}
+
+static void orc_cross3(const double a[3], const double b[3], double o[3]) {
+	o[0] = a[1] * b[2] - a[2] * b[1];
+	o[1] = a[2] * b[0] - a[0] * b[2];
+	o[2] = a[0] * b[1] - a[1] * b[0];
+}
+static double orc_dot3(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+typedef struct {
+	double x1[3], T[3], t[3], D, s2[3], v[3], up[3], r1, n1[3], r2, un[3];
+	size_t idx[8];
+	double w[8], dw[8][3];
+} orc_astep;
+
+static void orc_astep_fwd(const orc_agrid *G, double h, double kq, const double x[3], const double u[3], orc_astep *S,
+                          double xo[3], double uo[3]) {
+	for (int a = 0; a < 3; a++)
+		S->x1[a] = x[a] + u[a] * (h / 2);
+	orc_acell(G, S->x1, S->idx, S->w, S->dw);
+	for (int a = 0; a < 3; a++) {
+		double s = 0;
+		for (int c = 0; c < 8; c++)
+			s += S->w[c] * (double)G->g[3 * S->idx[c] + a];
+		S->T[a] = s;
+		S->t[a] = s * G->scale * kq;
+	}
+	S->D = 1 + orc_dot3(S->t, S->t);
+	double ut[3], vs[3];
+	for (int a = 0; a < 3; a++)
+		S->s2[a] = 2 * S->t[a] / S->D;
+	orc_cross3(u, S->t, ut);
+	for (int a = 0; a < 3; a++)
+		S->v[a] = u[a] + ut[a];
+	orc_cross3(S->v, S->s2, vs);
+	for (int a = 0; a < 3; a++)
+		S->up[a] = u[a] + vs[a];
+	S->r1 = sqrt(orc_dot3(S->up, S->up));
+	for (int a = 0; a < 3; a++)
+		S->n1[a] = S->up[a] / S->r1;
+	S->r2 = sqrt(orc_dot3(S->n1, S->n1));
+	for (int a = 0; a < 3; a++) {
+		S->un[a] = S->n1[a] / S->r2;
+		xo[a] = S->x1[a] + S->up[a] * (h / 2);
+		uo[a] = S->un[a];
+	}
+}
+
+void orc_adjoint_forward(const float *grid, int nx, int ny, int nz, const double origin[3], const double spacing[3],
+                         double scale, double h, int nsteps, size_t n, const double *x0, const double *u0,
+                         const double *kq, double *x_out, double *u_out) {
+	orc_agrid G = {grid, {nx, ny, nz}, {0, 0, 0}, {spacing[0], spacing[1], spacing[2]}, scale};
+	for (int a = 0; a < 3; a++)
+		G.go[a] = origin[a] + spacing[a] / 2;
+	for (size_t i = 0; i < n; i++) {
+		double x[3] = {x0[3 * i], x0[3 * i + 1], x0[3 * i + 2]}, u[3] = {u0[3 * i], u0[3 * i + 1], u0[3 * i + 2]};
+		orc_astep S;
+		for (int s = 0; s < nsteps; s++)
+			orc_astep_fwd(&G, h, kq[i], x, u, &S, x, u);
+		for (int a = 0; a < 3; a++) {
+			x_out[3 * i + a] = x[a];
+			u_out[3 * i + a] = u[a];
+		}
+	}
+}
+
+void orc_adjoint_backward(const float *grid, int nx, int ny, int nz, const double origin[3], const double spacing[3],
+                          double scale, double h, int nsteps, size_t n, const double *x0, const double *u0,
+                          const double *kq, const double *gx_out, const double *gu_out, double *grad_grid,
+                          double *grad_x0, double *grad_u0, double *grad_scale) {
+	orc_agrid G = {grid, {nx, ny, nz}, {0, 0, 0}, {spacing[0], spacing[1], spacing[2]}, scale};
+	for (int a = 0; a < 3; a++)
+		G.go[a] = origin[a] + spacing[a] / 2;
+	double *tape = (double *)malloc(sizeof(double) * 6 * ((size_t)nsteps + 1));
+	double gs = 0;
+	for (size_t i = 0; i < n; i++) {
+		double x[3] = {x0[3 * i], x0[3 * i + 1], x0[3 * i + 2]}, u[3] = {u0[3 * i], u0[3 * i + 1], u0[3 * i + 2]};
+		orc_astep S;
+		for (int s = 0; s < nsteps; s++) {
+			memcpy(tape + 6 * s, x, sizeof x);
+			memcpy(tape + 6 * s + 3, u, sizeof u);
+			orc_astep_fwd(&G, h, kq[i], x, u, &S, x, u);
+		}
+		double ax[3], au[3];
+		for (int a = 0; a < 3; a++) {
+			ax[a] = gx_out ? gx_out[3 * i + a] : 0.0;
+			au[a] = gu_out[3 * i + a];
+		}
+		for (int s = nsteps - 1; s >= 0; s--) {
+			double xo[3], uo[3];
+			orc_astep_fwd(&G, h, kq[i], tape + 6 * s, tape + 6 * s + 3, &S, xo, uo);
+			const double *us = tape + 6 * s + 3;
+			/* u_next = n1/r2: adjoint of a/|a| is (g - n (n.g))/|a| */
+			double an1[3], aup[3], c;
+			c = orc_dot3(S.un, au);
+			for (int a = 0; a < 3; a++)
+				an1[a] = (au[a] - S.un[a] * c) / S.r2;
+			c = orc_dot3(S.n1, an1);
+			for (int a = 0; a < 3; a++)
+				aup[a] = (an1[a] - S.n1[a] * c) / S.r1 + ax[a] * (h / 2);
+			/* u' = u + v x s2 ; v = u + u x t ; s2 = 2 t / D */
+			double av[3], as2[3], tav[3], at[3], aU[3];
+			orc_cross3(S.s2, aup, av);
+			orc_cross3(aup, S.v, as2);
+			orc_cross3(S.t, av, tav);
+			for (int a = 0; a < 3; a++)
+				aU[a] = aup[a] + av[a] + tav[a];
+			orc_cross3(av, us, at);
+			double ts2 = orc_dot3(S.t, as2);
+			for (int a = 0; a < 3; a++)
+				at[a] += 2.0 / S.D * as2[a] - 4.0 / (S.D * S.D) * ts2 * S.t[a];
+			/* t = kq * scale * T */
+			double aT[3];
+			for (int a = 0; a < 3; a++) {
+				aT[a] = at[a] * kq[i] * scale;
+				gs += S.T[a] * at[a] * kq[i];
+			}
+			double ax1[3] = {ax[0], ax[1], ax[2]};
+			for (int cc = 0; cc < 8; cc++) {
+				const float *gv = grid + 3 * S.idx[cc];
+				double d = (double)gv[0] * aT[0] + (double)gv[1] * aT[1] + (double)gv[2] * aT[2];
+				for (int a = 0; a < 3; a++) {
+					ax1[a] += S.dw[cc][a] * d;
+					grad_grid[3 * S.idx[cc] + a] += S.w[cc] * aT[a];
+				}
+			}
+			for (int a = 0; a < 3; a++) {
+				ax[a] = ax1[a];
+				au[a] = aU[a] + ax1[a] * (h / 2);
+			}
+		}
+		for (int a = 0; a < 3; a++) {
+			grad_x0[3 * i + a] = ax[a];
+			grad_u0[3 * i + a] = au[a];
+		}
+	}
+	*grad_scale += gs;
+	free(tape);
+}

@@ -89,6 +89,19 @@ void orc_planewave_init(double brms, double lmin, double lmax, double lbendover,
 long long orc_propagate(const orc_field *f, const orc_propagator *p, orc_state *s, size_t n, long long maxSteps,
                         int libm, int nthreads, long long *trials);
 
+/* FP64 reference gradient of the differentiable fixed-step Boris model (the
+ * model of the GPU adjoint: PropagationBP::dY with B = scale * FP64 trilinear of
+ * a periodic float32 grid, both normalisations).  Stores every state (no
+ * checkpoints), serial.  x0/u0/x_out/u_out/g* are AoS (n,3); kq[i] = q c h/(2E).
+ * grad_grid (3*nx*ny*nz) and *grad_scale are accumulated. */
+void orc_adjoint_forward(const float *grid, int nx, int ny, int nz, const double origin[3], const double spacing[3],
+                         double scale, double h, int nsteps, size_t n, const double *x0, const double *u0,
+                         const double *kq, double *x_out, double *u_out);
+void orc_adjoint_backward(const float *grid, int nx, int ny, int nz, const double origin[3], const double spacing[3],
+                          double scale, double h, int nsteps, size_t n, const double *x0, const double *u0,
+                          const double *kq, const double *gx_out, const double *gu_out, double *grad_grid,
+                          double *grad_x0, double *grad_u0, double *grad_scale);
+
 /* elementary functions, for the math tests */
 double orc_cos(double x, int libm);
 double orc_pow(double x, double y, int libm);

@@ -82,6 +82,10 @@ def lib():
         L.orc_rng_randint_batch.argtypes = [C.c_uint32, C.c_size_t, _P(C.c_uint32)]
         L.orc_sample_particles.argtypes = [C.c_uint32, C.c_size_t, C.c_double, C.c_double, _dp, C.c_size_t,
                                            _ip, _dp, _dp]
+        adj = [_fp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_int, C.c_size_t, _dp, _dp,
+               _dp]
+        L.orc_adjoint_forward.argtypes = adj + [_dp, _dp]
+        L.orc_adjoint_backward.argtypes = adj + [_dp] * 6
         _lib = L
     return _lib
 
@@ -227,6 +231,44 @@ def propagate(field, prop_type, state, steps, tolerance=0.42, min_step=0.0, max_
     f = C.byref(field.s) if field is not None else None
     n = lib().orc_propagate(f, C.byref(p), C.byref(s), state.n, steps, libm, nthreads, C.byref(trials))
     return n, trials.value
+
+
+def _adjoint_args(grid, origin, spacing, scale, h, nsteps, x0, u0, kq):
+    grid = np.ascontiguousarray(grid, dtype=np.float32)
+    nx, ny, nz = grid.shape[:3]
+    o = np.ascontiguousarray(origin, dtype=np.float64)
+    sp = np.ascontiguousarray(spacing, dtype=np.float64)
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    u0 = np.ascontiguousarray(u0, dtype=np.float64)
+    kq = np.ascontiguousarray(kq, dtype=np.float64)
+    keep = (grid, o, sp, x0, u0, kq)
+    args = [ptr(grid, _fp), nx, ny, nz, ptr(o), ptr(sp), float(scale), float(h), int(nsteps), len(kq), ptr(x0),
+            ptr(u0), ptr(kq)]
+    return keep, args
+
+
+def adjoint_forward(grid, origin, spacing, scale, h, nsteps, x0, u0, kq):
+    """FP64 fixed-step Boris model through scale * trilinear(grid): (x_final, u_final)."""
+    keep, args = _adjoint_args(grid, origin, spacing, scale, h, nsteps, x0, u0, kq)
+    n = len(keep[5])
+    xo = np.zeros((n, 3))
+    uo = np.zeros((n, 3))
+    lib().orc_adjoint_forward(*args, ptr(xo), ptr(uo))
+    return xo, uo
+
+
+def adjoint_backward(grid, origin, spacing, scale, h, nsteps, x0, u0, kq, gx_out, gu_out):
+    """Reverse-mode gradient of the same model: (grad_grid, grad_x0, grad_u0, grad_scale)."""
+    keep, args = _adjoint_args(grid, origin, spacing, scale, h, nsteps, x0, u0, kq)
+    n = len(keep[5])
+    gxo = None if gx_out is None else np.ascontiguousarray(gx_out, dtype=np.float64)
+    guo = np.ascontiguousarray(gu_out, dtype=np.float64)
+    gg = np.zeros(keep[0].shape)
+    gx0 = np.zeros((n, 3))
+    gu0 = np.zeros((n, 3))
+    gs = np.zeros(1)
+    lib().orc_adjoint_backward(*args, ptr(gxo), ptr(guo), ptr(gg), ptr(gx0), ptr(gu0), ptr(gs))
+    return gg, gx0, gu0, gs[0]
 
 
 def cos_batch(x, libm):
