@@ -37,6 +37,15 @@ class HostState(C.Structure):
                 ("prev_pos", _dp), ("prev_dir", _dp)]
 
 
+class LaunchConfig(C.Structure):
+    """crp_launch_config: block size, workgroups per CU, steps per launch, hipGraph replay."""
+    _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("steps_per_launch", C.c_longlong),
+                ("use_graph", C.c_int), ("particles_per_lane", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class CrpError(RuntimeError):
     """Error reported by the C ABI (the reference raises std::runtime_error)."""
 
@@ -70,6 +79,9 @@ def lib():
         L.crp_propagator_set_max_trajectory_length.argtypes = [_vp, C.c_double]
         L.crp_propagator_set_precision.argtypes = [_vp, C.c_int]
         L.crp_propagator_destroy.argtypes = [_vp]
+        L.crp_propagator_set_launch.argtypes = [_vp, C.POINTER(LaunchConfig)]
+        L.crp_propagator_get_launch.argtypes = [_vp, C.POINTER(LaunchConfig)]
+        L.crp_propagator_autotune.argtypes = [_vp, _vp, C.c_longlong, C.POINTER(LaunchConfig)]
         L.crp_batch_create.argtypes = [C.c_size_t, C.POINTER(_vp)]
         L.crp_batch_destroy.argtypes = [_vp]
         L.crp_batch_size.argtypes = [_vp, C.POINTER(C.c_size_t)]
@@ -203,6 +215,22 @@ class Propagator:
         st = RunStats()
         check(lib().crp_propagate(self.handle, batch.handle, max_steps, C.byref(st)))
         return st
+
+    def set_launch(self, block=256, blocks_per_cu=0, steps_per_launch=0, use_graph=False):
+        cfg = LaunchConfig(block, blocks_per_cu, steps_per_launch, 1 if use_graph else 0, 0.0)
+        check(lib().crp_propagator_set_launch(self.handle, C.byref(cfg)))
+        return self
+
+    def launch(self):
+        cfg = LaunchConfig()
+        check(lib().crp_propagator_get_launch(self.handle, C.byref(cfg)))
+        return cfg.as_dict()
+
+    def autotune(self, batch, probe_steps=32):
+        """Time short runs over launch configurations on a copy of `batch`; keep and return the fastest."""
+        cfg = LaunchConfig()
+        check(lib().crp_propagator_autotune(self.handle, batch.handle, probe_steps, C.byref(cfg)))
+        return cfg.as_dict()
 
 
 class Batch:

@@ -51,6 +51,7 @@ __device__ __forceinline__ void adj_cell(const AdjGrid &G, const V3 &p, TriPoint
 		double w = fl - (double)n[a] * __builtin_floor(fl / n[a]);
 		int i = (int)w;
 		i = i >= n[a] ? i - n[a] : (i < 0 ? i + n[a] : i);
+		i = min(max(i, 0), n[a] - 1); // rounding of w at |r| beyond 2^52 cells stays in the grid
 		lo[a] = i;
 		hi[a] = (i + 1) % n[a];
 	}

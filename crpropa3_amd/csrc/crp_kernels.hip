@@ -65,6 +65,12 @@ struct crp_propagator {
 	double tolerance, minStep, maxStep;
 	double maxLength;
 	int precision;
+	// launch configuration (crp_launch_config) and the geometry of the last call
+	int block, blocksPerCU;
+	long long stepsPerLaunch;
+	int useGraph;
+	mutable unsigned lastGrid, lastBlock;
+	mutable double lastPPL;
 };
 
 
@@ -79,6 +85,11 @@ struct crp_batch {
 	uint32_t *order; // queue order: particle indices, longest expected work first
 	unsigned long long *ctr;
 	hipEvent_t ev0, ev1;
+	// captured launch sequence of the last graph-enabled call, keyed by the exact bytes of every
+	// kernel argument and launch dimension it holds
+	hipGraphExec_t gexec;
+	size_t gkeyLen;
+	unsigned char gkey[1024];
 };
 
 
@@ -237,26 +248,147 @@ __global__ void k_field_eval(F field, const double *__restrict__ pos, double *__
 
 
 // ------------------------------------------------------------------ launch helpers
-template <class F, int KIND, bool MIX>
-static int launch_propagate(const F &f, const Params &P, crp_batch *b, int *launches) {
+static int num_cus() {
 	static int numCU = -1;
 	if (numCU < 0) {
-		int dev;
+		int dev, v = 0;
 		CRP_HIP_TRY(hipGetDevice(&dev));
-		CRP_HIP_TRY(hipDeviceGetAttribute(&numCU, hipDeviceAttributeMultiprocessorCount, dev));
+		CRP_HIP_TRY(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev));
+		numCU = v;
 	}
-	int bpc = 0;
-	CRP_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_propagate<F, KIND, MIX>, 256, 0));
-	if (bpc < 1)
-		bpc = 1;
-	size_t want = (b->n + 255) / 256;
-	size_t grid = std::min<size_t>(want, (size_t)numCU * bpc);
-	if (grid < 1)
-		grid = 1;
-	hipLaunchKernelGGL((k_propagate<F, KIND, MIX>), dim3((unsigned)grid), dim3(256), 0, b->stream, f, P, soa_of(b),
-	                   (unsigned long long)b->n, b->ctr);
+	return numCU;
+}
+
+// Append bytes to a graph key; false when the key buffer is full (then no graph is used).
+struct KeyBuf {
+	unsigned char *p;
+	size_t len, cap;
+	bool ok;
+	template <class T> void put(const T &v) {
+		if (!ok || len + sizeof(T) > cap) {
+			ok = false;
+			return;
+		}
+		std::memcpy(p + len, &v, sizeof(T));
+		len += sizeof(T);
+	}
+};
+
+// One crp_propagate call: [zero counters] then launches of k_propagate.  With
+// steps_per_launch S and a step budget M >= 0 the call is ceil(M/S) launches of
+// at most S steps (the queue counter is reset between them); with M < 0 it is
+// repeated S-step launches until a launch leaves no particle paused.
+template <class F, int KIND, bool MIX>
+static int launch_propagate(const F &f, const crp_propagator *p, const Params &P0, crp_batch *b, int *launches) {
+	auto kern = k_propagate<F, KIND, MIX>;
+	const int ncu = num_cus();
+	if (ncu < 0)
+		return CRP_E_HIP;
+	const int block = (p->block == 64 || p->block == 128) ? p->block : 256;
+	int bpcMax = 0;
+	CRP_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpcMax, kern, block, 0));
+	if (bpcMax < 1)
+		bpcMax = 1;
+	const int bpc = p->blocksPerCU > 0 ? std::min(p->blocksPerCU, bpcMax) : bpcMax;
+	const size_t want = (b->n + block - 1) / block;
+	const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>(want, (size_t)ncu * bpc));
+	p->lastGrid = grid;
+	p->lastBlock = (unsigned)block;
+	p->lastPPL = (double)b->n / ((double)grid * block);
+	const Soa soa = soa_of(b);
+	const unsigned long long n = b->n;
+	const long long M = P0.maxSteps, S = p->stepsPerLaunch;
+	const bool chunked = S > 0 && (M < 0 || M > S);
+	const bool open = chunked && M < 0; // to completion, S steps per launch
+	const long long nchunks = !chunked ? 1 : (open ? 1 : (M + S - 1) / S);
+
+	// the launch sequence, enqueued on stream st
+	auto enqueue = [&](hipStream_t st, bool zeroAll) -> int {
+		if (zeroAll)
+			hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, b->ctr, (int)CTR_COUNT);
+		for (long long j = 0; j < nchunks; j++) {
+			Params P = P0;
+			if (chunked)
+				P.maxSteps = open ? S : std::min(S, M - j * S);
+			if (j > 0 || !zeroAll)
+				hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, b->ctr, 1); // queue counter only
+			hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, f, P, soa, n, b->ctr);
+		}
+		CRP_HIP_TRY(hipGetLastError());
+		return CRP_OK;
+	};
+
+	// graph key: everything the captured sequence holds
+	unsigned char key[sizeof(b->gkey)];
+	KeyBuf kb{key, 0, sizeof(key), p->useGraph != 0};
+	kb.put((const void *)kern);
+	kb.put(f);
+	kb.put(P0);
+	kb.put(soa);
+	kb.put(n);
+	kb.put((const void *)b->ctr);
+	kb.put(grid);
+	kb.put(block);
+	kb.put(S);
+	kb.put(open);
+	const bool useGraph = kb.ok;
+
+	auto run_sequence = [&](bool zeroAll) -> int {
+		if (!useGraph)
+			return enqueue(b->stream, zeroAll);
+		KeyBuf k2 = kb;
+		k2.put(zeroAll);
+		if (!k2.ok)
+			return enqueue(b->stream, zeroAll);
+		if (!(b->gexec && b->gkeyLen == k2.len && std::memcmp(b->gkey, key, k2.len) == 0)) {
+			if (b->gexec) {
+				CRP_HIP_TRY(hipGraphExecDestroy(b->gexec));
+				b->gexec = nullptr;
+			}
+			hipGraph_t g = nullptr;
+			CRP_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+			int rc = enqueue(b->stream, zeroAll);
+			hipError_t e = hipStreamEndCapture(b->stream, &g);
+			if (rc != CRP_OK)
+				return rc;
+			if (e != hipSuccess)
+				return fail(CRP_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+			e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+			(void)hipGraphDestroy(g);
+			if (e != hipSuccess) {
+				b->gexec = nullptr;
+				return fail(CRP_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+			}
+			std::memcpy(b->gkey, key, k2.len);
+			b->gkeyLen = k2.len;
+		}
+		CRP_HIP_TRY(hipGraphLaunch(b->gexec, b->stream));
+		return CRP_OK;
+	};
+
+	if (!open) {
+		int rc = run_sequence(true);
+		if (rc != CRP_OK)
+			return rc;
+		*launches += (int)nchunks;
+		return CRP_OK;
+	}
+	// open-ended chunked run: counters zeroed once, then one S-step launch at a time
+	hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, b->stream, b->ctr, (int)CTR_COUNT);
 	CRP_HIP_TRY(hipGetLastError());
-	(*launches)++;
+	unsigned long long pausedBefore = 0;
+	for (;;) {
+		int rc = run_sequence(false);
+		if (rc != CRP_OK)
+			return rc;
+		(*launches)++;
+		unsigned long long paused = 0;
+		CRP_HIP_TRY(hipMemcpyAsync(&paused, b->ctr + CTR_PAUSED, sizeof(paused), hipMemcpyDeviceToHost, b->stream));
+		CRP_HIP_TRY(hipStreamSynchronize(b->stream));
+		if (paused == pausedBefore)
+			break;
+		pausedBefore = paused;
+	}
 	return CRP_OK;
 }
 
@@ -264,10 +396,10 @@ template <class F, bool MIX>
 static int dispatch_kind_p(const F &f, const crp_propagator *p, const Params &P, crp_batch *b, int *launches) {
 	const bool adaptive = !(p->minStep == p->maxStep);
 	if (p->type == 0)
-		return adaptive ? launch_propagate<F, PK_BP_ADAPTIVE, MIX>(f, P, b, launches)
-		                : launch_propagate<F, PK_BP_FIXED, MIX>(f, P, b, launches);
-	return adaptive ? launch_propagate<F, PK_CK_ADAPTIVE, MIX>(f, P, b, launches)
-	                : launch_propagate<F, PK_CK_FIXED, MIX>(f, P, b, launches);
+		return adaptive ? launch_propagate<F, PK_BP_ADAPTIVE, MIX>(f, p, P, b, launches)
+		                : launch_propagate<F, PK_BP_FIXED, MIX>(f, p, P, b, launches);
+	return adaptive ? launch_propagate<F, PK_CK_ADAPTIVE, MIX>(f, p, P, b, launches)
+	                : launch_propagate<F, PK_CK_FIXED, MIX>(f, p, P, b, launches);
 }
 
 template <class F>
@@ -307,7 +439,7 @@ static crp_field *new_field(int kind) {
 		return nullptr;
 	std::memset(f, 0, sizeof(*f));
 	f->kind = kind;
-	hipGetDevice(&f->device);
+	(void)hipGetDevice(&f->device);
 	return f;
 }
 
@@ -364,11 +496,11 @@ static int build_cells(crp_field *f) {
 	if (e == hipSuccess)
 		e = hipDeviceSynchronize();
 	if (e != hipSuccess) {
-		hipFree(cells);
+		(void)hipFree(cells);
 		return fail(CRP_E_HIP, std::string("crp_field: building the cell table: ") + hipGetErrorString(e));
 	}
 	if (f->cells)
-		hipFree(f->cells);
+		(void)hipFree(f->cells);
 	f->cells = cells;
 	return CRP_OK;
 }
@@ -404,7 +536,7 @@ int crp_field_create_grid3f(const float *values, int nx, int ny, int nz, const d
 	}
 	e = hipMemcpy(d, values, bytes, hipMemcpyHostToDevice);
 	if (e != hipSuccess) {
-		hipFree(d);
+		(void)hipFree(d);
 		delete f;
 		return fail(CRP_E_HIP, std::string("crp_field_create_grid3f: hipMemcpy: ") + hipGetErrorString(e));
 	}
@@ -473,7 +605,7 @@ int crp_field_set_grid_storage(crp_field_t *f, int storage) {
 		e = hipMalloc((void **)&cells, ncell * kCell16Bytes);
 	if (e != hipSuccess) {
 		if (scales)
-			hipFree(scales);
+			(void)hipFree(scales);
 		return fail(CRP_E_NOMEM, std::string("crp_field_set_grid_storage: hipMalloc: ") + hipGetErrorString(e));
 	}
 	hipLaunchKernelGGL(k_brick_scales, dim3((unsigned)nb), dim3(256), 0, 0, g.values, scales, g.nx, g.ny, g.nz, bx, by, bz);
@@ -482,9 +614,9 @@ int crp_field_set_grid_storage(crp_field_t *f, int storage) {
 	e = hipGetLastError();
 	if (e == hipSuccess)
 		e = hipDeviceSynchronize();
-	hipFree(scales);
+	(void)hipFree(scales);
 	if (e != hipSuccess) {
-		hipFree(cells);
+		(void)hipFree(cells);
 		return fail(CRP_E_HIP, std::string("crp_field_set_grid_storage: ") + hipGetErrorString(e));
 	}
 	f->cells16 = cells;
@@ -527,9 +659,9 @@ int crp_field_eval_ex(const crp_field_t *f, const double *pos, size_t n, double 
 	if (e == hipSuccess)
 		e = hipMemcpy(B, dB, 3 * n * sizeof(double), hipMemcpyDeviceToHost);
 	if (dpos)
-		hipFree(dpos);
+		(void)hipFree(dpos);
 	if (dB)
-		hipFree(dB);
+		(void)hipFree(dB);
 	if (e != hipSuccess)
 		return fail(CRP_E_HIP, std::string("crp_field_eval_ex: ") + hipGetErrorString(e));
 	return CRP_OK;
@@ -577,9 +709,9 @@ int crp_field_create_planewave(int nm, const double *k, const double *Ak, const 
 			e = hipMemcpy(d, src[j], sz[j], hipMemcpyHostToDevice);
 		if (e != hipSuccess) {
 			if (d)
-				hipFree(d);
+				(void)hipFree(d);
 			for (int i = 0; i < j; i++)
-				hipFree(f->owned[i]);
+				(void)hipFree(f->owned[i]);
 			delete f;
 			return fail(CRP_E_HIP, std::string("crp_field_create_planewave: ") + hipGetErrorString(e));
 		}
@@ -602,11 +734,11 @@ int crp_field_destroy(crp_field_t *f) {
 		return CRP_OK;
 	for (int i = 0; i < 8; i++)
 		if (f->owned[i])
-			hipFree(f->owned[i]);
+			(void)hipFree(f->owned[i]);
 	if (f->cells)
-		hipFree(f->cells);
+		(void)hipFree(f->cells);
 	if (f->cells16)
-		hipFree(f->cells16);
+		(void)hipFree(f->cells16);
 	delete f;
 	return CRP_OK;
 }
@@ -620,7 +752,7 @@ int crp_field_eval(const crp_field_t *f, const double *pos, size_t n, double *B)
 	CRP_HIP_TRY(hipMalloc(&dpos, 3 * n * sizeof(double)));
 	hipError_t e = hipMalloc(&dB, 3 * n * sizeof(double));
 	if (e != hipSuccess) {
-		hipFree(dpos);
+		(void)hipFree(dpos);
 		return fail(CRP_E_NOMEM, "crp_field_eval: hipMalloc failed");
 	}
 	e = hipMemcpy(dpos, pos, 3 * n * sizeof(double), hipMemcpyHostToDevice);
@@ -643,8 +775,8 @@ int crp_field_eval(const crp_field_t *f, const double *pos, size_t n, double *B)
 	}
 	if (e == hipSuccess)
 		e = hipMemcpy(B, dB, 3 * n * sizeof(double), hipMemcpyDeviceToHost);
-	hipFree(dpos);
-	hipFree(dB);
+	(void)hipFree(dpos);
+	(void)hipFree(dB);
 	if (e != hipSuccess)
 		return fail(CRP_E_HIP, std::string("crp_field_eval: ") + hipGetErrorString(e));
 	return CRP_OK;
@@ -678,6 +810,12 @@ static int create_prop(int type, const char *who, const crp_field_t *field, doub
 	p->maxStep = maxStep;
 	p->maxLength = 0;
 	p->precision = CRP_PRECISION_FP64;
+	p->block = 256;
+	p->blocksPerCU = 0;
+	p->stepsPerLaunch = 0;
+	p->useGraph = 0;
+	p->lastGrid = p->lastBlock = 0;
+	p->lastPPL = 0;
 	*out = p;
 	return CRP_OK;
 }
@@ -721,7 +859,7 @@ int crp_batch_create(size_t capacity, crp_batch_t **out) {
 		return fail(CRP_E_NOMEM, "crp_batch_create: out of host memory");
 	std::memset(b, 0, sizeof(*b));
 	b->cap = capacity;
-	hipGetDevice(&b->device);
+	(void)hipGetDevice(&b->device);
 	// one allocation: 18 double arrays, the active flags, the queue order, then a
 	// 256-byte aligned counter block
 	const size_t nd = 18;
@@ -746,7 +884,7 @@ int crp_batch_create(size_t capacity, crp_batch_t **out) {
 	if (e == hipSuccess)
 		e = hipEventCreate(&b->ev1);
 	if (e != hipSuccess) {
-		hipFree(b->base);
+		(void)hipFree(b->base);
 		delete b;
 		return fail(CRP_E_HIP, std::string("crp_batch_create: ") + hipGetErrorString(e));
 	}
@@ -757,11 +895,13 @@ int crp_batch_create(size_t capacity, crp_batch_t **out) {
 int crp_batch_destroy(crp_batch_t *b) {
 	if (!b)
 		return CRP_OK;
-	hipStreamSynchronize(b->stream);
-	hipEventDestroy(b->ev0);
-	hipEventDestroy(b->ev1);
-	hipStreamDestroy(b->stream);
-	hipFree(b->base);
+	(void)hipStreamSynchronize(b->stream);
+	if (b->gexec)
+		(void)hipGraphExecDestroy(b->gexec);
+	(void)hipEventDestroy(b->ev0);
+	(void)hipEventDestroy(b->ev1);
+	(void)hipStreamDestroy(b->stream);
+	(void)hipFree(b->base);
 	delete b;
 	return CRP_OK;
 }
@@ -894,8 +1034,6 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 	P.maxStep = p->maxStep;
 	P.maxLen = p->maxLength;
 	P.maxSteps = max_steps;
-	hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, b->stream, b->ctr, (int)CTR_COUNT);
-	CRP_HIP_TRY(hipGetLastError());
 	CRP_HIP_TRY(hipEventRecord(b->ev0, b->stream));
 	int rc;
 	const crp_field *f = p->field;
@@ -927,6 +1065,131 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 	if (stats)
 		*stats = st;
 	return CRP_OK;
+}
+
+int crp_propagator_set_launch(crp_propagator_t *p, const crp_launch_config *cfg) {
+	if (!p || !cfg)
+		return fail(CRP_E_INVALID, "crp_propagator_set_launch: NULL argument");
+	if (cfg->block != 0 && cfg->block != 64 && cfg->block != 128 && cfg->block != 256)
+		return fail(CRP_E_INVALID, "crp_propagator_set_launch: block must be 64, 128 or 256");
+	if (cfg->blocks_per_cu < 0)
+		return fail(CRP_E_INVALID, "crp_propagator_set_launch: blocks_per_cu must be >= 0");
+	p->block = cfg->block ? cfg->block : 256;
+	p->blocksPerCU = cfg->blocks_per_cu;
+	p->stepsPerLaunch = cfg->steps_per_launch > 0 ? cfg->steps_per_launch : 0;
+	p->useGraph = cfg->use_graph ? 1 : 0;
+	return CRP_OK;
+}
+
+int crp_propagator_get_launch(const crp_propagator_t *p, crp_launch_config *cfg) {
+	if (!p || !cfg)
+		return fail(CRP_E_INVALID, "crp_propagator_get_launch: NULL argument");
+	cfg->block = p->block;
+	cfg->blocks_per_cu = p->blocksPerCU;
+	cfg->steps_per_launch = p->stepsPerLaunch;
+	cfg->use_graph = p->useGraph;
+	cfg->particles_per_lane = p->lastPPL;
+	return CRP_OK;
+}
+
+int crp_propagator_autotune(crp_propagator_t *p, const crp_batch_t *b, long long probe_steps,
+                            crp_launch_config *chosen) {
+	if (!p || !b)
+		return fail(CRP_E_INVALID, "crp_propagator_autotune: NULL argument");
+	if (probe_steps < 1)
+		return fail(CRP_E_INVALID, "crp_propagator_autotune: probe_steps must be >= 1");
+	crp_launch_config keep;
+	crp_propagator_get_launch(p, &keep);
+	if (b->n == 0) {
+		if (chosen)
+			*chosen = keep;
+		return CRP_OK;
+	}
+	crp_batch *tmp = nullptr;
+	int rc = crp_batch_create(b->n, &tmp);
+	if (rc != CRP_OK)
+		return rc;
+	struct Cand {
+		int block, bpc;
+		long long spl;
+		unsigned grid;
+		double ms;
+	};
+	std::vector<Cand> tried;
+	// time one configuration: best of two runs on fresh copies of b
+	auto probe = [&](int block, int bpc, long long spl, Cand &out) -> int {
+		p->block = block;
+		p->blocksPerCU = bpc;
+		p->stepsPerLaunch = spl;
+		p->useGraph = 0;
+		double best = 1e300;
+		for (int r = 0; r < 2; r++) {
+			int e = crp_batch_copy(tmp, b);
+			if (e != CRP_OK)
+				return e;
+			crp_run_stats st;
+			e = crp_propagate(p, tmp, probe_steps, &st);
+			if (e != CRP_OK)
+				return e;
+			best = std::min(best, (double)st.kernel_ms);
+		}
+		out = Cand{block, bpc, spl, p->lastGrid, best};
+		return CRP_OK;
+	};
+	const int bpcs[] = {1, 2, 3, 4, 6, 8, 12, 16};
+	for (int block : {256, 128, 64}) {
+		for (int bpc : bpcs) {
+			Cand c;
+			rc = probe(block, bpc, 0, c);
+			if (rc != CRP_OK)
+				break;
+			bool dup = false;
+			for (const Cand &t : tried)
+				dup |= (t.block == c.block && t.grid == c.grid);
+			if (dup)
+				break; // the occupancy limit clamps blocks per CU: larger values launch the same grid
+			tried.push_back(c);
+			if ((size_t)c.grid * (size_t)block >= b->n)
+				break; // every particle has a lane already
+		}
+		if (rc != CRP_OK)
+			break;
+	}
+	if (rc == CRP_OK && !tried.empty()) {
+		Cand best = tried[0];
+		for (const Cand &t : tried)
+			if (t.ms < best.ms)
+				best = t;
+		// steps per launch: split the probe into launches of a quarter and a half of it
+		for (long long spl : {probe_steps / 4, probe_steps / 2}) {
+			if (spl < 1)
+				continue;
+			Cand c;
+			rc = probe(best.block, best.bpc, spl, c);
+			if (rc != CRP_OK)
+				break;
+			tried.push_back(c);
+			if (c.ms < best.ms)
+				best = c;
+		}
+		if (rc == CRP_OK) {
+			p->block = best.block;
+			p->blocksPerCU = best.bpc;
+			p->stepsPerLaunch = best.spl;
+			p->useGraph = keep.use_graph;
+			p->lastPPL = (double)b->n / ((double)best.grid * best.block);
+			if (chosen)
+				crp_propagator_get_launch(p, chosen);
+		}
+	}
+	if (rc != CRP_OK) {
+		p->block = keep.block;
+		p->blocksPerCU = keep.blocks_per_cu;
+		p->stepsPerLaunch = keep.steps_per_launch;
+		p->useGraph = keep.use_graph;
+	}
+	crp_batch_destroy(tmp);
+	return rc;
 }
 
 static int adj_args(const crp_adjoint_config *cfg, size_t n, AdjArgs &A) {

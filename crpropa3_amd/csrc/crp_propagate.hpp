@@ -21,7 +21,8 @@
 namespace crp {
 
 enum PropKind { PK_BP_FIXED = 0, PK_BP_ADAPTIVE = 1, PK_CK_FIXED = 2, PK_CK_ADAPTIVE = 3 };
-enum { CTR_QUEUE = 0, CTR_ACCEPTED, CTR_REJECTED, CTR_EVALS, CTR_FINISHED, CTR_COUNT };
+// CTR_PAUSED: particles stored still active because this launch's step budget ran out
+enum { CTR_QUEUE = 0, CTR_ACCEPTED, CTR_REJECTED, CTR_EVALS, CTR_FINISHED, CTR_PAUSED, CTR_COUNT };
 
 struct Soa {
 	double *px, *py, *pz, *dx, *dy, *dz, *E, *q, *cur, *next, *len, *time;
@@ -71,7 +72,7 @@ __global__ void __launch_bounds__(256) k_propagate(F field, Params P, Soa S, uns
 	float3 dirf = make_float3(0.f, 0.f, 0.f);
 	double E = 0, q = 0, cur = 0, next = 0, len = 0, tim = 0;
 	double step = 0, newStep = 0;
-	unsigned long long nAcc = 0, nRej = 0, nEval = 0, nFin = 0;
+	unsigned long long nAcc = 0, nRej = 0, nEval = 0, nFin = 0, nPause = 0;
 
 	for (;;) {
 		// ---- refill lanes without a particle from the queue
@@ -265,6 +266,7 @@ __global__ void __launch_bounds__(256) k_propagate(F field, Params P, Soa S, uns
 			S.active[idx] = alive ? 1 : 0;
 			S.ppx[idx] = ppos.x; S.ppy[idx] = ppos.y; S.ppz[idx] = ppos.z;
 			S.pdx[idx] = pdir.x; S.pdy[idx] = pdir.y; S.pdz[idx] = pdir.z;
+			nPause += alive ? 1 : 0;
 			have = false;
 		}
 	}
@@ -272,11 +274,13 @@ __global__ void __launch_bounds__(256) k_propagate(F field, Params P, Soa S, uns
 	nRej = wave_sum(nRej);
 	nEval = wave_sum(nEval);
 	nFin = wave_sum(nFin);
+	nPause = wave_sum(nPause);
 	if (lane == 0) {
 		atomicAdd(&ctr[CTR_ACCEPTED], nAcc);
 		atomicAdd(&ctr[CTR_REJECTED], nRej);
 		atomicAdd(&ctr[CTR_EVALS], nEval);
 		atomicAdd(&ctr[CTR_FINISHED], nFin);
+		atomicAdd(&ctr[CTR_PAUSED], nPause);
 	}
 }
 

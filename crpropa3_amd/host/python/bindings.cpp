@@ -307,7 +307,12 @@ PYBIND11_MODULE(_crpropa, m) {
 	    .def("getMinimumStep", &DevicePropagator::getMinimumStep)
 	    .def("getMaximumStep", &DevicePropagator::getMaximumStep)
 	    .def("setPrecision", &DevicePropagator::setPrecision)
-	    .def("getPrecision", &DevicePropagator::getPrecision);
+	    .def("getPrecision", &DevicePropagator::getPrecision)
+	    .def("setLaunchConfig", &DevicePropagator::setLaunchConfig, py::arg("block"), py::arg("blocksPerCU") = 0,
+	         py::arg("stepsPerLaunch") = 0, py::arg("useGraph") = false)
+	    .def("getLaunchConfig", &DevicePropagator::getLaunchConfig)
+	    .def("setAutotune", &DevicePropagator::setAutotune, py::arg("on"), py::arg("probeSteps") = 32)
+	    .def("getAutotune", &DevicePropagator::getAutotune);
 
 	py::class_<PropagationBP, DevicePropagator, ref_ptr<PropagationBP>>(m, "PropagationBP")
 	    .def(py::init<ref_ptr<MagneticField>, double>(), py::arg("field") = ref_ptr<MagneticField>(),

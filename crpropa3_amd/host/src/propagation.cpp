@@ -130,7 +130,52 @@ crp_propagator *DevicePropagator::makeDevicePropagator(double maxLength) const {
 		throwIf(crp_propagator_set_max_trajectory_length(p, maxLength));
 	if (precision != FP64)
 		throwIf(crp_propagator_set_precision(p, CRP_PRECISION_MIXED));
+	crp_launch_config lc;
+	lc.block = launchBlock;
+	lc.blocks_per_cu = launchBlocksPerCU;
+	lc.steps_per_launch = launchStepsPerLaunch;
+	lc.use_graph = launchGraph ? 1 : 0;
+	lc.particles_per_lane = 0;
+	throwIf(crp_propagator_set_launch(p, &lc));
 	return h.release();
+}
+
+DevicePropagator::~DevicePropagator() {}
+
+void DevicePropagator::setLaunchConfig(int block, int blocksPerCU, long long stepsPerLaunch, bool useGraph) {
+	if (block != 64 && block != 128 && block != 256)
+		throw std::runtime_error(std::string(name()) + ": launch block size must be 64, 128 or 256");
+	if (blocksPerCU < 0)
+		throw std::runtime_error(std::string(name()) + ": blocks per CU must be >= 0");
+	launchBlock = block;
+	launchBlocksPerCU = blocksPerCU;
+	launchStepsPerLaunch = stepsPerLaunch > 0 ? stepsPerLaunch : 0;
+	launchGraph = useGraph;
+}
+
+void DevicePropagator::setAutotune(bool on, long long probeSteps) {
+	if (probeSteps < 1)
+		throw std::runtime_error(std::string(name()) + ": autotune probe steps must be >= 1");
+	autotune = on;
+	autotuneProbe = probeSteps;
+	tuned = false;
+}
+
+std::vector<double> DevicePropagator::getLaunchConfig() const {
+	return {(double)launchBlock, (double)launchBlocksPerCU, (double)launchStepsPerLaunch, launchGraph ? 1. : 0.,
+	        lastPPL};
+}
+
+void DevicePropagator::tuneOn(crp_propagator *p, const crp_batch *b) const {
+	if (autotune && !tuned) {
+		crp_launch_config lc;
+		throwIf(crp_propagator_autotune(p, b, autotuneProbe, &lc));
+		DevicePropagator *self = const_cast<DevicePropagator *>(this);
+		self->launchBlock = lc.block;
+		self->launchBlocksPerCU = lc.blocks_per_cu;
+		self->launchStepsPerLaunch = lc.steps_per_launch;
+		tuned = true;
+	}
 }
 
 /* One step of one candidate on the GPU (correct, not fast: batch of one). */
@@ -295,9 +340,13 @@ bool ModuleList::runBatched(const candidate_vector_t &cands) {
 	pack(act, pk);
 	crp_host_state s = pk.view();
 	throwIf(crp_batch_upload(b, act.size(), &s));
+	prop->tuneOn(p.get(), b);
 	crp_run_stats st;
 	throwIf(crp_propagate(p.get(), b, steps, &st));
 	throwIf(crp_batch_download(b, &s));
+	crp_launch_config used;
+	throwIf(crp_propagator_get_launch(p.get(), &used));
+	prop->noteParticlesPerLane(used.particles_per_lane);
 	std::vector<char> moved(act.size());
 	for (size_t i = 0; i < act.size(); i++)
 		moved[i] = pk.len[i] != act[i]->getTrajectoryLength() || pk.cur[i] != act[i]->getCurrentStep() ||

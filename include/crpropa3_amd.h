@@ -157,6 +157,29 @@ int crp_batch_copy(crp_batch_t *dst, const crp_batch_t *src);
  * inactive, which needs a trajectory-length limit and minStep > 0).  Synchronous. */
 int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps, crp_run_stats *stats);
 
+/* ---- launch configuration, hipGraph replay and autotuning ------------------
+ * No reference counterpart (the reference loops over candidates with OpenMP,
+ * src/ModuleList.cpp.in:116-125).  Results do not depend on any of these
+ * settings: every particle's arithmetic is the same whichever lane, launch or
+ * graph replay runs it. */
+typedef struct {
+	int block;                  /* threads per workgroup: 64, 128 or 256 (0: 256) */
+	int blocks_per_cu;          /* resident workgroups per CU (0: the occupancy limit) */
+	long long steps_per_launch; /* split a call into launches of at most this many process() calls per
+	                               particle (<= 0: one launch) */
+	int use_graph;              /* capture the call's launch sequence in a hipGraph and replay it while the
+	                               propagator, field, batch size and step budget stay the same */
+	double particles_per_lane;  /* output only: batch size / lanes launched, from the last call */
+} crp_launch_config;
+
+int crp_propagator_set_launch(crp_propagator_t *p, const crp_launch_config *cfg);
+int crp_propagator_get_launch(const crp_propagator_t *p, crp_launch_config *cfg);
+/* Time short runs (probe_steps process() calls per particle, on a device copy of b, which is left
+ * unchanged) over block sizes, workgroups per CU and steps per launch; keep the fastest as the
+ * propagator's launch configuration and return it in *chosen (may be NULL). */
+int crp_propagator_autotune(crp_propagator_t *p, const crp_batch_t *b, long long probe_steps,
+                            crp_launch_config *chosen);
+
 /* ---- adjoint of the fixed-step Boris push through a periodic Grid3f -------
  * No reference counterpart: the reference has no derivatives.  The forward model is
  * PropagationBP::dY (src/module/PropagationBP.cpp:19-39) with both direction normalisations,

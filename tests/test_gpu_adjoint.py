@@ -102,7 +102,9 @@ def test_adjoint_matches_finite_differences_of_gpu_forward():
     flat = s["grid"].reshape(-1)
     order = np.argsort(-np.abs(got[3].reshape(-1)))
     picks = np.concatenate([order[:192], rng.choice(order[:2000], 64, replace=False)])
-    e = np.float32(1e-3) * np.abs(flat).max()
+    # central differences converge as e^2: at 1e-3 of the largest value the truncation error is ~3e-3 of
+    # the largest gradient, at 1e-5 it is ~3e-8 (measured on the oracle)
+    e = np.float32(1e-5) * np.abs(flat).max()
     fd = []
     for j in picks:
         gp = flat.copy()
@@ -115,8 +117,12 @@ def test_adjoint_matches_finite_differences_of_gpu_forward():
     fd = np.array(fd)
     ad = got[3].reshape(-1)[picks]
     err = np.abs(ad - fd) / np.abs(got[3]).max()
-    print("grid FD: %d values, max err %.2e of the largest gradient" % (len(picks), err.max()))
-    assert err.max() < 1e-4
+    big = np.abs(fd) > 1e-2 * np.abs(fd).max()
+    rel = np.abs(ad[big] - fd[big]) / np.abs(fd[big])
+    print("grid FD: %d values, max err %.2e of the largest gradient; %d large ones: max rel err %.2e"
+          % (len(picks), err.max(), big.sum(), rel.max()))
+    assert err.max() < 1e-6
+    assert rel.max() < 1e-4
     for i in range(12):
         for a in range(3):
             d = 1e-6

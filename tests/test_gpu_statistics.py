@@ -1,0 +1,133 @@
+"""Ensemble statistics over long, chaotic horizons.
+
+Per-particle agreement with the reference only holds while no rounding
+difference has been amplified (the reference's glibc cos/pow are not
+correctly rounded; the GPU's are), and the mixed path differs from FP64 at
+the FP32 level from the first step.  Over long horizons both are judged the
+way transport results are used: running diffusion coefficients
+D_perp(L) = <dx^2 + dy^2> / (4 L) and D_par(L) = <dz^2> / (2 L) (per unit
+path length L = c t) must agree within their statistical error, and the
+final-direction distributions must pass two-sample KS tests.  The reference
+side is the oracle with the host's glibc libm, i.e. the reference's own
+arithmetic (tests/test_oracle.py pins it bit for bit to the reference's
+trajectories)."""
+import numpy as np
+import pytest
+from scipy import stats
+
+import oracle_ctypes as O
+from crpropa3_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+
+def running_D(disp, L):
+    """D_perp, D_par and their standard errors from displacements (n,3) at path length L."""
+    perp = (disp[:, 0] ** 2 + disp[:, 1] ** 2) / (4 * L)
+    par = disp[:, 2] ** 2 / (2 * L)
+    n = len(disp)
+    return perp.mean(), perp.std(ddof=1) / np.sqrt(n), par.mean(), par.std(ddof=1) / np.sqrt(n)
+
+
+def compare_D(a, b, label, nsig=4.0):
+    """a, b: (D_perp, err, D_par, err); unpaired difference within nsig combined errors."""
+    for j, name in ((0, "perp"), (2, "par")):
+        sig = np.hypot(a[j + 1], b[j + 1])
+        z = abs(a[j] - b[j]) / sig
+        print("%s D_%s: %.4e vs %.4e (%.2f sigma)" % (label, name, a[j], b[j], z))
+        assert z < nsig, (label, name)
+
+
+def ks(a, b, label, pmin=1e-3):
+    p = stats.ks_2samp(a, b).pvalue
+    print("%s: KS p = %.3f" % (label, p))
+    assert p > pmin, label
+
+
+def planewave_setup(n=1024):
+    import bench
+    w = dict(bench.WORKLOADS["bp_planewave"])
+    fd = bench.make_field_data(w)
+    pos, d, E, q = bench.make_particles(w, n)
+    return w, fd, pos, d, E, q
+
+
+def test_planewave_running_diffusion_fp64_and_mixed_vs_oracle():
+    """C3 shape (PlaneWaveTurbulence Nm=256, Brms 1 muG, 1-100 pc, fixed 0.05 pc Boris steps, 1e15-1e18 eV)
+    with 1024 protons and snapshots at 1000, 2500, 5000 and 10000 steps."""
+    w, fd, pos, d, E, q = planewave_setup()
+    snaps = [1000, 2500, 5000, 10000]
+    h = w["min_step"]
+    gf = capi.Field.planewave(fd["k"], fd["Ak"], fd["xi"], fd["kappa"], fd["beta"])
+    of = O.OracleField.planewave(fd)
+    res = {}
+    for label, prec in (("fp64", capi.PRECISION_FP64), ("mixed", capi.PRECISION_MIXED)):
+        p = capi.Propagator("BP", gf, w["tolerance"], h, h, precision=prec)
+        b = capi.Batch(len(E))
+        b.upload(pos, d, E, q)
+        done, out = 0, []
+        for s in snaps:
+            p.propagate(b, max_steps=s - done)
+            done = s
+            out.append(b.download())
+        res[label] = out
+    st = O.OracleState(pos.copy(), d.copy(), E, q)
+    done, orc = 0, []
+    for s in snaps:
+        O.propagate(of, O.PROP_BP, st, s - done, min_step=h, max_step=h, libm=O.LIBM_GLIBC)
+        done = s
+        orc.append({"pos": st.pos.copy(), "dir": st.dir.copy()})
+    for k, s in enumerate(snaps):
+        L = s * h
+        ref = running_D(orc[k]["pos"] - pos, L)
+        g64 = running_D(res["fp64"][k]["pos"] - pos, L)
+        mix = running_D(res["mixed"][k]["pos"] - pos, L)
+        # FP64 GPU vs reference arithmetic: same particles, same trajectories up to amplified 1-ulp differences
+        rel = np.max(np.abs(res["fp64"][k]["pos"] - orc[k]["pos"]), axis=1) / np.max(np.abs(orc[k]["pos"]), axis=1)
+        print("%d steps: FP64 GPU vs oracle(glibc) per-particle rel. position error median %.1e, %.1f%% <= 1e-9"
+              % (s, np.median(rel), 100 * np.mean(rel <= 1e-9)))
+        if s <= 1000:
+            assert np.mean(rel <= 1e-9) >= 0.99
+        compare_D(g64, ref, "%d steps FP64 GPU vs oracle" % s)
+        compare_D(mix, ref, "%d steps mixed GPU vs oracle" % s)
+    for label in ("fp64", "mixed"):
+        for a in range(3):
+            ks(res[label][-1]["dir"][:, a], orc[-1]["dir"][:, a], "final dir[%d] %s vs oracle" % (a, label))
+
+
+def test_ck_grid_running_diffusion_fp64_and_mixed_vs_oracle():
+    """C2 shape at reduced size: adaptive CK (1e-4, 1 kpc - 1 Mpc) of 2048 nuclei through the 32^3 noise grid,
+    each run stopped by MaximumTrajectoryLength at 10, 30 and 100 Mpc (about 130, 400 and 1300 steps)."""
+    g = O.load_case("grid32")
+    gm = g["meta"]["grid"]
+    n = 2048
+    Z, E, d = O.sample_particles(77, n, 1 * O.EeV, 100 * O.EeV, weights=(0.5, 0.2, 0.15, 0.1, 0.05))
+    q = np.asarray([1, 2, 7, 14, 26], dtype=np.float64)[Z] * O.eplus
+    Lbox = gm["n"] * gm["spacing"]
+    pos = np.tile([Lbox / 2] * 3, (n, 1))
+    d = O.normalise(d)
+    gf = capi.Field.grid3f(g["grid"], gm["origin"], gm["spacing"])
+    of = O.OracleField.grid(g["grid"], gm["origin"], gm["spacing"])
+    for L in (10 * O.Mpc, 30 * O.Mpc, 100 * O.Mpc):
+        out = {}
+        for label, prec in (("fp64", capi.PRECISION_FP64), ("mixed", capi.PRECISION_MIXED)):
+            p = capi.Propagator("CK", gf, 1e-4, 1 * O.kpc, 1 * O.Mpc, max_length=L, precision=prec)
+            b = capi.Batch(n)
+            b.upload(pos, d, E, q)
+            p.propagate(b)
+            out[label] = b.download()
+            assert not out[label]["active"].any()
+        st = O.OracleState(pos.copy(), d.copy(), E, q)
+        O.propagate(of, O.PROP_CK, st, 10 ** 9, tolerance=1e-4, min_step=1 * O.kpc, max_step=1 * O.Mpc, max_len=L,
+                    libm=O.LIBM_GLIBC)
+        ref = running_D(st.pos - pos, L)
+        rel = np.max(np.abs(out["fp64"]["pos"] - st.pos), axis=1) / np.max(np.abs(st.pos), axis=1)
+        print("%.0f Mpc: FP64 GPU vs oracle(glibc) per-particle: %.1f%% <= 1e-9"
+              % (L / O.Mpc, 100 * np.mean(rel <= 1e-9)))
+        # the last step lands on L unless the remaining length is below minStep (then it overshoots by < minStep)
+        tl = out["fp64"]["trajectory_length"]
+        assert np.all(tl >= L * (1 - 1e-15)) and np.all(tl <= L + 1 * O.kpc)
+        compare_D(running_D(out["fp64"]["pos"] - pos, L), ref, "%.0f Mpc FP64 GPU vs oracle" % (L / O.Mpc))
+        compare_D(running_D(out["mixed"]["pos"] - pos, L), ref, "%.0f Mpc mixed GPU vs oracle" % (L / O.Mpc))
+        for a in range(3):
+            ks(out["mixed"]["dir"][:, a], st.dir[:, a], "%.0f Mpc final dir[%d] mixed vs oracle" % (L / O.Mpc, a))

@@ -209,3 +209,28 @@ def test_grid_interpolate_on_device():
     for i in range(0, 200, 7):
         b = f.getField(Vector3d(*c["pos"][i]))
         assert (b.x, b.y, b.z) == tuple(c["B_periodic"][i])
+
+
+def test_modulelist_autotuned_and_graph_launches_give_identical_results():
+    grid, gvals, gm, cands, E, Z, d = _grid_candidates(512, 45)
+    field = MagneticFieldGrid(grid)
+    results = []
+    for mode in ("default", "autotune", "graph+chunks"):
+        _, _, _, cs, _, _, _ = _grid_candidates(512, 45)
+        prop = PropagationCK(field, 1e-4, 1 * kpc, 1 * Mpc)
+        if mode == "autotune":
+            prop.setAutotune(True, 8)
+        elif mode == "graph+chunks":
+            prop.setLaunchConfig(128, 0, 25, True)
+        sim = ModuleList()
+        sim.add(prop)
+        sim.add(MaximumTrajectoryLength(2 * Mpc))
+        sim.run(cs)
+        cfg = prop.getLaunchConfig()
+        assert cfg[0] in (64, 128, 256) and cfg[4] > 0
+        results.append(np.array([[c.current.getPosition().x, c.current.getPosition().y, c.current.getPosition().z]
+                                 for c in cs]))
+    for r in results[1:]:
+        assert np.array_equal(r.view(np.uint64), results[0].view(np.uint64))
+    with pytest.raises(RuntimeError):
+        PropagationBP(field, 1 * kpc).setLaunchConfig(100)
