@@ -14,6 +14,15 @@ _crpropa); the compute path is the HIP library libcrpropa3_amd.so (C ABI in
 include/crpropa3_amd.h, Python binding in crpropa3_amd.capi).  There is no
 CPU fallback: importing fails if the libraries are not built.
 """
-from . import capi  # noqa: F401
-from ._crpropa import *  # noqa: F401,F403
+# PyTorch ships its own copy of the HIP runtime (libamdhip64, same soname as ROCm's).  Two copies
+# in one process cannot both use the GPU: whichever initialises second sees no device.  Importing
+# torch before this package's HIP library is loaded makes the library bind to torch's copy, so
+# torch tensors (crpropa3_amd.adjoint) and the propagator share one runtime.
+try:
+    import torch as _torch  # noqa: F401
+except ImportError:
+    pass
+
+from . import capi  # noqa: F401,E402
+from ._crpropa import *  # noqa: F401,F403,E402
 from ._crpropa import __doc__ as _native_doc  # noqa: F401

@@ -50,6 +50,23 @@ struct CK {
 // Vector3<T>::getR: sqrt((x*x + y*y) + z*z)
 __device__ __forceinline__ double norm(const V3 &v) { return __builtin_sqrt((v.x * v.x + v.y * v.y) + v.z * v.z); }
 
+// x / s, bit-identical to the IEEE division, from invs = RN(1/s) (one IEEE division shared by
+// several quotients with the same divisor): q = x * invs is within one ulp of x/s, the FMA
+// residual x - q*s is exact, and RN(q + residual * invs) is the correctly rounded quotient
+// (Markstein's correction theorem; J.-M. Muller et al., Handbook of Floating-Point Arithmetic,
+// division with an FMA).  The theorem assumes no underflow or overflow: magnitudes outside
+// [2^-500, 2^500] (and 0, inf, NaN) take the IEEE division.
+__device__ __forceinline__ double div_rn(double x, double s, double invs) {
+	const double ax = __builtin_fabs(x), as = __builtin_fabs(s);
+	if (!(ax > 0x1p-500 && ax < 0x1p500 && as > 0x1p-500 && as < 0x1p500))
+		return x / s;
+	const double q = x * invs;
+	const double r = __builtin_fma(-q, s, x);
+	return __builtin_fma(r, invs, q);
+}
+
+constexpr double kInvCLight = 1.0 / 299792458.0; // RN(1/c), folded by the compiler
+
 // ---------------------------------------------------------------- fields
 // Parameters as the host computes them from the reference's field objects.
 struct GridDesc {
@@ -59,7 +76,9 @@ struct GridDesc {
 	double gridOrigin[3]; // origin + spacing/2 (Grid.h:225-228)
 	double spacing[3];
 	double origin[3], edge[3]; // clipVolume box: origin, origin + N*spacing (Grid.h:320-327)
-	double invSpacing[3];      // 1/spacing (mixed-precision path only)
+	double invSpacing[3];      // RN(1/spacing), computed on the host
+	double invN[3];            // 1/N per axis, for the division-free periodic wrap
+	int fastDiv;               // spacing within [2^-500, 2^500]: div_cr may use its reciprocal path
 };
 
 struct PlaneWaveDesc {
@@ -119,10 +138,35 @@ __device__ __forceinline__ float3 trilinear_f32(const float *v, float fx, float 
 	return make_float3(b[0], b[1], b[2]);
 }
 
-__device__ __forceinline__ void periodic_clamp(double x, int n, int &lo, int &hi, double &frac) {
+// x / s rounded exactly like the IEEE division the reference performs, without a division:
+// q = x * RN(1/s) is within one ulp of x/s, the FMA residual x - q*s is exact, and
+// RN(q + residual * RN(1/s)) is the correctly rounded quotient when nothing underflows or
+// overflows (Markstein's correction theorem; J.-M. Muller et al., Handbook of Floating-Point
+// Arithmetic, the chapter on division with an FMA).  Magnitudes outside [2^-500, 2^500] (and
+// 0, inf, NaN) take the IEEE division.
+__device__ __forceinline__ double div_cr(double x, double s, double invs, int fast) {
+	const double ax = __builtin_fabs(x);
+	if (!(fast && ax > 0x1p-500 && ax < 0x1p500))
+		return x / s;
+	const double q = x * invs;
+	const double r = __builtin_fma(-q, s, x);
+	return __builtin_fma(r, invs, q);
+}
+
+// ((i % n) + n) % n (Grid.h:31-34) without an integer division: floor(i * (1/n)) is within one
+// of floor(i/n) for |i| < 2^31, the remainder is corrected into [0, n).
+__device__ __forceinline__ int wrap_index(int i, int n, double invn) {
+	const long long q = (long long)__builtin_floor((double)i * invn);
+	long long r = (long long)i - q * (long long)n;
+	r += (r < 0) ? n : 0;
+	r -= (r >= n) ? n : 0;
+	return (int)r;
+}
+
+__device__ __forceinline__ void periodic_clamp(double x, int n, double invn, int &lo, int &hi, double &frac) {
 	double fl = __builtin_floor(x);
-	lo = (((int)fl % n) + n) % n;
-	hi = (lo + 1) % n;
+	lo = wrap_index((int)fl, n, invn);
+	hi = (lo + 1 == n) ? 0 : lo + 1;
 	frac = x - fl;
 }
 
@@ -150,9 +194,9 @@ struct FieldGrid {
 			if (!in)
 				return v3(0.0, 0.0, 0.0);
 		}
-		double rx = (p.x - g.gridOrigin[0]) / g.spacing[0];
-		double ry = (p.y - g.gridOrigin[1]) / g.spacing[1];
-		double rz = (p.z - g.gridOrigin[2]) / g.spacing[2];
+		double rx = div_cr(p.x - g.gridOrigin[0], g.spacing[0], g.invSpacing[0], g.fastDiv);
+		double ry = div_cr(p.y - g.gridOrigin[1], g.spacing[1], g.invSpacing[1], g.fastDiv);
+		double rz = div_cr(p.z - g.gridOrigin[2], g.spacing[2], g.invSpacing[2], g.fastDiv);
 		int x0, x1, y0, y1, z0, z1;
 		double fx, fy, fz;
 		if (g.reflective) {
@@ -160,9 +204,9 @@ struct FieldGrid {
 			reflective_clamp(ry, g.ny, y0, y1, fy);
 			reflective_clamp(rz, g.nz, z0, z1, fz);
 		} else {
-			periodic_clamp(rx, g.nx, x0, x1, fx);
-			periodic_clamp(ry, g.ny, y0, y1, fy);
-			periodic_clamp(rz, g.nz, z0, z1, fz);
+			periodic_clamp(rx, g.nx, g.invN[0], x0, x1, fx);
+			periodic_clamp(ry, g.ny, g.invN[1], y0, y1, fy);
+			periodic_clamp(rz, g.nz, g.invN[2], z0, z1, fz);
 		}
 		// weights narrowed to float (Vector3<float>::operator*(float))
 		const float wx0 = (float)fx, wy0 = (float)fy, wz0 = (float)fz;
@@ -216,9 +260,9 @@ struct FieldGrid {
 			reflective_clamp(ry, g.ny, y0, y1, fy);
 			reflective_clamp(rz, g.nz, z0, z1, fz);
 		} else {
-			periodic_clamp(rx, g.nx, x0, x1, fx);
-			periodic_clamp(ry, g.ny, y0, y1, fy);
-			periodic_clamp(rz, g.nz, z0, z1, fz);
+			periodic_clamp(rx, g.nx, g.invN[0], x0, x1, fx);
+			periodic_clamp(ry, g.ny, g.invN[1], y0, y1, fy);
+			periodic_clamp(rz, g.nz, g.invN[2], z0, z1, fz);
 		}
 		const size_t sy = (size_t)g.nz, sx = (size_t)g.ny * g.nz;
 		const size_t X[2] = {(size_t)x0 * sx, (size_t)x1 * sx}, Yo[2] = {(size_t)y0 * sy, (size_t)y1 * sy};
@@ -255,14 +299,14 @@ struct FieldGridCells {
 			if (!in)
 				return v3(0.0, 0.0, 0.0);
 		}
-		double rx = (p.x - g.gridOrigin[0]) / g.spacing[0];
-		double ry = (p.y - g.gridOrigin[1]) / g.spacing[1];
-		double rz = (p.z - g.gridOrigin[2]) / g.spacing[2];
+		double rx = div_cr(p.x - g.gridOrigin[0], g.spacing[0], g.invSpacing[0], g.fastDiv);
+		double ry = div_cr(p.y - g.gridOrigin[1], g.spacing[1], g.invSpacing[1], g.fastDiv);
+		double rz = div_cr(p.z - g.gridOrigin[2], g.spacing[2], g.invSpacing[2], g.fastDiv);
 		int x0, x1, y0, y1, z0, z1;
 		double fx, fy, fz;
-		periodic_clamp(rx, g.nx, x0, x1, fx);
-		periodic_clamp(ry, g.ny, y0, y1, fy);
-		periodic_clamp(rz, g.nz, z0, z1, fz);
+		periodic_clamp(rx, g.nx, g.invN[0], x0, x1, fx);
+		periodic_clamp(ry, g.ny, g.invN[1], y0, y1, fy);
+		periodic_clamp(rz, g.nz, g.invN[2], z0, z1, fz);
 		const float wx0 = (float)fx, wy0 = (float)fy, wz0 = (float)fz;
 		const float wx1 = (float)(1 - fx), wy1 = (float)(1 - fy), wz1 = (float)(1 - fz);
 		const size_t cidx = ((size_t)x0 * g.ny + y0) * g.nz + z0;
@@ -481,14 +525,18 @@ struct Y {
 
 // PropagationBP::dY (src/module/PropagationBP.cpp:19-39)
 template <class F>
-__device__ __forceinline__ Y bp_dy(const F &field, const V3 &pos0, const V3 &dir, double step, double q, double m) {
+// (divisions as div_rn with shared reciprocals: invm = RN(1/m) from the caller)
+__device__ __forceinline__ Y bp_dy(const F &field, const V3 &pos0, const V3 &dir, double step, double q, double m,
+                                  double invm) {
 	V3 pos = v3(pos0.x + (dir.x * step) / 2., pos0.y + (dir.y * step) / 2., pos0.z + (dir.z * step) / 2.);
 	V3 B = field.eval(pos);
-	V3 t = v3(((((B.x * q) / 2) / m) * step) / kCLight, ((((B.y * q) / 2) / m) * step) / kCLight,
-	          ((((B.z * q) / 2) / m) * step) / kCLight);
+	V3 t = v3(div_rn(div_rn((B.x * q) / 2, m, invm) * step, kCLight, kInvCLight),
+	          div_rn(div_rn((B.y * q) / 2, m, invm) * step, kCLight, kInvCLight),
+	          div_rn(div_rn((B.z * q) / 2, m, invm) * step, kCLight, kInvCLight));
 	double tt = (t.x * t.x + t.y * t.y) + t.z * t.z;
 	double den = 1 + tt;
-	V3 s = v3((t.x * 2) / den, (t.y * 2) / den, (t.z * 2) / den);
+	const double invden = 1 / den;
+	V3 s = v3(div_rn(t.x * 2, den, invden), div_rn(t.y * 2, den, invden), div_rn(t.z * 2, den, invden));
 	V3 vh = v3(dir.x + (dir.y * t.z - t.y * dir.z), dir.y + (dir.z * t.x - t.z * dir.x),
 	           dir.z + (dir.x * t.y - t.x * dir.y));
 	V3 d = v3(dir.x + (vh.y * s.z - s.y * vh.z), dir.y + (vh.z * s.x - s.z * vh.x),
@@ -505,7 +553,8 @@ template <class F>
 __device__ __forceinline__ Y ck_dydt(const F &field, const Y &y, double qcE) {
 	Y d;
 	double n = norm(y.u);
-	d.x = v3((y.u.x / n) * kCLight, (y.u.y / n) * kCLight, (y.u.z / n) * kCLight);
+	const double invn = 1 / n;
+	d.x = v3(div_rn(y.u.x, n, invn) * kCLight, div_rn(y.u.y, n, invn) * kCLight, div_rn(y.u.z, n, invn) * kCLight);
 	V3 B = field.eval(y.x);
 	d.u = v3((d.x.y * B.z - B.y * d.x.z) * qcE, (d.x.z * B.x - B.z * d.x.x) * qcE,
 	         (d.x.x * B.y - B.x * d.x.y) * qcE);

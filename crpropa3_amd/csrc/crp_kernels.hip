@@ -479,7 +479,11 @@ static int fill_grid_desc(GridDesc &g, const float *d_values, int nx, int ny, in
 		g.gridOrigin[a] = origin[a] + spacing[a] / 2;      // Grid::setOrigin
 		g.edge[a] = origin[a] + (double)N[a] * spacing[a]; // Grid::interpolate clip box
 		g.invSpacing[a] = 1.0 / spacing[a];
+		g.invN[a] = 1.0 / (double)N[a];
 	}
+	g.fastDiv = 1;
+	for (int a = 0; a < 3; a++)
+		g.fastDiv &= (spacing[a] >= 0x1p-500 && spacing[a] <= 0x1p500) ? 1 : 0;
 	return CRP_OK;
 }
 

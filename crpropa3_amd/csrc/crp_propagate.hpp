@@ -155,7 +155,8 @@ __global__ void __launch_bounds__(256) k_propagate(F field, Params P, Soa S, uns
 						double eU;
 						ck_trystep(field, Y{pos, dir}, step / kCLight, (q * kCLight) / E, yOut, eU);
 					} else {
-						yOut = bp_dy(field, pos, dir, step, q, E / (kCLight * kCLight));
+						const double m_ = E / (kCLight * kCLight);
+						yOut = bp_dy(field, pos, dir, step, q, m_, 1 / m_);
 					}
 				}
 				nEval += kEvalsPerTrial;
@@ -196,9 +197,10 @@ __global__ void __launch_bounds__(256) k_propagate(F field, Params P, Soa S, uns
 					r = eU / P.tol;
 				} else {
 					const double m_ = E / (kCLight * kCLight);
-					yOut = bp_dy(field, pos, dir, step, q, m_);
-					Y half = bp_dy(field, pos, dir, step / 2, q, m_);
-					Y cmp = bp_dy(field, half.x, half.u, step / 2, q, m_);
+					const double im_ = 1 / m_;
+					yOut = bp_dy(field, pos, dir, step, q, m_, im_);
+					Y half = bp_dy(field, pos, dir, step / 2, q, m_, im_);
+					Y cmp = bp_dy(field, half.x, half.u, step / 2, q, m_, im_);
 					V3 d = v3(yOut.x.x - cmp.x.x, yOut.x.y - cmp.x.y, yOut.x.z - cmp.x.z);
 					double Sv = norm(d) / (step * (1 - 1 / 4.));
 					r = __builtin_sqrt((Sv * Sv + Sv * Sv) + Sv * Sv) / P.tol;
@@ -236,10 +238,10 @@ __global__ void __launch_bounds__(256) k_propagate(F field, Params P, Soa S, uns
 		} else {
 			pos = yOut.x;
 			if (!neutral) {
-				const double n1 = norm(yOut.u);
-				V3 u1 = v3(yOut.u.x / n1, yOut.u.y / n1, yOut.u.z / n1);
-				const double n2 = norm(u1);
-				dir = v3(u1.x / n2, u1.y / n2, u1.z / n2);
+				const double n1 = norm(yOut.u), i1 = 1 / n1;
+				V3 u1 = v3(div_rn(yOut.u.x, n1, i1), div_rn(yOut.u.y, n1, i1), div_rn(yOut.u.z, n1, i1));
+				const double n2 = norm(u1), i2 = 1 / n2;
+				dir = v3(div_rn(u1.x, n2, i2), div_rn(u1.y, n2, i2), div_rn(u1.z, n2, i2));
 			}
 		}
 		cur = step;

@@ -30,12 +30,12 @@ def c2_small(n=3000, seed=5):
     sp, E, d = O.sample_particles(seed, n, 1 * O.EeV, 300 * O.EeV, weights=[0.5, 0.2, 0.15, 0.1, 0.05])
     q = np.array([1, 2, 7, 14, 26])[sp] * O.eplus
     L = gm["n"] * gm["spacing"]
-    return g["grid"], gm, np.tile([L / 2] * 3, (n, 1)), O.normalise(d), E, q
+    return g["grid"], gm, np.tile([L / 2] * 3, (n, 1)), d, E, q  # d: raw draws, normalised on upload
 
 
 def run(prop, pos, d, E, q, chunks):
     b = capi.Batch(len(E))
-    b.upload(pos, d, E, q)
+    b.upload(pos, O.normalise(d), E, q)  # the oracle state normalises d itself (setDirection)
     out, stats = [], []
     for m in chunks:
         st = prop.propagate(b, max_steps=m)
@@ -82,7 +82,7 @@ def test_graph_replay_every_replay_matches_eager():
     eager = capi.Propagator("CK", field, 1e-4, 1 * O.kpc, 1 * O.Mpc, max_length=4 * O.Mpc)
     graph = capi.Propagator("CK", field, 1e-4, 1 * O.kpc, 1 * O.Mpc, max_length=4 * O.Mpc).set_launch(use_graph=True)
     init = capi.Batch(len(E))
-    init.upload(pos, d, E, q)
+    init.upload(pos, O.normalise(d), E, q)
     w_e, w_g = capi.Batch(len(E)), capi.Batch(len(E))
     for rep in range(4):
         for budget in (50, 50, 120):  # same budget replays the captured graph; a new one re-captures
@@ -102,7 +102,7 @@ def test_autotune_leaves_batch_unchanged_and_results_identical():
     field = capi.Field.grid3f(grid, gm["origin"], gm["spacing"])
     p = capi.Propagator("CK", field, 1e-4, 1 * O.kpc, 1 * O.Mpc, max_length=3 * O.Mpc)
     b = capi.Batch(len(E))
-    b.upload(pos, d, E, q)
+    b.upload(pos, O.normalise(d), E, q)
     before = b.download()
     cfg = p.autotune(b, probe_steps=16)
     print("autotune:", cfg)
@@ -113,7 +113,7 @@ def test_autotune_leaves_batch_unchanged_and_results_identical():
         assert np.array_equal(before[k], after[k]), k
     ref = capi.Propagator("CK", field, 1e-4, 1 * O.kpc, 1 * O.Mpc, max_length=3 * O.Mpc)
     b2 = capi.Batch(len(E))
-    b2.upload(pos, d, E, q)
+    b2.upload(pos, O.normalise(d), E, q)
     ref.propagate(b2)
     p.propagate(b)
     x, y = b.download(), b2.download()

@@ -105,7 +105,7 @@ def test_ck_grid_running_diffusion_fp64_and_mixed_vs_oracle():
     q = np.asarray([1, 2, 7, 14, 26], dtype=np.float64)[Z] * O.eplus
     Lbox = gm["n"] * gm["spacing"]
     pos = np.tile([Lbox / 2] * 3, (n, 1))
-    d = O.normalise(d)
+    d_raw, d = d, O.normalise(d)  # the oracle state normalises the raw draws itself (setDirection)
     gf = capi.Field.grid3f(g["grid"], gm["origin"], gm["spacing"])
     of = O.OracleField.grid(g["grid"], gm["origin"], gm["spacing"])
     for L in (10 * O.Mpc, 30 * O.Mpc, 100 * O.Mpc):
@@ -117,7 +117,7 @@ def test_ck_grid_running_diffusion_fp64_and_mixed_vs_oracle():
             p.propagate(b)
             out[label] = b.download()
             assert not out[label]["active"].any()
-        st = O.OracleState(pos.copy(), d.copy(), E, q)
+        st = O.OracleState(pos.copy(), d_raw, E, q)
         O.propagate(of, O.PROP_CK, st, 10 ** 9, tolerance=1e-4, min_step=1 * O.kpc, max_step=1 * O.Mpc, max_len=L,
                     libm=O.LIBM_GLIBC)
         ref = running_D(st.pos - pos, L)
