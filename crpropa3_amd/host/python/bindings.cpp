@@ -10,6 +10,7 @@
 
 #include "crpropa/Candidate.h"
 #include "crpropa/Grid.h"
+#include "crpropa/GridTools.h"
 #include "crpropa/Module.h"
 #include "crpropa/ModuleList.h"
 #include "crpropa/ParticleState.h"
@@ -232,6 +233,7 @@ PYBIND11_MODULE(_crpropa, m) {
 	    .def("setClipVolume", &Grid3f::setClipVolume)
 	    .def("getClipVolume", &Grid3f::getClipVolume)
 	    .def("setInterpolationType", &Grid3f::setInterpolationType)
+	    .def("getInterpolationType", &Grid3f::getInterpolationType)
 	    .def("getValue", &Grid3f::getValue)
 	    .def("setValue", &Grid3f::setValue)
 	    .def("interpolate", &Grid3f::interpolate)
@@ -263,6 +265,20 @@ PYBIND11_MODULE(_crpropa, m) {
 	    .def(py::init<ref_ptr<Grid3f>>())
 	    .def("setGrid", &MagneticFieldGrid::setGrid)
 	    .def("getGrid", &MagneticFieldGrid::getGrid);
+
+	// grid tools (include/crpropa/GridTools.h)
+	m.def("scaleGrid", &scaleGrid, py::arg("grid"), py::arg("a"));
+	m.def("meanFieldVector", &meanFieldVector);
+	m.def("meanFieldStrength", &meanFieldStrength);
+	m.def("rmsFieldStrength", &rmsFieldStrength);
+	m.def("rmsFieldStrengthPerAxis", &rmsFieldStrengthPerAxis);
+	m.def("fromMagneticField", &fromMagneticField, py::arg("grid"), py::arg("field"));
+	m.def("loadGrid", &loadGrid, py::arg("grid"), py::arg("filename"), py::arg("conversion") = 1.);
+	m.def("dumpGrid", &dumpGrid, py::arg("grid"), py::arg("filename"), py::arg("conversion") = 1.);
+	m.def("loadGridFromTxt", &loadGridFromTxt, py::arg("grid"), py::arg("filename"), py::arg("conversion") = 1.);
+	m.def("loadGrid3fFromTxt", &loadGrid3fFromTxt, py::arg("filename"), py::arg("conversion") = 1.);
+	m.def("dumpGridToTxt", &dumpGridToTxt, py::arg("grid"), py::arg("filename"), py::arg("conversion") = 1.,
+	      py::arg("saveProp") = false);
 
 	py::class_<TurbulenceSpectrum, ref_ptr<TurbulenceSpectrum>>(m, "TurbulenceSpectrum")
 	    .def(py::init<double, double, double, double, double, double>(), py::arg("Brms"), py::arg("lMin"),

@@ -12,6 +12,8 @@
 //   PropagationCK::process                               src/module/PropagationCK.cpp:77-136
 //   MaximumTrajectoryLength::process                     src/module/BreakCondition.cpp:42-64
 //   Random (MT19937, randUniform, randVector, randPowerLaw, randBin) src/Random.cpp:80-205
+//   GridTools: scaleGrid, mean/rms field strength, loadGrid/dumpGrid (raw float32 and text),
+//   loadGrid3fFromTxt, fromMagneticField                 src/GridTools.cpp:17-236,238-447
 //
 // Particle ids: every species uses nucleusId(60, Z) so that nuclearMass()
 // takes its analytic branch (A > 56, src/ParticleMass.cpp:70-78) and the
@@ -22,6 +24,7 @@
 
 #include "crpropa/Candidate.h"
 #include "crpropa/Grid.h"
+#include "crpropa/GridTools.h"
 #include "crpropa/ParticleID.h"
 #include "crpropa/Random.h"
 #include "crpropa/Units.h"
@@ -316,6 +319,64 @@ static std::string grid_meta(const std::string &extra, int n, double sp, uint32_
 	       "}";
 }
 
+// Grid tools on a non-cubic 6x5x4 grid with non-uniform spacing: raw and text
+// dumps (files kept as fixtures), loads with a conversion factor, the field
+// statistics and a resampling through MagneticFieldGrid (fromMagneticField).
+static void gen_gridtools() {
+	std::string d = casedir("gridtools");
+	ref_ptr<Grid3f> g = new Grid3f(Vector3d(-1 * Mpc, 0.5 * Mpc, 2 * Mpc), 6, 5, 4, Vector3d(50 * kpc, 40 * kpc, 30 * kpc));
+	Random rng(77);
+	for (int ix = 0; ix < 6; ix++)
+		for (int iy = 0; iy < 5; iy++)
+			for (int iz = 0; iz < 4; iz++) {
+				Vector3f &b = g->get(ix, iy, iz);
+				b.x = rng.randUniform(-1, 1) * nG;
+				b.y = rng.randUniform(-1, 1) * nG + 0.3 * nG;
+				b.z = rng.randUniform(-1, 1) * nG;
+			}
+	save_grid(d, g);
+	dumpGrid(g, d + "/dump_c1.raw", 1);
+	dumpGrid(g, d + "/dump_c3.raw", 3.5);
+	dumpGridToTxt(g, d + "/dump_props.txt", 1 / nG, true);
+	dumpGridToTxt(g, d + "/dump_plain.txt", 1 / nG, false);
+	{
+		ref_ptr<Grid3f> h = new Grid3f(g->getOrigin(), 6, 5, 4, g->getSpacing());
+		loadGrid(h, d + "/dump_c1.raw", 0.25);
+		save_grid(d, h);
+		rename((d + "/grid.npy").c_str(), (d + "/loaded_raw_c025.npy").c_str());
+	}
+	{
+		ref_ptr<Grid3f> h = loadGrid3fFromTxt(d + "/dump_props.txt", nG);
+		save_grid(d, h);
+		rename((d + "/grid.npy").c_str(), (d + "/loaded_txt.npy").c_str());
+	}
+	{
+		ref_ptr<Grid3f> h = new Grid3f(*g);
+		scaleGrid(h, 2.5);
+		save_grid(d, h);
+		rename((d + "/grid.npy").c_str(), (d + "/scaled_25.npy").c_str());
+	}
+	{
+		// resample the (periodic) grid on a finer, shifted 7x6x5 grid
+		ref_ptr<Grid3f> r = new Grid3f(Vector3d(-0.9 * Mpc, 0.52 * Mpc, 2.01 * Mpc), 7, 6, 5, Vector3d(33 * kpc));
+		fromMagneticField(r, new MagneticFieldGrid(g));
+		save_grid(d, r);
+		rename((d + "/grid.npy").c_str(), (d + "/resampled.npy").c_str());
+	}
+	save_grid(d, g);
+	Vector3f mv = meanFieldVector(g);
+	std::array<float, 3> pa = rmsFieldStrengthPerAxis(g);
+	save_f32(d, "mean_vector", {mv.x, mv.y, mv.z}, {3});
+	save_f32(d, "rms_per_axis", {pa[0], pa[1], pa[2]}, {3});
+	save_f64(d, "strengths", {meanFieldStrength(g), rmsFieldStrength(g)}, {2});
+	save_meta(d, "{\"origin\": [" + fmt(-1 * Mpc) + ", " + fmt(0.5 * Mpc) + ", " + fmt(2 * Mpc) +
+	                 "], \"n\": [6, 5, 4], \"spacing\": [" + fmt(50 * kpc) + ", " + fmt(40 * kpc) + ", " +
+	                 fmt(30 * kpc) + "], \"seed\": 77, \"dump_c3\": 3.5, \"txt_conversion\": " + fmt(1 / nG) +
+	                 ", \"load_raw_conversion\": 0.25, \"scale\": 2.5, \"resampled\": {\"origin\": [" +
+	                 fmt(-0.9 * Mpc) + ", " + fmt(0.52 * Mpc) + ", " + fmt(2.01 * Mpc) + "], \"n\": [7, 6, 5], \"spacing\": " +
+	                 fmt(33 * kpc) + "}}");
+}
+
 int main(int argc, char **argv) {
 	g_root = argc > 1 ? argv[1] : "tests/golden";
 	mkdir(g_root.c_str(), 0755);
@@ -329,6 +390,7 @@ int main(int argc, char **argv) {
 	}
 
 	gen_trilinear();
+	gen_gridtools();
 
 	// Plane waves: the C3 mode table (Simple spectrum), the reference test's
 	// general spectrum (test/testPropagation.cpp:492), and a 2-mode table.
