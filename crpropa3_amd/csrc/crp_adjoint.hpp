@@ -27,43 +27,41 @@ namespace crp {
 struct AdjGrid {
 	const float *g; // (nx*ny + iy)*nz + iz, 3 floats per point
 	int nx, ny, nz;
-	double gridOrigin[3], invSpacing[3];
+	double gridOrigin[3], invSpacing[3], invN[3];
 	double scale;
 };
 
-// FP64 trilinear interpolation and, optionally, the corner data needed by
-// the reverse pass (indices, fractions).
+// FP64 trilinear interpolation and the corner data the reverse pass needs:
+// lower and upper corner index per axis (periodic) and the fractions.
 struct TriPoint {
-	size_t idx[8];
+	int lo[3], hi[3];
 	double fx, fy, fz;
 };
 
 __device__ __forceinline__ void adj_cell(const AdjGrid &G, const V3 &p, TriPoint &tp) {
-	double r[3] = {(p.x - G.gridOrigin[0]) * G.invSpacing[0], (p.y - G.gridOrigin[1]) * G.invSpacing[1],
-	               (p.z - G.gridOrigin[2]) * G.invSpacing[2]};
-	int n[3] = {G.nx, G.ny, G.nz};
-	int lo[3], hi[3];
+	const double r[3] = {(p.x - G.gridOrigin[0]) * G.invSpacing[0], (p.y - G.gridOrigin[1]) * G.invSpacing[1],
+	                     (p.z - G.gridOrigin[2]) * G.invSpacing[2]};
+	const int n[3] = {G.nx, G.ny, G.nz};
 	double f[3];
 #pragma unroll
 	for (int a = 0; a < 3; a++) {
-		double fl = __builtin_floor(r[a]);
+		const double fl = __builtin_floor(r[a]);
 		f[a] = r[a] - fl;
-		double w = fl - (double)n[a] * __builtin_floor(fl / n[a]);
-		int i = (int)w;
-		i = i >= n[a] ? i - n[a] : (i < 0 ? i + n[a] : i);
-		i = min(max(i, 0), n[a] - 1); // rounding of w at |r| beyond 2^52 cells stays in the grid
-		lo[a] = i;
-		hi[a] = (i + 1) % n[a];
+		// (int) saturates beyond +-2^31; wrap_index maps any int into [0, n)
+		const int i = wrap_index((int)fl, n[a], G.invN[a]);
+		tp.lo[a] = i;
+		tp.hi[a] = (i + 1 == n[a]) ? 0 : i + 1;
 	}
 	tp.fx = f[0];
 	tp.fy = f[1];
 	tp.fz = f[2];
+}
+
+// flat point index of corner c (corner order of Grid.h:572-579)
+__device__ __forceinline__ size_t adj_corner(const AdjGrid &G, const TriPoint &tp, int c) {
 	const int cx[8] = {0, 1, 0, 0, 1, 0, 1, 1}, cy[8] = {0, 0, 1, 0, 0, 1, 1, 1}, cz[8] = {0, 0, 0, 1, 1, 1, 0, 1};
-#pragma unroll
-	for (int c = 0; c < 8; c++) {
-		const int X = cx[c] ? hi[0] : lo[0], Yv = cy[c] ? hi[1] : lo[1], Z = cz[c] ? hi[2] : lo[2];
-		tp.idx[c] = ((size_t)X * G.ny + Yv) * G.nz + Z;
-	}
+	const int X = cx[c] ? tp.hi[0] : tp.lo[0], Yv = cy[c] ? tp.hi[1] : tp.lo[1], Z = cz[c] ? tp.hi[2] : tp.lo[2];
+	return ((size_t)X * G.ny + Yv) * G.nz + Z;
 }
 
 // corner weight and its gradient with respect to the fractions
@@ -83,7 +81,7 @@ __device__ __forceinline__ V3 adj_interp(const AdjGrid &G, const TriPoint &tp) {
 	for (int c = 0; c < 8; c++) {
 		double w, a, bb, cc;
 		adj_weight(tp, c, w, a, bb, cc);
-		const float *v = G.g + 3 * tp.idx[c];
+		const float *v = G.g + 3 * adj_corner(G, tp, c);
 		b[0] += w * (double)v[0];
 		b[1] += w * (double)v[1];
 		b[2] += w * (double)v[2];
@@ -161,15 +159,18 @@ __device__ __forceinline__ void adj_step_reverse(const AdjGrid &G, double h, dou
 	for (int c = 0; c < 8; c++) {
 		double w, dwx, dwy, dwz;
 		adj_weight(tp, c, w, dwx, dwy, dwz);
-		const float *gv = G.g + 3 * tp.idx[c];
+		const size_t ic = adj_corner(G, tp, c);
+		const float *gv = G.g + 3 * ic;
 		const double dotc = (double)gv[0] * aT.x + (double)gv[1] * aT.y + (double)gv[2] * aT.z;
 		gx += dwx * dotc;
 		gy += dwy * dotc;
 		gz += dwz * dotc;
-		double *o = gradG + 3 * tp.idx[c];
-		atomicAdd(o, w * aT.x);
-		atomicAdd(o + 1, w * aT.y);
-		atomicAdd(o + 2, w * aT.z);
+		// hardware FP64 atomic add in L2 (the gradient buffer is ordinary device memory, not
+		// fine-grained host-coherent memory, so the compare-and-swap loop of atomicAdd is not needed)
+		double *o = gradG + 3 * ic;
+		unsafeAtomicAdd(o, w * aT.x);
+		unsafeAtomicAdd(o + 1, w * aT.y);
+		unsafeAtomicAdd(o + 2, w * aT.z);
 	}
 	ax1 = vadd(ax1, v3(gx * G.invSpacing[0], gy * G.invSpacing[1], gz * G.invSpacing[2]));
 	// x1 = x + u h/2
@@ -247,7 +248,7 @@ __global__ void __launch_bounds__(256) k_adjoint_backward(AdjArgs A) {
 	for (int o = 32; o > 0; o >>= 1)
 		gs += __shfl_xor(gs, o);
 	if ((threadIdx.x & 63) == 0)
-		atomicAdd(A.gradScale, gs);
+		unsafeAtomicAdd(A.gradScale, gs);
 }
 
 } // namespace crp

@@ -286,7 +286,8 @@ static int launch_propagate(const F &f, const crp_propagator *p, const Params &P
 		return CRP_E_HIP;
 	const int block = (p->block == 64 || p->block == 128) ? p->block : 256;
 	int bpcMax = 0;
-	CRP_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpcMax, kern, block, 0));
+	const size_t lds = (size_t)kColdDoubles * block * sizeof(double);
+	CRP_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpcMax, kern, block, lds));
 	if (bpcMax < 1)
 		bpcMax = 1;
 	const int bpc = p->blocksPerCU > 0 ? std::min(p->blocksPerCU, bpcMax) : bpcMax;
@@ -312,7 +313,7 @@ static int launch_propagate(const F &f, const crp_propagator *p, const Params &P
 				P.maxSteps = open ? S : std::min(S, M - j * S);
 			if (j > 0 || !zeroAll)
 				hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, b->ctr, 1); // queue counter only
-			hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, st, f, P, soa, n, b->ctr);
+			hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, st, f, P, soa, n, b->ctr);
 		}
 		CRP_HIP_TRY(hipGetLastError());
 		return CRP_OK;
@@ -1212,6 +1213,7 @@ static int adj_args(const crp_adjoint_config *cfg, size_t n, AdjArgs &A) {
 	for (int a = 0; a < 3; a++) {
 		A.G.gridOrigin[a] = cfg->origin[a] + cfg->spacing[a] / 2;
 		A.G.invSpacing[a] = 1.0 / cfg->spacing[a];
+		A.G.invN[a] = 1.0 / (double)(a == 0 ? cfg->nx : (a == 1 ? cfg->ny : cfg->nz));
 	}
 	A.G.scale = cfg->scale;
 	A.h = cfg->step;

@@ -23,6 +23,7 @@ namespace crp {
 enum PropKind { PK_BP_FIXED = 0, PK_BP_ADAPTIVE = 1, PK_CK_FIXED = 2, PK_CK_ADAPTIVE = 3 };
 // CTR_PAUSED: particles stored still active because this launch's step budget ran out
 enum { CTR_QUEUE = 0, CTR_ACCEPTED, CTR_REJECTED, CTR_EVALS, CTR_FINISHED, CTR_PAUSED, CTR_COUNT };
+constexpr int kColdDoubles = 9; // dynamic LDS per thread of k_propagate, in doubles
 
 struct Soa {
 	double *px, *py, *pz, *dx, *dy, *dz, *E, *q, *cur, *next, *len, *time;
@@ -66,15 +67,31 @@ __global__ void __launch_bounds__(256) k_propagate(F field, Params P, Soa S, uns
 	const int lane = __lane_id();
 
 	bool have = false, exhausted = false, inProc = false;
-	unsigned long long idx = 0;
+	unsigned idx = 0;
 	long long steps = 0;
-	V3 pos = v3(0, 0, 0), dir = v3(0, 0, 0), ppos = pos, pdir = dir;
+	V3 pos = v3(0, 0, 0), dir = v3(0, 0, 0);
 	float3 dirf = make_float3(0.f, 0.f, 0.f);
-	double E = 0, q = 0, cur = 0, next = 0, len = 0, tim = 0;
+	double E = 0, q = 0, next = 0;
 	double step = 0, newStep = 0;
-	unsigned long long nAcc = 0, nRej = 0, nEval = 0, nFin = 0, nPause = 0;
+	// per-lane counters, flushed to ctr[] before they could wrap
+	unsigned nAcc = 0, nRej = 0, nEval = 0, nFin = 0, nPause = 0;
+	// cold per-particle state (previous position and direction, currentStep, trajectoryLength,
+	// time: read at refill, commit or store only) lives in LDS, kColdDoubles * blockDim.x doubles
+	// passed as dynamic shared memory, which keeps it out of the register file
+	// Layout: per wave, kColdDoubles rows of 64 lanes (constant row stride for every block size).
+	extern __shared__ double crp_cold[];
+	double *const cw = crp_cold + (threadIdx.x >> 6) * (kColdDoubles * 64) + (threadIdx.x & 63);
+	double *const cPPx = cw, *const cPPy = cw + 64, *const cPPz = cw + 128;
+	double *const cPDx = cw + 192, *const cPDy = cw + 256, *const cPDz = cw + 320;
+	double *const cCur = cw + 384, *const cLen = cw + 448, *const cTim = cw + 512;
 
 	for (;;) {
+		if (nEval > 0x80000000u) { // flush this lane's counters long before they wrap
+			atomicAdd(&ctr[CTR_ACCEPTED], (unsigned long long)nAcc);
+			atomicAdd(&ctr[CTR_REJECTED], (unsigned long long)nRej);
+			atomicAdd(&ctr[CTR_EVALS], (unsigned long long)nEval);
+			nAcc = nRej = nEval = 0;
+		}
 		// ---- refill lanes without a particle from the queue
 		const bool need = !have && !exhausted;
 		const unsigned long long m = __ballot(need);
@@ -96,10 +113,10 @@ __global__ void __launch_bounds__(256) k_propagate(F field, Params P, Soa S, uns
 							dirf = make_float3((float)dir.x, (float)dir.y, (float)dir.z);
 						E = S.E[idx];
 						q = S.q[idx];
-						cur = S.cur[idx];
+						*cCur = S.cur[idx];
 						next = S.next[idx];
-						len = S.len[idx];
-						tim = S.time[idx];
+						*cLen = S.len[idx];
+						*cTim = S.time[idx];
 						steps = 0;
 						inProc = false;
 					}
@@ -123,8 +140,12 @@ __global__ void __launch_bounds__(256) k_propagate(F field, Params P, Soa S, uns
 		V3 oPos;         // mixed path result
 		float3 oDir;
 		if (!inProc) {
-			ppos = pos; // candidate->previous = current
-			pdir = MIX ? v3(dirf.x, dirf.y, dirf.z) : dir;
+			*cPPx = pos.x; *cPPy = pos.y; *cPPz = pos.z; // candidate->previous = current
+			if (MIX) {
+				*cPDx = dirf.x; *cPDy = dirf.y; *cPDz = dirf.z;
+			} else {
+				*cPDx = dir.x; *cPDy = dir.y; *cPDz = dir.z;
+			}
 			if (q == 0) {
 				// rectilinear propagation for neutral particles (PropagationBP.cpp:74-80)
 				neutral = true;
@@ -244,9 +265,10 @@ __global__ void __launch_bounds__(256) k_propagate(F field, Params P, Soa S, uns
 				dir = v3(div_rn(u1.x, n2, i2), div_rn(u1.y, n2, i2), div_rn(u1.z, n2, i2));
 			}
 		}
-		cur = step;
-		len += step;
-		tim += step / kCLight;
+		*cCur = step;
+		const double len = *cLen + step;
+		*cLen = len;
+		*cTim = *cTim + step / kCLight;
 		next = newStep;
 		nAcc++;
 		steps++;
@@ -264,25 +286,22 @@ __global__ void __launch_bounds__(256) k_propagate(F field, Params P, Soa S, uns
 				dir = v3(dirf.x, dirf.y, dirf.z);
 			S.px[idx] = pos.x; S.py[idx] = pos.y; S.pz[idx] = pos.z;
 			S.dx[idx] = dir.x; S.dy[idx] = dir.y; S.dz[idx] = dir.z;
-			S.cur[idx] = cur; S.next[idx] = next; S.len[idx] = len; S.time[idx] = tim;
+			S.cur[idx] = *cCur; S.next[idx] = next; S.len[idx] = len; S.time[idx] = *cTim;
 			S.active[idx] = alive ? 1 : 0;
-			S.ppx[idx] = ppos.x; S.ppy[idx] = ppos.y; S.ppz[idx] = ppos.z;
-			S.pdx[idx] = pdir.x; S.pdy[idx] = pdir.y; S.pdz[idx] = pdir.z;
+			S.ppx[idx] = *cPPx; S.ppy[idx] = *cPPy; S.ppz[idx] = *cPPz;
+			S.pdx[idx] = *cPDx; S.pdy[idx] = *cPDy; S.pdz[idx] = *cPDz;
 			nPause += alive ? 1 : 0;
 			have = false;
 		}
 	}
-	nAcc = wave_sum(nAcc);
-	nRej = wave_sum(nRej);
-	nEval = wave_sum(nEval);
-	nFin = wave_sum(nFin);
-	nPause = wave_sum(nPause);
+	const unsigned long long tAcc = wave_sum(nAcc), tRej = wave_sum(nRej), tEval = wave_sum(nEval),
+	                         tFin = wave_sum(nFin), tPause = wave_sum(nPause);
 	if (lane == 0) {
-		atomicAdd(&ctr[CTR_ACCEPTED], nAcc);
-		atomicAdd(&ctr[CTR_REJECTED], nRej);
-		atomicAdd(&ctr[CTR_EVALS], nEval);
-		atomicAdd(&ctr[CTR_FINISHED], nFin);
-		atomicAdd(&ctr[CTR_PAUSED], nPause);
+		atomicAdd(&ctr[CTR_ACCEPTED], tAcc);
+		atomicAdd(&ctr[CTR_REJECTED], tRej);
+		atomicAdd(&ctr[CTR_EVALS], tEval);
+		atomicAdd(&ctr[CTR_FINISHED], tFin);
+		atomicAdd(&ctr[CTR_PAUSED], tPause);
 	}
 }
 

@@ -462,6 +462,33 @@ int main(int argc, char **argv) {
 		             fmt(EeV) + ", \"Emax\": " + fmt(100 * EeV) + "}");
 	}
 
+	// ---- the same grid read with reflective continuation and with clipVolume:
+	// particles leave the 1.6 Mpc box within the run (Grid.h:49-59, 320-327)
+	{
+		ref_ptr<Grid3f> gr = new Grid3f(*g32);
+		gr->setReflective(true);
+		ref_ptr<MagneticField> fr = new MagneticFieldGrid(gr);
+		std::vector<Particle> ps = make_particles(128, 46, 1 * EeV, 100 * EeV, centre, proton, one);
+		PropagationBP bp(fr, 0.1 * Mpc);
+		run_traj("bp_fixed_grid32_reflective", bp, nullptr, ps, 600, {1, 10, 100, 600},
+		         "{\"propagator\": \"BP\", \"fixedStep\": " + fmt(0.1 * Mpc) +
+		             ", \"field\": \"grid32\", \"reflective\": true, \"particle_seed\": 46, \"Emin\": " + fmt(EeV) +
+		             ", \"Emax\": " + fmt(100 * EeV) + "}");
+	}
+	{
+		ref_ptr<Grid3f> gc = new Grid3f(*g32);
+		gc->setClipVolume(true);
+		ref_ptr<MagneticField> fc = new MagneticFieldGrid(gc);
+		std::vector<Particle> ps = make_particles(128, 47, 1 * EeV, 100 * EeV, centre, proton, one);
+		PropagationCK ck(fc, 1e-4, 1 * kpc, 1 * Mpc);
+		MaximumTrajectoryLength ml(10 * Mpc);
+		run_traj("ck_grid32_clip", ck, &ml, ps, 3000, {1, 10, 100, 1000, 3000},
+		         "{\"propagator\": \"CK\", \"tolerance\": 1e-4, \"minStep\": " + fmt(kpc) +
+		             ", \"maxStep\": " + fmt(Mpc) + ", \"maxTrajectoryLength\": " + fmt(10 * Mpc) +
+		             ", \"field\": \"grid32\", \"clip\": true, \"particle_seed\": 47, \"Emin\": " + fmt(EeV) +
+		             ", \"Emax\": " + fmt(100 * EeV) + "}");
+	}
+
 	// ---- C1-shaped: 64^3 grid (50 kpc), 4096 protons, BP fixed 0.1 Mpc x 2000
 	{
 		const int n64 = 64;

@@ -292,7 +292,8 @@ def field_for_case(case):
     if f == "grid32":
         g = load_case("grid32")
         gm = g["meta"]["grid"]
-        return OracleField.grid(g["grid"], gm["origin"], gm["spacing"])
+        return OracleField.grid(g["grid"], gm["origin"], gm["spacing"], reflective=meta.get("reflective", False),
+                                clip=meta.get("clip", False))
     if isinstance(f, dict) and f["type"] == "planewave":
         return OracleField.planewave(planewave_tables(f))
     if isinstance(f, dict) and f["type"] == "uniform":

@@ -20,9 +20,11 @@ from crpropa3_amd import capi
 pytestmark = pytest.mark.gpu
 
 TRAJ_CASES = ["bp_fixed_grid32", "bp_adaptive_grid32", "ck_grid32_maxlen", "ck_fixed_grid32",
-              "bp_fixed_planewave32", "ck_planewave32", "bp_fixed_uniform", "ck_uniform", "bp_fixed_grid64_c1"]
+              "bp_fixed_planewave32", "ck_planewave32", "bp_fixed_uniform", "ck_uniform", "bp_fixed_grid64_c1",
+              "bp_fixed_grid32_reflective", "ck_grid32_clip"]
 # cases without a transcendental on the path: bit-exact against the reference itself
-REF_EXACT = {"bp_fixed_grid32", "ck_fixed_grid32", "bp_fixed_uniform", "bp_fixed_grid64_c1"}
+REF_EXACT = {"bp_fixed_grid32", "ck_fixed_grid32", "bp_fixed_uniform", "bp_fixed_grid64_c1",
+             "bp_fixed_grid32_reflective"}
 
 
 def bits_equal(a, b):
@@ -37,7 +39,8 @@ def device_field(case):
     if f == "grid32":
         g = O.load_case("grid32")
         gm = g["meta"]["grid"]
-        return capi.Field.grid3f(g["grid"], gm["origin"], gm["spacing"])
+        return capi.Field.grid3f(g["grid"], gm["origin"], gm["spacing"], reflective=meta.get("reflective", False),
+                                 clip_volume=meta.get("clip", False))
     if isinstance(f, dict) and f["type"] == "planewave":
         t = O.planewave_tables(f)
         return capi.Field.planewave(t["k"], t["Ak"], t["xi"], t["kappa"], t["beta"])
@@ -110,7 +113,8 @@ def test_uniform_and_null_field():
 
 
 # ------------------------------------------------------------------ trajectories
-@pytest.mark.parametrize("name", TRAJ_CASES + ["bp_fixed_grid32:points", "ck_grid32_maxlen:points"])
+@pytest.mark.parametrize("name", TRAJ_CASES + ["bp_fixed_grid32:points", "ck_grid32_maxlen:points",
+                                  "ck_grid32_clip:points"])
 def test_trajectory_parity(name):
     name, _, layout = name.partition(":")
     c = O.load_case(name)

@@ -15,8 +15,9 @@ import pytest
 import oracle_ctypes as O
 
 TRAJ_CASES = ["bp_fixed_grid32", "bp_adaptive_grid32", "ck_grid32_maxlen", "ck_fixed_grid32",
-              "bp_fixed_planewave32", "ck_planewave32", "bp_fixed_uniform", "ck_uniform", "bp_fixed_grid64_c1"]
-NO_TRANSCENDENTAL = {"bp_fixed_grid32", "ck_fixed_grid32", "bp_fixed_uniform", "bp_fixed_grid64_c1"}
+              "bp_fixed_planewave32", "ck_planewave32", "bp_fixed_uniform", "ck_uniform", "bp_fixed_grid64_c1", "bp_fixed_grid32_reflective", "ck_grid32_clip"]
+NO_TRANSCENDENTAL = {"bp_fixed_grid32", "ck_fixed_grid32", "bp_fixed_uniform", "bp_fixed_grid64_c1",
+                     "bp_fixed_grid32_reflective"}
 
 
 def bits_equal(a, b):
