@@ -342,16 +342,20 @@ def main():
     capi.check(capi.lib().crp_device_synchronize())
     acc = evals = launches = 0
     kms = 0.0
-    t0 = time.perf_counter()
+    dt = 0.0
     for _ in range(args.steps):
+        # the batch is reset to its initial state before each step, outside the timed region
+        # (crp_batch_copy synchronises its stream); each step is bracketed by device synchronisation
         work.copy_from(init)
+        capi.check(capi.lib().crp_device_synchronize())
+        t0 = time.perf_counter()
         st = prop.propagate(work, max_steps=max_steps)
+        capi.check(capi.lib().crp_device_synchronize())
+        dt += time.perf_counter() - t0
         acc += st.accepted_steps
         evals += st.field_evals
         kms += st.kernel_ms
         launches += st.launches
-    capi.check(capi.lib().crp_device_synchronize())
-    dt = time.perf_counter() - t0
 
     kern_s = kms / 1e3 / launches
     per_trial = 6 if w["kind"] == "CK" else (1 if w["min_step"] == w["max_step"] else 3)

@@ -34,7 +34,7 @@ class RunStats(C.Structure):
 class HostState(C.Structure):
     _fields_ = [("pos", _dp), ("dir", _dp), ("energy", _dp), ("charge", _dp), ("current_step", _dp),
                 ("next_step", _dp), ("trajectory_length", _dp), ("time", _dp), ("active", _ip),
-                ("prev_pos", _dp), ("prev_dir", _dp)]
+                ("prev_pos", _dp), ("prev_dir", _dp), ("detection_index", _ip)]
 
 
 class LaunchConfig(C.Structure):
@@ -79,6 +79,8 @@ def lib():
         L.crp_propagator_set_max_trajectory_length.argtypes = [_vp, C.c_double]
         L.crp_propagator_set_precision.argtypes = [_vp, C.c_int]
         L.crp_propagator_destroy.argtypes = [_vp]
+        L.crp_propagator_set_observer.argtypes = [_vp, _dp, C.c_int, C.c_int]
+        L.crp_batch_detections.argtypes = [_vp, C.c_int, _dp]
         L.crp_propagator_set_launch.argtypes = [_vp, C.POINTER(LaunchConfig)]
         L.crp_propagator_get_launch.argtypes = [_vp, C.POINTER(LaunchConfig)]
         L.crp_propagator_autotune.argtypes = [_vp, _vp, C.c_longlong, C.POINTER(LaunchConfig)]
@@ -216,6 +218,14 @@ class Propagator:
         check(lib().crp_propagate(self.handle, batch.handle, max_steps, C.byref(st)))
         return st
 
+    def set_observer(self, times, deactivate=False):
+        """Fused Observer(ObserverTimeEvolution(times)) after the propagator and the length limit;
+        times=() removes it.  Detections are read with Batch.detections(len(times))."""
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        check(lib().crp_propagator_set_observer(self.handle, _p(t) if t.size else None, int(t.size),
+                                                1 if deactivate else 0))
+        return self
+
     def set_launch(self, block=256, blocks_per_cu=0, steps_per_launch=0, use_graph=False):
         cfg = LaunchConfig(block, blocks_per_cu, steps_per_launch, 1 if use_graph else 0, 0.0)
         check(lib().crp_propagator_set_launch(self.handle, C.byref(cfg)))
@@ -253,7 +263,7 @@ class Batch:
         return n.value
 
     def upload(self, pos, dir, energy, charge, current_step=None, next_step=None, trajectory_length=None,
-               time=None, active=None):
+               time=None, active=None, detection_index=None):
         pos = _f64(pos, (-1, 3))
         n = pos.shape[0]
         arrs = {
@@ -268,6 +278,10 @@ class Batch:
         for k, a in arrs.items():
             setattr(s, k, _p(a))
         s.active = _p(act, _ip)
+        det = None
+        if detection_index is not None:
+            det = np.ascontiguousarray(detection_index, dtype=np.int32)
+            s.detection_index = _p(det, _ip)
         check(lib().crp_batch_upload(self.handle, n, C.byref(s)))
 
     def download(self):
@@ -276,12 +290,24 @@ class Batch:
                "current_step": np.zeros(n), "next_step": np.zeros(n), "trajectory_length": np.zeros(n),
                "time": np.zeros(n), "prev_pos": np.zeros((n, 3)), "prev_dir": np.zeros((n, 3))}
         act = np.zeros(n, dtype=np.int32)
+        det = np.zeros(n, dtype=np.int32)
         s = HostState()
         for k, a in out.items():
             setattr(s, k, _p(a))
         s.active = _p(act, _ip)
+        s.detection_index = _p(det, _ip)
         check(lib().crp_batch_download(self.handle, C.byref(s)))
         out["active"] = act
+        out["detection_index"] = det
+        return out
+
+    def detections(self, ntimes):
+        """(ntimes, n, 9) array: state at each observer detection (position, direction, trajectory
+        length, time, nextStep); NaN for detections that did not happen."""
+        n = len(self)
+        out = np.full((ntimes, n, 9), np.nan)
+        if n and ntimes:
+            check(lib().crp_batch_detections(self.handle, ntimes, _p(out)))
         return out
 
     def copy_from(self, other):

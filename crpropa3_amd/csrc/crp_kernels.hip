@@ -71,6 +71,9 @@ struct crp_propagator {
 	int useGraph;
 	mutable unsigned lastGrid, lastBlock;
 	mutable double lastPPL;
+	// fused Observer(ObserverTimeEvolution): detection lengths on the device
+	double *obsT;
+	int nObs, obsDeactivate;
 };
 
 
@@ -85,6 +88,9 @@ struct crp_batch {
 	uint32_t *order; // queue order: particle indices, longest expected work first
 	unsigned long long *ctr;
 	hipEvent_t ev0, ev1;
+	int32_t *det;     // observer detections per particle (inside the main allocation)
+	double *snap;     // detection snapshots [snapK][kSnapDoubles][cap], allocated on first use
+	int snapK;
 	// captured launch sequence of the last graph-enabled call, keyed by the exact bytes of every
 	// kernel argument and launch dimension it holds
 	hipGraphExec_t gexec;
@@ -102,6 +108,9 @@ static Soa soa_of(const crp_batch *b) {
 	s.pdx = b->pdx; s.pdy = b->pdy; s.pdz = b->pdz;
 	s.active = b->active;
 	s.order = b->order;
+	s.det = b->det;
+	s.snap = b->snap;
+	s.cap = b->cap;
 	return s;
 }
 
@@ -821,6 +830,9 @@ static int create_prop(int type, const char *who, const crp_field_t *field, doub
 	p->useGraph = 0;
 	p->lastGrid = p->lastBlock = 0;
 	p->lastPPL = 0;
+	p->obsT = nullptr;
+	p->nObs = 0;
+	p->obsDeactivate = 0;
 	*out = p;
 	return CRP_OK;
 }
@@ -851,7 +863,31 @@ int crp_propagator_set_precision(crp_propagator_t *p, int precision) {
 	return CRP_OK;
 }
 
+int crp_propagator_set_observer(crp_propagator_t *p, const double *times, int ntimes, int deactivate) {
+	if (!p)
+		return fail(CRP_E_INVALID, "crp_propagator_set_observer: NULL propagator");
+	if (ntimes < 0 || (ntimes > 0 && !times))
+		return fail(CRP_E_INVALID, "crp_propagator_set_observer: bad time list");
+	if (p->obsT) {
+		CRP_HIP_TRY(hipDeviceSynchronize());
+		CRP_HIP_TRY(hipFree(p->obsT));
+		p->obsT = nullptr;
+	}
+	p->nObs = 0;
+	p->obsDeactivate = deactivate ? 1 : 0;
+	if (ntimes == 0)
+		return CRP_OK;
+	CRP_HIP_TRY(hipMalloc((void **)&p->obsT, (size_t)ntimes * sizeof(double)));
+	CRP_HIP_TRY(hipMemcpy(p->obsT, times, (size_t)ntimes * sizeof(double), hipMemcpyHostToDevice));
+	p->nObs = ntimes;
+	return CRP_OK;
+}
+
 int crp_propagator_destroy(crp_propagator_t *p) {
+	if (p && p->obsT) {
+		(void)hipDeviceSynchronize();
+		(void)hipFree(p->obsT);
+	}
 	delete p;
 	return CRP_OK;
 }
@@ -868,7 +904,7 @@ int crp_batch_create(size_t capacity, crp_batch_t **out) {
 	// one allocation: 18 double arrays, the active flags, the queue order, then a
 	// 256-byte aligned counter block
 	const size_t nd = 18;
-	size_t off = nd * capacity * sizeof(double) + capacity * sizeof(int32_t) + capacity * sizeof(uint32_t);
+	size_t off = nd * capacity * sizeof(double) + 2 * capacity * sizeof(int32_t) + capacity * sizeof(uint32_t);
 	off = (off + 255) & ~(size_t)255;
 	const size_t bytes = off + 64 * sizeof(unsigned long long);
 	hipError_t e = hipMalloc((void **)&b->base, bytes);
@@ -882,6 +918,7 @@ int crp_batch_create(size_t capacity, crp_batch_t **out) {
 		*arr[i] = b->base + i * capacity;
 	b->active = (int32_t *)(b->base + nd * capacity);
 	b->order = (uint32_t *)(b->active + capacity);
+	b->det = (int32_t *)(b->order + capacity);
 	b->ctr = (unsigned long long *)((char *)b->base + off);
 	e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
 	if (e == hipSuccess)
@@ -907,6 +944,8 @@ int crp_batch_destroy(crp_batch_t *b) {
 	(void)hipEventDestroy(b->ev1);
 	(void)hipStreamDestroy(b->stream);
 	(void)hipFree(b->base);
+	if (b->snap)
+		(void)hipFree(b->snap);
 	delete b;
 	return CRP_OK;
 }
@@ -947,6 +986,12 @@ int crp_batch_upload(crp_batch_t *b, size_t n, const crp_host_state *s) {
 		}
 	}
 	CRP_HIP_TRY(hipMemcpy(b->active, s->active, n * sizeof(int32_t), hipMemcpyHostToDevice));
+	{
+		std::vector<int32_t> det(n, 0);
+		if (s->detection_index)
+			std::copy(s->detection_index, s->detection_index + n, det.begin());
+		CRP_HIP_TRY(hipMemcpy(b->det, det.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+	}
 	// Queue order: ascending magnetic rigidity E/|q| (neutral particles last).  Low-rigidity
 	// particles take the most adaptive steps, so starting them first shortens the tail of a
 	// run (longest-processing-time-first scheduling); results do not depend on the order.
@@ -994,6 +1039,46 @@ int crp_batch_download(const crp_batch_t *b, crp_host_state *s) {
 			CRP_HIP_TRY(hipMemcpy(dst1[c], src1[c], n * sizeof(double), hipMemcpyDeviceToHost));
 	if (s->active)
 		CRP_HIP_TRY(hipMemcpy(s->active, b->active, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+	if (s->detection_index)
+		CRP_HIP_TRY(hipMemcpy(s->detection_index, b->det, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+	return CRP_OK;
+}
+
+static int ensure_snapshots(crp_batch *b, int k) {
+	if (b->snap && b->snapK >= k)
+		return CRP_OK;
+	double *ns = nullptr;
+	CRP_HIP_TRY(hipMalloc((void **)&ns, (size_t)k * kSnapDoubles * b->cap * sizeof(double)));
+	if (b->snap) {
+		CRP_HIP_TRY(hipMemcpy(ns, b->snap, (size_t)b->snapK * kSnapDoubles * b->cap * sizeof(double),
+		                      hipMemcpyDeviceToDevice));
+		CRP_HIP_TRY(hipFree(b->snap));
+	}
+	b->snap = ns;
+	b->snapK = k;
+	return CRP_OK;
+}
+
+int crp_batch_detections(const crp_batch_t *b, int ntimes, double *out) {
+	if (!b || (ntimes > 0 && !out) || ntimes < 0)
+		return fail(CRP_E_INVALID, "crp_batch_detections: bad argument");
+	const size_t n = b->n;
+	if (n == 0 || ntimes == 0)
+		return CRP_OK;
+	CRP_HIP_TRY(hipStreamSynchronize(b->stream));
+	std::vector<int32_t> det(n);
+	CRP_HIP_TRY(hipMemcpy(det.data(), b->det, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+	std::fill(out, out + (size_t)ntimes * n * kSnapDoubles, NAN);
+	const int kk = std::min(ntimes, b->snap ? b->snapK : 0);
+	std::vector<double> col(n);
+	for (int k = 0; k < kk; k++)
+		for (int c = 0; c < kSnapDoubles; c++) {
+			CRP_HIP_TRY(hipMemcpy(col.data(), b->snap + ((size_t)k * kSnapDoubles + c) * b->cap, n * sizeof(double),
+			                      hipMemcpyDeviceToHost));
+			for (size_t i = 0; i < n; i++)
+				if (k < det[i])
+					out[((size_t)k * n + i) * kSnapDoubles + c] = col[i];
+		}
 	return CRP_OK;
 }
 
@@ -1017,6 +1102,15 @@ int crp_batch_copy(crp_batch_t *dst, const crp_batch_t *src) {
 	CRP_HIP_TRY(hipGetLastError());
 	CRP_HIP_TRY(hipMemcpyAsync(dst->active, src->active, n * sizeof(int32_t), hipMemcpyDeviceToDevice, dst->stream));
 	CRP_HIP_TRY(hipMemcpyAsync(dst->order, src->order, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, dst->stream));
+	CRP_HIP_TRY(hipMemcpyAsync(dst->det, src->det, n * sizeof(int32_t), hipMemcpyDeviceToDevice, dst->stream));
+	if (src->snap) {
+		int rc = ensure_snapshots(dst, src->snapK);
+		if (rc != CRP_OK)
+			return rc;
+		for (int r = 0; r < src->snapK * kSnapDoubles; r++)
+			CRP_HIP_TRY(hipMemcpyAsync(dst->snap + (size_t)r * dst->cap, src->snap + (size_t)r * src->cap,
+			                           n * sizeof(double), hipMemcpyDeviceToDevice, dst->stream));
+	}
 	CRP_HIP_TRY(hipStreamSynchronize(dst->stream));
 	return CRP_OK;
 }
@@ -1039,6 +1133,14 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 	P.maxStep = p->maxStep;
 	P.maxLen = p->maxLength;
 	P.maxSteps = max_steps;
+	P.obsT = p->obsT;
+	P.nObs = p->nObs;
+	P.obsDeactivate = p->obsDeactivate;
+	if (p->nObs > 0) {
+		int rc0 = ensure_snapshots(b, p->nObs);
+		if (rc0 != CRP_OK)
+			return rc0;
+	}
 	CRP_HIP_TRY(hipEventRecord(b->ev0, b->stream));
 	int rc;
 	const crp_field *f = p->field;

@@ -23,18 +23,26 @@ namespace crp {
 enum PropKind { PK_BP_FIXED = 0, PK_BP_ADAPTIVE = 1, PK_CK_FIXED = 2, PK_CK_ADAPTIVE = 3 };
 // CTR_PAUSED: particles stored still active because this launch's step budget ran out
 enum { CTR_QUEUE = 0, CTR_ACCEPTED, CTR_REJECTED, CTR_EVALS, CTR_FINISHED, CTR_PAUSED, CTR_COUNT };
-constexpr int kColdDoubles = 9; // dynamic LDS per thread of k_propagate, in doubles
+constexpr int kColdDoubles = 10; // dynamic LDS per thread of k_propagate, in doubles
 
 struct Soa {
 	double *px, *py, *pz, *dx, *dy, *dz, *E, *q, *cur, *next, *len, *time;
 	double *ppx, *ppy, *ppz, *pdx, *pdy, *pdz;
 	int32_t *active;
 	const uint32_t *order;
+	int32_t *det;             // observer detections so far per particle ("DetectionIndex")
+	double *snap;             // state at detection k of particle i: snap[(k * kSnapDoubles + c) * cap + i]
+	unsigned long long cap;   // snapshot stride
 };
+
+constexpr int kSnapDoubles = 9; // position x,y,z, direction x,y,z, trajectory length, time, nextStep
 
 struct Params {
 	double tol, minStep, maxStep, maxLen;
 	long long maxSteps; // < 0: unlimited
+	// Observer with one ObserverTimeEvolution after the length limit (nObs = 0: none)
+	const double *obsT;
+	int nObs, obsDeactivate;
 };
 
 __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; } // std::max
@@ -58,8 +66,20 @@ __device__ __forceinline__ double pow_m02(double r) {
 	return crp_pow_cr(r, -0.2);
 }
 
+// Register budget: the FP64 path through the cell table is asked for 3 waves per SIMD (<= 168
+// VGPRs; it needs 169 unconstrained and then runs at 2), everything else is left to the compiler.
+template <class F>
+struct WavesHint {
+	static constexpr int fp64 = 1;
+};
+template <>
+struct WavesHint<FieldGridCells> {
+	static constexpr int fp64 = 3;
+};
+
 template <class F, int KIND, bool MIX>
-__global__ void __launch_bounds__(256) k_propagate(F field, Params P, Soa S, unsigned long long n,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIX ? 1 : WavesHint<F>::fp64, 8)))
+k_propagate(F field, Params P, Soa S, unsigned long long n,
                                                    unsigned long long *__restrict__ ctr) {
 	constexpr bool kCK = (KIND == PK_CK_FIXED || KIND == PK_CK_ADAPTIVE);
 	constexpr bool kAdaptive = (KIND == PK_BP_ADAPTIVE || KIND == PK_CK_ADAPTIVE);
@@ -84,6 +104,7 @@ __global__ void __launch_bounds__(256) k_propagate(F field, Params P, Soa S, uns
 	double *const cPPx = cw, *const cPPy = cw + 64, *const cPPz = cw + 128;
 	double *const cPDx = cw + 192, *const cPDy = cw + 256, *const cPDz = cw + 320;
 	double *const cCur = cw + 384, *const cLen = cw + 448, *const cTim = cw + 512;
+	double *const cDet = cw + 576; // observer detection count (small integer held as a double)
 
 	for (;;) {
 		if (nEval > 0x80000000u) { // flush this lane's counters long before they wrap
@@ -117,6 +138,8 @@ __global__ void __launch_bounds__(256) k_propagate(F field, Params P, Soa S, uns
 						next = S.next[idx];
 						*cLen = S.len[idx];
 						*cTim = S.time[idx];
+						if (P.nObs > 0)
+							*cDet = (double)S.det[idx];
 						steps = 0;
 						inProc = false;
 					}
@@ -281,6 +304,30 @@ __global__ void __launch_bounds__(256) k_propagate(F field, Params P, Soa S, uns
 				next = dmin(next, P.maxLen - len);
 			}
 		}
+		if (P.nObs > 0) {
+			// Observer::process with ObserverTimeEvolution::checkDetection (src/module/Observer.cpp:26-57,
+			// 279-306): at most one detection per step; the next step is limited towards the next time
+			const int k = (int)*cDet;
+			if (k < P.nObs) {
+				const double distance = len - P.obsT[k];
+				if (distance < 0.) {
+					next = dmin(next, -distance);
+				} else {
+					if (k < P.nObs - 2)
+						next = dmin(next, P.obsT[k + 1] - len);
+					*cDet = (double)(k + 1);
+					double *r = S.snap + (size_t)k * kSnapDoubles * S.cap + idx;
+					const V3 d = MIX ? v3(dirf.x, dirf.y, dirf.z) : dir;
+					r[0] = pos.x; r[S.cap] = pos.y; r[2 * S.cap] = pos.z;
+					r[3 * S.cap] = d.x; r[4 * S.cap] = d.y; r[5 * S.cap] = d.z;
+					r[6 * S.cap] = len; r[7 * S.cap] = *cTim; r[8 * S.cap] = next;
+					if (P.obsDeactivate && alive) {
+						alive = false;
+						nFin++;
+					}
+				}
+			}
+		}
 		if (!alive || (P.maxSteps >= 0 && steps >= P.maxSteps)) {
 			if (MIX)
 				dir = v3(dirf.x, dirf.y, dirf.z);
@@ -290,6 +337,8 @@ __global__ void __launch_bounds__(256) k_propagate(F field, Params P, Soa S, uns
 			S.active[idx] = alive ? 1 : 0;
 			S.ppx[idx] = *cPPx; S.ppy[idx] = *cPPy; S.ppz[idx] = *cPPz;
 			S.pdx[idx] = *cPDx; S.pdy[idx] = *cPDy; S.pdz[idx] = *cPDz;
+			if (P.nObs > 0)
+				S.det[idx] = (int32_t)*cDet;
 			nPause += alive ? 1 : 0;
 			have = false;
 		}

@@ -8,6 +8,7 @@
 
 #include "crpropa3_amd.h"
 
+#include <cstring>
 #include <iostream>
 #include <memory>
 #include <sstream>
@@ -35,6 +36,7 @@ typedef std::unique_ptr<crp_batch, BatchDeleter> BatchHandle;
 struct Packed {
 	std::vector<double> pos, dir, E, q, cur, next, len, time, ppos, pdir;
 	std::vector<int32_t> active;
+	std::vector<int32_t> det; // observer detections per candidate (empty: none tracked)
 	void resize(size_t n) {
 		pos.resize(3 * n); dir.resize(3 * n); ppos.resize(3 * n); pdir.resize(3 * n);
 		E.resize(n); q.resize(n); cur.resize(n); next.resize(n); len.resize(n); time.resize(n);
@@ -42,9 +44,11 @@ struct Packed {
 	}
 	crp_host_state view() {
 		crp_host_state s;
+		std::memset(&s, 0, sizeof(s));
 		s.pos = pos.data(); s.dir = dir.data(); s.energy = E.data(); s.charge = q.data();
 		s.current_step = cur.data(); s.next_step = next.data(); s.trajectory_length = len.data();
 		s.time = time.data(); s.active = active.data(); s.prev_pos = ppos.data(); s.prev_dir = pdir.data();
+		s.detection_index = det.empty() ? nullptr : det.data();
 		return s;
 	}
 };

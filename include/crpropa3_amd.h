@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define CRP_ABI_VERSION 1
+#define CRP_ABI_VERSION 2
 
 enum crp_status {
 	CRP_OK = 0,
@@ -141,6 +141,8 @@ typedef struct {
 	int32_t *active;           /* [n] */
 	double *prev_pos;          /* [3n] previous.position, written on download (may be NULL) */
 	double *prev_dir;          /* [3n] previous.direction (may be NULL) */
+	int32_t *detection_index;  /* [n] observer detections so far, the candidate property
+	                              "DetectionIndex" of src/module/Observer.cpp:282-290 (may be NULL: 0) */
 } crp_host_state;
 
 int crp_batch_create(size_t capacity, crp_batch_t **out);
@@ -156,6 +158,19 @@ int crp_batch_copy(crp_batch_t *dst, const crp_batch_t *src);
  * at most max_steps process() calls per active particle (max_steps < 0: until every particle is
  * inactive, which needs a trajectory-length limit and minStep > 0).  Synchronous. */
 int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps, crp_run_stats *stats);
+
+/* Observer with one ObserverTimeEvolution, fused after the propagator and the length limit
+ * (src/module/Observer.cpp:26-57, 266-306): after each accepted step a particle whose trajectory
+ * length reached times[k] (k = its detections so far) is detected (at most once per step), its state
+ * is recorded as detection k, and nextStep is limited towards times[k+1] (not for the last two
+ * times, as the reference); otherwise nextStep is limited to land on times[k].  deactivate != 0
+ * deactivates on detection (Observer::setDeactivateOnDetection, default true in the reference).
+ * ntimes = 0 removes the observer. */
+int crp_propagator_set_observer(crp_propagator_t *p, const double *times, int ntimes, int deactivate);
+/* Recorded detections: out[(k * n + i) * 9 + c] for k < ntimes, c = position x,y,z, direction x,y,z,
+ * trajectory length, time, nextStep after the observer; NaN where particle i has fewer than k+1
+ * detections. */
+int crp_batch_detections(const crp_batch_t *b, int ntimes, double *out);
 
 /* ---- launch configuration, hipGraph replay and autotuning ------------------
  * No reference counterpart (the reference loops over candidates with OpenMP,

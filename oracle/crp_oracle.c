@@ -523,6 +523,32 @@ static inline void orc_process(const orc_field *f, const orc_propagator *p, orc_
 		else
 			s->next_step[i] = orc_min(s->next_step[i], p->maxTrajectoryLength - len);
 	}
+	/* Observer::process with ObserverTimeEvolution::checkDetection (src/module/Observer.cpp:26-57,
+	 * 279-306): one detection per call at most; limitNextStep towards the next time */
+	if (p->nObs > 0) {
+		int k = s->det_index[i];
+		if (k < p->nObs) {
+			double len = s->trajectory_length[i];
+			double distance = len - p->obsTimes[k];
+			if (distance < 0.) {
+				s->next_step[i] = orc_min(s->next_step[i], -distance);
+			} else {
+				if (k < p->nObs - 2)
+					s->next_step[i] = orc_min(s->next_step[i], p->obsTimes[k + 1] - len);
+				s->det_index[i] = k + 1;
+				double *r = s->snap + ((size_t)k * s->n + i) * 9;
+				for (int a = 0; a < 3; a++) {
+					r[a] = pos[a];
+					r[3 + a] = dir[a];
+				}
+				r[6] = len;
+				r[7] = s->time[i];
+				r[8] = s->next_step[i];
+				if (p->obsDeactivate)
+					s->active[i] = 0;
+			}
+		}
+	}
 }
 
 long long orc_propagate(const orc_field *f, const orc_propagator *p, orc_state *s, size_t n, long long maxSteps,

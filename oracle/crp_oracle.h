@@ -44,6 +44,10 @@ typedef struct {
 	int type; /* ORC_PROP_BP / ORC_PROP_CK */
 	double tolerance, minStep, maxStep;
 	double maxTrajectoryLength; /* <= 0: no MaximumTrajectoryLength module */
+	/* Observer with one ObserverTimeEvolution after the length limit (nObs = 0: none):
+	 * detection lengths obsTimes[nObs], deactivate on detection */
+	const double *obsTimes;
+	int nObs, obsDeactivate;
 } orc_propagator;
 
 /* Particle state, structure of arrays.  pos/dir are AoS (3 per particle). */
@@ -53,6 +57,12 @@ typedef struct {
 	double *current_step, *next_step, *trajectory_length, *time;
 	int32_t *active;
 	double *prev_pos, *prev_dir; /* may be NULL */
+	/* observer: detections so far per particle ("DetectionIndex"), and the state at detection k
+	 * of particle i: snap[(k * n + i) * 9 + c], c = pos x,y,z, dir x,y,z, length, time, nextStep
+	 * (NULL when the propagator has no observer) */
+	int32_t *det_index;
+	double *snap;
+	size_t n; /* particles in the state (snapshot stride) */
 } orc_state;
 
 /* MT19937 as in src/Random.cpp */

@@ -12,6 +12,7 @@
 //   PropagationCK::process                               src/module/PropagationCK.cpp:77-136
 //   MaximumTrajectoryLength::process                     src/module/BreakCondition.cpp:42-64
 //   Random (MT19937, randUniform, randVector, randPowerLaw, randBin) src/Random.cpp:80-205
+//   Observer + ObserverTimeEvolution (detections, limitNextStep)  src/module/Observer.cpp:26-57,266-306
 //   GridTools: scaleGrid, mean/rms field strength, loadGrid/dumpGrid (raw float32 and text),
 //   loadGrid3fFromTxt, fromMagneticField                 src/GridTools.cpp:17-236,238-447
 //
@@ -32,6 +33,7 @@
 #include "crpropa/magneticField/MagneticFieldGrid.h"
 #include "crpropa/magneticField/turbulentField/PlaneWaveTurbulence.h"
 #include "crpropa/module/BreakCondition.h"
+#include "crpropa/module/Observer.h"
 #include "crpropa/module/PropagationBP.h"
 #include "crpropa/module/PropagationCK.h"
 
@@ -248,6 +250,90 @@ static void run_traj(const std::string &name, const Module &prop, const MaximumT
 }
 
 // ---------------------------------------------------------------- cases
+// ---------------------------------------------------------------- observers
+// Records every detection the Observer reports: particle (property "pid"),
+// state at detection.  One vector per particle, so OpenMP threads never share one.
+class DetectionRecorder : public Module {
+public:
+	struct Rec {
+		Vector3d pos, dir;
+		double len, time, next;
+	};
+	std::vector<std::vector<Rec>> *per;
+	void process(Candidate *c) const {
+		size_t pid = c->getProperty("pid").asUInt64();
+		(*per)[pid].push_back(Rec{c->current.getPosition(), c->current.getDirection(), c->getTrajectoryLength(),
+		                          c->getTime(), c->getNextStep()});
+	}
+};
+
+// ModuleList order [propagator, MaximumTrajectoryLength (optional), Observer(ObserverTimeEvolution)].
+static void run_observer_case(const std::string &name, const Module &prop, const MaximumTrajectoryLength *maxlen,
+                              ObserverTimeEvolution *ote, bool deactivate, const std::vector<Particle> &ps,
+                              size_t maxSteps, const std::string &meta) {
+	std::string dir = casedir(name);
+	save_particles(dir, ps);
+	const size_t N = ps.size();
+	std::vector<std::vector<DetectionRecorder::Rec>> per(N);
+	ref_ptr<DetectionRecorder> rec = new DetectionRecorder();
+	rec->per = &per;
+	ref_ptr<Observer> obs = new Observer();
+	obs->add(ote);
+	obs->setDeactivateOnDetection(deactivate);
+	obs->onDetection(rec);
+	std::vector<double> fpos(3 * N), fdir(3 * N), flen(N), fnext(N);
+	std::vector<int32_t> fact(N), nsteps(N);
+#pragma omp parallel for schedule(dynamic, 4)
+	for (size_t i = 0; i < N; i++) {
+		ParticleState st(ps[i].id, ps[i].E, ps[i].pos, ps[i].dir);
+		ref_ptr<Candidate> c = new Candidate(st);
+		c->setProperty("pid", Variant::fromUInt64(i));
+		size_t step = 0;
+		while (step < maxSteps && c->isActive()) {
+			prop.process(c);
+			if (maxlen) maxlen->process(c);
+			obs->process(c);
+			step++;
+		}
+		Vector3d p = c->current.getPosition(), u = c->current.getDirection();
+		fpos[3 * i] = p.x; fpos[3 * i + 1] = p.y; fpos[3 * i + 2] = p.z;
+		fdir[3 * i] = u.x; fdir[3 * i + 1] = u.y; fdir[3 * i + 2] = u.z;
+		flen[i] = c->getTrajectoryLength();
+		fnext[i] = c->getNextStep();
+		fact[i] = c->isActive() ? 1 : 0;
+		nsteps[i] = (int32_t)step;
+	}
+	const std::vector<double> &T = ote->getTimes();
+	const size_t K = T.size();
+	std::vector<double> dpos(K * N * 3, NAN), ddir(K * N * 3, NAN), dlen(K * N, NAN), dtim(K * N, NAN), dnext(K * N, NAN);
+	std::vector<int32_t> count(N);
+	for (size_t i = 0; i < N; i++) {
+		count[i] = (int32_t)per[i].size();
+		for (size_t k = 0; k < per[i].size() && k < K; k++) {
+			const DetectionRecorder::Rec &r = per[i][k];
+			size_t o = k * N + i;
+			dpos[3 * o] = r.pos.x; dpos[3 * o + 1] = r.pos.y; dpos[3 * o + 2] = r.pos.z;
+			ddir[3 * o] = r.dir.x; ddir[3 * o + 1] = r.dir.y; ddir[3 * o + 2] = r.dir.z;
+			dlen[o] = r.len; dtim[o] = r.time; dnext[o] = r.next;
+		}
+	}
+	save_f64(dir, "times", T, {K});
+	save_f64(dir, "det_pos", dpos, {K, N, 3});
+	save_f64(dir, "det_dir", ddir, {K, N, 3});
+	save_f64(dir, "det_len", dlen, {K, N});
+	save_f64(dir, "det_time", dtim, {K, N});
+	save_f64(dir, "det_next", dnext, {K, N});
+	save_i32(dir, "det_count", count, {N});
+	save_f64(dir, "final_pos", fpos, {N, 3});
+	save_f64(dir, "final_dir", fdir, {N, 3});
+	save_f64(dir, "final_len", flen, {N});
+	save_f64(dir, "final_next", fnext, {N});
+	save_i32(dir, "final_active", fact, {N});
+	save_i32(dir, "nsteps", nsteps, {N});
+	save_meta(dir, meta);
+	printf("  %-28s N=%zu steps<=%zu detections=%zu\n", name.c_str(), N, maxSteps, K);
+}
+
 static void gen_trilinear() {
 	// 16^3 grid with a non-trivial origin; spacing 50 kpc; amplitude 1 nG.
 	const int n = 16;
@@ -487,6 +573,29 @@ int main(int argc, char **argv) {
 		             ", \"maxStep\": " + fmt(Mpc) + ", \"maxTrajectoryLength\": " + fmt(10 * Mpc) +
 		             ", \"field\": \"grid32\", \"clip\": true, \"particle_seed\": 47, \"Emin\": " + fmt(EeV) +
 		             ", \"Emax\": " + fmt(100 * EeV) + "}");
+	}
+
+	// ---- ObserverTimeEvolution: log-spaced times with a length limit (CK), and an
+	// explicit list with deactivation on detection (fixed-step BP)
+	{
+		std::vector<Particle> ps = make_particles(256, 48, 1 * EeV, 300 * EeV, centre, species, wts);
+		PropagationCK ck(fgrid, 1e-4, 1 * kpc, 1 * Mpc);
+		MaximumTrajectoryLength ml(12 * Mpc);
+		run_observer_case("ck_grid32_observer", ck, &ml, new ObserverTimeEvolution(0.5 * Mpc, 10 * Mpc, 12, true), false,
+		                  ps, 100000,
+		                  "{\"propagator\": \"CK\", \"tolerance\": 1e-4, \"minStep\": " + fmt(kpc) +
+		                      ", \"maxStep\": " + fmt(Mpc) + ", \"maxTrajectoryLength\": " + fmt(12 * Mpc) +
+		                      ", \"field\": \"grid32\", \"observer\": {\"min\": " + fmt(0.5 * Mpc) + ", \"max\": " +
+		                      fmt(10 * Mpc) + ", \"n\": 12, \"log\": true, \"deactivate\": false}, \"particle_seed\": 48}");
+	}
+	{
+		std::vector<Particle> ps = make_particles(128, 49, 1 * EeV, 100 * EeV, centre, proton, one);
+		PropagationBP bp(fgrid, 0.1 * Mpc);
+		std::vector<double> tl = {0.35 * Mpc, 1.0 * Mpc, 1.05 * Mpc, 2.5 * Mpc, 7.77 * Mpc};
+		run_observer_case("bp_fixed_grid32_observer", bp, nullptr, new ObserverTimeEvolution(tl), false, ps, 100,
+		                  "{\"propagator\": \"BP\", \"fixedStep\": " + fmt(0.1 * Mpc) +
+		                      ", \"field\": \"grid32\", \"observer\": {\"list\": true, \"deactivate\": false}, "
+		                      "\"particle_seed\": 49}");
 	}
 
 	// ---- C1-shaped: 64^3 grid (50 kpc), 4096 protons, BP fixed 0.1 Mpc x 2000

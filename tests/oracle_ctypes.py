@@ -52,13 +52,15 @@ class Field(C.Structure):
 
 class Propagator(C.Structure):
     _fields_ = [("type", C.c_int), ("tolerance", C.c_double), ("minStep", C.c_double),
-                ("maxStep", C.c_double), ("maxTrajectoryLength", C.c_double)]
+                ("maxStep", C.c_double), ("maxTrajectoryLength", C.c_double),
+                ("obsTimes", _dp), ("nObs", C.c_int), ("obsDeactivate", C.c_int)]
 
 
 class State(C.Structure):
     _fields_ = [("pos", _dp), ("dir", _dp), ("energy", _dp), ("charge", _dp), ("redshift", _dp),
                 ("current_step", _dp), ("next_step", _dp), ("trajectory_length", _dp), ("time", _dp),
-                ("active", _ip), ("prev_pos", _dp), ("prev_dir", _dp)]
+                ("active", _ip), ("prev_pos", _dp), ("prev_dir", _dp),
+                ("det_index", _ip), ("snap", _dp), ("n", C.c_size_t)]
 
 
 _lib = None
@@ -213,6 +215,8 @@ class OracleState:
         self.active = np.ones(n, dtype=np.int32)
         self.prev_pos = self.pos.copy()
         self.prev_dir = self.dir.copy()
+        self.det_index = np.zeros(n, dtype=np.int32)
+        self.snap = None  # (K, n, 9) once an observer is used
 
     def struct(self):
         s = State()
@@ -220,12 +224,25 @@ class OracleState:
                      "trajectory_length", "time", "prev_pos", "prev_dir"):
             setattr(s, name, ptr(getattr(self, name)))
         s.active = ptr(self.active, _ip)
+        s.det_index = ptr(self.det_index, _ip)
+        s.snap = ptr(self.snap) if self.snap is not None else None
+        s.n = self.n
         return s
 
 
 def propagate(field, prop_type, state, steps, tolerance=0.42, min_step=0.0, max_step=0.0, max_len=0.0,
-              libm=LIBM_GLIBC, nthreads=0):
+              libm=LIBM_GLIBC, nthreads=0, observer_times=None, observer_deactivate=False):
+    """observer_times: detection lengths of an Observer(ObserverTimeEvolution) after the length limit;
+    detections land in state.snap[k, i, :] = (pos, dir, length, time, nextStep)."""
     p = Propagator(prop_type, tolerance, min_step, max_step, max_len)
+    keep = None
+    if observer_times is not None and len(observer_times):
+        keep = np.ascontiguousarray(observer_times, dtype=np.float64)
+        p.obsTimes = ptr(keep)
+        p.nObs = len(keep)
+        p.obsDeactivate = int(bool(observer_deactivate))
+        if state.snap is None or state.snap.shape[0] < len(keep):
+            state.snap = np.full((len(keep), state.n, 9), np.nan)
     s = state.struct()
     trials = C.c_longlong(0)
     f = C.byref(field.s) if field is not None else None

@@ -250,3 +250,29 @@ def test_noise_grid_matches_fixture():
     g = O.load_case("grid32")
     m = g["meta"]["grid"]
     assert np.array_equal(O.noise_grid(m["seed"], m["n"], m["amp"]).view(np.uint32), g["grid"].view(np.uint32))
+
+
+@pytest.mark.parametrize("name", ["ck_grid32_observer", "bp_fixed_grid32_observer"])
+def test_oracle_observer_bit_exact_vs_reference(name):
+    """Observer(ObserverTimeEvolution) after the propagator (and length limit): every detection's state
+    and the final state equal the reference's (src/module/Observer.cpp:26-57,279-306)."""
+    c = O.load_case(name)
+    m = c["meta"]
+    f = O.field_for_case(c)
+    st = O.OracleState(c["init_pos"], c["init_dir"], c["energy"], c["charge"])
+    pa = O.propagator_args(m)
+    O.propagate(f, steps=int(c["nsteps"].max()), max_len=m.get("maxTrajectoryLength", 0.0), libm=O.LIBM_GLIBC,
+                state=st, observer_times=c["times"], observer_deactivate=m["observer"]["deactivate"], **pa)
+    assert np.array_equal(st.det_index, c["det_count"])
+    snap = st.snap
+    for key, col in (("det_pos", slice(0, 3)), ("det_dir", slice(3, 6))):
+        ref = c[key]
+        ok = ~np.isnan(ref)
+        assert bits_equal(snap[:, :, col][ok], ref[ok]), key
+    for key, col in (("det_len", 6), ("det_time", 7), ("det_next", 8)):
+        ref = c[key]
+        ok = ~np.isnan(ref)
+        assert bits_equal(snap[:, :, col][ok], ref[ok]), key
+    assert bits_equal(st.pos, c["final_pos"]) and bits_equal(st.dir, c["final_dir"])
+    assert bits_equal(st.trajectory_length, c["final_len"]) and bits_equal(st.next_step, c["final_next"])
+    assert np.array_equal(st.active, c["final_active"])
