@@ -65,6 +65,9 @@ public:
 	bool hasProperty(const std::string &name) const { return properties.count(name) != 0; }
 	bool removeProperty(const std::string &name) { return properties.erase(name) != 0; }
 
+	/* copy of the state and properties (src/Candidate.cpp:206-230); a new serial number */
+	ref_ptr<Candidate> clone(bool recursive = false) const;
+
 	uint64_t getSerialNumber() const { return serialNumber; }
 	std::string getDescription() const;
 

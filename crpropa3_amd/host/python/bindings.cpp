@@ -21,6 +21,7 @@
 #include "crpropa/magneticField/MagneticFieldGrid.h"
 #include "crpropa/magneticField/turbulentField/PlaneWaveTurbulence.h"
 #include "crpropa/module/BreakCondition.h"
+#include "crpropa/module/Observer.h"
 #include "crpropa/module/PropagationBP.h"
 #include "crpropa/module/PropagationCK.h"
 
@@ -347,6 +348,46 @@ PYBIND11_MODULE(_crpropa, m) {
 	    .def("getMaximumTrajectoryLength", &MaximumTrajectoryLength::getMaximumTrajectoryLength)
 	    .def("addObserverPosition", &MaximumTrajectoryLength::addObserverPosition)
 	    .def("getObserverPositions", &MaximumTrajectoryLength::getObserverPositions);
+
+	// observers (include/crpropa/module/Observer.h)
+	py::enum_<DetectionState>(m, "DetectionState")
+	    .value("DETECTED", DETECTED)
+	    .value("VETO", VETO)
+	    .value("NOTHING", NOTHING)
+	    .export_values();
+	py::class_<ObserverFeature, ref_ptr<ObserverFeature>>(m, "ObserverFeature")
+	    .def("getDescription", &ObserverFeature::getDescription);
+	py::class_<ObserverTimeEvolution, ObserverFeature, ref_ptr<ObserverTimeEvolution>>(m, "ObserverTimeEvolution")
+	    .def(py::init<>())
+	    .def(py::init<double, double, double>(), py::arg("min"), py::arg("dist"), py::arg("numb"))
+	    .def(py::init<double, double, double, bool>(), py::arg("min"), py::arg("max"), py::arg("numb"),
+	         py::arg("log"))
+	    .def(py::init<const std::vector<double> &>())
+	    .def("checkDetection", &ObserverTimeEvolution::checkDetection)
+	    .def("addTime", &ObserverTimeEvolution::addTime)
+	    .def("addTimeRange", &ObserverTimeEvolution::addTimeRange, py::arg("min"), py::arg("max"), py::arg("numb"),
+	         py::arg("log") = false)
+	    .def("setTimes", &ObserverTimeEvolution::setTimes)
+	    .def("getTimes", &ObserverTimeEvolution::getTimes)
+	    .def("getTime", &ObserverTimeEvolution::getTime)
+	    .def("clear", &ObserverTimeEvolution::clear)
+	    .def("empty", &ObserverTimeEvolution::empty)
+	    .def("constructDetListIfEmpty", &ObserverTimeEvolution::constructDetListIfEmpty)
+	    .def("setMinimum", &ObserverTimeEvolution::setMinimum)
+	    .def("setMaximum", &ObserverTimeEvolution::setMaximum)
+	    .def("setNIntervals", &ObserverTimeEvolution::setNIntervals)
+	    .def("setIsLogarithmicScaling", &ObserverTimeEvolution::setIsLogarithmicScaling)
+	    .def("getMinimum", &ObserverTimeEvolution::getMinimum)
+	    .def("getMaximum", &ObserverTimeEvolution::getMaximum)
+	    .def("getNIntervals", &ObserverTimeEvolution::getNIntervals)
+	    .def("getIsLogarithmicScaling", &ObserverTimeEvolution::getIsLogarithmicScaling);
+	py::class_<Observer, Module, ref_ptr<Observer>>(m, "Observer")
+	    .def(py::init<>())
+	    .def("add", &Observer::add)
+	    .def("onDetection", &Observer::onDetection, py::arg("action"), py::arg("clone") = false)
+	    .def("setDeactivateOnDetection", &Observer::setDeactivateOnDetection)
+	    .def("setFlag", &Observer::setFlag)
+	    .def("process", [](const Observer &o, Candidate *c) { o.process(c); });
 
 	py::class_<ModuleList, Module, ref_ptr<ModuleList>>(m, "ModuleList")
 	    .def(py::init<>())

@@ -115,6 +115,31 @@ Candidate::Candidate(const ParticleState &state)
 	serialNumber = ++g_nextSerial;
 }
 
+ref_ptr<Candidate> Candidate::clone(bool recursive) const {
+	ref_ptr<Candidate> c = new Candidate();
+	c->source = source;
+	c->created = created;
+	c->current = current;
+	c->previous = previous;
+	c->properties = properties;
+	c->active = active;
+	c->redshift = redshift;
+	c->weight = weight;
+	c->trajectoryLength = trajectoryLength;
+	c->time = time;
+	c->currentStep = currentStep;
+	c->nextStep = nextStep;
+	if (recursive) {
+		c->secondaries.reserve(secondaries.size());
+		for (const ref_ptr<Candidate> &s : secondaries) {
+			ref_ptr<Candidate> sc = s->clone(true);
+			sc->parent = c.get();
+			c->secondaries.push_back(sc);
+		}
+	}
+	return c;
+}
+
 const std::string &Candidate::getProperty(const std::string &name) const {
 	auto it = properties.find(name);
 	if (it == properties.end())

@@ -3,6 +3,7 @@
 // device batches (include/crpropa3_amd.h).
 #include "crpropa/ModuleList.h"
 #include "crpropa/module/BreakCondition.h"
+#include "crpropa/module/Observer.h"
 #include "crpropa/module/PropagationBP.h"
 #include "crpropa/module/PropagationCK.h"
 
@@ -86,7 +87,8 @@ static void unpack(const std::vector<Candidate *> &c, const Packed &p, const std
 		cand->current.setDirectionExact(Vector3d(p.dir[3 * i], p.dir[3 * i + 1], p.dir[3 * i + 2]));
 		cand->setStepState(p.cur[i], p.next[i], p.len[i], p.time[i]);
 		if (!p.active[i] && cand->isActive()) {
-			if (limit)
+			// the length limit rejects (flag "Rejected"); otherwise an observer deactivated it
+			if (limit && p.len[i] >= limit->getMaximumTrajectoryLength())
 				limit->reject(cand);
 			else
 				cand->setActive(false);
@@ -296,24 +298,41 @@ void ModuleList::run(Candidate *candidate, bool recursive, bool secondariesFirst
 	run(&v, recursive, secondariesFirst);
 }
 
-/* The fused form: [DevicePropagator] or [DevicePropagator, MaximumTrajectoryLength
- * without observers (default flags)]. */
+/* The fused form: a DevicePropagator followed, in any order, by at most one
+ * MaximumTrajectoryLength without observer positions (default flags) and at most one
+ * Observer holding exactly one ObserverTimeEvolution.  Their order does not matter: both
+ * only limit nextStep (a minimum) and deactivate, and every module of a step runs even
+ * on a candidate deactivated earlier in that step (src/ModuleList.cpp.in:54-58). */
 static bool fusable(const ModuleList::module_list_t &mods, const DevicePropagator **prop,
-                    const MaximumTrajectoryLength **limit) {
+                    const MaximumTrajectoryLength **limit, const Observer **obs,
+                    const ObserverTimeEvolution **ote) {
 	*prop = nullptr;
 	*limit = nullptr;
-	if (mods.empty() || mods.size() > 2)
+	*obs = nullptr;
+	*ote = nullptr;
+	if (mods.empty() || mods.size() > 3)
 		return false;
 	auto it = mods.begin();
 	*prop = dynamic_cast<const DevicePropagator *>(it->get());
 	if (!*prop)
 		return false;
-	if (mods.size() == 2) {
-		++it;
-		*limit = dynamic_cast<const MaximumTrajectoryLength *>(it->get());
-		if (!*limit || !(*limit)->getObserverPositions().empty() || !(*limit)->getMakeRejectedInactive() ||
-		    (*limit)->hasRejectAction() || !((*limit)->getMaximumTrajectoryLength() > 0))
+	for (++it; it != mods.end(); ++it) {
+		if (const MaximumTrajectoryLength *l = dynamic_cast<const MaximumTrajectoryLength *>(it->get())) {
+			if (*limit || !l->getObserverPositions().empty() || !l->getMakeRejectedInactive() || l->hasRejectAction() ||
+			    !(l->getMaximumTrajectoryLength() > 0))
+				return false;
+			*limit = l;
+		} else if (const Observer *o = dynamic_cast<const Observer *>(it->get())) {
+			if (*obs || o->getFeatures().size() != 1)
+				return false;
+			const ObserverTimeEvolution *t = dynamic_cast<const ObserverTimeEvolution *>(o->getFeatures()[0].get());
+			if (!t || t->getNIntervals() <= 0)
+				return false;
+			*obs = o;
+			*ote = t;
+		} else {
 			return false;
+		}
 	}
 	return true;
 }
@@ -321,7 +340,9 @@ static bool fusable(const ModuleList::module_list_t &mods, const DevicePropagato
 bool ModuleList::runBatched(const candidate_vector_t &cands) {
 	const DevicePropagator *prop;
 	const MaximumTrajectoryLength *limit;
-	if (!fusable(modules, &prop, &limit))
+	const Observer *obs;
+	const ObserverTimeEvolution *ote;
+	if (!fusable(modules, &prop, &limit, &obs, &ote))
 		return false;
 	std::vector<Candidate *> act;
 	for (auto &c : cands)
@@ -337,17 +358,37 @@ bool ModuleList::runBatched(const candidate_vector_t &cands) {
 		throw std::runtime_error("ModuleList::run: the module list never deactivates candidates "
 		                         "(add MaximumTrajectoryLength and use minStep > 0, or setMaximumSteps)");
 	PropHandle p(prop->makeDevicePropagator(maxLen));
+	std::vector<double> times;
+	if (ote) {
+		times = ote->getTimes();
+		throwIf(crp_propagator_set_observer(p.get(), times.data(), (int)times.size(),
+		                                    obs->getDeactivateOnDetection() ? 1 : 0));
+	}
 	crp_batch *b = nullptr;
 	throwIf(crp_batch_create(act.size(), &b));
 	BatchHandle bh(b);
 	Packed pk;
 	pack(act, pk);
+	std::vector<int32_t> det0;
+	if (ote) {
+		// DetectionIndex as ObserverTimeEvolution::checkDetection keeps it (src/module/Observer.cpp:282-290)
+		pk.det.resize(act.size());
+		for (size_t i = 0; i < act.size(); i++)
+			pk.det[i] = act[i]->hasProperty("DetectionIndex") ? (int32_t)std::stol(act[i]->getProperty("DetectionIndex"))
+			                                                  : 0;
+		det0 = pk.det;
+	}
 	crp_host_state s = pk.view();
 	throwIf(crp_batch_upload(b, act.size(), &s));
 	prop->tuneOn(p.get(), b);
 	crp_run_stats st;
 	throwIf(crp_propagate(p.get(), b, steps, &st));
 	throwIf(crp_batch_download(b, &s));
+	std::vector<double> snaps;
+	if (ote) {
+		snaps.resize(times.size() * act.size() * 9);
+		throwIf(crp_batch_detections(b, (int)times.size(), snaps.data()));
+	}
 	crp_launch_config used;
 	throwIf(crp_propagator_get_launch(p.get(), &used));
 	prop->noteParticlesPerLane(used.particles_per_lane);
@@ -356,6 +397,36 @@ bool ModuleList::runBatched(const candidate_vector_t &cands) {
 		moved[i] = pk.len[i] != act[i]->getTrajectoryLength() || pk.cur[i] != act[i]->getCurrentStep() ||
 		           !pk.active[i];
 	unpack(act, pk, moved, limit);
+	if (ote) {
+		// Observer::process after the fact (src/module/Observer.cpp:26-57): per candidate, in detection
+		// order, the features' onDetection and the action run on a copy of the candidate in its state at
+		// detection (position, direction, trajectory length, time, nextStep; DetectionIndex = k + 1)
+		const size_t n = act.size();
+		for (size_t i = 0; i < n; i++) {
+			Candidate *cand = act[i];
+			for (int32_t k = det0[i]; k < pk.det[i] && k < (int32_t)times.size(); k++) {
+				const double *r = &snaps[((size_t)k * n + i) * 9];
+				ref_ptr<Candidate> view = cand->clone(false);
+				view->current.setPosition(Vector3d(r[0], r[1], r[2]));
+				view->current.setDirectionExact(Vector3d(r[3], r[4], r[5]));
+				view->setStepState(view->getCurrentStep(), r[8], r[6], r[7]);
+				// active when the observer ran, unless the length limit had rejected it in that step
+				view->setActive(!(limit && r[6] >= limit->getMaximumTrajectoryLength()));
+				view->setProperty("DetectionIndex", std::to_string(k + 1));
+				if (!obs->getFlagKey().empty())
+					view->setProperty(obs->getFlagKey(), obs->getFlagValue());
+				for (const ref_ptr<ObserverFeature> &f : obs->getFeatures())
+					f->onDetection(view.get());
+				if (Module *a = obs->getDetectionAction())
+					a->process(view.get());
+			}
+			if (pk.det[i] > det0[i]) {
+				cand->setProperty("DetectionIndex", std::to_string(pk.det[i]));
+				if (!obs->getFlagKey().empty())
+					cand->setProperty(obs->getFlagKey(), obs->getFlagValue());
+			}
+		}
+	}
 	lastAccepted = st.accepted_steps;
 	lastKernel = st.kernel_ms;
 	return true;

@@ -234,3 +234,60 @@ def test_modulelist_autotuned_and_graph_launches_give_identical_results():
         assert np.array_equal(r.view(np.uint64), results[0].view(np.uint64))
     with pytest.raises(RuntimeError):
         PropagationBP(field, 1 * kpc).setLaunchConfig(100)
+
+
+class _Recorder(Module):
+    """Detection action that records (DetectionIndex, position, trajectory length)."""
+
+    def __init__(self):
+        Module.__init__(self)
+        self.rows = []
+
+    def process(self, c):
+        p = c.current.getPosition()
+        self.rows.append((int(c.getProperty("DetectionIndex")), p.x, p.y, p.z, c.getTrajectoryLength()))
+
+
+def _observer_run(extra_module, n=96, seed=51):
+    from crpropa3_amd import Observer, ObserverTimeEvolution
+    grid, gvals, gm, cands, E, Z, d = _grid_candidates(n, seed)
+    ote = ObserverTimeEvolution(0.5 * Mpc, 6 * Mpc, 9, True)
+    obs = Observer()
+    obs.add(ote)
+    obs.setDeactivateOnDetection(False)
+    rec = _Recorder()
+    obs.onDetection(rec)
+    sim = ModuleList()
+    sim.add(PropagationCK(MagneticFieldGrid(grid), 1e-4, 1 * kpc, 1 * Mpc))
+    sim.add(obs)
+    sim.add(MaximumTrajectoryLength(7 * Mpc))
+    if extra_module is not None:
+        sim.add(extra_module)  # not fusable: every module then runs per step on the host
+    sim.run(cands)
+    return sorted(rec.rows), cands, np.array(ote.getTimes()), gvals, gm, E, Z, d
+
+
+def test_modulelist_observer_fused_equals_host_observer_and_oracle():
+    """[PropagationCK, Observer(ObserverTimeEvolution), MaximumTrajectoryLength] runs fused on the
+    device; with an extra host module it runs step by step with the host Observer.  Both report
+    the same detections, equal to the oracle's (kernels' libm)."""
+    class Noop(Module):
+        def process(self, c):
+            pass
+    fused, cf, times, gvals, gm, E, Z, d = _observer_run(None)
+    stepwise, cs, _, _, _, _, _, _ = _observer_run(Noop())
+    assert len(fused) == 9 * len(cf)
+    assert fused == stepwise
+    L = gm["n"] * gm["spacing"]
+    st = O.OracleState(np.tile([L / 2] * 3, (len(E), 1)), d, E, Z * O.eplus)
+    O.propagate(O.OracleField.grid(gvals, gm["origin"], gm["spacing"]), O.PROP_CK, st, 10 ** 9, tolerance=1e-4,
+                min_step=1 * kpc, max_step=1 * Mpc, max_len=7 * Mpc, libm=O.LIBM_CR, observer_times=times)
+    ref = sorted((k + 1, st.snap[k, i, 0], st.snap[k, i, 1], st.snap[k, i, 2], st.snap[k, i, 6])
+                 for k in range(len(times)) for i in range(len(E)))
+    assert fused == ref
+    for cands in (cf, cs):
+        pos = np.array([[c.current.getPosition().x, c.current.getPosition().y, c.current.getPosition().z]
+                        for c in cands])
+        assert np.array_equal(pos.view(np.uint64), st.pos.view(np.uint64))
+        assert all(c.getProperty("DetectionIndex") == "9" for c in cands)
+        assert all(c.getProperty("Rejected") == "MaximumTrajectoryLength" for c in cands)
