@@ -141,3 +141,35 @@ def test_python_module_subclass():
     ml.add(m)
     ml.process(c)
     assert m.n == 1
+
+
+def test_observer_time_evolution_lists_and_detection_logic():
+    """Host ObserverTimeEvolution: time lists bit-identical to the reference's (fixtures), and
+    checkDetection's limitNextStep / DetectionIndex bookkeeping (src/module/Observer.cpp:279-306)."""
+    import numpy as np
+    import oracle_ctypes as O
+    from crpropa3_amd import Candidate, Mpc, Observer, ObserverTimeEvolution, DETECTED, NOTHING
+    c = O.load_case("ck_grid32_observer")
+    ote = ObserverTimeEvolution(0.5 * Mpc, 10 * Mpc, 12, True)
+    assert np.array_equal(np.array(ote.getTimes()).view(np.uint64), c["times"].view(np.uint64))
+    b = O.load_case("bp_fixed_grid32_observer")
+    assert np.array_equal(np.array(ObserverTimeEvolution(list(b["times"])).getTimes()), b["times"])
+    lin = ObserverTimeEvolution(1.0, 2.0, 4)  # min, dist, numb
+    assert lin.getTimes() == [1.0, 3.0, 5.0, 7.0]
+    ote = ObserverTimeEvolution([1.0, 2.0, 3.0])
+    cand = Candidate()
+    cand.setNextStep(10.0)
+    cand.setCurrentStep(0.4)  # length 0.4 < 1: no detection, nextStep limited to 0.6
+    assert ote.checkDetection(cand) == NOTHING and abs(cand.getNextStep() - 0.6) < 1e-15
+    cand.setCurrentStep(0.7)  # length 1.1: detection 1, nextStep limited to 2 - 1.1
+    cand.setNextStep(10.0)
+    assert ote.checkDetection(cand) == DETECTED and cand.getProperty("DetectionIndex") == "1"
+    assert abs(cand.getNextStep() - 0.9) < 1e-15
+    cand.setCurrentStep(1.0)  # length 2.1: detection 2; index 1 = n - 2: nextStep not limited further
+    ns = cand.getNextStep()
+    assert ote.checkDetection(cand) == DETECTED and cand.getNextStep() == ns
+    obs = Observer()
+    obs.add(ObserverTimeEvolution([0.0]))
+    cand2 = Candidate()
+    obs.process(cand2)
+    assert not cand2.isActive()  # Observer deactivates on detection by default
