@@ -516,6 +516,25 @@ static inline void orc_process(const orc_field *f, const orc_propagator *p, orc_
 	s->trajectory_length[i] += step;
 	s->time[i] += step / C_LIGHT;
 	s->next_step[i] = newStep;
+	/* PeriodicBox::process (src/module/Boundary.cpp:16-27): n = floor((pos - origin) / size);
+	 * current, previous, source and created positions move by -n * size */
+	if (p->box) {
+		double nb[3];
+		for (int a = 0; a < 3; a++)
+			nb[a] = floor((pos[a] - p->boxOrigin[a]) / p->boxSize[a]);
+		if (!((nb[0] == 0) && (nb[1] == 0) && (nb[2] == 0))) {
+			for (int a = 0; a < 3; a++) {
+				const double sh = nb[a] * p->boxSize[a];
+				pos[a] = pos[a] - sh;
+				if (s->prev_pos)
+					s->prev_pos[3 * i + a] = s->prev_pos[3 * i + a] - sh;
+				if (s->source_pos)
+					s->source_pos[3 * i + a] = s->source_pos[3 * i + a] - sh;
+				if (s->created_pos)
+					s->created_pos[3 * i + a] = s->created_pos[3 * i + a] - sh;
+			}
+		}
+	}
 	if (p->maxTrajectoryLength > 0) {
 		double len = s->trajectory_length[i];
 		if (len >= p->maxTrajectoryLength)

@@ -48,6 +48,9 @@ typedef struct {
 	 * detection lengths obsTimes[nObs], deactivate on detection */
 	const double *obsTimes;
 	int nObs, obsDeactivate;
+	/* PeriodicBox(origin, size) right after the propagator (box = 0: none) */
+	int box;
+	double boxOrigin[3], boxSize[3];
 } orc_propagator;
 
 /* Particle state, structure of arrays.  pos/dir are AoS (3 per particle). */
@@ -63,6 +66,7 @@ typedef struct {
 	int32_t *det_index;
 	double *snap;
 	size_t n; /* particles in the state (snapshot stride) */
+	double *source_pos, *created_pos; /* [3n], shifted by PeriodicBox (may be NULL) */
 } orc_state;
 
 /* MT19937 as in src/Random.cpp */

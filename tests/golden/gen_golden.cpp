@@ -12,6 +12,7 @@
 //   PropagationCK::process                               src/module/PropagationCK.cpp:77-136
 //   MaximumTrajectoryLength::process                     src/module/BreakCondition.cpp:42-64
 //   Random (MT19937, randUniform, randVector, randPowerLaw, randBin) src/Random.cpp:80-205
+//   PeriodicBox (current, previous, source, created shifted)     src/module/Boundary.cpp:16-27
 //   Observer + ObserverTimeEvolution (detections, limitNextStep)  src/module/Observer.cpp:26-57,266-306
 //   GridTools: scaleGrid, mean/rms field strength, loadGrid/dumpGrid (raw float32 and text),
 //   loadGrid3fFromTxt, fromMagneticField                 src/GridTools.cpp:17-236,238-447
@@ -32,6 +33,7 @@
 #include "crpropa/magneticField/MagneticField.h"
 #include "crpropa/magneticField/MagneticFieldGrid.h"
 #include "crpropa/magneticField/turbulentField/PlaneWaveTurbulence.h"
+#include "crpropa/module/Boundary.h"
 #include "crpropa/module/BreakCondition.h"
 #include "crpropa/module/Observer.h"
 #include "crpropa/module/PropagationBP.h"
@@ -334,6 +336,63 @@ static void run_observer_case(const std::string &name, const Module &prop, const
 	printf("  %-28s N=%zu steps<=%zu detections=%zu\n", name.c_str(), N, maxSteps, K);
 }
 
+// ---------------------------------------------------------------- periodic box
+// ModuleList order [propagator, PeriodicBox, MaximumTrajectoryLength (optional)];
+// records current/previous/source/created positions after the listed step counts.
+static void run_box_case(const std::string &name, const Module &prop, const PeriodicBox &box,
+                         const MaximumTrajectoryLength *maxlen, const std::vector<Particle> &ps, size_t maxSteps,
+                         const std::vector<size_t> &snaps, const std::string &meta) {
+	std::string dir = casedir(name);
+	save_particles(dir, ps);
+	const size_t N = ps.size(), S = snaps.size();
+	std::vector<double> pos(S * N * 3), prv(S * N * 3), src(S * N * 3), cre(S * N * 3), d(S * N * 3), len(S * N),
+	    nxt(S * N);
+	std::vector<int32_t> act(S * N);
+#pragma omp parallel for schedule(dynamic, 4)
+	for (size_t i = 0; i < N; i++) {
+		ParticleState st(ps[i].id, ps[i].E, ps[i].pos, ps[i].dir);
+		ref_ptr<Candidate> c = new Candidate(st);
+		auto record = [&](size_t si) {
+			size_t k = si * N + i;
+			Vector3d a = c->current.getPosition(), b = c->previous.getPosition(), s0 = c->source.getPosition(),
+			         s1 = c->created.getPosition(), u = c->current.getDirection();
+			const Vector3d *vs[5] = {&a, &b, &s0, &s1, &u};
+			std::vector<double> *out[5] = {&pos, &prv, &src, &cre, &d};
+			for (int v = 0; v < 5; v++) {
+				(*out[v])[3 * k] = vs[v]->x;
+				(*out[v])[3 * k + 1] = vs[v]->y;
+				(*out[v])[3 * k + 2] = vs[v]->z;
+			}
+			len[k] = c->getTrajectoryLength();
+			nxt[k] = c->getNextStep();
+			act[k] = c->isActive() ? 1 : 0;
+		};
+		size_t si = 0, step = 0;
+		while (step < maxSteps && c->isActive()) {
+			prop.process(c);
+			box.process(c);
+			if (maxlen) maxlen->process(c);
+			step++;
+			while (si < S && snaps[si] == step)
+				record(si++);
+		}
+		for (; si < S; si++) // snapshots past the particle's end hold its final state
+			record(si);
+	}
+	std::vector<double> sn(snaps.begin(), snaps.end());
+	save_f64(dir, "snap_steps", sn, {S});
+	save_f64(dir, "pos", pos, {S, N, 3});
+	save_f64(dir, "previous_pos", prv, {S, N, 3});
+	save_f64(dir, "source_pos", src, {S, N, 3});
+	save_f64(dir, "created_pos", cre, {S, N, 3});
+	save_f64(dir, "dir", d, {S, N, 3});
+	save_f64(dir, "trajectory_length", len, {S, N});
+	save_f64(dir, "next_step", nxt, {S, N});
+	save_i32(dir, "active", act, {S, N});
+	save_meta(dir, meta);
+	printf("  %-28s N=%zu steps<=%zu\n", name.c_str(), N, maxSteps);
+}
+
 static void gen_trilinear() {
 	// 16^3 grid with a non-trivial origin; spacing 50 kpc; amplitude 1 nG.
 	const int n = 16;
@@ -573,6 +632,30 @@ int main(int argc, char **argv) {
 		             ", \"maxStep\": " + fmt(Mpc) + ", \"maxTrajectoryLength\": " + fmt(10 * Mpc) +
 		             ", \"field\": \"grid32\", \"clip\": true, \"particle_seed\": 47, \"Emin\": " + fmt(EeV) +
 		             ", \"Emax\": " + fmt(100 * EeV) + "}");
+	}
+
+	// ---- PeriodicBox around the 32^3 grid (origin 0, 1.6 Mpc): positions are wrapped back,
+	// source and created positions move the other way
+	{
+		PeriodicBox box(Vector3d(0.), Vector3d(n * sp));
+		std::vector<Particle> ps = make_particles(128, 50, 1 * EeV, 100 * EeV, centre, proton, one);
+		PropagationBP bp(fgrid, 0.1 * Mpc);
+		run_box_case("bp_fixed_grid32_box", bp, box, nullptr, ps, 500, {1, 10, 100, 500},
+		             "{\"propagator\": \"BP\", \"fixedStep\": " + fmt(0.1 * Mpc) +
+		                 ", \"field\": \"grid32\", \"box\": {\"origin\": [0, 0, 0], \"size\": [" + fmt(n * sp) +
+		                 ", " + fmt(n * sp) + ", " + fmt(n * sp) + "]}, \"particle_seed\": 50}");
+	}
+	{
+		PeriodicBox box(Vector3d(-0.2 * Mpc, 0.1 * Mpc, 0.), Vector3d(1.1 * Mpc, 1.3 * Mpc, 0.9 * Mpc));
+		std::vector<Particle> ps = make_particles(128, 51, 1 * EeV, 300 * EeV, centre, species, wts);
+		PropagationCK ck(fgrid, 1e-4, 1 * kpc, 1 * Mpc);
+		MaximumTrajectoryLength ml(15 * Mpc);
+		run_box_case("ck_grid32_box", ck, box, &ml, ps, 4000, {1, 10, 100, 1000, 4000},
+		             "{\"propagator\": \"CK\", \"tolerance\": 1e-4, \"minStep\": " + fmt(kpc) +
+		                 ", \"maxStep\": " + fmt(Mpc) + ", \"maxTrajectoryLength\": " + fmt(15 * Mpc) +
+		                 ", \"field\": \"grid32\", \"box\": {\"origin\": [" + fmt(-0.2 * Mpc) + ", " + fmt(0.1 * Mpc) +
+		                 ", 0], \"size\": [" + fmt(1.1 * Mpc) + ", " + fmt(1.3 * Mpc) + ", " + fmt(0.9 * Mpc) +
+		                 "]}, \"particle_seed\": 51}");
 	}
 
 	// ---- ObserverTimeEvolution: log-spaced times with a length limit (CK), and an

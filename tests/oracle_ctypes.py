@@ -53,14 +53,16 @@ class Field(C.Structure):
 class Propagator(C.Structure):
     _fields_ = [("type", C.c_int), ("tolerance", C.c_double), ("minStep", C.c_double),
                 ("maxStep", C.c_double), ("maxTrajectoryLength", C.c_double),
-                ("obsTimes", _dp), ("nObs", C.c_int), ("obsDeactivate", C.c_int)]
+                ("obsTimes", _dp), ("nObs", C.c_int), ("obsDeactivate", C.c_int),
+                ("box", C.c_int), ("boxOrigin", C.c_double * 3), ("boxSize", C.c_double * 3)]
 
 
 class State(C.Structure):
     _fields_ = [("pos", _dp), ("dir", _dp), ("energy", _dp), ("charge", _dp), ("redshift", _dp),
                 ("current_step", _dp), ("next_step", _dp), ("trajectory_length", _dp), ("time", _dp),
                 ("active", _ip), ("prev_pos", _dp), ("prev_dir", _dp),
-                ("det_index", _ip), ("snap", _dp), ("n", C.c_size_t)]
+                ("det_index", _ip), ("snap", _dp), ("n", C.c_size_t),
+                ("source_pos", _dp), ("created_pos", _dp)]
 
 
 _lib = None
@@ -217,6 +219,8 @@ class OracleState:
         self.prev_dir = self.dir.copy()
         self.det_index = np.zeros(n, dtype=np.int32)
         self.snap = None  # (K, n, 9) once an observer is used
+        self.source_pos = self.pos.copy()
+        self.created_pos = self.pos.copy()
 
     def struct(self):
         s = State()
@@ -225,16 +229,23 @@ class OracleState:
             setattr(s, name, ptr(getattr(self, name)))
         s.active = ptr(self.active, _ip)
         s.det_index = ptr(self.det_index, _ip)
+        s.source_pos = ptr(self.source_pos)
+        s.created_pos = ptr(self.created_pos)
         s.snap = ptr(self.snap) if self.snap is not None else None
         s.n = self.n
         return s
 
 
 def propagate(field, prop_type, state, steps, tolerance=0.42, min_step=0.0, max_step=0.0, max_len=0.0,
-              libm=LIBM_GLIBC, nthreads=0, observer_times=None, observer_deactivate=False):
+              libm=LIBM_GLIBC, nthreads=0, observer_times=None, observer_deactivate=False, box=None):
     """observer_times: detection lengths of an Observer(ObserverTimeEvolution) after the length limit;
     detections land in state.snap[k, i, :] = (pos, dir, length, time, nextStep)."""
     p = Propagator(prop_type, tolerance, min_step, max_step, max_len)
+    if box is not None:  # (origin[3], size[3]) of a PeriodicBox right after the propagator
+        p.box = 1
+        for a in range(3):
+            p.boxOrigin[a] = box[0][a]
+            p.boxSize[a] = box[1][a]
     keep = None
     if observer_times is not None and len(observer_times):
         keep = np.ascontiguousarray(observer_times, dtype=np.float64)

@@ -276,3 +276,25 @@ def test_oracle_observer_bit_exact_vs_reference(name):
     assert bits_equal(st.pos, c["final_pos"]) and bits_equal(st.dir, c["final_dir"])
     assert bits_equal(st.trajectory_length, c["final_len"]) and bits_equal(st.next_step, c["final_next"])
     assert np.array_equal(st.active, c["final_active"])
+
+
+@pytest.mark.parametrize("name", ["bp_fixed_grid32_box", "ck_grid32_box"])
+def test_oracle_periodic_box_bit_exact_vs_reference(name):
+    """PeriodicBox after the propagator: current, previous, source and created positions
+    (src/module/Boundary.cpp:16-27) equal the reference's at every snapshot."""
+    c = O.load_case(name)
+    m = c["meta"]
+    f = O.field_for_case(c)
+    st = O.OracleState(c["init_pos"], c["init_dir"], c["energy"], c["charge"])
+    pa = O.propagator_args(m)
+    box = (m["box"]["origin"], m["box"]["size"])
+    done = 0
+    for si, s in enumerate(c["snap_steps"]):
+        O.propagate(f, steps=int(s) - done, max_len=m.get("maxTrajectoryLength", 0.0), libm=O.LIBM_GLIBC, state=st,
+                    box=box, **pa)
+        done = int(s)
+        for key, ref in (("pos", "pos"), ("prev_pos", "previous_pos"), ("source_pos", "source_pos"),
+                         ("created_pos", "created_pos"), ("dir", "dir"), ("trajectory_length", "trajectory_length"),
+                         ("next_step", "next_step")):
+            assert bits_equal(getattr(st, key), c[ref][si]), (name, s, key)
+        assert np.array_equal(st.active, c["active"][si])
