@@ -34,7 +34,8 @@ class RunStats(C.Structure):
 class HostState(C.Structure):
     _fields_ = [("pos", _dp), ("dir", _dp), ("energy", _dp), ("charge", _dp), ("current_step", _dp),
                 ("next_step", _dp), ("trajectory_length", _dp), ("time", _dp), ("active", _ip),
-                ("prev_pos", _dp), ("prev_dir", _dp), ("detection_index", _ip)]
+                ("prev_pos", _dp), ("prev_dir", _dp), ("detection_index", _ip), ("source_pos", _dp),
+                ("created_pos", _dp)]
 
 
 class LaunchConfig(C.Structure):
@@ -80,6 +81,7 @@ def lib():
         L.crp_propagator_set_precision.argtypes = [_vp, C.c_int]
         L.crp_propagator_destroy.argtypes = [_vp]
         L.crp_propagator_set_observer.argtypes = [_vp, _dp, C.c_int, C.c_int]
+        L.crp_propagator_set_periodic_box.argtypes = [_vp, _dp, _dp]
         L.crp_batch_detections.argtypes = [_vp, C.c_int, _dp]
         L.crp_propagator_set_launch.argtypes = [_vp, C.POINTER(LaunchConfig)]
         L.crp_propagator_get_launch.argtypes = [_vp, C.POINTER(LaunchConfig)]
@@ -226,6 +228,13 @@ class Propagator:
                                                 1 if deactivate else 0))
         return self
 
+    def set_periodic_box(self, origin, size):
+        """Fused PeriodicBox(origin, size) right after the propagator; size (0,0,0) removes it."""
+        o = _f64(origin, (3,))
+        z = _f64(size, (3,))
+        check(lib().crp_propagator_set_periodic_box(self.handle, _p(o), _p(z)))
+        return self
+
     def set_launch(self, block=256, blocks_per_cu=0, steps_per_launch=0, use_graph=False):
         cfg = LaunchConfig(block, blocks_per_cu, steps_per_launch, 1 if use_graph else 0, 0.0)
         check(lib().crp_propagator_set_launch(self.handle, C.byref(cfg)))
@@ -263,7 +272,7 @@ class Batch:
         return n.value
 
     def upload(self, pos, dir, energy, charge, current_step=None, next_step=None, trajectory_length=None,
-               time=None, active=None, detection_index=None):
+               time=None, active=None, detection_index=None, source_pos=None, created_pos=None):
         pos = _f64(pos, (-1, 3))
         n = pos.shape[0]
         arrs = {
@@ -282,13 +291,19 @@ class Batch:
         if detection_index is not None:
             det = np.ascontiguousarray(detection_index, dtype=np.int32)
             s.detection_index = _p(det, _ip)
+        extra = {}
+        for name, a in (("source_pos", source_pos), ("created_pos", created_pos)):
+            if a is not None:
+                extra[name] = _f64(a, (-1, 3))
+                setattr(s, name, _p(extra[name]))
         check(lib().crp_batch_upload(self.handle, n, C.byref(s)))
 
     def download(self):
         n = len(self)
         out = {"pos": np.zeros((n, 3)), "dir": np.zeros((n, 3)), "energy": np.zeros(n), "charge": np.zeros(n),
                "current_step": np.zeros(n), "next_step": np.zeros(n), "trajectory_length": np.zeros(n),
-               "time": np.zeros(n), "prev_pos": np.zeros((n, 3)), "prev_dir": np.zeros((n, 3))}
+               "time": np.zeros(n), "prev_pos": np.zeros((n, 3)), "prev_dir": np.zeros((n, 3)),
+               "source_pos": np.zeros((n, 3)), "created_pos": np.zeros((n, 3))}
         act = np.zeros(n, dtype=np.int32)
         det = np.zeros(n, dtype=np.int32)
         s = HostState()

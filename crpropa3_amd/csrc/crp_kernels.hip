@@ -74,6 +74,9 @@ struct crp_propagator {
 	// fused Observer(ObserverTimeEvolution): detection lengths on the device
 	double *obsT;
 	int nObs, obsDeactivate;
+	// fused PeriodicBox
+	int box;
+	double boxO[3], boxS[3];
 };
 
 
@@ -84,6 +87,7 @@ struct crp_batch {
 	double *base;
 	double *px, *py, *pz, *dx, *dy, *dz, *E, *q, *cur, *next, *len, *time;
 	double *ppx, *ppy, *ppz, *pdx, *pdy, *pdz;
+	double *sx, *sy, *sz, *cx, *cy, *cz; // source and created positions
 	int32_t *active;
 	uint32_t *order; // queue order: particle indices, longest expected work first
 	unsigned long long *ctr;
@@ -106,6 +110,8 @@ static Soa soa_of(const crp_batch *b) {
 	s.E = b->E; s.q = b->q; s.cur = b->cur; s.next = b->next; s.len = b->len; s.time = b->time;
 	s.ppx = b->ppx; s.ppy = b->ppy; s.ppz = b->ppz;
 	s.pdx = b->pdx; s.pdy = b->pdy; s.pdz = b->pdz;
+	s.sx = b->sx; s.sy = b->sy; s.sz = b->sz;
+	s.cx = b->cx; s.cy = b->cy; s.cz = b->cz;
 	s.active = b->active;
 	s.order = b->order;
 	s.det = b->det;
@@ -833,6 +839,9 @@ static int create_prop(int type, const char *who, const crp_field_t *field, doub
 	p->obsT = nullptr;
 	p->nObs = 0;
 	p->obsDeactivate = 0;
+	p->box = 0;
+	for (int a = 0; a < 3; a++)
+		p->boxO[a] = p->boxS[a] = 0;
 	*out = p;
 	return CRP_OK;
 }
@@ -883,6 +892,20 @@ int crp_propagator_set_observer(crp_propagator_t *p, const double *times, int nt
 	return CRP_OK;
 }
 
+int crp_propagator_set_periodic_box(crp_propagator_t *p, const double origin[3], const double size[3]) {
+	if (!p || !origin || !size)
+		return fail(CRP_E_INVALID, "crp_propagator_set_periodic_box: NULL argument");
+	const bool off = size[0] == 0 && size[1] == 0 && size[2] == 0;
+	if (!off && !(size[0] > 0 && size[1] > 0 && size[2] > 0))
+		return fail(CRP_E_INVALID, "crp_propagator_set_periodic_box: size must be > 0 (or all 0 to remove the box)");
+	p->box = off ? 0 : 1;
+	for (int a = 0; a < 3; a++) {
+		p->boxO[a] = origin[a];
+		p->boxS[a] = size[a];
+	}
+	return CRP_OK;
+}
+
 int crp_propagator_destroy(crp_propagator_t *p) {
 	if (p && p->obsT) {
 		(void)hipDeviceSynchronize();
@@ -903,7 +926,7 @@ int crp_batch_create(size_t capacity, crp_batch_t **out) {
 	(void)hipGetDevice(&b->device);
 	// one allocation: 18 double arrays, the active flags, the queue order, then a
 	// 256-byte aligned counter block
-	const size_t nd = 18;
+	const size_t nd = 24;
 	size_t off = nd * capacity * sizeof(double) + 2 * capacity * sizeof(int32_t) + capacity * sizeof(uint32_t);
 	off = (off + 255) & ~(size_t)255;
 	const size_t bytes = off + 64 * sizeof(unsigned long long);
@@ -912,8 +935,9 @@ int crp_batch_create(size_t capacity, crp_batch_t **out) {
 		delete b;
 		return fail(CRP_E_NOMEM, std::string("crp_batch_create: hipMalloc: ") + hipGetErrorString(e));
 	}
-	double **arr[18] = {&b->px, &b->py, &b->pz, &b->dx, &b->dy, &b->dz, &b->E, &b->q, &b->cur,
-	                    &b->next, &b->len, &b->time, &b->ppx, &b->ppy, &b->ppz, &b->pdx, &b->pdy, &b->pdz};
+	double **arr[24] = {&b->px, &b->py, &b->pz, &b->dx, &b->dy, &b->dz, &b->E, &b->q, &b->cur,
+	                    &b->next, &b->len, &b->time, &b->ppx, &b->ppy, &b->ppz, &b->pdx, &b->pdy, &b->pdz,
+	                    &b->sx, &b->sy, &b->sz, &b->cx, &b->cy, &b->cz};
 	for (size_t i = 0; i < nd; i++)
 		*arr[i] = b->base + i * capacity;
 	b->active = (int32_t *)(b->base + nd * capacity);
@@ -1011,6 +1035,19 @@ int crp_batch_upload(crp_batch_t *b, size_t n, const crp_host_state *s) {
 	// previous := current until the first step
 	CRP_HIP_TRY(hipMemcpy(b->ppx, b->px, 3 * b->cap * sizeof(double), hipMemcpyDeviceToDevice));
 	CRP_HIP_TRY(hipMemcpy(b->pdx, b->dx, 3 * b->cap * sizeof(double), hipMemcpyDeviceToDevice));
+	// source and created positions: given, or the current position
+	const double *srcp[2] = {s->source_pos, s->created_pos};
+	double *dsts[2][3] = {{b->sx, b->sy, b->sz}, {b->cx, b->cy, b->cz}};
+	for (int v = 0; v < 2; v++)
+		for (int c = 0; c < 3; c++) {
+			if (srcp[v]) {
+				for (size_t i = 0; i < n; i++)
+					tmp[i] = srcp[v][3 * i + c];
+				CRP_HIP_TRY(hipMemcpy(dsts[v][c], tmp.data(), n * sizeof(double), hipMemcpyHostToDevice));
+			} else {
+				CRP_HIP_TRY(hipMemcpy(dsts[v][c], dst3[c], n * sizeof(double), hipMemcpyDeviceToDevice));
+			}
+		}
 	return CRP_OK;
 }
 
@@ -1041,6 +1078,16 @@ int crp_batch_download(const crp_batch_t *b, crp_host_state *s) {
 		CRP_HIP_TRY(hipMemcpy(s->active, b->active, n * sizeof(int32_t), hipMemcpyDeviceToHost));
 	if (s->detection_index)
 		CRP_HIP_TRY(hipMemcpy(s->detection_index, b->det, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+	const double *srcv[6] = {b->sx, b->sy, b->sz, b->cx, b->cy, b->cz};
+	double *dstv2[2] = {s->source_pos, s->created_pos};
+	for (int c = 0; c < 6; c++) {
+		double *dst = dstv2[c / 3];
+		if (!dst)
+			continue;
+		CRP_HIP_TRY(hipMemcpy(tmp.data(), srcv[c], n * sizeof(double), hipMemcpyDeviceToHost));
+		for (size_t i = 0; i < n; i++)
+			dst[3 * i + (c % 3)] = tmp[i];
+	}
 	return CRP_OK;
 }
 
@@ -1092,12 +1139,14 @@ int crp_batch_copy(crp_batch_t *dst, const crp_batch_t *src) {
 	if (n == 0)
 		return CRP_OK;
 	CRP_HIP_TRY(hipStreamSynchronize(src->stream));
-	const double *s[18] = {src->px, src->py, src->pz, src->dx, src->dy, src->dz, src->E, src->q, src->cur,
-	                       src->next, src->len, src->time, src->ppx, src->ppy, src->ppz, src->pdx, src->pdy, src->pdz};
-	double *d[18] = {dst->px, dst->py, dst->pz, dst->dx, dst->dy, dst->dz, dst->E, dst->q, dst->cur,
-	                 dst->next, dst->len, dst->time, dst->ppx, dst->ppy, dst->ppz, dst->pdx, dst->pdy, dst->pdz};
+	const double *s[24] = {src->px, src->py, src->pz, src->dx, src->dy, src->dz, src->E, src->q, src->cur,
+	                       src->next, src->len, src->time, src->ppx, src->ppy, src->ppz, src->pdx, src->pdy, src->pdz,
+	                       src->sx, src->sy, src->sz, src->cx, src->cy, src->cz};
+	double *d[24] = {dst->px, dst->py, dst->pz, dst->dx, dst->dy, dst->dz, dst->E, dst->q, dst->cur,
+	                 dst->next, dst->len, dst->time, dst->ppx, dst->ppy, dst->ppz, dst->pdx, dst->pdy, dst->pdz,
+	                 dst->sx, dst->sy, dst->sz, dst->cx, dst->cy, dst->cz};
 	const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
-	for (int c = 0; c < 18; c++)
+	for (int c = 0; c < 24; c++)
 		hipLaunchKernelGGL(k_copy_f64, dim3(blocks), dim3(256), 0, dst->stream, s[c], d[c], n);
 	CRP_HIP_TRY(hipGetLastError());
 	CRP_HIP_TRY(hipMemcpyAsync(dst->active, src->active, n * sizeof(int32_t), hipMemcpyDeviceToDevice, dst->stream));
@@ -1136,6 +1185,12 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 	P.obsT = p->obsT;
 	P.nObs = p->nObs;
 	P.obsDeactivate = p->obsDeactivate;
+	P.box = p->box;
+	for (int a = 0; a < 3; a++) {
+		P.boxO[a] = p->boxO[a];
+		P.boxS[a] = p->boxS[a];
+		P.boxInv[a] = p->box ? 1.0 / p->boxS[a] : 0.0;
+	}
 	if (p->nObs > 0) {
 		int rc0 = ensure_snapshots(b, p->nObs);
 		if (rc0 != CRP_OK)

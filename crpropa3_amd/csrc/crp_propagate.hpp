@@ -28,6 +28,7 @@ constexpr int kColdDoubles = 10; // dynamic LDS per thread of k_propagate, in do
 struct Soa {
 	double *px, *py, *pz, *dx, *dy, *dz, *E, *q, *cur, *next, *len, *time;
 	double *ppx, *ppy, *ppz, *pdx, *pdy, *pdz;
+	double *sx, *sy, *sz, *cx, *cy, *cz; // source and created positions (moved by PeriodicBox)
 	int32_t *active;
 	const uint32_t *order;
 	int32_t *det;             // observer detections so far per particle ("DetectionIndex")
@@ -43,6 +44,9 @@ struct Params {
 	// Observer with one ObserverTimeEvolution after the length limit (nObs = 0: none)
 	const double *obsT;
 	int nObs, obsDeactivate;
+	// PeriodicBox right after the propagator (box = 0: none); boxInv = RN(1/size)
+	int box;
+	double boxO[3], boxS[3], boxInv[3];
 };
 
 __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; } // std::max
@@ -286,6 +290,20 @@ k_propagate(F field, Params P, Soa S, unsigned long long n,
 				V3 u1 = v3(div_rn(yOut.u.x, n1, i1), div_rn(yOut.u.y, n1, i1), div_rn(yOut.u.z, n1, i1));
 				const double n2 = norm(u1), i2 = 1 / n2;
 				dir = v3(div_rn(u1.x, n2, i2), div_rn(u1.y, n2, i2), div_rn(u1.z, n2, i2));
+			}
+		}
+		if (P.box) {
+			// PeriodicBox::process (src/module/Boundary.cpp:16-27): n = floor((pos - origin) / size);
+			// current, previous, source and created positions move by -n * size
+			const double nbx = __builtin_floor(div_rn(pos.x - P.boxO[0], P.boxS[0], P.boxInv[0]));
+			const double nby = __builtin_floor(div_rn(pos.y - P.boxO[1], P.boxS[1], P.boxInv[1]));
+			const double nbz = __builtin_floor(div_rn(pos.z - P.boxO[2], P.boxS[2], P.boxInv[2]));
+			if (!((nbx == 0) && (nby == 0) && (nbz == 0))) {
+				const double shx = nbx * P.boxS[0], shy = nby * P.boxS[1], shz = nbz * P.boxS[2];
+				pos = v3(pos.x - shx, pos.y - shy, pos.z - shz);
+				*cPPx = *cPPx - shx; *cPPy = *cPPy - shy; *cPPz = *cPPz - shz;
+				S.sx[idx] = S.sx[idx] - shx; S.sy[idx] = S.sy[idx] - shy; S.sz[idx] = S.sz[idx] - shz;
+				S.cx[idx] = S.cx[idx] - shx; S.cy[idx] = S.cy[idx] - shy; S.cz[idx] = S.cz[idx] - shz;
 			}
 		}
 		*cCur = step;

@@ -20,6 +20,7 @@
 #include "crpropa/magneticField/MagneticField.h"
 #include "crpropa/magneticField/MagneticFieldGrid.h"
 #include "crpropa/magneticField/turbulentField/PlaneWaveTurbulence.h"
+#include "crpropa/module/Boundary.h"
 #include "crpropa/module/BreakCondition.h"
 #include "crpropa/module/Observer.h"
 #include "crpropa/module/PropagationBP.h"
@@ -348,6 +349,13 @@ PYBIND11_MODULE(_crpropa, m) {
 	    .def("getMaximumTrajectoryLength", &MaximumTrajectoryLength::getMaximumTrajectoryLength)
 	    .def("addObserverPosition", &MaximumTrajectoryLength::addObserverPosition)
 	    .def("getObserverPositions", &MaximumTrajectoryLength::getObserverPositions);
+
+	py::class_<PeriodicBox, Module, ref_ptr<PeriodicBox>>(m, "PeriodicBox")
+	    .def(py::init<>())
+	    .def(py::init<Vector3d, Vector3d>())
+	    .def("setOrigin", &PeriodicBox::setOrigin)
+	    .def("setSize", &PeriodicBox::setSize)
+	    .def("process", [](const PeriodicBox &b, Candidate *c) { b.process(c); });
 
 	// observers (include/crpropa/module/Observer.h)
 	py::enum_<DetectionState>(m, "DetectionState")

@@ -2,6 +2,7 @@
 // handling with the reference's validation, and the packing of Candidates into
 // device batches (include/crpropa3_amd.h).
 #include "crpropa/ModuleList.h"
+#include "crpropa/module/Boundary.h"
 #include "crpropa/module/BreakCondition.h"
 #include "crpropa/module/Observer.h"
 #include "crpropa/module/PropagationBP.h"
@@ -38,6 +39,7 @@ struct Packed {
 	std::vector<double> pos, dir, E, q, cur, next, len, time, ppos, pdir;
 	std::vector<int32_t> active;
 	std::vector<int32_t> det; // observer detections per candidate (empty: none tracked)
+	std::vector<double> src, cre; // source / created positions (empty: not tracked)
 	void resize(size_t n) {
 		pos.resize(3 * n); dir.resize(3 * n); ppos.resize(3 * n); pdir.resize(3 * n);
 		E.resize(n); q.resize(n); cur.resize(n); next.resize(n); len.resize(n); time.resize(n);
@@ -50,6 +52,8 @@ struct Packed {
 		s.current_step = cur.data(); s.next_step = next.data(); s.trajectory_length = len.data();
 		s.time = time.data(); s.active = active.data(); s.prev_pos = ppos.data(); s.prev_dir = pdir.data();
 		s.detection_index = det.empty() ? nullptr : det.data();
+		s.source_pos = src.empty() ? nullptr : src.data();
+		s.created_pos = cre.empty() ? nullptr : cre.data();
 		return s;
 	}
 };
@@ -305,17 +309,24 @@ void ModuleList::run(Candidate *candidate, bool recursive, bool secondariesFirst
  * on a candidate deactivated earlier in that step (src/ModuleList.cpp.in:54-58). */
 static bool fusable(const ModuleList::module_list_t &mods, const DevicePropagator **prop,
                     const MaximumTrajectoryLength **limit, const Observer **obs,
-                    const ObserverTimeEvolution **ote) {
+                    const ObserverTimeEvolution **ote, const PeriodicBox **box) {
 	*prop = nullptr;
 	*limit = nullptr;
 	*obs = nullptr;
 	*ote = nullptr;
-	if (mods.empty() || mods.size() > 3)
+	*box = nullptr;
+	if (mods.empty() || mods.size() > 4)
 		return false;
 	auto it = mods.begin();
 	*prop = dynamic_cast<const DevicePropagator *>(it->get());
 	if (!*prop)
 		return false;
+	// a PeriodicBox is fused only directly after the propagator (it moves positions that the
+	// observer records)
+	auto nx = std::next(it);
+	if (nx != mods.end())
+		if ((*box = dynamic_cast<const PeriodicBox *>(nx->get())))
+			++it;
 	for (++it; it != mods.end(); ++it) {
 		if (const MaximumTrajectoryLength *l = dynamic_cast<const MaximumTrajectoryLength *>(it->get())) {
 			if (*limit || !l->getObserverPositions().empty() || !l->getMakeRejectedInactive() || l->hasRejectAction() ||
@@ -342,7 +353,8 @@ bool ModuleList::runBatched(const candidate_vector_t &cands) {
 	const MaximumTrajectoryLength *limit;
 	const Observer *obs;
 	const ObserverTimeEvolution *ote;
-	if (!fusable(modules, &prop, &limit, &obs, &ote))
+	const PeriodicBox *box;
+	if (!fusable(modules, &prop, &limit, &obs, &ote, &box))
 		return false;
 	std::vector<Candidate *> act;
 	for (auto &c : cands)
@@ -364,11 +376,25 @@ bool ModuleList::runBatched(const candidate_vector_t &cands) {
 		throwIf(crp_propagator_set_observer(p.get(), times.data(), (int)times.size(),
 		                                    obs->getDeactivateOnDetection() ? 1 : 0));
 	}
+	if (box) {
+		const Vector3d o = box->getOrigin(), z = box->getSize();
+		const double oo[3] = {o.x, o.y, o.z}, zz[3] = {z.x, z.y, z.z};
+		throwIf(crp_propagator_set_periodic_box(p.get(), oo, zz));
+	}
 	crp_batch *b = nullptr;
 	throwIf(crp_batch_create(act.size(), &b));
 	BatchHandle bh(b);
 	Packed pk;
 	pack(act, pk);
+	if (box) {
+		pk.src.resize(3 * act.size());
+		pk.cre.resize(3 * act.size());
+		for (size_t i = 0; i < act.size(); i++) {
+			const Vector3d a = act[i]->source.getPosition(), c = act[i]->created.getPosition();
+			pk.src[3 * i] = a.x; pk.src[3 * i + 1] = a.y; pk.src[3 * i + 2] = a.z;
+			pk.cre[3 * i] = c.x; pk.cre[3 * i + 1] = c.y; pk.cre[3 * i + 2] = c.z;
+		}
+	}
 	std::vector<int32_t> det0;
 	if (ote) {
 		// DetectionIndex as ObserverTimeEvolution::checkDetection keeps it (src/module/Observer.cpp:282-290)
@@ -397,6 +423,11 @@ bool ModuleList::runBatched(const candidate_vector_t &cands) {
 		moved[i] = pk.len[i] != act[i]->getTrajectoryLength() || pk.cur[i] != act[i]->getCurrentStep() ||
 		           !pk.active[i];
 	unpack(act, pk, moved, limit);
+	if (box)
+		for (size_t i = 0; i < act.size(); i++) {
+			act[i]->source.setPosition(Vector3d(pk.src[3 * i], pk.src[3 * i + 1], pk.src[3 * i + 2]));
+			act[i]->created.setPosition(Vector3d(pk.cre[3 * i], pk.cre[3 * i + 1], pk.cre[3 * i + 2]));
+		}
 	if (ote) {
 		// Observer::process after the fact (src/module/Observer.cpp:26-57): per candidate, in detection
 		// order, the features' onDetection and the action run on a copy of the candidate in its state at

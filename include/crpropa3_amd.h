@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define CRP_ABI_VERSION 2
+#define CRP_ABI_VERSION 3
 
 enum crp_status {
 	CRP_OK = 0,
@@ -143,6 +143,9 @@ typedef struct {
 	double *prev_dir;          /* [3n] previous.direction (may be NULL) */
 	int32_t *detection_index;  /* [n] observer detections so far, the candidate property
 	                              "DetectionIndex" of src/module/Observer.cpp:282-290 (may be NULL: 0) */
+	double *source_pos;        /* [3n] source.position, moved by a periodic box (may be NULL: the
+	                              current position on upload, not written on download) */
+	double *created_pos;       /* [3n] created.position, likewise */
 } crp_host_state;
 
 int crp_batch_create(size_t capacity, crp_batch_t **out);
@@ -158,6 +161,11 @@ int crp_batch_copy(crp_batch_t *dst, const crp_batch_t *src);
  * at most max_steps process() calls per active particle (max_steps < 0: until every particle is
  * inactive, which needs a trajectory-length limit and minStep > 0).  Synchronous. */
 int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps, crp_run_stats *stats);
+
+/* PeriodicBox(origin, size) fused right after the propagator (src/module/Boundary.cpp:16-27): after
+ * each accepted step n = floor((pos - origin) / size) per axis and, if n != 0, the current, previous,
+ * source and created positions move by -n * size.  size all 0 removes it. */
+int crp_propagator_set_periodic_box(crp_propagator_t *p, const double origin[3], const double size[3]);
 
 /* Observer with one ObserverTimeEvolution, fused after the propagator and the length limit
  * (src/module/Observer.cpp:26-57, 266-306): after each accepted step a particle whose trajectory

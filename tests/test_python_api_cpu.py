@@ -173,3 +173,19 @@ def test_observer_time_evolution_lists_and_detection_logic():
     cand2 = Candidate()
     obs.process(cand2)
     assert not cand2.isActive()  # Observer deactivates on detection by default
+
+
+@pytest.mark.parametrize("origin,size,cur,cre,ecur,ecre", [
+    ((2, 2, 2), (2, 2, 2), (4.5, 4.3, 4.4), (3, 3, 3), (2.5, 2.3, 2.4), (1, 1, 1)),
+    ((0, 0, 0), (2, 2, 2), (-2.5, -0.3, -0.4), (1, 1, 1), (1.5, 1.7, 1.6), (5, 3, 3))])
+def test_periodic_box(origin, size, cur, cre, ecur, ecre):
+    """test/testBreakCondition.cpp:376-414 (PeriodicBox high / low), host process()."""
+    from crpropa3_amd import Candidate, PeriodicBox, Vector3d
+    box = PeriodicBox(Vector3d(*origin), Vector3d(*size))
+    c = Candidate()
+    c.current.setPosition(Vector3d(*cur))
+    c.created.setPosition(Vector3d(*cre))
+    box.process(c)
+    p, q = c.current.getPosition(), c.created.getPosition()
+    assert [p.x, p.y, p.z] == pytest.approx(list(ecur), rel=4 * 2.2e-16)
+    assert [q.x, q.y, q.z] == pytest.approx(list(ecre), rel=4 * 2.2e-16)
