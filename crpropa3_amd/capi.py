@@ -5,6 +5,7 @@ uses to reach the HIP library.  It loads crpropa3_amd/libcrpropa3_amd.so and
 raises if the library is missing: there is no CPU fallback.
 """
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -82,6 +83,8 @@ def lib():
         L.crp_propagator_destroy.argtypes = [_vp]
         L.crp_propagator_set_observer.argtypes = [_vp, _dp, C.c_int, C.c_int]
         L.crp_propagator_set_periodic_box.argtypes = [_vp, _dp, _dp]
+        L.crp_propagator_set_spherical_boundary.argtypes = [_vp, _dp, C.c_double, C.c_double, C.c_int]
+        L.crp_propagator_set_module_order.argtypes = [_vp, C.POINTER(C.c_int), C.c_int]
         L.crp_batch_detections.argtypes = [_vp, C.c_int, _dp]
         L.crp_propagator_set_launch.argtypes = [_vp, C.POINTER(LaunchConfig)]
         L.crp_propagator_get_launch.argtypes = [_vp, C.POINTER(LaunchConfig)]
@@ -195,6 +198,11 @@ class Field:
         return B
 
 
+MODULES = {"box": 1, "sphere": 2, "maxlen": 3, "observer": 4}  # enum crp_module
+# SphericalBoundary's default margin, 0.1 kpc (include/crpropa/Units.h:109-114)
+MODULE_SPHERE_DEFAULT_MARGIN = 0.1 * (1e3 * (648000 / math.pi * 149597870700.0))
+
+
 class Propagator:
     """PropagationBP / PropagationCK with a fused MaximumTrajectoryLength."""
 
@@ -233,6 +241,23 @@ class Propagator:
         o = _f64(origin, (3,))
         z = _f64(size, (3,))
         check(lib().crp_propagator_set_periodic_box(self.handle, _p(o), _p(z)))
+        return self
+
+    def set_spherical_boundary(self, center, radius, margin=MODULE_SPHERE_DEFAULT_MARGIN, limit_step=True):
+        """Fused SphericalBoundary(center, radius) after the box; center None removes it."""
+        if center is None:
+            check(lib().crp_propagator_set_spherical_boundary(self.handle, None, 0.0, 0.0, 0))
+            return self
+        c = _f64(center, (3,))
+        check(lib().crp_propagator_set_spherical_boundary(self.handle, _p(c), radius, margin, 1 if limit_step else 0))
+        return self
+
+    def set_module_order(self, names):
+        """Order of the fused modules after the propagator: names from "box", "sphere", "maxlen",
+        "observer" (ModuleList order); [] restores box, sphere, maxlen, observer."""
+        ids = [MODULES[nm] for nm in names]
+        arr = (C.c_int * max(1, len(ids)))(*ids)
+        check(lib().crp_propagator_set_module_order(self.handle, arr, len(ids)))
         return self
 
     def set_launch(self, block=256, blocks_per_cu=0, steps_per_launch=0, use_graph=False):

@@ -77,6 +77,11 @@ struct crp_propagator {
 	// fused PeriodicBox
 	int box;
 	double boxO[3], boxS[3];
+	// fused SphericalBoundary
+	int sph, sphLimit;
+	double sphC[3], sphR, sphMargin;
+	// Params::postObs of the module order
+	int postObs;
 };
 
 
@@ -841,7 +846,9 @@ static int create_prop(int type, const char *who, const crp_field_t *field, doub
 	p->obsDeactivate = 0;
 	p->box = 0;
 	for (int a = 0; a < 3; a++)
-		p->boxO[a] = p->boxS[a] = 0;
+		p->boxO[a] = p->boxS[a] = p->sphC[a] = 0;
+	p->sph = p->sphLimit = p->postObs = 0;
+	p->sphR = p->sphMargin = 0;
 	*out = p;
 	return CRP_OK;
 }
@@ -903,6 +910,44 @@ int crp_propagator_set_periodic_box(crp_propagator_t *p, const double origin[3],
 		p->boxO[a] = origin[a];
 		p->boxS[a] = size[a];
 	}
+	return CRP_OK;
+}
+
+int crp_propagator_set_spherical_boundary(crp_propagator_t *p, const double center[3], double radius,
+                                          double margin, int limit_step) {
+	if (!p)
+		return fail(CRP_E_INVALID, "crp_propagator_set_spherical_boundary: NULL propagator");
+	if (!center) {
+		p->sph = 0;
+		return CRP_OK;
+	}
+	if (!(radius == radius) || !(margin == margin))
+		return fail(CRP_E_INVALID, "crp_propagator_set_spherical_boundary: radius and margin must be numbers");
+	p->sph = 1;
+	p->sphLimit = limit_step ? 1 : 0;
+	for (int a = 0; a < 3; a++)
+		p->sphC[a] = center[a];
+	p->sphR = radius;
+	p->sphMargin = margin;
+	return CRP_OK;
+}
+
+int crp_propagator_set_module_order(crp_propagator_t *p, const int *modules, int n) {
+	if (!p || n < 0 || (n > 0 && !modules))
+		return fail(CRP_E_INVALID, "crp_propagator_set_module_order: bad arguments");
+	int seen = 0, obsAt = -1, sphAt = -1, lenAt = -1;
+	for (int m = 0; m < n; m++) {
+		const int id = modules[m];
+		if (id < CRP_MODULE_PERIODIC_BOX || id > CRP_MODULE_OBSERVER || (seen & (1 << id)))
+			return fail(CRP_E_INVALID, "crp_propagator_set_module_order: unknown or repeated module");
+		if (id == CRP_MODULE_PERIODIC_BOX && m != 0)
+			return fail(CRP_E_INVALID, "crp_propagator_set_module_order: the periodic box must come first");
+		seen |= 1 << id;
+		if (id == CRP_MODULE_OBSERVER) obsAt = m;
+		if (id == CRP_MODULE_SPHERICAL_BOUNDARY) sphAt = m;
+		if (id == CRP_MODULE_MAX_TRAJECTORY_LENGTH) lenAt = m;
+	}
+	p->postObs = (obsAt >= 0 && sphAt > obsAt ? 1 : 0) | (obsAt >= 0 && lenAt > obsAt ? 2 : 0);
 	return CRP_OK;
 }
 
@@ -1177,6 +1222,7 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 		return CRP_OK;
 	}
 	Params P;
+	std::memset(&P, 0, sizeof(P)); // padding too: the bytes key the captured graph
 	P.tol = p->tolerance;
 	P.minStep = p->minStep;
 	P.maxStep = p->maxStep;
@@ -1190,7 +1236,13 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 		P.boxO[a] = p->boxO[a];
 		P.boxS[a] = p->boxS[a];
 		P.boxInv[a] = p->box ? 1.0 / p->boxS[a] : 0.0;
+		P.sphC[a] = p->sphC[a];
 	}
+	P.sph = p->sph;
+	P.sphLimit = p->sphLimit;
+	P.sphR = p->sphR;
+	P.sphMargin = p->sphMargin;
+	P.postObs = p->postObs;
 	if (p->nObs > 0) {
 		int rc0 = ensure_snapshots(b, p->nObs);
 		if (rc0 != CRP_OK)

@@ -47,6 +47,13 @@ struct Params {
 	// PeriodicBox right after the propagator (box = 0: none); boxInv = RN(1/size)
 	int box;
 	double boxO[3], boxS[3], boxInv[3];
+	// SphericalBoundary after the box (sph = 0: none): reject at |pos - center| >= radius;
+	// sphLimit: limitNextStep(radius - d + margin)
+	int sph, sphLimit;
+	double sphC[3], sphR, sphMargin;
+	// conditions that run after the observer in the module list (bit 0: spherical boundary, bit 1: length
+	// limit); the observer sees, and records, nextStep limited by the modules before it only
+	int postObs;
 };
 
 __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; } // std::max
@@ -314,12 +321,27 @@ k_propagate(F field, Params P, Soa S, unsigned long long n,
 		nAcc++;
 		steps++;
 		bool alive = true;
+		double nextObs = next; // nextStep as the observer sees it
+		if (P.sph) {
+			// SphericalBoundary::process (src/module/Boundary.cpp:216-223), d = Vector3::getR of pos - center
+			const double d = norm(v3(pos.x - P.sphC[0], pos.y - P.sphC[1], pos.z - P.sphC[2]));
+			if (d >= P.sphR)
+				alive = false;
+			if (P.sphLimit) {
+				const double l = (P.sphR - d) + P.sphMargin;
+				next = dmin(next, l);
+				if (!(P.postObs & 1))
+					nextObs = dmin(nextObs, l);
+			}
+		}
 		if (P.maxLen > 0) { // MaximumTrajectoryLength::process
 			if (len >= P.maxLen) {
 				alive = false;
-				nFin++;
 			} else {
-				next = dmin(next, P.maxLen - len);
+				const double l = P.maxLen - len;
+				next = dmin(next, l);
+				if (!(P.postObs & 2))
+					nextObs = dmin(nextObs, l);
 			}
 		}
 		if (P.nObs > 0) {
@@ -331,21 +353,23 @@ k_propagate(F field, Params P, Soa S, unsigned long long n,
 				if (distance < 0.) {
 					next = dmin(next, -distance);
 				} else {
-					if (k < P.nObs - 2)
-						next = dmin(next, P.obsT[k + 1] - len);
+					if (k < P.nObs - 2) {
+						const double l = P.obsT[k + 1] - len;
+						next = dmin(next, l);
+						nextObs = dmin(nextObs, l);
+					}
 					*cDet = (double)(k + 1);
 					double *r = S.snap + (size_t)k * kSnapDoubles * S.cap + idx;
 					const V3 d = MIX ? v3(dirf.x, dirf.y, dirf.z) : dir;
 					r[0] = pos.x; r[S.cap] = pos.y; r[2 * S.cap] = pos.z;
 					r[3 * S.cap] = d.x; r[4 * S.cap] = d.y; r[5 * S.cap] = d.z;
-					r[6 * S.cap] = len; r[7 * S.cap] = *cTim; r[8 * S.cap] = next;
-					if (P.obsDeactivate && alive) {
+					r[6 * S.cap] = len; r[7 * S.cap] = *cTim; r[8 * S.cap] = nextObs;
+					if (P.obsDeactivate)
 						alive = false;
-						nFin++;
-					}
 				}
 			}
 		}
+		nFin += alive ? 0 : 1;
 		if (!alive || (P.maxSteps >= 0 && steps >= P.maxSteps)) {
 			if (MIX)
 				dir = v3(dirf.x, dirf.y, dirf.z);

@@ -8,6 +8,7 @@
 #include "crpropa/Referenced.h"
 
 #include <string>
+#include <typeinfo>
 
 namespace crpropa {
 
@@ -28,8 +29,10 @@ private:
  * class name) and, by default, makes it inactive (src/Module.cpp:19-37). */
 class AbstractCondition : public Module {
 public:
-	AbstractCondition(const std::string &flagValue = "AbstractCondition")
-	    : makeRejectedInactive(true), rejectFlagKey("Rejected"), rejectFlagValue(flagValue) {}
+	// the flag value is the type name seen while the base is constructed, i.e. that of
+	// AbstractCondition itself, for every condition (src/Module.cpp:20-23)
+	AbstractCondition()
+	    : makeRejectedInactive(true), rejectFlagKey("Rejected"), rejectFlagValue(typeid(*this).name()) {}
 	void reject(Candidate *candidate) const {
 		if (!candidate)
 			return;

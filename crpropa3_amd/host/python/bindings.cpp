@@ -357,6 +357,15 @@ PYBIND11_MODULE(_crpropa, m) {
 	    .def("setSize", &PeriodicBox::setSize)
 	    .def("process", [](const PeriodicBox &b, Candidate *c) { b.process(c); });
 
+	py::class_<SphericalBoundary, AbstractCondition, ref_ptr<SphericalBoundary>>(m, "SphericalBoundary")
+	    .def(py::init<>())
+	    .def(py::init<Vector3d, double>())
+	    .def("setCenter", &SphericalBoundary::setCenter)
+	    .def("setRadius", &SphericalBoundary::setRadius)
+	    .def("setMargin", &SphericalBoundary::setMargin)
+	    .def("setLimitStep", &SphericalBoundary::setLimitStep)
+	    .def("process", [](const SphericalBoundary &b, Candidate *c) { b.process(c); });
+
 	// observers (include/crpropa/module/Observer.h)
 	py::enum_<DetectionState>(m, "DetectionState")
 	    .value("DETECTED", DETECTED)

@@ -175,28 +175,3 @@ std::string ObserverTimeEvolution::getDescription() const {
 }
 
 } // namespace crpropa
-
-// ------------------------------------------------------------------ PeriodicBox
-#include "crpropa/module/Boundary.h"
-
-namespace crpropa {
-
-void PeriodicBox::process(Candidate *c) const {
-	Vector3d pos = c->current.getPosition();
-	Vector3d n = ((pos - origin) / size).floor();
-	if ((n.x == 0) && (n.y == 0) && (n.z == 0))
-		return; // inside the box
-	c->current.setPosition(pos - n * size);
-	c->previous.setPosition(c->previous.getPosition() - n * size);
-	c->source.setPosition(c->source.getPosition() - n * size);
-	c->created.setPosition(c->created.getPosition() - n * size);
-}
-
-std::string PeriodicBox::getDescription() const {
-	std::stringstream s;
-	s << "Periodic box: origin " << origin / Mpc << " Mpc, ";
-	s << "size " << size / Mpc << " Mpc";
-	return s.str();
-}
-
-} // namespace crpropa

@@ -76,9 +76,9 @@ static void pack(const std::vector<Candidate *> &c, Packed &p) {
 }
 
 /* Writes device results back; `moved[i]` tells whether candidate i took a step
- * (its previous state is then the state before its last step). */
-static void unpack(const std::vector<Candidate *> &c, const Packed &p, const std::vector<char> &moved,
-                   const MaximumTrajectoryLength *limit) {
+ * (its previous state is then the state before its last step).  A candidate the device
+ * deactivated is left active here: the caller replays the rejections in module order. */
+static void unpack(const std::vector<Candidate *> &c, const Packed &p, const std::vector<char> &moved) {
 	for (size_t i = 0; i < c.size(); i++) {
 		if (!moved[i])
 			continue;
@@ -90,13 +90,6 @@ static void unpack(const std::vector<Candidate *> &c, const Packed &p, const std
 		cand->current.setPosition(Vector3d(p.pos[3 * i], p.pos[3 * i + 1], p.pos[3 * i + 2]));
 		cand->current.setDirectionExact(Vector3d(p.dir[3 * i], p.dir[3 * i + 1], p.dir[3 * i + 2]));
 		cand->setStepState(p.cur[i], p.next[i], p.len[i], p.time[i]);
-		if (!p.active[i] && cand->isActive()) {
-			// the length limit rejects (flag "Rejected"); otherwise an observer deactivated it
-			if (limit && p.len[i] >= limit->getMaximumTrajectoryLength())
-				limit->reject(cand);
-			else
-				cand->setActive(false);
-		}
 	}
 }
 
@@ -202,7 +195,9 @@ void DevicePropagator::process(Candidate *candidate) const {
 	throwIf(crp_batch_upload(b, 1, &s));
 	throwIf(crp_propagate(p.get(), b, 1, nullptr));
 	throwIf(crp_batch_download(b, &s));
-	unpack(one, pk, std::vector<char>(1, 1), nullptr);
+	unpack(one, pk, std::vector<char>(1, 1));
+	if (!pk.active[0])
+		one[0]->setActive(false);
 }
 
 PropagationBP::PropagationBP(ref_ptr<MagneticField> f, double fixedStep) : DevicePropagator(BP) {
@@ -246,7 +241,7 @@ std::string PropagationCK::getDescription() const {
 
 // ------------------------------------------------------------------ MaximumTrajectoryLength
 MaximumTrajectoryLength::MaximumTrajectoryLength(double length)
-    : AbstractCondition("MaximumTrajectoryLength"), maxLength(length) {}
+    : maxLength(length) {}
 
 void MaximumTrajectoryLength::process(Candidate *c) const {
 	double length = c->getTrajectoryLength();
@@ -302,60 +297,86 @@ void ModuleList::run(Candidate *candidate, bool recursive, bool secondariesFirst
 	run(&v, recursive, secondariesFirst);
 }
 
-/* The fused form: a DevicePropagator followed, in any order, by at most one
- * MaximumTrajectoryLength without observer positions (default flags) and at most one
- * Observer holding exactly one ObserverTimeEvolution.  Their order does not matter: both
- * only limit nextStep (a minimum) and deactivate, and every module of a step runs even
- * on a candidate deactivated earlier in that step (src/ModuleList.cpp.in:54-58). */
-static bool fusable(const ModuleList::module_list_t &mods, const DevicePropagator **prop,
-                    const MaximumTrajectoryLength **limit, const Observer **obs,
-                    const ObserverTimeEvolution **ote, const PeriodicBox **box) {
-	*prop = nullptr;
-	*limit = nullptr;
-	*obs = nullptr;
-	*ote = nullptr;
-	*box = nullptr;
-	if (mods.empty() || mods.size() > 4)
+/* The fused form: a DevicePropagator, optionally a PeriodicBox right after it, then in any order at
+ * most one SphericalBoundary and one MaximumTrajectoryLength (default flags: they deactivate, no
+ * reject action, no observer positions) and at most one Observer holding exactly one
+ * ObserverTimeEvolution.  Every module of a step runs even on a candidate deactivated earlier in
+ * that step (src/ModuleList.cpp.in:54-58); the kernel applies the limits (minima, so their order
+ * does not matter) and deactivations, and the observer records nextStep as limited by the modules
+ * listed before it (crp_propagator_set_module_order). */
+struct Fused {
+	const DevicePropagator *prop = nullptr;
+	const PeriodicBox *box = nullptr;
+	const SphericalBoundary *sphere = nullptr;
+	const MaximumTrajectoryLength *limit = nullptr;
+	const Observer *obs = nullptr;
+	const ObserverTimeEvolution *ote = nullptr;
+	std::vector<int> order;                      // CRP_MODULE_* after the propagator, in list order
+	std::vector<const AbstractCondition *> conds; // rejecting modules in list order
+	size_t condsBeforeObs = 0;                   // how many of them run before the observer
+
+	// would condition c reject a candidate at this position and trajectory length?
+	bool rejects(const AbstractCondition *c, const Vector3d &pos, double len) const {
+		if (c == sphere)
+			return (pos - sphere->getCenter()).getR() >= sphere->getRadius();
+		return len >= limit->getMaximumTrajectoryLength();
+	}
+};
+
+static bool fusable(const ModuleList::module_list_t &mods, Fused &f) {
+	f = Fused();
+	if (mods.empty() || mods.size() > 5)
 		return false;
 	auto it = mods.begin();
-	*prop = dynamic_cast<const DevicePropagator *>(it->get());
-	if (!*prop)
+	f.prop = dynamic_cast<const DevicePropagator *>(it->get());
+	if (!f.prop)
 		return false;
-	// a PeriodicBox is fused only directly after the propagator (it moves positions that the
-	// observer records)
+	// a PeriodicBox is fused only directly after the propagator (it moves the positions that the
+	// boundary tests and the observer records)
 	auto nx = std::next(it);
 	if (nx != mods.end())
-		if ((*box = dynamic_cast<const PeriodicBox *>(nx->get())))
+		if ((f.box = dynamic_cast<const PeriodicBox *>(nx->get()))) {
+			f.order.push_back(CRP_MODULE_PERIODIC_BOX);
 			++it;
+		}
 	for (++it; it != mods.end(); ++it) {
 		if (const MaximumTrajectoryLength *l = dynamic_cast<const MaximumTrajectoryLength *>(it->get())) {
-			if (*limit || !l->getObserverPositions().empty() || !l->getMakeRejectedInactive() || l->hasRejectAction() ||
+			if (f.limit || !l->getObserverPositions().empty() || !l->getMakeRejectedInactive() || l->hasRejectAction() ||
 			    !(l->getMaximumTrajectoryLength() > 0))
 				return false;
-			*limit = l;
+			f.limit = l;
+			f.conds.push_back(l);
+			f.order.push_back(CRP_MODULE_MAX_TRAJECTORY_LENGTH);
+		} else if (const SphericalBoundary *b = dynamic_cast<const SphericalBoundary *>(it->get())) {
+			if (f.sphere || !b->getMakeRejectedInactive() || b->hasRejectAction())
+				return false;
+			f.sphere = b;
+			f.conds.push_back(b);
+			f.order.push_back(CRP_MODULE_SPHERICAL_BOUNDARY);
 		} else if (const Observer *o = dynamic_cast<const Observer *>(it->get())) {
-			if (*obs || o->getFeatures().size() != 1)
+			if (f.obs || o->getFeatures().size() != 1)
 				return false;
 			const ObserverTimeEvolution *t = dynamic_cast<const ObserverTimeEvolution *>(o->getFeatures()[0].get());
 			if (!t || t->getNIntervals() <= 0)
 				return false;
-			*obs = o;
-			*ote = t;
+			f.obs = o;
+			f.ote = t;
+			f.condsBeforeObs = f.conds.size();
+			f.order.push_back(CRP_MODULE_OBSERVER);
 		} else {
 			return false;
 		}
 	}
+	if (!f.obs)
+		f.condsBeforeObs = f.conds.size();
 	return true;
 }
 
 bool ModuleList::runBatched(const candidate_vector_t &cands) {
-	const DevicePropagator *prop;
-	const MaximumTrajectoryLength *limit;
-	const Observer *obs;
-	const ObserverTimeEvolution *ote;
-	const PeriodicBox *box;
-	if (!fusable(modules, &prop, &limit, &obs, &ote, &box))
+	Fused f;
+	if (!fusable(modules, f))
 		return false;
+	const DevicePropagator *prop = f.prop;
 	std::vector<Candidate *> act;
 	for (auto &c : cands)
 		if (c.valid() && c->isActive())
@@ -364,29 +385,36 @@ bool ModuleList::runBatched(const candidate_vector_t &cands) {
 	lastKernel = 0;
 	if (act.empty())
 		return true;
-	const double maxLen = limit ? limit->getMaximumTrajectoryLength() : 0;
+	const double maxLen = f.limit ? f.limit->getMaximumTrajectoryLength() : 0;
 	long long steps = maxStepsPerRun;
 	if (steps < 0 && !(maxLen > 0 && prop->getMinimumStep() > 0))
 		throw std::runtime_error("ModuleList::run: the module list never deactivates candidates "
 		                         "(add MaximumTrajectoryLength and use minStep > 0, or setMaximumSteps)");
 	PropHandle p(prop->makeDevicePropagator(maxLen));
 	std::vector<double> times;
-	if (ote) {
-		times = ote->getTimes();
+	if (f.ote) {
+		times = f.ote->getTimes();
 		throwIf(crp_propagator_set_observer(p.get(), times.data(), (int)times.size(),
-		                                    obs->getDeactivateOnDetection() ? 1 : 0));
+		                                    f.obs->getDeactivateOnDetection() ? 1 : 0));
 	}
-	if (box) {
-		const Vector3d o = box->getOrigin(), z = box->getSize();
+	if (f.box) {
+		const Vector3d o = f.box->getOrigin(), z = f.box->getSize();
 		const double oo[3] = {o.x, o.y, o.z}, zz[3] = {z.x, z.y, z.z};
 		throwIf(crp_propagator_set_periodic_box(p.get(), oo, zz));
 	}
+	if (f.sphere) {
+		const Vector3d c = f.sphere->getCenter();
+		const double cc[3] = {c.x, c.y, c.z};
+		throwIf(crp_propagator_set_spherical_boundary(p.get(), cc, f.sphere->getRadius(), f.sphere->getMargin(),
+		                                              f.sphere->getLimitStep() ? 1 : 0));
+	}
+	throwIf(crp_propagator_set_module_order(p.get(), f.order.data(), (int)f.order.size()));
 	crp_batch *b = nullptr;
 	throwIf(crp_batch_create(act.size(), &b));
 	BatchHandle bh(b);
 	Packed pk;
 	pack(act, pk);
-	if (box) {
+	if (f.box) {
 		pk.src.resize(3 * act.size());
 		pk.cre.resize(3 * act.size());
 		for (size_t i = 0; i < act.size(); i++) {
@@ -396,7 +424,7 @@ bool ModuleList::runBatched(const candidate_vector_t &cands) {
 		}
 	}
 	std::vector<int32_t> det0;
-	if (ote) {
+	if (f.ote) {
 		// DetectionIndex as ObserverTimeEvolution::checkDetection keeps it (src/module/Observer.cpp:282-290)
 		pk.det.resize(act.size());
 		for (size_t i = 0; i < act.size(); i++)
@@ -411,7 +439,7 @@ bool ModuleList::runBatched(const candidate_vector_t &cands) {
 	throwIf(crp_propagate(p.get(), b, steps, &st));
 	throwIf(crp_batch_download(b, &s));
 	std::vector<double> snaps;
-	if (ote) {
+	if (f.ote) {
 		snaps.resize(times.size() * act.size() * 9);
 		throwIf(crp_batch_detections(b, (int)times.size(), snaps.data()));
 	}
@@ -422,41 +450,57 @@ bool ModuleList::runBatched(const candidate_vector_t &cands) {
 	for (size_t i = 0; i < act.size(); i++)
 		moved[i] = pk.len[i] != act[i]->getTrajectoryLength() || pk.cur[i] != act[i]->getCurrentStep() ||
 		           !pk.active[i];
-	unpack(act, pk, moved, limit);
-	if (box)
+	unpack(act, pk, moved);
+	if (f.box)
 		for (size_t i = 0; i < act.size(); i++) {
 			act[i]->source.setPosition(Vector3d(pk.src[3 * i], pk.src[3 * i + 1], pk.src[3 * i + 2]));
 			act[i]->created.setPosition(Vector3d(pk.cre[3 * i], pk.cre[3 * i + 1], pk.cre[3 * i + 2]));
 		}
-	if (ote) {
+	if (f.ote) {
 		// Observer::process after the fact (src/module/Observer.cpp:26-57): per candidate, in detection
 		// order, the features' onDetection and the action run on a copy of the candidate in its state at
-		// detection (position, direction, trajectory length, time, nextStep; DetectionIndex = k + 1)
+		// detection (position, direction, trajectory length, time, nextStep; DetectionIndex = k + 1),
+		// rejected by the conditions listed before the observer that reject that state
 		const size_t n = act.size();
 		for (size_t i = 0; i < n; i++) {
 			Candidate *cand = act[i];
 			for (int32_t k = det0[i]; k < pk.det[i] && k < (int32_t)times.size(); k++) {
 				const double *r = &snaps[((size_t)k * n + i) * 9];
 				ref_ptr<Candidate> view = cand->clone(false);
-				view->current.setPosition(Vector3d(r[0], r[1], r[2]));
+				const Vector3d vpos(r[0], r[1], r[2]);
+				view->current.setPosition(vpos);
 				view->current.setDirectionExact(Vector3d(r[3], r[4], r[5]));
 				view->setStepState(view->getCurrentStep(), r[8], r[6], r[7]);
-				// active when the observer ran, unless the length limit had rejected it in that step
-				view->setActive(!(limit && r[6] >= limit->getMaximumTrajectoryLength()));
+				view->setActive(true);
+				for (size_t c = 0; c < f.condsBeforeObs; c++)
+					if (f.rejects(f.conds[c], vpos, r[6]))
+						f.conds[c]->reject(view.get());
 				view->setProperty("DetectionIndex", std::to_string(k + 1));
-				if (!obs->getFlagKey().empty())
-					view->setProperty(obs->getFlagKey(), obs->getFlagValue());
-				for (const ref_ptr<ObserverFeature> &f : obs->getFeatures())
-					f->onDetection(view.get());
-				if (Module *a = obs->getDetectionAction())
+				if (!f.obs->getFlagKey().empty())
+					view->setProperty(f.obs->getFlagKey(), f.obs->getFlagValue());
+				for (const ref_ptr<ObserverFeature> &ft : f.obs->getFeatures())
+					ft->onDetection(view.get());
+				if (Module *a = f.obs->getDetectionAction())
 					a->process(view.get());
 			}
 			if (pk.det[i] > det0[i]) {
 				cand->setProperty("DetectionIndex", std::to_string(pk.det[i]));
-				if (!obs->getFlagKey().empty())
-					cand->setProperty(obs->getFlagKey(), obs->getFlagValue());
+				if (!f.obs->getFlagKey().empty())
+					cand->setProperty(f.obs->getFlagKey(), f.obs->getFlagValue());
 			}
 		}
+	}
+	// deactivations of the last step, in module order: each condition that rejects the final state
+	// rejects the candidate (flag, inactive); otherwise the observer deactivated it
+	for (size_t i = 0; i < act.size(); i++) {
+		Candidate *cand = act[i];
+		if (pk.active[i] || !cand->isActive())
+			continue;
+		const Vector3d pos = cand->current.getPosition();
+		for (const AbstractCondition *c : f.conds)
+			if (f.rejects(c, pos, pk.len[i]))
+				c->reject(cand);
+		cand->setActive(false);
 	}
 	lastAccepted = st.accepted_steps;
 	lastKernel = st.kernel_ms;
@@ -490,7 +534,7 @@ void ModuleList::runStepwise(const candidate_vector_t &cands) {
 		throwIf(crp_batch_upload(b, act.size(), &s));
 		throwIf(crp_propagate(p.get(), b, 1, nullptr));
 		throwIf(crp_batch_download(b, &s));
-		unpack(act, pk, std::vector<char>(act.size(), 1), nullptr);
+		unpack(act, pk, std::vector<char>(act.size(), 1));
 		for (Candidate *c : act)
 			for (auto m = std::next(modules.begin()); m != modules.end(); ++m)
 				(*m)->process(c);

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define CRP_ABI_VERSION 3
+#define CRP_ABI_VERSION 4
 
 enum crp_status {
 	CRP_OK = 0,
@@ -166,6 +166,25 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
  * each accepted step n = floor((pos - origin) / size) per axis and, if n != 0, the current, previous,
  * source and created positions move by -n * size.  size all 0 removes it. */
 int crp_propagator_set_periodic_box(crp_propagator_t *p, const double origin[3], const double size[3]);
+
+/* SphericalBoundary(center, radius) fused after the box (src/module/Boundary.cpp:208-223): after each
+ * accepted step d = |pos - center|; d >= radius rejects the particle (inactive); with limit_step the
+ * next step is limited to radius - d + margin (the reference's defaults: limit_step 1, margin 0.1 kpc).
+ * center NULL removes it. */
+int crp_propagator_set_spherical_boundary(crp_propagator_t *p, const double center[3], double radius,
+                                          double margin, int limit_step);
+
+/* The order of the fused modules after the propagator, as in the ModuleList (src/ModuleList.cpp.in:54-58):
+ * a permutation of the configured modules, CRP_MODULE_* values.  Positions and rejections do not depend
+ * on it; an observer's detections see nextStep limited only by the boundary and length limit listed
+ * before it.  The periodic box, if listed, must come first.  n = 0: box, sphere, length limit, observer. */
+enum crp_module {
+	CRP_MODULE_PERIODIC_BOX = 1,
+	CRP_MODULE_SPHERICAL_BOUNDARY = 2,
+	CRP_MODULE_MAX_TRAJECTORY_LENGTH = 3,
+	CRP_MODULE_OBSERVER = 4
+};
+int crp_propagator_set_module_order(crp_propagator_t *p, const int *modules, int n);
 
 /* Observer with one ObserverTimeEvolution, fused after the propagator and the length limit
  * (src/module/Observer.cpp:26-57, 266-306): after each accepted step a particle whose trajectory

@@ -432,6 +432,83 @@ static inline double orc_min(double a, double b) { return (b < a) ? b : a; } /* 
 /* One process() call of PropagationBP (src/module/PropagationBP.cpp:62-124) or
  * PropagationCK (src/module/PropagationCK.cpp:77-136), followed by
  * MaximumTrajectoryLength::process (src/module/BreakCondition.cpp:42-64). */
+/* PeriodicBox::process (src/module/Boundary.cpp:16-27): n = floor((pos - origin) / size);
+ * current, previous, source and created positions move by -n * size */
+static void orc_periodic_box(const orc_propagator *p, orc_state *s, size_t i, double *pos) {
+	if (!p->box)
+		return;
+	double nb[3];
+	for (int a = 0; a < 3; a++)
+		nb[a] = floor((pos[a] - p->boxOrigin[a]) / p->boxSize[a]);
+	if ((nb[0] == 0) && (nb[1] == 0) && (nb[2] == 0))
+		return;
+	for (int a = 0; a < 3; a++) {
+		const double sh = nb[a] * p->boxSize[a];
+		pos[a] = pos[a] - sh;
+		if (s->prev_pos)
+			s->prev_pos[3 * i + a] = s->prev_pos[3 * i + a] - sh;
+		if (s->source_pos)
+			s->source_pos[3 * i + a] = s->source_pos[3 * i + a] - sh;
+		if (s->created_pos)
+			s->created_pos[3 * i + a] = s->created_pos[3 * i + a] - sh;
+	}
+}
+
+/* SphericalBoundary::process (src/module/Boundary.cpp:216-223): d = |pos - center|
+ * (Vector3::getR, include/crpropa/Vector3.h:126-128); reject when d >= radius;
+ * limitNextStep(radius - d + margin) */
+static void orc_spherical_boundary(const orc_propagator *p, orc_state *s, size_t i, const double *pos) {
+	if (!p->sphere)
+		return;
+	const double dx = pos[0] - p->sphereCenter[0], dy = pos[1] - p->sphereCenter[1], dz = pos[2] - p->sphereCenter[2];
+	const double d = sqrt(dx * dx + dy * dy + dz * dz);
+	if (d >= p->sphereRadius)
+		s->active[i] = 0;
+	if (p->sphereLimitStep)
+		s->next_step[i] = orc_min(s->next_step[i], p->sphereRadius - d + p->sphereMargin);
+}
+
+/* MaximumTrajectoryLength::process (src/module/BreakCondition.cpp:42-64) without observer positions */
+static void orc_max_length(const orc_propagator *p, orc_state *s, size_t i) {
+	if (!(p->maxTrajectoryLength > 0))
+		return;
+	const double len = s->trajectory_length[i];
+	if (len >= p->maxTrajectoryLength)
+		s->active[i] = 0;
+	else
+		s->next_step[i] = orc_min(s->next_step[i], p->maxTrajectoryLength - len);
+}
+
+/* Observer::process with ObserverTimeEvolution::checkDetection (src/module/Observer.cpp:26-57,
+ * 279-306): one detection per call at most; limitNextStep towards the next time; the snapshot
+ * holds the state the observer sees (nextStep limited by the modules before it only) */
+static void orc_observer(const orc_propagator *p, orc_state *s, size_t i, const double *pos, const double *dir) {
+	if (p->nObs <= 0)
+		return;
+	const int k = s->det_index[i];
+	if (k >= p->nObs)
+		return;
+	const double len = s->trajectory_length[i];
+	const double distance = len - p->obsTimes[k];
+	if (distance < 0.) {
+		s->next_step[i] = orc_min(s->next_step[i], -distance);
+		return;
+	}
+	if (k < p->nObs - 2)
+		s->next_step[i] = orc_min(s->next_step[i], p->obsTimes[k + 1] - len);
+	s->det_index[i] = k + 1;
+	double *r = s->snap + ((size_t)k * s->n + i) * 9;
+	for (int a = 0; a < 3; a++) {
+		r[a] = pos[a];
+		r[3 + a] = dir[a];
+	}
+	r[6] = len;
+	r[7] = s->time[i];
+	r[8] = s->next_step[i];
+	if (p->obsDeactivate)
+		s->active[i] = 0;
+}
+
 static inline void orc_process(const orc_field *f, const orc_propagator *p, orc_state *s, size_t i, int libm,
                                long long *trials) {
 	double *pos = s->pos + 3 * i, *dir = s->dir + 3 * i;
@@ -516,56 +593,15 @@ static inline void orc_process(const orc_field *f, const orc_propagator *p, orc_
 	s->trajectory_length[i] += step;
 	s->time[i] += step / C_LIGHT;
 	s->next_step[i] = newStep;
-	/* PeriodicBox::process (src/module/Boundary.cpp:16-27): n = floor((pos - origin) / size);
-	 * current, previous, source and created positions move by -n * size */
-	if (p->box) {
-		double nb[3];
-		for (int a = 0; a < 3; a++)
-			nb[a] = floor((pos[a] - p->boxOrigin[a]) / p->boxSize[a]);
-		if (!((nb[0] == 0) && (nb[1] == 0) && (nb[2] == 0))) {
-			for (int a = 0; a < 3; a++) {
-				const double sh = nb[a] * p->boxSize[a];
-				pos[a] = pos[a] - sh;
-				if (s->prev_pos)
-					s->prev_pos[3 * i + a] = s->prev_pos[3 * i + a] - sh;
-				if (s->source_pos)
-					s->source_pos[3 * i + a] = s->source_pos[3 * i + a] - sh;
-				if (s->created_pos)
-					s->created_pos[3 * i + a] = s->created_pos[3 * i + a] - sh;
-			}
-		}
-	}
-	if (p->maxTrajectoryLength > 0) {
-		double len = s->trajectory_length[i];
-		if (len >= p->maxTrajectoryLength)
-			s->active[i] = 0;
-		else
-			s->next_step[i] = orc_min(s->next_step[i], p->maxTrajectoryLength - len);
-	}
-	/* Observer::process with ObserverTimeEvolution::checkDetection (src/module/Observer.cpp:26-57,
-	 * 279-306): one detection per call at most; limitNextStep towards the next time */
-	if (p->nObs > 0) {
-		int k = s->det_index[i];
-		if (k < p->nObs) {
-			double len = s->trajectory_length[i];
-			double distance = len - p->obsTimes[k];
-			if (distance < 0.) {
-				s->next_step[i] = orc_min(s->next_step[i], -distance);
-			} else {
-				if (k < p->nObs - 2)
-					s->next_step[i] = orc_min(s->next_step[i], p->obsTimes[k + 1] - len);
-				s->det_index[i] = k + 1;
-				double *r = s->snap + ((size_t)k * s->n + i) * 9;
-				for (int a = 0; a < 3; a++) {
-					r[a] = pos[a];
-					r[3 + a] = dir[a];
-				}
-				r[6] = len;
-				r[7] = s->time[i];
-				r[8] = s->next_step[i];
-				if (p->obsDeactivate)
-					s->active[i] = 0;
-			}
+	static const int defaultOrder[4] = {ORC_MOD_BOX, ORC_MOD_SPHERE, ORC_MOD_MAXLEN, ORC_MOD_OBSERVER};
+	const int *order = p->nOrder > 0 ? p->order : defaultOrder;
+	const int nOrder = p->nOrder > 0 ? p->nOrder : 4;
+	for (int m = 0; m < nOrder; m++) {
+		switch (order[m]) {
+		case ORC_MOD_BOX: orc_periodic_box(p, s, i, pos); break;
+		case ORC_MOD_SPHERE: orc_spherical_boundary(p, s, i, pos); break;
+		case ORC_MOD_MAXLEN: orc_max_length(p, s, i); break;
+		case ORC_MOD_OBSERVER: orc_observer(p, s, i, pos, dir); break;
 		}
 	}
 }

@@ -26,6 +26,8 @@ extern "C" {
 enum { ORC_LIBM_GLIBC = 0, ORC_LIBM_CR = 1 };
 enum { ORC_FIELD_NONE = 0, ORC_FIELD_UNIFORM = 1, ORC_FIELD_GRID = 2, ORC_FIELD_PLANEWAVE = 3 };
 enum { ORC_PROP_BP = 0, ORC_PROP_CK = 1 };
+/* modules after the propagator, for orc_propagator.order */
+enum { ORC_MOD_BOX = 1, ORC_MOD_SPHERE = 2, ORC_MOD_MAXLEN = 3, ORC_MOD_OBSERVER = 4 };
 
 typedef struct {
 	int type;
@@ -51,6 +53,12 @@ typedef struct {
 	/* PeriodicBox(origin, size) right after the propagator (box = 0: none) */
 	int box;
 	double boxOrigin[3], boxSize[3];
+	/* SphericalBoundary(center, radius) after the box (sphere = 0: none); limitStep with margin */
+	int sphere, sphereLimitStep;
+	double sphereCenter[3], sphereRadius, sphereMargin;
+	/* the order in which the configured modules run after the propagator (ModuleList order);
+	 * nOrder = 0: box, sphere, length limit, observer */
+	int order[4], nOrder;
 } orc_propagator;
 
 /* Particle state, structure of arrays.  pos/dir are AoS (3 per particle). */

@@ -260,19 +260,22 @@ public:
 	struct Rec {
 		Vector3d pos, dir;
 		double len, time, next;
+		int active, rejected; // the candidate's flags as the observer saw them
 	};
 	std::vector<std::vector<Rec>> *per;
 	void process(Candidate *c) const {
 		size_t pid = c->getProperty("pid").asUInt64();
 		(*per)[pid].push_back(Rec{c->current.getPosition(), c->current.getDirection(), c->getTrajectoryLength(),
-		                          c->getTime(), c->getNextStep()});
+		                          c->getTime(), c->getNextStep(), c->isActive() ? 1 : 0,
+		                          c->hasProperty("Rejected") ? 1 : 0});
 	}
 };
 
 // ModuleList order [propagator, MaximumTrajectoryLength (optional), Observer(ObserverTimeEvolution)].
-static void run_observer_case(const std::string &name, const Module &prop, const MaximumTrajectoryLength *maxlen,
-                              ObserverTimeEvolution *ote, bool deactivate, const std::vector<Particle> &ps,
-                              size_t maxSteps, const std::string &meta) {
+// `pre` run between the propagator and the observer, `post` after it.
+static void run_observer_case(const std::string &name, const Module &prop, const std::vector<const Module *> &pre,
+                              const std::vector<const Module *> &post, ObserverTimeEvolution *ote, bool deactivate,
+                              const std::vector<Particle> &ps, size_t maxSteps, const std::string &meta) {
 	std::string dir = casedir(name);
 	save_particles(dir, ps);
 	const size_t N = ps.size();
@@ -293,8 +296,11 @@ static void run_observer_case(const std::string &name, const Module &prop, const
 		size_t step = 0;
 		while (step < maxSteps && c->isActive()) {
 			prop.process(c);
-			if (maxlen) maxlen->process(c);
+			for (const Module *m : pre)
+				m->process(c);
 			obs->process(c);
+			for (const Module *m : post)
+				m->process(c);
 			step++;
 		}
 		Vector3d p = c->current.getPosition(), u = c->current.getDirection();
@@ -308,7 +314,7 @@ static void run_observer_case(const std::string &name, const Module &prop, const
 	const std::vector<double> &T = ote->getTimes();
 	const size_t K = T.size();
 	std::vector<double> dpos(K * N * 3, NAN), ddir(K * N * 3, NAN), dlen(K * N, NAN), dtim(K * N, NAN), dnext(K * N, NAN);
-	std::vector<int32_t> count(N);
+	std::vector<int32_t> count(N), dact(K * N, -1), drej(K * N, -1);
 	for (size_t i = 0; i < N; i++) {
 		count[i] = (int32_t)per[i].size();
 		for (size_t k = 0; k < per[i].size() && k < K; k++) {
@@ -317,6 +323,7 @@ static void run_observer_case(const std::string &name, const Module &prop, const
 			dpos[3 * o] = r.pos.x; dpos[3 * o + 1] = r.pos.y; dpos[3 * o + 2] = r.pos.z;
 			ddir[3 * o] = r.dir.x; ddir[3 * o + 1] = r.dir.y; ddir[3 * o + 2] = r.dir.z;
 			dlen[o] = r.len; dtim[o] = r.time; dnext[o] = r.next;
+			dact[o] = r.active; drej[o] = r.rejected;
 		}
 	}
 	save_f64(dir, "times", T, {K});
@@ -326,6 +333,8 @@ static void run_observer_case(const std::string &name, const Module &prop, const
 	save_f64(dir, "det_time", dtim, {K, N});
 	save_f64(dir, "det_next", dnext, {K, N});
 	save_i32(dir, "det_count", count, {N});
+	save_i32(dir, "det_active", dact, {K, N});
+	save_i32(dir, "det_rejected", drej, {K, N});
 	save_f64(dir, "final_pos", fpos, {N, 3});
 	save_f64(dir, "final_dir", fdir, {N, 3});
 	save_f64(dir, "final_len", flen, {N});
@@ -339,9 +348,11 @@ static void run_observer_case(const std::string &name, const Module &prop, const
 // ---------------------------------------------------------------- periodic box
 // ModuleList order [propagator, PeriodicBox, MaximumTrajectoryLength (optional)];
 // records current/previous/source/created positions after the listed step counts.
-static void run_box_case(const std::string &name, const Module &prop, const PeriodicBox &box,
-                         const MaximumTrajectoryLength *maxlen, const std::vector<Particle> &ps, size_t maxSteps,
-                         const std::vector<size_t> &snaps, const std::string &meta) {
+// propagator followed by modules that move or reject the candidate (PeriodicBox, SphericalBoundary,
+// MaximumTrajectoryLength), in list order, as ModuleList::process runs them
+static void run_modules_case(const std::string &name, const Module &prop, const std::vector<const Module *> &mods,
+                             const std::vector<Particle> &ps, size_t maxSteps, const std::vector<size_t> &snaps,
+                             const std::string &meta) {
 	std::string dir = casedir(name);
 	save_particles(dir, ps);
 	const size_t N = ps.size(), S = snaps.size();
@@ -370,8 +381,8 @@ static void run_box_case(const std::string &name, const Module &prop, const Peri
 		size_t si = 0, step = 0;
 		while (step < maxSteps && c->isActive()) {
 			prop.process(c);
-			box.process(c);
-			if (maxlen) maxlen->process(c);
+			for (const Module *m : mods)
+				m->process(c);
 			step++;
 			while (si < S && snaps[si] == step)
 				record(si++);
@@ -640,7 +651,7 @@ int main(int argc, char **argv) {
 		PeriodicBox box(Vector3d(0.), Vector3d(n * sp));
 		std::vector<Particle> ps = make_particles(128, 50, 1 * EeV, 100 * EeV, centre, proton, one);
 		PropagationBP bp(fgrid, 0.1 * Mpc);
-		run_box_case("bp_fixed_grid32_box", bp, box, nullptr, ps, 500, {1, 10, 100, 500},
+		run_modules_case("bp_fixed_grid32_box", bp, {&box}, ps, 500, {1, 10, 100, 500},
 		             "{\"propagator\": \"BP\", \"fixedStep\": " + fmt(0.1 * Mpc) +
 		                 ", \"field\": \"grid32\", \"box\": {\"origin\": [0, 0, 0], \"size\": [" + fmt(n * sp) +
 		                 ", " + fmt(n * sp) + ", " + fmt(n * sp) + "]}, \"particle_seed\": 50}");
@@ -650,12 +661,40 @@ int main(int argc, char **argv) {
 		std::vector<Particle> ps = make_particles(128, 51, 1 * EeV, 300 * EeV, centre, species, wts);
 		PropagationCK ck(fgrid, 1e-4, 1 * kpc, 1 * Mpc);
 		MaximumTrajectoryLength ml(15 * Mpc);
-		run_box_case("ck_grid32_box", ck, box, &ml, ps, 4000, {1, 10, 100, 1000, 4000},
+		run_modules_case("ck_grid32_box", ck, {&box, &ml}, ps, 4000, {1, 10, 100, 1000, 4000},
 		             "{\"propagator\": \"CK\", \"tolerance\": 1e-4, \"minStep\": " + fmt(kpc) +
 		                 ", \"maxStep\": " + fmt(Mpc) + ", \"maxTrajectoryLength\": " + fmt(15 * Mpc) +
 		                 ", \"field\": \"grid32\", \"box\": {\"origin\": [" + fmt(-0.2 * Mpc) + ", " + fmt(0.1 * Mpc) +
 		                 ", 0], \"size\": [" + fmt(1.1 * Mpc) + ", " + fmt(1.3 * Mpc) + ", " + fmt(0.9 * Mpc) +
 		                 "]}, \"particle_seed\": 51}");
+	}
+
+	// ---- SphericalBoundary (reject outside, limitStep with margin): fixed-step BP alone, adaptive CK
+	// with a length limit
+	{
+		std::vector<Particle> ps = make_particles(128, 54, 1 * EeV, 100 * EeV, centre, proton, one);
+		PropagationBP bp(fgrid, 0.1 * Mpc);
+		Vector3d sc = centre + Vector3d(0.1 * Mpc, -0.05 * Mpc, 0.);
+		SphericalBoundary sph(sc, 1.2 * Mpc);
+		run_modules_case("bp_fixed_grid32_sphere", bp, {&sph}, ps, 300, {1, 10, 100, 300},
+		                 "{\"propagator\": \"BP\", \"fixedStep\": " + fmt(0.1 * Mpc) +
+		                     ", \"field\": \"grid32\", \"sphere\": {\"center\": [" + fmt(sc.x) + ", " + fmt(sc.y) +
+		                     ", " + fmt(sc.z) + "], \"radius\": " + fmt(1.2 * Mpc) + ", \"margin\": " + fmt(0.1 * kpc) +
+		                     ", \"limitStep\": true}, \"particle_seed\": 54}");
+	}
+	{
+		std::vector<Particle> ps = make_particles(128, 55, 1 * EeV, 300 * EeV, centre, species, wts);
+		PropagationCK ck(fgrid, 1e-4, 1 * kpc, 1 * Mpc);
+		Vector3d sc = centre + Vector3d(-0.1 * Mpc, 0., 0.2 * Mpc);
+		SphericalBoundary sph(sc, 2 * Mpc);
+		sph.setMargin(0.05 * Mpc);
+		MaximumTrajectoryLength ml(15 * Mpc);
+		run_modules_case("ck_grid32_sphere", ck, {&sph, &ml}, ps, 4000, {1, 10, 100, 1000, 4000},
+		                 "{\"propagator\": \"CK\", \"tolerance\": 1e-4, \"minStep\": " + fmt(kpc) +
+		                     ", \"maxStep\": " + fmt(Mpc) + ", \"maxTrajectoryLength\": " + fmt(15 * Mpc) +
+		                     ", \"field\": \"grid32\", \"sphere\": {\"center\": [" + fmt(sc.x) + ", " + fmt(sc.y) +
+		                     ", " + fmt(sc.z) + "], \"radius\": " + fmt(2 * Mpc) + ", \"margin\": " + fmt(0.05 * Mpc) +
+		                     ", \"limitStep\": true}, \"particle_seed\": 55}");
 	}
 
 	// ---- ObserverTimeEvolution: log-spaced times with a length limit (CK), and an
@@ -664,7 +703,7 @@ int main(int argc, char **argv) {
 		std::vector<Particle> ps = make_particles(256, 48, 1 * EeV, 300 * EeV, centre, species, wts);
 		PropagationCK ck(fgrid, 1e-4, 1 * kpc, 1 * Mpc);
 		MaximumTrajectoryLength ml(12 * Mpc);
-		run_observer_case("ck_grid32_observer", ck, &ml, new ObserverTimeEvolution(0.5 * Mpc, 10 * Mpc, 12, true), false,
+		run_observer_case("ck_grid32_observer", ck, {&ml}, {}, new ObserverTimeEvolution(0.5 * Mpc, 10 * Mpc, 12, true), false,
 		                  ps, 100000,
 		                  "{\"propagator\": \"CK\", \"tolerance\": 1e-4, \"minStep\": " + fmt(kpc) +
 		                      ", \"maxStep\": " + fmt(Mpc) + ", \"maxTrajectoryLength\": " + fmt(12 * Mpc) +
@@ -675,10 +714,46 @@ int main(int argc, char **argv) {
 		std::vector<Particle> ps = make_particles(128, 49, 1 * EeV, 100 * EeV, centre, proton, one);
 		PropagationBP bp(fgrid, 0.1 * Mpc);
 		std::vector<double> tl = {0.35 * Mpc, 1.0 * Mpc, 1.05 * Mpc, 2.5 * Mpc, 7.77 * Mpc};
-		run_observer_case("bp_fixed_grid32_observer", bp, nullptr, new ObserverTimeEvolution(tl), false, ps, 100,
+		run_observer_case("bp_fixed_grid32_observer", bp, {}, {}, new ObserverTimeEvolution(tl), false, ps, 100,
 		                  "{\"propagator\": \"BP\", \"fixedStep\": " + fmt(0.1 * Mpc) +
 		                      ", \"field\": \"grid32\", \"observer\": {\"list\": true, \"deactivate\": false}, "
 		                      "\"particle_seed\": 49}");
+	}
+
+	// ---- module order around the observer: observer before the boundary and the length limit
+	// (the detection sees nextStep before their limits), and a boundary before the observer
+	// (detections after the rejection see an inactive, flagged candidate)
+	{
+		std::vector<Particle> ps = make_particles(128, 56, 1 * EeV, 300 * EeV, centre, species, wts);
+		PropagationCK ck(fgrid, 1e-4, 1 * kpc, 1 * Mpc);
+		Vector3d sc = centre + Vector3d(0.05 * Mpc, 0., -0.1 * Mpc);
+		SphericalBoundary sph(sc, 3 * Mpc);
+		sph.setMargin(0.02 * Mpc);
+		MaximumTrajectoryLength ml(6 * Mpc);
+		std::vector<double> tl = {0.2 * Mpc, 0.9 * Mpc, 1.7 * Mpc, 2.6 * Mpc, 3.0 * Mpc, 4.4 * Mpc, 5.9 * Mpc, 6.0 * Mpc};
+		run_observer_case("ck_grid32_observer_first", ck, {}, {&sph, &ml}, new ObserverTimeEvolution(tl), false, ps,
+		                  100000,
+		                  "{\"propagator\": \"CK\", \"tolerance\": 1e-4, \"minStep\": " + fmt(kpc) +
+		                      ", \"maxStep\": " + fmt(Mpc) + ", \"maxTrajectoryLength\": " + fmt(6 * Mpc) +
+		                      ", \"field\": \"grid32\", \"order\": [\"observer\", \"sphere\", \"maxlen\"], "
+		                      "\"sphere\": {\"center\": [" + fmt(sc.x) + ", " + fmt(sc.y) + ", " + fmt(sc.z) +
+		                      "], \"radius\": " + fmt(3 * Mpc) + ", \"margin\": " + fmt(0.02 * Mpc) +
+		                      ", \"limitStep\": true}, \"observer\": {\"list\": true, \"deactivate\": false}, "
+		                      "\"particle_seed\": 56}");
+	}
+	{
+		std::vector<Particle> ps = make_particles(128, 57, 1 * EeV, 100 * EeV, centre, proton, one);
+		PropagationBP bp(fgrid, 0.1 * Mpc);
+		SphericalBoundary sph(centre, 0.75 * Mpc);
+		std::vector<double> tl = {0.3 * Mpc, 0.7 * Mpc, 0.8 * Mpc, 0.9 * Mpc, 1.0 * Mpc};
+		run_observer_case("bp_fixed_grid32_sphere_observer", bp, {&sph}, {}, new ObserverTimeEvolution(tl), false, ps,
+		                  100,
+		                  "{\"propagator\": \"BP\", \"fixedStep\": " + fmt(0.1 * Mpc) +
+		                      ", \"field\": \"grid32\", \"order\": [\"sphere\", \"observer\"], "
+		                      "\"sphere\": {\"center\": [" + fmt(centre.x) + ", " + fmt(centre.y) + ", " +
+		                      fmt(centre.z) + "], \"radius\": " + fmt(0.75 * Mpc) + ", \"margin\": " + fmt(0.1 * kpc) +
+		                      ", \"limitStep\": true}, \"observer\": {\"list\": true, \"deactivate\": false}, "
+		                      "\"particle_seed\": 57}");
 	}
 
 	// ---- C1-shaped: 64^3 grid (50 kpc), 4096 protons, BP fixed 0.1 Mpc x 2000

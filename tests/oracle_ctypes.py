@@ -16,6 +16,7 @@ LIBM_GLIBC = 0
 LIBM_CR = 1
 FIELD_NONE, FIELD_UNIFORM, FIELD_GRID, FIELD_PLANEWAVE = 0, 1, 2, 3
 PROP_BP, PROP_CK = 0, 1
+MOD_BOX, MOD_SPHERE, MOD_MAXLEN, MOD_OBSERVER = 1, 2, 3, 4  # orc_propagator.order
 
 # SI units (include/crpropa/Units.h)
 c_light = 2.99792458e8
@@ -54,7 +55,10 @@ class Propagator(C.Structure):
     _fields_ = [("type", C.c_int), ("tolerance", C.c_double), ("minStep", C.c_double),
                 ("maxStep", C.c_double), ("maxTrajectoryLength", C.c_double),
                 ("obsTimes", _dp), ("nObs", C.c_int), ("obsDeactivate", C.c_int),
-                ("box", C.c_int), ("boxOrigin", C.c_double * 3), ("boxSize", C.c_double * 3)]
+                ("box", C.c_int), ("boxOrigin", C.c_double * 3), ("boxSize", C.c_double * 3),
+                ("sphere", C.c_int), ("sphereLimitStep", C.c_int), ("sphereCenter", C.c_double * 3),
+                ("sphereRadius", C.c_double), ("sphereMargin", C.c_double), ("order", C.c_int * 4),
+                ("nOrder", C.c_int)]
 
 
 class State(C.Structure):
@@ -237,7 +241,8 @@ class OracleState:
 
 
 def propagate(field, prop_type, state, steps, tolerance=0.42, min_step=0.0, max_step=0.0, max_len=0.0,
-              libm=LIBM_GLIBC, nthreads=0, observer_times=None, observer_deactivate=False, box=None):
+              libm=LIBM_GLIBC, nthreads=0, observer_times=None, observer_deactivate=False, box=None,
+              sphere=None, order=None):
     """observer_times: detection lengths of an Observer(ObserverTimeEvolution) after the length limit;
     detections land in state.snap[k, i, :] = (pos, dir, length, time, nextStep)."""
     p = Propagator(prop_type, tolerance, min_step, max_step, max_len)
@@ -246,6 +251,16 @@ def propagate(field, prop_type, state, steps, tolerance=0.42, min_step=0.0, max_
         for a in range(3):
             p.boxOrigin[a] = box[0][a]
             p.boxSize[a] = box[1][a]
+    if order is not None:  # module names after the propagator in ModuleList order
+        ids = {"box": MOD_BOX, "sphere": MOD_SPHERE, "maxlen": MOD_MAXLEN, "observer": MOD_OBSERVER}
+        for a, nm in enumerate(order):
+            p.order[a] = ids[nm]
+        p.nOrder = len(order)
+    if sphere is not None:  # (center[3], radius, margin, limit_step) of a SphericalBoundary after the box
+        p.sphere = 1
+        for a in range(3):
+            p.sphereCenter[a] = sphere[0][a]
+        p.sphereRadius, p.sphereMargin, p.sphereLimitStep = sphere[1], sphere[2], int(sphere[3])
     keep = None
     if observer_times is not None and len(observer_times):
         keep = np.ascontiguousarray(observer_times, dtype=np.float64)
@@ -340,3 +355,17 @@ def propagator_args(meta):
         return dict(prop_type=PROP_BP, tolerance=meta["tolerance"], min_step=meta["minStep"],
                     max_step=meta["maxStep"])
     return dict(prop_type=PROP_CK, tolerance=meta["tolerance"], min_step=meta["minStep"], max_step=meta["maxStep"])
+
+
+def module_args(meta):
+    """propagate() keyword arguments for the modules a golden case lists after its propagator."""
+    kw = {}
+    if "box" in meta:
+        kw["box"] = (meta["box"]["origin"], meta["box"]["size"])
+    if "sphere" in meta:
+        sp = meta["sphere"]
+        kw["sphere"] = (sp["center"], sp["radius"], sp["margin"], sp["limitStep"])
+    if "order" in meta:
+        kw["order"] = list(meta["order"]) + (["maxlen"] if "maxTrajectoryLength" in meta and "maxlen" not in meta["order"]
+                                             else [])
+    return kw

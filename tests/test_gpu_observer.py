@@ -30,6 +30,11 @@ def run_gpu(c, deactivate, steps, launch=None):
     p = capi.Propagator(kind, grid_field(m), a["tolerance"], a["min_step"], a["max_step"],
                         max_length=m.get("maxTrajectoryLength", 0.0))
     p.set_observer(c["times"], deactivate=deactivate)
+    if "sphere" in m:
+        sp = m["sphere"]
+        p.set_spherical_boundary(sp["center"], sp["radius"], sp["margin"], sp["limitStep"])
+    if "order" in m:
+        p.set_module_order(O.module_args(m)["order"])
     if launch:
         p.set_launch(**launch)
     b = capi.Batch(len(c["energy"]))
@@ -42,7 +47,8 @@ def run_oracle(c, deactivate, steps, libm=O.LIBM_CR):
     m = c["meta"]
     st = O.OracleState(c["init_pos"], c["init_dir"], c["energy"], c["charge"])
     O.propagate(O.field_for_case(c), steps=steps, max_len=m.get("maxTrajectoryLength", 0.0), libm=libm, state=st,
-                observer_times=c["times"], observer_deactivate=deactivate, **O.propagator_args(m))
+                observer_times=c["times"], observer_deactivate=deactivate, **O.module_args(m),
+                **O.propagator_args(m))
     return st
 
 
@@ -51,7 +57,8 @@ def same_detections(det, snap):
     return np.array_equal(ok, ~np.isnan(det)) and bits_equal(det[ok], snap[ok])
 
 
-@pytest.mark.parametrize("name", ["ck_grid32_observer", "bp_fixed_grid32_observer"])
+@pytest.mark.parametrize("name", ["ck_grid32_observer", "bp_fixed_grid32_observer", "ck_grid32_observer_first",
+                                  "bp_fixed_grid32_sphere_observer"])
 @pytest.mark.parametrize("deactivate", [False, True])
 def test_observer_matches_oracle(name, deactivate):
     c = O.load_case(name)
@@ -67,15 +74,18 @@ def test_observer_matches_oracle(name, deactivate):
         assert np.all(g["detection_index"] <= 1) and not g["active"].any()
 
 
-def test_observer_fixed_step_bit_exact_vs_reference():
-    c = O.load_case("bp_fixed_grid32_observer")
+@pytest.mark.parametrize("name", ["bp_fixed_grid32_observer", "bp_fixed_grid32_sphere_observer"])
+def test_observer_fixed_step_bit_exact_vs_reference(name):
+    """Fixed-step BP: every detection's state, nextStep as limited by the modules listed before the
+    observer (a spherical boundary in the second case), equals the reference's."""
+    c = O.load_case(name)
     g, det = run_gpu(c, False, int(c["nsteps"].max()))
     assert np.array_equal(g["detection_index"], c["det_count"])
-    for key, col in (("det_pos", slice(0, 3)), ("det_dir", slice(3, 6))):
-        assert bits_equal(det[:, :, col], c[key]), key
-    for key, col in (("det_len", 6), ("det_time", 7), ("det_next", 8)):
-        assert bits_equal(det[:, :, col], c[key]), key
+    for key, col in (("det_pos", slice(0, 3)), ("det_dir", slice(3, 6)), ("det_len", 6), ("det_time", 7),
+                     ("det_next", 8)):
+        assert same_detections(det[:, :, col], c[key]), key
     assert bits_equal(g["pos"], c["final_pos"]) and bits_equal(g["next_step"], c["final_next"])
+    assert np.array_equal(g["active"], c["final_active"])
 
 
 def test_observer_adaptive_lands_on_the_times():

@@ -252,17 +252,20 @@ def test_noise_grid_matches_fixture():
     assert np.array_equal(O.noise_grid(m["seed"], m["n"], m["amp"]).view(np.uint32), g["grid"].view(np.uint32))
 
 
-@pytest.mark.parametrize("name", ["ck_grid32_observer", "bp_fixed_grid32_observer"])
+@pytest.mark.parametrize("name", ["ck_grid32_observer", "bp_fixed_grid32_observer", "ck_grid32_observer_first",
+                                  "bp_fixed_grid32_sphere_observer"])
 def test_oracle_observer_bit_exact_vs_reference(name):
-    """Observer(ObserverTimeEvolution) after the propagator (and length limit): every detection's state
-    and the final state equal the reference's (src/module/Observer.cpp:26-57,279-306)."""
+    """Observer(ObserverTimeEvolution) after the propagator, before or after a length limit and a
+    spherical boundary: every detection's state (nextStep as limited by the modules before the
+    observer only) and the final state equal the reference's (src/module/Observer.cpp:26-57,279-306)."""
     c = O.load_case(name)
     m = c["meta"]
     f = O.field_for_case(c)
     st = O.OracleState(c["init_pos"], c["init_dir"], c["energy"], c["charge"])
     pa = O.propagator_args(m)
     O.propagate(f, steps=int(c["nsteps"].max()), max_len=m.get("maxTrajectoryLength", 0.0), libm=O.LIBM_GLIBC,
-                state=st, observer_times=c["times"], observer_deactivate=m["observer"]["deactivate"], **pa)
+                state=st, observer_times=c["times"], observer_deactivate=m["observer"]["deactivate"],
+                **O.module_args(m), **pa)
     assert np.array_equal(st.det_index, c["det_count"])
     snap = st.snap
     for key, col in (("det_pos", slice(0, 3)), ("det_dir", slice(3, 6))):
@@ -298,3 +301,45 @@ def test_oracle_periodic_box_bit_exact_vs_reference(name):
                          ("next_step", "next_step")):
             assert bits_equal(getattr(st, key), c[ref][si]), (name, s, key)
         assert np.array_equal(st.active, c["active"][si])
+
+
+@pytest.mark.parametrize("name", ["bp_fixed_grid32_sphere", "ck_grid32_sphere"])
+def test_oracle_spherical_boundary_bit_exact_vs_reference(name):
+    """SphericalBoundary after the propagator (src/module/Boundary.cpp:216-223): rejection at
+    |pos - center| >= radius and limitNextStep(radius - d + margin) equal the reference's at every
+    snapshot, including the step sizes the limit imposes on the adaptive CK run."""
+    c = O.load_case(name)
+    m = c["meta"]
+    sp = m["sphere"]
+    st = O.OracleState(c["init_pos"], c["init_dir"], c["energy"], c["charge"])
+    done = 0
+    for si, s in enumerate(c["snap_steps"]):
+        O.propagate(O.field_for_case(c), steps=int(s) - done, max_len=m.get("maxTrajectoryLength", 0.0),
+                    libm=O.LIBM_GLIBC, state=st, sphere=(sp["center"], sp["radius"], sp["margin"], sp["limitStep"]),
+                    **O.propagator_args(m))
+        done = int(s)
+        for key, ref in (("pos", "pos"), ("prev_pos", "previous_pos"), ("dir", "dir"),
+                         ("trajectory_length", "trajectory_length"), ("next_step", "next_step")):
+            assert bits_equal(getattr(st, key), c[ref][si]), (name, s, key)
+        assert np.array_equal(st.active, c["active"][si])
+    # every particle ended outside the sphere by less than its last step
+    d = np.linalg.norm(st.pos - np.array(sp["center"]), axis=1)
+    assert not st.active.any() and np.all(d >= sp["radius"])
+    assert np.all(d - sp["radius"] <= st.current_step)
+
+
+@pytest.mark.parametrize("name,wrong", [("ck_grid32_observer_first", ["sphere", "maxlen", "observer"]),
+                                        ("bp_fixed_grid32_sphere_observer", ["observer", "sphere"])])
+def test_oracle_module_order_is_pinned(name, wrong):
+    """The order fixtures tell module orders apart: run in the wrong order, the oracle's snapshot
+    nextStep values no longer match the reference's."""
+    c = O.load_case(name)
+    m = c["meta"]
+    kw = O.module_args(m)
+    kw["order"] = wrong
+    st = O.OracleState(c["init_pos"], c["init_dir"], c["energy"], c["charge"])
+    O.propagate(O.field_for_case(c), steps=int(c["nsteps"].max()), max_len=m.get("maxTrajectoryLength", 0.0),
+                libm=O.LIBM_GLIBC, state=st, observer_times=c["times"], **kw, **O.propagator_args(m))
+    ref = c["det_next"]
+    ok = ~np.isnan(ref)
+    assert not bits_equal(st.snap[:, :, 8][ok], ref[ok])

@@ -189,3 +189,43 @@ def test_periodic_box(origin, size, cur, cre, ecur, ecre):
     p, q = c.current.getPosition(), c.created.getPosition()
     assert [p.x, p.y, p.z] == pytest.approx(list(ecur), rel=4 * 2.2e-16)
     assert [q.x, q.y, q.z] == pytest.approx(list(ecre), rel=4 * 2.2e-16)
+
+
+def test_spherical_boundary():
+    """test/testBreakCondition.cpp:531-559 (SphericalBoundary inside / outside / limitStep), host process()."""
+    from crpropa3_amd import Candidate, SphericalBoundary, Vector3d
+    sphere = SphericalBoundary(Vector3d(0, 0, 0), 10)
+    c = Candidate()
+    c.current.setPosition(Vector3d(9, 0, 0))
+    sphere.process(c)
+    assert c.isActive() and not c.hasProperty("Rejected")
+
+    sphere = SphericalBoundary(Vector3d(0, 0, 0), 10)
+    sphere.setRejectFlag("I passed the galactic border", "Nothing happened")
+    c = Candidate()
+    c.current.setPosition(Vector3d(0, -10.1, 0))
+    sphere.process(c)
+    assert not c.isActive() and c.hasProperty("I passed the galactic border")
+
+    sphere = SphericalBoundary(Vector3d(0, 0, 0), 10)
+    sphere.setLimitStep(True)
+    sphere.setMargin(1)
+    c = Candidate()
+    c.setNextStep(100)
+    c.current.setPosition(Vector3d(0, 0, 9.5))
+    sphere.process(c)
+    assert c.getNextStep() == pytest.approx(1.5, rel=4 * 2.2e-16)
+
+
+def test_reject_flag_value_is_the_references():
+    """Every condition's default reject flag value is typeid(*this).name() taken in the
+    AbstractCondition constructor (src/Module.cpp:20-23): the base's mangled name."""
+    from crpropa3_amd import Candidate, MaximumTrajectoryLength, SphericalBoundary, Vector3d
+    c = Candidate()
+    c.current.setPosition(Vector3d(3, 0, 0))
+    SphericalBoundary(Vector3d(0, 0, 0), 1).process(c)
+    assert c.getProperty("Rejected") == "N7crpropa17AbstractConditionE"
+    c = Candidate()
+    c.setTrajectoryLength(2.0)
+    MaximumTrajectoryLength(1.0).process(c)
+    assert c.getProperty("Rejected") == "N7crpropa17AbstractConditionE"

@@ -11,6 +11,9 @@ from crpropa3_amd import (Candidate, EeV, Grid3f, MagneticField, MagneticFieldGr
 pytestmark = pytest.mark.gpu
 
 DEQ = 4 * 2.220446049250313e-16  # gtest EXPECT_DOUBLE_EQ: within 4 ulp
+# the reference's default reject flag value: typeid(*this).name() in the AbstractCondition
+# constructor (src/Module.cpp:20-23), the same for every condition
+REJECTED = "N7crpropa17AbstractConditionE"
 
 
 def proton(E=100 * EeV, d=(0, 1, 0)):
@@ -145,7 +148,7 @@ def test_functional_gyroradius():
     rg = E / (O.eplus * B * O.c_light)
     p = c.current.getPosition()
     assert not c.isActive()
-    assert c.getProperty("Rejected") == "MaximumTrajectoryLength"
+    assert c.getProperty("Rejected") == REJECTED
     assert np.hypot(p.x, p.y + rg) == pytest.approx(rg, rel=1e-3)
 
 
@@ -290,7 +293,7 @@ def test_modulelist_observer_fused_equals_host_observer_and_oracle():
                         for c in cands])
         assert np.array_equal(pos.view(np.uint64), st.pos.view(np.uint64))
         assert all(c.getProperty("DetectionIndex") == "9" for c in cands)
-        assert all(c.getProperty("Rejected") == "MaximumTrajectoryLength" for c in cands)
+        assert all(c.getProperty("Rejected") == REJECTED for c in cands)
 
 
 def _box_run(extra_module, n=96, seed=53):
@@ -327,3 +330,69 @@ def test_modulelist_periodic_box_fused_equals_host_and_oracle():
     assert np.array_equal(fused.view(np.uint64), ref.view(np.uint64))
     lo, hi = np.array(box[0]), np.array(box[0]) + np.array(box[1])
     assert np.all((fused[:, :3] >= lo - 1e-9 * Mpc) & (fused[:, :3] <= hi + 1e-9 * Mpc))
+
+
+class _FullRecorder(Module):
+    """Detection action that records the whole view: index, position, length, nextStep, active,
+    reject flag."""
+
+    def __init__(self):
+        Module.__init__(self)
+        self.rows = []
+
+    def process(self, c):
+        p = c.current.getPosition()
+        self.rows.append((int(c.getProperty("DetectionIndex")), p.x, p.y, p.z, c.getTrajectoryLength(),
+                          c.getNextStep(), c.isActive(), c.getProperty("Rejected") if c.hasProperty("Rejected")
+                          else ""))
+
+
+@pytest.mark.parametrize("order", [("observer", "sphere", "maxlen"), ("sphere", "observer", "maxlen"),
+                                   ("maxlen", "sphere", "observer")])
+def test_modulelist_sphere_observer_order_fused_equals_host(order):
+    """[PropagationCK, then SphericalBoundary, MaximumTrajectoryLength and Observer in the given
+    order] runs fused; with an extra host module it runs step by step on the host.  Detection views
+    (nextStep, active, reject flag) and final states agree bit for bit, and with the oracle run in
+    the same module order."""
+    from crpropa3_amd import Observer, ObserverTimeEvolution, SphericalBoundary
+
+    def run(extra):
+        grid, gvals, gm, cands, E, Z, d = _grid_candidates(96, 58)
+        L = gm["n"] * gm["spacing"]
+        times = [0.3 * Mpc, 0.8 * Mpc, 1.5 * Mpc, 2.2 * Mpc, 2.9 * Mpc, 4.0 * Mpc]
+        obs = Observer()
+        obs.add(ObserverTimeEvolution(times))
+        obs.setDeactivateOnDetection(False)
+        rec = _FullRecorder()
+        obs.onDetection(rec)
+        sph = SphericalBoundary(Vector3d(L / 2 + 0.1 * Mpc, L / 2, L / 2), 2.5 * Mpc)
+        sph.setMargin(0.03 * Mpc)
+        mods = {"observer": obs, "sphere": sph, "maxlen": MaximumTrajectoryLength(3.5 * Mpc)}
+        sim = ModuleList()
+        sim.add(PropagationCK(MagneticFieldGrid(grid), 1e-4, 1 * kpc, 1 * Mpc))
+        for nm in order:
+            sim.add(mods[nm])
+        if extra is not None:
+            sim.add(extra)
+        sim.run(cands)
+        fin = [(c.current.getPosition().x, c.current.getPosition().y, c.current.getPosition().z, c.getNextStep(),
+                c.isActive(), c.getProperty("Rejected")) for c in cands]
+        return sorted(rec.rows), fin, gvals, gm, E, Z, d, times, L
+
+    class Noop(Module):
+        def process(self, c):
+            pass
+
+    fused, ffin, gvals, gm, E, Z, d, times, L = run(None)
+    step, sfin = run(Noop())[:2]
+    assert fused == step
+    assert ffin == sfin
+    st = O.OracleState(np.tile([L / 2] * 3, (len(E), 1)), d, E, Z * O.eplus)
+    O.propagate(O.OracleField.grid(gvals, gm["origin"], gm["spacing"]), O.PROP_CK, st, 10 ** 9, tolerance=1e-4,
+                min_step=1 * kpc, max_step=1 * Mpc, max_len=3.5 * Mpc, libm=O.LIBM_CR, observer_times=times,
+                sphere=((L / 2 + 0.1 * Mpc, L / 2, L / 2), 2.5 * Mpc, 0.03 * Mpc, True), order=list(order))
+    ref = sorted((k + 1, st.snap[k, i, 0], st.snap[k, i, 1], st.snap[k, i, 2], st.snap[k, i, 6], st.snap[k, i, 8])
+                 for k in range(len(times)) for i in range(len(E)) if k < st.det_index[i])
+    assert [r[:6] for r in fused] == ref
+    assert np.array_equal(np.array([f[:3] for f in ffin]).view(np.uint64), st.pos.view(np.uint64))
+    assert all(f[5] == REJECTED for f in ffin)
