@@ -96,6 +96,8 @@ def lib():
         L.crp_batch_download.argtypes = [_vp, C.POINTER(HostState)]
         L.crp_batch_copy.argtypes = [_vp, _vp]
         L.crp_propagate.argtypes = [_vp, _vp, C.c_longlong, C.POINTER(RunStats)]
+        L.crp_turbulence_mode_count.argtypes = [C.POINTER(TurbulenceDesc), C.POINTER(C.c_uint64)]
+        L.crp_turbulence_generate.argtypes = [C.POINTER(TurbulenceDesc), C.POINTER(C.c_uint32), C.c_uint64, _fp]
         L.crp_set_device.argtypes = [C.c_int]
         L.crp_get_device_count.argtypes = [C.POINTER(C.c_int)]
         _lib = L
@@ -196,6 +198,38 @@ class Field:
         else:
             check(lib().crp_field_eval_ex(self.handle, _p(pos), pos.shape[0], _p(B), precision))
         return B
+
+
+class TurbulenceDesc(C.Structure):
+    """crp_turbulence_desc"""
+    _fields_ = [("n", C.c_int), ("spacing", C.c_double), ("lmin", C.c_double), ("lmax", C.c_double),
+                ("kind", C.c_int), ("alpha", C.c_double), ("lbendover", C.c_double), ("sindex", C.c_double),
+                ("qindex", C.c_double)]
+
+
+TURBULENCE_SIMPLE, TURBULENCE_GENERAL, TURBULENCE_GENERAL_POWERLAW = 0, 1, 2
+
+
+def turbulence_desc(n, spacing, lmin, lmax, kind=TURBULENCE_SIMPLE, alpha=-11 / 3., lbendover=1.0, sindex=5 / 3.,
+                    qindex=4.0):
+    return TurbulenceDesc(n, spacing, lmin, lmax, kind, alpha, lbendover, sindex, qindex)
+
+
+def turbulence_mode_count(desc):
+    """Modes inside the turbulence range (the generator takes 4 draws per mode for the simple
+    spectrum, 2 otherwise)."""
+    m = C.c_uint64()
+    check(lib().crp_turbulence_mode_count(C.byref(desc), C.byref(m)))
+    return m.value
+
+
+def turbulence_generate(desc, draws):
+    """Unnormalised turbulent grid, (n, n, n, 3) float32, from MT19937 randInt() draws."""
+    d = np.ascontiguousarray(draws, dtype=np.uint32)
+    out = np.zeros((desc.n, desc.n, desc.n, 3), dtype=np.float32)
+    check(lib().crp_turbulence_generate(C.byref(desc), d.ctypes.data_as(C.POINTER(C.c_uint32)), d.size,
+                                         out.ctypes.data_as(_fp)))
+    return out
 
 
 MODULES = {"box": 1, "sphere": 2, "maxlen": 3, "observer": 4}  # enum crp_module

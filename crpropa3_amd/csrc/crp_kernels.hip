@@ -26,7 +26,11 @@
 #include "crp_device.hpp"
 #include "crp_propagate.hpp"
 #include "crp_adjoint.hpp"
+#include "crp_turbulence.hpp"
 #include "crpropa3_amd.h"
+
+#include <dlfcn.h>
+#include <hipfft/hipfft.h>
 
 using namespace crp;
 
@@ -1488,6 +1492,182 @@ int crp_adjoint_backward(const crp_adjoint_config *cfg, size_t n, const double *
 	A.gradScale = grad_scale;
 	hipLaunchKernelGGL(k_adjoint_backward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, A);
 	CRP_HIP_TRY(hipGetLastError());
+	return CRP_OK;
+}
+
+// ------------------------------------------------------------------ turbulent grids
+// hipFFT is loaded when first needed, so the propagator does not depend on it.
+struct HipfftApi {
+	decltype(&hipfftPlan3d) plan3d = nullptr;
+	decltype(&hipfftExecC2R) execC2R = nullptr;
+	decltype(&hipfftDestroy) destroy = nullptr;
+	bool ok = false;
+};
+
+static const HipfftApi &hipfft_api() {
+	static HipfftApi api = [] {
+		HipfftApi a;
+		void *h = dlopen("libhipfft.so.0", RTLD_NOW | RTLD_GLOBAL);
+		if (!h)
+			h = dlopen("libhipfft.so", RTLD_NOW | RTLD_GLOBAL);
+		if (!h)
+			return a;
+		a.plan3d = reinterpret_cast<decltype(&hipfftPlan3d)>(dlsym(h, "hipfftPlan3d"));
+		a.execC2R = reinterpret_cast<decltype(&hipfftExecC2R)>(dlsym(h, "hipfftExecC2R"));
+		a.destroy = reinterpret_cast<decltype(&hipfftDestroy)>(dlsym(h, "hipfftDestroy"));
+		a.ok = a.plan3d && a.execC2R && a.destroy;
+		return a;
+	}();
+	return api;
+}
+
+// GridTurbulence::checkGridRequirements (GridTurbulence.cpp:126-143) for a cubic grid of equal spacing
+static int turb_desc(const char *fn, const crp_turbulence_desc *d, TurbDesc &D) {
+	if (!d)
+		return fail(CRP_E_INVALID, std::string(fn) + ": NULL descriptor");
+	if (d->n < 2 || d->n > 2048 || !(d->spacing > 0))
+		return fail(CRP_E_INVALID, std::string(fn) + ": need 2 <= n <= 2048 and spacing > 0");
+	if (d->lmin < 2 * d->spacing)
+		return fail(CRP_E_INVALID, "turbulentField: lMin < 2 * spacing");
+	if (d->lmax > d->n * d->spacing)
+		return fail(CRP_E_INVALID, "turbulentField: lMax > size");
+	if (d->lmax < d->lmin)
+		return fail(CRP_E_INVALID, "lMax < lMin");
+	if (d->kind != CRP_TURBULENCE_SIMPLE && d->kind != CRP_TURBULENCE_GENERAL &&
+	    d->kind != CRP_TURBULENCE_GENERAL_POWERLAW)
+		return fail(CRP_E_INVALID, std::string(fn) + ": unknown spectrum kind");
+	D.n = d->n;
+	D.n2 = d->n / 2 + 1;
+	D.kind = d->kind;
+	D.kMin = d->spacing / d->lmax;
+	D.kMax = d->spacing / d->lmin;
+	D.alpha = d->alpha;
+	D.lambda = 1 / d->spacing * 2 * M_PI;
+	D.lbo = d->lbendover;
+	D.sidx = d->sindex;
+	D.qidx = d->qindex;
+	return CRP_OK;
+}
+
+// per-row inside-mode counts; rowOffset gets their exclusive prefix sums, total the sum
+static int turb_rows(const TurbDesc &D, std::vector<uint64_t> &rowOffset, uint64_t &total) {
+	const int rows = D.n * D.n;
+	uint32_t *dCount = nullptr;
+	CRP_HIP_TRY(hipMalloc((void **)&dCount, rows * sizeof(uint32_t)));
+	hipLaunchKernelGGL(k_turb_row_count, dim3((rows + 255) / 256), dim3(256), 0, 0, D, dCount);
+	std::vector<uint32_t> count(rows);
+	hipError_t e = hipGetLastError();
+	if (e == hipSuccess)
+		e = hipMemcpy(count.data(), dCount, rows * sizeof(uint32_t), hipMemcpyDeviceToHost);
+	(void)hipFree(dCount);
+	if (e != hipSuccess)
+		return fail(CRP_E_HIP, std::string("turbulence row count: ") + hipGetErrorString(e));
+	rowOffset.resize(rows);
+	total = 0;
+	for (int r = 0; r < rows; r++) {
+		rowOffset[r] = total;
+		total += count[r];
+	}
+	return CRP_OK;
+}
+
+int crp_turbulence_mode_count(const crp_turbulence_desc *d, uint64_t *modes) {
+	TurbDesc D;
+	int rc = turb_desc("crp_turbulence_mode_count", d, D);
+	if (rc != CRP_OK)
+		return rc;
+	if (!modes)
+		return fail(CRP_E_INVALID, "crp_turbulence_mode_count: NULL output");
+	std::vector<uint64_t> off;
+	uint64_t total = 0;
+	rc = turb_rows(D, off, total);
+	if (rc == CRP_OK)
+		*modes = total;
+	return rc;
+}
+
+int crp_turbulence_generate(const crp_turbulence_desc *d, const uint32_t *draws, uint64_t ndraws, float *grid) {
+	TurbDesc D;
+	int rc = turb_desc("crp_turbulence_generate", d, D);
+	if (rc != CRP_OK)
+		return rc;
+	if (!grid || (ndraws > 0 && !draws))
+		return fail(CRP_E_INVALID, "crp_turbulence_generate: NULL argument");
+	std::vector<uint64_t> off;
+	uint64_t total = 0;
+	rc = turb_rows(D, off, total);
+	if (rc != CRP_OK)
+		return rc;
+	const uint64_t perMode = D.kind == TURB_SIMPLE ? 4 : 2;
+	if (ndraws != perMode * total)
+		return fail(CRP_E_INVALID, "crp_turbulence_generate: " + std::to_string(total) + " modes need " +
+		                               std::to_string(perMode * total) + " draws, got " + std::to_string(ndraws));
+	const HipfftApi &fft = hipfft_api();
+	if (!fft.ok)
+		return fail(CRP_E_STATE, "crp_turbulence_generate: hipFFT (libhipfft.so) could not be loaded");
+	const int rows = D.n * D.n;
+	const size_t plane = (size_t)rows * D.n2; // complex points per component
+	const size_t npts = (size_t)rows * D.n;
+	uint64_t *dOff = nullptr;
+	uint32_t *dDraws = nullptr;
+	float2 *B = nullptr;
+	float *dGrid = nullptr;
+	hipfftHandle plan = nullptr;
+	hipError_t e = hipSuccess;
+	auto cleanup = [&]() {
+		if (plan)
+			(void)fft.destroy(plan);
+		(void)hipFree(dOff);
+		(void)hipFree(dDraws);
+		(void)hipFree(B);
+		(void)hipFree(dGrid);
+	};
+#define TURB_TRY(call)                                                                                         \
+	do {                                                                                                       \
+		e = (call);                                                                                            \
+		if (e != hipSuccess) {                                                                                 \
+			cleanup();                                                                                         \
+			return fail(e == hipErrorOutOfMemory ? CRP_E_NOMEM : CRP_E_HIP,                                    \
+			            std::string("crp_turbulence_generate: ") + hipGetErrorString(e));                      \
+		}                                                                                                      \
+	} while (0)
+	TURB_TRY(hipMalloc((void **)&dOff, rows * sizeof(uint64_t)));
+	TURB_TRY(hipMemcpy(dOff, off.data(), rows * sizeof(uint64_t), hipMemcpyHostToDevice));
+	TURB_TRY(hipMalloc((void **)&dDraws, std::max<uint64_t>(ndraws, 1) * sizeof(uint32_t)));
+	if (ndraws)
+		TURB_TRY(hipMemcpy(dDraws, draws, ndraws * sizeof(uint32_t), hipMemcpyHostToDevice));
+	TURB_TRY(hipMalloc((void **)&B, 3 * plane * sizeof(float2)));
+	hipLaunchKernelGGL(k_turb_coeffs, dim3((rows + 255) / 256), dim3(256), 0, 0, D, dOff, dDraws, B);
+	TURB_TRY(hipGetLastError());
+	hipLaunchKernelGGL(k_turb_hermitize, dim3((rows + 255) / 256), dim3(256), 0, 0, B, D.n, D.n2, 0);
+	TURB_TRY(hipGetLastError());
+	if (D.n % 2 == 0) {
+		hipLaunchKernelGGL(k_turb_hermitize, dim3((rows + 255) / 256), dim3(256), 0, 0, B, D.n, D.n2, D.n / 2);
+		TURB_TRY(hipGetLastError());
+	}
+	TURB_TRY(hipFree(dDraws));
+	dDraws = nullptr;
+	// in place, complex to real, unnormalised, per component (GridTurbulence.cpp:155-171)
+	if (fft.plan3d(&plan, D.n, D.n, D.n, HIPFFT_C2R) != HIPFFT_SUCCESS) {
+		plan = nullptr;
+		cleanup();
+		return fail(CRP_E_HIP, "crp_turbulence_generate: hipfftPlan3d failed");
+	}
+	for (int c = 0; c < 3; c++) {
+		hipfftComplex *in = reinterpret_cast<hipfftComplex *>(B + c * plane);
+		if (fft.execC2R(plan, in, reinterpret_cast<hipfftReal *>(in)) != HIPFFT_SUCCESS) {
+			cleanup();
+			return fail(CRP_E_HIP, "crp_turbulence_generate: hipfftExecC2R failed");
+		}
+	}
+	TURB_TRY(hipDeviceSynchronize());
+	TURB_TRY(hipMalloc((void **)&dGrid, 3 * npts * sizeof(float)));
+	hipLaunchKernelGGL(k_turb_pack, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, 0,
+	                   reinterpret_cast<const float *>(B), D.n, D.n2, dGrid);
+	TURB_TRY(hipGetLastError());
+	TURB_TRY(hipMemcpy(grid, dGrid, 3 * npts * sizeof(float), hipMemcpyDefault));
+#undef TURB_TRY
+	cleanup();
 	return CRP_OK;
 }
 

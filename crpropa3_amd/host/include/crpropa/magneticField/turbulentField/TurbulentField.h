@@ -33,6 +33,17 @@ public:
 		double kHat = k * lBendover;
 		return std::pow(kHat, qIndex) / std::pow(1.0 + kHat * kHat, (sIndex + qIndex) / 2.0 + 1.0);
 	}
+	/* l_c = 4 pi / ((s + 2) s) N lBendover, N = G((s+q)/2) / (2 G((s-1)/2) G((q+1)/2))
+	 * (TurbulentField.h:33-39, 82-91) */
+	virtual double getCorrelationLength() const {
+		return 4 * M_PI / ((sIndex + 2.0) * sIndex) * spectrumNormalization() * lBendover;
+	}
+
+protected:
+	double spectrumNormalization() const {
+		return std::tgamma((sIndex + qIndex) / 2.0) /
+		       (2.0 * std::tgamma((sIndex - 1) / 2.0) * std::tgamma((qIndex + 1) / 2.0));
+	}
 
 private:
 	const double Brms, sIndex, qIndex, lBendover, lMin, lMax;
@@ -44,17 +55,25 @@ public:
 	SimpleTurbulenceSpectrum(double Brms, double lMin, double lMax, double sIndex = 5. / 3)
 	    : TurbulenceSpectrum(Brms, lMin, lMax, 1000 * lMax, sIndex, 0) {}
 	double energySpectrum(double k) const { return std::pow(k, -getSindex() - 2); }
+	/* Harari et al. JHEP03(2002)045 (SimpleGridTurbulence.h:39-51) */
+	double getCorrelationLength() const { return turbulentCorrelationLength(getLmin(), getLmax(), getSindex()); }
+	static double turbulentCorrelationLength(double lMin, double lMax, double s) {
+		double r = lMin / lMax;
+		return lMax / 2 * (s - 1) / s * (1 - std::pow(r, s)) / (1 - std::pow(r, s - 1));
+	}
 };
 
 class TurbulentField : public MagneticField {
 public:
-	/* Keeps its own copy of the spectrum parameters (the reference holds a reference). */
+	/* Keeps its own copy of the spectrum (the reference holds a reference to the caller's). */
 	TurbulentField(const TurbulenceSpectrum &spectrum)
-	    : brms(spectrum.getBrms()), lmin(spectrum.getLmin()), lmax(spectrum.getLmax()) {}
+	    : brms(spectrum.getBrms()), lmin(spectrum.getLmin()), lmax(spectrum.getLmax()),
+	      lc(spectrum.getCorrelationLength()) {}
 	double getBrms() const { return brms; }
+	virtual double getCorrelationLength() const { return lc; }
 
 protected:
-	double brms, lmin, lmax;
+	double brms, lmin, lmax, lc;
 };
 
 } // namespace crpropa

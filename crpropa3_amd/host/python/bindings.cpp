@@ -19,6 +19,7 @@
 #include "crpropa/Vector3.h"
 #include "crpropa/magneticField/MagneticField.h"
 #include "crpropa/magneticField/MagneticFieldGrid.h"
+#include "crpropa/magneticField/turbulentField/GridTurbulence.h"
 #include "crpropa/magneticField/turbulentField/PlaneWaveTurbulence.h"
 #include "crpropa/module/Boundary.h"
 #include "crpropa/module/BreakCondition.h"
@@ -291,15 +292,40 @@ PYBIND11_MODULE(_crpropa, m) {
 	    .def("getLbendover", &TurbulenceSpectrum::getLbendover)
 	    .def("getSindex", &TurbulenceSpectrum::getSindex)
 	    .def("getQindex", &TurbulenceSpectrum::getQindex)
-	    .def("energySpectrum", &TurbulenceSpectrum::energySpectrum);
+	    .def("energySpectrum", &TurbulenceSpectrum::energySpectrum)
+	    .def("getCorrelationLength", &TurbulenceSpectrum::getCorrelationLength);
 
 	py::class_<SimpleTurbulenceSpectrum, TurbulenceSpectrum, ref_ptr<SimpleTurbulenceSpectrum>>(
 	    m, "SimpleTurbulenceSpectrum")
 	    .def(py::init<double, double, double, double>(), py::arg("Brms"), py::arg("lMin"), py::arg("lMax"),
-	         py::arg("sIndex") = 5. / 3.);
+	         py::arg("sIndex") = 5. / 3.)
+	    .def_static("turbulentCorrelationLength", &SimpleTurbulenceSpectrum::turbulentCorrelationLength);
 
 	py::class_<TurbulentField, MagneticField, ref_ptr<TurbulentField>>(m, "TurbulentField")
-	    .def("getBrms", &TurbulentField::getBrms);
+	    .def("getBrms", &TurbulentField::getBrms)
+	    .def("getCorrelationLength", &TurbulentField::getCorrelationLength);
+
+	py::class_<GridTurbulence, TurbulentField, ref_ptr<GridTurbulence>>(m, "GridTurbulence")
+	    .def(py::init<const TurbulenceSpectrum &, const GridProperties &, unsigned int>(), py::arg("spectrum"),
+	         py::arg("gridProp"), py::arg("seed") = 0, py::call_guard<py::gil_scoped_release>())
+	    .def("getGrid", &GridTurbulence::getGrid)
+	    .def("getMeanFieldVector", &GridTurbulence::getMeanFieldVector)
+	    .def("getMeanFieldStrength", &GridTurbulence::getMeanFieldStrength)
+	    .def("getRmsFieldStrength", &GridTurbulence::getRmsFieldStrength)
+	    .def("getRmsFieldStrengthPerAxis", &GridTurbulence::getRmsFieldStrengthPerAxis)
+	    .def("dumpToFile", &GridTurbulence::dumpToFile)
+	    .def_static("checkGridRequirements", &GridTurbulence::checkGridRequirements);
+
+	py::class_<SimpleGridTurbulence, GridTurbulence, ref_ptr<SimpleGridTurbulence>>(m, "SimpleGridTurbulence")
+	    .def(py::init<const SimpleTurbulenceSpectrum &, const GridProperties &, unsigned int>(), py::arg("spectrum"),
+	         py::arg("gridProp"), py::arg("seed") = 0, py::call_guard<py::gil_scoped_release>())
+	    .def_static("initTurbulence", &SimpleGridTurbulence::initTurbulence, py::arg("grid"), py::arg("Brms"),
+	                py::arg("lMin"), py::arg("lMax"), py::arg("alpha"), py::arg("seed"),
+	                py::call_guard<py::gil_scoped_release>());
+	m.def("initTurbulence", &initTurbulence, py::arg("grid"), py::arg("Brms"), py::arg("lMin"), py::arg("lMax"),
+	      py::arg("alpha") = -11 / 3., py::arg("seed") = 0, py::call_guard<py::gil_scoped_release>());
+	m.def("turbulentCorrelationLength", &turbulentCorrelationLength, py::arg("lMin"), py::arg("lMax"),
+	      py::arg("alpha") = -11 / 3.);
 
 	py::class_<PlaneWaveTurbulence, TurbulentField, ref_ptr<PlaneWaveTurbulence>>(m, "PlaneWaveTurbulence")
 	    .def(py::init<const TurbulenceSpectrum &, int, int>(), py::arg("spectrum"), py::arg("Nm") = 64,

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define CRP_ABI_VERSION 4
+#define CRP_ABI_VERSION 5
 
 enum crp_status {
 	CRP_OK = 0,
@@ -248,6 +248,34 @@ int crp_adjoint_forward(const crp_adjoint_config *cfg, size_t n, const double *x
 int crp_adjoint_backward(const crp_adjoint_config *cfg, size_t n, const double *kq, const double *checkpoints,
                          const double *grad_x_out, const double *grad_u_out, double *scratch, double *grad_grid,
                          double *grad_x0, double *grad_u0, double *grad_scale, void *stream);
+
+/* ---------------------------------------------------------------- turbulent grids
+ * SimpleGridTurbulence::initTurbulence (src/magneticField/turbulentField/SimpleGridTurbulence.cpp:18-113)
+ * and GridTurbulence::initTurbulence (src/magneticField/turbulentField/GridTurbulence.cpp:29-124,
+ * 147-185) on the device, up to (not including) the final scaleGrid to Brms: random k-space
+ * coefficients for every mode with spacing/lMax <= |k| <= spacing/lMin, one in-place
+ * complex-to-real FFT per component (hipFFT), Grid3f layout.  The random numbers are the caller's
+ * MT19937 Random::randInt() outputs, in the reference's order: 4 per mode for
+ * CRP_TURBULENCE_SIMPLE (orientation, randNorm, phase), 2 for the general spectra. */
+enum crp_turbulence_kind {
+	CRP_TURBULENCE_SIMPLE = 0,          /* amplitude randNorm() * k^(alpha/2) */
+	CRP_TURBULENCE_GENERAL = 1,         /* amplitude sqrt(E(k 2pi/spacing)), TurbulenceSpectrum */
+	CRP_TURBULENCE_GENERAL_POWERLAW = 2 /* the same with SimpleTurbulenceSpectrum's E(k) = k^(-s-2) */
+};
+typedef struct {
+	int n;                            /* grid points per axis (cubic grid), 2..2048 */
+	double spacing;                   /* equal on all axes */
+	double lmin, lmax;                /* 2 spacing <= lmin <= lmax <= n spacing */
+	int kind;                         /* crp_turbulence_kind */
+	double alpha;                     /* SIMPLE: <B^2(k)> ~ k^alpha */
+	double lbendover, sindex, qindex; /* GENERAL: TurbulenceSpectrum parameters; POWERLAW: sindex */
+} crp_turbulence_desc;
+
+/* Number of modes inside the turbulence range (draws needed: 4 or 2 per mode). */
+int crp_turbulence_mode_count(const crp_turbulence_desc *d, uint64_t *modes);
+/* The unnormalised grid: grid[3 * ((ix * n + iy) * n + iz) + c], n^3 x 3 floats, host or device
+ * memory.  ndraws must equal draws-per-mode x crp_turbulence_mode_count. */
+int crp_turbulence_generate(const crp_turbulence_desc *d, const uint32_t *draws, uint64_t ndraws, float *grid);
 
 #ifdef __cplusplus
 }

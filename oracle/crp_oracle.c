@@ -803,3 +803,96 @@ void orc_adjoint_backward(const float *grid, int nx, int ny, int nz, const doubl
 	*grad_scale += gs;
 	free(tape);
 }
+
+/* ------------------------------------------------------------------ turbulent grids */
+/* Vector3<float>::getAngleTo / isParallelTo (include/crpropa/Vector3.h:172-187), all in float */
+static int orc_parallel_to_111(const float e[3], float maxAngle) {
+	const float n0[3] = {1.f, 1.f, 1.f};
+	const float dot = e[0] * n0[0] + e[1] * n0[1] + e[2] * n0[2];
+	const float rv = sqrtf(n0[0] * n0[0] + n0[1] * n0[1] + n0[2] * n0[2]);
+	const float re = sqrtf(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+	const float cosd = dot / rv / re;
+	float ang;
+	if (cosd >= 1.)
+		ang = 0;
+	else if (cosd <= -1.)
+		ang = (float)M_PI;
+	else
+		ang = acosf(cosd);
+	return ang < maxAngle;
+}
+
+/* SimpleGridTurbulence::initTurbulence (src/magneticField/turbulentField/SimpleGridTurbulence.cpp:18-113)
+ * and GridTurbulence::initTurbulence (GridTurbulence.cpp:29-124) up to the inverse FFT */
+size_t orc_turbulence_coeffs(int n, double spacing, double lmin, double lmax, int kind, double alpha, double lbendover,
+                             double sindex, double qindex, uint32_t seed, float *bk) {
+	const size_t nn = (size_t)n, n2 = nn / 2 + 1, plane = nn * nn * n2;
+	orc_rng rng;
+	orc_rng_seed(&rng, seed);
+	double *K = (double *)malloc(nn * sizeof(double));
+	for (size_t i = 0; i < nn; i++)
+		K[i] = (double)i / nn - (double)(i / (nn / 2));
+	const double kMin = spacing / lmax, kMax = spacing / lmin;
+	const double lambda = 1 / spacing * 2 * M_PI;
+	size_t inside = 0;
+	for (size_t ix = 0; ix < nn; ix++)
+		for (size_t iy = 0; iy < nn; iy++)
+			for (size_t iz = 0; iz < n2; iz++) {
+				const size_t i = ix * nn * n2 + iy * n2 + iz;
+				float ek[3] = {(float)K[ix], (float)K[iy], (float)K[iz]};
+				const double k = sqrtf(ek[0] * ek[0] + ek[1] * ek[1] + ek[2] * ek[2]);
+				float *o[3] = {bk + 2 * i, bk + 2 * (plane + i), bk + 2 * (2 * plane + i)};
+				if ((k < kMin) || (k > kMax)) {
+					for (int c = 0; c < 3; c++)
+						o[c][0] = o[c][1] = 0;
+					continue;
+				}
+				inside++;
+				float e1[3], e2[3];
+				if (orc_parallel_to_111(ek, 1e-3f)) {
+					e1[0] = -1.f; e1[1] = 1.f; e1[2] = 0.f;
+					e2[0] = 1.f; e2[1] = 1.f; e2[2] = -2.f;
+				} else {
+					/* e1 = n0 x ek, e2 = ek x e1 (Vector3::cross, Vector3.h:268-271) */
+					const float n0[3] = {1.f, 1.f, 1.f};
+					e1[0] = n0[1] * ek[2] - ek[1] * n0[2];
+					e1[1] = n0[2] * ek[0] - ek[2] * n0[0];
+					e1[2] = n0[0] * ek[1] - ek[0] * n0[1];
+					e2[0] = ek[1] * e1[2] - e1[1] * ek[2];
+					e2[1] = ek[2] * e1[0] - e1[2] * ek[0];
+					e2[2] = ek[0] * e1[1] - e1[0] * ek[1];
+				}
+				const float r1 = sqrtf(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+				const float r2 = sqrtf(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+				for (int a = 0; a < 3; a++) {
+					e1[a] /= r1;
+					e2[a] /= r2;
+				}
+				const double theta = 2 * M_PI * orc_rng_rand(&rng);
+				const float ct = (float)cos(theta), st = (float)sin(theta);
+				float b[3];
+				for (int a = 0; a < 3; a++)
+					b[a] = e1[a] * ct + e2[a] * st;
+				float amp;
+				if (kind == ORC_TURB_SIMPLE) {
+					amp = (float)(orc_rng_norm(&rng, 0.0, 1.0) * pow(k, alpha / 2));
+				} else if (kind == ORC_TURB_GENERAL_POWERLAW) {
+					/* SimpleTurbulenceSpectrum::energySpectrum (SimpleGridTurbulence.h:35-37) */
+					amp = (float)sqrt(pow(k * lambda, -sindex - 2));
+				} else {
+					/* TurbulenceSpectrum::energySpectrum (TurbulentField.h:75-80) */
+					const double kHat = k * lambda * lbendover;
+					amp = (float)sqrt(pow(kHat, qindex) / pow(1.0 + kHat * kHat, (sindex + qindex) / 2.0 + 1.0));
+				}
+				for (int a = 0; a < 3; a++)
+					b[a] *= amp;
+				const double phase = 2 * M_PI * orc_rng_rand(&rng);
+				const double cp = cos(phase), sp = sin(phase);
+				for (int c = 0; c < 3; c++) {
+					o[c][0] = (float)(b[c] * cp);
+					o[c][1] = (float)(b[c] * sp);
+				}
+			}
+	free(K);
+	return inside;
+}

@@ -93,6 +93,17 @@ void orc_rng_vector(orc_rng *r, double out[3]);
 double orc_rng_powerlaw(orc_rng *r, double index, double min, double max);
 size_t orc_rng_bin(orc_rng *r, const double *cdf, size_t n);
 
+/* Turbulent grid coefficients before the inverse FFT: SimpleGridTurbulence::initTurbulence
+ * (kind ORC_TURB_SIMPLE: amplitude randNorm() * k^(alpha/2), 4 draws per mode) and
+ * GridTurbulence::initTurbulence (kind ORC_TURB_GENERAL: amplitude sqrt(E(k * 2 pi / spacing)),
+ * E(kh = k lBendover) = kh^q / (1 + kh^2)^((s + q)/2 + 1), 2 draws per mode).  bk holds
+ * 3 components x n x n x (n/2 + 1) complex floats (re, im); returns the number of modes inside
+ * [spacing/lMax, spacing/lMin]. */
+enum { ORC_TURB_SIMPLE = 0, ORC_TURB_GENERAL = 1, ORC_TURB_GENERAL_POWERLAW = 2 };
+/* ORC_TURB_GENERAL_POWERLAW: GridTurbulence with a SimpleTurbulenceSpectrum, E(k) = k^(-s-2) */
+size_t orc_turbulence_coeffs(int n, double spacing, double lmin, double lmax, int kind, double alpha, double lbendover,
+                             double sindex, double qindex, uint32_t seed, float *bk);
+
 /* batch helpers for the tests */
 void orc_rng_randint_batch(uint32_t seed, size_t n, uint32_t *out);
 void orc_noise_grid(uint32_t seed, int nx, int ny, int nz, double amp, float *out);

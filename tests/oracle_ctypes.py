@@ -88,6 +88,9 @@ def lib():
         L.orc_pow_batch.argtypes = [_dp, C.c_double, C.c_size_t, _dp, C.c_int]
         L.orc_noise_grid.argtypes = [C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_double, _fp]
         L.orc_rng_randint_batch.argtypes = [C.c_uint32, C.c_size_t, _P(C.c_uint32)]
+        L.orc_turbulence_coeffs.restype = C.c_size_t
+        L.orc_turbulence_coeffs.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int] + \
+            [C.c_double] * 4 + [C.c_uint32, _fp]
         L.orc_sample_particles.argtypes = [C.c_uint32, C.c_size_t, C.c_double, C.c_double, _dp, C.c_size_t,
                                            _ip, _dp, _dp]
         adj = [_fp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_int, C.c_size_t, _dp, _dp,
@@ -369,3 +372,32 @@ def module_args(meta):
         kw["order"] = list(meta["order"]) + (["maxlen"] if "maxTrajectoryLength" in meta and "maxlen" not in meta["order"]
                                              else [])
     return kw
+
+
+TURB_SIMPLE, TURB_GENERAL, TURB_GENERAL_POWERLAW = 0, 1, 2
+
+
+def turbulence_coeffs(n, spacing, lmin, lmax, kind=TURB_SIMPLE, alpha=-11 / 3., lbendover=1.0, sindex=5 / 3.,
+                      qindex=4.0, seed=1):
+    """k-space field of the reference's turbulence generators (glibc libm, MT19937 seeded with seed):
+    complex64 array (3, n, n, n//2 + 1) and the number of modes in range."""
+    n2 = n // 2 + 1
+    bk = np.zeros((3, n, n, n2, 2), dtype=np.float32)
+    modes = lib().orc_turbulence_coeffs(n, spacing, lmin, lmax, kind, alpha, lbendover, sindex, qindex, seed,
+                                        ptr(bk, _fp))
+    return bk[..., 0] + 1j * bk[..., 1], modes
+
+
+def turbulence_grid(n, spacing, lmin, lmax, **kw):
+    """The unnormalised grid: the coefficients through an unnormalised inverse real FFT in double
+    precision (GridTurbulence.cpp:147-185), as an (n, n, n, 3) float64 array."""
+    X, modes = turbulence_coeffs(n, spacing, lmin, lmax, **kw)
+    out = np.stack([np.fft.irfftn(X[c].astype(np.complex128), s=(n, n, n), axes=(0, 1, 2), norm="forward")
+                    for c in range(3)], axis=-1)
+    return out, modes
+
+
+def randint_batch(seed, n):
+    out = np.zeros(n, dtype=np.uint32)
+    lib().orc_rng_randint_batch(seed, n, out.ctypes.data_as(_P(C.c_uint32)))
+    return out

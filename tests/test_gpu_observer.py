@@ -89,12 +89,15 @@ def test_observer_fixed_step_bit_exact_vs_reference(name):
 
 
 def test_observer_adaptive_lands_on_the_times():
-    """With adaptive steps the observer's limitNextStep makes every detection land on its time
-    (to the rounding of len + (t - len))."""
+    """With adaptive steps the observer's limitNextStep makes detections land on their times (to the
+    rounding of len + (t - len)), except where a rejected trial shrank the limited step, so that the
+    next step overshoots the time: the reference's own run has 23 such detections of 3072 here."""
     c = O.load_case("ck_grid32_observer")
     g, det = run_gpu(c, False, int(c["nsteps"].max()))
     assert np.all(g["detection_index"] == len(c["times"]))
-    assert np.allclose(det[:, :, 6], c["times"][:, None], rtol=1e-14, atol=0)
+    on_time = np.isclose(det[:, :, 6], c["times"][:, None], rtol=1e-14, atol=0)
+    ref_on_time = np.isclose(c["det_len"], c["times"][:, None], rtol=1e-14, atol=0)
+    assert np.array_equal(on_time, ref_on_time) and on_time.mean() > 0.99
     # the reference recorded the same lengths
     assert np.allclose(det[:, :, 6], c["det_len"], rtol=1e-14, atol=0)
 
