@@ -376,6 +376,15 @@ def main():
                     "kernel": "k_propagate<%s, %s>" % ("FieldGridCells16" if fp16 else (
                         "FieldGridCells" if args.grid_layout == "cells" else "FieldGrid"), w["kind"]),
                     "kernel_ms_avg": kern_s * 1e3, "algorithmic_bytes_per_launch": bytes_per_launch}
+        if args.grid_layout == "cells":
+            # every evaluation reads one random line of the cell table; the GPU's rate of such reads,
+            # measured on a table of the same size, is the ceiling this kernel can reach
+            line = 64 if fp16 else 96
+            table = w["grid_n"] ** 3 * (64 if fp16 else 128)
+            ceiling = capi.measure_gather(table, line)
+            evals_s = evals / launches / kern_s
+            roofline["random_line_ceiling"] = {"lines_per_s": ceiling, "bytes_per_line": line, "table_bytes": table,
+                                               "evals_per_s": evals_s, "frac": evals_s / ceiling}
     else:
         config.update({"field": "PlaneWaveTurbulence Nm=%d" % w["nm"]})
         flops = evals * (w["nm"] * FLOP_PER_MODE + FLOP_BORIS) / launches

@@ -98,6 +98,7 @@ def lib():
         L.crp_propagate.argtypes = [_vp, _vp, C.c_longlong, C.POINTER(RunStats)]
         L.crp_turbulence_mode_count.argtypes = [C.POINTER(TurbulenceDesc), C.POINTER(C.c_uint64)]
         L.crp_turbulence_generate.argtypes = [C.POINTER(TurbulenceDesc), C.POINTER(C.c_uint32), C.c_uint64, _fp]
+        L.crp_measure_gather.argtypes = [C.c_uint64, C.c_int, _dp]
         L.crp_set_device.argtypes = [C.c_int]
         L.crp_get_device_count.argtypes = [C.POINTER(C.c_int)]
         _lib = L
@@ -392,3 +393,10 @@ def device_count():
     n = C.c_int(0)
     check(lib().crp_get_device_count(C.byref(n)))
     return n.value
+
+
+def measure_gather(table_bytes, bytes_per_line=96):
+    """Random line reads per second of this GPU (the cell-table evaluation's ceiling)."""
+    r = C.c_double()
+    check(lib().crp_measure_gather(int(table_bytes), int(bytes_per_line), C.byref(r)))
+    return r.value

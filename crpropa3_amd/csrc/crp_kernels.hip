@@ -27,6 +27,7 @@
 #include "crp_propagate.hpp"
 #include "crp_adjoint.hpp"
 #include "crp_turbulence.hpp"
+#include "crp_probe.hpp"
 #include "crpropa3_amd.h"
 
 #include <dlfcn.h>
@@ -1668,6 +1669,55 @@ int crp_turbulence_generate(const crp_turbulence_desc *d, const uint32_t *draws,
 	TURB_TRY(hipMemcpy(grid, dGrid, 3 * npts * sizeof(float), hipMemcpyDefault));
 #undef TURB_TRY
 	cleanup();
+	return CRP_OK;
+}
+
+// ------------------------------------------------------------------ random-gather ceiling
+int crp_measure_gather(uint64_t table_bytes, int bytes_per_line, double *lines_per_s) {
+	if (!lines_per_s || (bytes_per_line != 64 && bytes_per_line != 96 && bytes_per_line != 128) ||
+	    table_bytes < 128 * 1024 || table_bytes / 128 > 0xffffffffull)
+		return fail(CRP_E_INVALID, "crp_measure_gather: bytes_per_line 64/96/128, table 128 KiB .. 512 GiB");
+	int dev = 0, ncu = 0;
+	CRP_HIP_TRY(hipGetDevice(&dev));
+	CRP_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+	const unsigned nlines = (unsigned)(table_bytes / 128);
+	const unsigned blocks = (unsigned)ncu * 8, threads = blocks * 256;
+	const int iters = 256;
+	float4 *tab = nullptr;
+	float *out = nullptr;
+	hipEvent_t a = nullptr, b = nullptr;
+	auto cleanup = [&]() {
+		(void)hipFree(tab);
+		(void)hipFree(out);
+		if (a) (void)hipEventDestroy(a);
+		if (b) (void)hipEventDestroy(b);
+	};
+	hipError_t e = hipMalloc((void **)&tab, (size_t)nlines * 128);
+	if (e == hipSuccess) e = hipMalloc((void **)&out, threads * sizeof(float));
+	if (e == hipSuccess) e = hipMemset(tab, 0, (size_t)nlines * 128);
+	if (e == hipSuccess) e = hipEventCreate(&a);
+	if (e == hipSuccess) e = hipEventCreate(&b);
+	float ms = 0.f;
+	for (int rep = 0; rep < 2 && e == hipSuccess; rep++) { // first pass warms up
+		e = hipEventRecord(a, 0);
+		for (int r = 0; r < 3 && e == hipSuccess; r++) {
+			if (bytes_per_line == 64)
+				hipLaunchKernelGGL(k_probe_gather<4>, blocks, 256, 0, 0, tab, nlines, iters, out);
+			else if (bytes_per_line == 96)
+				hipLaunchKernelGGL(k_probe_gather<6>, blocks, 256, 0, 0, tab, nlines, iters, out);
+			else
+				hipLaunchKernelGGL(k_probe_gather<8>, blocks, 256, 0, 0, tab, nlines, iters, out);
+			e = hipGetLastError();
+		}
+		if (e == hipSuccess) e = hipEventRecord(b, 0);
+		if (e == hipSuccess) e = hipEventSynchronize(b);
+		if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+	}
+	cleanup();
+	if (e != hipSuccess)
+		return fail(e == hipErrorOutOfMemory ? CRP_E_NOMEM : CRP_E_HIP,
+		            std::string("crp_measure_gather: ") + hipGetErrorString(e));
+	*lines_per_s = 3.0 * threads * (double)iters / (ms * 1e-3);
 	return CRP_OK;
 }
 

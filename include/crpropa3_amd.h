@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define CRP_ABI_VERSION 5
+#define CRP_ABI_VERSION 6
 
 enum crp_status {
 	CRP_OK = 0,
@@ -276,6 +276,12 @@ int crp_turbulence_mode_count(const crp_turbulence_desc *d, uint64_t *modes);
 /* The unnormalised grid: grid[3 * ((ix * n + iy) * n + iz) + c], n^3 x 3 floats, host or device
  * memory.  ndraws must equal draws-per-mode x crp_turbulence_mode_count. */
 int crp_turbulence_generate(const crp_turbulence_desc *d, const uint32_t *draws, uint64_t ndraws, float *grid);
+
+/* ---------------------------------------------------------------- measurement
+ * No counterpart in the reference: the rate of random line reads of this GPU (lines per second),
+ * each lane reading bytes_per_line (64, 96 or 128) from a random 128-byte line of a table of
+ * table_bytes; the ceiling of the cell-table field evaluation (96 bytes per evaluation). */
+int crp_measure_gather(uint64_t table_bytes, int bytes_per_line, double *lines_per_s);
 
 #ifdef __cplusplus
 }
