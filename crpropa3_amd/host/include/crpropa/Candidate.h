@@ -69,6 +69,20 @@ public:
 	ref_ptr<Candidate> clone(bool recursive = false) const;
 
 	uint64_t getSerialNumber() const { return serialNumber; }
+	void setSerialNumber(uint64_t snr) { serialNumber = snr; }
+	/* the first ancestor's serial number, and the parent's (src/Candidate.cpp:240-252) */
+	uint64_t getSourceSerialNumber() const { return parent ? parent->getSourceSerialNumber() : serialNumber; }
+	uint64_t getCreatedSerialNumber() const { return parent ? parent->getSerialNumber() : serialNumber; }
+	static void setNextSerialNumber(uint64_t snr);
+	static uint64_t getNextSerialNumber();
+	/* active again at the source state, length and time 0 (src/Candidate.cpp:264-270) */
+	void restart() {
+		setActive(true);
+		setTrajectoryLength(0);
+		setTime(0);
+		previous = source;
+		current = source;
+	}
 	std::string getDescription() const;
 
 private:

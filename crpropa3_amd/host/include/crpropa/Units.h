@@ -52,6 +52,18 @@ static const double kpc = kiloparsec;
 static const double Mpc = megaparsec;
 static const double Gpc = gigaparsec;
 
+static const double minute = 60 * second;
+static const double hour = 3600 * second;
+static const double day = 24 * hour;
+static const double year = 365.25 * 24 * hour;
+static const double kiloyear = 1e3 * year;
+static const double Megayear = 1e6 * year;
+static const double Gigayear = 1e9 * year;
+static const double yr = year;
+static const double kyr = kiloyear;
+static const double Myr = Megayear;
+static const double Gyr = Gigayear;
+
 static const double kilometer = 1000 * meter;
 static const double km = kilometer;
 

@@ -24,6 +24,7 @@
 #include "crpropa/module/Boundary.h"
 #include "crpropa/module/BreakCondition.h"
 #include "crpropa/module/Observer.h"
+#include "crpropa/module/Output.h"
 #include "crpropa/module/PropagationBP.h"
 #include "crpropa/module/PropagationCK.h"
 
@@ -117,6 +118,8 @@ PYBIND11_MODULE(_crpropa, m) {
 	UNIT(electronvolt); UNIT(eV); UNIT(keV); UNIT(MeV); UNIT(GeV); UNIT(TeV); UNIT(PeV); UNIT(EeV);
 	UNIT(au); UNIT(parsec); UNIT(kiloparsec); UNIT(megaparsec); UNIT(gigaparsec); UNIT(pc); UNIT(kpc); UNIT(Mpc);
 	UNIT(Gpc); UNIT(kilometer); UNIT(km);
+	UNIT(minute); UNIT(hour); UNIT(day); UNIT(year); UNIT(kiloyear); UNIT(Megayear); UNIT(Gigayear);
+	UNIT(yr); UNIT(kyr); UNIT(Myr); UNIT(Gyr);
 #undef UNIT
 
 	bind_vector3<double>(m, "Vector3d");
@@ -181,6 +184,12 @@ PYBIND11_MODULE(_crpropa, m) {
 	    .def("hasProperty", &Candidate::hasProperty)
 	    .def("removeProperty", &Candidate::removeProperty)
 	    .def("getSerialNumber", &Candidate::getSerialNumber)
+	    .def("setSerialNumber", &Candidate::setSerialNumber)
+	    .def("getSourceSerialNumber", &Candidate::getSourceSerialNumber)
+	    .def("getCreatedSerialNumber", &Candidate::getCreatedSerialNumber)
+	    .def_static("setNextSerialNumber", &Candidate::setNextSerialNumber)
+	    .def_static("getNextSerialNumber", &Candidate::getNextSerialNumber)
+	    .def("restart", &Candidate::restart)
 	    .def("getDescription", &Candidate::getDescription);
 
 	py::class_<Module, PyModule, ref_ptr<Module>>(m, "Module")
@@ -431,6 +440,98 @@ PYBIND11_MODULE(_crpropa, m) {
 	    .def("setDeactivateOnDetection", &Observer::setDeactivateOnDetection)
 	    .def("setFlag", &Observer::setFlag)
 	    .def("process", [](const Observer &o, Candidate *c) { o.process(c); });
+
+	// outputs (include/crpropa/module/Output.h, TextOutput.h, ParticleCollector.h)
+	{
+		py::class_<Output, Module, ref_ptr<Output>> out(m, "Output");
+		py::enum_<Output::OutputColumn>(out, "OutputColumn")
+		    .value("TrajectoryLengthColumn", Output::TrajectoryLengthColumn)
+		    .value("ColumnDensityColumn", Output::ColumnDensityColumn)
+		    .value("RedshiftColumn", Output::RedshiftColumn)
+		    .value("CurrentIdColumn", Output::CurrentIdColumn)
+		    .value("CurrentEnergyColumn", Output::CurrentEnergyColumn)
+		    .value("CurrentPositionColumn", Output::CurrentPositionColumn)
+		    .value("CurrentDirectionColumn", Output::CurrentDirectionColumn)
+		    .value("SourceIdColumn", Output::SourceIdColumn)
+		    .value("SourceEnergyColumn", Output::SourceEnergyColumn)
+		    .value("SourcePositionColumn", Output::SourcePositionColumn)
+		    .value("SourceDirectionColumn", Output::SourceDirectionColumn)
+		    .value("CreatedIdColumn", Output::CreatedIdColumn)
+		    .value("CreatedEnergyColumn", Output::CreatedEnergyColumn)
+		    .value("CreatedPositionColumn", Output::CreatedPositionColumn)
+		    .value("CreatedDirectionColumn", Output::CreatedDirectionColumn)
+		    .value("CandidateTagColumn", Output::CandidateTagColumn)
+		    .value("SerialNumberColumn", Output::SerialNumberColumn)
+		    .value("WeightColumn", Output::WeightColumn)
+		    .value("TimeColumn", Output::TimeColumn)
+		    .export_values();
+		py::enum_<Output::OutputType>(out, "OutputType")
+		    .value("Trajectory1D", Output::Trajectory1D)
+		    .value("Trajectory3D", Output::Trajectory3D)
+		    .value("Event1D", Output::Event1D)
+		    .value("Event3D", Output::Event3D)
+		    .value("Everything", Output::Everything)
+		    .export_values();
+		out.def(py::init<>())
+		    .def(py::init<Output::OutputType>())
+		    .def("setEnergyScale", &Output::setEnergyScale)
+		    .def("getEnergyScale", &Output::getEnergyScale)
+		    .def("setLengthScale", &Output::setLengthScale)
+		    .def("getLengthScale", &Output::getLengthScale)
+		    .def("setTimeScale", &Output::setTimeScale)
+		    .def("getTimeScale", &Output::getTimeScale)
+		    .def("setOutputType", &Output::setOutputType)
+		    .def("set", &Output::set)
+		    .def("enable", &Output::enable)
+		    .def("disable", &Output::disable)
+		    .def("enableAll", &Output::enableAll)
+		    .def("disableAll", &Output::disableAll)
+		    .def("set1D", &Output::set1D)
+		    .def("size", &Output::size)
+		    .def("enableProperty",
+		         [](Output &o, const std::string &name, py::object dflt, const std::string &comment) {
+			         o.enableProperty(name, py::str(dflt), comment);
+		         },
+		         py::arg("property"), py::arg("defaultValue"), py::arg("comment") = "")
+		    .def("process", [](const Output &o, Candidate *c) { o.process(c); });
+	}
+	py::class_<TextOutput, Output, ref_ptr<TextOutput>>(m, "TextOutput")
+	    .def(py::init<>())
+	    .def(py::init<Output::OutputType>())
+	    .def(py::init<const std::string &>())
+	    .def(py::init<const std::string &, Output::OutputType>())
+	    .def("process", [](const TextOutput &o, Candidate *c) { o.process(c); })
+	    .def("close", &TextOutput::close)
+	    .def("gzip", &TextOutput::gzip)
+	    .def("enableRandomSeeds", &TextOutput::enableRandomSeeds)
+	    .def_static("load", &TextOutput::load);
+	py::class_<ParticleCollector, Module, ref_ptr<ParticleCollector>>(m, "ParticleCollector")
+	    .def(py::init<>())
+	    .def(py::init<std::size_t>())
+	    .def(py::init<std::size_t, bool>())
+	    .def(py::init<std::size_t, bool, bool>())
+	    .def("process", [](const ParticleCollector &p, Candidate *c) { p.process(c); })
+	    .def("reprocess", &ParticleCollector::reprocess)
+	    .def("dump", &ParticleCollector::dump)
+	    .def("load", &ParticleCollector::load)
+	    .def("size", &ParticleCollector::size)
+	    .def("__len__", &ParticleCollector::size)
+	    .def("__getitem__",
+	         [](const ParticleCollector &p, std::size_t i) {
+		         if (i >= p.size())
+			         throw py::index_error();
+		         return p[i];
+	         })
+	    .def("__iter__",
+	         [](ParticleCollector &p) { return py::make_iterator(p.getContainer().begin(), p.getContainer().end()); },
+	         py::keep_alive<0, 1>())
+	    .def("clearContainer", &ParticleCollector::clearContainer)
+	    .def("getContainer", [](const ParticleCollector &p) { return p.getContainer(); })
+	    .def("setClone", &ParticleCollector::setClone)
+	    .def("getClone", &ParticleCollector::getClone)
+	    .def("getTrajectory", [](const ParticleCollector &p, ModuleList *ml, std::size_t i, Module *out) {
+		    p.getTrajectory(ml, i, out);
+	    });
 
 	py::class_<ModuleList, Module, ref_ptr<ModuleList>>(m, "ModuleList")
 	    .def(py::init<>())

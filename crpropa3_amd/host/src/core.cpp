@@ -98,6 +98,9 @@ std::string ParticleState::getDescription() const {
 // ------------------------------------------------------------------ Candidate
 static std::atomic<uint64_t> g_nextSerial(0);
 
+void Candidate::setNextSerialNumber(uint64_t snr) { g_nextSerial = snr; }
+uint64_t Candidate::getNextSerialNumber() { return g_nextSerial; }
+
 Candidate::Candidate(int id, double E, Vector3d pos, Vector3d dir, double z, double weight, std::string tagOrigin)
     : parent(nullptr), active(true), weight(weight), redshift(z), trajectoryLength(0), currentStep(0), nextStep(0),
       tagOrigin(tagOrigin), time(0) {
