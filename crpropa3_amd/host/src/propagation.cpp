@@ -467,6 +467,11 @@ bool ModuleList::runBatched(const candidate_vector_t &cands) {
 			for (int32_t k = det0[i]; k < pk.det[i] && k < (int32_t)times.size(); k++) {
 				const double *r = &snaps[((size_t)k * n + i) * 9];
 				ref_ptr<Candidate> view = cand->clone(false);
+				if (!f.obs->getClone()) {
+					// the reference hands over the candidate itself: same serial number and parent
+					view->setSerialNumber(cand->getSerialNumber());
+					view->parent = cand->parent;
+				}
 				const Vector3d vpos(r[0], r[1], r[2]);
 				view->current.setPosition(vpos);
 				view->current.setDirectionExact(Vector3d(r[3], r[4], r[5]));

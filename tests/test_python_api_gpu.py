@@ -396,3 +396,38 @@ def test_modulelist_sphere_observer_order_fused_equals_host(order):
     assert [r[:6] for r in fused] == ref
     assert np.array_equal(np.array([f[:3] for f in ffin]).view(np.uint64), st.pos.view(np.uint64))
     assert all(f[5] == REJECTED for f in ffin)
+
+
+def test_modulelist_text_output_as_detection_action(tmp_path):
+    """A TextOutput (all columns) as the detection action of a fused Observer writes the same lines as
+    the step-wise host run (up to line order: fused detections come per candidate)."""
+    from crpropa3_amd import Observer, ObserverTimeEvolution, Output, TextOutput
+
+    class Noop(Module):
+        def process(self, c):
+            pass
+
+    texts = []
+    for extra in (None, Noop()):
+        grid, gvals, gm, cands, E, Z, d = _grid_candidates(48, 61)
+        fn = str(tmp_path / ("out%d.txt" % len(texts)))
+        out = TextOutput(fn, Output.Everything)
+        obs = Observer()
+        obs.add(ObserverTimeEvolution(0.5 * Mpc, 0.5 * Mpc, 6))
+        obs.setDeactivateOnDetection(False)
+        obs.onDetection(out)
+        sim = ModuleList()
+        sim.add(PropagationCK(MagneticFieldGrid(grid), 1e-4, 1 * kpc, 1 * Mpc))
+        sim.add(obs)
+        sim.add(MaximumTrajectoryLength(3.2 * Mpc))
+        if extra is not None:
+            sim.add(extra)
+        sim.run(cands)
+        out.close()
+        lines = open(fn).read().splitlines()
+        body = [ln for ln in lines if not ln.startswith("#")]
+        # serial numbers differ between the two runs' candidates: compare the other columns
+        texts.append(sorted("\t".join(c for k, c in enumerate(ln.split("\t")) if k not in (3, 12, 21))
+                            for ln in body))
+        assert len(body) == 6 * 48
+    assert texts[0] == texts[1]
