@@ -268,11 +268,12 @@ CRP_MATH_FN void crp_sincos_dd_reduced(crp_dd a, crp_dd *s_out, crp_dd *c_out) {
 }
 
 /*
- * cos(x), correctly rounded.  The reduction keeps about 2^-130 absolute
- * accuracy for |x| < 2^23 * pi/2 (1.3e7); beyond it products q*P_k stay exact
- * (two_prod) but cancellation is only tracked to double-double precision.
+ * cos(x), accurate phase: about 2^-100 relative accuracy.  The reduction keeps
+ * about 2^-130 absolute accuracy for |x| < 2^23 * pi/2 (1.3e7); beyond it
+ * products q*P_k stay exact (two_prod) but cancellation is only tracked to
+ * double-double precision.
  */
-CRP_MATH_FN double crp_cos_cr(double x) {
+CRP_MATH_FN double crp_cos_accurate(double x) {
 	double ax = x < 0 ? -x : x;
 	if (!(ax < 1.0 / 0.0))
 		return x - x; /* NaN for inf and NaN */
@@ -312,6 +313,87 @@ CRP_MATH_FN double crp_cos_cr(double x) {
 	default: res = s; break;
 	}
 	return res.hi + res.lo;
+}
+
+/*
+ * cos(x), quick phase (Ziv's strategy): the same reduction, then sin or cos of
+ * the reduced argument a = |r| <= pi/4 in double arithmetic around the table
+ * value: with a = j/32 + d,
+ *   cos a = C + (C (cos d - 1) - S sin d),   sin a = S + (S (cos d - 1) + C sin d),
+ * where the leading product S*d_hi (or C*d_hi) is formed exactly (two_prod) and
+ * summed exactly with the table's high part, and everything else (at most 2^-13
+ * of the result) in double.  Error budget, relative to the result: polynomial
+ * truncation < 2^-80, cos d - 1 and its rounding < 2^-63, the double sum of the
+ * small terms < 2^-64, dropped second-order terms of the low parts < 2^-65; in
+ * total below 2^-62.  The result y_h + y_l is returned when every value within
+ * 2^-60 |y_h| of it rounds to y_h (half an ulp, a quarter below a power of two);
+ * otherwise the accurate phase decides (about 1.5 % of arguments).
+ * Returns 0 when the quick phase does not apply or cannot decide.
+ */
+CRP_MATH_FN int crp_cos_quick(double x, double *out) {
+	double ax = x < 0 ? -x : x;
+	if (!(ax < 1.0 / 0.0) || ax < 0x1p-27)
+		return 0;
+	double q = crp_rint(x * CRP_TWO_OVER_PI);
+	if (!(q < 8388608.0 && q > -8388608.0))
+		return 0;
+	crp_dd r;
+	if (q == 0.0) {
+		r = crp_mkdd(x, 0.0);
+	} else {
+		double r0 = x - q * CRP_PIO2_1; /* exact */
+		r = crp_two_sum(r0, -q * CRP_PIO2_2);
+		r = crp_dd_add_d(r, -q * CRP_PIO2_3);
+		r = crp_dd_add_d(r, -q * CRP_PIO2_4);
+		r = crp_dd_add_d(r, -q * CRP_PIO2_5);
+	}
+	int neg = r.hi < 0;
+	double ah = neg ? -r.hi : r.hi, al = neg ? -r.lo : r.lo;
+	int j = (int)crp_rint(ah * 32.0);
+	double dh = ah - (double)j * (1.0 / 32.0); /* exact (Sterbenz for j >= 1) */
+	double dl = al;                             /* |dl| <= 2^-54 */
+	double z = dh * dh;
+	double sdm = (dh * z) * (-1.0 / 6 + z * (1.0 / 120 + z * (-1.0 / 5040 + z * (1.0 / 362880.0)))); /* sin dh - dh */
+	double cdm1 = z * (-0.5 + z * (1.0 / 24 + z * (-1.0 / 720 + z * (1.0 / 40320.0)))) - dl * dh; /* cos d - 1 */
+	long long qi = (long long)q;
+	int n = (int)(qi & 3);
+	crp_dd y;
+	if (j == 0) {
+		y = (n & 1) ? crp_fast_two_sum(dh, dl + sdm) : crp_fast_two_sum(1.0, cdm1);
+	} else {
+		double Sh = crp_sincos_tab[4 * j], Sl = crp_sincos_tab[4 * j + 1];
+		double Ch = crp_sincos_tab[4 * j + 2], Cl = crp_sincos_tab[4 * j + 3];
+		if (n & 1) { /* sin a */
+			crp_dd P = crp_two_prod(Ch, dh);
+			crp_dd s = crp_two_sum(Sh, P.hi);
+			double t = (Sl + P.lo) + ((Sh * cdm1 + Ch * (dl + sdm)) + Cl * dh);
+			y = crp_fast_two_sum(s.hi, s.lo + t);
+		} else { /* cos a */
+			crp_dd P = crp_two_prod(Sh, dh);
+			crp_dd s = crp_two_sum(Ch, -P.hi);
+			double t = (Cl - P.lo) + ((Ch * cdm1 - Sh * (dl + sdm)) - Sl * dh);
+			y = crp_fast_two_sum(s.hi, s.lo + t);
+		}
+	}
+	/* rounding test; y.hi > 0 here */
+	uint64_t ub = crp_as_u64(y.hi);
+	double u = crp_from_u64(((ub >> 52) - 52) << 52); /* ulp(y.hi) */
+	double lim = ((ub & 0x000fffffffffffffULL) == 0 && y.lo < 0) ? 0.25 * u : 0.5 * u;
+	double ayl = y.lo < 0 ? -y.lo : y.lo;
+	if (!(ayl + y.hi * 0x1p-60 < lim))
+		return 0;
+	/* cos x = cos a, -sin r, -cos a, sin r for n = 0..3, with sin r = -sin a when r < 0 */
+	int minus = (n == 2) || (n == 1 && !neg) || (n == 3 && neg);
+	*out = minus ? -y.hi : y.hi;
+	return 1;
+}
+
+/* cos(x), correctly rounded */
+CRP_MATH_FN double crp_cos_cr(double x) {
+	double v;
+	if (crp_cos_quick(x, &v))
+		return v;
+	return crp_cos_accurate(x);
 }
 
 #endif

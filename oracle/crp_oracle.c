@@ -896,3 +896,10 @@ size_t orc_turbulence_coeffs(int n, double spacing, double lmin, double lmax, in
 	free(K);
 	return inside;
 }
+
+void orc_cos_phases(const double *x, size_t n, double *quick, int32_t *decided, double *accurate) {
+	for (size_t i = 0; i < n; i++) {
+		decided[i] = crp_cos_quick(x[i], &quick[i]);
+		accurate[i] = crp_cos_accurate(x[i]);
+	}
+}

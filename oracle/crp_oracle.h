@@ -104,6 +104,9 @@ enum { ORC_TURB_SIMPLE = 0, ORC_TURB_GENERAL = 1, ORC_TURB_GENERAL_POWERLAW = 2 
 size_t orc_turbulence_coeffs(int n, double spacing, double lmin, double lmax, int kind, double alpha, double lbendover,
                              double sindex, double qindex, uint32_t seed, float *bk);
 
+/* the two phases of the correctly rounded cos separately: quick[i] valid where decided[i] != 0 */
+void orc_cos_phases(const double *x, size_t n, double *quick, int32_t *decided, double *accurate);
+
 /* batch helpers for the tests */
 void orc_rng_randint_batch(uint32_t seed, size_t n, uint32_t *out);
 void orc_noise_grid(uint32_t seed, int nx, int ny, int nz, double amp, float *out);

@@ -88,6 +88,7 @@ def lib():
         L.orc_pow_batch.argtypes = [_dp, C.c_double, C.c_size_t, _dp, C.c_int]
         L.orc_noise_grid.argtypes = [C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_double, _fp]
         L.orc_rng_randint_batch.argtypes = [C.c_uint32, C.c_size_t, _P(C.c_uint32)]
+        L.orc_cos_phases.argtypes = [_dp, C.c_size_t, _dp, _P(C.c_int32), _dp]
         L.orc_turbulence_coeffs.restype = C.c_size_t
         L.orc_turbulence_coeffs.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int] + \
             [C.c_double] * 4 + [C.c_uint32, _fp]
@@ -401,3 +402,12 @@ def randint_batch(seed, n):
     out = np.zeros(n, dtype=np.uint32)
     lib().orc_rng_randint_batch(seed, n, out.ctypes.data_as(_P(C.c_uint32)))
     return out
+
+
+def cos_phases(x):
+    """(quick value, decided flag, accurate value) of the correctly rounded cos per argument."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    q, a = np.zeros_like(x), np.zeros_like(x)
+    d = np.zeros(x.shape, dtype=np.int32)
+    lib().orc_cos_phases(ptr(x), x.size, ptr(q), d.ctypes.data_as(_P(C.c_int32)), ptr(a))
+    return q, d.astype(bool), a

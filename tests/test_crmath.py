@@ -66,3 +66,16 @@ def test_glibc_rounding_difference_rates():
     rc = np.mean(O.cos_batch(x, O.LIBM_CR) != O.cos_batch(x, O.LIBM_GLIBC))
     assert rp < 3e-3 and rc < 5e-3
     print("pow(x,-0.2) glibc != CR: %.2e; cos glibc != CR: %.2e" % (rp, rc))
+
+
+def test_cos_quick_phase_agrees_with_accurate_phase():
+    """The quick phase returns a value only where its rounding test decides it; every such value equals
+    the accurate phase's (2e6 arguments here; 1e9 checked the same way during development)."""
+    rng = np.random.default_rng(5)
+    n = 400000
+    k = np.floor(rng.uniform(0, 1e5, n))
+    x = np.concatenate([rng.uniform(-10, 10, n), rng.uniform(-1e6, 1e6, n), rng.uniform(-1.3e7, 1.3e7, n),
+                        k * (np.pi / 2) + rng.uniform(-5e-7, 5e-7, n), np.ldexp(rng.uniform(0, 1, n), -rng.integers(0, 30, n))])
+    q, dec, a = O.cos_phases(x)
+    assert dec.mean() > 0.9
+    assert np.array_equal(q[dec].view(np.uint64), a[dec].view(np.uint64))
