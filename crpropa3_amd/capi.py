@@ -69,6 +69,9 @@ def lib():
                                               C.POINTER(_vp)]
         L.crp_field_create_grid3f_device.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int,
                                                      C.POINTER(_vp)]
+        L.crp_field_create_modulated_grid3f.argtypes = [_fp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int,
+                                                        _fp, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int,
+                                                        C.POINTER(_vp)]
         L.crp_field_create_planewave.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, C.POINTER(_vp)]
         L.crp_field_destroy.argtypes = [_vp]
         L.crp_field_set_grid_layout.argtypes = [_vp, C.c_int]
@@ -152,6 +155,24 @@ class Field:
         h = _vp()
         check(lib().crp_field_create_grid3f(_p(v, _fp), v.shape[0], v.shape[1], v.shape[2], _p(o), _p(s),
                                             int(reflective), int(clip_volume), C.byref(h)))
+        return Field(h)
+
+    @staticmethod
+    def modulated_grid3f(values, origin, spacing, modulation, morigin, mspacing, reflective=False,
+                         clip_volume=False, mreflective=True, mclip_volume=False):
+        """ModulatedMagneticFieldGrid: Grid3f values (Nx, Ny, Nz, 3) times a Grid1f (Mx, My, Mz)."""
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        m = np.ascontiguousarray(modulation, dtype=np.float32)
+        if v.ndim != 4 or v.shape[3] != 3 or m.ndim != 3:
+            raise ValueError("grid values (Nx, Ny, Nz, 3) and modulation (Mx, My, Mz) expected")
+        o, mo = _f64(origin, (3,)), _f64(morigin, (3,))
+        s = _f64(np.broadcast_to(np.asarray(spacing, dtype=np.float64), (3,)))
+        ms = _f64(np.broadcast_to(np.asarray(mspacing, dtype=np.float64), (3,)))
+        h = _vp()
+        check(lib().crp_field_create_modulated_grid3f(_p(v, _fp), v.shape[0], v.shape[1], v.shape[2], _p(o), _p(s),
+                                                      int(reflective), int(clip_volume), _p(m, _fp), m.shape[0],
+                                                      m.shape[1], m.shape[2], _p(mo), _p(ms), int(mreflective),
+                                                      int(mclip_volume), C.byref(h)))
         return Field(h)
 
     @staticmethod

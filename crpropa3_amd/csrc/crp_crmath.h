@@ -316,19 +316,19 @@ CRP_MATH_FN double crp_cos_accurate(double x) {
 }
 
 /*
- * cos(x), quick phase (Ziv's strategy): the same reduction, then sin or cos of
- * the reduced argument a = |r| <= pi/4 in double arithmetic around the table
- * value: with a = j/32 + d,
- *   cos a = C + (C (cos d - 1) - S sin d),   sin a = S + (S (cos d - 1) + C sin d),
- * where the leading product S*d_hi (or C*d_hi) is formed exactly (two_prod) and
- * summed exactly with the table's high part, and everything else (at most 2^-13
- * of the result) in double.  Error budget, relative to the result: polynomial
- * truncation < 2^-80, cos d - 1 and its rounding < 2^-63, the double sum of the
- * small terms < 2^-64, dropped second-order terms of the low parts < 2^-65; in
- * total below 2^-62.  The result y_h + y_l is returned when every value within
- * 2^-60 |y_h| of it rounds to y_h (half an ulp, a quarter below a power of two);
- * otherwise the accurate phase decides (about 1.5 % of arguments).
- * Returns 0 when the quick phase does not apply or cannot decide.
+ * cos(x), quick phase (Ziv's strategy).  On a GPU every lane of a wave waits for
+ * the slowest, so the accurate phase must be rare per wave, not only per call:
+ * this phase aims at 2^-67 relative accuracy.  After the same reduction,
+ * a = |r| = j/32 + d with d = d_h + d_l, z = d_h^2 = z_h + z_l (exact), and
+ *   cos a = C - S d_h - C z_h/2 + (small),   sin a = S + C d_h - S z_h/2 + (small),
+ * the three leading terms formed and added exactly (two_prod, two_sum), the rest
+ * (at most 2^-19 of the result: the low parts, C or S times the z^2 series, the
+ * d^3 series, the d_l cross terms) in double.  Error budget relative to the
+ * result: series truncation < 2^-80, the small terms' rounding < 2^-69.5, the
+ * reduction < 2^-96 for |r| >= 2^-30 (smaller |r| goes to the accurate phase).
+ * The result is returned when every value within 2^-66 |y_h| of y_h + y_l rounds
+ * to y_h (half an ulp, a quarter below a power of two): about 1 argument in
+ * 4000 is left to the accurate phase.  Returns 0 when it does not decide.
  */
 CRP_MATH_FN int crp_cos_quick(double x, double *out) {
 	double ax = x < 0 ? -x : x;
@@ -349,30 +349,48 @@ CRP_MATH_FN int crp_cos_quick(double x, double *out) {
 	}
 	int neg = r.hi < 0;
 	double ah = neg ? -r.hi : r.hi, al = neg ? -r.lo : r.lo;
+	if (ah < 0x1p-30)
+		return 0;
 	int j = (int)crp_rint(ah * 32.0);
 	double dh = ah - (double)j * (1.0 / 32.0); /* exact (Sterbenz for j >= 1) */
 	double dl = al;                             /* |dl| <= 2^-54 */
-	double z = dh * dh;
-	double sdm = (dh * z) * (-1.0 / 6 + z * (1.0 / 120 + z * (-1.0 / 5040 + z * (1.0 / 362880.0)))); /* sin dh - dh */
-	double cdm1 = z * (-0.5 + z * (1.0 / 24 + z * (-1.0 / 720 + z * (1.0 / 40320.0)))) - dl * dh; /* cos d - 1 */
+	crp_dd zz = crp_two_prod(dh, dh);
+	double z = zz.hi, hz = 0.5 * z, hzl = 0.5 * zz.lo; /* z/2 = hz + hzl exactly */
+	double cz = (z * z) * (1.0 / 24 + z * (-1.0 / 720 + z * (1.0 / 40320 + z * (-1.0 / 3628800.0)))); /* cos d_h - 1 + z/2 */
+	double sdm = (dh * z) * (-1.0 / 6 + z * (1.0 / 120 + z * (-1.0 / 5040 + z * (1.0 / 362880.0)))); /* sin d_h - d_h */
+	double dcross = dl * dh;   /* d_l sin d_h, to 2^-113 */
+	double dsin = dl - dl * hz; /* d_l cos d_h, to 2^-120 */
 	long long qi = (long long)q;
 	int n = (int)(qi & 3);
 	crp_dd y;
 	if (j == 0) {
-		y = (n & 1) ? crp_fast_two_sum(dh, dl + sdm) : crp_fast_two_sum(1.0, cdm1);
+		if (n & 1) { /* sin a = d_h + (sdm + d_l cos d_h) */
+			y = crp_fast_two_sum(dh, sdm + dsin);
+		} else { /* cos a = 1 - z/2 + (cz - d_l d_h) */
+			crp_dd s = crp_two_sum(1.0, -hz);
+			y = crp_fast_two_sum(s.hi, s.lo + ((cz - hzl) - dcross));
+		}
 	} else {
 		double Sh = crp_sincos_tab[4 * j], Sl = crp_sincos_tab[4 * j + 1];
 		double Ch = crp_sincos_tab[4 * j + 2], Cl = crp_sincos_tab[4 * j + 3];
-		if (n & 1) { /* sin a */
+		if (n & 1) { /* sin a = S cos d + C sin d */
 			crp_dd P = crp_two_prod(Ch, dh);
-			crp_dd s = crp_two_sum(Sh, P.hi);
-			double t = (Sl + P.lo) + ((Sh * cdm1 + Ch * (dl + sdm)) + Cl * dh);
-			y = crp_fast_two_sum(s.hi, s.lo + t);
-		} else { /* cos a */
+			crp_dd Q = crp_two_prod(Sh, hz);
+			crp_dd s1 = crp_two_sum(Sh, P.hi);
+			crp_dd s2 = crp_two_sum(s1.hi, -Q.hi);
+			double small = Ch * (sdm + dsin) + Sh * (cz - dcross);
+			small += ((Sl + Cl * dh) - Sl * hz) - Sh * hzl;
+			small += (P.lo - Q.lo) + (s1.lo + s2.lo);
+			y = crp_fast_two_sum(s2.hi, small);
+		} else { /* cos a = C cos d - S sin d */
 			crp_dd P = crp_two_prod(Sh, dh);
-			crp_dd s = crp_two_sum(Ch, -P.hi);
-			double t = (Cl - P.lo) + ((Ch * cdm1 - Sh * (dl + sdm)) - Sl * dh);
-			y = crp_fast_two_sum(s.hi, s.lo + t);
+			crp_dd Q = crp_two_prod(Ch, hz);
+			crp_dd s1 = crp_two_sum(Ch, -P.hi);
+			crp_dd s2 = crp_two_sum(s1.hi, -Q.hi);
+			double small = Ch * (cz - dcross) - Sh * (sdm + dsin);
+			small += ((Cl - Sl * dh) - Cl * hz) - Ch * hzl;
+			small += (s1.lo + s2.lo) - (P.lo + Q.lo);
+			y = crp_fast_two_sum(s2.hi, small);
 		}
 	}
 	/* rounding test; y.hi > 0 here */
@@ -380,7 +398,7 @@ CRP_MATH_FN int crp_cos_quick(double x, double *out) {
 	double u = crp_from_u64(((ub >> 52) - 52) << 52); /* ulp(y.hi) */
 	double lim = ((ub & 0x000fffffffffffffULL) == 0 && y.lo < 0) ? 0.25 * u : 0.5 * u;
 	double ayl = y.lo < 0 ? -y.lo : y.lo;
-	if (!(ayl + y.hi * 0x1p-60 < lim))
+	if (!(ayl + y.hi * 0x1p-66 < lim))
 		return 0;
 	/* cos x = cos a, -sin r, -cos a, sin r for n = 0..3, with sin r = -sin a when r < 0 */
 	int minus = (n == 2) || (n == 1 && !neg) || (n == 3 && neg);

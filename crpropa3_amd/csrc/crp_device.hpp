@@ -280,6 +280,85 @@ struct FieldGrid {
 	}
 };
 
+// Grid<float>::interpolate + trilinearInterpolate (Grid.h:318-335, 540-582) with T = float: each
+// corner value times the double weights in double, and every b += term rounds the double sum to
+// float.  `g.values` holds one float per point.
+__device__ __forceinline__ float scalar_grid_eval(const GridDesc &g, const V3 &p) {
+	if (g.clip) {
+		bool in = (p.x >= g.origin[0]) && (p.x <= g.edge[0]);
+		in &= (p.y >= g.origin[1]) && (p.y <= g.edge[1]);
+		in &= (p.z >= g.origin[2]) && (p.z <= g.edge[2]);
+		if (!in)
+			return 0.f;
+	}
+	const double rx = div_cr(p.x - g.gridOrigin[0], g.spacing[0], g.invSpacing[0], g.fastDiv);
+	const double ry = div_cr(p.y - g.gridOrigin[1], g.spacing[1], g.invSpacing[1], g.fastDiv);
+	const double rz = div_cr(p.z - g.gridOrigin[2], g.spacing[2], g.invSpacing[2], g.fastDiv);
+	int x0, x1, y0, y1, z0, z1;
+	double fx, fy, fz;
+	if (g.reflective) {
+		reflective_clamp(rx, g.nx, x0, x1, fx);
+		reflective_clamp(ry, g.ny, y0, y1, fy);
+		reflective_clamp(rz, g.nz, z0, z1, fz);
+	} else {
+		periodic_clamp(rx, g.nx, g.invN[0], x0, x1, fx);
+		periodic_clamp(ry, g.ny, g.invN[1], y0, y1, fy);
+		periodic_clamp(rz, g.nz, g.invN[2], z0, z1, fz);
+	}
+	const double gx = 1 - fx, gy = 1 - fy, gz = 1 - fz;
+	const size_t sy = (size_t)g.nz, sx = (size_t)g.ny * g.nz;
+	const size_t bx0 = (size_t)x0 * sx, bx1 = (size_t)x1 * sx, by0 = (size_t)y0 * sy, by1 = (size_t)y1 * sy;
+	const float *v = g.values;
+	float b = 0.f;
+	b = (float)((double)b + (((double)v[bx0 + by0 + z0] * gx) * gy) * gz);
+	b = (float)((double)b + (((double)v[bx1 + by0 + z0] * fx) * gy) * gz);
+	b = (float)((double)b + (((double)v[bx0 + by1 + z0] * gx) * fy) * gz);
+	b = (float)((double)b + (((double)v[bx0 + by0 + z1] * gx) * gy) * fz);
+	b = (float)((double)b + (((double)v[bx1 + by0 + z1] * fx) * gy) * fz);
+	b = (float)((double)b + (((double)v[bx0 + by1 + z1] * gx) * fy) * fz);
+	b = (float)((double)b + (((double)v[bx1 + by1 + z0] * fx) * fy) * gz);
+	b = (float)((double)b + (((double)v[bx1 + by1 + z1] * fx) * fy) * fz);
+	return b;
+}
+
+// mixed precision: reciprocal spacing, FP32 weights, fused sum
+__device__ __forceinline__ float scalar_grid_evalf(const GridDesc &g, const V3 &p) {
+	if (g.clip) {
+		bool in = (p.x >= g.origin[0]) && (p.x <= g.edge[0]);
+		in &= (p.y >= g.origin[1]) && (p.y <= g.edge[1]);
+		in &= (p.z >= g.origin[2]) && (p.z <= g.edge[2]);
+		if (!in)
+			return 0.f;
+	}
+	const double rx = (p.x - g.gridOrigin[0]) * g.invSpacing[0];
+	const double ry = (p.y - g.gridOrigin[1]) * g.invSpacing[1];
+	const double rz = (p.z - g.gridOrigin[2]) * g.invSpacing[2];
+	int x0, x1, y0, y1, z0, z1;
+	double fx, fy, fz;
+	if (g.reflective) {
+		reflective_clamp(rx, g.nx, x0, x1, fx);
+		reflective_clamp(ry, g.ny, y0, y1, fy);
+		reflective_clamp(rz, g.nz, z0, z1, fz);
+	} else {
+		periodic_clamp(rx, g.nx, g.invN[0], x0, x1, fx);
+		periodic_clamp(ry, g.ny, g.invN[1], y0, y1, fy);
+		periodic_clamp(rz, g.nz, g.invN[2], z0, z1, fz);
+	}
+	const float wx = (float)fx, wy = (float)fy, wz = (float)fz, vx = 1.f - wx, vy = 1.f - wy, vz = 1.f - wz;
+	const size_t sy = (size_t)g.nz, sx = (size_t)g.ny * g.nz;
+	const size_t bx0 = (size_t)x0 * sx, bx1 = (size_t)x1 * sx, by0 = (size_t)y0 * sy, by1 = (size_t)y1 * sy;
+	const float *v = g.values;
+	float b = v[bx0 + by0 + z0] * (vx * vy * vz);
+	b = __builtin_fmaf(v[bx1 + by0 + z0], wx * vy * vz, b);
+	b = __builtin_fmaf(v[bx0 + by1 + z0], vx * wy * vz, b);
+	b = __builtin_fmaf(v[bx0 + by0 + z1], vx * vy * wz, b);
+	b = __builtin_fmaf(v[bx1 + by0 + z1], wx * vy * wz, b);
+	b = __builtin_fmaf(v[bx0 + by1 + z1], vx * wy * wz, b);
+	b = __builtin_fmaf(v[bx1 + by1 + z0], wx * wy * vz, b);
+	b = __builtin_fmaf(v[bx1 + by1 + z1], wx * wy * wz, b);
+	return b;
+}
+
 // Periodic Grid3f read through a cell-major corner table: cell (i,j,k) holds
 // the 8 corner values of the reference's corner order (Grid.h:572-579) as 24
 // floats, padded to 128 bytes, so one evaluation reads one aligned line
@@ -396,6 +475,23 @@ struct FieldGridCells16 {
 		const float scale = __uint_as_float(w[12]);
 		float3 b = trilinear_f32(v, fx, fy, fz);
 		return make_float3(b.x * scale, b.y * scale, b.z * scale);
+	}
+};
+
+// ModulatedMagneticFieldGrid::getField (src/magneticField/MagneticFieldGrid.cpp:51-55): the Grid3f
+// value widened to double times the Grid1f modulation widened to double
+template <class G> struct FieldModulated {
+	G grid;
+	GridDesc mod;
+	__device__ __forceinline__ V3 eval(const V3 &p) const {
+		const double m = (double)scalar_grid_eval(mod, p);
+		const V3 b = grid.eval(p);
+		return v3(b.x * m, b.y * m, b.z * m);
+	}
+	__device__ __forceinline__ float3 evalf(const V3 &p) const {
+		const float m = scalar_grid_evalf(mod, p);
+		const float3 b = grid.evalf(p);
+		return make_float3(b.x * m, b.y * m, b.z * m);
 	}
 };
 

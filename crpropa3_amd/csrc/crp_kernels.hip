@@ -51,13 +51,14 @@ static int fail(int code, const std::string &msg) {
 	} while (0)
 
 // ------------------------------------------------------------------ objects
-enum FieldKind { FK_NONE = 0, FK_UNIFORM = 1, FK_GRID = 2, FK_PLANEWAVE = 3 };
+enum FieldKind { FK_NONE = 0, FK_UNIFORM = 1, FK_GRID = 2, FK_PLANEWAVE = 3, FK_MODULATED = 4 };
 
 struct crp_field {
 	int kind;
 	int device;
 	FieldUniform uniform;
 	GridDesc grid;
+	GridDesc mod; // FK_MODULATED: the Grid1f modulation, one float per point
 	PlaneWaveDesc pw;
 	float *cells; // cell-major corner table (periodic grids), or nullptr
 	unsigned char *cells16; // FP16 corner table with brick scales (mixed path), or nullptr
@@ -592,8 +593,41 @@ int crp_field_create_grid3f_device(const float *d_values, int nx, int ny, int nz
 	return build_cells_default(f);
 }
 
+int crp_field_create_modulated_grid3f(const float *values, int nx, int ny, int nz, const double origin[3],
+                                      const double spacing[3], int reflective, int clip_volume, const float *modulation,
+                                      int mnx, int mny, int mnz, const double morigin[3], const double mspacing[3],
+                                      int mreflective, int mclip_volume, crp_field_t **out) {
+	if (!modulation || !out)
+		return fail(CRP_E_INVALID, "crp_field_create_modulated_grid3f: NULL argument");
+	crp_field *f = nullptr;
+	int rc = crp_field_create_grid3f(values, nx, ny, nz, origin, spacing, reflective, clip_volume, &f);
+	if (rc != CRP_OK)
+		return rc;
+	f->kind = FK_MODULATED;
+	rc = fill_grid_desc(f->mod, nullptr, mnx, mny, mnz, morigin, mspacing, mreflective, mclip_volume);
+	if (rc != CRP_OK) {
+		(void)crp_field_destroy(f);
+		return rc;
+	}
+	const size_t bytes = (size_t)mnx * mny * mnz * sizeof(float);
+	void *d = nullptr;
+	hipError_t e = hipMalloc(&d, bytes);
+	if (e == hipSuccess)
+		e = hipMemcpy(d, modulation, bytes, hipMemcpyHostToDevice);
+	if (e != hipSuccess) {
+		if (d)
+			(void)hipFree(d);
+		(void)crp_field_destroy(f);
+		return fail(CRP_E_HIP, std::string("crp_field_create_modulated_grid3f: ") + hipGetErrorString(e));
+	}
+	f->mod.values = (const float *)d;
+	f->owned[1] = d;
+	*out = f;
+	return CRP_OK;
+}
+
 int crp_field_set_grid_layout(crp_field_t *f, int layout) {
-	if (!f || f->kind != FK_GRID)
+	if (!f || (f->kind != FK_GRID && f->kind != FK_MODULATED))
 		return fail(CRP_E_INVALID, "crp_field_set_grid_layout: not a grid field");
 	if (layout == CRP_GRID_LAYOUT_POINTS) {
 		if (f->cells) {
@@ -682,6 +716,14 @@ int crp_field_eval_ex(const crp_field_t *f, const double *pos, size_t n, double 
 				hipLaunchKernelGGL(k_field_evalf<FieldGrid>, blocks, 256, 0, 0, FieldGrid{f->grid}, dpos, dB, n);
 			break;
 		case FK_PLANEWAVE: hipLaunchKernelGGL(k_field_evalf<FieldPlaneWave>, blocks, 256, 0, 0, FieldPlaneWave{f->pw}, dpos, dB, n); break;
+		case FK_MODULATED:
+			if (f->cells)
+				hipLaunchKernelGGL(k_field_evalf<FieldModulated<FieldGridCells>>, blocks, 256, 0, 0,
+				                   FieldModulated<FieldGridCells>{FieldGridCells{f->grid, f->cells}, f->mod}, dpos, dB, n);
+			else
+				hipLaunchKernelGGL(k_field_evalf<FieldModulated<FieldGrid>>, blocks, 256, 0, 0,
+				                   FieldModulated<FieldGrid>{FieldGrid{f->grid}, f->mod}, dpos, dB, n);
+			break;
 		default: hipLaunchKernelGGL(k_field_evalf<FieldNone>, blocks, 256, 0, 0, FieldNone{}, dpos, dB, n); break;
 		}
 		e = hipGetLastError();
@@ -799,6 +841,14 @@ int crp_field_eval(const crp_field_t *f, const double *pos, size_t n, double *B)
 				hipLaunchKernelGGL(k_field_eval<FieldGrid>, blocks, 256, 0, 0, FieldGrid{f->grid}, dpos, dB, n);
 			break;
 		case FK_PLANEWAVE: hipLaunchKernelGGL(k_field_eval<FieldPlaneWave>, blocks, 256, 0, 0, FieldPlaneWave{f->pw}, dpos, dB, n); break;
+		case FK_MODULATED:
+			if (f->cells)
+				hipLaunchKernelGGL(k_field_eval<FieldModulated<FieldGridCells>>, blocks, 256, 0, 0,
+				                   FieldModulated<FieldGridCells>{FieldGridCells{f->grid, f->cells}, f->mod}, dpos, dB, n);
+			else
+				hipLaunchKernelGGL(k_field_eval<FieldModulated<FieldGrid>>, blocks, 256, 0, 0,
+				                   FieldModulated<FieldGrid>{FieldGrid{f->grid}, f->mod}, dpos, dB, n);
+			break;
 		default: hipLaunchKernelGGL(k_field_eval<FieldNone>, blocks, 256, 0, 0, FieldNone{}, dpos, dB, n); break;
 		}
 		e = hipGetLastError();
@@ -1266,6 +1316,11 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 			              : dispatch_kind(FieldGrid{f->grid}, p, P, b, &st.launches);
 		break;
 	case FK_PLANEWAVE: rc = dispatch_kind(FieldPlaneWave{f->pw}, p, P, b, &st.launches); break;
+	case FK_MODULATED:
+		rc = f->cells ? dispatch_kind(FieldModulated<FieldGridCells>{FieldGridCells{f->grid, f->cells}, f->mod}, p, P, b,
+		                              &st.launches)
+		              : dispatch_kind(FieldModulated<FieldGrid>{FieldGrid{f->grid}, f->mod}, p, P, b, &st.launches);
+		break;
 	default: rc = dispatch_kind(FieldNone{}, p, P, b, &st.launches); break;
 	}
 	if (rc != CRP_OK)

@@ -9,6 +9,7 @@
 #include "crpropa/Referenced.h"
 #include "crpropa/Vector3.h"
 
+#include <cstdint>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -84,7 +85,12 @@ public:
 	/* Device field for this grid (created or refreshed on demand). */
 	crp_field *deviceField() const;
 	/* Marks host values as modified after writing through getGrid()/data(). */
-	void changed() { dirty = true; }
+	void changed() {
+		dirty = true;
+		++version;
+	}
+	/* increases with every change (device copies made from the grid compare it) */
+	uint64_t getVersion() const { return version; }
 
 private:
 	std::vector<Vector3f> grid;
@@ -95,6 +101,49 @@ private:
 	mutable crp_field *dev;
 	mutable bool dirty;
 	mutable std::mutex mtx;
+	uint64_t version = 0;
+};
+
+/* Grid1f = Grid<float> (include/crpropa/Grid.h:587): one float per point, the same geometry and
+ * continuation rules as Grid3f.  Used as the modulation of ModulatedMagneticFieldGrid, whose
+ * evaluation runs on the GPU. */
+class Grid1f : public Referenced {
+public:
+	Grid1f(Vector3d origin, size_t N, double spacing) : Grid1f(origin, N, N, N, Vector3d(spacing)) {}
+	Grid1f(Vector3d origin, size_t nx, size_t ny, size_t nz, double spacing)
+	    : Grid1f(origin, nx, ny, nz, Vector3d(spacing)) {}
+	Grid1f(Vector3d origin, size_t nx, size_t ny, size_t nz, Vector3d spacing)
+	    : grid(nx * ny * nz, 0.f), Nx(nx), Ny(ny), Nz(nz), origin(origin), spacing(spacing), clipVolume(false),
+	      reflective(false) {}
+	Grid1f(const GridProperties &p)
+	    : grid(p.Nx * p.Ny * p.Nz, 0.f), Nx(p.Nx), Ny(p.Ny), Nz(p.Nz), origin(p.origin), spacing(p.spacing),
+	      clipVolume(p.clipVolume), reflective(p.reflective) {}
+	void setOrigin(Vector3d o) { origin = o; ++version; }
+	void setSpacing(Vector3d s) { spacing = s; ++version; }
+	void setReflective(bool b) { reflective = b; ++version; }
+	void setClipVolume(bool b) { clipVolume = b; ++version; }
+	Vector3d getOrigin() const { return origin; }
+	Vector3d getSpacing() const { return spacing; }
+	size_t getNx() const { return Nx; }
+	size_t getNy() const { return Ny; }
+	size_t getNz() const { return Nz; }
+	bool isReflective() const { return reflective; }
+	bool getClipVolume() const { return clipVolume; }
+	float &get(size_t ix, size_t iy, size_t iz) { ++version; return grid[(ix * Ny + iy) * Nz + iz]; }
+	const float &get(size_t ix, size_t iy, size_t iz) const { return grid[(ix * Ny + iy) * Nz + iz]; }
+	float getValue(size_t ix, size_t iy, size_t iz) const { return grid[(ix * Ny + iy) * Nz + iz]; }
+	void setValue(size_t ix, size_t iy, size_t iz, float v) { ++version; grid[(ix * Ny + iy) * Nz + iz] = v; }
+	std::vector<float> &getGrid() { ++version; return grid; }
+	const std::vector<float> &getGrid() const { return grid; }
+	void changed() { ++version; }
+	uint64_t getVersion() const { return version; }
+
+private:
+	std::vector<float> grid;
+	size_t Nx, Ny, Nz;
+	Vector3d origin, spacing;
+	bool clipVolume, reflective;
+	uint64_t version = 0;
 };
 
 } // namespace crpropa

@@ -1,10 +1,12 @@
-// MagneticFieldGrid (src/magneticField/MagneticFieldGrid.cpp:5-19): a field
-// defined by a Grid3f, interpolated on the GPU.
+// MagneticFieldGrid and ModulatedMagneticFieldGrid (src/magneticField/MagneticFieldGrid.cpp:5-55):
+// fields defined by a Grid3f (times a Grid1f modulation), interpolated on the GPU.
 #ifndef CRPROPA_AMD_MAGNETICFIELDGRID_H
 #define CRPROPA_AMD_MAGNETICFIELDGRID_H
 
 #include "crpropa/Grid.h"
 #include "crpropa/magneticField/MagneticField.h"
+
+#include <mutex>
 
 namespace crpropa {
 
@@ -18,6 +20,30 @@ public:
 
 private:
 	ref_ptr<Grid3f> grid;
+};
+
+class ModulatedMagneticFieldGrid : public MagneticField {
+public:
+	ModulatedMagneticFieldGrid() {}
+	/* makes grid periodic and modGrid reflective, as the reference's constructor does */
+	ModulatedMagneticFieldGrid(ref_ptr<Grid3f> grid, ref_ptr<Grid1f> modGrid);
+	~ModulatedMagneticFieldGrid();
+	void setGrid(ref_ptr<Grid3f> g) { grid = g; }
+	void setModulationGrid(ref_ptr<Grid1f> g) { modGrid = g; }
+	ref_ptr<Grid3f> getGrid() { return grid; }
+	ref_ptr<Grid1f> getModulationGrid() { return modGrid; }
+	void setReflective(bool gridReflective, bool modGridReflective);
+	Vector3d getField(const Vector3d &position) const;
+	crp_field *deviceField() const;
+
+private:
+	ref_ptr<Grid3f> grid;
+	ref_ptr<Grid1f> modGrid;
+	mutable crp_field *dev = nullptr;
+	mutable const Grid3f *devGrid = nullptr;
+	mutable const Grid1f *devMod = nullptr;
+	mutable uint64_t devGridVersion = 0, devModVersion = 0;
+	mutable std::mutex mtx;
 };
 
 } // namespace crpropa

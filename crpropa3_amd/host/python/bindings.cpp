@@ -278,6 +278,47 @@ PYBIND11_MODULE(_crpropa, m) {
 	    .def("setGrid", &MagneticFieldGrid::setGrid)
 	    .def("getGrid", &MagneticFieldGrid::getGrid);
 
+	py::class_<Grid1f, ref_ptr<Grid1f>>(m, "Grid1f")
+	    .def(py::init<Vector3d, size_t, double>(), py::arg("origin"), py::arg("N"), py::arg("spacing"))
+	    .def(py::init<Vector3d, size_t, size_t, size_t, double>())
+	    .def(py::init<Vector3d, size_t, size_t, size_t, Vector3d>())
+	    .def(py::init<const GridProperties &>())
+	    .def("setOrigin", &Grid1f::setOrigin)
+	    .def("setSpacing", &Grid1f::setSpacing)
+	    .def("setReflective", &Grid1f::setReflective)
+	    .def("setClipVolume", &Grid1f::setClipVolume)
+	    .def("getOrigin", &Grid1f::getOrigin)
+	    .def("getSpacing", &Grid1f::getSpacing)
+	    .def("getNx", &Grid1f::getNx)
+	    .def("getNy", &Grid1f::getNy)
+	    .def("getNz", &Grid1f::getNz)
+	    .def("isReflective", &Grid1f::isReflective)
+	    .def("getClipVolume", &Grid1f::getClipVolume)
+	    .def("getValue", &Grid1f::getValue)
+	    .def("setValue", &Grid1f::setValue)
+	    .def("array",
+	         [](py::object self) {
+		         Grid1f &g = self.cast<Grid1f &>();
+		         g.changed();
+		         return py::array_t<float>(py::buffer_info(g.getGrid().data(), sizeof(float),
+		                                                   py::format_descriptor<float>::format(), 3,
+		                                                   {g.getNx(), g.getNy(), g.getNz()},
+		                                                   {sizeof(float) * g.getNy() * g.getNz(),
+		                                                    sizeof(float) * g.getNz(), sizeof(float)}),
+		                                   self);
+	         },
+	         "Writable numpy view (Nx, Ny, Nz) of the grid values.");
+
+	py::class_<ModulatedMagneticFieldGrid, MagneticField, ref_ptr<ModulatedMagneticFieldGrid>>(
+	    m, "ModulatedMagneticFieldGrid")
+	    .def(py::init<>())
+	    .def(py::init<ref_ptr<Grid3f>, ref_ptr<Grid1f>>())
+	    .def("setGrid", &ModulatedMagneticFieldGrid::setGrid)
+	    .def("setModulationGrid", &ModulatedMagneticFieldGrid::setModulationGrid)
+	    .def("getGrid", &ModulatedMagneticFieldGrid::getGrid)
+	    .def("getModulationGrid", &ModulatedMagneticFieldGrid::getModulationGrid)
+	    .def("setReflective", &ModulatedMagneticFieldGrid::setReflective);
+
 	// grid tools (include/crpropa/GridTools.h)
 	m.def("scaleGrid", &scaleGrid, py::arg("grid"), py::arg("a"));
 	m.def("meanFieldVector", &meanFieldVector);

@@ -203,3 +203,64 @@ Vector3d PlaneWaveTurbulence::getField(const Vector3d &pos) const {
 }
 
 } // namespace crpropa
+
+namespace crpropa {
+
+// ------------------------------------------------------------------ ModulatedMagneticFieldGrid
+ModulatedMagneticFieldGrid::ModulatedMagneticFieldGrid(ref_ptr<Grid3f> g, ref_ptr<Grid1f> m) {
+	g->setReflective(false);
+	m->setReflective(true);
+	setGrid(g);
+	setModulationGrid(m);
+}
+
+ModulatedMagneticFieldGrid::~ModulatedMagneticFieldGrid() {
+	if (dev)
+		crp_field_destroy(dev);
+}
+
+void ModulatedMagneticFieldGrid::setReflective(bool gridReflective, bool modGridReflective) {
+	grid->setReflective(gridReflective);
+	modGrid->setReflective(modGridReflective);
+}
+
+crp_field *ModulatedMagneticFieldGrid::deviceField() const {
+	if (!grid.valid() || !modGrid.valid())
+		return nullptr;
+	std::lock_guard<std::mutex> lock(mtx);
+	if (dev && devGrid == grid.get() && devMod == modGrid.get() && devGridVersion == grid->getVersion() &&
+	    devModVersion == modGrid->getVersion())
+		return dev;
+	if (dev) {
+		crp_field_destroy(dev);
+		dev = nullptr;
+	}
+	const Grid3f &g = *grid;
+	const Grid1f &m = *modGrid;
+	const Vector3d go = g.getOrigin(), gs = g.getSpacing(), mo = m.getOrigin(), ms = m.getSpacing();
+	const double o3[3] = {go.x, go.y, go.z}, s3[3] = {gs.x, gs.y, gs.z};
+	const double o1[3] = {mo.x, mo.y, mo.z}, s1[3] = {ms.x, ms.y, ms.z};
+	crp_field *f = nullptr;
+	throwIf(crp_field_create_modulated_grid3f(
+	    reinterpret_cast<const float *>(g.getGrid().data()), (int)g.getNx(), (int)g.getNy(), (int)g.getNz(), o3, s3,
+	    g.isReflective() ? 1 : 0, g.getClipVolume() ? 1 : 0, m.getGrid().data(), (int)m.getNx(), (int)m.getNy(),
+	    (int)m.getNz(), o1, s1, m.isReflective() ? 1 : 0, m.getClipVolume() ? 1 : 0, &f));
+	dev = f;
+	devGrid = grid.get();
+	devMod = modGrid.get();
+	devGridVersion = grid->getVersion();
+	devModVersion = modGrid->getVersion();
+	return dev;
+}
+
+Vector3d ModulatedMagneticFieldGrid::getField(const Vector3d &position) const {
+	const double p[3] = {position.x, position.y, position.z};
+	double B[3];
+	crp_field *d = deviceField();
+	if (!d)
+		throw std::runtime_error("ModulatedMagneticFieldGrid: grid or modulation grid not set");
+	throwIf(crp_field_eval(d, p, 1, B));
+	return Vector3d(B[0], B[1], B[2]);
+}
+
+} // namespace crpropa

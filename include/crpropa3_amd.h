@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define CRP_ABI_VERSION 6
+#define CRP_ABI_VERSION 7
 
 enum crp_status {
 	CRP_OK = 0,
@@ -76,6 +76,16 @@ int crp_field_create_grid3f(const float *values, int nx, int ny, int nz, const d
 /* Same, with values already on the device (borrowed: the caller keeps them alive). */
 int crp_field_create_grid3f_device(const float *d_values, int nx, int ny, int nz, const double origin[3],
                                    const double spacing[3], int reflective, int clip_volume, crp_field_t **out);
+
+/* ModulatedMagneticFieldGrid (src/magneticField/MagneticFieldGrid.cpp:20-55): the Grid3f above times a
+ * Grid1f `modulation` (one float per point, index (ix * mny + iy) * mnz + iz) with its own origin,
+ * spacing, size, reflective and clipVolume flags; B = Grid3f(pos) * Grid1f(pos), each interpolated as
+ * the reference does.  (The reference's constructor makes the Grid3f periodic and the Grid1f
+ * reflective; the flags here are taken as given.) */
+int crp_field_create_modulated_grid3f(const float *values, int nx, int ny, int nz, const double origin[3],
+                                      const double spacing[3], int reflective, int clip_volume, const float *modulation,
+                                      int mnx, int mny, int mnz, const double morigin[3], const double mspacing[3],
+                                      int mreflective, int mclip_volume, crp_field_t **out);
 
 /* PlaneWaveTurbulence::getField -- src/magneticField/turbulentField/PlaneWaveTurbulence.cpp:231-240.
  * The mode table as built by the PlaneWaveTurbulence constructor (:99-178): k[nm], Ak[nm],

@@ -253,6 +253,59 @@ static void orc_grid_eval1(const orc_field *f, const double p[3], double B[3]) {
 	B[2] = b[2];
 }
 
+/* Grid<float>::interpolate + trilinearInterpolate (include/crpropa/Grid.h:318-335,540-582) with
+ * T = float: each corner value times the double weights in double, and every b += term
+ * rounds the double sum to float. */
+static float orc_grid1_eval(const orc_field *f, const double p[3]) {
+	if (f->mclip) {
+		for (int a = 0; a < 3; a++) {
+			const double n = a == 0 ? f->mnx : (a == 1 ? f->mny : f->mnz);
+			if (!((p[a] >= f->morigin[a]) && (p[a] <= f->morigin[a] + n * f->mspacing[a])))
+				return 0.f;
+		}
+	}
+	double r[3];
+	for (int a = 0; a < 3; a++) {
+		double go = f->morigin[a] + f->mspacing[a] / 2;
+		r[a] = (p[a] - go) / f->mspacing[a];
+	}
+	int n[3] = {f->mnx, f->mny, f->mnz};
+	int lo[3], hi[3];
+	double f0[3];
+	for (int a = 0; a < 3; a++) {
+		if (f->mreflective) {
+			double res;
+			orc_reflective_clamp(r[a], n[a], &lo[a], &hi[a], &res);
+			f0[a] = res - floor(res);
+		} else {
+			orc_periodic_clamp(r[a], n[a], &lo[a], &hi[a]);
+			f0[a] = r[a] - floor(r[a]);
+		}
+	}
+	const double f1[3] = {1 - f0[0], 1 - f0[1], 1 - f0[2]};
+	const int X[8] = {lo[0], hi[0], lo[0], lo[0], hi[0], lo[0], hi[0], hi[0]};
+	const int Y[8] = {lo[1], lo[1], hi[1], lo[1], lo[1], hi[1], hi[1], hi[1]};
+	const int Z[8] = {lo[2], lo[2], lo[2], hi[2], hi[2], hi[2], lo[2], hi[2]};
+	const double WX[8] = {f1[0], f0[0], f1[0], f1[0], f0[0], f1[0], f0[0], f0[0]};
+	const double WY[8] = {f1[1], f1[1], f0[1], f1[1], f1[1], f0[1], f0[1], f0[1]};
+	const double WZ[8] = {f1[2], f1[2], f1[2], f0[2], f0[2], f0[2], f1[2], f0[2]};
+	float b = 0.f;
+	for (int c = 0; c < 8; c++) {
+		const float v = f->mod[((size_t)X[c] * f->mny + Y[c]) * f->mnz + Z[c]];
+		b = (float)((double)b + (((double)v * WX[c]) * WY[c]) * WZ[c]);
+	}
+	return b;
+}
+
+/* ModulatedMagneticFieldGrid::getField (src/magneticField/MagneticFieldGrid.cpp:51-55):
+ * the Grid3f value widened to double, times the modulation widened to double */
+static void orc_modulated_eval1(const orc_field *f, const double p[3], double B[3]) {
+	const float m = orc_grid1_eval(f, p);
+	orc_grid_eval1(f, p, B);
+	for (int a = 0; a < 3; a++)
+		B[a] = B[a] * (double)m;
+}
+
 /* PlaneWaveTurbulence::getField (default build), src/magneticField/turbulentField/PlaneWaveTurbulence.cpp:231-240 */
 static void orc_planewave_eval1(const orc_field *f, const double p[3], double B[3], int libm) {
 	double bx = 0, by = 0, bz = 0;
@@ -278,6 +331,7 @@ static inline void orc_field_eval1(const orc_field *f, const double p[3], double
 	case ORC_FIELD_UNIFORM: B[0] = f->B0[0]; B[1] = f->B0[1]; B[2] = f->B0[2]; break;
 	case ORC_FIELD_GRID: orc_grid_eval1(f, p, B); break;
 	case ORC_FIELD_PLANEWAVE: orc_planewave_eval1(f, p, B, libm); break;
+	case ORC_FIELD_MODULATED: orc_modulated_eval1(f, p, B); break;
 	default: B[0] = B[1] = B[2] = 0; break;
 	}
 }

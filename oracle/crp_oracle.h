@@ -24,7 +24,7 @@ extern "C" {
 #endif
 
 enum { ORC_LIBM_GLIBC = 0, ORC_LIBM_CR = 1 };
-enum { ORC_FIELD_NONE = 0, ORC_FIELD_UNIFORM = 1, ORC_FIELD_GRID = 2, ORC_FIELD_PLANEWAVE = 3 };
+enum { ORC_FIELD_NONE = 0, ORC_FIELD_UNIFORM = 1, ORC_FIELD_GRID = 2, ORC_FIELD_PLANEWAVE = 3, ORC_FIELD_MODULATED = 4 };
 enum { ORC_PROP_BP = 0, ORC_PROP_CK = 1 };
 /* modules after the propagator, for orc_propagator.order */
 enum { ORC_MOD_BOX = 1, ORC_MOD_SPHERE = 2, ORC_MOD_MAXLEN = 3, ORC_MOD_OBSERVER = 4 };
@@ -40,6 +40,11 @@ typedef struct {
 	/* plane waves: Nm modes, xi/kappa AoS 3*Nm */
 	int nm;
 	const double *k, *Ak, *xi, *kappa, *beta;
+	/* ModulatedMagneticFieldGrid: the Grid3f above times a Grid1f (one float per point) */
+	const float *mod;
+	int mnx, mny, mnz;
+	double morigin[3], mspacing[3];
+	int mreflective, mclip;
 } orc_field;
 
 typedef struct {

@@ -450,6 +450,75 @@ static void gen_trilinear() {
 	printf("  trilinear16 M=%zu\n", M);
 }
 
+// ModulatedMagneticFieldGrid (src/magneticField/MagneticFieldGrid.cpp:20-55): a Grid3f times a
+// Grid1f of other origin, spacing and size; the constructor makes the Grid3f periodic and the
+// Grid1f reflective
+static ref_ptr<Grid1f> make_mod_grid(int nx, int ny, int nz, double spacing, Vector3d origin, uint32_t seed) {
+	ref_ptr<Grid1f> m = new Grid1f(origin, nx, ny, nz, spacing);
+	Random rng(seed);
+	for (int ix = 0; ix < nx; ix++)
+		for (int iy = 0; iy < ny; iy++)
+			for (int iz = 0; iz < nz; iz++)
+				m->get(ix, iy, iz) = (float)rng.randUniform(0.2, 1.8);
+	return m;
+}
+
+static void gen_modulated() {
+	const int n = 16;
+	const double sp = 50 * kpc;
+	Vector3d origin(-3.1 * kpc, 7.25 * kpc, 0.0);
+	ref_ptr<Grid3f> g = make_noise_grid(n, sp, origin, 1234, nG);
+	Vector3d mo(11 * kpc, -40 * kpc, 5 * kpc);
+	const double msp = 70 * kpc;
+	ref_ptr<Grid1f> m = make_mod_grid(12, 10, 14, msp, mo, 4321);
+	Random rng(98);
+	const size_t M = 4096;
+	const double L = n * sp;
+	std::vector<double> pos(M * 3);
+	for (size_t i = 0; i < M; i++)
+		for (int a = 0; a < 3; a++) {
+			double x;
+			switch (i % 4) {
+			case 0: x = origin.data[a] + rng.randUniform(0, L); break;
+			case 1: x = origin.data[a] + rng.randUniform(-3 * L, 4 * L); break;
+			case 2: x = mo.data[a] + msp * (int)rng.randUniform(-10, 30); break; // on modulation cell faces
+			default: x = rng.randUniform(-1e3, 1e3) * L; break;
+			}
+			pos[3 * i + a] = x;
+		}
+	std::string dir = casedir("modulated16");
+	save_grid(dir, g);
+	std::vector<float> mv(12 * 10 * 14);
+	for (int ix = 0; ix < 12; ix++)
+		for (int iy = 0; iy < 10; iy++)
+			for (int iz = 0; iz < 14; iz++)
+				mv[(ix * 10 + iy) * 14 + iz] = m->get(ix, iy, iz);
+	save_f32(dir, "mod", mv, {12, 10, 14});
+	save_f64(dir, "pos", pos, {M, 3});
+	// as constructed (grid periodic, modulation reflective); both periodic; modulation clipped
+	ModulatedMagneticFieldGrid field(g, m);
+	const char *names[3] = {"B_default", "B_periodic", "B_modclip"};
+	for (int mode = 0; mode < 3; mode++) {
+		if (mode == 1)
+			field.setReflective(false, false);
+		if (mode == 2) {
+			field.setReflective(false, true);
+			m->setClipVolume(true);
+		}
+		std::vector<double> B(M * 3);
+		for (size_t i = 0; i < M; i++) {
+			Vector3d b = field.getField(Vector3d(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]));
+			B[3 * i] = b.x; B[3 * i + 1] = b.y; B[3 * i + 2] = b.z;
+		}
+		save_f64(dir, names[mode], B, {M, 3});
+	}
+	m->setClipVolume(false);
+	save_meta(dir, "{\"n\": 16, \"spacing\": " + fmt(sp) + ", \"origin\": [" + fmt(origin.x) + ", " + fmt(origin.y) +
+	                   ", " + fmt(origin.z) + "], \"mod_shape\": [12, 10, 14], \"mod_spacing\": " + fmt(msp) +
+	                   ", \"mod_origin\": [" + fmt(mo.x) + ", " + fmt(mo.y) + ", " + fmt(mo.z) + "]}");
+	printf("  modulated16 M=%zu\n", M);
+}
+
 static void gen_planewave_case(const std::string &name, const TurbulenceSpectrum &spec, int Nm, int seed,
                                double posScale, const std::string &meta) {
 	PlaneWaveTurbulence pw(spec, Nm, seed);
@@ -546,6 +615,7 @@ int main(int argc, char **argv) {
 	}
 
 	gen_trilinear();
+	gen_modulated();
 	gen_gridtools();
 
 	// Plane waves: the C3 mode table (Simple spectrum), the reference test's
@@ -582,6 +652,40 @@ int main(int argc, char **argv) {
 	std::vector<int> species = {nucleusId(60, 1), nucleusId(60, 2), nucleusId(60, 7), nucleusId(60, 14), nucleusId(60, 26)};
 	std::vector<double> one = {1.0};
 	std::vector<double> wts = {0.5, 0.2, 0.15, 0.1, 0.05};
+
+	// ---- ModulatedMagneticFieldGrid: grid32 times a reflective 9x11x10 modulation grid
+	{
+		Vector3d mo(-0.3 * Mpc, 0.1 * Mpc, 0.25 * Mpc);
+		const double msp = 0.2 * Mpc;
+		ref_ptr<Grid1f> mg = make_mod_grid(9, 11, 10, msp, mo, 777);
+		ref_ptr<MagneticField> fmod = new ModulatedMagneticFieldGrid(g32, mg);
+		std::vector<float> mv(9 * 11 * 10);
+		for (int ix = 0; ix < 9; ix++)
+			for (int iy = 0; iy < 11; iy++)
+				for (int iz = 0; iz < 10; iz++)
+					mv[(ix * 11 + iy) * 10 + iz] = mg->get(ix, iy, iz);
+		const std::string modmeta = "\"field\": \"grid32_modulated\", \"mod_shape\": [9, 11, 10], \"mod_spacing\": " +
+		                            fmt(msp) + ", \"mod_origin\": [" + fmt(mo.x) + ", " + fmt(mo.y) + ", " + fmt(mo.z) +
+		                            "], \"mod_reflective\": true";
+		{
+			std::vector<Particle> ps = make_particles(128, 58, 1 * EeV, 100 * EeV, centre, proton, one);
+			PropagationBP bp(fmod, 0.1 * Mpc);
+			run_traj("bp_fixed_grid32_modulated", bp, nullptr, ps, 300, {1, 10, 100, 300},
+			         "{\"propagator\": \"BP\", \"fixedStep\": " + fmt(0.1 * Mpc) + ", " + modmeta +
+			             ", \"particle_seed\": 58}");
+			save_f32(casedir("bp_fixed_grid32_modulated"), "mod", mv, {9, 11, 10});
+		}
+		{
+			std::vector<Particle> ps = make_particles(128, 59, 1 * EeV, 300 * EeV, centre, species, wts);
+			PropagationCK ck(fmod, 1e-4, 1 * kpc, 1 * Mpc);
+			MaximumTrajectoryLength ml(12 * Mpc);
+			run_traj("ck_grid32_modulated", ck, &ml, ps, 3000, {1, 10, 100, 1000, 3000},
+			         "{\"propagator\": \"CK\", \"tolerance\": 1e-4, \"minStep\": " + fmt(kpc) +
+			             ", \"maxStep\": " + fmt(Mpc) + ", \"maxTrajectoryLength\": " + fmt(12 * Mpc) + ", " + modmeta +
+			             ", \"particle_seed\": 59}");
+			save_f32(casedir("ck_grid32_modulated"), "mod", mv, {9, 11, 10});
+		}
+	}
 
 	{
 		std::vector<Particle> ps = make_particles(256, 42, 1 * EeV, 100 * EeV, centre, proton, one);

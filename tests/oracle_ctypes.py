@@ -14,7 +14,7 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 LIBM_GLIBC = 0
 LIBM_CR = 1
-FIELD_NONE, FIELD_UNIFORM, FIELD_GRID, FIELD_PLANEWAVE = 0, 1, 2, 3
+FIELD_NONE, FIELD_UNIFORM, FIELD_GRID, FIELD_PLANEWAVE, FIELD_MODULATED = 0, 1, 2, 3, 4
 PROP_BP, PROP_CK = 0, 1
 MOD_BOX, MOD_SPHERE, MOD_MAXLEN, MOD_OBSERVER = 1, 2, 3, 4  # orc_propagator.order
 
@@ -48,6 +48,8 @@ class Field(C.Structure):
         ("reflective", C.c_int), ("clip", C.c_int),
         ("nm", C.c_int),
         ("k", _dp), ("Ak", _dp), ("xi", _dp), ("kappa", _dp), ("beta", _dp),
+        ("mod", _fp), ("mnx", C.c_int), ("mny", C.c_int), ("mnz", C.c_int),
+        ("morigin", C.c_double * 3), ("mspacing", C.c_double * 3), ("mreflective", C.c_int), ("mclip", C.c_int),
     ]
 
 
@@ -150,6 +152,24 @@ class OracleField:
             f.s.spacing[a] = sp[a]
         f.s.reflective = int(reflective)
         f.s.clip = int(clip)
+        return f
+
+    @staticmethod
+    def modulated(values, origin, spacing, mod, morigin, mspacing, reflective=False, mreflective=True,
+                  clip=False, mclip=False):
+        """ModulatedMagneticFieldGrid: Grid3f values times a Grid1f (one float per point)."""
+        f = OracleField.grid(values, origin, spacing, reflective, clip)
+        m = np.ascontiguousarray(mod, dtype=np.float32)
+        f._keep.append(m)
+        f.s.type = FIELD_MODULATED
+        f.s.mod = ptr(m, _fp)
+        f.s.mnx, f.s.mny, f.s.mnz = m.shape[:3]
+        msp = np.broadcast_to(np.asarray(mspacing, dtype=np.float64), (3,))
+        for a in range(3):
+            f.s.morigin[a] = morigin[a]
+            f.s.mspacing[a] = msp[a]
+        f.s.mreflective = int(mreflective)
+        f.s.mclip = int(mclip)
         return f
 
     @staticmethod
@@ -341,6 +361,11 @@ def field_for_case(case):
         gm = g["meta"]["grid"]
         return OracleField.grid(g["grid"], gm["origin"], gm["spacing"], reflective=meta.get("reflective", False),
                                 clip=meta.get("clip", False))
+    if f == "grid32_modulated":
+        g = load_case("grid32")
+        gm = g["meta"]["grid"]
+        return OracleField.modulated(g["grid"], gm["origin"], gm["spacing"], case["mod"], meta["mod_origin"],
+                                     meta["mod_spacing"], mreflective=meta["mod_reflective"])
     if isinstance(f, dict) and f["type"] == "planewave":
         return OracleField.planewave(planewave_tables(f))
     if isinstance(f, dict) and f["type"] == "uniform":

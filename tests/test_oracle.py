@@ -343,3 +343,30 @@ def test_oracle_module_order_is_pinned(name, wrong):
     ref = c["det_next"]
     ok = ~np.isnan(ref)
     assert not bits_equal(st.snap[:, :, 8][ok], ref[ok])
+
+
+@pytest.mark.parametrize("mode", ["default", "periodic", "modclip"])
+def test_oracle_modulated_field_bit_exact_vs_reference(mode):
+    """ModulatedMagneticFieldGrid::getField (src/magneticField/MagneticFieldGrid.cpp:51-55): the Grid1f
+    interpolation (double weights, float accumulation) times the Grid3f value, as constructed (modulation
+    reflective), both periodic, and with the modulation clipped."""
+    c = O.load_case("modulated16")
+    m = c["meta"]
+    f = O.OracleField.modulated(c["grid"], m["origin"], m["spacing"], c["mod"], m["mod_origin"], m["mod_spacing"],
+                                mreflective=(mode != "periodic"), mclip=(mode == "modclip"))
+    B = f.eval(c["pos"], O.LIBM_GLIBC)
+    assert bits_equal(B, c["B_" + mode])
+    assert np.any(B != 0) and not np.array_equal(c["B_default"], c["B_periodic"])
+
+
+@pytest.mark.parametrize("name", ["bp_fixed_grid32_modulated", "ck_grid32_modulated"])
+def test_oracle_modulated_trajectories_bit_exact_vs_reference(name):
+    c = O.load_case(name)
+    m = c["meta"]
+    st = O.OracleState(c["init_pos"], c["init_dir"], c["energy"], c["charge"])
+    done = 0
+    for si, s in enumerate(c["snap_steps"]):
+        O.propagate(O.field_for_case(c), steps=int(s) - done, max_len=m.get("maxTrajectoryLength", 0.0),
+                    libm=O.LIBM_GLIBC, state=st, **O.propagator_args(m))
+        done = int(s)
+        assert bits_equal(st.pos, c["pos"][si]) and bits_equal(st.dir, c["dir"][si]), (name, s)
