@@ -29,9 +29,11 @@
 
 #if defined(__HIPCC__)
 #define CRP_MATH_FN static inline __device__
+#define CRP_MATH_COLD static __device__ __attribute__((noinline)) /* rare paths: kept out of the callers' registers */
 #define CRP_TABLE static __device__ __constant__ const
 #else
 #define CRP_MATH_FN static inline
+#define CRP_MATH_COLD static
 #define CRP_TABLE static const
 #endif
 
@@ -233,6 +235,74 @@ CRP_MATH_FN double crp_pow_cr(double x, double y) {
 	return v * crp_pow2i(s1) * crp_pow2i(s2);
 }
 
+/* ---------------------------------------------------------- large arguments */
+/*
+ * |x| = M 2^e reduced modulo pi/2 for any finite |x| >= 2^23 pi/2 (Payne and Hanek): only
+ * the bits of 2/pi from 2^-(e-1) on matter modulo 4, so a 224-bit window of the table
+ * is multiplied by the 53-bit integer M exactly (32-bit limbs).  The two bits above
+ * the binary point are the quadrant, the 128 below it the fraction f; with f >= 1/2
+ * the quadrant advances and f - 1 is used, so r = f pi/2 has |r| <= pi/4.  The
+ * window carries at least 190 fraction bits, so f is exact to 2^-128.
+ */
+CRP_MATH_COLD crp_dd crp_reduce_large(double ax, int *quadrant) {
+	const uint64_t u = crp_as_u64(ax);
+	const int e = (int)(u >> 52) - 1075; /* ax = M 2^e */
+	const uint64_t M = (u & 0x000fffffffffffffULL) | (1ULL << 52);
+	int k0 = e - 1; /* first bit of 2/pi that matters modulo 4 */
+	if (k0 < 1)
+		k0 = 1;
+	const int w0 = (k0 - 1) / 32, sh = (k0 - 1) % 32;
+	uint32_t W[7]; /* little endian: W[0] least significant */
+	for (int i = 0; i < 7; i++)
+		W[i] = crp_two_over_pi_bits[w0 + 6 - i];
+	if (sh)
+		W[6] &= 0xffffffffu >> sh;
+	const uint32_t Mw[2] = {(uint32_t)M, (uint32_t)(M >> 32)};
+	uint32_t R[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+	for (int i = 0; i < 7; i++) {
+		uint64_t carry = 0;
+		for (int j = 0; j < 2; j++) {
+			const uint64_t t = (uint64_t)W[i] * Mw[j] + R[i + j] + carry;
+			R[i + j] = (uint32_t)t;
+			carry = t >> 32;
+		}
+		for (int k = i + 2; carry && k < 9; k++) {
+			const uint64_t t = (uint64_t)R[k] + carry;
+			R[k] = (uint32_t)t;
+			carry = t >> 32;
+		}
+	}
+	const int F = 32 * (w0 + 7) - e; /* fraction bits of R */
+	uint64_t hi = 0, lo = 0;
+	{
+		/* hi = bits F-64 .. F-1, lo = bits F-128 .. F-65 */
+		const int p1 = F - 64, p2 = F - 128;
+		uint64_t t1 = 0, t2 = 0;
+		for (int b = 0; b < 64; b++) {
+			const int q1 = p1 + b, q2 = p2 + b;
+			t1 |= (uint64_t)((R[q1 / 32] >> (q1 % 32)) & 1u) << b;
+			t2 |= (uint64_t)((R[q2 / 32] >> (q2 % 32)) & 1u) << b;
+		}
+		hi = t1;
+		lo = t2;
+	}
+	int n = (int)((R[F / 32] >> (F % 32)) & 1u) | (int)(((R[(F + 1) / 32] >> ((F + 1) % 32)) & 1u) << 1);
+	double sign = 1.0;
+	if (hi >> 63) { /* f >= 1/2: use f - 1 = -(2^128 - (hi, lo)) / 2^128 */
+		n = (n + 1) & 3;
+		lo = ~lo + 1;
+		hi = ~hi + (lo == 0 ? 1 : 0);
+		sign = -1.0;
+	}
+	/* f as a double-double: the top 53 bits, then the next 64 rounded */
+	const double f1 = (double)(hi >> 11) * 0x1p-53;
+	const double f2 = (double)(((hi & 0x7ffULL) << 53) | (lo >> 11)) * 0x1p-117;
+	crp_dd f = crp_fast_two_sum(f1, f2);
+	crp_dd r = crp_dd_mul(f, crp_mkdd(CRP_PIO2_DD_HI, CRP_PIO2_DD_LO));
+	*quadrant = n;
+	return crp_mkdd(sign * r.hi, sign * r.lo);
+}
+
 /* ---------------------------------------------------------- cos */
 /* sin and cos of a double-double a with 0 <= a <= pi/4 + tiny */
 CRP_MATH_FN void crp_sincos_dd_reduced(crp_dd a, crp_dd *s_out, crp_dd *c_out) {
@@ -273,12 +343,17 @@ CRP_MATH_FN void crp_sincos_dd_reduced(crp_dd a, crp_dd *s_out, crp_dd *c_out) {
  * products q*P_k stay exact (two_prod) but cancellation is only tracked to
  * double-double precision.
  */
-CRP_MATH_FN double crp_cos_accurate(double x) {
+CRP_MATH_COLD double crp_cos_accurate_shift(double x, int shift);
+CRP_MATH_FN double crp_cos_accurate(double x) { return crp_cos_accurate_shift(x, 0); }
+
+/* cos(x) for shift 0, sin(x) for shift 3: sin(q pi/2 + r) = cos((q - 1) pi/2 + r) */
+CRP_MATH_COLD double crp_cos_accurate_shift(double x, int shift) {
+	int flip = 0;
 	double ax = x < 0 ? -x : x;
 	if (!(ax < 1.0 / 0.0))
 		return x - x; /* NaN for inf and NaN */
 	if (ax < 0x1p-27)
-		return 1.0;
+		return shift ? x : 1.0; /* sin x = x, cos x = 1 after rounding */
 	double q = crp_rint(x * CRP_TWO_OVER_PI);
 	crp_dd r;
 	if (q == 0.0) {
@@ -290,12 +365,12 @@ CRP_MATH_FN double crp_cos_accurate(double x) {
 		r = crp_dd_add_d(r, -q * CRP_PIO2_4);
 		r = crp_dd_add_d(r, -q * CRP_PIO2_5);
 	} else {
-		crp_dd p1 = crp_two_prod(q, CRP_PIO2_1);
-		r = crp_two_sum(x - p1.hi, -p1.lo);
-		r = crp_dd_add(r, crp_dd_neg(crp_two_prod(q, CRP_PIO2_2)));
-		r = crp_dd_add(r, crp_dd_neg(crp_two_prod(q, CRP_PIO2_3)));
-		r = crp_dd_add(r, crp_dd_neg(crp_two_prod(q, CRP_PIO2_4)));
-		r = crp_dd_add_d(r, -q * CRP_PIO2_5);
+		/* cos is even and sin odd: reduce |x|, flip the sine's sign at the end */
+		int n0;
+		r = crp_reduce_large(ax, &n0);
+		q = (double)n0;
+		if (x < 0 && shift)
+			flip = 1;
 	}
 	int neg = r.hi < 0;
 	crp_dd a = neg ? crp_dd_neg(r) : r;
@@ -304,7 +379,7 @@ CRP_MATH_FN double crp_cos_accurate(double x) {
 	if (neg)
 		s = crp_dd_neg(s);
 	long long qi = (long long)q;
-	int n = (int)(qi & 3);
+	int n = (int)((qi + shift) & 3);
 	crp_dd res;
 	switch (n) {
 	case 0: res = c; break;
@@ -312,7 +387,7 @@ CRP_MATH_FN double crp_cos_accurate(double x) {
 	case 2: res = crp_dd_neg(c); break;
 	default: res = s; break;
 	}
-	return res.hi + res.lo;
+	return flip ? -(res.hi + res.lo) : res.hi + res.lo;
 }
 
 /*
@@ -330,7 +405,7 @@ CRP_MATH_FN double crp_cos_accurate(double x) {
  * to y_h (half an ulp, a quarter below a power of two): about 1 argument in
  * 4000 is left to the accurate phase.  Returns 0 when it does not decide.
  */
-CRP_MATH_FN int crp_cos_quick(double x, double *out) {
+CRP_MATH_FN int crp_cos_quick_shift(double x, int shift, double *out) {
 	double ax = x < 0 ? -x : x;
 	if (!(ax < 1.0 / 0.0) || ax < 0x1p-27)
 		return 0;
@@ -361,7 +436,7 @@ CRP_MATH_FN int crp_cos_quick(double x, double *out) {
 	double dcross = dl * dh;   /* d_l sin d_h, to 2^-113 */
 	double dsin = dl - dl * hz; /* d_l cos d_h, to 2^-120 */
 	long long qi = (long long)q;
-	int n = (int)(qi & 3);
+	int n = (int)((qi + shift) & 3);
 	crp_dd y;
 	if (j == 0) {
 		if (n & 1) { /* sin a = d_h + (sdm + d_l cos d_h) */
@@ -406,12 +481,29 @@ CRP_MATH_FN int crp_cos_quick(double x, double *out) {
 	return 1;
 }
 
+CRP_MATH_FN int crp_cos_quick(double x, double *out) { return crp_cos_quick_shift(x, 0, out); }
+
 /* cos(x), correctly rounded */
 CRP_MATH_FN double crp_cos_cr(double x) {
 	double v;
-	if (crp_cos_quick(x, &v))
+	if (crp_cos_quick_shift(x, 0, &v))
 		return v;
-	return crp_cos_accurate(x);
+	return crp_cos_accurate_shift(x, 0);
+}
+
+/* sin(x), correctly rounded (the same reduction, the quadrant shifted by one) */
+CRP_MATH_FN double crp_sin_cr(double x) {
+	double v;
+	if (crp_cos_quick_shift(x, 3, &v))
+		return v;
+	return crp_cos_accurate_shift(x, 3);
+}
+
+/* log(x) for finite x > 0: the double-double logarithm (absolute error about 2^-104)
+ * rounded once; the same bits on the host and the device */
+CRP_MATH_FN double crp_log_dd_rn(double x) {
+	crp_dd l = crp_log_dd(x);
+	return l.hi + l.lo;
 }
 
 #endif

@@ -11,6 +11,7 @@
  */
 #include "crp_oracle.h"
 #include "crp_crmath.h"
+#include "crp_philox.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -955,5 +956,21 @@ void orc_cos_phases(const double *x, size_t n, double *quick, int32_t *decided, 
 	for (size_t i = 0; i < n; i++) {
 		decided[i] = crp_cos_quick(x[i], &quick[i]);
 		accurate[i] = crp_cos_accurate(x[i]);
+	}
+}
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+	crp_philox_ctr c;
+	for (int i = 0; i < 4; i++)
+		c.v[i] = ctr[i];
+	crp_philox_ctr o = crp_philox4x32_10(c, key[0], key[1]);
+	for (int i = 0; i < 4; i++)
+		out[i] = o.v[i];
+}
+
+void orc_sin_log_batch(const double *x, size_t n, double *sin_out, double *log_out) {
+	for (size_t i = 0; i < n; i++) {
+		sin_out[i] = crp_sin_cr(x[i]);
+		log_out[i] = x[i] > 0 ? crp_log_dd_rn(x[i]) : 0.0;
 	}
 }

@@ -109,6 +109,11 @@ enum { ORC_TURB_SIMPLE = 0, ORC_TURB_GENERAL = 1, ORC_TURB_GENERAL_POWERLAW = 2 
 size_t orc_turbulence_coeffs(int n, double spacing, double lmin, double lmax, int kind, double alpha, double lbendover,
                              double sindex, double qindex, uint32_t seed, float *bk);
 
+/* Philox4x32-10 of one counter and key (known-answer tests); out[4] */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* correctly rounded sin and the rounded double-double log of this package, per element */
+void orc_sin_log_batch(const double *x, size_t n, double *sin_out, double *log_out);
+
 /* the two phases of the correctly rounded cos separately: quick[i] valid where decided[i] != 0 */
 void orc_cos_phases(const double *x, size_t n, double *quick, int32_t *decided, double *accurate);
 

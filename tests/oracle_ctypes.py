@@ -91,6 +91,8 @@ def lib():
         L.orc_noise_grid.argtypes = [C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_double, _fp]
         L.orc_rng_randint_batch.argtypes = [C.c_uint32, C.c_size_t, _P(C.c_uint32)]
         L.orc_cos_phases.argtypes = [_dp, C.c_size_t, _dp, _P(C.c_int32), _dp]
+        L.orc_philox4x32_10.argtypes = [_P(C.c_uint32), _P(C.c_uint32), _P(C.c_uint32)]
+        L.orc_sin_log_batch.argtypes = [_dp, C.c_size_t, _dp, _dp]
         L.orc_turbulence_coeffs.restype = C.c_size_t
         L.orc_turbulence_coeffs.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int] + \
             [C.c_double] * 4 + [C.c_uint32, _fp]
@@ -436,3 +438,19 @@ def cos_phases(x):
     d = np.zeros(x.shape, dtype=np.int32)
     lib().orc_cos_phases(ptr(x), x.size, ptr(q), d.ctypes.data_as(_P(C.c_int32)), ptr(a))
     return q, d.astype(bool), a
+
+
+def philox4x32_10(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def sin_log(x):
+    """(correctly rounded sin, rounded double-double log) of this package's math header."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    s, lg = np.zeros_like(x), np.zeros_like(x)
+    lib().orc_sin_log_batch(ptr(x), x.size, ptr(s), ptr(lg))
+    return s, lg

@@ -79,3 +79,43 @@ def test_cos_quick_phase_agrees_with_accurate_phase():
     q, dec, a = O.cos_phases(x)
     assert dec.mean() > 0.9
     assert np.array_equal(q[dec].view(np.uint64), a[dec].view(np.uint64))
+
+
+def test_philox_known_answers():
+    """Philox4x32-10 known-answer vectors (Salmon et al., SC'11, the Random123 distribution)."""
+    assert O.philox4x32_10([0, 0, 0, 0], [0, 0]) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert O.philox4x32_10([0xffffffff] * 4, [0xffffffff] * 2) == [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert O.philox4x32_10([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0]) == \
+        [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def test_sin_correctly_rounded_and_log():
+    """sin via the cos reduction with the quadrant shifted: equal to 70-digit decimal arithmetic
+    rounded once; the double-double log rounded once equals the correctly rounded log on the same
+    samples (a hard case for log would need 2^-104 more accuracy; none occurs here)."""
+    from decimal import Decimal, getcontext
+    import math
+    getcontext().prec = 70
+    rng = np.random.default_rng(8)
+    x = np.concatenate([rng.uniform(-10, 10, 1500), rng.uniform(-1e6, 1e6, 500), np.ldexp(rng.uniform(-1, 1, 200), -30),
+                        np.floor(rng.uniform(0, 1e4, 300)) * (np.pi / 2) + rng.uniform(-1e-7, 1e-7, 300)])
+    s, _ = O.sin_log(x)
+    pi = Decimal("3.141592653589793238462643383279502884197169399375105820974944592307816406")
+
+    def dsin(v):
+        v = Decimal(float(v))
+        k = (v / (2 * pi)).to_integral_value()
+        v = v - k * 2 * pi
+        term, tot, n = v, v, 1
+        while abs(term) > Decimal(10) ** -68:
+            term = -term * v * v / ((2 * n) * (2 * n + 1))
+            tot += term
+            n += 1
+        return float(tot)
+    ref = np.array([dsin(v) for v in x])
+    assert np.array_equal(s.view(np.uint64), ref.view(np.uint64))
+    assert np.mean(s == np.sin(x)) > 0.99  # glibc agrees almost everywhere
+    y = np.exp(rng.uniform(-700, 700, 3000))
+    _, lg = O.sin_log(y)
+    refl = np.array([float(Decimal(float(v)).ln()) for v in y])
+    assert np.array_equal(lg.view(np.uint64), refl.view(np.uint64))
