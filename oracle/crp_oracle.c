@@ -974,3 +974,232 @@ void orc_sin_log_batch(const double *x, size_t n, double *sin_out, double *log_o
 		log_out[i] = x[i] > 0 ? crp_log_dd_rn(x[i]) : 0.0;
 	}
 }
+
+/* ------------------------------------------------------------------ DiffusionSDE */
+static const double KPC = 1e3 * (648000 / M_PI * 149597870700.0);
+
+typedef struct {
+	int kind;
+	orc_rng *mt;
+	crp_sde_rng ph;
+} orc_sde_rng;
+
+/* Random::rand / randExc / randDblExc (src/Random.cpp:80-98); one 53-bit uniform in [0, 1) each on
+ * the Philox stream */
+static double sde_rand(orc_sde_rng *r) { return r->kind == ORC_SDE_MT ? orc_rng_rand(r->mt) : crp_sde_uniform(&r->ph); }
+static double sde_randexc(orc_sde_rng *r) {
+	return r->kind == ORC_SDE_MT ? (double)orc_rng_randint(r->mt) * (1.0 / 4294967296.0) : crp_sde_uniform(&r->ph);
+}
+static double sde_randdblexc(orc_sde_rng *r) {
+	return r->kind == ORC_SDE_MT ? ((double)orc_rng_randint(r->mt) + 0.5) * (1.0 / 4294967296.0)
+	                             : crp_sde_uniform(&r->ph);
+}
+static double sde_log(int k, double x) { return k == ORC_SDE_MT ? log(x) : crp_log_dd_rn(x); }
+static double sde_cos(int k, double x) { return k == ORC_SDE_MT ? cos(x) : crp_cos_cr(x); }
+static double sde_sin(int k, double x) { return k == ORC_SDE_MT ? sin(x) : crp_sin_cr(x); }
+static double sde_pow(int k, double x, double y) { return k == ORC_SDE_MT ? pow(x, y) : crp_pow_cr(x, y); }
+
+/* Random::randNorm(0, 1) (src/Random.cpp:111-115) */
+static double sde_randnorm(orc_sde_rng *r) {
+	double rr = sqrt(-2.0 * sde_log(r->kind, 1.0 - sde_randdblexc(r))) * 1.0;
+	double phi = 2.0 * 3.14159265358979323846264338328 * sde_randexc(r);
+	return 0.0 + rr * sde_cos(r->kind, phi);
+}
+
+/* Random::randVector (src/Random.cpp:141-146) */
+static void sde_randvector(orc_sde_rng *r, double v[3]) {
+	double z = -1.0 + (1.0 - -1.0) * sde_rand(r);
+	double t = -1.0 * M_PI + (M_PI - -1.0 * M_PI) * sde_rand(r);
+	double rr = sqrt(1 - z * z);
+	v[0] = rr * sde_cos(r->kind, t);
+	v[1] = rr * sde_sin(r->kind, t);
+	v[2] = z;
+}
+
+static void sde_cross(const double a[3], const double b[3], double o[3]) { /* Vector3.h:268-271 */
+	o[0] = a[1] * b[2] - b[1] * a[2];
+	o[1] = a[2] * b[0] - b[2] * a[0];
+	o[2] = a[0] * b[1] - b[0] * a[1];
+}
+static double sde_r(const double v[3]) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+static void sde_unit(const double v[3], double o[3]) { /* getUnitVector: v / |v| */
+	double r = sde_r(v);
+	for (int a = 0; a < 3; a++)
+		o[a] = v[a] / r;
+}
+
+/* Random::randConeVector(mean, pi/2) (src/Random.cpp:148-162) and Vector3::getRotated
+ * (include/crpropa/Vector3.h:213-226).  On the reference's stream theta = acos(w) and cos/sin of
+ * theta are taken; on the Philox stream cos theta = w, sin theta = sqrt(1 - w^2) directly. */
+static void sde_randcone(orc_sde_rng *r, const double mean[3], double o[3]) {
+	const int k = r->kind;
+	double w = 1 + (sde_cos(k, M_PI / 2.) - 1) * sde_rand(r);
+	double c, s;
+	if (k == ORC_SDE_MT) {
+		double theta = acos(w);
+		c = cos(theta);
+		s = sin(theta);
+	} else {
+		c = w;
+		s = sqrt((1 - w) * (1 + w));
+	}
+	double rv[3], u[3];
+	sde_randvector(r, rv);
+	sde_cross(mean, rv, u);
+	double ur = sde_r(u);
+	if (ur != 0.)
+		for (int a = 0; a < 3; a++)
+			u[a] = u[a] / ur;
+	double Rx[3] = {c + u[0] * u[0] * (1 - c), u[0] * u[1] * (1 - c) - u[2] * s, u[0] * u[2] * (1 - c) + u[1] * s};
+	double Ry[3] = {u[1] * u[0] * (1 - c) + u[2] * s, c + u[1] * u[1] * (1 - c), u[1] * u[2] * (1 - c) - u[0] * s};
+	double Rz[3] = {u[2] * u[0] * (1 - c) - u[1] * s, u[2] * u[1] * (1 - c) + u[0] * s, c + u[2] * u[2] * (1 - c)};
+	o[0] = mean[0] * Rx[0] + mean[1] * Rx[1] + mean[2] * Rx[2];
+	o[1] = mean[0] * Ry[0] + mean[1] * Ry[1] + mean[2] * Ry[2];
+	o[2] = mean[0] * Rz[0] + mean[1] * Rz[1] + mean[2] * Rz[2];
+}
+
+/* DiffusionSDE::tryStep (src/module/DiffusionSDE.cpp:233-254): Cash-Karp along the unit field direction */
+static void sde_trystep(const orc_field *f, const double in[3], double out[3], double err[3], double t, int libm) {
+	double k[6][3];
+	for (int a = 0; a < 3; a++)
+		out[a] = in[a];
+	for (int i = 0; i < 6; i++) {
+		double y[3] = {in[0], in[1], in[2]};
+		for (int j = 0; j < i; j++)
+			for (int a = 0; a < 3; a++)
+				y[a] += (k[j][a] * CK_A[i * 6 + j]) * t;
+		double B[3], ub[3];
+		orc_field_eval1(f, y, B, libm);
+		sde_unit(B, ub);
+		const double bi = CK_B[i], ei = CK_B[i] - CK_BS[i];
+		for (int a = 0; a < 3; a++) {
+			k[i][a] = ub[a] * C_LIGHT;
+			out[a] += (k[i][a] * bi) * t;
+			err[a] += ((k[i][a] * ei) * t) / KPC;
+		}
+	}
+}
+
+/* DiffusionSDE::process (src/module/DiffusionSDE.cpp:48-209) */
+static void orc_sde_process(const orc_field *f, const orc_sde *p, orc_state *s, size_t i, orc_sde_rng *r) {
+	const int k = r->kind, libm = k == ORC_SDE_MT ? ORC_LIBM_GLIBC : ORC_LIBM_CR;
+	double *pos = s->pos + 3 * i, *dir = s->dir + 3 * i;
+	if (s->prev_pos)
+		for (int a = 0; a < 3; a++) {
+			s->prev_pos[3 * i + a] = pos[a];
+			s->prev_dir[3 * i + a] = dir[a];
+		}
+	const double h = orc_clip(s->next_step[i], p->minStep, p->maxStep) / C_LIGHT;
+	double lin[3] = {0, 0, 0};
+	if (p->advection)
+		for (int a = 0; a < 3; a++)
+			lin[a] = 0 + p->adv[a] * h;
+	if (s->charge[i] == 0) { /* rectilinear, with the advection drift */
+		for (int a = 0; a < 3; a++)
+			pos[a] = (pos[a] + lin[a]) + (dir[a] * h) * C_LIGHT;
+		s->current_step[i] = h * C_LIGHT;
+		s->trajectory_length[i] += h * C_LIGHT;
+		s->time[i] += (h * C_LIGHT) / C_LIGHT;
+		s->next_step[i] = p->maxStep;
+		return;
+	}
+	const double rig = s->energy[i] / s->charge[i];
+	/* calculateBTensor (:258-266) */
+	const double D = p->scale * 6.1e24 * sde_pow(k, fabs(rig) / 4.0e9, p->alpha);
+	const double B0 = sde_pow(k, 2 * D, 0.5), B4 = sde_pow(k, 2 * p->epsilon * D, 0.5), B8 = B4;
+	double eta[3];
+	for (int a = 0; a < 3; a++)
+		eta[a] = sde_randnorm(r);
+	const double TStep = B0 * eta[0], NStep = B4 * eta[1], BStep = B8 * eta[2];
+	double in[3] = {pos[0], pos[1], pos[2]}, out[3], err[3];
+	double propTime = TStep * sqrt(h) / C_LIGHT;
+	int counter = 0;
+	double rr;
+	do {
+		err[0] = err[1] = err[2] = 0;
+		sde_trystep(f, in, out, err, propTime, libm);
+		rr = sde_r(err) / p->tolerance;
+		propTime *= 0.5;
+		counter += 1;
+	} while (rr > 1 && fabs(propTime) >= p->minStep / C_LIGHT);
+	const double stepNumber = ldexp(1.0, counter - 1); /* pow(2, counter - 1) */
+	const double allowedTime = TStep * sqrt(h) / C_LIGHT / stepNumber;
+	double start[3] = {in[0], in[1], in[2]};
+	err[0] = err[1] = err[2] = 0;
+	for (double j = 0; j < stepNumber; j++) {
+		sde_trystep(f, start, out, err, allowedTime, libm);
+		start[0] = out[0];
+		start[1] = out[1];
+		start[2] = out[2];
+	}
+	double d[3] = {out[0] - in[0], out[1] - in[1], out[2] - in[2]}, T[3];
+	sde_unit(d, T);
+	const double tTest = sde_r(T);
+	if (tTest != tTest) { /* no field: advection only, or rectilinear */
+		for (int a = 0; a < 3; a++)
+			pos[a] = p->advection ? in[a] + lin[a] : in[a] + (dir[a] * h) * C_LIGHT;
+		s->current_step[i] = h * C_LIGHT;
+		s->trajectory_length[i] += h * C_LIGHT;
+		s->time[i] += (h * C_LIGHT) / C_LIGHT;
+		s->next_step[i] = orc_clip(5 * h * C_LIGHT, p->minStep, p->maxStep);
+		return;
+	}
+	double N[3] = {0, 0, 0}, Bv[3], tmp[3];
+	while (sde_r(N) == 0.) {
+		double rv[3];
+		sde_randvector(r, rv);
+		sde_cross(T, rv, N);
+	}
+	sde_unit(N, tmp);
+	N[0] = tmp[0]; N[1] = tmp[1]; N[2] = tmp[2];
+	sde_cross(T, N, tmp);
+	sde_unit(tmp, Bv);
+	double PO[3];
+	const double sh = sqrt(h);
+	for (int a = 0; a < 3; a++)
+		PO[a] = (out[a] + lin[a]) + (N[a] * NStep + Bv[a] * BStep) * sh;
+	if (isnan(sde_r(PO))) {
+		s->active[i] = 0;
+		return;
+	}
+	double D2[3];
+	sde_randcone(r, T, D2);
+	double dr = sde_r(D2);
+	for (int a = 0; a < 3; a++) {
+		pos[a] = PO[a];
+		dir[a] = D2[a] / dr; /* ParticleState::setDirection */
+	}
+	s->current_step[i] = h * C_LIGHT;
+	s->trajectory_length[i] += h * C_LIGHT;
+	s->time[i] += (h * C_LIGHT) / C_LIGHT;
+	s->next_step[i] = stepNumber > 1 ? h * sde_pow(k, stepNumber, -2.) * C_LIGHT : 4 * h * C_LIGHT;
+}
+
+long long orc_sde_propagate(const orc_field *f, const orc_sde *p, orc_state *s, size_t n, long long maxSteps,
+                            uint32_t *sde_step) {
+	orc_rng mt;
+	if (p->rng == ORC_SDE_MT)
+		orc_rng_seed(&mt, p->seed);
+	long long total = 0;
+	for (size_t i = 0; i < n; i++) {
+		for (long long k = 0; k < maxSteps && s->active[i]; k++) {
+			orc_sde_rng r;
+			r.kind = p->rng;
+			r.mt = &mt;
+			if (p->rng == ORC_SDE_PHILOX)
+				crp_sde_rng_init(&r.ph, p->seed, (uint32_t)i, sde_step ? sde_step[i] : (uint32_t)k);
+			orc_sde_process(f, p, s, i, &r);
+			if (sde_step)
+				sde_step[i]++;
+			if (p->maxTrajectoryLength > 0) { /* MaximumTrajectoryLength::process */
+				const double len = s->trajectory_length[i];
+				if (len >= p->maxTrajectoryLength)
+					s->active[i] = 0;
+				else
+					s->next_step[i] = orc_min(s->next_step[i], p->maxTrajectoryLength - len);
+			}
+			total++;
+		}
+	}
+	return total;
+}

@@ -38,6 +38,8 @@
 #include "crpropa/module/Observer.h"
 #include "crpropa/module/PropagationBP.h"
 #include "crpropa/module/PropagationCK.h"
+#include "crpropa/module/DiffusionSDE.h"
+#include "crpropa/advectionField/AdvectionField.h"
 
 #include <cstdio>
 #include <cstdint>
@@ -519,6 +521,92 @@ static void gen_modulated() {
 	printf("  modulated16 M=%zu\n", M);
 }
 
+// ---------------------------------------------------------------- DiffusionSDE
+// Serial run with Random::instance() seeded once (omp thread 0): particle i takes all its steps
+// before particle i+1, [DiffusionSDE, MaximumTrajectoryLength] per step.
+static void run_sde_case(const std::string &name, const DiffusionSDE &sde, double maxLen,
+                         const std::vector<Particle> &ps, size_t maxSteps, uint32_t seed, const std::string &meta) {
+	std::string dir = casedir(name);
+	save_particles(dir, ps);
+	const size_t N = ps.size();
+	MaximumTrajectoryLength ml(maxLen);
+	Random::instance().seed(seed);
+	std::vector<double> pos(N * 3), d(N * 3), len(N), nxt(N), tim(N);
+	std::vector<int32_t> act(N), nsteps(N);
+	for (size_t i = 0; i < N; i++) {
+		ParticleState st(ps[i].id, ps[i].E, ps[i].pos, ps[i].dir);
+		ref_ptr<Candidate> c = new Candidate(st);
+		size_t step = 0;
+		while (step < maxSteps && c->isActive()) {
+			sde.process(c);
+			if (maxLen > 0)
+				ml.process(c);
+			step++;
+		}
+		Vector3d p = c->current.getPosition(), u = c->current.getDirection();
+		pos[3 * i] = p.x; pos[3 * i + 1] = p.y; pos[3 * i + 2] = p.z;
+		d[3 * i] = u.x; d[3 * i + 1] = u.y; d[3 * i + 2] = u.z;
+		len[i] = c->getTrajectoryLength();
+		nxt[i] = c->getNextStep();
+		tim[i] = c->getTime();
+		act[i] = c->isActive() ? 1 : 0;
+		nsteps[i] = (int32_t)step;
+	}
+	save_f64(dir, "final_pos", pos, {N, 3});
+	save_f64(dir, "final_dir", d, {N, 3});
+	save_f64(dir, "final_len", len, {N});
+	save_f64(dir, "final_next", nxt, {N});
+	save_f64(dir, "final_time", tim, {N});
+	save_i32(dir, "final_active", act, {N});
+	save_i32(dir, "nsteps", nsteps, {N});
+	save_meta(dir, meta);
+	printf("  %-28s N=%zu steps<=%zu\n", name.c_str(), N, maxSteps);
+}
+
+static void gen_sde() {
+	std::vector<int> species = {nucleusId(60, 1), nucleusId(60, 2), nucleusId(60, 26), nucleusId(60, 0)};
+	std::vector<double> wts = {0.5, 0.2, 0.2, 0.1};
+	{
+		// uniform 1 nG field along z, epsilon 0.1
+		ref_ptr<MagneticField> f = new UniformMagneticField(Vector3d(0, 0, 1 * nG));
+		DiffusionSDE sde(f, 1e-4, 1 * pc, 10 * kpc, 0.1);
+		std::vector<Particle> ps = make_particles(96, 70, 1 * TeV, 1 * PeV, Vector3d(0.), species, wts);
+		run_sde_case("sde_uniform", sde, 50 * kpc, ps, 400, 4711,
+		             "{\"field\": {\"type\": \"uniform\", \"B\": [0, 0, " + fmt(nG) + "]}, \"tolerance\": 1e-4, "
+		             "\"minStep\": " + fmt(pc) + ", \"maxStep\": " + fmt(10 * kpc) + ", \"epsilon\": 0.1, \"alpha\": " +
+		                 fmt(1. / 3.) + ", \"scale\": 1, \"maxTrajectoryLength\": " + fmt(50 * kpc) + ", \"seed\": 4711}");
+	}
+	{
+		// 32^3 noise grid with 20 pc spacing: the field line curves within a step, the step search halves
+		const double sp = 20 * pc;
+		ref_ptr<Grid3f> g = make_noise_grid(32, sp, Vector3d(0.), 909, 1 * muG);
+		std::string dir = casedir("sde_grid32");
+		save_grid(dir, g);
+		ref_ptr<MagneticField> f = new MagneticFieldGrid(g);
+		DiffusionSDE sde(f, 1e-6, 0.1 * pc, 100 * pc, 0.3);
+		sde.setAlpha(0.5);
+		sde.setScale(0.2);
+		Vector3d c(16 * sp);
+		std::vector<Particle> ps = make_particles(64, 71, 1 * TeV, 10 * PeV, c, species, wts);
+		run_sde_case("sde_grid32", sde, 2 * kpc, ps, 600, 4712,
+		             "{\"grid_spacing\": " + fmt(sp) + ", \"tolerance\": 1e-6, \"minStep\": " + fmt(0.1 * pc) +
+		                 ", \"maxStep\": " + fmt(100 * pc) + ", \"epsilon\": 0.3, \"alpha\": 0.5, \"scale\": 0.2, "
+		                 "\"maxTrajectoryLength\": " + fmt(2 * kpc) + ", \"seed\": 4712}");
+	}
+	{
+		// advection with a field, and neutral particles drifting
+		ref_ptr<MagneticField> f = new UniformMagneticField(Vector3d(0.3 * nG, 0, 1 * nG));
+		ref_ptr<AdvectionField> adv = new UniformAdvectionField(Vector3d(1e5, -2e5, 3e4));
+		DiffusionSDE sde(f, adv, 1e-4, 1 * pc, 10 * kpc, 0.2);
+		std::vector<Particle> ps = make_particles(64, 72, 1 * TeV, 1 * PeV, Vector3d(0.), species, wts);
+		run_sde_case("sde_advection", sde, 20 * kpc, ps, 300, 4713,
+		             "{\"field\": {\"type\": \"uniform\", \"B\": [" + fmt(0.3 * nG) + ", 0, " + fmt(nG) +
+		                 "]}, \"advection\": [100000.0, -200000.0, 30000.0], \"tolerance\": 1e-4, \"minStep\": " +
+		                 fmt(pc) + ", \"maxStep\": " + fmt(10 * kpc) + ", \"epsilon\": 0.2, \"alpha\": " + fmt(1. / 3.) +
+		                 ", \"scale\": 1, \"maxTrajectoryLength\": " + fmt(20 * kpc) + ", \"seed\": 4713}");
+	}
+}
+
 static void gen_planewave_case(const std::string &name, const TurbulenceSpectrum &spec, int Nm, int seed,
                                double posScale, const std::string &meta) {
 	PlaneWaveTurbulence pw(spec, Nm, seed);
@@ -616,6 +704,7 @@ int main(int argc, char **argv) {
 
 	gen_trilinear();
 	gen_modulated();
+	gen_sde();
 	gen_gridtools();
 
 	// Plane waves: the C3 mode table (Simple spectrum), the reference test's

@@ -53,6 +53,15 @@ class Field(C.Structure):
     ]
 
 
+class SDE(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("minStep", C.c_double), ("maxStep", C.c_double), ("epsilon", C.c_double),
+                ("alpha", C.c_double), ("scale", C.c_double), ("advection", C.c_int), ("adv", C.c_double * 3),
+                ("maxTrajectoryLength", C.c_double), ("rng", C.c_int), ("seed", C.c_uint32)]
+
+
+SDE_MT, SDE_PHILOX = 0, 1
+
+
 class Propagator(C.Structure):
     _fields_ = [("type", C.c_int), ("tolerance", C.c_double), ("minStep", C.c_double),
                 ("maxStep", C.c_double), ("maxTrajectoryLength", C.c_double),
@@ -93,6 +102,8 @@ def lib():
         L.orc_cos_phases.argtypes = [_dp, C.c_size_t, _dp, _P(C.c_int32), _dp]
         L.orc_philox4x32_10.argtypes = [_P(C.c_uint32), _P(C.c_uint32), _P(C.c_uint32)]
         L.orc_sin_log_batch.argtypes = [_dp, C.c_size_t, _dp, _dp]
+        L.orc_sde_propagate.restype = C.c_longlong
+        L.orc_sde_propagate.argtypes = [_P(Field), _P(SDE), _P(State), C.c_size_t, C.c_longlong, _P(C.c_uint32)]
         L.orc_turbulence_coeffs.restype = C.c_size_t
         L.orc_turbulence_coeffs.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int] + \
             [C.c_double] * 4 + [C.c_uint32, _fp]
@@ -454,3 +465,34 @@ def sin_log(x):
     s, lg = np.zeros_like(x), np.zeros_like(x)
     lib().orc_sin_log_batch(ptr(x), x.size, ptr(s), ptr(lg))
     return s, lg
+
+
+def sde_propagate(field, state, steps, tolerance, min_step, max_step, epsilon=0.1, alpha=1 / 3., scale=1.0,
+                  advection=None, max_len=0.0, rng=SDE_MT, seed=1, sde_step=None):
+    """DiffusionSDE steps (src/module/DiffusionSDE.cpp) on the oracle state; rng SDE_MT follows the
+    reference's stream (serial, particle by particle), SDE_PHILOX the device's."""
+    p = SDE(tolerance, min_step, max_step, epsilon, alpha, scale, 0, (C.c_double * 3)(0, 0, 0), max_len, rng, seed)
+    if advection is not None:
+        p.advection = 1
+        for a in range(3):
+            p.adv[a] = advection[a]
+    s = state.struct()
+    cnt = None
+    if sde_step is not None:
+        cnt = sde_step.ctypes.data_as(_P(C.c_uint32))
+    return lib().orc_sde_propagate(C.byref(field.s), C.byref(p), C.byref(s), state.n, steps, cnt)
+
+
+def sde_args(meta):
+    kw = dict(tolerance=meta["tolerance"], min_step=meta["minStep"], max_step=meta["maxStep"], epsilon=meta["epsilon"],
+              alpha=meta["alpha"], scale=meta["scale"], max_len=meta.get("maxTrajectoryLength", 0.0), seed=meta["seed"])
+    if "advection" in meta:
+        kw["advection"] = meta["advection"]
+    return kw
+
+
+def sde_field(case):
+    m = case["meta"]
+    if "grid_spacing" in m:
+        return OracleField.grid(case["grid"], [0, 0, 0], m["grid_spacing"])
+    return OracleField.uniform(m["field"]["B"])
