@@ -36,7 +36,7 @@ class HostState(C.Structure):
     _fields_ = [("pos", _dp), ("dir", _dp), ("energy", _dp), ("charge", _dp), ("current_step", _dp),
                 ("next_step", _dp), ("trajectory_length", _dp), ("time", _dp), ("active", _ip),
                 ("prev_pos", _dp), ("prev_dir", _dp), ("detection_index", _ip), ("source_pos", _dp),
-                ("created_pos", _dp)]
+                ("created_pos", _dp), ("sde_step", C.POINTER(C.c_uint32)), ("sde_stream", C.POINTER(C.c_uint64))]
 
 
 class LaunchConfig(C.Structure):
@@ -81,6 +81,9 @@ def lib():
         L.crp_field_eval.argtypes = [_vp, _dp, C.c_size_t, _dp]
         for nm in ("crp_propagator_create_bp", "crp_propagator_create_ck"):
             getattr(L, nm).argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.POINTER(_vp)]
+        L.crp_propagator_create_sde.argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(_vp)]
+        L.crp_propagator_set_sde.argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.c_uint32]
+        L.crp_propagator_set_sde_advection.argtypes = [_vp, _dp]
         L.crp_propagator_set_max_trajectory_length.argtypes = [_vp, C.c_double]
         L.crp_propagator_set_precision.argtypes = [_vp, C.c_int]
         L.crp_propagator_destroy.argtypes = [_vp]
@@ -260,12 +263,19 @@ MODULE_SPHERE_DEFAULT_MARGIN = 0.1 * (1e3 * (648000 / math.pi * 149597870700.0))
 
 
 class Propagator:
-    """PropagationBP / PropagationCK with a fused MaximumTrajectoryLength."""
+    """PropagationBP / PropagationCK / DiffusionSDE ("BP", "CK", "SDE") with a fused
+    MaximumTrajectoryLength.  For "SDE": epsilon, alpha, scale, seed and advection (a uniform
+    advection velocity, or None)."""
 
-    def __init__(self, kind, field, tolerance, min_step, max_step, max_length=0.0, precision=PRECISION_FP64):
+    def __init__(self, kind, field, tolerance, min_step, max_step, max_length=0.0, precision=PRECISION_FP64,
+                 epsilon=0.1, alpha=1 / 3., scale=1.0, seed=0, advection=None):
         h = _vp()
-        fn = lib().crp_propagator_create_bp if kind == "BP" else lib().crp_propagator_create_ck
-        check(fn(field.handle if field is not None else None, tolerance, min_step, max_step, C.byref(h)))
+        fh = field.handle if field is not None else None
+        if kind == "SDE":
+            check(lib().crp_propagator_create_sde(fh, tolerance, min_step, max_step, epsilon, C.byref(h)))
+        else:
+            fn = lib().crp_propagator_create_bp if kind == "BP" else lib().crp_propagator_create_ck
+            check(fn(fh, tolerance, min_step, max_step, C.byref(h)))
         self.handle = h
         self.field = field
         self.kind = kind
@@ -273,6 +283,23 @@ class Propagator:
             check(lib().crp_propagator_set_max_trajectory_length(h, max_length))
         if precision != PRECISION_FP64:
             check(lib().crp_propagator_set_precision(h, precision))
+        if kind == "SDE":
+            self.set_sde(alpha, scale, epsilon, seed)
+            if advection is not None:
+                self.set_sde_advection(advection)
+
+    def set_sde(self, alpha, scale, epsilon, seed):
+        check(lib().crp_propagator_set_sde(self.handle, alpha, scale, epsilon, int(seed) & 0xFFFFFFFF))
+        return self
+
+    def set_sde_advection(self, v):
+        """UniformAdvectionField(v) of the DiffusionSDE; None removes it."""
+        if v is None:
+            check(lib().crp_propagator_set_sde_advection(self.handle, None))
+        else:
+            a = _f64(v, (3,))
+            check(lib().crp_propagator_set_sde_advection(self.handle, _p(a)))
+        return self
 
     def __del__(self):
         if getattr(self, "handle", None) and _lib is not None:
@@ -353,7 +380,8 @@ class Batch:
         return n.value
 
     def upload(self, pos, dir, energy, charge, current_step=None, next_step=None, trajectory_length=None,
-               time=None, active=None, detection_index=None, source_pos=None, created_pos=None):
+               time=None, active=None, detection_index=None, source_pos=None, created_pos=None, sde_step=None,
+               sde_stream=None):
         pos = _f64(pos, (-1, 3))
         n = pos.shape[0]
         arrs = {
@@ -377,6 +405,14 @@ class Batch:
             if a is not None:
                 extra[name] = _f64(a, (-1, 3))
                 setattr(s, name, _p(extra[name]))
+        if sde_step is not None:
+            extra["sde_step"] = np.ascontiguousarray(sde_step, dtype=np.uint32)
+            s.sde_step = extra["sde_step"].ctypes.data_as(C.POINTER(C.c_uint32))
+        if sde_stream is not None:
+            extra["sde_stream"] = np.ascontiguousarray(sde_stream, dtype=np.uint64)
+            s.sde_stream = extra["sde_stream"].ctypes.data_as(C.POINTER(C.c_uint64))
+        for a in extra.values():
+            assert a.shape[0] == n
         check(lib().crp_batch_upload(self.handle, n, C.byref(s)))
 
     def download(self):
@@ -387,14 +423,17 @@ class Batch:
                "source_pos": np.zeros((n, 3)), "created_pos": np.zeros((n, 3))}
         act = np.zeros(n, dtype=np.int32)
         det = np.zeros(n, dtype=np.int32)
+        sst = np.zeros(n, dtype=np.uint32)
         s = HostState()
         for k, a in out.items():
             setattr(s, k, _p(a))
         s.active = _p(act, _ip)
         s.detection_index = _p(det, _ip)
+        s.sde_step = sst.ctypes.data_as(C.POINTER(C.c_uint32))
         check(lib().crp_batch_download(self.handle, C.byref(s)))
         out["active"] = act
         out["detection_index"] = det
+        out["sde_step"] = sst
         return out
 
     def detections(self, ntimes):

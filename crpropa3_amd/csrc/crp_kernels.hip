@@ -25,6 +25,7 @@
 
 #include "crp_device.hpp"
 #include "crp_propagate.hpp"
+#include "crp_sde.hpp"
 #include "crp_adjoint.hpp"
 #include "crp_turbulence.hpp"
 #include "crp_probe.hpp"
@@ -66,7 +67,7 @@ struct crp_field {
 };
 
 struct crp_propagator {
-	int type; // 0 BP, 1 CK
+	int type; // 0 BP, 1 CK, 2 DiffusionSDE
 	const crp_field *field;
 	double tolerance, minStep, maxStep;
 	double maxLength;
@@ -88,6 +89,10 @@ struct crp_propagator {
 	double sphC[3], sphR, sphMargin;
 	// Params::postObs of the module order
 	int postObs;
+	// DiffusionSDE (type 2)
+	double sdeScale, sdeAlpha, sdeEpsilon, sdeAdv[3];
+	int sdeAdvection;
+	uint32_t sdeSeed;
 };
 
 
@@ -104,6 +109,8 @@ struct crp_batch {
 	unsigned long long *ctr;
 	hipEvent_t ev0, ev1;
 	int32_t *det;     // observer detections per particle (inside the main allocation)
+	uint32_t *sdeStep; // DiffusionSDE steps taken per particle: the Philox counter (main allocation)
+	uint64_t *sdeStream; // DiffusionSDE random stream per particle (main allocation, 8-byte aligned)
 	double *snap;     // detection snapshots [snapK][kSnapDoubles][cap], allocated on first use
 	int snapK;
 	// captured launch sequence of the last graph-enabled call, keyed by the exact bytes of every
@@ -128,6 +135,8 @@ static Soa soa_of(const crp_batch *b) {
 	s.det = b->det;
 	s.snap = b->snap;
 	s.cap = b->cap;
+	s.sdeStep = b->sdeStep;
+	s.sdeStream = b->sdeStream;
 	return s;
 }
 
@@ -305,8 +314,16 @@ struct KeyBuf {
 // at most S steps (the queue counter is reset between them); with M < 0 it is
 // repeated S-step launches until a launch leaves no particle paused.
 template <class F, int KIND, bool MIX>
+static auto propagate_kernel() {
+	if constexpr (KIND == PK_SDE)
+		return k_sde<F>;
+	else
+		return k_propagate<F, KIND, MIX>;
+}
+
+template <class F, int KIND, bool MIX>
 static int launch_propagate(const F &f, const crp_propagator *p, const Params &P0, crp_batch *b, int *launches) {
-	auto kern = k_propagate<F, KIND, MIX>;
+	auto kern = propagate_kernel<F, KIND, MIX>();
 	const int ncu = num_cus();
 	if (ncu < 0)
 		return CRP_E_HIP;
@@ -422,6 +439,9 @@ static int launch_propagate(const F &f, const crp_propagator *p, const Params &P
 template <class F, bool MIX>
 static int dispatch_kind_p(const F &f, const crp_propagator *p, const Params &P, crp_batch *b, int *launches) {
 	const bool adaptive = !(p->minStep == p->maxStep);
+	if constexpr (!MIX)
+		if (p->type == 2)
+			return launch_propagate<F, PK_SDE, false>(f, p, P, b, launches);
 	if (p->type == 0)
 		return adaptive ? launch_propagate<F, PK_BP_ADAPTIVE, MIX>(f, p, P, b, launches)
 		                : launch_propagate<F, PK_BP_FIXED, MIX>(f, p, P, b, launches);
@@ -431,7 +451,7 @@ static int dispatch_kind_p(const F &f, const crp_propagator *p, const Params &P,
 
 template <class F>
 static int dispatch_kind(const F &f, const crp_propagator *p, const Params &P, crp_batch *b, int *launches) {
-	if (p->precision == CRP_PRECISION_MIXED)
+	if (p->precision == CRP_PRECISION_MIXED && p->type != 2)
 		return dispatch_kind_p<F, true>(f, p, P, b, launches);
 	return dispatch_kind_p<F, false>(f, p, P, b, launches);
 }
@@ -904,6 +924,12 @@ static int create_prop(int type, const char *who, const crp_field_t *field, doub
 		p->boxO[a] = p->boxS[a] = p->sphC[a] = 0;
 	p->sph = p->sphLimit = p->postObs = 0;
 	p->sphR = p->sphMargin = 0;
+	p->sdeScale = 1;
+	p->sdeAlpha = 1. / 3.;
+	p->sdeEpsilon = 0.1;
+	p->sdeAdv[0] = p->sdeAdv[1] = p->sdeAdv[2] = 0;
+	p->sdeAdvection = 0;
+	p->sdeSeed = 0;
 	*out = p;
 	return CRP_OK;
 }
@@ -918,6 +944,50 @@ int crp_propagator_create_ck(const crp_field_t *field, double tolerance, double 
 	return create_prop(1, "PropagationCK", field, tolerance, min_step, max_step, out);
 }
 
+int crp_propagator_create_sde(const crp_field_t *field, double tolerance, double min_step, double max_step,
+                              double epsilon, crp_propagator_t **out) {
+	// DiffusionSDE's setters (src/module/DiffusionSDE.cpp:272-300)
+	if (!out)
+		return fail(CRP_E_INVALID, "DiffusionSDE: out is NULL");
+	if (min_step < 0)
+		return fail(CRP_E_INVALID, "DiffusionSDE: minStep < 0 ");
+	if (min_step > max_step)
+		return fail(CRP_E_INVALID, "DiffusionSDE: minStep > maxStep");
+	if ((tolerance > 1) || (tolerance < 0))
+		return fail(CRP_E_INVALID, "DiffusionSDE: tolerance error not in range 0-1");
+	if ((epsilon > 1) || (epsilon < 0))
+		return fail(CRP_E_INVALID, "DiffusionSDE: epsilon not in range 0-1");
+	int rc = create_prop(2, "DiffusionSDE", field, tolerance, min_step, max_step, out);
+	if (rc == CRP_OK)
+		(*out)->sdeEpsilon = epsilon;
+	return rc;
+}
+
+int crp_propagator_set_sde(crp_propagator_t *p, double alpha, double scale, double epsilon, uint32_t seed) {
+	if (!p || p->type != 2)
+		return fail(CRP_E_INVALID, "crp_propagator_set_sde: not a DiffusionSDE propagator");
+	if ((alpha > 2.) || (alpha < 0))
+		return fail(CRP_E_INVALID, "DiffusionSDE: alpha not in range 0-2");
+	if (scale < 0)
+		return fail(CRP_E_INVALID, "DiffusionSDE: Scale error: Scale < 0");
+	if ((epsilon > 1) || (epsilon < 0))
+		return fail(CRP_E_INVALID, "DiffusionSDE: epsilon not in range 0-1");
+	p->sdeAlpha = alpha;
+	p->sdeScale = scale;
+	p->sdeEpsilon = epsilon;
+	p->sdeSeed = seed;
+	return CRP_OK;
+}
+
+int crp_propagator_set_sde_advection(crp_propagator_t *p, const double v[3]) {
+	if (!p || p->type != 2)
+		return fail(CRP_E_INVALID, "crp_propagator_set_sde_advection: not a DiffusionSDE propagator");
+	p->sdeAdvection = v ? 1 : 0;
+	for (int a = 0; a < 3; a++)
+		p->sdeAdv[a] = v ? v[a] : 0.0;
+	return CRP_OK;
+}
+
 int crp_propagator_set_max_trajectory_length(crp_propagator_t *p, double max_length) {
 	if (!p)
 		return fail(CRP_E_INVALID, "crp_propagator_set_max_trajectory_length: NULL propagator");
@@ -930,6 +1000,8 @@ int crp_propagator_set_precision(crp_propagator_t *p, int precision) {
 		return fail(CRP_E_INVALID, "crp_propagator_set_precision: NULL propagator");
 	if (precision != CRP_PRECISION_FP64 && precision != CRP_PRECISION_MIXED)
 		return fail(CRP_E_INVALID, "crp_propagator_set_precision: unknown precision");
+	if (p->type == 2 && precision != CRP_PRECISION_FP64)
+		return fail(CRP_E_INVALID, "crp_propagator_set_precision: DiffusionSDE runs in FP64 only");
 	p->precision = precision;
 	return CRP_OK;
 }
@@ -1027,7 +1099,8 @@ int crp_batch_create(size_t capacity, crp_batch_t **out) {
 	// one allocation: 18 double arrays, the active flags, the queue order, then a
 	// 256-byte aligned counter block
 	const size_t nd = 24;
-	size_t off = nd * capacity * sizeof(double) + 2 * capacity * sizeof(int32_t) + capacity * sizeof(uint32_t);
+	size_t off = nd * capacity * sizeof(double) + 2 * capacity * sizeof(int32_t) + 2 * capacity * sizeof(uint32_t) +
+	             capacity * sizeof(uint64_t);
 	off = (off + 255) & ~(size_t)255;
 	const size_t bytes = off + 64 * sizeof(unsigned long long);
 	hipError_t e = hipMalloc((void **)&b->base, bytes);
@@ -1043,6 +1116,8 @@ int crp_batch_create(size_t capacity, crp_batch_t **out) {
 	b->active = (int32_t *)(b->base + nd * capacity);
 	b->order = (uint32_t *)(b->active + capacity);
 	b->det = (int32_t *)(b->order + capacity);
+	b->sdeStep = (uint32_t *)(b->det + capacity);
+	b->sdeStream = (uint64_t *)(b->sdeStep + capacity); // 16 bytes per particle before it: 8-byte aligned
 	b->ctr = (unsigned long long *)((char *)b->base + off);
 	e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
 	if (e == hipSuccess)
@@ -1115,6 +1190,14 @@ int crp_batch_upload(crp_batch_t *b, size_t n, const crp_host_state *s) {
 		if (s->detection_index)
 			std::copy(s->detection_index, s->detection_index + n, det.begin());
 		CRP_HIP_TRY(hipMemcpy(b->det, det.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+		std::vector<uint32_t> sst(n, 0);
+		if (s->sde_step)
+			std::copy(s->sde_step, s->sde_step + n, sst.begin());
+		CRP_HIP_TRY(hipMemcpy(b->sdeStep, sst.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+		std::vector<uint64_t> sid(n);
+		for (size_t i = 0; i < n; i++)
+			sid[i] = s->sde_stream ? s->sde_stream[i] : (uint64_t)i;
+		CRP_HIP_TRY(hipMemcpy(b->sdeStream, sid.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice));
 	}
 	// Queue order: ascending magnetic rigidity E/|q| (neutral particles last).  Low-rigidity
 	// particles take the most adaptive steps, so starting them first shortens the tail of a
@@ -1178,6 +1261,8 @@ int crp_batch_download(const crp_batch_t *b, crp_host_state *s) {
 		CRP_HIP_TRY(hipMemcpy(s->active, b->active, n * sizeof(int32_t), hipMemcpyDeviceToHost));
 	if (s->detection_index)
 		CRP_HIP_TRY(hipMemcpy(s->detection_index, b->det, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+	if (s->sde_step)
+		CRP_HIP_TRY(hipMemcpy(s->sde_step, b->sdeStep, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
 	const double *srcv[6] = {b->sx, b->sy, b->sz, b->cx, b->cy, b->cz};
 	double *dstv2[2] = {s->source_pos, s->created_pos};
 	for (int c = 0; c < 6; c++) {
@@ -1252,6 +1337,9 @@ int crp_batch_copy(crp_batch_t *dst, const crp_batch_t *src) {
 	CRP_HIP_TRY(hipMemcpyAsync(dst->active, src->active, n * sizeof(int32_t), hipMemcpyDeviceToDevice, dst->stream));
 	CRP_HIP_TRY(hipMemcpyAsync(dst->order, src->order, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, dst->stream));
 	CRP_HIP_TRY(hipMemcpyAsync(dst->det, src->det, n * sizeof(int32_t), hipMemcpyDeviceToDevice, dst->stream));
+	CRP_HIP_TRY(hipMemcpyAsync(dst->sdeStep, src->sdeStep, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, dst->stream));
+	CRP_HIP_TRY(hipMemcpyAsync(dst->sdeStream, src->sdeStream, n * sizeof(uint64_t), hipMemcpyDeviceToDevice,
+	                           dst->stream));
 	if (src->snap) {
 		int rc = ensure_snapshots(dst, src->snapK);
 		if (rc != CRP_OK)
@@ -1298,6 +1386,18 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 	P.sphR = p->sphR;
 	P.sphMargin = p->sphMargin;
 	P.postObs = p->postObs;
+	if (p->type == 2) {
+		if (p->nObs > 0 || p->box || p->sph)
+			return fail(CRP_E_INVALID, "crp_propagate: DiffusionSDE runs with MaximumTrajectoryLength as its only fused "
+			                           "module (no observer, periodic box or spherical boundary)");
+		P.sdeScale = p->sdeScale;
+		P.sdeAlpha = p->sdeAlpha;
+		P.sdeEpsilon = p->sdeEpsilon;
+		P.sdeAdvection = p->sdeAdvection;
+		for (int a = 0; a < 3; a++)
+			P.sdeAdv[a] = p->sdeAdv[a];
+		P.sdeSeed = p->sdeSeed;
+	}
 	if (p->nObs > 0) {
 		int rc0 = ensure_snapshots(b, p->nObs);
 		if (rc0 != CRP_OK)
@@ -1338,6 +1438,10 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 	st.kernel_ms = ms;
 	if (stats)
 		*stats = st;
+	if (h[CTR_SDE_LIMIT] > 0)
+		return fail(CRP_E_STATE, "DiffusionSDE: " + std::to_string(h[CTR_SDE_LIMIT]) +
+		                             " particle(s) needed more than 2^" + std::to_string(kSdeMaxHalvings) +
+		                             " sub-steps in one step (minStep far below the field's scale) and were stopped");
 	return CRP_OK;
 }
 

@@ -48,18 +48,21 @@ CRP_PHILOX_FN crp_philox_ctr crp_philox4x32_10(crp_philox_ctr c, uint32_t k0, ui
 
 /*
  * The uniform stream of one particle in one step: draw d uses block d/2 of the
- * counter (particle, step, block, 0) with key (seed, 0x43525053 'CRPS'); each
- * block yields two doubles with 53 random bits, in [0, 1).
+ * counter (stream low word, step, block, stream high word) with key
+ * (seed, 0x43525053 'CRPS'); each block yields two doubles with 53 random bits,
+ * in [0, 1).  `stream` names the particle (its index, or a serial number that
+ * stays with it from call to call).
  */
 typedef struct {
-	uint32_t seed, particle, step, next;
+	uint32_t seed, particle, step, next, streamHi;
 	double cache;
 	int cached;
 } crp_sde_rng;
 
-CRP_PHILOX_FN void crp_sde_rng_init(crp_sde_rng *r, uint32_t seed, uint32_t particle, uint32_t step) {
+CRP_PHILOX_FN void crp_sde_rng_init(crp_sde_rng *r, uint32_t seed, uint64_t stream, uint32_t step) {
 	r->seed = seed;
-	r->particle = particle;
+	r->particle = (uint32_t)stream;
+	r->streamHi = (uint32_t)(stream >> 32);
 	r->step = step;
 	r->next = 0;
 	r->cached = 0;
@@ -75,7 +78,7 @@ CRP_PHILOX_FN double crp_sde_uniform(crp_sde_rng *r) {
 	c.v[0] = r->particle;
 	c.v[1] = r->step;
 	c.v[2] = r->next++;
-	c.v[3] = 0;
+	c.v[3] = r->streamHi;
 	crp_philox_ctr o = crp_philox4x32_10(c, r->seed, 0x43525053u);
 	const double s = 1.0 / 9007199254740992.0; /* 2^-53 */
 	double u0 = (double)(((uint64_t)o.v[0] << 21) | (o.v[1] >> 11)) * s;

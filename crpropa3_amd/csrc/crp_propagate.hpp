@@ -20,9 +20,11 @@
 
 namespace crp {
 
-enum PropKind { PK_BP_FIXED = 0, PK_BP_ADAPTIVE = 1, PK_CK_FIXED = 2, PK_CK_ADAPTIVE = 3 };
+// PK_SDE: DiffusionSDE (k_sde, crp_sde.hpp)
+enum PropKind { PK_BP_FIXED = 0, PK_BP_ADAPTIVE = 1, PK_CK_FIXED = 2, PK_CK_ADAPTIVE = 3, PK_SDE = 4 };
 // CTR_PAUSED: particles stored still active because this launch's step budget ran out
-enum { CTR_QUEUE = 0, CTR_ACCEPTED, CTR_REJECTED, CTR_EVALS, CTR_FINISHED, CTR_PAUSED, CTR_COUNT };
+// CTR_SDE_LIMIT: DiffusionSDE particles stopped because a step needed too many sub-steps
+enum { CTR_QUEUE = 0, CTR_ACCEPTED, CTR_REJECTED, CTR_EVALS, CTR_FINISHED, CTR_PAUSED, CTR_SDE_LIMIT, CTR_COUNT };
 constexpr int kColdDoubles = 10; // dynamic LDS per thread of k_propagate, in doubles
 
 struct Soa {
@@ -34,6 +36,8 @@ struct Soa {
 	int32_t *det;             // observer detections so far per particle ("DetectionIndex")
 	double *snap;             // state at detection k of particle i: snap[(k * kSnapDoubles + c) * cap + i]
 	unsigned long long cap;   // snapshot stride
+	uint32_t *sdeStep;        // DiffusionSDE: steps taken (the random-number counter)
+	const uint64_t *sdeStream; // DiffusionSDE: the particle's random stream
 };
 
 constexpr int kSnapDoubles = 9; // position x,y,z, direction x,y,z, trajectory length, time, nextStep
@@ -54,6 +58,11 @@ struct Params {
 	// conditions that run after the observer in the module list (bit 0: spherical boundary, bit 1: length
 	// limit); the observer sees, and records, nextStep limited by the modules before it only
 	int postObs;
+	// DiffusionSDE (k_sde): diffusion coefficient scale * 6.1e24 * (|E/q| / 4e9)^alpha, perpendicular
+	// fraction epsilon, uniform advection velocity, Philox seed
+	double sdeScale, sdeAlpha, sdeEpsilon, sdeAdv[3];
+	int sdeAdvection;
+	uint32_t sdeSeed;
 };
 
 __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; } // std::max

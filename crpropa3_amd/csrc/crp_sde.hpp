@@ -1,0 +1,410 @@
+// DiffusionSDE on the device (included by crp_kernels.hip): stochastic
+// transport of cosmic rays along the magnetic field lines,
+// src/module/DiffusionSDE.cpp:48-266.
+//
+// One particle per lane, persistent, with the queue refill of k_propagate.
+// A step of DiffusionSDE::process is a variable number of Cash-Karp steps
+// along the unit field direction (tryStep): trials with the time halved until
+// the error is below the tolerance, then 2^(trials-1) sub-steps.  Each loop
+// iteration of the kernel runs ONE tryStep for every busy lane, whichever
+// phase (trial or sub-step) the lane is in, so the field evaluations stay
+// converged across the wave; the random draws at the start and end of a step
+// are the divergent part.
+//
+// Bit-exact shortcuts (the same floating-point operations, not repeated):
+//  - every trial of a step starts at the same point: the first stage's field
+//    value is evaluated once per step;
+//  - the first sub-step uses the last trial's time (propTime halved k-1 times
+//    equals propTime / 2^(k-1) unless it underflows; the bits are compared)
+//    and start point, so its result is the last trial's: it is not recomputed.
+//
+// Random numbers: Philox4x32-10 per (particle stream, step) (crp_philox.h);
+// randNorm, randVector and randConeVector as src/Random.cpp:111-162 with the
+// correctly rounded cos, sin and pow and the correctly rounded double-double
+// log of crp_crmath.h, and cos/sin of acos(w) taken as w and sqrt(1 - w^2).
+#pragma once
+
+#include "crp_philox.h"
+#include "crp_propagate.hpp"
+
+namespace crp {
+
+// include/crpropa/Units.h: au, parsec = 648000 / pi * au, kpc = 1e3 parsec
+constexpr double kKpc = 1e3 * (648000 / 3.14159265358979323846 * 149597870700.0);
+constexpr double kInvKpc = 1.0 / kKpc;
+// a step whose trials end past 2^kSdeMaxHalvings sub-steps stops the particle (CTR_SDE_LIMIT)
+constexpr int kSdeMaxHalvings = 20;
+
+// B / |B| * c_light (Vector3::getUnitVector, then the product)
+__device__ __forceinline__ V3 sde_unit_c(const V3 &b) {
+	const double r = norm(b), ir = 1 / r;
+	return v3(div_rn(b.x, r, ir) * kCLight, div_rn(b.y, r, ir) * kCLight, div_rn(b.z, r, ir) * kCLight);
+}
+
+__device__ __forceinline__ V3 sde_unit(const V3 &b) {
+	const double r = norm(b), ir = 1 / r;
+	return v3(div_rn(b.x, r, ir), div_rn(b.y, r, ir), div_rn(b.z, r, ir));
+}
+
+// Vector3::cross (include/crpropa/Vector3.h:268-271)
+__device__ __forceinline__ V3 sde_cross(const V3 &a, const V3 &b) {
+	return v3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+
+// DiffusionSDE::tryStep (src/module/DiffusionSDE.cpp:233-254) from x with time t; k0 = the first
+// stage's unit field times c at x.  out = x + sum k_i b_i t, err = sum (k_i (b_i - b*_i)) t / kpc.
+template <class F>
+__device__ __forceinline__ void sde_trystep(const F &field, const V3 &x, const V3 &k0, double t, V3 &out, V3 &err) {
+	const double e0 = CK::b0 - CK::s0, e1 = CK::b1 - CK::s1, e2 = CK::b2 - CK::s2, e3 = CK::b3 - CK::s3,
+	             e4 = CK::b4 - CK::s4, e5 = CK::b5 - CK::s5;
+	out = x;
+	err = v3(0.0, 0.0, 0.0);
+	auto acc = [&](const V3 &k, double b, double e) {
+		out.x += (k.x * b) * t;
+		out.y += (k.y * b) * t;
+		out.z += (k.z * b) * t;
+		err.x += div_rn((k.x * e) * t, kKpc, kInvKpc);
+		err.y += div_rn((k.y * e) * t, kKpc, kInvKpc);
+		err.z += div_rn((k.z * e) * t, kKpc, kInvKpc);
+	};
+	// y_n = x + sum_j (k_j a_ij) t, in the reference's order
+	auto ax = [&](V3 &y, const V3 &k, double a) {
+		y.x += (k.x * a) * t;
+		y.y += (k.y * a) * t;
+		y.z += (k.z * a) * t;
+	};
+	acc(k0, CK::b0, e0);
+	V3 y = x;
+	ax(y, k0, CK::a10);
+	const V3 k1 = sde_unit_c(field.eval(y));
+	acc(k1, CK::b1, e1);
+	y = x;
+	ax(y, k0, CK::a20);
+	ax(y, k1, CK::a21);
+	const V3 k2 = sde_unit_c(field.eval(y));
+	acc(k2, CK::b2, e2);
+	y = x;
+	ax(y, k0, CK::a30);
+	ax(y, k1, CK::a31);
+	ax(y, k2, CK::a32);
+	const V3 k3 = sde_unit_c(field.eval(y));
+	acc(k3, CK::b3, e3);
+	y = x;
+	ax(y, k0, CK::a40);
+	ax(y, k1, CK::a41);
+	ax(y, k2, CK::a42);
+	ax(y, k3, CK::a43);
+	const V3 k4 = sde_unit_c(field.eval(y));
+	acc(k4, CK::b4, e4);
+	y = x;
+	ax(y, k0, CK::a50);
+	ax(y, k1, CK::a51);
+	ax(y, k2, CK::a52);
+	ax(y, k3, CK::a53);
+	ax(y, k4, CK::a54);
+	const V3 k5 = sde_unit_c(field.eval(y));
+	acc(k5, CK::b5, e5);
+}
+
+// Random::randNorm(0, 1) (src/Random.cpp:111-115): randDblExc, then randExc
+__device__ __forceinline__ double sde_randnorm(crp_sde_rng &r) {
+	const double rr = __builtin_sqrt(-2.0 * crp_log_dd_rn(1.0 - crp_sde_uniform(&r))) * 1.0;
+	const double phi = 2.0 * 3.14159265358979323846264338328 * crp_sde_uniform(&r);
+	return 0.0 + rr * crp_cos_cr(phi);
+}
+
+// Random::randVector (src/Random.cpp:141-146)
+__device__ __forceinline__ V3 sde_randvector(crp_sde_rng &r) {
+	const double pi = 3.14159265358979323846;
+	const double z = -1.0 + (1.0 - -1.0) * crp_sde_uniform(&r);
+	const double t = -1.0 * pi + (pi - -1.0 * pi) * crp_sde_uniform(&r);
+	const double rr = __builtin_sqrt(1 - z * z);
+	return v3(rr * crp_cos_cr(t), rr * crp_sin_cr(t), z);
+}
+
+// Random::randConeVector(mean, pi/2) and Vector3::getRotated (src/Random.cpp:148-162,
+// include/crpropa/Vector3.h:213-226); cosHalfPi = cos(pi/2) rounded
+__device__ __forceinline__ V3 sde_randcone(crp_sde_rng &r, const V3 &mean, double cosHalfPi) {
+	const double w = 1 + (cosHalfPi - 1) * crp_sde_uniform(&r);
+	const double c = w, s = __builtin_sqrt((1 - w) * (1 + w));
+	const V3 rv = sde_randvector(r);
+	V3 u = sde_cross(mean, rv);
+	const double ur = norm(u);
+	if (ur != 0.) {
+		const double iu = 1 / ur;
+		u = v3(div_rn(u.x, ur, iu), div_rn(u.y, ur, iu), div_rn(u.z, ur, iu));
+	}
+	const double oc = 1 - c;
+	const V3 rx = v3(c + u.x * u.x * oc, u.x * u.y * oc - u.z * s, u.x * u.z * oc + u.y * s);
+	const V3 ry = v3(u.y * u.x * oc + u.z * s, c + u.y * u.y * oc, u.y * u.z * oc - u.x * s);
+	const V3 rz = v3(u.z * u.x * oc - u.y * s, u.z * u.y * oc + u.x * s, c + u.z * u.z * oc);
+	return v3(mean.x * rx.x + mean.y * rx.y + mean.z * rx.z, mean.x * ry.x + mean.y * ry.y + mean.z * ry.z,
+	          mean.x * rz.x + mean.y * rz.y + mean.z * rz.z);
+}
+
+enum SdePhase { SDE_BEGIN = 0, SDE_TRIAL = 1, SDE_SUB = 2 };
+
+template <class F>
+__global__ void __launch_bounds__(256) k_sde(F field, Params P, Soa S, unsigned long long n,
+                                             unsigned long long *__restrict__ ctr) {
+	const int lane = __lane_id();
+	const double cosHalfPi = crp_cos_cr(3.14159265358979323846 / 2.);
+	const double minTime = P.minStep / kCLight; // minStep / c_light of the trial loop's test
+
+	bool have = false, exhausted = false;
+	unsigned idx = 0;
+	long long steps = 0;
+	int phase = SDE_BEGIN;
+	V3 pos = v3(0, 0, 0), dir = v3(0, 0, 0);
+	double q = 0, next = 0, B0 = 0, B4 = 0;
+	uint32_t rstep = 0;
+	uint64_t stream = 0;
+	// the step in progress
+	double h = 0, sh = 0, TStep = 0, NStep = 0, BStep = 0, propTime = 0, lastT = 0, allowed = 0;
+	double stepNumber = 0, j = 0;
+	int counter = 0;
+	V3 k0 = v3(0, 0, 0), start = v3(0, 0, 0), out = v3(0, 0, 0);
+	unsigned nAcc = 0, nRej = 0, nEval = 0, nFin = 0, nPause = 0, nLimit = 0;
+	extern __shared__ double crp_cold[];
+	double *const cw = crp_cold + (threadIdx.x >> 6) * (kColdDoubles * 64) + (threadIdx.x & 63);
+	double *const cPPx = cw, *const cPPy = cw + 64, *const cPPz = cw + 128;
+	double *const cPDx = cw + 192, *const cPDy = cw + 256, *const cPDz = cw + 320;
+	double *const cCur = cw + 384, *const cLen = cw + 448, *const cTim = cw + 512;
+
+	for (;;) {
+		if (nEval > 0x80000000u) {
+			atomicAdd(&ctr[CTR_ACCEPTED], (unsigned long long)nAcc);
+			atomicAdd(&ctr[CTR_REJECTED], (unsigned long long)nRej);
+			atomicAdd(&ctr[CTR_EVALS], (unsigned long long)nEval);
+			nAcc = nRej = nEval = 0;
+		}
+		// ---- refill lanes without a particle from the queue
+		const bool need = !have && !exhausted;
+		const unsigned long long m = __ballot(need);
+		if (m) {
+			const int leader = __ffsll((long long)m) - 1;
+			unsigned long long base = 0;
+			if (lane == leader)
+				base = atomicAdd(&ctr[CTR_QUEUE], (unsigned long long)__popcll(m));
+			base = __shfl(base, leader);
+			if (need) {
+				const unsigned long long my = base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
+				if (my < n) {
+					idx = S.order[my];
+					have = S.active[idx] != 0;
+					if (have) {
+						pos = v3(S.px[idx], S.py[idx], S.pz[idx]);
+						dir = v3(S.dx[idx], S.dy[idx], S.dz[idx]);
+						q = S.q[idx];
+						next = S.next[idx];
+						*cCur = S.cur[idx];
+						*cLen = S.len[idx];
+						*cTim = S.time[idx];
+						rstep = S.sdeStep[idx];
+						stream = S.sdeStream[idx];
+						if (q != 0) {
+							// calculateBTensor (src/module/DiffusionSDE.cpp:258-266): constant per particle;
+							// pow(x, 0.5) correctly rounded is sqrt(x)
+							const double rig = S.E[idx] / q;
+							const double D = P.sdeScale * 6.1e24 * crp_pow_cr(__builtin_fabs(rig) / 4.0e9, P.sdeAlpha);
+							B0 = __builtin_sqrt(2 * D);
+							B4 = __builtin_sqrt(2 * P.sdeEpsilon * D);
+						}
+						steps = 0;
+						phase = SDE_BEGIN;
+					}
+				} else {
+					exhausted = true;
+				}
+			}
+		}
+		if (__ballot(have) == 0) {
+			if (__ballot(!exhausted) == 0)
+				break;
+			continue;
+		}
+		if (!have)
+			continue;
+
+		bool done = false;   // the step is complete (commit below)
+		bool alive = true;   // the particle stays active after this step
+		bool moved = true;   // the step sets position and step state (false: deactivated with NaN)
+		V3 nPos = pos, nDir = dir;
+		double lin[3] = {0, 0, 0};
+
+		if (phase == SDE_BEGIN) {
+			// candidate->previous = current; h = clip(nextStep, minStep, maxStep) / c_light
+			*cPPx = pos.x; *cPPy = pos.y; *cPPz = pos.z;
+			*cPDx = dir.x; *cPDy = dir.y; *cPDz = dir.z;
+			h = div_rn(dclip(next, P.minStep, P.maxStep), kCLight, kInvCLight);
+			if (q == 0) {
+				// rectilinear, with the advection drift (DiffusionSDE.cpp:62-75)
+				if (P.sdeAdvection)
+					for (int a = 0; a < 3; a++)
+						lin[a] = 0 + P.sdeAdv[a] * h;
+				nPos = v3((pos.x + lin[0]) + (dir.x * h) * kCLight, (pos.y + lin[1]) + (dir.y * h) * kCLight,
+				          (pos.z + lin[2]) + (dir.z * h) * kCLight);
+				next = P.maxStep;
+				done = true;
+			} else {
+				crp_sde_rng r;
+				crp_sde_rng_init(&r, P.sdeSeed, stream, rstep);
+				const double e0 = sde_randnorm(r), e1 = sde_randnorm(r), e2 = sde_randnorm(r);
+				TStep = B0 * e0;
+				NStep = B4 * e1;
+				BStep = B4 * e2;
+				sh = __builtin_sqrt(h);
+				propTime = div_rn(TStep * sh, kCLight, kInvCLight);
+				counter = 0;
+				k0 = sde_unit_c(field.eval(pos));
+				nEval += 1;
+				phase = SDE_TRIAL;
+			}
+		}
+
+		if (phase != SDE_BEGIN) {
+			// one tryStep: a trial from pos with propTime, or a sub-step from start with allowed
+			const bool trial = phase == SDE_TRIAL;
+			const V3 x = trial ? pos : start;
+			const double t = trial ? propTime : allowed;
+			V3 kx = k0;
+			if (!trial) {
+				kx = sde_unit_c(field.eval(start));
+				nEval += 1;
+			}
+			V3 err;
+			sde_trystep(field, x, kx, t, out, err);
+			nEval += 5;
+			if (trial) {
+				const double rr = norm(err) / P.tol;
+				lastT = propTime;
+				propTime *= 0.5;
+				counter += 1;
+				const bool more = rr > 1 && __builtin_fabs(propTime) >= minTime;
+				if (more && counter > kSdeMaxHalvings) {
+					// another trial would need more than 2^kSdeMaxHalvings sub-steps: stop the particle
+					// without moving it and report it (CTR_SDE_LIMIT)
+					nLimit++;
+					alive = false;
+					moved = false;
+					done = true;
+				} else if (!more) {
+					{
+						stepNumber = (double)(1u << (counter - 1)); // pow(2, counter - 1)
+						allowed = div_rn(div_rn(TStep * sh, kCLight, kInvCLight), stepNumber, 1 / stepNumber);
+						nRej += counter - 1;
+						if (__double_as_longlong(allowed) == __double_as_longlong(lastT)) {
+							start = out; // the first sub-step is the last trial
+							j = 1;
+						} else {
+							start = pos;
+							j = 0;
+						}
+						phase = SDE_SUB;
+						if (j == stepNumber)
+							done = true;
+					}
+				}
+			} else {
+				start = out;
+				j += 1;
+				if (j == stepNumber)
+					done = true;
+			}
+			if (done && moved) {
+				// DiffusionSDE.cpp:121-209: the tangent, the no-field case, the normal and binormal
+				// displacements, the new direction
+				const V3 T = sde_unit(v3(out.x - pos.x, out.y - pos.y, out.z - pos.z));
+				const double tTest = norm(T);
+				if (P.sdeAdvection)
+					for (int a = 0; a < 3; a++)
+						lin[a] = 0 + P.sdeAdv[a] * h;
+				if (tTest != tTest) {
+					if (P.sdeAdvection)
+						nPos = v3(pos.x + lin[0], pos.y + lin[1], pos.z + lin[2]);
+					else
+						nPos = v3(pos.x + (dir.x * h) * kCLight, pos.y + (dir.y * h) * kCLight,
+						          pos.z + (dir.z * h) * kCLight);
+					next = dclip(5 * h * kCLight, P.minStep, P.maxStep);
+				} else {
+					crp_sde_rng r;
+					crp_sde_rng_init(&r, P.sdeSeed, stream, rstep);
+					r.next = 3; // after the three blocks of the eta draws
+					V3 N = v3(0, 0, 0);
+					int tries = 0;
+					while (norm(N) == 0. && tries < 64) {
+						N = sde_cross(T, sde_randvector(r));
+						tries++;
+					}
+					N = sde_unit(N);
+					const V3 Bv = sde_unit(sde_cross(T, N));
+					const V3 PO = v3((out.x + lin[0]) + (N.x * NStep + Bv.x * BStep) * sh,
+					                 (out.y + lin[1]) + (N.y * NStep + Bv.y * BStep) * sh,
+					                 (out.z + lin[2]) + (N.z * NStep + Bv.z * BStep) * sh);
+					const double por = norm(PO);
+					if (por != por) {
+						alive = false;
+						moved = false;
+					} else {
+						const V3 D2 = sde_randcone(r, T, cosHalfPi);
+						const double dr = norm(D2), idr = 1 / dr;
+						nPos = PO;
+						nDir = v3(div_rn(D2.x, dr, idr), div_rn(D2.y, dr, idr), div_rn(D2.z, dr, idr));
+						// nextStep: h * pow(stepNumber, -2) * c_light; the power of two is exact
+						next = stepNumber > 1 ? (h * __builtin_ldexp(1.0, -2 * (counter - 1))) * kCLight
+						                      : 4 * h * kCLight;
+					}
+				}
+			}
+		}
+		if (!done)
+			continue;
+
+		// ---- commit: position, direction, setCurrentStep(h c), then MaximumTrajectoryLength
+		phase = SDE_BEGIN;
+		double len = *cLen;
+		if (moved) {
+			pos = nPos;
+			dir = nDir;
+			const double hc = h * kCLight;
+			*cCur = hc;
+			len = len + hc;
+			*cLen = len;
+			*cTim = *cTim + div_rn(hc, kCLight, kInvCLight);
+		}
+		rstep++;
+		nAcc++;
+		steps++;
+		if (P.maxLen > 0) {
+			if (len >= P.maxLen)
+				alive = false;
+			else
+				next = dmin(next, P.maxLen - len);
+		}
+		nFin += alive ? 0 : 1;
+		if (!alive || (P.maxSteps >= 0 && steps >= P.maxSteps)) {
+			S.px[idx] = pos.x; S.py[idx] = pos.y; S.pz[idx] = pos.z;
+			S.dx[idx] = dir.x; S.dy[idx] = dir.y; S.dz[idx] = dir.z;
+			S.cur[idx] = *cCur; S.next[idx] = next; S.len[idx] = len; S.time[idx] = *cTim;
+			S.active[idx] = alive ? 1 : 0;
+			S.ppx[idx] = *cPPx; S.ppy[idx] = *cPPy; S.ppz[idx] = *cPPz;
+			S.pdx[idx] = *cPDx; S.pdy[idx] = *cPDy; S.pdz[idx] = *cPDz;
+			S.sdeStep[idx] = rstep;
+			nPause += alive ? 1 : 0;
+			have = false;
+		}
+	}
+	const unsigned long long tAcc = wave_sum(nAcc), tRej = wave_sum(nRej), tEval = wave_sum(nEval),
+	                         tFin = wave_sum(nFin), tPause = wave_sum(nPause), tLim = wave_sum(nLimit);
+	if (lane == 0) {
+		atomicAdd(&ctr[CTR_ACCEPTED], tAcc);
+		atomicAdd(&ctr[CTR_REJECTED], tRej);
+		atomicAdd(&ctr[CTR_EVALS], tEval);
+		atomicAdd(&ctr[CTR_FINISHED], tFin);
+		atomicAdd(&ctr[CTR_PAUSED], tPause);
+		if (tLim)
+			atomicAdd(&ctr[CTR_SDE_LIMIT], tLim);
+	}
+}
+
+} // namespace crp

@@ -84,8 +84,13 @@ public:
 		current = source;
 	}
 	std::string getDescription() const;
+	/* DiffusionSDE steps taken on the device: where the candidate's random numbers continue in its
+	 * stream (the stream is named by the serial number) */
+	uint32_t getDiffusionSteps() const { return diffusionSteps; }
+	void setDiffusionSteps(uint32_t n) { diffusionSteps = n; }
 
 private:
+	uint32_t diffusionSteps = 0;
 	bool active;
 	double weight;
 	double redshift;

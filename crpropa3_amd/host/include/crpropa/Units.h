@@ -52,6 +52,7 @@ static const double kpc = kiloparsec;
 static const double Mpc = megaparsec;
 static const double Gpc = gigaparsec;
 
+static const double sec = second;
 static const double minute = 60 * second;
 static const double hour = 3600 * second;
 static const double day = 24 * hour;

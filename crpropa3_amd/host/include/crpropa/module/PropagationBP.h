@@ -18,10 +18,10 @@ struct crp_batch;
 
 namespace crpropa {
 
-/* Shared implementation of the two GPU propagators. */
+/* Shared implementation of the GPU propagators (PropagationBP, PropagationCK, DiffusionSDE). */
 class DevicePropagator : public Module {
 public:
-	enum Kind { BP = 0, CK = 1 };
+	enum Kind { BP = 0, CK = 1, SDE = 2 };
 	enum Precision { FP64 = 0, MIXED = 1 };
 	DevicePropagator(Kind kind) : kind(kind), tolerance(0), minStep(0), maxStep(0), precision(FP64) {}
 	virtual ~DevicePropagator();
@@ -60,7 +60,7 @@ public:
 	crp_propagator *makeDevicePropagator(double maxLength) const;
 
 protected:
-	const char *name() const { return kind == BP ? "PropagationBP" : "PropagationCK"; }
+	const char *name() const { return kind == BP ? "PropagationBP" : (kind == CK ? "PropagationCK" : "DiffusionSDE"); }
 	Kind kind;
 	ref_ptr<MagneticField> field;
 	double tolerance, minStep, maxStep;

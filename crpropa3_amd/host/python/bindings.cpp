@@ -27,6 +27,8 @@
 #include "crpropa/module/Output.h"
 #include "crpropa/module/PropagationBP.h"
 #include "crpropa/module/PropagationCK.h"
+#include "crpropa/module/DiffusionSDE.h"
+#include "crpropa/advectionField/AdvectionField.h"
 
 namespace py = pybind11;
 using namespace crpropa;
@@ -118,7 +120,7 @@ PYBIND11_MODULE(_crpropa, m) {
 	UNIT(electronvolt); UNIT(eV); UNIT(keV); UNIT(MeV); UNIT(GeV); UNIT(TeV); UNIT(PeV); UNIT(EeV);
 	UNIT(au); UNIT(parsec); UNIT(kiloparsec); UNIT(megaparsec); UNIT(gigaparsec); UNIT(pc); UNIT(kpc); UNIT(Mpc);
 	UNIT(Gpc); UNIT(kilometer); UNIT(km);
-	UNIT(minute); UNIT(hour); UNIT(day); UNIT(year); UNIT(kiloyear); UNIT(Megayear); UNIT(Gigayear);
+	UNIT(sec); UNIT(minute); UNIT(hour); UNIT(day); UNIT(year); UNIT(kiloyear); UNIT(Megayear); UNIT(Gigayear);
 	UNIT(yr); UNIT(kyr); UNIT(Myr); UNIT(Gyr);
 #undef UNIT
 
@@ -184,6 +186,8 @@ PYBIND11_MODULE(_crpropa, m) {
 	    .def("hasProperty", &Candidate::hasProperty)
 	    .def("removeProperty", &Candidate::removeProperty)
 	    .def("getSerialNumber", &Candidate::getSerialNumber)
+	    .def("getDiffusionSteps", &Candidate::getDiffusionSteps)
+	    .def("setDiffusionSteps", &Candidate::setDiffusionSteps)
 	    .def("setSerialNumber", &Candidate::setSerialNumber)
 	    .def("getSourceSerialNumber", &Candidate::getSourceSerialNumber)
 	    .def("getCreatedSerialNumber", &Candidate::getCreatedSerialNumber)
@@ -417,6 +421,37 @@ PYBIND11_MODULE(_crpropa, m) {
 	py::class_<PropagationCK, DevicePropagator, ref_ptr<PropagationCK>>(m, "PropagationCK")
 	    .def(py::init<ref_ptr<MagneticField>, double, double, double>(), py::arg("field") = ref_ptr<MagneticField>(),
 	         py::arg("tolerance") = 1e-4, py::arg("minStep") = 0.1 * kpc, py::arg("maxStep") = 1 * Gpc);
+
+	// ---- advection fields and DiffusionSDE
+	py::class_<AdvectionField, ref_ptr<AdvectionField>>(m, "AdvectionField")
+	    .def("getField", &AdvectionField::getField, py::arg("position"), py::arg("time") = 0.)
+	    .def("getDivergence", &AdvectionField::getDivergence, py::arg("position"), py::arg("time") = 0.);
+	py::class_<UniformAdvectionField, AdvectionField, ref_ptr<UniformAdvectionField>>(m, "UniformAdvectionField")
+	    .def(py::init<const Vector3d &>())
+	    .def("getDescription", &UniformAdvectionField::getDescription);
+
+	py::class_<DiffusionSDE, DevicePropagator, ref_ptr<DiffusionSDE>>(m, "DiffusionSDE")
+	    .def(py::init<ref_ptr<MagneticField>, double, double, double, double>(), py::arg("magneticField"),
+	         py::arg("tolerance") = 1e-4, py::arg("minStep") = 10 * pc, py::arg("maxStep") = 1 * kpc,
+	         py::arg("epsilon") = 0.1)
+	    .def(py::init<ref_ptr<MagneticField>, ref_ptr<AdvectionField>, double, double, double, double>(),
+	         py::arg("magneticField"), py::arg("advectionField"), py::arg("tolerance") = 1e-4,
+	         py::arg("minStep") = 10 * pc, py::arg("maxStep") = 1 * kpc, py::arg("epsilon") = 0.1)
+	    .def("setEpsilon", &DiffusionSDE::setEpsilon)
+	    .def("setAlpha", &DiffusionSDE::setAlpha)
+	    .def("setScale", &DiffusionSDE::setScale)
+	    .def("setMagneticField", &DiffusionSDE::setMagneticField)
+	    .def("setAdvectionField", &DiffusionSDE::setAdvectionField)
+	    .def("getEpsilon", &DiffusionSDE::getEpsilon)
+	    .def("getAlpha", &DiffusionSDE::getAlpha)
+	    .def("getScale", &DiffusionSDE::getScale)
+	    .def("getMagneticField", &DiffusionSDE::getMagneticField)
+	    .def("getAdvectionField", &DiffusionSDE::getAdvectionField)
+	    .def("getMagneticFieldAtPosition", &DiffusionSDE::getMagneticFieldAtPosition)
+	    .def("getAdvectionFieldAtPosition", &DiffusionSDE::getAdvectionFieldAtPosition)
+	    .def("setSeed", &DiffusionSDE::setSeed)
+	    .def("getSeed", &DiffusionSDE::getSeed)
+	    .def("getDescription", &DiffusionSDE::getDescription);
 
 	py::class_<MaximumTrajectoryLength, AbstractCondition, ref_ptr<MaximumTrajectoryLength>>(
 	    m, "MaximumTrajectoryLength")

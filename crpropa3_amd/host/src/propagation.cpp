@@ -2,8 +2,10 @@
 // handling with the reference's validation, and the packing of Candidates into
 // device batches (include/crpropa3_amd.h).
 #include "crpropa/ModuleList.h"
+#include "crpropa/Random.h"
 #include "crpropa/module/Boundary.h"
 #include "crpropa/module/BreakCondition.h"
+#include "crpropa/module/DiffusionSDE.h"
 #include "crpropa/module/Observer.h"
 #include "crpropa/module/PropagationBP.h"
 #include "crpropa/module/PropagationCK.h"
@@ -40,6 +42,8 @@ struct Packed {
 	std::vector<int32_t> active;
 	std::vector<int32_t> det; // observer detections per candidate (empty: none tracked)
 	std::vector<double> src, cre; // source / created positions (empty: not tracked)
+	std::vector<uint32_t> sdeStep; // DiffusionSDE: steps taken (empty: not tracked)
+	std::vector<uint64_t> sdeStream; // DiffusionSDE: random stream = serial number
 	void resize(size_t n) {
 		pos.resize(3 * n); dir.resize(3 * n); ppos.resize(3 * n); pdir.resize(3 * n);
 		E.resize(n); q.resize(n); cur.resize(n); next.resize(n); len.resize(n); time.resize(n);
@@ -54,12 +58,22 @@ struct Packed {
 		s.detection_index = det.empty() ? nullptr : det.data();
 		s.source_pos = src.empty() ? nullptr : src.data();
 		s.created_pos = cre.empty() ? nullptr : cre.data();
+		s.sde_step = sdeStep.empty() ? nullptr : sdeStep.data();
+		s.sde_stream = sdeStream.empty() ? nullptr : sdeStream.data();
 		return s;
 	}
 };
 
-static void pack(const std::vector<Candidate *> &c, Packed &p) {
+static void pack(const std::vector<Candidate *> &c, Packed &p, bool sde = false) {
 	p.resize(c.size());
+	if (sde) {
+		p.sdeStep.resize(c.size());
+		p.sdeStream.resize(c.size());
+		for (size_t i = 0; i < c.size(); i++) {
+			p.sdeStep[i] = c[i]->getDiffusionSteps();
+			p.sdeStream[i] = c[i]->getSerialNumber();
+		}
+	}
 	for (size_t i = 0; i < c.size(); i++) {
 		const ParticleState &s = c[i]->current;
 		const Vector3d &x = s.getPosition(), &u = s.getDirection();
@@ -91,12 +105,16 @@ static void unpack(const std::vector<Candidate *> &c, const Packed &p, const std
 		cand->current.setDirectionExact(Vector3d(p.dir[3 * i], p.dir[3 * i + 1], p.dir[3 * i + 2]));
 		cand->setStepState(p.cur[i], p.next[i], p.len[i], p.time[i]);
 	}
+	if (!p.sdeStep.empty())
+		for (size_t i = 0; i < c.size(); i++)
+			c[i]->setDiffusionSteps(p.sdeStep[i]);
 }
 
 // ------------------------------------------------------------------ DevicePropagator
 void DevicePropagator::setTolerance(double tol) {
 	if ((tol > 1) || (tol < 0))
-		throw std::runtime_error(std::string(name()) + ": target error not in range 0-1");
+		throw std::runtime_error(std::string(name()) +
+		                         (kind == SDE ? ": tolerance error not in range 0-1" : ": target error not in range 0-1"));
 	tolerance = tol;
 }
 
@@ -126,9 +144,14 @@ crp_propagator *DevicePropagator::makeDevicePropagator(double maxLength) const {
 	crp_propagator *p = nullptr;
 	if (kind == BP)
 		throwIf(crp_propagator_create_bp(f, tolerance, minStep, maxStep, &p));
-	else
+	else if (kind == CK)
 		throwIf(crp_propagator_create_ck(f, tolerance, minStep, maxStep, &p));
+	else
+		throwIf(crp_propagator_create_sde(f, tolerance, minStep, maxStep, static_cast<const DiffusionSDE *>(this)->getEpsilon(),
+		                                  &p));
 	PropHandle h(p);
+	if (kind == SDE)
+		static_cast<const DiffusionSDE *>(this)->configureDevice(p);
 	if (maxLength > 0)
 		throwIf(crp_propagator_set_max_trajectory_length(p, maxLength));
 	if (precision != FP64)
@@ -189,7 +212,7 @@ void DevicePropagator::process(Candidate *candidate) const {
 	BatchHandle bh(b);
 	std::vector<Candidate *> one(1, candidate);
 	Packed pk;
-	pack(one, pk);
+	pack(one, pk, kind == SDE);
 	pk.active[0] = 1;
 	crp_host_state s = pk.view();
 	throwIf(crp_batch_upload(b, 1, &s));
@@ -236,6 +259,85 @@ std::string PropagationCK::getDescription() const {
 	s << " Target error: " << tolerance;
 	s << ", Minimum Step: " << minStep / kpc << " kpc";
 	s << ", Maximum Step: " << maxStep / kpc << " kpc";
+	return s.str();
+}
+
+// ------------------------------------------------------------------ DiffusionSDE
+DiffusionSDE::DiffusionSDE(ref_ptr<MagneticField> magneticField, double tol, double minS, double maxS, double e)
+    : DevicePropagator(SDE) {
+	// src/module/DiffusionSDE.cpp:21-32: maxStep is set before minStep (minStep starts at 0)
+	setMagneticField(magneticField);
+	setMaximumStep(maxS);
+	setMinimumStep(minS);
+	setTolerance(tol);
+	setEpsilon(e);
+	setScale(1.);
+	setAlpha(1. / 3.);
+	seed = Random::instance().randInt();
+}
+
+DiffusionSDE::DiffusionSDE(ref_ptr<MagneticField> magneticField, ref_ptr<AdvectionField> adv, double tol, double minS,
+                           double maxS, double e)
+    : DiffusionSDE(magneticField, tol, minS, maxS, e) {
+	setAdvectionField(adv);
+}
+
+void DiffusionSDE::setEpsilon(double e) {
+	if ((e > 1) || (e < 0))
+		throw std::runtime_error("DiffusionSDE: epsilon not in range 0-1");
+	epsilon = e;
+}
+
+void DiffusionSDE::setAlpha(double a) {
+	if ((a > 2.) || (a < 0))
+		throw std::runtime_error("DiffusionSDE: alpha not in range 0-2");
+	alpha = a;
+}
+
+void DiffusionSDE::setScale(double s) {
+	if (s < 0)
+		throw std::runtime_error("DiffusionSDE: Scale error: Scale < 0");
+	scale = s;
+}
+
+Vector3d DiffusionSDE::getMagneticFieldAtPosition(Vector3d pos, double z) const {
+	Vector3d B(0, 0, 0);
+	if (field.valid())
+		B = field->getField(pos, z);
+	return B;
+}
+
+Vector3d DiffusionSDE::getAdvectionFieldAtPosition(Vector3d pos, double t) const {
+	Vector3d v(0.);
+	if (advectionField.valid())
+		v = advectionField->getField(pos, t);
+	return v;
+}
+
+void DiffusionSDE::configureDevice(crp_propagator *p) const {
+	throwIf(crp_propagator_set_sde(p, alpha, scale, epsilon, seed));
+	if (advectionField.valid()) {
+		Vector3d v;
+		if (!advectionField->uniformValue(v))
+			throw std::runtime_error("DiffusionSDE: the advection field has no GPU implementation (only "
+			                         "UniformAdvectionField runs on the device)");
+		const double vv[3] = {v.x, v.y, v.z};
+		throwIf(crp_propagator_set_sde_advection(p, vv));
+	}
+}
+
+std::string DiffusionSDE::getDescription() const {
+	// src/module/DiffusionSDE.cpp:386-405
+	std::stringstream s;
+	s << "minStep: " << minStep / kpc << " kpc, ";
+	s << "maxStep: " << maxStep / kpc << " kpc, ";
+	s << "tolerance: " << tolerance << "\n";
+	if (epsilon != 0.1)
+		s << "epsilon: " << epsilon << ", ";
+	if (alpha != 1. / 3.)
+		s << "alpha: " << alpha << "\n";
+	if (scale != 1.)
+		s << "D_0: " << scale * 6.1e24 << " m^2/s" << "\n";
 	return s.str();
 }
 
@@ -369,6 +471,9 @@ static bool fusable(const ModuleList::module_list_t &mods, Fused &f) {
 	}
 	if (!f.obs)
 		f.condsBeforeObs = f.conds.size();
+	// the device runs DiffusionSDE with the length limit as its only fused module
+	if (f.prop->getKind() == DevicePropagator::SDE && (f.box || f.sphere || f.obs))
+		return false;
 	return true;
 }
 
@@ -413,7 +518,7 @@ bool ModuleList::runBatched(const candidate_vector_t &cands) {
 	throwIf(crp_batch_create(act.size(), &b));
 	BatchHandle bh(b);
 	Packed pk;
-	pack(act, pk);
+	pack(act, pk, prop->getKind() == DevicePropagator::SDE);
 	if (f.box) {
 		pk.src.resize(3 * act.size());
 		pk.cre.resize(3 * act.size());
@@ -534,7 +639,7 @@ void ModuleList::runStepwise(const candidate_vector_t &cands) {
 		if (act.empty())
 			break;
 		Packed pk;
-		pack(act, pk);
+		pack(act, pk, prop->getKind() == DevicePropagator::SDE);
 		crp_host_state s = pk.view();
 		throwIf(crp_batch_upload(b, act.size(), &s));
 		throwIf(crp_propagate(p.get(), b, 1, nullptr));

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define CRP_ABI_VERSION 7
+#define CRP_ABI_VERSION 8
 
 enum crp_status {
 	CRP_OK = 0,
@@ -130,6 +130,23 @@ int crp_propagator_create_bp(const crp_field_t *field, double tolerance, double 
 int crp_propagator_create_ck(const crp_field_t *field, double tolerance, double min_step, double max_step,
                              crp_propagator_t **out);
 
+/* DiffusionSDE(field, tolerance, minStep, maxStep, epsilon) -- src/module/DiffusionSDE.cpp:21-46,
+ * 272-315: stochastic transport along the field lines, alpha = 1/3 and scale = 1 until
+ * crp_propagator_set_sde.  Validation and messages as the reference's setters (tolerance and
+ * epsilon in [0,1], 0 <= minStep <= maxStep).  Runs in FP64 (crp_propagator_set_precision
+ * refuses CRP_PRECISION_MIXED) with MaximumTrajectoryLength as the only fused module.  Random
+ * numbers: Philox4x32-10 with key (seed, 0x43525053) and counter (stream low word, step, block,
+ * stream high word) per particle (crp_host_state.sde_step / sde_stream).  A step whose trials
+ * would need more than 2^20 sub-steps (minStep far below the field's scale) stops that particle
+ * where it is and crp_propagate returns CRP_E_STATE. */
+int crp_propagator_create_sde(const crp_field_t *field, double tolerance, double min_step, double max_step,
+                              double epsilon, crp_propagator_t **out);
+/* setAlpha / setScale / setEpsilon (src/module/DiffusionSDE.cpp:294-314) and the random seed */
+int crp_propagator_set_sde(crp_propagator_t *p, double alpha, double scale, double epsilon, uint32_t seed);
+/* UniformAdvectionField(v) of DiffusionSDE(field, advectionField, ...) -- src/advectionField/
+ * AdvectionField.cpp:37-50; v == NULL removes it */
+int crp_propagator_set_sde_advection(crp_propagator_t *p, const double v[3]);
+
 /* MaximumTrajectoryLength(maxLength) fused after each step -- src/module/BreakCondition.cpp:42-64
  * (no observer positions).  max_length <= 0 removes it. */
 int crp_propagator_set_max_trajectory_length(crp_propagator_t *p, double max_length);
@@ -156,6 +173,10 @@ typedef struct {
 	double *source_pos;        /* [3n] source.position, moved by a periodic box (may be NULL: the
 	                              current position on upload, not written on download) */
 	double *created_pos;       /* [3n] created.position, likewise */
+	uint32_t *sde_step;        /* [n] DiffusionSDE steps taken, the particle's random-number counter
+	                              (may be NULL: 0 on upload) */
+	const uint64_t *sde_stream; /* [n] DiffusionSDE random stream of each particle, e.g. its serial
+	                              number (upload only; may be NULL: the particle's index) */
 } crp_host_state;
 
 int crp_batch_create(size_t capacity, crp_batch_t **out);

@@ -1176,7 +1176,7 @@ static void orc_sde_process(const orc_field *f, const orc_sde *p, orc_state *s, 
 }
 
 long long orc_sde_propagate(const orc_field *f, const orc_sde *p, orc_state *s, size_t n, long long maxSteps,
-                            uint32_t *sde_step) {
+                            uint32_t *sde_step, const uint64_t *sde_stream) {
 	orc_rng mt;
 	if (p->rng == ORC_SDE_MT)
 		orc_rng_seed(&mt, p->seed);
@@ -1187,7 +1187,8 @@ long long orc_sde_propagate(const orc_field *f, const orc_sde *p, orc_state *s, 
 			r.kind = p->rng;
 			r.mt = &mt;
 			if (p->rng == ORC_SDE_PHILOX)
-				crp_sde_rng_init(&r.ph, p->seed, (uint32_t)i, sde_step ? sde_step[i] : (uint32_t)k);
+				crp_sde_rng_init(&r.ph, p->seed, sde_stream ? sde_stream[i] : (uint64_t)i,
+				                 sde_step ? sde_step[i] : (uint32_t)k);
 			orc_sde_process(f, p, s, i, &r);
 			if (sde_step)
 				sde_step[i]++;

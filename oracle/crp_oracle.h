@@ -113,7 +113,8 @@ size_t orc_turbulence_coeffs(int n, double spacing, double lmin, double lmax, in
  * rng ORC_SDE_MT: the reference's stream, one MT19937 seeded with `seed` and shared by the particles
  * in index order (each particle's steps before the next one's), glibc libm;
  * rng ORC_SDE_PHILOX: the device's stream, Philox4x32-10 per (particle, step) with this package's
- * correctly rounded cos/sin/pow and double-double log; sde_step[i] counts particle i's steps. */
+ * correctly rounded cos/sin/pow and double-double log; sde_step[i] counts particle i's steps and
+ * sde_stream[i] names its random stream (NULL: the index i). */
 enum { ORC_SDE_MT = 0, ORC_SDE_PHILOX = 1 };
 typedef struct {
 	double tolerance, minStep, maxStep, epsilon, alpha, scale;
@@ -124,7 +125,7 @@ typedef struct {
 	uint32_t seed;
 } orc_sde;
 long long orc_sde_propagate(const orc_field *f, const orc_sde *p, orc_state *s, size_t n, long long maxSteps,
-                            uint32_t *sde_step);
+                            uint32_t *sde_step, const uint64_t *sde_stream);
 
 /* Philox4x32-10 of one counter and key (known-answer tests); out[4] */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -103,7 +103,8 @@ def lib():
         L.orc_philox4x32_10.argtypes = [_P(C.c_uint32), _P(C.c_uint32), _P(C.c_uint32)]
         L.orc_sin_log_batch.argtypes = [_dp, C.c_size_t, _dp, _dp]
         L.orc_sde_propagate.restype = C.c_longlong
-        L.orc_sde_propagate.argtypes = [_P(Field), _P(SDE), _P(State), C.c_size_t, C.c_longlong, _P(C.c_uint32)]
+        L.orc_sde_propagate.argtypes = [_P(Field), _P(SDE), _P(State), C.c_size_t, C.c_longlong, _P(C.c_uint32),
+                                        _P(C.c_uint64)]
         L.orc_turbulence_coeffs.restype = C.c_size_t
         L.orc_turbulence_coeffs.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int] + \
             [C.c_double] * 4 + [C.c_uint32, _fp]
@@ -468,9 +469,10 @@ def sin_log(x):
 
 
 def sde_propagate(field, state, steps, tolerance, min_step, max_step, epsilon=0.1, alpha=1 / 3., scale=1.0,
-                  advection=None, max_len=0.0, rng=SDE_MT, seed=1, sde_step=None):
+                  advection=None, max_len=0.0, rng=SDE_MT, seed=1, sde_step=None, sde_stream=None):
     """DiffusionSDE steps (src/module/DiffusionSDE.cpp) on the oracle state; rng SDE_MT follows the
-    reference's stream (serial, particle by particle), SDE_PHILOX the device's."""
+    reference's stream (serial, particle by particle), SDE_PHILOX the device's (sde_step: uint32 step
+    counters, updated; sde_stream: uint64 stream numbers, default the particle index)."""
     p = SDE(tolerance, min_step, max_step, epsilon, alpha, scale, 0, (C.c_double * 3)(0, 0, 0), max_len, rng, seed)
     if advection is not None:
         p.advection = 1
@@ -479,8 +481,13 @@ def sde_propagate(field, state, steps, tolerance, min_step, max_step, epsilon=0.
     s = state.struct()
     cnt = None
     if sde_step is not None:
+        assert sde_step.dtype == np.uint32 and sde_step.flags.c_contiguous
         cnt = sde_step.ctypes.data_as(_P(C.c_uint32))
-    return lib().orc_sde_propagate(C.byref(field.s), C.byref(p), C.byref(s), state.n, steps, cnt)
+    stream = None
+    if sde_stream is not None:
+        sde_stream = np.ascontiguousarray(sde_stream, dtype=np.uint64)
+        stream = sde_stream.ctypes.data_as(_P(C.c_uint64))
+    return lib().orc_sde_propagate(C.byref(field.s), C.byref(p), C.byref(s), state.n, steps, cnt, stream)
 
 
 def sde_args(meta):

@@ -1,0 +1,134 @@
+"""DiffusionSDE through the reference's Python interface on the GPU: the reference's unit tests
+(test/testDiffusionSDE.py:165-233: neutral, no field, advection) and ModuleList runs, whose random
+numbers depend only on the seed, the candidates' serial numbers and their step counts -- not on
+how the run is split -- and equal the oracle's on the same streams."""
+import numpy as np
+import pytest
+
+import crpropa3_amd as crp
+import oracle_ctypes as O
+
+pytestmark = pytest.mark.gpu
+
+
+def candidate(Z, E, direction=(1, 0, 0)):
+    c = crp.Candidate()
+    c.current.setId(crp.nucleusId(1, Z))
+    c.current.setEnergy(E)
+    c.current.setDirection(crp.Vector3d(*direction))
+    return c
+
+
+def test_reference_neutral_propagation():
+    B = crp.UniformMagneticField(crp.Vector3d(0, 0, 1 * crp.nG))
+    d = crp.DiffusionSDE(B, 1e-4, 1 * crp.pc, 10 * crp.kpc, 0.)
+    c = candidate(0, 10 * crp.TeV)
+    d.process(c)
+    pos = c.current.getPosition()
+    assert pos.x / crp.pc == pytest.approx(1., abs=1e-7)
+    assert pos.y == 0. and pos.z == 0.
+    assert c.getNextStep() / (10 * crp.kpc) == pytest.approx(1., abs=1e-7)
+    assert c.getDiffusionSteps() == 1
+
+
+def test_reference_no_field_propagation():
+    d = crp.DiffusionSDE(crp.UniformMagneticField(crp.Vector3d(0.)), 1e-4, 1 * crp.pc, 10 * crp.kpc, 0.)
+    c = candidate(1, 10 * crp.TeV)
+    d.process(c)
+    pos = c.current.getPosition()
+    assert pos.x / crp.pc == pytest.approx(1., abs=1e-7)
+    assert pos.y == 0. and pos.z == 0.
+    assert c.getNextStep() / (1 * crp.pc) == pytest.approx(5., abs=1e-7)
+
+
+def test_reference_advective_propagation():
+    adv = crp.UniformAdvectionField(crp.Vector3d(0., 1e6, 0.))
+    d = crp.DiffusionSDE(crp.UniformMagneticField(crp.Vector3d(0.)), adv, 1e-4, 1 * crp.pc, 10 * crp.kpc, 0.)
+    c = candidate(1, 10 * crp.TeV)
+    d.process(c)
+    pos = c.current.getPosition()
+    assert pos.x == 0.
+    assert pos.y == pytest.approx(1 * crp.pc / crp.c_light * 1e6, abs=1e-7)
+    assert pos.z == 0.
+    assert c.getNextStep() / (1 * crp.pc) == pytest.approx(5., abs=1e-7)
+
+
+def make_candidates(n, seed=3, serials=None):
+    """n candidates (Z 0..2, 1-100 PeV) from a seeded generator; serials: their serial numbers"""
+    rng = np.random.default_rng(seed)
+    cands = []
+    for i in range(n):
+        Z = int(rng.integers(0, 3))
+        c = crp.Candidate(crp.nucleusId(max(1, 2 * Z), Z), float(rng.uniform(1, 100)) * crp.PeV,
+                          crp.Vector3d(*rng.uniform(0, 1, 3) * crp.kpc), crp.Vector3d(*rng.normal(size=3)))
+        if serials is not None:
+            c.setSerialNumber(int(serials[i]))
+        cands.append(c)
+    return cands
+
+
+def state(cands):
+    pos = np.array([[c.current.getPosition().x, c.current.getPosition().y, c.current.getPosition().z] for c in cands])
+    d = np.array([[c.current.getDirection().x, c.current.getDirection().y, c.current.getDirection().z] for c in cands])
+    length = np.array([c.getTrajectoryLength() for c in cands])
+    nxt = np.array([c.getNextStep() for c in cands])
+    steps = np.array([c.getDiffusionSteps() for c in cands])
+    return pos, d, length, nxt, steps
+
+
+def sim_for(seed):
+    B = crp.UniformMagneticField(crp.Vector3d(1e-13, 0, 3e-13))
+    d = crp.DiffusionSDE(B, crp.UniformAdvectionField(crp.Vector3d(0, 0, 5e5)), 1e-4, 1 * crp.pc, 200 * crp.pc, 0.3)
+    d.setSeed(seed)
+    sim = crp.ModuleList()
+    sim.add(d)
+    sim.add(crp.MaximumTrajectoryLength(20 * crp.kpc))
+    return sim, d
+
+
+def test_modulelist_split_runs_and_process_loop_agree():
+    """One run to the end, the same candidates in runs of a few steps, and process() calls of the
+    module one candidate at a time give the same numbers (same serial numbers, same seed)."""
+    n = 24
+    a = make_candidates(n)
+    serials = [c.getSerialNumber() for c in a]
+    b = make_candidates(n, serials=serials)
+    cc = make_candidates(n, serials=serials)
+    sim, d = sim_for(77)
+    sim.run(a)
+    sim2, _ = sim_for(77)
+    sim2.setMaximumSteps(5)
+    for _ in range(200):
+        sim2.run(b)
+    sim3, d3 = sim_for(77)
+    maxlen = crp.MaximumTrajectoryLength(20 * crp.kpc)
+    for c in cc:
+        while c.isActive():
+            d3.process(c)
+            maxlen.process(c)
+    sa, sb, sc = state(a), state(b), state(cc)
+    for x, y, z in zip(sa, sb, sc):
+        assert np.array_equal(x, y) and np.array_equal(x, z)
+    assert not any(c.isActive() for c in a + b)
+
+
+def test_modulelist_matches_oracle_streams():
+    n = 48
+    cands = make_candidates(n, seed=9)
+    pos0, dir0, _, _, _ = state(cands)
+    E = np.array([c.current.getEnergy() for c in cands])
+    q = np.array([c.current.getCharge() for c in cands])
+    serial = np.array([c.getSerialNumber() for c in cands], dtype=np.uint64)
+    sim, d = sim_for(5)
+    sim.run(cands)
+    st = O.OracleState(pos0, dir0, E, q)
+    st.dir = dir0.copy()  # the candidates' unit vectors as they are (normalising again may change bits)
+    steps = np.zeros(n, dtype=np.uint32)
+    O.sde_propagate(O.OracleField.uniform([1e-13, 0, 3e-13]), st, 10 ** 6, tolerance=1e-4, min_step=1 * crp.pc,
+                    max_step=200 * crp.pc, epsilon=0.3, advection=[0, 0, 5e5], max_len=20 * crp.kpc,
+                    rng=O.SDE_PHILOX, seed=5, sde_step=steps, sde_stream=serial)
+    pos, dd, length, nxt, k = state(cands)
+    assert np.array_equal(pos.view(np.uint64), st.pos.view(np.uint64))
+    assert np.array_equal(dd.view(np.uint64), st.dir.view(np.uint64))
+    assert np.array_equal(length.view(np.uint64), st.trajectory_length.view(np.uint64))
+    assert np.array_equal(k, steps)
