@@ -12,7 +12,11 @@ Other workloads (--workload): bp_grid64 (4096 protons, 2000 fixed 0.1 Mpc
 Boris steps, 64^3 grid of 50 kpc), bp_planewave (2^20 protons 1e15-1e18 eV,
 fixed 0.05 pc Boris steps in PlaneWaveTurbulence with Nm=256, Brms 1 muG,
 lMin 1 pc, lMax 100 pc, seed 11, 10000 steps), bp_grid512 / ck_grid512 (2^21
-protons, 1000 fixed 20 kpc steps in a 512^3 grid of 10 kpc, seed 3), and
+protons, 1000 fixed 20 kpc steps in a 512^3 grid of 10 kpc, seed 3),
+sde_grid256 (DiffusionSDE: 2^20 protons 1 TeV-1 PeV from the centre of a
+periodic 256^3 Kolmogorov grid of 1 pc (lMin 2 cells, lMax 128 cells, seed 7),
+tolerance 1e-4, minStep 0.1 pc, maxStep 10 pc, epsilon 0.1, alpha 1/3,
+scale 1, Philox seed 11, MaximumTrajectoryLength 1 kpc), and
 adjoint_grid128 (forward + backward of the differentiable Boris model: 65536
 protons 1-100 EeV, 4096 steps of 10 kpc through a 128^3 grid of 20 kpc, seed
 5, checkpoints every 64 steps; loss = sum |u_final - target|^2).
@@ -65,6 +69,10 @@ WORKLOADS = {
                        lmax_cells=256, seed=3, tolerance=0.42, min_step=20 * kpc, max_step=20 * kpc,
                        max_length=0.0, steps=1000, emin=1 * EeV, emax=100 * EeV, species=(1,), weights=(1.0,),
                        start="centre"),
+    "sde_grid256": dict(kind="SDE", n=2 ** 20, field="grid", grid_n=256, spacing=1 * pc, lmin_cells=2,
+                        lmax_cells=128, seed=7, tolerance=1e-4, min_step=0.1 * pc, max_step=10 * pc,
+                        max_length=1 * kpc, epsilon=0.1, alpha=1 / 3., scale=1.0, sde_seed=11,
+                        emin=1e12 * eV, emax=1e15 * eV, species=(1,), weights=(1.0,), start="centre"),
     "ck_grid512": dict(kind="CK", n=2 ** 21, field="grid", grid_n=512, spacing=10 * kpc, lmin_cells=2,
                        lmax_cells=256, seed=3, tolerance=1e-4, min_step=20 * kpc, max_step=20 * kpc,
                        max_length=0.0, steps=1000, emin=1 * EeV, emax=100 * EeV, species=(1,), weights=(1.0,),
@@ -118,8 +126,15 @@ def cpu_baseline(w, fdata, pos, d, E, q, target_s=15.0):
     def run(m, nsteps):
         st = O.OracleState(pos[:m], d[:m], E[:m], q[:m])
         t0 = time.perf_counter()
-        nacc, _ = O.propagate(f, kind, st, nsteps, tolerance=w["tolerance"], min_step=w["min_step"],
-                              max_step=w["max_step"], max_len=w["max_length"], libm=O.LIBM_GLIBC, nthreads=threads)
+        if w["kind"] == "SDE":
+            # the reference's threading: one MT19937 per OpenMP thread, glibc libm
+            nacc = O.sde_propagate_threads(f, st, min(nsteps, 2 ** 62), w["tolerance"], w["min_step"], w["max_step"],
+                                           epsilon=w["epsilon"], alpha=w["alpha"], scale=w["scale"],
+                                           max_len=w["max_length"], seed=w["sde_seed"], nthreads=threads)
+        else:
+            nacc, _ = O.propagate(f, kind, st, nsteps, tolerance=w["tolerance"], min_step=w["min_step"],
+                                  max_step=w["max_step"], max_len=w["max_length"], libm=O.LIBM_GLIBC,
+                                  nthreads=threads)
         return nacc, time.perf_counter() - t0
 
     m = min(len(E), 16 * threads)
@@ -329,8 +344,14 @@ def main():
     else:
         field = capi.Field.planewave(fdata["k"], fdata["Ak"], fdata["xi"], fdata["kappa"], fdata["beta"])
     prec = capi.PRECISION_MIXED if args.precision == "mixed" else capi.PRECISION_FP64
-    prop = capi.Propagator(w["kind"], field, w["tolerance"], w["min_step"], w["max_step"],
-                           max_length=w["max_length"], precision=prec)
+    if w["kind"] == "SDE":
+        if args.precision != "fp64":
+            sys.exit("bench.py: DiffusionSDE runs in FP64 only")
+        prop = capi.Propagator("SDE", field, w["tolerance"], w["min_step"], w["max_step"], max_length=w["max_length"],
+                               epsilon=w["epsilon"], alpha=w["alpha"], scale=w["scale"], seed=w["sde_seed"])
+    else:
+        prop = capi.Propagator(w["kind"], field, w["tolerance"], w["min_step"], w["max_step"],
+                               max_length=w["max_length"], precision=prec)
     init = capi.Batch(n)
     init.upload(pos, d, E, q)
     work = capi.Batch(n)
@@ -362,8 +383,12 @@ def main():
     config = {"workload": args.workload, "propagator": w["kind"], "precision": args.precision,
               "particles": n, "tolerance": w["tolerance"], "min_step_m": w["min_step"], "max_step_m": w["max_step"],
               "max_length_Mpc": w["max_length"] / Mpc, "propagation_steps": w.get("steps"),
-              "accepted_steps_per_pass": acc // args.steps,
-              "trials_per_pass": (evals // per_trial) // args.steps}
+              "accepted_steps_per_pass": acc // args.steps}
+    if w["kind"] == "SDE":
+        config.update({"propagator": "DiffusionSDE", "epsilon": w["epsilon"], "alpha": w["alpha"],
+                       "scale": w["scale"], "field_evals_per_pass": evals // args.steps})
+    else:
+        config["trials_per_pass"] = (evals // per_trial) // args.steps
     if w["field"] == "grid":
         fp16 = args.grid_storage == "fp16"
         config.update({"grid": "%d^3 %s, spacing %.0f kpc" % (w["grid_n"], "FP16+brick scale" if fp16 else "FP32",
@@ -373,8 +398,8 @@ def main():
         achieved = bytes_per_launch / kern_s / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(args.workload, args.precision),
-                    "kernel": "k_propagate<%s, %s>" % ("FieldGridCells16" if fp16 else (
-                        "FieldGridCells" if args.grid_layout == "cells" else "FieldGrid"), w["kind"]),
+                    "kernel": ("k_sde<%s>" if w["kind"] == "SDE" else "k_propagate<%s, " + w["kind"] + ">") % (
+                        "FieldGridCells16" if fp16 else ("FieldGridCells" if args.grid_layout == "cells" else "FieldGrid")),
                     "kernel_ms_avg": kern_s * 1e3, "algorithmic_bytes_per_launch": bytes_per_launch}
         if args.grid_layout == "cells":
             # every evaluation reads one random line of the cell table; the GPU's rate of such reads,

@@ -1175,6 +1175,39 @@ static void orc_sde_process(const orc_field *f, const orc_sde *p, orc_state *s, 
 	s->next_step[i] = stepNumber > 1 ? h * sde_pow(k, stepNumber, -2.) * C_LIGHT : 4 * h * C_LIGHT;
 }
 
+/* The CPU baseline's form: ModuleList::run's OpenMP loop over the candidates with one MT19937 per
+ * thread (Random::instance() is per thread, src/Random.cpp), seeded seed + thread number; glibc libm.
+ * The numbers depend on the thread schedule, as the reference's do. */
+long long orc_sde_propagate_threads(const orc_field *f, const orc_sde *p, orc_state *s, size_t n, long long maxSteps,
+                                    int nthreads) {
+	if (nthreads <= 0)
+		nthreads = omp_get_max_threads();
+	long long total = 0;
+#pragma omp parallel num_threads(nthreads) reduction(+ : total)
+	{
+		orc_rng mt;
+		orc_rng_seed(&mt, p->seed + (uint32_t)omp_get_thread_num());
+		orc_sde_rng r;
+		r.kind = ORC_SDE_MT;
+		r.mt = &mt;
+#pragma omp for schedule(dynamic, 16)
+		for (size_t i = 0; i < n; i++) {
+			for (long long k = 0; k < maxSteps && s->active[i]; k++) {
+				orc_sde_process(f, p, s, i, &r);
+				if (p->maxTrajectoryLength > 0) {
+					const double len = s->trajectory_length[i];
+					if (len >= p->maxTrajectoryLength)
+						s->active[i] = 0;
+					else
+						s->next_step[i] = orc_min(s->next_step[i], p->maxTrajectoryLength - len);
+				}
+				total++;
+			}
+		}
+	}
+	return total;
+}
+
 long long orc_sde_propagate(const orc_field *f, const orc_sde *p, orc_state *s, size_t n, long long maxSteps,
                             uint32_t *sde_step, const uint64_t *sde_stream) {
 	orc_rng mt;

@@ -126,6 +126,9 @@ typedef struct {
 } orc_sde;
 long long orc_sde_propagate(const orc_field *f, const orc_sde *p, orc_state *s, size_t n, long long maxSteps,
                             uint32_t *sde_step, const uint64_t *sde_stream);
+/* ModuleList::run with OpenMP: one MT19937 per thread (seed + thread number), glibc libm (CPU baseline) */
+long long orc_sde_propagate_threads(const orc_field *f, const orc_sde *p, orc_state *s, size_t n, long long maxSteps,
+                                    int nthreads);
 
 /* Philox4x32-10 of one counter and key (known-answer tests); out[4] */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

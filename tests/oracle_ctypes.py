@@ -102,6 +102,8 @@ def lib():
         L.orc_cos_phases.argtypes = [_dp, C.c_size_t, _dp, _P(C.c_int32), _dp]
         L.orc_philox4x32_10.argtypes = [_P(C.c_uint32), _P(C.c_uint32), _P(C.c_uint32)]
         L.orc_sin_log_batch.argtypes = [_dp, C.c_size_t, _dp, _dp]
+        L.orc_sde_propagate_threads.restype = C.c_longlong
+        L.orc_sde_propagate_threads.argtypes = [_P(Field), _P(SDE), _P(State), C.c_size_t, C.c_longlong, C.c_int]
         L.orc_sde_propagate.restype = C.c_longlong
         L.orc_sde_propagate.argtypes = [_P(Field), _P(SDE), _P(State), C.c_size_t, C.c_longlong, _P(C.c_uint32),
                                         _P(C.c_uint64)]
@@ -488,6 +490,18 @@ def sde_propagate(field, state, steps, tolerance, min_step, max_step, epsilon=0.
         sde_stream = np.ascontiguousarray(sde_stream, dtype=np.uint64)
         stream = sde_stream.ctypes.data_as(_P(C.c_uint64))
     return lib().orc_sde_propagate(C.byref(field.s), C.byref(p), C.byref(s), state.n, steps, cnt, stream)
+
+
+def sde_propagate_threads(field, state, steps, tolerance, min_step, max_step, epsilon=0.1, alpha=1 / 3., scale=1.0,
+                          advection=None, max_len=0.0, seed=1, nthreads=0):
+    """DiffusionSDE with one MT19937 per OpenMP thread (the reference's threading); returns the steps."""
+    p = SDE(tolerance, min_step, max_step, epsilon, alpha, scale, 0, (C.c_double * 3)(0, 0, 0), max_len, SDE_MT, seed)
+    if advection is not None:
+        p.advection = 1
+        for a in range(3):
+            p.adv[a] = advection[a]
+    s = state.struct()
+    return lib().orc_sde_propagate_threads(C.byref(field.s), C.byref(p), C.byref(s), state.n, steps, nthreads)
 
 
 def sde_args(meta):

@@ -24,3 +24,12 @@ def test_oracle_sde_bit_exact_vs_reference(name):
     assert bits_equal(st.trajectory_length, c["final_len"]) and bits_equal(st.next_step, c["final_next"])
     assert bits_equal(st.time, c["final_time"])
     assert np.array_equal(st.active, c["final_active"])
+
+
+def test_oracle_sde_threads_one_thread_is_the_reference_stream():
+    """The CPU baseline's threaded form seeds thread t with seed + t: one thread is the reference."""
+    c = O.load_case("sde_uniform")
+    st = O.OracleState(c["init_pos"], c["init_dir"], c["energy"], c["charge"])
+    n = O.sde_propagate_threads(O.sde_field(c), st, 10 ** 6, nthreads=1, **O.sde_args(c["meta"]))
+    assert n == int(c["nsteps"].sum())
+    assert bits_equal(st.pos, c["final_pos"]) and bits_equal(st.dir, c["final_dir"])
