@@ -39,6 +39,18 @@ class HostState(C.Structure):
                 ("created_pos", _dp), ("sde_step", C.POINTER(C.c_uint32)), ("sde_stream", C.POINTER(C.c_uint64))]
 
 
+class JF12Params(C.Structure):
+    """crp_jf12_params: JF12Field's switches and constants"""
+    _fields_ = [(n, C.c_int) for n in ("use_regular", "use_disk", "use_toroidal_halo", "use_x", "use_striated",
+                                       "use_turbulent")] + \
+               [("r_arms", C.c_double * 8), ("b_disk", C.c_double * 8)] + \
+               [(n, C.c_double) for n in ("sin_pitch", "cos_pitch", "tan90_minus_pitch", "b_ring", "h_disk", "w_disk",
+                                          "b_north", "b_south", "r_north", "r_south", "w_halo", "z0", "b_x",
+                                          "sin_theta_x0", "cos_theta_x0", "tan_theta_x0", "r_xc", "r_x", "sqrt_beta")] + \
+               [("b_disk_turb", C.c_double * 8)] + \
+               [(n, C.c_double) for n in ("b_disk_turb5", "z_disk_turb", "b_halo_turb", "r_halo_turb", "z_halo_turb")]
+
+
 class LaunchConfig(C.Structure):
     """crp_launch_config: block size, workgroups per CU, steps per launch, hipGraph replay."""
     _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("steps_per_launch", C.c_longlong),
@@ -74,6 +86,9 @@ def lib():
                                                         C.POINTER(_vp)]
         L.crp_field_create_planewave.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, C.POINTER(_vp)]
         L.crp_field_destroy.argtypes = [_vp]
+        L.crp_jf12_default_params.argtypes = [C.c_int, C.POINTER(JF12Params)]
+        L.crp_field_create_jf12.argtypes = [C.POINTER(JF12Params), _fp, C.POINTER(C.c_int), _dp, _dp, C.c_int, _fp,
+                                            C.POINTER(C.c_int), _dp, _dp, C.c_int, C.c_int, C.POINTER(_vp)]
         L.crp_field_set_grid_layout.argtypes = [_vp, C.c_int]
         L.crp_field_get_grid_layout.argtypes = [_vp, C.POINTER(C.c_int)]
         L.crp_field_set_grid_storage.argtypes = [_vp, C.c_int]
@@ -179,6 +194,38 @@ class Field:
         return Field(h)
 
     @staticmethod
+    def jf12(params=None, planck=False, striated=None, striated_origin=(0, 0, 0), striated_spacing=None,
+             striated_reflective=False, turbulent=None, turbulent_origin=(0, 0, 0), turbulent_spacing=None,
+             turbulent_reflective=False, turbulent_clip=False):
+        """JF12Field (planck: PlanckJF12bField): params a JF12Params (default: the constructor's values
+        with use_striated / use_turbulent set when the grid is given); striated a (Nx, Ny, Nz) float32
+        Grid1f, turbulent a (Nx, Ny, Nz, 3) float32 Grid3f."""
+        if params is None:
+            params = jf12_default_params(planck)
+            params.use_striated = int(striated is not None)
+            params.use_turbulent = int(turbulent is not None)
+        keep = []
+
+        def grid(v, origin, spacing, comps):
+            if v is None:
+                return None, None, None, None
+            v = np.ascontiguousarray(v, dtype=np.float32)
+            if v.ndim != (3 if comps == 1 else 4) or (comps == 3 and v.shape[3] != 3):
+                raise ValueError("grid of shape (Nx, Ny, Nz%s) expected" % ("" if comps == 1 else ", 3"))
+            n = (C.c_int * 3)(*v.shape[:3])
+            o = _f64(origin, (3,))
+            sp = _f64(np.broadcast_to(np.asarray(spacing, dtype=np.float64), (3,)))
+            keep.extend([v, n, o, sp])
+            return _p(v, _fp), n, _p(o), _p(sp)
+
+        sv, sn, so, ss = grid(striated, striated_origin, striated_spacing, 1)
+        tv, tn, to, ts = grid(turbulent, turbulent_origin, turbulent_spacing, 3)
+        h = _vp()
+        check(lib().crp_field_create_jf12(C.byref(params), sv, sn, so, ss, int(striated_reflective), tv, tn, to, ts,
+                                          int(turbulent_reflective), int(turbulent_clip), C.byref(h)))
+        return Field(h)
+
+    @staticmethod
     def grid3f_device(ptr, shape, origin, spacing, reflective=False, clip_volume=False, keep=None):
         """Borrow grid values already in device memory (e.g. a torch tensor's data_ptr())."""
         o = _f64(origin, (3,))
@@ -260,6 +307,13 @@ def turbulence_generate(desc, draws):
 MODULES = {"box": 1, "sphere": 2, "maxlen": 3, "observer": 4}  # enum crp_module
 # SphericalBoundary's default margin, 0.1 kpc (include/crpropa/Units.h:109-114)
 MODULE_SPHERE_DEFAULT_MARGIN = 0.1 * (1e3 * (648000 / math.pi * 149597870700.0))
+
+
+def jf12_default_params(planck=False):
+    """The JF12Field (PlanckJF12bField) constructor's constants (crp_jf12_default_params)."""
+    p = JF12Params()
+    check(lib().crp_jf12_default_params(int(planck), C.byref(p)))
+    return p
 
 
 class Propagator:

@@ -506,4 +506,88 @@ CRP_MATH_FN double crp_log_dd_rn(double x) {
 	return l.hi + l.lo;
 }
 
+/* exp(x): the double-double exponential (relative error about 2^-100) rounded once, so correctly
+ * rounded except for arguments whose exact value lies within 2^-100 of a rounding boundary; results
+ * below 2^-1022 are rounded twice (the scaling of a subnormal result) */
+CRP_MATH_FN double crp_exp_rn(double x) {
+	if (x != x)
+		return x + x;
+	if (x > 709.782712893384)
+		return 1.0 / 0.0;
+	if (x < -745.1332191019412)
+		return 0.0;
+	int sc;
+	crp_dd r = crp_exp_dd(crp_mkdd(x, 0.0), &sc);
+	double v = r.hi + r.lo; /* the one rounding */
+	int s1 = sc / 2, s2 = sc - s1;
+	return v * crp_pow2i(s1) * crp_pow2i(s2);
+}
+
+/* a / b in double-double: three quotient digits with exact remainders (about 2^-104) */
+CRP_MATH_FN crp_dd crp_dd_div(crp_dd a, crp_dd b) {
+	double q1 = a.hi / b.hi;
+	crp_dd r = crp_dd_add(a, crp_dd_neg(crp_dd_mul_d(b, q1)));
+	double q2 = r.hi / b.hi;
+	r = crp_dd_add(r, crp_dd_neg(crp_dd_mul_d(b, q2)));
+	double q3 = r.hi / b.hi;
+	return crp_dd_add_d(crp_fast_two_sum(q1, q2), q3);
+}
+
+/*
+ * atan2(y, x) with C99's special cases: t = min(|x|,|y|) / max(|x|,|y|) in double-double,
+ * atan t = atan(j/64) + atan(u), u = (t - j/64) / (1 + t j/64), |u| <= 1/128, from the table and
+ * the odd series to u^17 (terms from u^9 on in double); then pi/2 - a for |y| > |x| and pi - a for
+ * x < 0, the sign of y last; rounded once (about 2^-100 before the rounding).
+ */
+CRP_MATH_FN double crp_atan2_rn(double y, double x) {
+	if (x != x || y != y)
+		return x + y;
+	const double ax = x < 0 ? -x : x, ay = y < 0 ? -y : y;
+	const int nx = (crp_as_u64(x) >> 63) != 0, ny = (crp_as_u64(y) >> 63) != 0;
+	const crp_dd PI = crp_mkdd(CRP_PI_DD_HI, CRP_PI_DD_LO), PIO2 = crp_mkdd(CRP_PIO2_DD_HI, CRP_PIO2_DD_LO);
+	crp_dd a;
+	const double inf = 1.0 / 0.0;
+	if (ay == 0.0) {
+		a = nx ? PI : crp_mkdd(0.0, 0.0);
+	} else if (ax == 0.0) {
+		a = PIO2;
+	} else if (ax == inf && ay == inf) {
+		a = crp_dd_mul_d(PIO2, 0.5);
+		if (nx)
+			a = crp_dd_add(PI, crp_dd_neg(a));
+	} else if (ax == inf) {
+		a = nx ? PI : crp_mkdd(0.0, 0.0);
+	} else if (ay == inf) {
+		a = PIO2;
+	} else {
+		const int swap = ay > ax;
+		const double num = swap ? ax : ay, den = swap ? ay : ax;
+		const double th = num / den;
+		crp_dd at;
+		if (th < 0x1p-500) {
+			at = crp_mkdd(th, 0.0); /* atan t = t (t^3/3 is far below the rounding) */
+		} else {
+			const double tl = __builtin_fma(-th, den, num) / den;
+			const double j = crp_rint(th * 64.0), t0 = j * (1.0 / 64.0);
+			const crp_dd nu = crp_two_sum(th - t0, tl); /* th - t0 is exact */
+			crp_dd dn = crp_two_prod(th, t0);
+			dn = crp_dd_add_d(crp_dd_add_d(dn, tl * t0), 1.0);
+			const crp_dd u = crp_dd_div(nu, dn);
+			const crp_dd u2 = crp_dd_mul(u, u);
+			const double w = u2.hi;
+			const double tail = 1.0 / 9 - w * (1.0 / 11 - w * (1.0 / 13 - w * (1.0 / 15 - w * (1.0 / 17))));
+			crp_dd sr = crp_dd_add(crp_dd_neg(crp_tab_dd(crp_recip_tab, 7)), crp_dd_mul_d(u2, tail));
+			sr = crp_dd_add(crp_tab_dd(crp_recip_tab, 5), crp_dd_mul(u2, sr));
+			sr = crp_dd_add(crp_dd_neg(crp_tab_dd(crp_recip_tab, 3)), crp_dd_mul(u2, sr));
+			sr = crp_dd_add(crp_mkdd(1.0, 0.0), crp_dd_mul(u2, sr));
+			at = crp_dd_add(crp_tab_dd(crp_atan_tab, (int)j), crp_dd_mul(u, sr));
+		}
+		a = swap ? crp_dd_add(PIO2, crp_dd_neg(at)) : at;
+		if (nx)
+			a = crp_dd_add(PI, crp_dd_neg(a));
+	}
+	const double v = a.hi + a.lo;
+	return ny ? -v : v;
+}
+
 #endif

@@ -26,6 +26,7 @@
 #include "crp_device.hpp"
 #include "crp_propagate.hpp"
 #include "crp_sde.hpp"
+#include "crp_jf12.hpp"
 #include "crp_adjoint.hpp"
 #include "crp_turbulence.hpp"
 #include "crp_probe.hpp"
@@ -52,7 +53,7 @@ static int fail(int code, const std::string &msg) {
 	} while (0)
 
 // ------------------------------------------------------------------ objects
-enum FieldKind { FK_NONE = 0, FK_UNIFORM = 1, FK_GRID = 2, FK_PLANEWAVE = 3, FK_MODULATED = 4 };
+enum FieldKind { FK_NONE = 0, FK_UNIFORM = 1, FK_GRID = 2, FK_PLANEWAVE = 3, FK_MODULATED = 4, FK_JF12 = 5 };
 
 struct crp_field {
 	int kind;
@@ -60,6 +61,7 @@ struct crp_field {
 	FieldUniform uniform;
 	GridDesc grid;
 	GridDesc mod; // FK_MODULATED: the Grid1f modulation, one float per point
+	FieldJF12 jf12; // FK_JF12: parameters and grids
 	PlaneWaveDesc pw;
 	float *cells; // cell-major corner table (periodic grids), or nullptr
 	unsigned char *cells16; // FP16 corner table with brick scales (mixed path), or nullptr
@@ -646,6 +648,118 @@ int crp_field_create_modulated_grid3f(const float *values, int nx, int ny, int n
 	return CRP_OK;
 }
 
+int crp_jf12_default_params(int planck, crp_jf12_params *out) {
+	// JF12Field::JF12Field (src/magneticField/JF12Field.cpp:8-79) and PlanckJF12bField (:348-363),
+	// with the units of include/crpropa/Units.h
+	if (!out)
+		return fail(CRP_E_INVALID, "crp_jf12_default_params: NULL argument");
+	const double tesla = 1.0, gauss = 1e-4 * tesla, muG = 1e-6 * gauss;
+	const double kpc = 1e3 * (648000 / M_PI * 149597870700.0);
+	crp_jf12_params &p = *out;
+	std::memset(&p, 0, sizeof(p));
+	p.use_regular = 1;
+	p.use_disk = 1;
+	p.use_toroidal_halo = 1;
+	p.use_x = 1;
+	volatile double pitchv = 11.5 * M_PI / 180; // computed at run time with the host libm
+	const double pitch = pitchv;
+	p.sin_pitch = std::sin(pitch);
+	p.cos_pitch = std::cos(pitch);
+	volatile double hp = M_PI / 2 - pitch;
+	p.tan90_minus_pitch = std::tan((double)hp);
+	const double arms[8] = {5.1, 6.3, 7.1, 8.3, 9.8, 11.4, 12.7, 15.5};
+	const double disk[8] = {0.1, 3.0, -0.9, -0.8, -2.0, -4.2, 0.0, 2.7};
+	const double turb[8] = {10.81, 6.96, 9.59, 6.96, 1.96, 16.34, 37.29, 10.35};
+	for (int i = 0; i < 8; i++) {
+		p.r_arms[i] = arms[i] * kpc;
+		p.b_disk[i] = disk[i] * muG;
+		p.b_disk_turb[i] = turb[i] * muG;
+	}
+	p.b_ring = 0.1 * muG;
+	p.h_disk = 0.40 * kpc;
+	p.w_disk = 0.27 * kpc;
+	p.b_north = 1.4 * muG;
+	p.b_south = -1.1 * muG;
+	p.r_north = 9.22 * kpc;
+	p.r_south = 17 * kpc;
+	p.w_halo = 0.20 * kpc;
+	p.z0 = 5.3 * kpc;
+	p.b_x = 4.6 * muG;
+	volatile double thv = 49.0 * M_PI / 180;
+	const double thetaX0 = thv;
+	p.sin_theta_x0 = std::sin(thetaX0);
+	p.cos_theta_x0 = std::cos(thetaX0);
+	p.tan_theta_x0 = std::tan(thetaX0);
+	p.r_xc = 4.8 * kpc;
+	p.r_x = 2.9 * kpc;
+	volatile double beta = 1.36;
+	p.sqrt_beta = std::sqrt((double)beta);
+	p.b_disk_turb5 = 7.63 * muG;
+	p.z_disk_turb = 0.61 * kpc;
+	p.b_halo_turb = 4.68 * muG;
+	p.r_halo_turb = 10.97 * kpc;
+	p.z_halo_turb = 2.84 * kpc;
+	if (planck) {
+		const double pt[8] = {3.12, 6.24, 3.12, 6.24, 3.12, 6.24, 3.12, 6.24};
+		p.b_disk[5] = -3.5 * muG;
+		p.b_x = 1.8 * muG;
+		for (int i = 0; i < 8; i++)
+			p.b_disk_turb[i] = pt[i] * muG;
+		p.b_disk_turb5 = 3.90 * muG;
+		p.b_halo_turb = 7.332 * muG;
+	}
+	return CRP_OK;
+}
+
+int crp_field_create_jf12(const crp_jf12_params *p, const float *striated, const int striated_n[3],
+                          const double striated_origin[3], const double striated_spacing[3], int striated_reflective,
+                          const float *turbulent, const int turbulent_n[3], const double turbulent_origin[3],
+                          const double turbulent_spacing[3], int turbulent_reflective, int turbulent_clip,
+                          crp_field_t **out) {
+	if (!p || !out)
+		return fail(CRP_E_INVALID, "crp_field_create_jf12: NULL argument");
+	if (p->use_striated && (!striated || !striated_n))
+		return fail(CRP_E_INVALID, "crp_field_create_jf12: use_striated needs a striated grid");
+	if (p->use_turbulent && (!turbulent || !turbulent_n))
+		return fail(CRP_E_INVALID, "crp_field_create_jf12: use_turbulent needs a turbulent grid");
+	crp_field *f = new_field(FK_JF12);
+	if (!f)
+		return fail(CRP_E_NOMEM, "crp_field_create_jf12: out of host memory");
+	std::memset(&f->jf12, 0, sizeof(f->jf12));
+	f->jf12.p = *p;
+	auto upload = [&](GridDesc &g, const float *v, const int n[3], const double *o, const double *sp, int refl, int clip,
+	                  int comps, int slot) -> int {
+		int rc = fill_grid_desc(g, nullptr, n[0], n[1], n[2], o, sp, refl, clip);
+		if (rc != CRP_OK)
+			return rc;
+		const size_t bytes = (size_t)n[0] * n[1] * n[2] * comps * sizeof(float);
+		void *d = nullptr;
+		hipError_t e = hipMalloc(&d, bytes);
+		if (e == hipSuccess)
+			e = hipMemcpy(d, v, bytes, hipMemcpyHostToDevice);
+		if (e != hipSuccess) {
+			if (d)
+				(void)hipFree(d);
+			return fail(CRP_E_HIP, std::string("crp_field_create_jf12: ") + hipGetErrorString(e));
+		}
+		g.values = (const float *)d;
+		f->owned[slot] = d;
+		return CRP_OK;
+	};
+	int rc = CRP_OK;
+	if (striated && striated_n)
+		rc = upload(f->jf12.striated, striated, striated_n, striated_origin, striated_spacing, striated_reflective, 0, 1, 0);
+	if (rc == CRP_OK && turbulent && turbulent_n)
+		rc = upload(f->jf12.turbulent, turbulent, turbulent_n, turbulent_origin, turbulent_spacing, turbulent_reflective,
+		            turbulent_clip, 3, 1);
+	if (rc != CRP_OK) {
+		(void)crp_field_destroy(f);
+		return rc;
+	}
+	*out = f;
+	return CRP_OK;
+}
+
 int crp_field_set_grid_layout(crp_field_t *f, int layout) {
 	if (!f || (f->kind != FK_GRID && f->kind != FK_MODULATED))
 		return fail(CRP_E_INVALID, "crp_field_set_grid_layout: not a grid field");
@@ -744,6 +858,7 @@ int crp_field_eval_ex(const crp_field_t *f, const double *pos, size_t n, double 
 				hipLaunchKernelGGL(k_field_evalf<FieldModulated<FieldGrid>>, blocks, 256, 0, 0,
 				                   FieldModulated<FieldGrid>{FieldGrid{f->grid}, f->mod}, dpos, dB, n);
 			break;
+		case FK_JF12: hipLaunchKernelGGL(k_field_evalf<FieldJF12>, blocks, 256, 0, 0, f->jf12, dpos, dB, n); break;
 		default: hipLaunchKernelGGL(k_field_evalf<FieldNone>, blocks, 256, 0, 0, FieldNone{}, dpos, dB, n); break;
 		}
 		e = hipGetLastError();
@@ -869,6 +984,7 @@ int crp_field_eval(const crp_field_t *f, const double *pos, size_t n, double *B)
 				hipLaunchKernelGGL(k_field_eval<FieldModulated<FieldGrid>>, blocks, 256, 0, 0,
 				                   FieldModulated<FieldGrid>{FieldGrid{f->grid}, f->mod}, dpos, dB, n);
 			break;
+		case FK_JF12: hipLaunchKernelGGL(k_field_eval<FieldJF12>, blocks, 256, 0, 0, f->jf12, dpos, dB, n); break;
 		default: hipLaunchKernelGGL(k_field_eval<FieldNone>, blocks, 256, 0, 0, FieldNone{}, dpos, dB, n); break;
 		}
 		e = hipGetLastError();
@@ -1421,6 +1537,7 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 		                              &st.launches)
 		              : dispatch_kind(FieldModulated<FieldGrid>{FieldGrid{f->grid}, f->mod}, p, P, b, &st.launches);
 		break;
+	case FK_JF12: rc = dispatch_kind(f->jf12, p, P, b, &st.launches); break;
 	default: rc = dispatch_kind(FieldNone{}, p, P, b, &st.launches); break;
 	}
 	if (rc != CRP_OK)

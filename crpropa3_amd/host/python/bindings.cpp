@@ -28,6 +28,7 @@
 #include "crpropa/module/PropagationBP.h"
 #include "crpropa/module/PropagationCK.h"
 #include "crpropa/module/DiffusionSDE.h"
+#include "crpropa/magneticField/JF12Field.h"
 #include "crpropa/advectionField/AdvectionField.h"
 
 namespace py = pybind11;
@@ -421,6 +422,32 @@ PYBIND11_MODULE(_crpropa, m) {
 	py::class_<PropagationCK, DevicePropagator, ref_ptr<PropagationCK>>(m, "PropagationCK")
 	    .def(py::init<ref_ptr<MagneticField>, double, double, double>(), py::arg("field") = ref_ptr<MagneticField>(),
 	         py::arg("tolerance") = 1e-4, py::arg("minStep") = 0.1 * kpc, py::arg("maxStep") = 1 * Gpc);
+
+	// ---- JF12 (src/magneticField/JF12Field.cpp)
+	py::class_<JF12Field, MagneticField, ref_ptr<JF12Field>>(m, "JF12Field")
+	    .def(py::init<>())
+	    .def("randomStriated", &JF12Field::randomStriated, py::arg("seed") = 0)
+	    .def("randomTurbulent", &JF12Field::randomTurbulent, py::arg("seed") = 0)
+	    .def("setStriatedGrid", &JF12Field::setStriatedGrid)
+	    .def("setTurbulentGrid", &JF12Field::setTurbulentGrid)
+	    .def("getStriatedGrid", &JF12Field::getStriatedGrid)
+	    .def("getTurbulentGrid", &JF12Field::getTurbulentGrid)
+	    .def("setUseRegularField", &JF12Field::setUseRegularField)
+	    .def("setUseDiskField", &JF12Field::setUseDiskField)
+	    .def("setUseToroidalHaloField", &JF12Field::setUseToroidalHaloField)
+	    .def("setUseXField", &JF12Field::setUseXField)
+	    .def("setUseStriatedField", &JF12Field::setUseStriatedField)
+	    .def("setUseTurbulentField", &JF12Field::setUseTurbulentField)
+	    .def("isUsingRegularField", &JF12Field::isUsingRegularField)
+	    .def("isUsingDiskField", &JF12Field::isUsingDiskField)
+	    .def("isUsingToroidalHaloField", &JF12Field::isUsingToroidalHaloField)
+	    .def("isUsingXField", &JF12Field::isUsingXField)
+	    .def("isUsingStriatedField", &JF12Field::isUsingStriatedField)
+	    .def("isUsingTurbulentField", &JF12Field::isUsingTurbulentField)
+	    .def("getRegularField", &JF12Field::getRegularField)
+	    .def("getStriatedField", &JF12Field::getStriatedField)
+	    .def("getTurbulentField", &JF12Field::getTurbulentField);
+	py::class_<PlanckJF12bField, JF12Field, ref_ptr<PlanckJF12bField>>(m, "PlanckJF12bField").def(py::init<>());
 
 	// ---- advection fields and DiffusionSDE
 	py::class_<AdvectionField, ref_ptr<AdvectionField>>(m, "AdvectionField")

@@ -4,11 +4,16 @@
 #include "crpropa/Random.h"
 #include "crpropa/magneticField/MagneticField.h"
 #include "crpropa/magneticField/MagneticFieldGrid.h"
+#include "crpropa/magneticField/JF12Field.h"
+#include "crpropa/magneticField/turbulentField/GridTurbulence.h"
+#include "crpropa/Units.h"
 #include "crpropa/magneticField/turbulentField/PlaneWaveTurbulence.h"
 
 #include "crpropa3_amd.h"
 
 #include <cmath>
+#include <cstring>
+#include <iostream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -260,6 +265,136 @@ Vector3d ModulatedMagneticFieldGrid::getField(const Vector3d &position) const {
 	if (!d)
 		throw std::runtime_error("ModulatedMagneticFieldGrid: grid or modulation grid not set");
 	throwIf(crp_field_eval(d, p, 1, B));
+	return Vector3d(B[0], B[1], B[2]);
+}
+
+} // namespace crpropa
+
+// ------------------------------------------------------------------ JF12Field
+namespace crpropa {
+
+JF12Field::JF12Field() : JF12Field(0) {}
+
+JF12Field::JF12Field(int planck) {
+	if (crp_jf12_default_params(planck, &p) != CRP_OK)
+		throw std::runtime_error(crp_last_error());
+}
+
+JF12Field::~JF12Field() {
+	if (dev)
+		crp_field_destroy(dev);
+}
+
+void JF12Field::randomStriated(int seed) {
+	// JF12Field.cpp:81-94: 100^3 points at 0.1 kpc, round(rand()) * 2 - 1 in storage order
+	const int N = 100;
+	ref_ptr<Grid1f> g = new Grid1f(Vector3d(0.), N, 0.1 * kpc);
+	Random random;
+	if (seed != 0)
+		random.seed(seed);
+	for (int ix = 0; ix < N; ix++)
+		for (int iy = 0; iy < N; iy++)
+			for (int iz = 0; iz < N; iz++)
+				g->get(ix, iy, iz) = (float)(std::round(random.rand()) * 2 - 1);
+	setStriatedGrid(g);
+}
+
+void JF12Field::randomTurbulent(int seed) {
+	// JF12Field.cpp:97-108: SimpleGridTurbulence(Brms 1, lMin 8 pc, lMax 272 pc) on 256^3 points at 4 pc
+	ref_ptr<Grid3f> g = new Grid3f(Vector3d(0.), 256, 4 * pc);
+	initTurbulence(g, 1, 8 * pc, 272 * pc, -11. / 3., seed);
+	setTurbulentGrid(g);
+}
+
+void JF12Field::setStriatedGrid(ref_ptr<Grid1f> grid) {
+	p.use_striated = 1;
+	striatedGrid = grid;
+}
+
+void JF12Field::setTurbulentGrid(ref_ptr<Grid3f> grid) {
+	p.use_turbulent = 1;
+	turbulentGrid = grid;
+}
+
+void JF12Field::setUseRegularField(bool use) { p.use_regular = use; }
+void JF12Field::setUseDiskField(bool use) { p.use_disk = use; }
+void JF12Field::setUseToroidalHaloField(bool use) { p.use_toroidal_halo = use; }
+void JF12Field::setUseXField(bool use) { p.use_x = use; }
+
+void JF12Field::setUseStriatedField(bool use) {
+	if (use && !striatedGrid.valid()) {
+		std::cerr << "JF12Field: No striated field set: ignored. Run e.g. randomStriated()." << std::endl;
+		return;
+	}
+	p.use_striated = use;
+}
+
+void JF12Field::setUseTurbulentField(bool use) {
+	if (use && !turbulentGrid.valid()) {
+		std::cerr << "JF12Field: No turbulent field set: ignored. Run e.g. randomTurbulent()." << std::endl;
+		return;
+	}
+	p.use_turbulent = use;
+}
+
+crp_field *JF12Field::deviceField() const {
+	std::lock_guard<std::mutex> lock(mtx);
+	const Grid1f *s = p.use_striated ? striatedGrid.get() : nullptr;
+	const Grid3f *t = p.use_turbulent ? turbulentGrid.get() : nullptr;
+	if (dev && std::memcmp(&devParams, &p, sizeof(p)) == 0 && devStr == s && devTurb == t &&
+	    (!s || devStrVersion == s->getVersion()) && (!t || devTurbVersion == t->getVersion()))
+		return dev;
+	if (dev) {
+		crp_field_destroy(dev);
+		dev = nullptr;
+	}
+	int sn[3] = {0, 0, 0}, tn[3] = {0, 0, 0};
+	double so[3] = {0, 0, 0}, ss[3] = {1, 1, 1}, to[3] = {0, 0, 0}, ts[3] = {1, 1, 1};
+	if (s) {
+		const Vector3d o = s->getOrigin(), sp = s->getSpacing();
+		sn[0] = (int)s->getNx(); sn[1] = (int)s->getNy(); sn[2] = (int)s->getNz();
+		so[0] = o.x; so[1] = o.y; so[2] = o.z;
+		ss[0] = sp.x; ss[1] = sp.y; ss[2] = sp.z;
+	}
+	if (t) {
+		const Vector3d o = t->getOrigin(), sp = t->getSpacing();
+		tn[0] = (int)t->getNx(); tn[1] = (int)t->getNy(); tn[2] = (int)t->getNz();
+		to[0] = o.x; to[1] = o.y; to[2] = o.z;
+		ts[0] = sp.x; ts[1] = sp.y; ts[2] = sp.z;
+	}
+	crp_field *f = nullptr;
+	throwIf(crp_field_create_jf12(&p, s ? s->getGrid().data() : nullptr, s ? sn : nullptr, so, ss,
+	                              s && s->isReflective() ? 1 : 0,
+	                              t ? reinterpret_cast<const float *>(t->getGrid().data()) : nullptr, t ? tn : nullptr,
+	                              to, ts, t && t->isReflective() ? 1 : 0, t && t->getClipVolume() ? 1 : 0, &f));
+	dev = f;
+	devParams = p;
+	devStr = s;
+	devTurb = t;
+	devStrVersion = s ? s->getVersion() : 0;
+	devTurbVersion = t ? t->getVersion() : 0;
+	return dev;
+}
+
+Vector3d JF12Field::evalWith(const Vector3d &pos, int regular, int striated, int turbulent) const {
+	JF12Field tmp(0);
+	tmp.p = p;
+	tmp.p.use_regular = regular;
+	tmp.p.use_striated = striated && striatedGrid.valid();
+	tmp.p.use_turbulent = turbulent && turbulentGrid.valid();
+	tmp.striatedGrid = striatedGrid;
+	tmp.turbulentGrid = turbulentGrid;
+	return tmp.getField(pos);
+}
+
+Vector3d JF12Field::getRegularField(const Vector3d &pos) const { return evalWith(pos, 1, 0, 0); }
+Vector3d JF12Field::getStriatedField(const Vector3d &pos) const { return evalWith(pos, 1, 1, 0); }
+Vector3d JF12Field::getTurbulentField(const Vector3d &pos) const { return evalWith(pos, 0, 0, 1); }
+
+Vector3d JF12Field::getField(const Vector3d &pos) const {
+	const double x[3] = {pos.x, pos.y, pos.z};
+	double B[3];
+	evaluateOnDevice(*this, x, 1, B);
 	return Vector3d(B[0], B[1], B[2]);
 }
 

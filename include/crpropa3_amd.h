@@ -117,6 +117,28 @@ int crp_field_destroy(crp_field_t *f);
 /* B at n host positions (AoS xyz), evaluated on the device with the FP64 path. */
 int crp_field_eval(const crp_field_t *f, const double *pos, size_t n, double *B);
 
+/* JF12Field / PlanckJF12bField (src/magneticField/JF12Field.cpp): the model's constants as the
+ * reference's constructors set them (JF12Field.cpp:8-79, 348-363) and its switches. */
+typedef struct {
+	int use_regular, use_disk, use_toroidal_halo, use_x, use_striated, use_turbulent;
+	double r_arms[8], b_disk[8], sin_pitch, cos_pitch, tan90_minus_pitch;
+	double b_ring, h_disk, w_disk, b_north, b_south, r_north, r_south, w_halo, z0;
+	double b_x, sin_theta_x0, cos_theta_x0, tan_theta_x0, r_xc, r_x, sqrt_beta;
+	double b_disk_turb[8], b_disk_turb5, z_disk_turb, b_halo_turb, r_halo_turb, z_halo_turb;
+} crp_jf12_params;
+/* the constructor's values (planck = 0: JF12Field, 1: PlanckJF12bField), computed with the host's
+ * libm like the reference's constructor; regular field on, striated and turbulent off */
+int crp_jf12_default_params(int planck, crp_jf12_params *out);
+/* JF12Field::getField (JF12Field.cpp:230-346) on the device: the regular field, times
+ * (1 + sqrt(beta) x the striated Grid1f's closest value) when use_striated, plus the turbulent
+ * Grid3f's trilinear value times the turbulent strength when use_turbulent.  The grids are host
+ * arrays (one float per point; (x,y,z) floats per point), NULL when unused. */
+int crp_field_create_jf12(const crp_jf12_params *p, const float *striated, const int striated_n[3],
+                          const double striated_origin[3], const double striated_spacing[3], int striated_reflective,
+                          const float *turbulent, const int turbulent_n[3], const double turbulent_origin[3],
+                          const double turbulent_spacing[3], int turbulent_reflective, int turbulent_clip,
+                          crp_field_t **out);
+
 /* ---- propagators ------------------------------------------------------ */
 
 /* PropagationBP(field, tolerance, minStep, maxStep) -- src/module/PropagationBP.cpp:43-59,162-183.

@@ -1,3 +1,4 @@
+#define _GNU_SOURCE /* sincos */
 /*
  * ORACLE -- test infrastructure, not product code (see crp_oracle.h).
  *
@@ -26,6 +27,8 @@ static const double C_LIGHT = 2.99792458e8;
 /* ------------------------------------------------------------------ libm */
 double orc_cos(double x, int libm) { return libm == ORC_LIBM_CR ? crp_cos_cr(x) : cos(x); }
 double orc_pow(double x, double y, int libm) { return libm == ORC_LIBM_CR ? crp_pow_cr(x, y) : pow(x, y); }
+static double orc_exp(double x, int libm) { return libm == ORC_LIBM_CR ? crp_exp_rn(x) : exp(x); }
+static double orc_atan2(double y, double x, int libm) { return libm == ORC_LIBM_CR ? crp_atan2_rn(y, x) : atan2(y, x); }
 
 void orc_cos_batch(const double *x, size_t n, double *out, int libm) {
 	for (size_t i = 0; i < n; i++)
@@ -325,6 +328,226 @@ static void orc_planewave_eval1(const orc_field *f, const double p[3], double B[
 	B[2] = bz;
 }
 
+/* ---------------------------------------------------------------- JF12 */
+static const double J_KPC = 1e3 * (648000 / M_PI * 149597870700.0);
+
+void orc_jf12_defaults(int planck, crp_jf12_params *p) {
+	const double muG = 1e-6 * (1e-4 * 1.0);
+	memset(p, 0, sizeof(*p));
+	p->use_regular = p->use_disk = p->use_toroidal_halo = p->use_x = 1;
+	const double pitch = 11.5 * M_PI / 180;
+	p->sin_pitch = sin(pitch);
+	p->cos_pitch = cos(pitch);
+	p->tan90_minus_pitch = tan(M_PI / 2 - pitch);
+	const double arms[8] = {5.1, 6.3, 7.1, 8.3, 9.8, 11.4, 12.7, 15.5};
+	const double disk[8] = {0.1, 3.0, -0.9, -0.8, -2.0, -4.2, 0.0, 2.7};
+	const double turb[8] = {10.81, 6.96, 9.59, 6.96, 1.96, 16.34, 37.29, 10.35};
+	for (int i = 0; i < 8; i++) {
+		p->r_arms[i] = arms[i] * J_KPC;
+		p->b_disk[i] = disk[i] * muG;
+		p->b_disk_turb[i] = turb[i] * muG;
+	}
+	p->b_ring = 0.1 * muG; p->h_disk = 0.40 * J_KPC; p->w_disk = 0.27 * J_KPC;
+	p->b_north = 1.4 * muG; p->b_south = -1.1 * muG; p->r_north = 9.22 * J_KPC; p->r_south = 17 * J_KPC;
+	p->w_halo = 0.20 * J_KPC; p->z0 = 5.3 * J_KPC;
+	p->b_x = 4.6 * muG;
+	const double thetaX0 = 49.0 * M_PI / 180;
+	p->sin_theta_x0 = sin(thetaX0); p->cos_theta_x0 = cos(thetaX0); p->tan_theta_x0 = tan(thetaX0);
+	p->r_xc = 4.8 * J_KPC; p->r_x = 2.9 * J_KPC;
+	p->sqrt_beta = sqrt(1.36);
+	p->b_disk_turb5 = 7.63 * muG; p->z_disk_turb = 0.61 * J_KPC;
+	p->b_halo_turb = 4.68 * muG; p->r_halo_turb = 10.97 * J_KPC; p->z_halo_turb = 2.84 * J_KPC;
+	if (planck) {
+		const double pt[8] = {3.12, 6.24, 3.12, 6.24, 3.12, 6.24, 3.12, 6.24};
+		p->b_disk[5] = -3.5 * muG;
+		p->b_x = 1.8 * muG;
+		for (int i = 0; i < 8; i++)
+			p->b_disk_turb[i] = pt[i] * muG;
+		p->b_disk_turb5 = 3.90 * muG;
+		p->b_halo_turb = 7.332 * muG;
+	}
+}
+
+/* Grid<float>::closestValue (include/crpropa/Grid.h:383-405) on the modulation-grid slots */
+static float orc_grid1_closest(const orc_field *f, const double p[3]) {
+	const int n[3] = {f->mnx, f->mny, f->mnz};
+	int ix[3];
+	for (int a = 0; a < 3; a++) {
+		const double go = f->morigin[a] + f->mspacing[a] / 2;
+		const double r = (p[a] - go) / f->mspacing[a];
+		if (f->mreflective) {
+			int i = (int)round(r);
+			while ((i < -0.5) || (i > (n[a] - 0.5)))
+				i = 2 * n[a] * (i > (n[a] - 0.5)) - i - 1;
+			ix[a] = i;
+		} else {
+			int i = (int)round(fmod(r, n[a]));
+			ix[a] = (i + n[a] * (i < 0)) % n[a];
+		}
+	}
+	return f->mod[((size_t)ix[0] * f->mny + ix[1]) * f->mnz + ix[2]];
+}
+
+/* pow(x, 2.) as the reference's build evaluates it: GCC rewrites the call as x * x (the correctly
+ * rounded square) */
+static double jf12_sq(double x) { return x * x; }
+
+/* sin and cos of one angle: GCC merges the reference's two calls into one sincos() */
+static void jf12_sincos(double x, double *s, double *c, int libm) {
+	if (libm == ORC_LIBM_CR) {
+		*s = crp_sin_cr(x);
+		*c = crp_cos_cr(x);
+	} else {
+		sincos(x, s, c);
+	}
+}
+
+static double jf12_logistic(double x, double x0, double w, int libm) { /* JF12Field.cpp:188-190 */
+	return 1. / (1. + orc_exp(-2. * (fabs(x) - x0) / w, libm));
+}
+
+static double jf12_phi(const double p[3], int libm) { /* Vector3::getPhi, include/crpropa/Vector3.h:136-142 */
+	const double eps = 2.2250738585072014e-308;
+	if ((fabs(p[0]) < eps) && (fabs(p[1]) < eps))
+		return 0.0;
+	return orc_atan2(p[1], p[0], libm);
+}
+
+static double jf12_rnegx(const crp_jf12_params *J, double r, double phi, int libm) {
+	double rn = r * orc_exp(-(phi - M_PI) / J->tan90_minus_pitch, libm);
+	if (rn > J->r_arms[7])
+		rn = r * orc_exp(-(phi + M_PI) / J->tan90_minus_pitch, libm);
+	if (rn > J->r_arms[7])
+		rn = r * orc_exp(-(phi + 3 * M_PI) / J->tan90_minus_pitch, libm);
+	return rn;
+}
+
+/* JF12Field::getRegularField, getDiskField, getToroidalHaloField, getXField (JF12Field.cpp:192-300) */
+static void jf12_regular(const crp_jf12_params *J, const double p[3], double B[3], int libm) {
+	double b[3] = {0, 0, 0};
+	const double d = sqrt((p[0] * p[0] + p[1] * p[1]) + p[2] * p[2]);
+	if (d < 20 * J_KPC) {
+		const double r = sqrt(p[0] * p[0] + p[1] * p[1]);
+		const double phi = jf12_phi(p, libm);
+		double sinPhi, cosPhi;
+		jf12_sincos(phi, &sinPhi, &cosPhi, libm);
+		const double z = p[2];
+		double dk[3] = {0, 0, 0};
+		if (J->use_disk) {
+			const double lfDisk = jf12_logistic(z, J->h_disk, J->w_disk, libm);
+			if (r > 3 * J_KPC) {
+				double bMag = 0;
+				if (r < 5 * J_KPC) {
+					bMag = J->b_ring * (5 * J_KPC / r) * (1 - lfDisk);
+					dk[0] += -bMag * sinPhi;
+					dk[1] += bMag * cosPhi;
+				} else {
+					const double rn = jf12_rnegx(J, r, phi, libm);
+					for (int i = 7; i >= 0; i--)
+						if (rn < J->r_arms[i])
+							bMag = J->b_disk[i];
+					bMag *= (5 * J_KPC / r) * (1 - lfDisk);
+					dk[0] += bMag * (J->sin_pitch * cosPhi - J->cos_pitch * sinPhi);
+					dk[1] += bMag * (J->sin_pitch * sinPhi + J->cos_pitch * cosPhi);
+				}
+			}
+		}
+		for (int a = 0; a < 3; a++)
+			b[a] += dk[a];
+		double h[3] = {0, 0, 0};
+		if (J->use_toroidal_halo && (r * r + z * z > 1 * J_KPC * J_KPC)) {
+			const double lfDisk = jf12_logistic(z, J->h_disk, J->w_disk, libm);
+			double bMagH = orc_exp(-fabs(z) / J->z0, libm) * lfDisk;
+			if (z >= 0)
+				bMagH *= J->b_north * (1 - jf12_logistic(r, J->r_north, J->w_halo, libm));
+			else
+				bMagH *= J->b_south * (1 - jf12_logistic(r, J->r_south, J->w_halo, libm));
+			h[0] += -bMagH * sinPhi;
+			h[1] += bMagH * cosPhi;
+		}
+		for (int a = 0; a < 3; a++)
+			b[a] += h[a];
+		double x[3] = {0, 0, 0};
+		if (J->use_x && (r * r + z * z > 1 * J_KPC * J_KPC)) {
+			double bMagX, sinThetaX, cosThetaX, rp;
+			const double rc = J->r_xc + fabs(z) / J->tan_theta_x0;
+			if (r < rc) {
+				rp = r * J->r_xc / rc;
+				bMagX = J->b_x * orc_exp(-1 * rp / J->r_x, libm) * jf12_sq(J->r_xc / rc);
+				double thetaX = orc_atan2(fabs(z), (r - rp), libm);
+				if (z == 0)
+					thetaX = M_PI / 2.;
+				jf12_sincos(thetaX, &sinThetaX, &cosThetaX, libm);
+			} else {
+				rp = r - fabs(z) / J->tan_theta_x0;
+				bMagX = J->b_x * orc_exp(-rp / J->r_x, libm) * (rp / r);
+				sinThetaX = J->sin_theta_x0;
+				cosThetaX = J->cos_theta_x0;
+			}
+			const double zsign = z < 0 ? -1 : 1;
+			x[0] += zsign * bMagX * cosThetaX * cosPhi;
+			x[1] += zsign * bMagX * cosThetaX * sinPhi;
+			x[2] += bMagX * sinThetaX;
+		}
+		for (int a = 0; a < 3; a++)
+			b[a] += x[a];
+	}
+	B[0] = b[0];
+	B[1] = b[1];
+	B[2] = b[2];
+}
+
+/* JF12Field::getTurbulentStrength (JF12Field.cpp:307-331) */
+static double jf12_turb_strength(const crp_jf12_params *J, const double p[3], int libm) {
+	if (sqrt((p[0] * p[0] + p[1] * p[1]) + p[2] * p[2]) > 20 * J_KPC)
+		return 0;
+	const double r = sqrt(p[0] * p[0] + p[1] * p[1]);
+	const double phi = jf12_phi(p, libm);
+	double bDisk = 0;
+	if (r < 5 * J_KPC) {
+		bDisk = J->b_disk_turb5;
+	} else {
+		const double rn = jf12_rnegx(J, r, phi, libm);
+		for (int i = 7; i >= 0; i--)
+			if (rn < J->r_arms[i])
+				bDisk = J->b_disk_turb[i];
+		bDisk *= (5 * J_KPC) / r;
+	}
+	bDisk *= orc_exp(-0.5 * jf12_sq(p[2] / J->z_disk_turb), libm);
+	const double bHalo = J->b_halo_turb * orc_exp(-r / J->r_halo_turb, libm) *
+	                     orc_exp(-0.5 * jf12_sq(p[2] / J->z_halo_turb), libm);
+	return sqrt(jf12_sq(bDisk) + jf12_sq(bHalo));
+}
+
+/* JF12Field::getField (JF12Field.cpp:337-346): turbulent (Vector3f * float(strength) in float),
+ * then striated (regular * (1 + sqrt(beta) * closest value)) or regular */
+static void orc_jf12_eval1(const orc_field *f, const double p[3], double B[3], int libm) {
+	const crp_jf12_params *J = f->jf12;
+	double b[3] = {0, 0, 0};
+	if (J->use_turbulent) {
+		double g[3];
+		orc_grid_eval1(f, p, g);
+		const float s = (float)jf12_turb_strength(J, p, libm);
+		for (int a = 0; a < 3; a++)
+			b[a] += (double)((float)g[a] * s);
+	}
+	if (J->use_striated) {
+		double rg[3];
+		jf12_regular(J, p, rg, libm);
+		const double m = 1. + J->sqrt_beta * (double)orc_grid1_closest(f, p);
+		for (int a = 0; a < 3; a++)
+			b[a] += rg[a] * m;
+	} else if (J->use_regular) {
+		double rg[3];
+		jf12_regular(J, p, rg, libm);
+		for (int a = 0; a < 3; a++)
+			b[a] += rg[a];
+	}
+	B[0] = b[0];
+	B[1] = b[1];
+	B[2] = b[2];
+}
+
 /* MagneticField::getField dispatch + PropagationBP/CK::getFieldAtPosition
  * (src/module/PropagationBP.cpp:137-149): a missing field gives B = 0. */
 static inline void orc_field_eval1(const orc_field *f, const double p[3], double B[3], int libm) {
@@ -333,6 +556,7 @@ static inline void orc_field_eval1(const orc_field *f, const double p[3], double
 	case ORC_FIELD_GRID: orc_grid_eval1(f, p, B); break;
 	case ORC_FIELD_PLANEWAVE: orc_planewave_eval1(f, p, B, libm); break;
 	case ORC_FIELD_MODULATED: orc_modulated_eval1(f, p, B); break;
+	case ORC_FIELD_JF12: orc_jf12_eval1(f, p, B, libm); break;
 	default: B[0] = B[1] = B[2] = 0; break;
 	}
 }
@@ -966,6 +1190,13 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 	crp_philox_ctr o = crp_philox4x32_10(c, key[0], key[1]);
 	for (int i = 0; i < 4; i++)
 		out[i] = o.v[i];
+}
+
+void orc_exp_atan2_batch(const double *x, const double *y, size_t n, double *exp_out, double *atan2_out) {
+	for (size_t i = 0; i < n; i++) {
+		exp_out[i] = crp_exp_rn(x[i]);
+		atan2_out[i] = crp_atan2_rn(y[i], x[i]);
+	}
 }
 
 void orc_sin_log_batch(const double *x, size_t n, double *sin_out, double *log_out) {

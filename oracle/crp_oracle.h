@@ -19,12 +19,15 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "../include/crpropa3_amd.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
 
 enum { ORC_LIBM_GLIBC = 0, ORC_LIBM_CR = 1 };
-enum { ORC_FIELD_NONE = 0, ORC_FIELD_UNIFORM = 1, ORC_FIELD_GRID = 2, ORC_FIELD_PLANEWAVE = 3, ORC_FIELD_MODULATED = 4 };
+enum { ORC_FIELD_NONE = 0, ORC_FIELD_UNIFORM = 1, ORC_FIELD_GRID = 2, ORC_FIELD_PLANEWAVE = 3, ORC_FIELD_MODULATED = 4,
+       ORC_FIELD_JF12 = 5 };
 enum { ORC_PROP_BP = 0, ORC_PROP_CK = 1 };
 /* modules after the propagator, for orc_propagator.order */
 enum { ORC_MOD_BOX = 1, ORC_MOD_SPHERE = 2, ORC_MOD_MAXLEN = 3, ORC_MOD_OBSERVER = 4 };
@@ -45,7 +48,13 @@ typedef struct {
 	int mnx, mny, mnz;
 	double morigin[3], mspacing[3];
 	int mreflective, mclip;
+	/* JF12Field: the model's constants; the turbulent Grid3f is the grid above, the striated Grid1f
+	 * the modulation grid's slots */
+	const crp_jf12_params *jf12;
 } orc_field;
+
+/* JF12Field / PlanckJF12bField constructor values (src/magneticField/JF12Field.cpp:8-79,348-363) */
+void orc_jf12_defaults(int planck, crp_jf12_params *p);
 
 typedef struct {
 	int type; /* ORC_PROP_BP / ORC_PROP_CK */
@@ -129,6 +138,9 @@ long long orc_sde_propagate(const orc_field *f, const orc_sde *p, orc_state *s, 
 /* ModuleList::run with OpenMP: one MT19937 per thread (seed + thread number), glibc libm (CPU baseline) */
 long long orc_sde_propagate_threads(const orc_field *f, const orc_sde *p, orc_state *s, size_t n, long long maxSteps,
                                     int nthreads);
+
+/* correctly rounded exp(x[i]) and atan2(y[i], x[i]) of crp_crmath.h (tests) */
+void orc_exp_atan2_batch(const double *x, const double *y, size_t n, double *exp_out, double *atan2_out);
 
 /* Philox4x32-10 of one counter and key (known-answer tests); out[4] */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -40,6 +40,7 @@
 #include "crpropa/module/PropagationCK.h"
 #include "crpropa/module/DiffusionSDE.h"
 #include "crpropa/advectionField/AdvectionField.h"
+#include "crpropa/magneticField/JF12Field.h"
 
 #include <cstdio>
 #include <cstdint>
@@ -521,6 +522,91 @@ static void gen_modulated() {
 	printf("  modulated16 M=%zu\n", M);
 }
 
+// ---------------------------------------------------------------- JF12
+// JF12Field samples (regular parts, striated from randomStriated(42), a 16^3 noise grid as the
+// turbulent grid at 4 pc, all together, PlanckJF12bField) and one Cash-Karp trajectory case of
+// charged nuclei from Earth through the full model.
+static void gen_jf12() {
+	Random rng(77);
+	const size_t M = 4096;
+	std::vector<double> pos(M * 3);
+	for (size_t i = 0; i < M; i++) {
+		double x, y, z;
+		switch (i % 8) {
+		case 0: case 1: case 2:
+			x = rng.randUniform(-22, 22) * kpc; y = rng.randUniform(-22, 22) * kpc; z = rng.randUniform(-22, 22) * kpc;
+			break;
+		case 3: case 4: // the disk
+			x = rng.randUniform(-20, 20) * kpc; y = rng.randUniform(-20, 20) * kpc; z = rng.randUniform(-1, 1) * kpc;
+			break;
+		case 5: { // the ring 3 < r < 5 kpc and inside 3 kpc
+			double r = rng.randUniform(0, 5.5) * kpc, ph = rng.randUniform(-M_PI, M_PI);
+			x = r * cos(ph); y = r * sin(ph); z = rng.randUniform(-2, 2) * kpc;
+			break;
+		}
+		case 6: // z = 0 exactly, and on the z axis
+			x = rng.randUniform(-20, 20) * kpc; y = rng.randUniform(-20, 20) * kpc; z = 0;
+			if (i % 16 == 14) { x = 0; y = 0; z = rng.randUniform(-15, 15) * kpc; }
+			break;
+		default: // the X-field's inner region r < rc, near the plane
+			x = rng.randUniform(-6, 6) * kpc; y = rng.randUniform(-6, 6) * kpc; z = rng.randUniform(-3, 3) * kpc;
+			break;
+		}
+		pos[3 * i] = x; pos[3 * i + 1] = y; pos[3 * i + 2] = z;
+	}
+	std::string dir = casedir("jf12");
+	save_f64(dir, "pos", pos, {M, 3});
+	ref_ptr<Grid3f> turb = make_noise_grid(16, 4 * pc, Vector3d(0.), 555, 1.0);
+	save_grid(dir, turb);
+	auto sample = [&](const MagneticField &f, const char *name) {
+		std::vector<double> B(M * 3);
+		for (size_t i = 0; i < M; i++) {
+			Vector3d b = f.getField(Vector3d(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]));
+			B[3 * i] = b.x; B[3 * i + 1] = b.y; B[3 * i + 2] = b.z;
+		}
+		save_f64(dir, name, B, {M, 3});
+	};
+	JF12Field reg;
+	sample(reg, "B_regular");
+	const char *parts[3] = {"B_disk", "B_toroidal", "B_x"};
+	for (int k = 0; k < 3; k++) {
+		JF12Field f;
+		f.setUseDiskField(k == 0);
+		f.setUseToroidalHaloField(k == 1);
+		f.setUseXField(k == 2);
+		sample(f, parts[k]);
+	}
+	JF12Field str;
+	str.randomStriated(42);
+	sample(str, "B_striated");
+	JF12Field tur;
+	tur.setTurbulentGrid(turb);
+	tur.setUseRegularField(false);
+	sample(tur, "B_turbulent");
+	JF12Field full;
+	full.randomStriated(42);
+	full.setTurbulentGrid(turb);
+	sample(full, "B_full");
+	PlanckJF12bField planck;
+	planck.randomStriated(42);
+	planck.setTurbulentGrid(turb);
+	sample(planck, "B_planck");
+	save_meta(dir, "{\"striated_seed\": 42, \"turb_n\": 16, \"turb_spacing\": " + fmt(4 * pc) + "}");
+	printf("  jf12 M=%zu\n", M);
+
+	// nuclei (A = 60, Z = 1: no mass table needed) from Earth, 1-10 EeV, through the full model
+	std::vector<Particle> ps = make_particles(64, 31, 1 * EeV, 10 * EeV, Vector3d(-8.5 * kpc, 0, 0), {nucleusId(60, 1)}, {1.0});
+	ref_ptr<JF12Field> fm = new JF12Field();
+	fm->randomStriated(42);
+	fm->setTurbulentGrid(turb);
+	PropagationCK ck(fm, 1e-4, 0.1 * pc, 1 * kpc);
+	MaximumTrajectoryLength ml(25 * kpc);
+	run_traj("ck_jf12", ck, &ml, ps, 100000, {10, 50, 200, 100000},
+	         "{\"propagator\": \"CK\", \"tolerance\": 1e-4, \"minStep\": " + fmt(0.1 * pc) + ", \"maxStep\": " +
+	             fmt(1 * kpc) + ", \"maxTrajectoryLength\": " + fmt(25 * kpc) +
+	             ", \"field\": \"jf12_full\", \"striated_seed\": 42}");
+}
+
 // ---------------------------------------------------------------- DiffusionSDE
 // Serial run with Random::instance() seeded once (omp thread 0): particle i takes all its steps
 // before particle i+1, [DiffusionSDE, MaximumTrajectoryLength] per step.
@@ -705,6 +791,7 @@ int main(int argc, char **argv) {
 	gen_trilinear();
 	gen_modulated();
 	gen_sde();
+	gen_jf12();
 	gen_gridtools();
 
 	// Plane waves: the C3 mode table (Simple spectrum), the reference test's

@@ -14,7 +14,7 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 LIBM_GLIBC = 0
 LIBM_CR = 1
-FIELD_NONE, FIELD_UNIFORM, FIELD_GRID, FIELD_PLANEWAVE, FIELD_MODULATED = 0, 1, 2, 3, 4
+FIELD_NONE, FIELD_UNIFORM, FIELD_GRID, FIELD_PLANEWAVE, FIELD_MODULATED, FIELD_JF12 = 0, 1, 2, 3, 4, 5
 PROP_BP, PROP_CK = 0, 1
 MOD_BOX, MOD_SPHERE, MOD_MAXLEN, MOD_OBSERVER = 1, 2, 3, 4  # orc_propagator.order
 
@@ -50,7 +50,21 @@ class Field(C.Structure):
         ("k", _dp), ("Ak", _dp), ("xi", _dp), ("kappa", _dp), ("beta", _dp),
         ("mod", _fp), ("mnx", C.c_int), ("mny", C.c_int), ("mnz", C.c_int),
         ("morigin", C.c_double * 3), ("mspacing", C.c_double * 3), ("mreflective", C.c_int), ("mclip", C.c_int),
+        ("jf12", C.c_void_p),
     ]
+
+
+class JF12Params(C.Structure):
+    """crp_jf12_params: JF12Field's switches and constants"""
+    _fields_ = [(n, C.c_int) for n in ("use_regular", "use_disk", "use_toroidal_halo", "use_x", "use_striated",
+                                       "use_turbulent")] + \
+               [("r_arms", C.c_double * 8), ("b_disk", C.c_double * 8)] + \
+               [(n, C.c_double) for n in ("sin_pitch", "cos_pitch", "tan90_minus_pitch", "b_ring", "h_disk", "w_disk",
+                                          "b_north", "b_south", "r_north", "r_south", "w_halo", "z0", "b_x",
+                                          "sin_theta_x0", "cos_theta_x0", "tan_theta_x0", "r_xc", "r_x", "sqrt_beta")] + \
+               [("b_disk_turb", C.c_double * 8)] + \
+               [(n, C.c_double) for n in ("b_disk_turb5", "z_disk_turb", "b_halo_turb", "r_halo_turb", "z_halo_turb")]
+
 
 
 class SDE(C.Structure):
@@ -186,6 +200,28 @@ class OracleField:
             f.s.mspacing[a] = msp[a]
         f.s.mreflective = int(mreflective)
         f.s.mclip = int(mclip)
+        return f
+
+    @staticmethod
+    def jf12(params, striated=None, striated_spacing=None, turbulent=None, turbulent_spacing=None):
+        """JF12Field with the given JF12Params; striated (Nx,Ny,Nz) and turbulent (Nx,Ny,Nz,3) grids at the
+        origin (periodic)"""
+        if turbulent is not None:
+            f = OracleField.grid(turbulent, [0, 0, 0], turbulent_spacing)
+        else:
+            f = OracleField()
+        f.s.type = FIELD_JF12
+        if striated is not None:
+            m = np.ascontiguousarray(striated, dtype=np.float32)
+            f._keep.append(m)
+            f.s.mod = ptr(m, _fp)
+            f.s.mnx, f.s.mny, f.s.mnz = m.shape[:3]
+            for a in range(3):
+                f.s.morigin[a] = 0.0
+                f.s.mspacing[a] = striated_spacing
+            f.s.mreflective = 0
+        f._keep.append(params)
+        f.s.jf12 = C.cast(C.pointer(params), C.c_void_p)
         return f
 
     @staticmethod
@@ -468,6 +504,30 @@ def sin_log(x):
     s, lg = np.zeros_like(x), np.zeros_like(x)
     lib().orc_sin_log_batch(ptr(x), x.size, ptr(s), ptr(lg))
     return s, lg
+
+
+def exp_atan2(x, y):
+    """crp_exp_rn(x) and crp_atan2_rn(y, x) element-wise"""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    e, a = np.zeros_like(x), np.zeros_like(x)
+    L = lib()
+    L.orc_exp_atan2_batch.argtypes = [_P(C.c_double), _P(C.c_double), C.c_size_t, _P(C.c_double), _P(C.c_double)]
+    L.orc_exp_atan2_batch(ptr(x), ptr(y), x.size, ptr(e), ptr(a))
+    return e, a
+
+
+def jf12_defaults(planck=False):
+    p = JF12Params()
+    lib().orc_jf12_defaults(int(planck), C.byref(p))
+    return p
+
+
+def jf12_striated(seed, n=100):
+    """JF12Field::randomStriated(seed) (src/magneticField/JF12Field.cpp:81-94): round(rand()) * 2 - 1 per
+    point in storage order, rand() = randInt() / (2^32 - 1)"""
+    r = randint_batch(seed, n ** 3).astype(np.float64) * (1.0 / 4294967295.0)
+    return (np.round(r) * 2 - 1).astype(np.float32).reshape(n, n, n)
 
 
 def sde_propagate(field, state, steps, tolerance, min_step, max_step, epsilon=0.1, alpha=1 / 3., scale=1.0,

@@ -119,3 +119,67 @@ def test_sin_correctly_rounded_and_log():
     _, lg = O.sin_log(y)
     refl = np.array([float(Decimal(float(v)).ln()) for v in y])
     assert np.array_equal(lg.view(np.uint64), refl.view(np.uint64))
+
+
+def _datan(x):
+    """atan in decimal arithmetic: three argument halvings, then the series"""
+    from decimal import Decimal
+    neg = x < 0
+    x = -x if neg else x
+    inv = x > 1
+    if inv:
+        x = 1 / x
+    for _ in range(3):
+        x = x / (1 + (1 + x * x).sqrt())
+    x2, term, total, k = x * x, x, x, 1
+    while True:
+        term = term * x2
+        t = term / (2 * k + 1)
+        if t == 0 or t.adjusted() < -80:
+            break
+        total = total - t if k % 2 else total + t
+        k += 1
+    total = 8 * total
+    if inv:
+        total = _PI / 2 - total
+    return -total if neg else total
+
+
+_PI = None
+
+
+def test_exp_and_atan2_correctly_rounded():
+    """exp and atan2 against 70-digit decimal arithmetic rounded once (arguments of the JF12
+    field's kind: logistic and exponential profiles, azimuths and elevation angles), plus
+    C99's special cases of atan2."""
+    from decimal import Decimal, getcontext
+    import math
+    global _PI
+    getcontext().prec = 70
+    _PI = Decimal("3.141592653589793238462643383279502884197169399375105820974944592307816406286")
+    rng = np.random.default_rng(12)
+    x = np.concatenate([rng.uniform(-700, 700, 800), rng.uniform(-20, 20, 1200), rng.uniform(-1e-3, 1e-3, 200),
+                        np.ldexp(rng.uniform(-1, 1, 100), -40)])
+    e, _ = O.exp_atan2(x, np.ones_like(x))
+    ref = np.array([float(Decimal(float(v)).exp()) for v in x])
+    assert np.array_equal(e.view(np.uint64), ref.view(np.uint64))
+    X = np.concatenate([rng.uniform(-30, 30, 1500), rng.normal(size=300) * 1e-6, rng.uniform(0, 1e4, 200)])
+    Y = np.concatenate([rng.uniform(-30, 30, 1500), rng.normal(size=300), rng.uniform(-1e-3, 1e-3, 200)])
+    _, a = O.exp_atan2(X, Y)
+
+    def datan2(y, x):
+        y, x = Decimal(float(y)), Decimal(float(x))
+        if x > 0:
+            return _datan(y / x)
+        if x < 0:
+            return _datan(y / x) + (_PI if y >= 0 else -_PI)
+        return _PI / 2 if y > 0 else -_PI / 2
+
+    ref = np.array([float(datan2(y, x)) for x, y in zip(X, Y)])
+    assert np.array_equal(a.view(np.uint64), ref.view(np.uint64))
+    # special cases (C99 F.9.1.4)
+    sx = np.array([1.0, -1.0, 0.0, -0.0, 0.0, -0.0, np.inf, -np.inf, np.inf, -np.inf, 5.0, -5.0, 0.0])
+    sy = np.array([0.0, 0.0, 0.0, 0.0, -0.0, -0.0, np.inf, np.inf, 1.0, 1.0, np.inf, -np.inf, 2.0])
+    _, a = O.exp_atan2(sx, sy)
+    assert np.array_equal(a.view(np.uint64), np.arctan2(sy, sx).view(np.uint64))
+    assert math.isnan(O.exp_atan2(np.array([np.nan]), np.array([1.0]))[1][0])
