@@ -13,7 +13,10 @@ Boris steps, 64^3 grid of 50 kpc), bp_planewave (2^20 protons 1e15-1e18 eV,
 fixed 0.05 pc Boris steps in PlaneWaveTurbulence with Nm=256, Brms 1 muG,
 lMin 1 pc, lMax 100 pc, seed 11, 10000 steps), bp_grid512 / ck_grid512 (2^21
 protons, 1000 fixed 20 kpc steps in a 512^3 grid of 10 kpc, seed 3),
-sde_grid256 (DiffusionSDE: 2^20 protons 1 TeV-1 PeV from the centre of a
+ck_jf12 (the reference notebook's Galactic case: 2^16 antiprotons of 1 EeV
+from Earth (-8.5, 0, 0) kpc, isotropic, fixed 0.1 pc Cash-Karp steps through
+JF12Field with randomStriated(42) and randomTurbulent(7) (256^3 at 4 pc),
+20000 steps), sde_grid256 (DiffusionSDE: 2^20 protons 1 TeV-1 PeV from the centre of a
 periodic 256^3 Kolmogorov grid of 1 pc (lMin 2 cells, lMax 128 cells, seed 7),
 tolerance 1e-4, minStep 0.1 pc, maxStep 10 pc, epsilon 0.1, alpha 1/3,
 scale 1, Philox seed 11, MaximumTrajectoryLength 1 kpc), and
@@ -73,6 +76,9 @@ WORKLOADS = {
                         lmax_cells=128, seed=7, tolerance=1e-4, min_step=0.1 * pc, max_step=10 * pc,
                         max_length=1 * kpc, epsilon=0.1, alpha=1 / 3., scale=1.0, sde_seed=11,
                         emin=1e12 * eV, emax=1e15 * eV, species=(1,), weights=(1.0,), start="centre"),
+    "ck_jf12": dict(kind="CK", n=2 ** 16, field="jf12", striated_seed=42, turb_seed=7, tolerance=1e-4,
+                    min_step=0.1 * pc, max_step=0.1 * pc, max_length=0.0, steps=20000, emin=1 * EeV, emax=1 * EeV,
+                    species=(-1,), weights=(1.0,), start="earth", seed=13),
     "ck_grid512": dict(kind="CK", n=2 ** 21, field="grid", grid_n=512, spacing=10 * kpc, lmin_cells=2,
                        lmax_cells=256, seed=3, tolerance=1e-4, min_step=20 * kpc, max_step=20 * kpc,
                        max_length=0.0, steps=1000, emin=1 * EeV, emax=100 * EeV, species=(1,), weights=(1.0,),
@@ -90,7 +96,9 @@ def make_particles(w, n):
     r = np.sqrt(1 - z * z)
     d = np.stack([r * np.cos(t), r * np.sin(t), z], axis=1)
     d /= np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])[:, None]
-    if w["start"] == "centre":
+    if w["start"] == "earth":
+        pos = np.tile([-8.5 * kpc, 0.0, 0.0], (n, 1))
+    elif w["start"] == "centre":
         L = w["grid_n"] * w["spacing"]
         pos = np.tile([L / 2, L / 2, L / 2], (n, 1))
     else:
@@ -100,6 +108,13 @@ def make_particles(w, n):
 
 def make_field_data(w):
     """Grid values (numpy) or plane-wave mode table (dict) of the workload."""
+    if w["field"] == "jf12":
+        from crpropa3_amd import JF12Field
+        f = JF12Field()
+        f.randomStriated(w["striated_seed"])
+        f.randomTurbulent(w["turb_seed"])
+        return {"striated": np.array(f.getStriatedGrid().array(), copy=True),
+                "turbulent": np.array(f.getTurbulentGrid().array(), copy=True)}
     if w["field"] == "grid":
         from crpropa3_amd.turbulence import simple_grid_turbulence
         return simple_grid_turbulence(w["grid_n"], w["spacing"], 1 * nG, w["lmin_cells"] * w["spacing"],
@@ -118,6 +133,10 @@ def cpu_baseline(w, fdata, pos, d, E, q, target_s=15.0):
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
     if w["field"] == "grid":
         f = O.OracleField.grid(fdata, [0, 0, 0], [w["spacing"]] * 3)
+    elif w["field"] == "jf12":
+        jp = O.jf12_defaults(False)
+        jp.use_striated = jp.use_turbulent = 1
+        f = O.OracleField.jf12(jp, fdata["striated"], 0.1 * kpc, fdata["turbulent"], 4 * pc)
     else:
         f = O.OracleField.planewave(fdata)
     kind = O.PROP_CK if w["kind"] == "CK" else O.PROP_BP
@@ -304,6 +323,14 @@ def read_traffic(workload, precision):
         return json.load(fh).get("hbm_bytes_per_launch")
 
 
+def read_flops(workload):
+    path = os.path.join(ROOT, "profiles", "flops_%s.json" % workload)
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh).get("fp64_flops_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -335,7 +362,10 @@ def main():
     fdata = make_field_data(w)
     pos, d, E, q = make_particles(w, n)
     capi.check(capi.lib().crp_set_device(0))
-    if w["field"] == "grid":
+    if w["field"] == "jf12":
+        field = capi.Field.jf12(striated=fdata["striated"], striated_spacing=0.1 * kpc, turbulent=fdata["turbulent"],
+                                turbulent_spacing=4 * pc)
+    elif w["field"] == "grid":
         field = capi.Field.grid3f(fdata, [0, 0, 0], [w["spacing"]] * 3).set_layout(args.grid_layout)
         if args.grid_storage == "fp16":
             if args.precision != "mixed":
@@ -410,6 +440,18 @@ def main():
             evals_s = evals / launches / kern_s
             roofline["random_line_ceiling"] = {"lines_per_s": ceiling, "bytes_per_line": line, "table_bytes": table,
                                                "evals_per_s": evals_s, "frac": evals_s / ceiling}
+    elif w["field"] == "jf12":
+        config.update({"field": "JF12Field + randomStriated(%d) + randomTurbulent(%d), 256^3 at 4 pc"
+                                % (w["striated_seed"], w["turb_seed"])})
+        # FP64 VALU work per launch from the instruction counters of the committed profile (64 lanes per
+        # wave instruction, FMA = 2 FLOP): the field's correctly rounded transcendental functions dominate
+        fl = read_flops(args.workload)
+        achieved = fl / kern_s / 1e12 if fl else None
+        roofline = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": achieved / FP64_PEAK_TFLOPS if achieved else None,
+                    "traffic": read_traffic(args.workload, args.precision), "kernel": "k_propagate<FieldJF12, CK>",
+                    "kernel_ms_avg": kern_s * 1e3, "fp64_flops_per_launch": fl,
+                    "field_evals_per_s": evals / launches / kern_s}
     else:
         config.update({"field": "PlaneWaveTurbulence Nm=%d" % w["nm"]})
         flops = evals * (w["nm"] * FLOP_PER_MODE + FLOP_BORIS) / launches
