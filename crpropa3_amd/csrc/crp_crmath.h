@@ -506,6 +506,57 @@ CRP_MATH_FN double crp_log_dd_rn(double x) {
 	return l.hi + l.lo;
 }
 
+/* ---------------------------------------------------------- fast, deterministic (not correctly rounded)
+ * For the random draws of DiffusionSDE's own stream, where only the same bits on the host and the
+ * device matter (errors of about one ulp): plain double arithmetic in a fixed order.
+ *
+ * log(x), finite x > 0: x = 2^e m, m in [1/sqrt2, sqrt2), f = m - 1 (exact), s = f / (2 + f),
+ * log m = 2 atanh s = 2 s (1 + s^2/3 + s^4/5 + ... + s^20/21) (|s| <= 0.172: the next term is below
+ * 2^-60), plus e ln2 with ln2 in two parts (the 40-bit first part times e is exact). */
+CRP_MATH_FN double crp_log_fast(double x) {
+	if (!(x > 0.0) || x == 1.0 / 0.0)
+		return x == 0.0 ? -1.0 / 0.0 : (x > 0.0 ? x : (x - x) / (x - x));
+	const uint64_t u = crp_as_u64(x);
+	int e = (int)((u >> 52) & 0x7ff) - 1023;
+	double m = crp_from_u64((u & 0x000fffffffffffffull) | 0x3ff0000000000000ull);
+	if (e == -1023) { /* subnormal: scale up first */
+		const double y = x * 0x1p54;
+		const uint64_t v = crp_as_u64(y);
+		e = (int)((v >> 52) & 0x7ff) - 1023 - 54;
+		m = crp_from_u64((v & 0x000fffffffffffffull) | 0x3ff0000000000000ull);
+	}
+	if (m > 1.4142135623730951) {
+		m *= 0.5;
+		e += 1;
+	}
+	const double f = m - 1.0;
+	const double sv = f / (2.0 + f);
+	const double z = sv * sv;
+	const double P = z * (1.0 / 3 + z * (1.0 / 5 + z * (1.0 / 7 + z * (1.0 / 9 + z * (1.0 / 11 + z * (1.0 / 13 +
+	                 z * (1.0 / 15 + z * (1.0 / 17 + z * (1.0 / 19 + z * (1.0 / 21))))))))));
+	const double lm = 2.0 * sv + 2.0 * sv * P;
+	const double de = (double)e;
+	return de * CRP_LN2_1 + (lm + (de * CRP_LN2_2 + de * CRP_LN2_3));
+}
+
+/* sin and cos of x, |x| < 2^20: k = rint(x 2/pi), r = x - k pi/2 with the three 30-bit parts of pi/2
+ * (products exact) and the rest, |r| <= pi/4, Taylor series of sin r and cos r to r^17 and r^18
+ * (the next terms are below 2^-60), then the quadrant of k */
+CRP_MATH_FN void crp_sincos_fast(double x, double *sin_out, double *cos_out) {
+	const double k = crp_rint(x * CRP_TWO_OVER_PI);
+	const double r = (((x - k * CRP_PIO2_1) - k * CRP_PIO2_2) - k * CRP_PIO2_3) - k * CRP_PIO2_4;
+	const double z = r * r;
+	const double sr = r + r * z * (-1.0 / 6 + z * (1.0 / 120 + z * (-1.0 / 5040 + z * (1.0 / 362880 +
+	                  z * (-1.0 / 39916800 + z * (1.0 / 6227020800.0 + z * (-1.0 / 1307674368000.0 +
+	                  z * (1.0 / 355687428096000.0))))))));
+	const double cr = 1.0 - 0.5 * z + z * z * (1.0 / 24 + z * (-1.0 / 720 + z * (1.0 / 40320 + z * (-1.0 / 3628800 +
+	                  z * (1.0 / 479001600.0 + z * (-1.0 / 87178291200.0 + z * (1.0 / 20922789888000.0 +
+	                  z * (-1.0 / 6402373705728000.0))))))));
+	const int q = (int)((long long)k & 3);
+	*sin_out = q == 0 ? sr : (q == 1 ? cr : (q == 2 ? -sr : -cr));
+	*cos_out = q == 0 ? cr : (q == 1 ? -sr : (q == 2 ? -cr : sr));
+}
+
 /* exp(x): the double-double exponential (relative error about 2^-100) rounded once, so correctly
  * rounded except for arguments whose exact value lies within 2^-100 of a rounding boundary; results
  * below 2^-1022 are rounded twice (the scaling of a subnormal result) */

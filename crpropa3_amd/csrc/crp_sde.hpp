@@ -20,8 +20,8 @@
 //
 // Random numbers: Philox4x32-10 per (particle stream, step) (crp_philox.h);
 // randNorm, randVector and randConeVector as src/Random.cpp:111-162 with the
-// correctly rounded cos, sin and pow and the correctly rounded double-double
-// log of crp_crmath.h, and cos/sin of acos(w) taken as w and sqrt(1 - w^2).
+// fast deterministic log and sin/cos of crp_crmath.h (about one ulp; the same
+// bits in the oracle), and cos/sin of acos(w) taken as w and sqrt(1 - w^2).
 #pragma once
 
 #include "crp_philox.h"
@@ -108,9 +108,11 @@ __device__ __forceinline__ void sde_trystep(const F &field, const V3 &x, const V
 
 // Random::randNorm(0, 1) (src/Random.cpp:111-115): randDblExc, then randExc
 __device__ __forceinline__ double sde_randnorm(crp_sde_rng &r) {
-	const double rr = __builtin_sqrt(-2.0 * crp_log_dd_rn(1.0 - crp_sde_uniform(&r))) * 1.0;
+	const double rr = __builtin_sqrt(-2.0 * crp_log_fast(1.0 - crp_sde_uniform(&r))) * 1.0;
 	const double phi = 2.0 * 3.14159265358979323846264338328 * crp_sde_uniform(&r);
-	return 0.0 + rr * crp_cos_cr(phi);
+	double sp, cp;
+	crp_sincos_fast(phi, &sp, &cp);
+	return 0.0 + rr * cp;
 }
 
 // Random::randVector (src/Random.cpp:141-146)
@@ -119,7 +121,9 @@ __device__ __forceinline__ V3 sde_randvector(crp_sde_rng &r) {
 	const double z = -1.0 + (1.0 - -1.0) * crp_sde_uniform(&r);
 	const double t = -1.0 * pi + (pi - -1.0 * pi) * crp_sde_uniform(&r);
 	const double rr = __builtin_sqrt(1 - z * z);
-	return v3(rr * crp_cos_cr(t), rr * crp_sin_cr(t), z);
+	double st, ct;
+	crp_sincos_fast(t, &st, &ct);
+	return v3(rr * ct, rr * st, z);
 }
 
 // Random::randConeVector(mean, pi/2) and Vector3::getRotated (src/Random.cpp:148-162,

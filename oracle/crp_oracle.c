@@ -1192,6 +1192,13 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 		out[i] = o.v[i];
 }
 
+void orc_fast_batch(const double *x, size_t n, double *sin_out, double *cos_out, double *log_out) {
+	for (size_t i = 0; i < n; i++) {
+		crp_sincos_fast(x[i], sin_out + i, cos_out + i);
+		log_out[i] = crp_log_fast(x[i]);
+	}
+}
+
 void orc_exp_atan2_batch(const double *x, const double *y, size_t n, double *exp_out, double *atan2_out) {
 	for (size_t i = 0; i < n; i++) {
 		exp_out[i] = crp_exp_rn(x[i]);
@@ -1225,9 +1232,22 @@ static double sde_randdblexc(orc_sde_rng *r) {
 	return r->kind == ORC_SDE_MT ? ((double)orc_rng_randint(r->mt) + 0.5) * (1.0 / 4294967296.0)
 	                             : crp_sde_uniform(&r->ph);
 }
-static double sde_log(int k, double x) { return k == ORC_SDE_MT ? log(x) : crp_log_dd_rn(x); }
-static double sde_cos(int k, double x) { return k == ORC_SDE_MT ? cos(x) : crp_cos_cr(x); }
-static double sde_sin(int k, double x) { return k == ORC_SDE_MT ? sin(x) : crp_sin_cr(x); }
+/* the reference's stream: glibc; the device's: the fast deterministic functions of crp_crmath.h */
+static double sde_log(int k, double x) { return k == ORC_SDE_MT ? log(x) : crp_log_fast(x); }
+static double sde_cos(int k, double x) {
+	double s, c;
+	if (k == ORC_SDE_MT)
+		return cos(x);
+	crp_sincos_fast(x, &s, &c);
+	return c;
+}
+static double sde_sin(int k, double x) {
+	double s, c;
+	if (k == ORC_SDE_MT)
+		return sin(x);
+	crp_sincos_fast(x, &s, &c);
+	return s;
+}
 static double sde_pow(int k, double x, double y) { return k == ORC_SDE_MT ? pow(x, y) : crp_pow_cr(x, y); }
 
 /* Random::randNorm(0, 1) (src/Random.cpp:111-115) */

@@ -506,6 +506,16 @@ def sin_log(x):
     return s, lg
 
 
+def fast_sincos_log(x):
+    """crp_sincos_fast and crp_log_fast element-wise: (sin, cos, log)"""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    sn, cs, lg = np.zeros_like(x), np.zeros_like(x), np.zeros_like(x)
+    L = lib()
+    L.orc_fast_batch.argtypes = [_P(C.c_double), C.c_size_t, _P(C.c_double), _P(C.c_double), _P(C.c_double)]
+    L.orc_fast_batch(ptr(x), x.size, ptr(sn), ptr(cs), ptr(lg))
+    return sn, cs, lg
+
+
 def exp_atan2(x, y):
     """crp_exp_rn(x) and crp_atan2_rn(y, x) element-wise"""
     x = np.ascontiguousarray(x, dtype=np.float64)

@@ -183,3 +183,46 @@ def test_exp_and_atan2_correctly_rounded():
     _, a = O.exp_atan2(sx, sy)
     assert np.array_equal(a.view(np.uint64), np.arctan2(sy, sx).view(np.uint64))
     assert math.isnan(O.exp_atan2(np.array([np.nan]), np.array([1.0]))[1][0])
+
+
+def test_fast_sincos_and_log_within_two_ulp():
+    """The fast deterministic functions of the DiffusionSDE draws: within 2 ulp of the correctly rounded
+    values (70-digit decimal) on the draws' ranges (angles in [-pi, 2pi), logarithms of (0, 1]) and
+    beyond (angles to +-1e5, arguments to 1e300 and subnormal)."""
+    from decimal import Decimal, getcontext
+    import math
+    getcontext().prec = 60
+    rng = np.random.default_rng(31)
+    x = np.concatenate([rng.uniform(-np.pi, 2 * np.pi, 3000), rng.uniform(-1e5, 1e5, 300), [0.0, 1e-300, -0.5]])
+    sn, cs, _ = O.fast_sincos_log(x)
+    def ulps(a, ref):
+        return np.abs(a - ref) / np.spacing(np.abs(ref))
+    ref_s = np.array([float(_dsin(Decimal(float(v)))) for v in x])
+    ref_c = np.array([float(_dcos(Decimal(float(v)))) for v in x])
+    big = np.abs(ref_s) > 1e-300
+    assert ulps(sn[big], ref_s[big]).max() <= 2 and ulps(cs, ref_c).max() <= 2
+    u = np.concatenate([1.0 - rng.uniform(0, 1, 3000), np.ldexp(1.0, -rng.integers(1, 1074, 200)), rng.uniform(1, 1e300, 200),
+                        [1.0, 2.0, 0.5]])
+    _, _, lg = O.fast_sincos_log(u)
+    ref_l = np.array([float(Decimal(float(v)).ln()) for v in u])
+    nz = ref_l != 0
+    assert ulps(lg[nz], ref_l[nz]).max() <= 2 and np.all(lg[~nz] == 0)
+
+
+def _dsin(x):
+    """sin in decimal arithmetic: reduce by 2 pi, Taylor"""
+    from decimal import Decimal
+    pi = Decimal("3.14159265358979323846264338327950288419716939937510582097494459")
+    x = x % (2 * pi)
+    term, total, k = x, x, 1
+    while abs(term) > Decimal(10) ** -55:
+        term = -term * x * x / ((2 * k) * (2 * k + 1))
+        total += term
+        k += 1
+    return total
+
+
+def _dcos(x):
+    from decimal import Decimal
+    pi = Decimal("3.14159265358979323846264338327950288419716939937510582097494459")
+    return _dsin(x + pi / 2)
