@@ -560,6 +560,48 @@ CRP_MATH_FN void crp_sincos_fast(double x, double *sin_out, double *cos_out) {
 /* exp(x): the double-double exponential (relative error about 2^-100) rounded once, so correctly
  * rounded except for arguments whose exact value lies within 2^-100 of a rounding boundary; results
  * below 2^-1022 are rounded twice (the scaling of a subnormal result) */
+CRP_MATH_COLD double crp_exp_accurate(double x) {
+	int sc;
+	crp_dd r = crp_exp_dd(crp_mkdd(x, 0.0), &sc);
+	double v = r.hi + r.lo; /* the one rounding */
+	int s1 = sc / 2, s2 = sc - s1;
+	return v * crp_pow2i(s1) * crp_pow2i(s2);
+}
+
+/*
+ * Quick phase of exp (Ziv): the reduction of crp_exp_dd (x = k ln2/256 + r, r = rh + rl with
+ * |rh| <= 2^-9.4), expm1 r = rh + [rl (1 + rh) + rh^2 (1/2 + rh/6 + rh^2/24 + rh^3/120 + rh^4/720)] in
+ * double, and 2^(j/256) (1 + expm1 r) with the one large product T.hi * rh formed exactly.  Error
+ * budget relative to the result: the bracketed term (<= 2^-19.8) to a few ulps, 2^-72; the series'
+ * truncation 2^-79; T.lo and the products with the bracket 2^-72; the final two additions 2^-73; the
+ * table 2^-106 and the reduction 2^-100: below 2^-69 in all.  The value is returned when everything
+ * within 2^-66 of it rounds the same way; otherwise the double-double phase decides.  Normal results
+ * only (|x| <= 708).
+ */
+CRP_MATH_FN int crp_exp_quick(double x, double *out) {
+	const double k = crp_rint(x * CRP_INV_LN2_256);
+	const double r0 = x - k * CRP_LN2_256_1; /* exact */
+	const crp_dd r = crp_two_sum(r0, -k * CRP_LN2_256_2);
+	const double rh = r.hi, rl = r.lo - k * CRP_LN2_256_3;
+	const double q = rh * rh * (0.5 + rh * (1.0 / 6 + rh * (1.0 / 24 + rh * (1.0 / 120 + rh * (1.0 / 720)))));
+	const double em = rl + rh * rl + q; /* expm1(r) - rh */
+	const long long ki = (long long)k;
+	const int j = (int)(ki & 255);
+	const long long m = (ki - j) / 256;
+	const crp_dd T = crp_tab_dd(crp_exp2_tab, j);
+	const crp_dd p = crp_two_prod(T.hi, rh);
+	const double lo = p.lo + (T.hi * em + (T.lo + T.lo * (rh + em)));
+	const crp_dd sh = crp_fast_two_sum(T.hi, p.hi);
+	const double hi = sh.hi, l2 = sh.lo + lo;
+	const double eps = hi * 0x1p-66;
+	const double a = hi + (l2 + eps), b = hi + (l2 - eps);
+	if (a != b)
+		return 0;
+	const int s1 = (int)(m / 2), s2 = (int)(m - s1);
+	*out = (hi + l2) * crp_pow2i(s1) * crp_pow2i(s2);
+	return 1;
+}
+
 CRP_MATH_FN double crp_exp_rn(double x) {
 	if (x != x)
 		return x + x;
@@ -567,11 +609,10 @@ CRP_MATH_FN double crp_exp_rn(double x) {
 		return 1.0 / 0.0;
 	if (x < -745.1332191019412)
 		return 0.0;
-	int sc;
-	crp_dd r = crp_exp_dd(crp_mkdd(x, 0.0), &sc);
-	double v = r.hi + r.lo; /* the one rounding */
-	int s1 = sc / 2, s2 = sc - s1;
-	return v * crp_pow2i(s1) * crp_pow2i(s2);
+	double v;
+	if (x >= -708.0 && x <= 708.0 && crp_exp_quick(x, &v))
+		return v;
+	return crp_exp_accurate(x);
 }
 
 /* a / b in double-double: three quotient digits with exact remainders (about 2^-104) */

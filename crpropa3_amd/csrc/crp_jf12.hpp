@@ -68,21 +68,47 @@ struct FieldJF12 {
 		return rn;
 	}
 
+	// What getRegularField and getTurbulentStrength both compute from the position, once per evaluation
+	// (the same operations, so the same bits): |pos|, the in-plane radius, the azimuth (Vector3::getPhi,
+	// include/crpropa/Vector3.h:136-142) and the arm radius r_negx when r >= 5 kpc.
+	struct Geo {
+		double d, r, phi, rn;
+		bool haveRn;
+	};
+	__device__ __forceinline__ Geo geo(const V3 &pos) const {
+		Geo g;
+		g.d = norm(pos);
+		g.r = __builtin_sqrt(pos.x * pos.x + pos.y * pos.y);
+		const double eps = 2.2250738585072014e-308; // numeric_limits<double>::min()
+		g.phi = (__builtin_fabs(pos.x) < eps && __builtin_fabs(pos.y) < eps) ? 0.0 : crp_atan2_rn(pos.y, pos.x);
+		g.haveRn = false;
+		g.rn = 0;
+		return g;
+	}
+	__device__ __forceinline__ double arm_radius(Geo &g) const {
+		if (!g.haveRn) {
+			g.rn = r_negx(g.r, g.phi);
+			g.haveRn = true;
+		}
+		return g.rn;
+	}
+
 	// JF12Field::getRegularField with getDiskField, getToroidalHaloField, getXField (:192-300)
-	__device__ V3 regular(const V3 &pos) const {
+	__device__ V3 regular(const V3 &pos, Geo &gm) const {
 		double bx = 0, by = 0, bz = 0;
-		const double d = norm(pos);
+		const double d = gm.d;
 		if (d < 20 * kJKpc) {
-			const double r = __builtin_sqrt(pos.x * pos.x + pos.y * pos.y);
-			const double eps = 2.2250738585072014e-308; // numeric_limits<double>::min()
-			const double phi = (__builtin_fabs(pos.x) < eps && __builtin_fabs(pos.y) < eps) ? 0.0
-			                                                                               : crp_atan2_rn(pos.y, pos.x);
+			const double r = gm.r;
+			const double phi = gm.phi;
 			const double sinPhi = crp_sin_cr(phi), cosPhi = crp_cos_cr(phi);
 			const double z = pos.z;
+			// the disk's logistic factor, shared by the disk and the toroidal halo
+			const bool needLf = p.use_disk || (p.use_toroidal_halo && (r * r + z * z > 1 * kJKpc * kJKpc));
+			const double lfShared = needLf ? logistic(z, p.h_disk, p.w_disk) : 0.0;
 			// disk
 			double dx = 0, dy = 0;
 			if (p.use_disk) {
-				const double lfDisk = logistic(z, p.h_disk, p.w_disk);
+				const double lfDisk = lfShared;
 				if (r > 3 * kJKpc) {
 					double bMag = 0;
 					if (r < 5 * kJKpc) {
@@ -90,7 +116,7 @@ struct FieldJF12 {
 						dx += -bMag * sinPhi;
 						dy += bMag * cosPhi;
 					} else {
-						const double rn = r_negx(r, phi);
+						const double rn = arm_radius(gm);
 						for (int i = 7; i >= 0; i--)
 							if (rn < p.r_arms[i])
 								bMag = p.b_disk[i];
@@ -106,7 +132,7 @@ struct FieldJF12 {
 			// toroidal halo
 			double hx = 0, hy = 0;
 			if (p.use_toroidal_halo && (r * r + z * z > 1 * kJKpc * kJKpc)) {
-				const double lfDisk = logistic(z, p.h_disk, p.w_disk);
+				const double lfDisk = lfShared;
 				double bMagH = crp_exp_rn(-__builtin_fabs(z) / p.z0) * lfDisk;
 				if (z >= 0)
 					bMagH *= p.b_north * (1 - logistic(r, p.r_north, p.w_halo));
@@ -150,18 +176,15 @@ struct FieldJF12 {
 	}
 
 	// JF12Field::getTurbulentStrength (:307-331)
-	__device__ double turbulent_strength(const V3 &pos) const {
-		if (norm(pos) > 20 * kJKpc)
+	__device__ double turbulent_strength(const V3 &pos, Geo &gm) const {
+		if (gm.d > 20 * kJKpc)
 			return 0;
-		const double r = __builtin_sqrt(pos.x * pos.x + pos.y * pos.y);
-		const double eps = 2.2250738585072014e-308;
-		const double phi =
-		    (__builtin_fabs(pos.x) < eps && __builtin_fabs(pos.y) < eps) ? 0.0 : crp_atan2_rn(pos.y, pos.x);
+		const double r = gm.r;
 		double bDisk = 0;
 		if (r < 5 * kJKpc) {
 			bDisk = p.b_disk_turb5;
 		} else {
-			const double rn = r_negx(r, phi);
+			const double rn = arm_radius(gm);
 			for (int i = 7; i >= 0; i--)
 				if (rn < p.r_arms[i])
 					bDisk = p.b_disk_turb[i];
@@ -177,21 +200,22 @@ struct FieldJF12 {
 	// Out of line: one copy serves the six stages of a Cash-Karp step.
 	__device__ __attribute__((noinline)) V3 eval(const V3 &pos) const {
 		double bx = 0, by = 0, bz = 0;
+		Geo gm = geo(pos);
 		if (p.use_turbulent) {
 			const V3 g = FieldGrid{turbulent}.eval(pos);
-			const float s = (float)turbulent_strength(pos);
+			const float s = (float)turbulent_strength(pos, gm);
 			bx += (double)((float)g.x * s);
 			by += (double)((float)g.y * s);
 			bz += (double)((float)g.z * s);
 		}
 		if (p.use_striated) {
-			const V3 b = regular(pos);
+			const V3 b = regular(pos, gm);
 			const double f = 1. + p.sqrt_beta * (double)scalar_grid_closest(striated, pos);
 			bx += b.x * f;
 			by += b.y * f;
 			bz += b.z * f;
 		} else if (p.use_regular) {
-			const V3 b = regular(pos);
+			const V3 b = regular(pos, gm);
 			bx += b.x;
 			by += b.y;
 			bz += b.z;

@@ -1192,6 +1192,16 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 		out[i] = o.v[i];
 }
 
+/* the quick phase of exp: decided flag, quick value, accurate value (tests) */
+void orc_exp_phases(const double *x, size_t n, int *decided, double *quick, double *accurate) {
+	for (size_t i = 0; i < n; i++) {
+		double q = 0;
+		decided[i] = crp_exp_quick(x[i], &q);
+		quick[i] = q;
+		accurate[i] = crp_exp_accurate(x[i]);
+	}
+}
+
 void orc_fast_batch(const double *x, size_t n, double *sin_out, double *cos_out, double *log_out) {
 	for (size_t i = 0; i < n; i++) {
 		crp_sincos_fast(x[i], sin_out + i, cos_out + i);

@@ -506,6 +506,17 @@ def sin_log(x):
     return s, lg
 
 
+def exp_phases(x):
+    """(decided flag, quick value, accurate value) of the correctly rounded exp per argument"""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    d = np.zeros(x.size, dtype=np.int32)
+    q, a = np.zeros_like(x), np.zeros_like(x)
+    L = lib()
+    L.orc_exp_phases.argtypes = [_P(C.c_double), C.c_size_t, _ip, _P(C.c_double), _P(C.c_double)]
+    L.orc_exp_phases(ptr(x), x.size, ptr(d, _ip), ptr(q), ptr(a))
+    return d.astype(bool), q, a
+
+
 def fast_sincos_log(x):
     """crp_sincos_fast and crp_log_fast element-wise: (sin, cos, log)"""
     x = np.ascontiguousarray(x, dtype=np.float64)

@@ -226,3 +226,15 @@ def _dcos(x):
     from decimal import Decimal
     pi = Decimal("3.14159265358979323846264338327950288419716939937510582097494459")
     return _dsin(x + pi / 2)
+
+
+def test_exp_quick_phase_equals_accurate_phase():
+    """exp's quick phase (double arithmetic, Ziv rounding test at 2^-66) returns the double-double
+    phase's value whenever it decides (checked on 8e7 arguments when written; 2e6 here), and decides
+    almost always"""
+    rng = np.random.default_rng(17)
+    x = np.concatenate([rng.uniform(-708, 708, 800000), rng.uniform(-30, 30, 800000), rng.uniform(-1e-3, 1e-3, 200000),
+                        np.ldexp(rng.uniform(-1, 1, 200000), -rng.integers(1, 60, 200000))])
+    d, q, a = O.exp_phases(x)
+    assert d.mean() > 0.999
+    assert np.array_equal(q[d].view(np.uint64), a[d].view(np.uint64))
