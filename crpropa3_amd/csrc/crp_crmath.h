@@ -631,7 +631,67 @@ CRP_MATH_FN crp_dd crp_dd_div(crp_dd a, crp_dd b) {
  * the odd series to u^17 (terms from u^9 on in double); then pi/2 - a for |y| > |x| and pi - a for
  * x < 0, the sign of y last; rounded once (about 2^-100 before the rounding).
  */
+/*
+ * Quick phase of atan2 for finite, nonzero x and y with t = min/max in [2^-500, 1] (Ziv): the same
+ * reduction with the numerator th - t0 + tl and the denominator 1 + t t0 kept as unevaluated pairs and
+ * one correction step of the division (residuals exact by FMA), u = uh + ul; atan u = uh + [ul + uh z
+ * (-1/3 + z/5 - z^2/7 + ... - z^7/15)], z = uh^2, the bracket's polynomial part (<= 2^-21 relative) in
+ * double; then atan(j/64) from the table, pi/2 - a and pi - a with the large parts added exactly.
+ * Error budget relative to the result (>= 2^-8 or, for j = 0, about u): the polynomial 2^-73, the
+ * division 2^-100, the low-part additions 2^-100, the table 2^-106: below 2^-72.  Returned when
+ * everything within 2^-66 of the value rounds the same way.
+ */
+CRP_MATH_FN int crp_atan2_quick(double y, double x, double *out) {
+	const double ax = x < 0 ? -x : x, ay = y < 0 ? -y : y;
+	const int nx = x < 0, ny = y < 0;
+	const int swap = ay > ax;
+	const double num = swap ? ax : ay, den = swap ? ay : ax;
+	const double th = num / den;
+	if (!(th >= 0x1p-500) || !(den < 0x1p1000) || !(num > 0x1p-1000))
+		return 0;
+	const double tl = __builtin_fma(-th, den, num) / den;
+	const double j = crp_rint(th * 64.0), t0 = j * (1.0 / 64.0);
+	const double nh = th - t0; /* exact */
+	const crp_dd pr = crp_two_prod(th, t0);
+	const crp_dd dd = crp_fast_two_sum(1.0, pr.hi);
+	const double dh = dd.hi, dl = dd.lo + (pr.lo + tl * t0);
+	const double uh = nh / dh;
+	const double ul = (__builtin_fma(-uh, dh, nh) + (tl - uh * dl)) / dh;
+	const double z = uh * uh;
+	const double poly = uh * z * (-1.0 / 3 + z * (1.0 / 5 + z * (-1.0 / 7 + z * (1.0 / 9 + z * (-1.0 / 11 + z * (1.0 / 13 +
+	                    z * (-1.0 / 15)))))));
+	const crp_dd A = crp_tab_dd(crp_atan_tab, (int)j);
+	crp_dd s = crp_two_sum(A.hi, uh);
+	double lo = s.lo + (A.lo + (ul + poly));
+	double hi = s.hi;
+	if (swap) {
+		s = crp_two_sum(CRP_PIO2_DD_HI, -hi);
+		lo = s.lo + (CRP_PIO2_DD_LO - lo);
+		hi = s.hi;
+	}
+	if (nx) {
+		s = crp_two_sum(CRP_PI_DD_HI, -hi);
+		lo = s.lo + (CRP_PI_DD_LO - lo);
+		hi = s.hi;
+	}
+	const double eps = hi * 0x1p-66;
+	if (hi + (lo + eps) != hi + (lo - eps))
+		return 0;
+	const double v = hi + lo;
+	*out = ny ? -v : v;
+	return 1;
+}
+
+CRP_MATH_COLD double crp_atan2_accurate(double y, double x);
+
 CRP_MATH_FN double crp_atan2_rn(double y, double x) {
+	double q;
+	if (x == x && y == y && x != 0.0 && y != 0.0 && crp_atan2_quick(y, x, &q))
+		return q;
+	return crp_atan2_accurate(y, x);
+}
+
+CRP_MATH_COLD double crp_atan2_accurate(double y, double x) {
 	if (x != x || y != y)
 		return x + y;
 	const double ax = x < 0 ? -x : x, ay = y < 0 ? -y : y;

@@ -1202,6 +1202,16 @@ void orc_exp_phases(const double *x, size_t n, int *decided, double *quick, doub
 	}
 }
 
+/* the quick phase of atan2: decided flag, quick value, accurate value (tests) */
+void orc_atan2_phases(const double *y, const double *x, size_t n, int *decided, double *quick, double *accurate) {
+	for (size_t i = 0; i < n; i++) {
+		double q = 0;
+		decided[i] = crp_atan2_quick(y[i], x[i], &q);
+		quick[i] = q;
+		accurate[i] = crp_atan2_accurate(y[i], x[i]);
+	}
+}
+
 void orc_fast_batch(const double *x, size_t n, double *sin_out, double *cos_out, double *log_out) {
 	for (size_t i = 0; i < n; i++) {
 		crp_sincos_fast(x[i], sin_out + i, cos_out + i);

@@ -139,6 +139,8 @@ long long orc_sde_propagate(const orc_field *f, const orc_sde *p, orc_state *s, 
 long long orc_sde_propagate_threads(const orc_field *f, const orc_sde *p, orc_state *s, size_t n, long long maxSteps,
                                     int nthreads);
 
+/* crp_atan2_quick / crp_atan2_accurate per argument pair (tests) */
+void orc_atan2_phases(const double *y, const double *x, size_t n, int *decided, double *quick, double *accurate);
 /* crp_exp_quick / crp_exp_accurate per argument (tests) */
 void orc_exp_phases(const double *x, size_t n, int *decided, double *quick, double *accurate);
 /* crp_sincos_fast and crp_log_fast of x[i] (tests) */

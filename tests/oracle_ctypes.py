@@ -506,6 +506,18 @@ def sin_log(x):
     return s, lg
 
 
+def atan2_phases(y, x):
+    """(decided flag, quick value, accurate value) of the correctly rounded atan2 per argument pair"""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    d = np.zeros(x.size, dtype=np.int32)
+    q, a = np.zeros_like(x), np.zeros_like(x)
+    L = lib()
+    L.orc_atan2_phases.argtypes = [_P(C.c_double), _P(C.c_double), C.c_size_t, _ip, _P(C.c_double), _P(C.c_double)]
+    L.orc_atan2_phases(ptr(y), ptr(x), x.size, ptr(d, _ip), ptr(q), ptr(a))
+    return d.astype(bool), q, a
+
+
 def exp_phases(x):
     """(decided flag, quick value, accurate value) of the correctly rounded exp per argument"""
     x = np.ascontiguousarray(x, dtype=np.float64)

@@ -238,3 +238,17 @@ def test_exp_quick_phase_equals_accurate_phase():
     d, q, a = O.exp_phases(x)
     assert d.mean() > 0.999
     assert np.array_equal(q[d].view(np.uint64), a[d].view(np.uint64))
+
+
+def test_atan2_quick_phase_equals_accurate_phase():
+    """atan2's quick phase returns the double-double phase's value whenever it decides (8e7 argument
+    pairs when written; 2e6 here across quadrants, tiny ratios and near-diagonal pairs)"""
+    rng = np.random.default_rng(23)
+    n = 400000
+    X = np.concatenate([rng.uniform(-30, 30, n), rng.normal(size=n) * 1e-6, rng.uniform(-1, 1, n),
+                        np.ldexp(rng.uniform(-1, 1, n), -rng.integers(0, 200, n)), rng.uniform(1, 2, n)])
+    Y = np.concatenate([rng.uniform(-30, 30, n), rng.normal(size=n), rng.uniform(-1, 1, n) * 1e-3,
+                        rng.uniform(-1, 1, n), X[-n:] * (1 + rng.uniform(-1e-9, 1e-9, n))])
+    d, q, a = O.atan2_phases(Y, X)
+    assert d.mean() > 0.999
+    assert np.array_equal(q[d].view(np.uint64), a[d].view(np.uint64))
