@@ -1503,9 +1503,6 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 	P.sphMargin = p->sphMargin;
 	P.postObs = p->postObs;
 	if (p->type == 2) {
-		if (p->nObs > 0 || p->box || p->sph)
-			return fail(CRP_E_INVALID, "crp_propagate: DiffusionSDE runs with MaximumTrajectoryLength as its only fused "
-			                           "module (no observer, periodic box or spherical boundary)");
 		P.sdeScale = p->sdeScale;
 		P.sdeAlpha = p->sdeAlpha;
 		P.sdeEpsilon = p->sdeEpsilon;

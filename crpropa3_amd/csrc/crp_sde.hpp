@@ -174,6 +174,7 @@ __global__ void __launch_bounds__(256) k_sde(F field, Params P, Soa S, unsigned 
 	double *const cPPx = cw, *const cPPy = cw + 64, *const cPPz = cw + 128;
 	double *const cPDx = cw + 192, *const cPDy = cw + 256, *const cPDz = cw + 320;
 	double *const cCur = cw + 384, *const cLen = cw + 448, *const cTim = cw + 512;
+	double *const cDet = cw + 576; // observer detection count (small integer held as a double)
 
 	for (;;) {
 		if (nEval > 0x80000000u) {
@@ -205,6 +206,8 @@ __global__ void __launch_bounds__(256) k_sde(F field, Params P, Soa S, unsigned 
 						*cLen = S.len[idx];
 						*cTim = S.time[idx];
 						rstep = S.sdeStep[idx];
+						if (P.nObs > 0)
+							*cDet = (double)S.det[idx];
 						stream = S.sdeStream[idx];
 						if (q != 0) {
 							// calculateBTensor (src/module/DiffusionSDE.cpp:258-266): constant per particle;
@@ -379,11 +382,66 @@ __global__ void __launch_bounds__(256) k_sde(F field, Params P, Soa S, unsigned 
 		rstep++;
 		nAcc++;
 		steps++;
-		if (P.maxLen > 0) {
-			if (len >= P.maxLen)
+		// the modules after DiffusionSDE in the list, as k_propagate applies them (they run even when the
+		// step deactivated the particle, as ModuleList::process does)
+		if (P.box) {
+			// PeriodicBox::process (src/module/Boundary.cpp:16-27)
+			const double nbx = __builtin_floor(div_rn(pos.x - P.boxO[0], P.boxS[0], P.boxInv[0]));
+			const double nby = __builtin_floor(div_rn(pos.y - P.boxO[1], P.boxS[1], P.boxInv[1]));
+			const double nbz = __builtin_floor(div_rn(pos.z - P.boxO[2], P.boxS[2], P.boxInv[2]));
+			if (!((nbx == 0) && (nby == 0) && (nbz == 0))) {
+				const double shx = nbx * P.boxS[0], shy = nby * P.boxS[1], shz = nbz * P.boxS[2];
+				pos = v3(pos.x - shx, pos.y - shy, pos.z - shz);
+				*cPPx = *cPPx - shx; *cPPy = *cPPy - shy; *cPPz = *cPPz - shz;
+				S.sx[idx] = S.sx[idx] - shx; S.sy[idx] = S.sy[idx] - shy; S.sz[idx] = S.sz[idx] - shz;
+				S.cx[idx] = S.cx[idx] - shx; S.cy[idx] = S.cy[idx] - shy; S.cz[idx] = S.cz[idx] - shz;
+			}
+		}
+		double nextObs = next; // nextStep as the observer sees it
+		if (P.sph) {
+			// SphericalBoundary::process (src/module/Boundary.cpp:216-223)
+			const double d = norm(v3(pos.x - P.sphC[0], pos.y - P.sphC[1], pos.z - P.sphC[2]));
+			if (d >= P.sphR)
 				alive = false;
-			else
-				next = dmin(next, P.maxLen - len);
+			if (P.sphLimit) {
+				const double l = (P.sphR - d) + P.sphMargin;
+				next = dmin(next, l);
+				if (!(P.postObs & 1))
+					nextObs = dmin(nextObs, l);
+			}
+		}
+		if (P.maxLen > 0) { // MaximumTrajectoryLength::process
+			if (len >= P.maxLen) {
+				alive = false;
+			} else {
+				const double l = P.maxLen - len;
+				next = dmin(next, l);
+				if (!(P.postObs & 2))
+					nextObs = dmin(nextObs, l);
+			}
+		}
+		if (P.nObs > 0) {
+			// Observer with ObserverTimeEvolution (src/module/Observer.cpp:26-57, 279-306)
+			const int k = (int)*cDet;
+			if (k < P.nObs) {
+				const double distance = len - P.obsT[k];
+				if (distance < 0.) {
+					next = dmin(next, -distance);
+				} else {
+					if (k < P.nObs - 2) {
+						const double l = P.obsT[k + 1] - len;
+						next = dmin(next, l);
+						nextObs = dmin(nextObs, l);
+					}
+					*cDet = (double)(k + 1);
+					double *r = S.snap + (size_t)k * kSnapDoubles * S.cap + idx;
+					r[0] = pos.x; r[S.cap] = pos.y; r[2 * S.cap] = pos.z;
+					r[3 * S.cap] = dir.x; r[4 * S.cap] = dir.y; r[5 * S.cap] = dir.z;
+					r[6 * S.cap] = len; r[7 * S.cap] = *cTim; r[8 * S.cap] = nextObs;
+					if (P.obsDeactivate)
+						alive = false;
+				}
+			}
 		}
 		nFin += alive ? 0 : 1;
 		if (!alive || (P.maxSteps >= 0 && steps >= P.maxSteps)) {
@@ -394,6 +452,8 @@ __global__ void __launch_bounds__(256) k_sde(F field, Params P, Soa S, unsigned 
 			S.ppx[idx] = *cPPx; S.ppy[idx] = *cPPy; S.ppz[idx] = *cPPz;
 			S.pdx[idx] = *cPDx; S.pdy[idx] = *cPDy; S.pdz[idx] = *cPDz;
 			S.sdeStep[idx] = rstep;
+			if (P.nObs > 0)
+				S.det[idx] = (int32_t)*cDet;
 			nPause += alive ? 1 : 0;
 			have = false;
 		}

@@ -471,9 +471,6 @@ static bool fusable(const ModuleList::module_list_t &mods, Fused &f) {
 	}
 	if (!f.obs)
 		f.condsBeforeObs = f.conds.size();
-	// the device runs DiffusionSDE with the length limit as its only fused module
-	if (f.prop->getKind() == DevicePropagator::SDE && (f.box || f.sphere || f.obs))
-		return false;
 	return true;
 }
 

@@ -156,7 +156,8 @@ int crp_propagator_create_ck(const crp_field_t *field, double tolerance, double 
  * 272-315: stochastic transport along the field lines, alpha = 1/3 and scale = 1 until
  * crp_propagator_set_sde.  Validation and messages as the reference's setters (tolerance and
  * epsilon in [0,1], 0 <= minStep <= maxStep).  Runs in FP64 (crp_propagator_set_precision
- * refuses CRP_PRECISION_MIXED) with MaximumTrajectoryLength as the only fused module.  Random
+ * refuses CRP_PRECISION_MIXED); the fused modules (length limit, periodic box, spherical boundary,
+ * observer, module order) apply after each DiffusionSDE step as after a BP/CK step.  Random
  * numbers: Philox4x32-10 with key (seed, 0x43525053) and counter (stream low word, step, block,
  * stream high word) per particle (crp_host_state.sde_step / sde_stream).  A step whose trials
  * would need more than 2^20 sub-steps (minStep far below the field's scale) stops that particle

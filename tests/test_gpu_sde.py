@@ -139,15 +139,10 @@ def test_sde_modulated_field_matches_oracle():
     compare(out, st, steps)
 
 
-def test_sde_refuses_mixed_precision_and_unfused_modules():
+def test_sde_refuses_mixed_precision():
     p = capi.Propagator("SDE", capi.Field.uniform([0, 0, 1e-13]), 1e-4, pc, kpc, max_length=kpc)
     with pytest.raises(capi.CrpError, match="FP64 only"):
         capi.check(capi.lib().crp_propagator_set_precision(p.handle, capi.PRECISION_MIXED))
-    p.set_observer([0.5 * kpc])
-    b = capi.Batch(4)
-    b.upload(np.zeros((4, 3)), np.tile([1.0, 0, 0], (4, 1)), np.full(4, 1e-6), np.full(4, O.eplus))
-    with pytest.raises(capi.CrpError, match="only fused"):
-        p.propagate(b, max_steps=3)
 
 
 # ---------------------------------------------------------------- the reference's tests

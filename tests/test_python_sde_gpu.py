@@ -132,3 +132,63 @@ def test_modulelist_matches_oracle_streams():
     assert np.array_equal(dd.view(np.uint64), st.dir.view(np.uint64))
     assert np.array_equal(length.view(np.uint64), st.trajectory_length.view(np.uint64))
     assert np.array_equal(k, steps)
+
+
+class _Noop(crp.Module):
+    """a host module after the others: the list no longer fuses and runs step by step"""
+
+    def process(self, c):
+        pass
+
+
+class _Recorder(crp.Module):
+    """detection action: the candidate's state at detection (the step-by-step path hands over the live
+    candidate, as the reference does, so a collector would show later states)"""
+
+    def __init__(self):
+        crp.Module.__init__(self)
+        self.rows = []
+
+    def process(self, c):
+        p = c.current.getPosition()
+        self.rows.append((c.getSerialNumber(), c.getTrajectoryLength(), p.x, p.y, p.z, c.getNextStep()))
+
+
+def _fused_or_stepwise(stepwise, serials=None):
+    cands = make_candidates(40, seed=12, serials=serials)
+    B = crp.UniformMagneticField(crp.Vector3d(1e-13, 2e-13, 3e-13))
+    d = crp.DiffusionSDE(B, 1e-4, 1 * crp.pc, 100 * crp.pc, 0.2)
+    d.setSeed(3)
+    rec = _Recorder()
+    obs = crp.Observer()
+    obs.add(crp.ObserverTimeEvolution(0.5 * crp.kpc, 4 * crp.kpc, 6))
+    obs.setDeactivateOnDetection(False)
+    obs.onDetection(rec)
+    sim = crp.ModuleList()
+    sim.add(d)
+    sim.add(crp.PeriodicBox(crp.Vector3d(-2 * crp.kpc), crp.Vector3d(4 * crp.kpc)))
+    sim.add(crp.SphericalBoundary(crp.Vector3d(0.), 1.5 * crp.kpc))
+    sim.add(obs)
+    sim.add(crp.MaximumTrajectoryLength(5 * crp.kpc))
+    if stepwise:
+        sim.add(_Noop())
+    sim.run(cands)
+    return cands, sorted(rec.rows)
+
+
+def test_modulelist_sde_with_box_sphere_observer_fused_equals_stepwise():
+    """[DiffusionSDE, PeriodicBox, SphericalBoundary, Observer(ObserverTimeEvolution), MaximumTrajectoryLength]
+    in one device run equals the same list run step by step with the host modules: positions, lengths,
+    flags, step counts and every detection"""
+    a, det_a = _fused_or_stepwise(False)
+    b, det_b = _fused_or_stepwise(True, serials=[c.getSerialNumber() for c in a])
+    sa, sb = state(a), state(b)
+    for x, y in zip(sa, sb):
+        assert np.array_equal(x, y)
+    assert [c.isActive() for c in a] == [c.isActive() for c in b]
+    flags = [(c.hasProperty("Rejected"), c.hasProperty("DetectionIndex") and c.getProperty("DetectionIndex"))
+             for c in a]
+    assert flags == [(c.hasProperty("Rejected"), c.hasProperty("DetectionIndex") and c.getProperty("DetectionIndex"))
+                     for c in b]
+    assert len(det_a) > 20 and det_a == det_b
+    assert any(not c.isActive() and c.getTrajectoryLength() < 5 * crp.kpc for c in a)  # the sphere rejected some
