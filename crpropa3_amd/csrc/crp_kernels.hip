@@ -318,7 +318,7 @@ struct KeyBuf {
 template <class F, int KIND, bool MIX>
 static auto propagate_kernel() {
 	if constexpr (KIND == PK_SDE)
-		return k_sde<F>;
+		return k_sde<F, MIX>; // for DiffusionSDE (FP64 only) the flag selects the compiled-in module chain
 	else
 		return k_propagate<F, KIND, MIX>;
 }
@@ -441,9 +441,6 @@ static int launch_propagate(const F &f, const crp_propagator *p, const Params &P
 template <class F, bool MIX>
 static int dispatch_kind_p(const F &f, const crp_propagator *p, const Params &P, crp_batch *b, int *launches) {
 	const bool adaptive = !(p->minStep == p->maxStep);
-	if constexpr (!MIX)
-		if (p->type == 2)
-			return launch_propagate<F, PK_SDE, false>(f, p, P, b, launches);
 	if (p->type == 0)
 		return adaptive ? launch_propagate<F, PK_BP_ADAPTIVE, MIX>(f, p, P, b, launches)
 		                : launch_propagate<F, PK_BP_FIXED, MIX>(f, p, P, b, launches);
@@ -453,7 +450,12 @@ static int dispatch_kind_p(const F &f, const crp_propagator *p, const Params &P,
 
 template <class F>
 static int dispatch_kind(const F &f, const crp_propagator *p, const Params &P, crp_batch *b, int *launches) {
-	if (p->precision == CRP_PRECISION_MIXED && p->type != 2)
+	if (p->type == 2) {
+		if (p->box || p->sph || p->nObs > 0)
+			return launch_propagate<F, PK_SDE, true>(f, p, P, b, launches);
+		return launch_propagate<F, PK_SDE, false>(f, p, P, b, launches);
+	}
+	if (p->precision == CRP_PRECISION_MIXED)
 		return dispatch_kind_p<F, true>(f, p, P, b, launches);
 	return dispatch_kind_p<F, false>(f, p, P, b, launches);
 }

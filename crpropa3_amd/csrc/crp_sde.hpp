@@ -148,7 +148,9 @@ __device__ __forceinline__ V3 sde_randcone(crp_sde_rng &r, const V3 &mean, doubl
 
 enum SdePhase { SDE_BEGIN = 0, SDE_TRIAL = 1, SDE_SUB = 2 };
 
-template <class F>
+// MODS: the periodic box, spherical boundary and observer are compiled in (the length limit always is);
+// without them the kernel keeps fewer registers
+template <class F, bool MODS>
 __global__ void __launch_bounds__(256) k_sde(F field, Params P, Soa S, unsigned long long n,
                                              unsigned long long *__restrict__ ctr) {
 	const int lane = __lane_id();
@@ -206,7 +208,7 @@ __global__ void __launch_bounds__(256) k_sde(F field, Params P, Soa S, unsigned 
 						*cLen = S.len[idx];
 						*cTim = S.time[idx];
 						rstep = S.sdeStep[idx];
-						if (P.nObs > 0)
+						if (MODS && P.nObs > 0)
 							*cDet = (double)S.det[idx];
 						stream = S.sdeStream[idx];
 						if (q != 0) {
@@ -384,7 +386,7 @@ __global__ void __launch_bounds__(256) k_sde(F field, Params P, Soa S, unsigned 
 		steps++;
 		// the modules after DiffusionSDE in the list, as k_propagate applies them (they run even when the
 		// step deactivated the particle, as ModuleList::process does)
-		if (P.box) {
+		if (MODS && P.box) {
 			// PeriodicBox::process (src/module/Boundary.cpp:16-27)
 			const double nbx = __builtin_floor(div_rn(pos.x - P.boxO[0], P.boxS[0], P.boxInv[0]));
 			const double nby = __builtin_floor(div_rn(pos.y - P.boxO[1], P.boxS[1], P.boxInv[1]));
@@ -398,7 +400,7 @@ __global__ void __launch_bounds__(256) k_sde(F field, Params P, Soa S, unsigned 
 			}
 		}
 		double nextObs = next; // nextStep as the observer sees it
-		if (P.sph) {
+		if (MODS && P.sph) {
 			// SphericalBoundary::process (src/module/Boundary.cpp:216-223)
 			const double d = norm(v3(pos.x - P.sphC[0], pos.y - P.sphC[1], pos.z - P.sphC[2]));
 			if (d >= P.sphR)
@@ -420,7 +422,7 @@ __global__ void __launch_bounds__(256) k_sde(F field, Params P, Soa S, unsigned 
 					nextObs = dmin(nextObs, l);
 			}
 		}
-		if (P.nObs > 0) {
+		if (MODS && P.nObs > 0) {
 			// Observer with ObserverTimeEvolution (src/module/Observer.cpp:26-57, 279-306)
 			const int k = (int)*cDet;
 			if (k < P.nObs) {
@@ -452,7 +454,7 @@ __global__ void __launch_bounds__(256) k_sde(F field, Params P, Soa S, unsigned 
 			S.ppx[idx] = *cPPx; S.ppy[idx] = *cPPy; S.ppz[idx] = *cPPz;
 			S.pdx[idx] = *cPDx; S.pdy[idx] = *cPDy; S.pdz[idx] = *cPDz;
 			S.sdeStep[idx] = rstep;
-			if (P.nObs > 0)
+			if (MODS && P.nObs > 0)
 				S.det[idx] = (int32_t)*cDet;
 			nPause += alive ? 1 : 0;
 			have = false;
