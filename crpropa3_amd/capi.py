@@ -99,6 +99,9 @@ def lib():
         L.crp_propagator_create_sde.argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(_vp)]
         L.crp_propagator_set_sde.argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.c_uint32]
         L.crp_propagator_set_sde_advection.argtypes = [_vp, _dp]
+        L.crp_propagator_set_sde_constant_spherical_advection.argtypes = [_vp, _dp, C.c_double]
+        L.crp_propagator_set_sde_spherical_advection.argtypes = [_vp, _dp, C.c_double, C.c_double, C.c_double,
+                                                                 C.c_double]
         L.crp_propagator_set_max_trajectory_length.argtypes = [_vp, C.c_double]
         L.crp_propagator_set_precision.argtypes = [_vp, C.c_int]
         L.crp_propagator_destroy.argtypes = [_vp]
@@ -347,8 +350,15 @@ class Propagator:
         return self
 
     def set_sde_advection(self, v):
-        """UniformAdvectionField(v) of the DiffusionSDE; None removes it."""
-        if v is None:
+        """UniformAdvectionField(v) of the DiffusionSDE; None removes it; a dict {"kind": 2, "origin", "par":
+        [vWind]} is a ConstantSphericalAdvectionField, {"kind": 3, "origin", "par": [radius, vMax, tau, alpha]}
+        a SphericalAdvectionField."""
+        if isinstance(v, dict):
+            o = _f64(v["origin"], (3,))
+            fn = (lib().crp_propagator_set_sde_constant_spherical_advection if v["kind"] == 2
+                  else lib().crp_propagator_set_sde_spherical_advection)
+            check(fn(self.handle, _p(o), *[float(x) for x in v["par"]]))
+        elif v is None:
             check(lib().crp_propagator_set_sde_advection(self.handle, None))
         else:
             a = _f64(v, (3,))

@@ -92,7 +92,7 @@ struct crp_propagator {
 	// Params::postObs of the module order
 	int postObs;
 	// DiffusionSDE (type 2)
-	double sdeScale, sdeAlpha, sdeEpsilon, sdeAdv[3];
+	double sdeScale, sdeAlpha, sdeEpsilon, sdeAdv[3], sdeAdvO[3], sdeAdvPar[4];
 	int sdeAdvection;
 	uint32_t sdeSeed;
 };
@@ -1045,7 +1045,10 @@ static int create_prop(int type, const char *who, const crp_field_t *field, doub
 	p->sdeScale = 1;
 	p->sdeAlpha = 1. / 3.;
 	p->sdeEpsilon = 0.1;
-	p->sdeAdv[0] = p->sdeAdv[1] = p->sdeAdv[2] = 0;
+	for (int a = 0; a < 3; a++)
+		p->sdeAdv[a] = p->sdeAdvO[a] = 0;
+	for (int a = 0; a < 4; a++)
+		p->sdeAdvPar[a] = 0;
 	p->sdeAdvection = 0;
 	p->sdeSeed = 0;
 	*out = p;
@@ -1103,6 +1106,30 @@ int crp_propagator_set_sde_advection(crp_propagator_t *p, const double v[3]) {
 	p->sdeAdvection = v ? 1 : 0;
 	for (int a = 0; a < 3; a++)
 		p->sdeAdv[a] = v ? v[a] : 0.0;
+	return CRP_OK;
+}
+
+int crp_propagator_set_sde_constant_spherical_advection(crp_propagator_t *p, const double origin[3], double v_wind) {
+	if (!p || p->type != 2 || !origin)
+		return fail(CRP_E_INVALID, "crp_propagator_set_sde_constant_spherical_advection: bad arguments");
+	p->sdeAdvection = 2;
+	for (int a = 0; a < 3; a++)
+		p->sdeAdvO[a] = origin[a];
+	p->sdeAdvPar[0] = v_wind;
+	return CRP_OK;
+}
+
+int crp_propagator_set_sde_spherical_advection(crp_propagator_t *p, const double origin[3], double radius,
+                                               double v_max, double tau, double alpha) {
+	if (!p || p->type != 2 || !origin)
+		return fail(CRP_E_INVALID, "crp_propagator_set_sde_spherical_advection: bad arguments");
+	p->sdeAdvection = 3;
+	for (int a = 0; a < 3; a++)
+		p->sdeAdvO[a] = origin[a];
+	p->sdeAdvPar[0] = radius;
+	p->sdeAdvPar[1] = v_max;
+	p->sdeAdvPar[2] = tau;
+	p->sdeAdvPar[3] = alpha;
 	return CRP_OK;
 }
 
@@ -1509,8 +1536,12 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 		P.sdeAlpha = p->sdeAlpha;
 		P.sdeEpsilon = p->sdeEpsilon;
 		P.sdeAdvection = p->sdeAdvection;
-		for (int a = 0; a < 3; a++)
+		for (int a = 0; a < 3; a++) {
 			P.sdeAdv[a] = p->sdeAdv[a];
+			P.sdeAdvO[a] = p->sdeAdvO[a];
+		}
+		for (int a = 0; a < 4; a++)
+			P.sdeAdvPar[a] = p->sdeAdvPar[a];
 		P.sdeSeed = p->sdeSeed;
 	}
 	if (p->nObs > 0) {

@@ -60,7 +60,9 @@ struct Params {
 	int postObs;
 	// DiffusionSDE (k_sde): diffusion coefficient scale * 6.1e24 * (|E/q| / 4e9)^alpha, perpendicular
 	// fraction epsilon, uniform advection velocity, Philox seed
-	double sdeScale, sdeAlpha, sdeEpsilon, sdeAdv[3];
+	// sdeAdvection: 0 none, 1 uniform (sdeAdv), 2 constant spherical wind (origin sdeAdvO, vWind
+	// sdeAdvPar[0]), 3 SphericalAdvectionField (origin, radius, vMax, tau, alpha = sdeAdvPar[0..3])
+	double sdeScale, sdeAlpha, sdeEpsilon, sdeAdv[3], sdeAdvO[3], sdeAdvPar[4];
 	int sdeAdvection;
 	uint32_t sdeSeed;
 };

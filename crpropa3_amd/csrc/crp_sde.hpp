@@ -146,6 +146,25 @@ __device__ __forceinline__ V3 sde_randcone(crp_sde_rng &r, const V3 &mean, doubl
 	          mean.x * rz.x + mean.y * rz.y + mean.z * rz.z);
 }
 
+// driftStep (src/module/DiffusionSDE.cpp:212-216): lin = 0 + v(pos) h with the advection field's
+// velocity at the step's start (src/advectionField/AdvectionField.cpp:37-130)
+__device__ __forceinline__ void sde_drift(const Params &P, const V3 &pos, double h, double lin[3]) {
+	double v[3] = {P.sdeAdv[0], P.sdeAdv[1], P.sdeAdv[2]};
+	if (P.sdeAdvection >= 2) {
+		const V3 d = v3(pos.x - P.sdeAdvO[0], pos.y - P.sdeAdvO[1], pos.z - P.sdeAdvO[2]);
+		const double R = norm(d);
+		double vr = P.sdeAdvPar[0];
+		const bool outside = P.sdeAdvection == 3 && R > P.sdeAdvPar[0];
+		if (P.sdeAdvection == 3 && !outside)
+			vr = P.sdeAdvPar[1] * (1 - crp_exp_rn(-(crp_pow_cr(R, P.sdeAdvPar[3]) / P.sdeAdvPar[2])));
+		v[0] = outside ? 0.0 : (d.x / R) * vr;
+		v[1] = outside ? 0.0 : (d.y / R) * vr;
+		v[2] = outside ? 0.0 : (d.z / R) * vr;
+	}
+	for (int a = 0; a < 3; a++)
+		lin[a] = 0 + v[a] * h;
+}
+
 enum SdePhase { SDE_BEGIN = 0, SDE_TRIAL = 1, SDE_SUB = 2 };
 
 // MODS: the periodic box, spherical boundary and observer are compiled in (the length limit always is);
@@ -249,8 +268,7 @@ __global__ void __launch_bounds__(256) k_sde(F field, Params P, Soa S, unsigned 
 			if (q == 0) {
 				// rectilinear, with the advection drift (DiffusionSDE.cpp:62-75)
 				if (P.sdeAdvection)
-					for (int a = 0; a < 3; a++)
-						lin[a] = 0 + P.sdeAdv[a] * h;
+					sde_drift(P, pos, h, lin);
 				nPos = v3((pos.x + lin[0]) + (dir.x * h) * kCLight, (pos.y + lin[1]) + (dir.y * h) * kCLight,
 				          (pos.z + lin[2]) + (dir.z * h) * kCLight);
 				next = P.maxStep;
@@ -326,8 +344,7 @@ __global__ void __launch_bounds__(256) k_sde(F field, Params P, Soa S, unsigned 
 				const V3 T = sde_unit(v3(out.x - pos.x, out.y - pos.y, out.z - pos.z));
 				const double tTest = norm(T);
 				if (P.sdeAdvection)
-					for (int a = 0; a < 3; a++)
-						lin[a] = 0 + P.sdeAdv[a] * h;
+					sde_drift(P, pos, h, lin);
 				if (tTest != tTest) {
 					if (P.sdeAdvection)
 						nPos = v3(pos.x + lin[0], pos.y + lin[1], pos.z + lin[2]);

@@ -457,6 +457,29 @@ PYBIND11_MODULE(_crpropa, m) {
 	    .def(py::init<const Vector3d &>())
 	    .def("getDescription", &UniformAdvectionField::getDescription);
 
+	py::class_<ConstantSphericalAdvectionField, AdvectionField, ref_ptr<ConstantSphericalAdvectionField>>(
+	    m, "ConstantSphericalAdvectionField")
+	    .def(py::init<const Vector3d, double>())
+	    .def("setOrigin", &ConstantSphericalAdvectionField::setOrigin)
+	    .def("setVWind", &ConstantSphericalAdvectionField::setVWind)
+	    .def("getOrigin", &ConstantSphericalAdvectionField::getOrigin)
+	    .def("getVWind", &ConstantSphericalAdvectionField::getVWind)
+	    .def("getDescription", &ConstantSphericalAdvectionField::getDescription);
+	py::class_<SphericalAdvectionField, AdvectionField, ref_ptr<SphericalAdvectionField>>(m, "SphericalAdvectionField")
+	    .def(py::init<const Vector3d, double, double, double, double>())
+	    .def("getV", &SphericalAdvectionField::getV)
+	    .def("setOrigin", &SphericalAdvectionField::setOrigin)
+	    .def("setRadius", &SphericalAdvectionField::setRadius)
+	    .def("setVMax", &SphericalAdvectionField::setVMax)
+	    .def("setTau", &SphericalAdvectionField::setTau)
+	    .def("setAlpha", &SphericalAdvectionField::setAlpha)
+	    .def("getOrigin", &SphericalAdvectionField::getOrigin)
+	    .def("getRadius", &SphericalAdvectionField::getRadius)
+	    .def("getVMax", &SphericalAdvectionField::getVMax)
+	    .def("getTau", &SphericalAdvectionField::getTau)
+	    .def("getAlpha", &SphericalAdvectionField::getAlpha)
+	    .def("getDescription", &SphericalAdvectionField::getDescription);
+
 	py::class_<DiffusionSDE, DevicePropagator, ref_ptr<DiffusionSDE>>(m, "DiffusionSDE")
 	    .def(py::init<ref_ptr<MagneticField>, double, double, double, double>(), py::arg("magneticField"),
 	         py::arg("tolerance") = 1e-4, py::arg("minStep") = 10 * pc, py::arg("maxStep") = 1 * kpc,

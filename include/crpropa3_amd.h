@@ -169,6 +169,13 @@ int crp_propagator_set_sde(crp_propagator_t *p, double alpha, double scale, doub
 /* UniformAdvectionField(v) of DiffusionSDE(field, advectionField, ...) -- src/advectionField/
  * AdvectionField.cpp:37-50; v == NULL removes it */
 int crp_propagator_set_sde_advection(crp_propagator_t *p, const double v[3]);
+/* ConstantSphericalAdvectionField(origin, vWind): v = vWind (x - origin) / |x - origin|
+ * (src/advectionField/AdvectionField.cpp:52-56) */
+int crp_propagator_set_sde_constant_spherical_advection(crp_propagator_t *p, const double origin[3], double v_wind);
+/* SphericalAdvectionField(origin, radius, vMax, tau, alpha): v = vMax (1 - exp(-R^alpha / tau)) along
+ * x - origin inside the radius, 0 outside (AdvectionField.cpp:90-120) */
+int crp_propagator_set_sde_spherical_advection(crp_propagator_t *p, const double origin[3], double radius,
+                                               double v_max, double tau, double alpha);
 
 /* MaximumTrajectoryLength(maxLength) fused after each step -- src/module/BreakCondition.cpp:42-64
  * (no observer positions).  max_length <= 0 removes it. */

@@ -1362,9 +1362,21 @@ static void orc_sde_process(const orc_field *f, const orc_sde *p, orc_state *s, 
 		}
 	const double h = orc_clip(s->next_step[i], p->minStep, p->maxStep) / C_LIGHT;
 	double lin[3] = {0, 0, 0};
-	if (p->advection)
+	if (p->advection) {
+		/* driftStep (DiffusionSDE.cpp:212-216): LinProp(0) += advField(pos) * h */
+		double v[3] = {p->adv[0], p->adv[1], p->adv[2]};
+		if (p->adv_kind == 2 || p->adv_kind == 3) {
+			const double P[3] = {pos[0] - p->adv_origin[0], pos[1] - p->adv_origin[1], pos[2] - p->adv_origin[2]};
+			const double R = sde_r(P);
+			double vr = p->adv_par[0]; /* ConstantSphericalAdvectionField: vWind */
+			if (p->adv_kind == 3) /* SphericalAdvectionField::getV, zero beyond the radius */
+				vr = R > p->adv_par[0] ? 0.0 : p->adv_par[1] * (1 - orc_exp(-(orc_pow(R, p->adv_par[3], libm) / p->adv_par[2]), libm));
+			for (int a = 0; a < 3; a++)
+				v[a] = (p->adv_kind == 3 && R > p->adv_par[0]) ? 0.0 : (P[a] / R) * vr;
+		}
 		for (int a = 0; a < 3; a++)
-			lin[a] = 0 + p->adv[a] * h;
+			lin[a] = 0 + v[a] * h;
+	}
 	if (s->charge[i] == 0) { /* rectilinear, with the advection drift */
 		for (int a = 0; a < 3; a++)
 			pos[a] = (pos[a] + lin[a]) + (dir[a] * h) * C_LIGHT;

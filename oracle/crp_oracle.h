@@ -132,6 +132,11 @@ typedef struct {
 	double maxTrajectoryLength; /* MaximumTrajectoryLength after the step (<= 0: none) */
 	int rng;
 	uint32_t seed;
+	/* advection kind when advection != 0: 1 uniform (adv), 2 ConstantSphericalAdvectionField (origin,
+	 * vWind = par[0]), 3 SphericalAdvectionField (origin, radius, vMax, tau, alpha = par[0..3]);
+	 * src/advectionField/AdvectionField.cpp:37-130 */
+	int adv_kind;
+	double adv_origin[3], adv_par[4];
 } orc_sde;
 long long orc_sde_propagate(const orc_field *f, const orc_sde *p, orc_state *s, size_t n, long long maxSteps,
                             uint32_t *sde_step, const uint64_t *sde_stream);

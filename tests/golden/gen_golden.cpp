@@ -691,6 +691,26 @@ static void gen_sde() {
 		                 fmt(pc) + ", \"maxStep\": " + fmt(10 * kpc) + ", \"epsilon\": 0.2, \"alpha\": " + fmt(1. / 3.) +
 		                 ", \"scale\": 1, \"maxTrajectoryLength\": " + fmt(20 * kpc) + ", \"seed\": 4713}");
 	}
+	{
+		// galactic winds: ConstantSphericalAdvectionField and SphericalAdvectionField around an offset origin
+		ref_ptr<MagneticField> f = new UniformMagneticField(Vector3d(0.3 * nG, 0, 1 * nG));
+		const Vector3d o(1 * kpc, -0.5 * kpc, 0.2 * kpc);
+		ref_ptr<AdvectionField> a1 = new ConstantSphericalAdvectionField(o, 3e5);
+		ref_ptr<AdvectionField> a2 = new SphericalAdvectionField(o, 6 * kpc, 4e5, 2 * kpc, 1.5);
+		const char *names[2] = {"sde_advection_cspherical", "sde_advection_spherical"};
+		const std::string par[2] = {"[300000.0]", "[" + fmt(6 * kpc) + ", 400000.0, " + fmt(2 * kpc) + ", 1.5]"};
+		for (int k = 0; k < 2; k++) {
+			DiffusionSDE sde(f, k == 0 ? a1 : a2, 1e-4, 1 * pc, 10 * kpc, 0.2);
+			std::vector<Particle> ps = make_particles(32, 81 + k, 1 * TeV, 1 * PeV, Vector3d(0.), species, wts);
+			run_sde_case(names[k], sde, 20 * kpc, ps, 300, 4714 + k,
+			             "{\"field\": {\"type\": \"uniform\", \"B\": [" + fmt(0.3 * nG) + ", 0, " + fmt(nG) +
+			                 "]}, \"advection_kind\": " + std::to_string(k + 2) + ", \"advection_origin\": [" + fmt(o.x) +
+			                 ", " + fmt(o.y) + ", " + fmt(o.z) + "], \"advection_par\": " + par[k] +
+			                 ", \"tolerance\": 1e-4, \"minStep\": " + fmt(pc) + ", \"maxStep\": " + fmt(10 * kpc) +
+			                 ", \"epsilon\": 0.2, \"alpha\": " + fmt(1. / 3.) + ", \"scale\": 1, \"maxTrajectoryLength\": " +
+			                 fmt(20 * kpc) + ", \"seed\": " + std::to_string(4714 + k) + "}");
+		}
+	}
 }
 
 static void gen_planewave_case(const std::string &name, const TurbulenceSpectrum &spec, int Nm, int seed,

@@ -70,7 +70,8 @@ class JF12Params(C.Structure):
 class SDE(C.Structure):
     _fields_ = [("tolerance", C.c_double), ("minStep", C.c_double), ("maxStep", C.c_double), ("epsilon", C.c_double),
                 ("alpha", C.c_double), ("scale", C.c_double), ("advection", C.c_int), ("adv", C.c_double * 3),
-                ("maxTrajectoryLength", C.c_double), ("rng", C.c_int), ("seed", C.c_uint32)]
+                ("maxTrajectoryLength", C.c_double), ("rng", C.c_int), ("seed", C.c_uint32), ("adv_kind", C.c_int),
+                ("adv_origin", C.c_double * 3), ("adv_par", C.c_double * 4)]
 
 
 SDE_MT, SDE_PHILOX = 0, 1
@@ -569,10 +570,7 @@ def sde_propagate(field, state, steps, tolerance, min_step, max_step, epsilon=0.
     reference's stream (serial, particle by particle), SDE_PHILOX the device's (sde_step: uint32 step
     counters, updated; sde_stream: uint64 stream numbers, default the particle index)."""
     p = SDE(tolerance, min_step, max_step, epsilon, alpha, scale, 0, (C.c_double * 3)(0, 0, 0), max_len, rng, seed)
-    if advection is not None:
-        p.advection = 1
-        for a in range(3):
-            p.adv[a] = advection[a]
+    set_advection(p, advection)
     s = state.struct()
     cnt = None
     if sde_step is not None:
@@ -585,14 +583,28 @@ def sde_propagate(field, state, steps, tolerance, min_step, max_step, epsilon=0.
     return lib().orc_sde_propagate(C.byref(field.s), C.byref(p), C.byref(s), state.n, steps, cnt, stream)
 
 
+def set_advection(p, advection):
+    """advection: None, a uniform velocity (3 numbers), or a dict {"kind": 2|3, "origin": [..], "par": [..]}"""
+    if advection is None:
+        return
+    p.advection = 1
+    if isinstance(advection, dict):
+        p.adv_kind = advection["kind"]
+        for a in range(3):
+            p.adv_origin[a] = advection["origin"][a]
+        for a, v in enumerate(advection["par"]):
+            p.adv_par[a] = v
+    else:
+        p.adv_kind = 1
+        for a in range(3):
+            p.adv[a] = advection[a]
+
+
 def sde_propagate_threads(field, state, steps, tolerance, min_step, max_step, epsilon=0.1, alpha=1 / 3., scale=1.0,
                           advection=None, max_len=0.0, seed=1, nthreads=0):
     """DiffusionSDE with one MT19937 per OpenMP thread (the reference's threading); returns the steps."""
     p = SDE(tolerance, min_step, max_step, epsilon, alpha, scale, 0, (C.c_double * 3)(0, 0, 0), max_len, SDE_MT, seed)
-    if advection is not None:
-        p.advection = 1
-        for a in range(3):
-            p.adv[a] = advection[a]
+    set_advection(p, advection)
     s = state.struct()
     return lib().orc_sde_propagate_threads(C.byref(field.s), C.byref(p), C.byref(s), state.n, steps, nthreads)
 
@@ -602,6 +614,9 @@ def sde_args(meta):
               alpha=meta["alpha"], scale=meta["scale"], max_len=meta.get("maxTrajectoryLength", 0.0), seed=meta["seed"])
     if "advection" in meta:
         kw["advection"] = meta["advection"]
+    if "advection_kind" in meta:
+        kw["advection"] = {"kind": meta["advection_kind"], "origin": meta["advection_origin"],
+                           "par": meta["advection_par"]}
     return kw
 
 

@@ -55,7 +55,8 @@ def compare(out, st, steps):
     assert np.array_equal(out["sde_step"], steps)
 
 
-@pytest.mark.parametrize("name", ["sde_uniform", "sde_grid32", "sde_advection"])
+@pytest.mark.parametrize("name", ["sde_uniform", "sde_grid32", "sde_advection", "sde_advection_cspherical",
+                                  "sde_advection_spherical"])
 @pytest.mark.parametrize("split", [None, (3, 5, -1), "launch"])
 def test_sde_bit_exact_vs_oracle(name, split):
     c = O.load_case(name)

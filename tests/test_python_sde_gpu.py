@@ -76,9 +76,18 @@ def state(cands):
     return pos, d, length, nxt, steps
 
 
-def sim_for(seed):
+ADVECTIONS = {
+    "uniform": ([0, 0, 5e5], lambda: crp.UniformAdvectionField(crp.Vector3d(0, 0, 5e5))),
+    "cspherical": ({"kind": 2, "origin": [0.5 * O.kpc, 0, 0], "par": [3e5]},
+                   lambda: crp.ConstantSphericalAdvectionField(crp.Vector3d(0.5 * O.kpc, 0, 0), 3e5)),
+    "spherical": ({"kind": 3, "origin": [0.5 * O.kpc, 0, 0], "par": [8 * O.kpc, 4e5, 2 * O.kpc, 1.5]},
+                  lambda: crp.SphericalAdvectionField(crp.Vector3d(0.5 * O.kpc, 0, 0), 8 * O.kpc, 4e5, 2 * O.kpc, 1.5)),
+}
+
+
+def sim_for(seed, adv="uniform"):
     B = crp.UniformMagneticField(crp.Vector3d(1e-13, 0, 3e-13))
-    d = crp.DiffusionSDE(B, crp.UniformAdvectionField(crp.Vector3d(0, 0, 5e5)), 1e-4, 1 * crp.pc, 200 * crp.pc, 0.3)
+    d = crp.DiffusionSDE(B, ADVECTIONS[adv][1](), 1e-4, 1 * crp.pc, 200 * crp.pc, 0.3)
     d.setSeed(seed)
     sim = crp.ModuleList()
     sim.add(d)
@@ -112,20 +121,21 @@ def test_modulelist_split_runs_and_process_loop_agree():
     assert not any(c.isActive() for c in a + b)
 
 
-def test_modulelist_matches_oracle_streams():
+@pytest.mark.parametrize("adv", sorted(ADVECTIONS))
+def test_modulelist_matches_oracle_streams(adv):
     n = 48
     cands = make_candidates(n, seed=9)
     pos0, dir0, _, _, _ = state(cands)
     E = np.array([c.current.getEnergy() for c in cands])
     q = np.array([c.current.getCharge() for c in cands])
     serial = np.array([c.getSerialNumber() for c in cands], dtype=np.uint64)
-    sim, d = sim_for(5)
+    sim, d = sim_for(5, adv)
     sim.run(cands)
     st = O.OracleState(pos0, dir0, E, q)
     st.dir = dir0.copy()  # the candidates' unit vectors as they are (normalising again may change bits)
     steps = np.zeros(n, dtype=np.uint32)
     O.sde_propagate(O.OracleField.uniform([1e-13, 0, 3e-13]), st, 10 ** 6, tolerance=1e-4, min_step=1 * crp.pc,
-                    max_step=200 * crp.pc, epsilon=0.3, advection=[0, 0, 5e5], max_len=20 * crp.kpc,
+                    max_step=200 * crp.pc, epsilon=0.3, advection=ADVECTIONS[adv][0], max_len=20 * crp.kpc,
                     rng=O.SDE_PHILOX, seed=5, sde_step=steps, sde_stream=serial)
     pos, dd, length, nxt, k = state(cands)
     assert np.array_equal(pos.view(np.uint64), st.pos.view(np.uint64))

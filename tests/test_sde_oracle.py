@@ -14,7 +14,8 @@ def bits_equal(a, b):
     return np.array_equal(a.view(np.uint64), b.view(np.uint64))
 
 
-@pytest.mark.parametrize("name", ["sde_uniform", "sde_grid32", "sde_advection"])
+@pytest.mark.parametrize("name", ["sde_uniform", "sde_grid32", "sde_advection", "sde_advection_cspherical",
+                                  "sde_advection_spherical"])
 def test_oracle_sde_bit_exact_vs_reference(name):
     c = O.load_case(name)
     m = c["meta"]
