@@ -57,6 +57,9 @@ protected:
 
 private:
 	Vector3d evalWith(const Vector3d &pos, int regular, int striated, int turbulent) const;
+	// one field per part (regular, striated, turbulent) for the part getters: each keeps its device
+	// copy while the switches and grids stay the same
+	mutable ref_ptr<JF12Field> parts[3];
 	mutable crp_field *dev = nullptr;
 	mutable crp_jf12_params devParams;
 	mutable const Grid1f *devStr = nullptr;

@@ -377,14 +377,19 @@ crp_field *JF12Field::deviceField() const {
 }
 
 Vector3d JF12Field::evalWith(const Vector3d &pos, int regular, int striated, int turbulent) const {
-	JF12Field tmp(0);
-	tmp.p = p;
-	tmp.p.use_regular = regular;
-	tmp.p.use_striated = striated && striatedGrid.valid();
-	tmp.p.use_turbulent = turbulent && turbulentGrid.valid();
-	tmp.striatedGrid = striatedGrid;
-	tmp.turbulentGrid = turbulentGrid;
-	return tmp.getField(pos);
+	const int k = turbulent ? 2 : (striated ? 1 : 0);
+	// the part fields are shared by all calls on this field: one call at a time
+	std::lock_guard<std::mutex> lock(mtx);
+	if (!parts[k].valid())
+		parts[k] = new JF12Field(0);
+	ref_ptr<JF12Field> f = parts[k];
+	f->p = p;
+	f->p.use_regular = regular;
+	f->p.use_striated = striated && striatedGrid.valid();
+	f->p.use_turbulent = turbulent && turbulentGrid.valid();
+	f->striatedGrid = striatedGrid;
+	f->turbulentGrid = turbulentGrid;
+	return f->getField(pos);
 }
 
 Vector3d JF12Field::getRegularField(const Vector3d &pos) const { return evalWith(pos, 1, 0, 0); }
