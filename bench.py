@@ -86,15 +86,31 @@ WORKLOADS = {
 }
 
 
+def draw_particles(seed, n, emin, emax, weights=(1.0,)):
+    """Species bins, energies and raw directions as a reference steering script draws them: one
+    crpropa Random seeded with `seed`; per particle randBin over the cumulative weights (when there
+    is more than one species), randPowerLaw(-1, emin, emax), randVector (src/Random.cpp:129-146,
+    195-198)."""
+    from crpropa3_amd import Random
+    r = Random(int(seed))
+    cdf = list(np.cumsum(np.asarray(weights, dtype=np.float64)))
+    sp = np.zeros(n, dtype=np.int64)
+    E = np.empty(n)
+    d = np.empty((n, 3))
+    for i in range(n):
+        if len(cdf) > 1:
+            sp[i] = r.randBin(cdf)
+        E[i] = r.randPowerLaw(-1.0, emin, emax)
+        v = r.randVector()
+        d[i] = (v.x, v.y, v.z)
+    return sp, E, d
+
+
 def make_particles(w, n):
-    rng = np.random.Generator(np.random.PCG64(w["seed"] + 1000))
-    sp = rng.choice(len(w["species"]), size=n, p=np.asarray(w["weights"]) / np.sum(w["weights"]))
+    """Start positions, unit directions, energies and charges of a workload's first n particles, drawn
+    with the workload's seed (draw_particles); directions normalised as ParticleState::setDirection."""
+    sp, E, d = draw_particles(w["seed"], n, w["emin"], w["emax"], w["weights"])
     Z = np.asarray(w["species"], dtype=np.float64)[sp]
-    E = np.exp(rng.uniform(np.log(w["emin"]), np.log(w["emax"]), n))
-    z = rng.uniform(-1, 1, n)
-    t = rng.uniform(-np.pi, np.pi, n)
-    r = np.sqrt(1 - z * z)
-    d = np.stack([r * np.cos(t), r * np.sin(t), z], axis=1)
     d /= np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])[:, None]
     if w["start"] == "earth":
         pos = np.tile([-8.5 * kpc, 0.0, 0.0], (n, 1))
@@ -126,11 +142,22 @@ def make_field_data(w):
             "kappa": np.array([[v.x, v.y, v.z] for v in pw.getKappa()])}
 
 
-def cpu_baseline(w, fdata, pos, d, E, q, target_s=15.0):
-    """The oracle (CPU restatement with glibc libm, OpenMP) on a bounded sample."""
+def host_threads():
+    """OpenMP threads for the CPU baseline: every CPU this process may run on, capped by
+    OMP_NUM_THREADS when the environment sets it (the host's share of CPUs for this job)."""
+    avail = len(os.sched_getaffinity(0))
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return (min(env, avail) if env > 0 else avail), avail
+
+
+def cpu_baseline(w, fdata, pos, d, E, q, gpu_final=None, target_s=15.0):
+    """The oracle (CPU restatement with glibc libm, OpenMP) on a bounded sample.  With `gpu_final`
+    (the GPU pass's final states) the sample is also a parity check: the GPU against the oracle with
+    glibc (the reference's arithmetic) and, on its first particles, bit for bit against the oracle
+    with the correctly rounded functions the kernels use."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ctypes as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    threads, avail = host_threads()
     if w["field"] == "grid":
         f = O.OracleField.grid(fdata, [0, 0, 0], [w["spacing"]] * 3)
     elif w["field"] == "jf12":
@@ -143,7 +170,7 @@ def cpu_baseline(w, fdata, pos, d, E, q, target_s=15.0):
     steps = w.get("steps", 10 ** 12)
 
     def run(m, nsteps):
-        st = O.OracleState(pos[:m], d[:m], E[:m], q[:m])
+        st = O.OracleState(pos[:m], d[:m], E[:m], q[:m], normalised=True)
         t0 = time.perf_counter()
         if w["kind"] == "SDE":
             # the reference's threading: one MT19937 per OpenMP thread, glibc libm
@@ -154,20 +181,20 @@ def cpu_baseline(w, fdata, pos, d, E, q, target_s=15.0):
             nacc, _ = O.propagate(f, kind, st, nsteps, tolerance=w["tolerance"], min_step=w["min_step"],
                                   max_step=w["max_step"], max_len=w["max_length"], libm=O.LIBM_GLIBC,
                                   nthreads=threads)
-        return nacc, time.perf_counter() - t0
+        return nacc, time.perf_counter() - t0, st
 
     m = min(len(E), 16 * threads)
     nsteps = steps
-    nacc, dt = run(m, nsteps)
+    nacc, dt, st = run(m, nsteps)
     if dt > target_s and steps < 10 ** 12:  # too slow even for the calibration: fewer steps
         nsteps = max(1, int(steps * target_s / dt / 4))
-        nacc, dt = run(m, nsteps)
+        nacc, dt, st = run(m, nsteps)
     while dt < target_s / 8 and m < len(E):  # grow the sample until every thread has a share of it
         m = min(len(E), 4 * m)
-        nacc, dt = run(m, nsteps)
+        nacc, dt, st = run(m, nsteps)
     m2 = int(min(len(E), max(m, m * target_s / max(dt, 1e-3))))
     if m2 > m:
-        nacc, dt = run(m2, nsteps)
+        nacc, dt, st = run(m2, nsteps)
         m = m2
     reps = 1
     while dt * reps < 1.0 and reps < 64:  # whole workload too short to time once: repeat it
@@ -175,15 +202,33 @@ def cpu_baseline(w, fdata, pos, d, E, q, target_s=15.0):
     if reps > 1:
         t = 0.0
         for _ in range(reps):
-            nacc, d1 = run(m, nsteps)
+            nacc, d1, st = run(m, nsteps)
             t += d1
         nacc, dt = nacc * reps, t
     what = "run to completion" if nsteps >= steps else "%d of %d steps" % (nsteps, steps)
     if reps > 1:
         what += ", %d times" % reps
-    return {"value": nacc / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": "first %d of the %d particles, same field and propagator, %s (%d accepted steps in %.1f s)"
-                      % (m, len(E), what, nacc, dt)}
+    out = {"value": nacc / dt, "unit": "particle-steps/s", "cores": threads, "host_cpus": avail, "kind": "port",
+           "sample": "first %d of the %d particles, same field and propagator, %s (%d accepted steps in %.1f s)"
+                     % (m, len(E), what, nacc, dt)}
+    if gpu_final is not None and w["kind"] != "SDE":
+        if nsteps < steps:
+            out["parity"] = "not compared: the sample ran fewer steps than the GPU pass"
+        else:
+            rel = np.max(np.abs(gpu_final["pos"][:m] - st.pos), axis=1) / np.max(np.abs(st.pos), axis=1)
+            k = min(m, 4096)
+            cr = O.OracleState(pos[:k], d[:k], E[:k], q[:k], normalised=True)
+            O.propagate(f, kind, cr, nsteps, tolerance=w["tolerance"], min_step=w["min_step"],
+                        max_step=w["max_step"], max_len=w["max_length"], libm=O.LIBM_CR, nthreads=threads)
+            exact = np.ones(k, dtype=bool)
+            for key in ("pos", "dir", "trajectory_length", "next_step"):
+                a = np.ascontiguousarray(gpu_final[key][:k]).view(np.uint64).reshape(k, -1)
+                b = np.ascontiguousarray(getattr(cr, key)).view(np.uint64).reshape(k, -1)
+                exact &= np.all(a == b, axis=1)
+            out["parity"] = {"vs_oracle_correctly_rounded": {"particles": k, "bit_exact": int(exact.sum())},
+                             "vs_oracle_glibc": {"particles": m, "within_1e-9_relative": float(np.mean(rel <= 1e-9)),
+                                                 "median_relative": float(np.median(rel))}}
+    return out
 
 
 ADJOINT = dict(n=65536, grid_n=128, spacing=20 * kpc, lmin_cells=2, lmax_cells=128, seed=5, step=10 * kpc,
@@ -197,12 +242,12 @@ def adjoint_inputs(a, n):
     from crpropa3_amd.turbulence import simple_grid_turbulence
     grid = simple_grid_turbulence(a["grid_n"], a["spacing"], 1 * nG, a["lmin_cells"] * a["spacing"],
                                   a["lmax_cells"] * a["spacing"], a["seed"]).astype(np.float32)
-    rng = np.random.Generator(np.random.PCG64(a["seed"] + 1000))
-    E = np.exp(rng.uniform(np.log(a["emin"]), np.log(a["emax"]), n))
-    d = rng.normal(size=(n, 3))
-    d /= np.linalg.norm(d, axis=1)[:, None]
-    tgt = rng.normal(size=(n, 3))
-    tgt /= np.linalg.norm(tgt, axis=1)[:, None]
+    # particles as draw_particles with the grid's seed, then the target directions from the next
+    # seed (randVector, normalised)
+    _, E, d = draw_particles(a["seed"], n, a["emin"], a["emax"])
+    _, _, tgt = draw_particles(a["seed"] + 1, n, a["emin"], a["emax"])
+    d /= np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])[:, None]
+    tgt /= np.sqrt((tgt[:, 0] * tgt[:, 0] + tgt[:, 1] * tgt[:, 1]) + tgt[:, 2] * tgt[:, 2])[:, None]
     L = a["grid_n"] * a["spacing"]
     pos = np.tile([L / 2] * 3, (n, 1))
     return grid, pos, d, E, tgt
@@ -236,15 +281,15 @@ def adjoint_cpu_baseline(a, grid, pos, d, E, tgt, target_s=15.0):
                                       % (m, len(E), a["steps"], dt)}
 
 
-def bench_adjoint(args):
+def adjoint_measure(args, steps, warmup):
     """Forward (checkpoint every K) + backward (recompute + reverse step, scatter-add) of the
-    differentiable Boris model; value = differentiated particle-steps per second."""
+    differentiable Boris model on the C5 workload; HIP-event times of both halves on torch's stream."""
     import torch
     from crpropa3_amd.adjoint import boris_grid_propagate
     a = dict(ADJOINT)
-    if args.propagation_steps:
+    if args.workload == "adjoint_grid128" and args.propagation_steps:
         a["steps"] = args.propagation_steps
-    n = args.particles or a["n"]
+    n = (args.particles if args.workload == "adjoint_grid128" else 0) or a["n"]
     grid_np, pos, d, E, tgt = adjoint_inputs(a, n)
     dev = torch.device("cuda:0")
     grid = torch.tensor(grid_np, device=dev, requires_grad=True)
@@ -267,20 +312,34 @@ def bench_adjoint(args):
         ev[2].record()
         return ev
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         one()
     torch.cuda.synchronize()
     evs = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         evs.append(one())
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    fwd = sum(e[0].elapsed_time(e[1]) for e in evs) / args.steps
-    bwd = sum(e[1].elapsed_time(e[2]) for e in evs) / args.steps
+    fwd = sum(e[0].elapsed_time(e[1]) for e in evs) / steps
+    bwd = sum(e[1].elapsed_time(e[2]) for e in evs) / steps
+    nsteps = n * a["steps"]
+    return {"workload": "adjoint_grid128", "particles": n, "propagation_steps": a["steps"], "step_m": a["step"],
+            "checkpoint_every": a["K"], "grid": "%d^3 FP32, spacing %.0f kpc" % (a["grid_n"], a["spacing"] / kpc),
+            "forward_ms": fwd, "backward_ms": bwd, "adjoint_over_forward": bwd / fwd,
+            "gradient_over_forward": (fwd + bwd) / fwd, "value": nsteps * steps / dt,
+            "unit": "particle-steps/s (loss and gradient)", "ms_per_step": dt / steps * 1e3,
+            "inputs": (a, grid_np, pos, d, E, tgt)}
+
+
+def bench_adjoint(args):
+    """The adjoint workload alone; value = differentiated particle-steps per second."""
+    r = adjoint_measure(args, args.steps, args.warmup)
+    a, grid_np, pos, d, E, tgt = r.pop("inputs")
+    n = r["particles"]
     # the product path's forward on the same particles (FP32 trilinear, fixed-step BP), for comparison
     from crpropa3_amd import capi
-    f = capi.Field.grid3f(grid_np, [0, 0, 0], sp)
+    f = capi.Field.grid3f(grid_np, [0, 0, 0], [a["spacing"]] * 3)
     prop = capi.Propagator("BP", f, 0.42, a["step"], a["step"], precision=capi.PRECISION_FP64)
     init = capi.Batch(n)
     init.upload(pos, d, E, np.full(n, eplus))
@@ -289,30 +348,101 @@ def bench_adjoint(args):
     prop.propagate(work, max_steps=a["steps"])
     work.copy_from(init)
     st = prop.propagate(work, max_steps=a["steps"])
-    steps = n * a["steps"]
-    bytes_per_launch = steps * ADJ_BYTES_PER_STEP
-    achieved = bytes_per_launch / (bwd / 1e3) / 1e9
+    bytes_per_launch = n * a["steps"] * ADJ_BYTES_PER_STEP
+    achieved = bytes_per_launch / (r["backward_ms"] / 1e3) / 1e9
+    config = {k: r[k] for k in ("workload", "particles", "propagation_steps", "step_m", "checkpoint_every", "grid",
+                                "forward_ms", "backward_ms", "adjoint_over_forward", "gradient_over_forward")}
+    config["propagate_bp_fp64_ms"] = st.kernel_ms
     out = {
         "metric": "particle-steps/s (1 MI355X), FP64 vs mixed at stated error; adjoint/fwd time",
-        "value": steps * args.steps / dt,
-        "unit": "particle-steps/s (loss and gradient)",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
+        "value": r["value"], "unit": r["unit"],
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: seeded turbulent field, particles and target directions",
-        "config": {"workload": "adjoint_grid128", "particles": n, "propagation_steps": a["steps"],
-                   "step_m": a["step"], "checkpoint_every": a["K"],
-                   "grid": "%d^3 FP32, spacing %.0f kpc" % (a["grid_n"], a["spacing"] / kpc),
-                   "forward_ms": fwd, "backward_ms": bwd, "adjoint_over_forward": bwd / fwd,
-                   "gradient_over_forward": (fwd + bwd) / fwd,
-                   "propagate_bp_fp64_ms": st.kernel_ms / max(st.launches, 1) * st.launches},
+        "config": config,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_adjoint_backward",
-                     "kernel_ms_avg": bwd, "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "kernel_ms_avg": r["backward_ms"], "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = adjoint_cpu_baseline(a, grid_np, pos, d, E, tgt)
     print(json.dumps(out), flush=True)
+
+
+def timed_passes(capi, prop, init, work, max_steps, steps, warmup):
+    """`warmup` untimed passes, then `steps` timed passes of the whole batch.  The batch is reset to
+    its initial state before each pass, outside the timed region (crp_batch_copy synchronises its
+    stream); each pass is bracketed by device synchronisation.  Returns accepted steps, field
+    evaluations, kernel milliseconds (HIP events on the kernel's stream), launches and wall seconds."""
+    for _ in range(warmup):
+        work.copy_from(init)
+        prop.propagate(work, max_steps=max_steps)
+    capi.check(capi.lib().crp_device_synchronize())
+    acc = evals = launches = 0
+    kms = dt = 0.0
+    for _ in range(steps):
+        work.copy_from(init)
+        capi.check(capi.lib().crp_device_synchronize())
+        t0 = time.perf_counter()
+        st = prop.propagate(work, max_steps=max_steps)
+        capi.check(capi.lib().crp_device_synchronize())
+        dt += time.perf_counter() - t0
+        acc += st.accepted_steps
+        evals += st.field_evals
+        kms += st.kernel_ms
+        launches += st.launches
+    return acc, evals, kms, launches, dt
+
+
+def position_error(a, b):
+    """Relative final-position error of run a against run b per particle: max-component |dx| / |x|."""
+    return np.max(np.abs(a - b), axis=1) / np.max(np.abs(b), axis=1)
+
+
+def paired_diffusion(pos_a, pos_b, x0, L):
+    """Running diffusion coefficients D_perp = <dx^2+dy^2>/(4L), D_par = <dz^2>/(2L) of runs a and b of the
+    same particles, their paired difference and its standard error."""
+    out = {}
+    for name, sel, den in (("perp", (0, 1), 4 * L), ("par", (2,), 2 * L)):
+        a = np.sum((pos_a - x0)[:, sel] ** 2, axis=1) / den
+        b = np.sum((pos_b - x0)[:, sel] ** 2, axis=1) / den
+        diff = a - b
+        se = diff.std(ddof=1) / np.sqrt(len(diff))
+        out[name] = {"mixed": a.mean(), "fp64": b.mean(), "relative_difference": diff.mean() / b.mean(),
+                     "standard_errors": diff.mean() / se if se > 0 else 0.0}
+    return out
+
+
+def mixed_block(capi, w, fdata, field, init, work, pos, final64, value64, args):
+    """The mixed path (FP32 field/rotation/direction stages, FP64 positions and step control) on the same
+    particles with FP32 cells and with FP16 bricks: rate, speed-up over the FP64 value, and the error of
+    its final states against the FP64 pass's."""
+    out = {}
+    L = w["max_length"] if w["max_length"] > 0 else w.get("steps", 0) * w["max_step"]
+    for storage in ("fp32", "fp16"):
+        if storage == "fp16":
+            field.set_storage("fp16")
+        prop = capi.Propagator(w["kind"], field, w["tolerance"], w["min_step"], w["max_step"],
+                               max_length=w["max_length"], precision=capi.PRECISION_MIXED)
+        acc, evals, kms, launches, dt = timed_passes(capi, prop, init, work, w.get("steps", -1), args.steps,
+                                                     min(args.warmup, 2))
+        fin = work.download()
+        e = position_error(fin["pos"], final64["pos"])
+        de = np.max(np.abs(fin["dir"] - final64["dir"]), axis=1)
+        kern_s = kms / 1e3 / launches
+        bpe = BYTES_PER_EVAL_FP16 if storage == "fp16" else BYTES_PER_EVAL
+        achieved = evals * bpe / launches / kern_s / 1e9
+        out[storage] = {
+            "value": acc / dt, "speedup_vs_fp64": (acc / dt) / value64, "ms_per_step": dt / args.steps * 1e3,
+            "kernel_ms_avg": kern_s * 1e3, "accepted_steps_per_pass": acc // args.steps,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "bytes_per_eval": bpe},
+            "final_position_error_vs_fp64": {"median": float(np.median(e)), "p90": float(np.percentile(e, 90)),
+                                             "p99": float(np.percentile(e, 99)), "max": float(e.max())},
+            "final_direction_error_vs_fp64": {"median": float(np.median(de)), "p90": float(np.percentile(de, 90))},
+            "diffusion_vs_fp64": paired_diffusion(fin["pos"], final64["pos"], pos, L) if L > 0 else None}
+    return out
 
 
 def read_traffic(workload, precision):
@@ -346,6 +476,8 @@ def main():
     ap.add_argument("--grid-storage", default="fp32", choices=["fp32", "fp16"],
                     help="grid values read by the mixed path: FP32, or FP16 with a per-8^3-brick FP32 scale")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision block of the default workload")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint block of the default workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it (got --gpus %d)"
@@ -387,27 +519,8 @@ def main():
     work = capi.Batch(n)
     max_steps = w.get("steps", -1)
 
-    for _ in range(args.warmup):
-        work.copy_from(init)
-        prop.propagate(work, max_steps=max_steps)
-    capi.check(capi.lib().crp_device_synchronize())
-    acc = evals = launches = 0
-    kms = 0.0
-    dt = 0.0
-    for _ in range(args.steps):
-        # the batch is reset to its initial state before each step, outside the timed region
-        # (crp_batch_copy synchronises its stream); each step is bracketed by device synchronisation
-        work.copy_from(init)
-        capi.check(capi.lib().crp_device_synchronize())
-        t0 = time.perf_counter()
-        st = prop.propagate(work, max_steps=max_steps)
-        capi.check(capi.lib().crp_device_synchronize())
-        dt += time.perf_counter() - t0
-        acc += st.accepted_steps
-        evals += st.field_evals
-        kms += st.kernel_ms
-        launches += st.launches
-
+    acc, evals, kms, launches, dt = timed_passes(capi, prop, init, work, max_steps, args.steps, args.warmup)
+    final = work.download()
     kern_s = kms / 1e3 / launches
     per_trial = 6 if w["kind"] == "CK" else (1 if w["min_step"] == w["max_step"] else 3)
     config = {"workload": args.workload, "propagator": w["kind"], "precision": args.precision,
@@ -479,8 +592,19 @@ def main():
         "roofline": roofline,
         "cpu_baseline": None,
     }
+    headline = args.workload == "ck_grid256" and args.precision == "fp64" and args.grid_layout == "cells"
+    if headline and not args.no_mixed:
+        # the metric's second half: the mixed path on the same particles, error against this FP64 pass
+        out["mixed"] = mixed_block(capi, w, fdata, field, init, work, pos, final, acc / dt, args)
+    del init, work, prop, field
+    if headline and not args.no_adjoint:
+        # the metric's third half: adjoint / forward time on the C5 workload
+        a = adjoint_measure(args, max(1, min(args.steps, 5)), max(1, min(args.warmup, 2)))
+        out["adjoint"] = {k: a[k] for k in ("workload", "particles", "propagation_steps", "checkpoint_every",
+                                            "forward_ms", "backward_ms", "adjoint_over_forward",
+                                            "gradient_over_forward", "value", "unit")}
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(w, fdata, pos, d, E, q)
+        out["cpu_baseline"] = cpu_baseline(w, fdata, pos, d, E, q, gpu_final=final)
     print(json.dumps(out), flush=True)
 
 

@@ -283,11 +283,13 @@ def normalise(d):
 
 
 class OracleState:
-    def __init__(self, pos, dir, energy, charge, next_step=None):
+    def __init__(self, pos, dir, energy, charge, next_step=None, normalised=False):
+        """dir: raw directions, normalised here as ParticleState::setDirection does, or (normalised=True)
+        directions that already went through setDirection, taken as they are."""
         n = len(energy)
         self.n = n
         self.pos = np.ascontiguousarray(pos, dtype=np.float64).copy()
-        self.dir = normalise(dir)
+        self.dir = np.ascontiguousarray(dir, dtype=np.float64).copy() if normalised else normalise(dir)
         self.energy = np.ascontiguousarray(energy, dtype=np.float64).copy()
         self.charge = np.ascontiguousarray(charge, dtype=np.float64).copy()
         self.redshift = np.zeros(n)

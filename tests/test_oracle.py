@@ -59,16 +59,29 @@ def test_oracle_cr_bit_exact_without_transcendentals(name):
         assert bits_equal(s["dir"], c["dir"][si])
 
 
-def test_cr_vs_glibc_divergence_is_bounded_by_first_rounding_difference():
-    """Adaptive and plane-wave cases: with correctly rounded pow/cos the first
-    snapshot is still bit-exact; later ones only differ for a fraction of particles."""
-    for name in ("bp_adaptive_grid32", "ck_grid32_maxlen", "ck_planewave32", "ck_uniform"):
-        c, snaps, _ = run_case(name, O.LIBM_CR)
-        assert bits_equal(snaps[0]["pos"], c["pos"][0]), name
-        last = snaps[-1]["pos"]
-        rel = np.max(np.abs(last - c["pos"][-1]), axis=1) / np.maximum(np.max(np.abs(c["pos"][-1]), axis=1), 1e-300)
-        # most particles keep 1e-9 agreement; the rest are chaotic divergence after a 1-ulp step change
-        assert np.mean(rel <= 1e-9) >= 0.35, (name, np.mean(rel <= 1e-9))
+# particles within 1e-9 relative of the reference at each snapshot of the adaptive and plane-wave cases
+# when the oracle uses correctly rounded pow/cos instead of glibc's (deterministic: both sides are fixed)
+CR_VS_GLIBC_WITHIN_1E9 = {"bp_adaptive_grid32": [256, 256, 256, 240, 101],      # of 256; steps 1,2,10,100,1000
+                          "ck_grid32_maxlen": [256, 256, 256, 256, 246, 246],   # of 256; 1,2,10,100,1000,4000
+                          "ck_planewave32": [128, 128, 128, 105],               # of 128; 1,10,100,300
+                          "ck_uniform": [64, 64, 64]}                           # of 64; 1,10,100
+
+
+@pytest.mark.parametrize("name", sorted(CR_VS_GLIBC_WITHIN_1E9))
+def test_cr_vs_glibc_divergence_is_bounded_by_first_rounding_difference(name):
+    """Adaptive and plane-wave cases: with correctly rounded pow/cos the early snapshots are still
+    within 1e-9 of the reference for every particle; later ones for the measured number of particles
+    (the rest diverge chaotically after a 1-ulp step-size or field difference)."""
+    c, snaps, _ = run_case(name, O.LIBM_CR)
+    assert bits_equal(snaps[0]["pos"], c["pos"][0]), name
+    counts = []
+    for si, s in enumerate(snaps):
+        ref = c["pos"][si]
+        rel = np.max(np.abs(s["pos"] - ref), axis=1) / np.maximum(np.max(np.abs(ref), axis=1), 1e-300)
+        counts.append(int(np.sum(rel <= 1e-9)))
+    print(name, counts)
+    assert len(counts) == len(CR_VS_GLIBC_WITHIN_1E9[name])
+    assert all(a >= b for a, b in zip(counts, CR_VS_GLIBC_WITHIN_1E9[name])), (name, counts)
 
 
 @pytest.mark.parametrize("mode,reflective,clip", [("B_periodic", 0, 0), ("B_reflective", 1, 0), ("B_clip", 0, 1)])

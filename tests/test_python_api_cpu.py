@@ -229,3 +229,19 @@ def test_reject_flag_value_is_the_references():
     c.setTrajectoryLength(2.0)
     MaximumTrajectoryLength(1.0).process(c)
     assert c.getProperty("Rejected") == "N7crpropa17AbstractConditionE"
+
+
+def test_bench_particles_are_the_reference_draws():
+    """bench.py draws its particles with the host Random in the golden fixtures' order (randBin,
+    randPowerLaw(-1), randVector per particle), which the oracle's sampler reproduces bit for bit."""
+    import bench
+    import oracle_ctypes as O
+    w = bench.WORKLOADS["ck_grid256"]
+    sp, E, d = bench.draw_particles(w["seed"], 4096, w["emin"], w["emax"], w["weights"])
+    osp, oE, od = O.sample_particles(w["seed"], 4096, w["emin"], w["emax"], weights=w["weights"])
+    assert np.array_equal(sp, osp)
+    assert np.array_equal(E.view(np.uint64), oE.view(np.uint64))
+    assert np.array_equal(d.view(np.uint64), od.view(np.uint64))
+    pos, dn, E2, q = bench.make_particles(w, 4096)
+    assert np.array_equal(dn.view(np.uint64), O.normalise(od).view(np.uint64))
+    assert set(np.unique(np.round(q / O.eplus))) == {1.0, 2.0, 7.0, 14.0, 26.0}
