@@ -94,6 +94,7 @@ def lib():
         L.crp_field_set_grid_storage.argtypes = [_vp, C.c_int]
         L.crp_field_eval_ex.argtypes = [_vp, _dp, C.c_size_t, _dp, C.c_int]
         L.crp_field_eval.argtypes = [_vp, _dp, C.c_size_t, _dp]
+        L.crp_field_eval_noalloc.argtypes = [_vp, _dp, C.c_size_t, _dp, C.c_int]
         for nm in ("crp_propagator_create_bp", "crp_propagator_create_ck"):
             getattr(L, nm).argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.POINTER(_vp)]
         L.crp_propagator_create_sde.argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(_vp)]

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -823,23 +824,36 @@ int crp_field_set_grid_storage(crp_field_t *f, int storage) {
 	return CRP_OK;
 }
 
-int crp_field_eval_ex(const crp_field_t *f, const double *pos, size_t n, double *B, int precision) {
-	if (precision == CRP_PRECISION_FP64)
-		return crp_field_eval(f, pos, n, B);
-	if (precision != CRP_PRECISION_MIXED)
-		return fail(CRP_E_INVALID, "crp_field_eval_ex: unknown precision");
-	if (!pos || !B)
-		return fail(CRP_E_INVALID, "crp_field_eval_ex: NULL argument");
-	if (n == 0)
-		return CRP_OK;
-	double *dpos = nullptr, *dB = nullptr;
-	CRP_HIP_TRY(hipMalloc(&dpos, 3 * n * sizeof(double)));
-	hipError_t e = hipMalloc(&dB, 3 * n * sizeof(double));
-	if (e == hipSuccess)
-		e = hipMemcpy(dpos, pos, 3 * n * sizeof(double), hipMemcpyHostToDevice);
+// Launches the field evaluation of n device positions dpos into dB (both AoS xyz) on the null stream:
+// the FP64 evaluation (k_field_eval) or the mixed path's FP32 evaluation (k_field_evalf, FP16 storage
+// if set).  Shared by crp_field_eval, crp_field_eval_ex and crp_field_eval_noalloc.
+static hipError_t launch_field_eval(const crp_field_t *f, const double *dpos, double *dB, size_t n, int precision) {
 	const unsigned blocks = (unsigned)((n + 255) / 256);
-	if (e == hipSuccess) {
-		switch (f ? f->kind : FK_NONE) {
+	const int kind = f ? f->kind : FK_NONE;
+	if (precision == CRP_PRECISION_FP64) {
+		switch (kind) {
+		case FK_UNIFORM: hipLaunchKernelGGL(k_field_eval<FieldUniform>, blocks, 256, 0, 0, f->uniform, dpos, dB, n); break;
+		case FK_GRID:
+			if (f->cells)
+				hipLaunchKernelGGL(k_field_eval<FieldGridCells>, blocks, 256, 0, 0, FieldGridCells{f->grid, f->cells},
+				                   dpos, dB, n);
+			else
+				hipLaunchKernelGGL(k_field_eval<FieldGrid>, blocks, 256, 0, 0, FieldGrid{f->grid}, dpos, dB, n);
+			break;
+		case FK_PLANEWAVE: hipLaunchKernelGGL(k_field_eval<FieldPlaneWave>, blocks, 256, 0, 0, FieldPlaneWave{f->pw}, dpos, dB, n); break;
+		case FK_MODULATED:
+			if (f->cells)
+				hipLaunchKernelGGL(k_field_eval<FieldModulated<FieldGridCells>>, blocks, 256, 0, 0,
+				                   FieldModulated<FieldGridCells>{FieldGridCells{f->grid, f->cells}, f->mod}, dpos, dB, n);
+			else
+				hipLaunchKernelGGL(k_field_eval<FieldModulated<FieldGrid>>, blocks, 256, 0, 0,
+				                   FieldModulated<FieldGrid>{FieldGrid{f->grid}, f->mod}, dpos, dB, n);
+			break;
+		case FK_JF12: hipLaunchKernelGGL(k_field_eval<FieldJF12>, blocks, 256, 0, 0, f->jf12, dpos, dB, n); break;
+		default: hipLaunchKernelGGL(k_field_eval<FieldNone>, blocks, 256, 0, 0, FieldNone{}, dpos, dB, n); break;
+		}
+	} else {
+		switch (kind) {
 		case FK_UNIFORM: hipLaunchKernelGGL(k_field_evalf<FieldUniform>, blocks, 256, 0, 0, f->uniform, dpos, dB, n); break;
 		case FK_GRID:
 			if (f->cells16)
@@ -863,8 +877,26 @@ int crp_field_eval_ex(const crp_field_t *f, const double *pos, size_t n, double 
 		case FK_JF12: hipLaunchKernelGGL(k_field_evalf<FieldJF12>, blocks, 256, 0, 0, f->jf12, dpos, dB, n); break;
 		default: hipLaunchKernelGGL(k_field_evalf<FieldNone>, blocks, 256, 0, 0, FieldNone{}, dpos, dB, n); break;
 		}
-		e = hipGetLastError();
 	}
+	return hipGetLastError();
+}
+
+int crp_field_eval_ex(const crp_field_t *f, const double *pos, size_t n, double *B, int precision) {
+	if (precision == CRP_PRECISION_FP64)
+		return crp_field_eval(f, pos, n, B);
+	if (precision != CRP_PRECISION_MIXED)
+		return fail(CRP_E_INVALID, "crp_field_eval_ex: unknown precision");
+	if (!pos || !B)
+		return fail(CRP_E_INVALID, "crp_field_eval_ex: NULL argument");
+	if (n == 0)
+		return CRP_OK;
+	double *dpos = nullptr, *dB = nullptr;
+	CRP_HIP_TRY(hipMalloc(&dpos, 3 * n * sizeof(double)));
+	hipError_t e = hipMalloc(&dB, 3 * n * sizeof(double));
+	if (e == hipSuccess)
+		e = hipMemcpy(dpos, pos, 3 * n * sizeof(double), hipMemcpyHostToDevice);
+	if (e == hipSuccess)
+		e = launch_field_eval(f, dpos, dB, n, CRP_PRECISION_MIXED);
 	if (e == hipSuccess)
 		e = hipMemcpy(B, dB, 3 * n * sizeof(double), hipMemcpyDeviceToHost);
 	if (dpos)
@@ -873,6 +905,65 @@ int crp_field_eval_ex(const crp_field_t *f, const double *pos, size_t n, double 
 		(void)hipFree(dB);
 	if (e != hipSuccess)
 		return fail(CRP_E_HIP, std::string("crp_field_eval_ex: ") + hipGetErrorString(e));
+	return CRP_OK;
+}
+
+// ---------------------------------------------------------------- field evaluation without allocation
+// Staging of crp_field_eval_noalloc: CRP_EVAL_NOALLOC_MAX positions followed by as many field values
+// (AoS xyz doubles), 6 MiB per device for the life of the process.  Host code never uses this name as
+// an address: the device address comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) double scratch[2 * 3 * CRP_EVAL_NOALLOC_MAX];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kMaxDevices] = {};
+} // namespace
+
+// The device address of `scratch` on the current device (looked up on the first call per device).
+static int noalloc_scratch(double **out) {
+	int dev = 0;
+	hipError_t e = hipGetDevice(&dev);
+	if (e != hipSuccess)
+		return fail(CRP_E_HIP, std::string("crp_field_eval_noalloc: hipGetDevice: ") + hipGetErrorString(e));
+	if (dev < 0 || dev >= kMaxDevices)
+		return fail(CRP_E_INVALID, "crp_field_eval_noalloc: device index beyond the supported 64");
+	std::lock_guard<std::mutex> lock(g_scratch_mutex);
+	if (!g_scratch_ptr[dev]) {
+		void *p = nullptr;
+		e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !p)
+			return fail(CRP_E_HIP, std::string("crp_field_eval_noalloc: hipGetSymbolAddress: ") +
+			                           (e != hipSuccess ? hipGetErrorString(e) : "null address"));
+		g_scratch_ptr[dev] = (double *)p;
+	}
+	*out = g_scratch_ptr[dev];
+	return CRP_OK;
+}
+
+int crp_field_eval_noalloc(const crp_field_t *f, const double *pos, size_t n, double *B, int precision) {
+	if (precision != CRP_PRECISION_FP64 && precision != CRP_PRECISION_MIXED)
+		return fail(CRP_E_INVALID, "crp_field_eval_noalloc: unknown precision");
+	if (!pos || !B)
+		return fail(CRP_E_INVALID, "crp_field_eval_noalloc: NULL argument");
+	if (n > CRP_EVAL_NOALLOC_MAX)
+		return fail(CRP_E_INVALID, "crp_field_eval_noalloc: " + std::to_string(n) +
+		                               " positions exceed the staging capacity of " +
+		                               std::to_string((unsigned long long)CRP_EVAL_NOALLOC_MAX) +
+		                               " per call (split the call, or use crp_field_eval)");
+	if (n == 0)
+		return CRP_OK;
+	double *st = nullptr;
+	if (int rc = noalloc_scratch(&st))
+		return rc;
+	double *dpos = st, *dB = st + 3 * (size_t)CRP_EVAL_NOALLOC_MAX;
+	hipError_t e = hipMemcpy(dpos, pos, 3 * n * sizeof(double), hipMemcpyHostToDevice);
+	if (e == hipSuccess)
+		e = launch_field_eval(f, dpos, dB, n, precision);
+	if (e == hipSuccess)
+		e = hipMemcpy(B, dB, 3 * n * sizeof(double), hipMemcpyDeviceToHost);
+	if (e != hipSuccess)
+		return fail(CRP_E_HIP, std::string("crp_field_eval_noalloc: ") + hipGetErrorString(e));
 	return CRP_OK;
 }
 
@@ -965,32 +1056,8 @@ int crp_field_eval(const crp_field_t *f, const double *pos, size_t n, double *B)
 		return fail(CRP_E_NOMEM, "crp_field_eval: hipMalloc failed");
 	}
 	e = hipMemcpy(dpos, pos, 3 * n * sizeof(double), hipMemcpyHostToDevice);
-	const unsigned blocks = (unsigned)((n + 255) / 256);
-	if (e == hipSuccess) {
-		const int kind = f ? f->kind : FK_NONE;
-		switch (kind) {
-		case FK_UNIFORM: hipLaunchKernelGGL(k_field_eval<FieldUniform>, blocks, 256, 0, 0, f->uniform, dpos, dB, n); break;
-		case FK_GRID:
-			if (f->cells)
-				hipLaunchKernelGGL(k_field_eval<FieldGridCells>, blocks, 256, 0, 0, FieldGridCells{f->grid, f->cells},
-				                   dpos, dB, n);
-			else
-				hipLaunchKernelGGL(k_field_eval<FieldGrid>, blocks, 256, 0, 0, FieldGrid{f->grid}, dpos, dB, n);
-			break;
-		case FK_PLANEWAVE: hipLaunchKernelGGL(k_field_eval<FieldPlaneWave>, blocks, 256, 0, 0, FieldPlaneWave{f->pw}, dpos, dB, n); break;
-		case FK_MODULATED:
-			if (f->cells)
-				hipLaunchKernelGGL(k_field_eval<FieldModulated<FieldGridCells>>, blocks, 256, 0, 0,
-				                   FieldModulated<FieldGridCells>{FieldGridCells{f->grid, f->cells}, f->mod}, dpos, dB, n);
-			else
-				hipLaunchKernelGGL(k_field_eval<FieldModulated<FieldGrid>>, blocks, 256, 0, 0,
-				                   FieldModulated<FieldGrid>{FieldGrid{f->grid}, f->mod}, dpos, dB, n);
-			break;
-		case FK_JF12: hipLaunchKernelGGL(k_field_eval<FieldJF12>, blocks, 256, 0, 0, f->jf12, dpos, dB, n); break;
-		default: hipLaunchKernelGGL(k_field_eval<FieldNone>, blocks, 256, 0, 0, FieldNone{}, dpos, dB, n); break;
-		}
-		e = hipGetLastError();
-	}
+	if (e == hipSuccess)
+		e = launch_field_eval(f, dpos, dB, n, CRP_PRECISION_FP64);
 	if (e == hipSuccess)
 		e = hipMemcpy(B, dB, 3 * n * sizeof(double), hipMemcpyDeviceToHost);
 	(void)hipFree(dpos);
@@ -1778,7 +1845,7 @@ int crp_adjoint_forward(const crp_adjoint_config *cfg, size_t n, const double *x
 }
 
 int crp_adjoint_backward(const crp_adjoint_config *cfg, size_t n, const double *kq, const double *checkpoints,
-                         const double *grad_x_out, const double *grad_u_out, double *scratch, double *grad_grid,
+                         const double *grad_x_out, const double *grad_u_out, double *segment_scratch, double *grad_grid,
                          double *grad_x0, double *grad_u0, double *grad_scale, void *stream) {
 	AdjArgs A;
 	int rc = adj_args(cfg, n, A);
@@ -1786,13 +1853,13 @@ int crp_adjoint_backward(const crp_adjoint_config *cfg, size_t n, const double *
 		return rc;
 	if (n == 0)
 		return CRP_OK;
-	if (!kq || !checkpoints || !grad_u_out || !scratch || !grad_grid || !grad_x0 || !grad_u0 || !grad_scale)
+	if (!kq || !checkpoints || !grad_u_out || !segment_scratch || !grad_grid || !grad_x0 || !grad_u0 || !grad_scale)
 		return fail(CRP_E_INVALID, "crp_adjoint_backward: NULL pointer");
 	A.kq = kq;
 	A.ckpt = const_cast<double *>(checkpoints);
 	A.gxout = grad_x_out;
 	A.guout = grad_u_out;
-	A.scratch = scratch;
+	A.scratch = segment_scratch;
 	A.gradG = grad_grid;
 	A.gradX0 = grad_x0;
 	A.gradU0 = grad_u0;

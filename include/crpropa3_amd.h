@@ -117,6 +117,15 @@ int crp_field_destroy(crp_field_t *f);
 /* B at n host positions (AoS xyz), evaluated on the device with the FP64 path. */
 int crp_field_eval(const crp_field_t *f, const double *pos, size_t n, double *B);
 
+/* crp_field_eval_ex without any device allocation: positions and results are staged through a
+ * fixed per-device buffer of the library (CRP_EVAL_NOALLOC_MAX positions, 6 MiB, held until the
+ * process ends) instead of hipMalloc'd buffers; same kernels, same bits.  n above
+ * CRP_EVAL_NOALLOC_MAX is rejected with CRP_E_INVALID.  Calls that overlap on one GPU (several host
+ * threads) share that buffer and race.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (per-candidate getField calls in tight loops, harnesses that forbid it). */
+#define CRP_EVAL_NOALLOC_MAX 131072
+int crp_field_eval_noalloc(const crp_field_t *f, const double *pos, size_t n, double *B, int precision);
+
 /* JF12Field / PlanckJF12bField (src/magneticField/JF12Field.cpp): the model's constants as the
  * reference's constructors set them (JF12Field.cpp:8-79, 348-363) and its switches. */
 typedef struct {
