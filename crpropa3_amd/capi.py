@@ -36,7 +36,8 @@ class HostState(C.Structure):
     _fields_ = [("pos", _dp), ("dir", _dp), ("energy", _dp), ("charge", _dp), ("current_step", _dp),
                 ("next_step", _dp), ("trajectory_length", _dp), ("time", _dp), ("active", _ip),
                 ("prev_pos", _dp), ("prev_dir", _dp), ("detection_index", _ip), ("source_pos", _dp),
-                ("created_pos", _dp), ("sde_step", C.POINTER(C.c_uint32)), ("sde_stream", C.POINTER(C.c_uint64))]
+                ("created_pos", _dp), ("sde_step", C.POINTER(C.c_uint32)), ("sde_stream", C.POINTER(C.c_uint64)),
+                ("redshift", _dp)]
 
 
 class JF12Params(C.Structure):
@@ -95,6 +96,11 @@ def lib():
         L.crp_field_eval_ex.argtypes = [_vp, _dp, C.c_size_t, _dp, C.c_int]
         L.crp_field_eval.argtypes = [_vp, _dp, C.c_size_t, _dp]
         L.crp_field_eval_noalloc.argtypes = [_vp, _dp, C.c_size_t, _dp, C.c_int]
+        L.crp_field_create_periodic.argtypes = [_vp, _dp, _dp, C.c_int, C.POINTER(_vp)]
+        L.crp_field_create_list.argtypes = [C.POINTER(_vp)]
+        L.crp_field_list_add.argtypes = [_vp, _vp]
+        L.crp_field_create_evolution.argtypes = [_vp, C.c_double, C.POINTER(_vp)]
+        L.crp_field_eval_z.argtypes = [_vp, _dp, _dp, C.c_size_t, _dp]
         for nm in ("crp_propagator_create_bp", "crp_propagator_create_ck"):
             getattr(L, nm).argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.POINTER(_vp)]
         L.crp_propagator_create_sde.argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(_vp)]
@@ -246,6 +252,39 @@ class Field:
         h = _vp()
         check(lib().crp_field_create_planewave(len(k), _p(k), _p(Ak), _p(xi), _p(kappa), _p(beta), C.byref(h)))
         return Field(h)
+
+    @staticmethod
+    def periodic(field, extends, origin=(0, 0, 0), reflective=False):
+        """PeriodicMagneticField(field, extends, origin, reflective); keeps `field` alive."""
+        e, o = _f64(extends, (3,)), _f64(origin, (3,))
+        h = _vp()
+        check(lib().crp_field_create_periodic(field.handle, _p(e), _p(o), int(reflective), C.byref(h)))
+        return Field(h, keep=(field,))
+
+    @staticmethod
+    def field_list(fields):
+        """MagneticFieldList with addField(f) for each f in order; keeps the fields alive."""
+        h = _vp()
+        check(lib().crp_field_create_list(C.byref(h)))
+        f = Field(h, keep=tuple(fields))
+        for c in fields:
+            check(lib().crp_field_list_add(h, c.handle))
+        return f
+
+    @staticmethod
+    def evolution(field, m):
+        """MagneticFieldEvolution(field, m); keeps `field` alive."""
+        h = _vp()
+        check(lib().crp_field_create_evolution(field.handle, float(m), C.byref(h)))
+        return Field(h, keep=(field,))
+
+    def eval_z(self, pos, z):
+        """getField(position, z) with a redshift per position (FP64)."""
+        pos = _f64(pos, (-1, 3))
+        zz = _f64(np.broadcast_to(np.asarray(z, dtype=np.float64), (pos.shape[0],)))
+        B = np.zeros_like(pos)
+        check(lib().crp_field_eval_z(self.handle, _p(pos), _p(zz), pos.shape[0], _p(B)))
+        return B
 
     LAYOUTS = {"points": 0, "cells": 1}
 
@@ -446,7 +485,7 @@ class Batch:
 
     def upload(self, pos, dir, energy, charge, current_step=None, next_step=None, trajectory_length=None,
                time=None, active=None, detection_index=None, source_pos=None, created_pos=None, sde_step=None,
-               sde_stream=None):
+               sde_stream=None, redshift=None):
         pos = _f64(pos, (-1, 3))
         n = pos.shape[0]
         arrs = {
@@ -476,6 +515,9 @@ class Batch:
         if sde_stream is not None:
             extra["sde_stream"] = np.ascontiguousarray(sde_stream, dtype=np.uint64)
             s.sde_stream = extra["sde_stream"].ctypes.data_as(C.POINTER(C.c_uint64))
+        if redshift is not None:
+            extra["redshift"] = _f64(np.broadcast_to(np.asarray(redshift, dtype=np.float64), (n,)))
+            s.redshift = _p(extra["redshift"])
         for a in extra.values():
             assert a.shape[0] == n
         check(lib().crp_batch_upload(self.handle, n, C.byref(s)))

@@ -448,25 +448,22 @@ CRP_MATH_FN int crp_cos_quick_shift(double x, int shift, double *out) {
 	} else {
 		double Sh = crp_sincos_tab[4 * j], Sl = crp_sincos_tab[4 * j + 1];
 		double Ch = crp_sincos_tab[4 * j + 2], Cl = crp_sincos_tab[4 * j + 3];
-		if (n & 1) { /* sin a = S cos d + C sin d */
-			crp_dd P = crp_two_prod(Ch, dh);
-			crp_dd Q = crp_two_prod(Sh, hz);
-			crp_dd s1 = crp_two_sum(Sh, P.hi);
-			crp_dd s2 = crp_two_sum(s1.hi, -Q.hi);
-			double small = Ch * (sdm + dsin) + Sh * (cz - dcross);
-			small += ((Sl + Cl * dh) - Sl * hz) - Sh * hzl;
-			small += (P.lo - Q.lo) + (s1.lo + s2.lo);
-			y = crp_fast_two_sum(s2.hi, small);
-		} else { /* cos a = C cos d - S sin d */
-			crp_dd P = crp_two_prod(Sh, dh);
-			crp_dd Q = crp_two_prod(Ch, hz);
-			crp_dd s1 = crp_two_sum(Ch, -P.hi);
-			crp_dd s2 = crp_two_sum(s1.hi, -Q.hi);
-			double small = Ch * (cz - dcross) - Sh * (sdm + dsin);
-			small += ((Cl - Sl * dh) - Cl * hz) - Ch * hzl;
-			small += (s1.lo + s2.lo) - (P.lo + Q.lo);
-			y = crp_fast_two_sum(s2.hi, small);
-		}
+		/* sin a = S cos d + C sin d and cos a = C cos d - S sin d are one computation with
+		 * (A, B, sg) = (S, C, +1) or (C, S, -1): A - A z/2 + sg B d_h, the small terms likewise.  Every
+		 * operation is the same IEEE operation as in the two separate forms (sg only negates, which is
+		 * exact, and the sums are the same operand pairs), so the result is identical, and the lanes
+		 * of a wave do not split between two paths. */
+		const int sn = n & 1;
+		const double Ah = sn ? Sh : Ch, Al = sn ? Sl : Cl, Bh = sn ? Ch : Sh, Bl = sn ? Cl : Sl;
+		const double sg = sn ? 1.0 : -1.0;
+		crp_dd P = crp_two_prod(Bh, dh);
+		crp_dd Q = crp_two_prod(Ah, hz);
+		crp_dd s1 = crp_two_sum(Ah, sg * P.hi);
+		crp_dd s2 = crp_two_sum(s1.hi, -Q.hi);
+		double small = Ah * (cz - dcross) + sg * (Bh * (sdm + dsin));
+		small += ((Al + sg * (Bl * dh)) - Al * hz) - Ah * hzl;
+		small += (s1.lo + s2.lo) + (sg * P.lo - Q.lo);
+		y = crp_fast_two_sum(s2.hi, small);
 	}
 	/* rounding test; y.hi > 0 here */
 	uint64_t ub = crp_as_u64(y.hi);

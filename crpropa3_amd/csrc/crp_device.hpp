@@ -81,16 +81,24 @@ struct GridDesc {
 	int fastDiv;               // spacing within [2^-500, 2^500]: div_cr may use its reciprocal path
 };
 
+// Plane-wave mode tables, one record per mode, read through the constant address space so that
+// the compiler loads them with scalar (SMEM) loads into SGPRs: every lane of a wave evaluates the
+// same mode at the same time, so one scalar load serves the wave and the VALU instructions take the
+// mode's values as scalar operands.  The tables are immutable while kernels run.
+struct PwMode64 {    // FP64 path: k, beta, kappa, xi*A (xi*A is the first product of getField)
+	double k, beta, kx, ky, kz, ax, ay, az;
+};
+struct PwModeMixed { // mixed path: phase in turns = p . kt + bt (FP64), amplitude xi*A in FP32
+	double ktx, kty, ktz, bt;
+	float ax, ay, az, pad;
+};
+typedef __attribute__((address_space(4))) const PwMode64 PwMode64Const;
+typedef __attribute__((address_space(4))) const PwModeMixed PwModeMixedConst;
+
 struct PlaneWaveDesc {
 	int nm;
-	const double *k;     // [nm]
-	const double *beta;  // [nm]
-	const double *kappa; // [3*nm]
-	const double *xiA;   // [3*nm]: xi[i] * Ak[i], per component (the first product of getField)
-	// mixed-precision path: phase in turns = p . kt + bt, amplitude vectors in FP32
-	const double *kt;  // [3*nm]: k[i] * kappa[i] / (2 pi)
-	const double *bt;  // [nm]: beta[i] / (2 pi)
-	const float *xiAf; // [3*nm]
+	const PwMode64 *m64;   // [nm]
+	const PwModeMixed *mm; // [nm]
 };
 
 struct FieldUniform {
@@ -498,31 +506,37 @@ template <class G> struct FieldModulated {
 struct FieldPlaneWave {
 	PlaneWaveDesc w;
 
+	// B = sum_i (xi_i A_i) cos(k_i (x . kappa_i) + beta_i), accumulated in mode order
+	// (PlaneWaveTurbulence.cpp:231-240), with the correctly rounded cos
 	__device__ __forceinline__ V3 eval(const V3 &p) const {
 		double bx = 0.0, by = 0.0, bz = 0.0;
-		for (int i = 0; i < w.nm; i++) {
-			const double kx = w.kappa[3 * i], ky = w.kappa[3 * i + 1], kz = w.kappa[3 * i + 2];
+		PwMode64Const *m = (PwMode64Const *)w.m64;
+		const int nm = w.nm;
+#pragma unroll 2
+		for (int i = 0; i < nm; i++) {
+			const double kx = m[i].kx, ky = m[i].ky, kz = m[i].kz;
 			double z_ = (p.x * kx + p.y * ky) + p.z * kz;
-			double c = crp_cos_cr(w.k[i] * z_ + w.beta[i]);
-			bx += w.xiA[3 * i] * c;
-			by += w.xiA[3 * i + 1] * c;
-			bz += w.xiA[3 * i + 2] * c;
+			double c = crp_cos_cr(m[i].k * z_ + m[i].beta);
+			bx += m[i].ax * c;
+			by += m[i].ay * c;
+			bz += m[i].az * c;
 		}
 		return v3(bx, by, bz);
 	}
 
-	// mixed precision: phase in turns in FP64 (positions span many wavelengths),
-	// reduced to [-1/2, 1/2] and evaluated with the hardware cosine
-	// (v_cos_f32 computes cos(2 pi x)); amplitudes and the sum in FP32.
+	// mixed precision: phase in turns in FP64 (positions span many wavelengths), its fractional part
+	// evaluated with the hardware cosine (v_cos_f32 computes cos(2 pi x)); amplitudes and the sum in FP32
 	__device__ __forceinline__ float3 evalf(const V3 &p) const {
 		float bx = 0.f, by = 0.f, bz = 0.f;
-		for (int i = 0; i < w.nm; i++) {
-			double t = __builtin_fma(p.x, w.kt[3 * i], __builtin_fma(p.y, w.kt[3 * i + 1], __builtin_fma(p.z, w.kt[3 * i + 2], w.bt[i])));
-			const float fr = (float)(t - __builtin_rint(t));
-			const float c = __builtin_amdgcn_cosf(fr);
-			bx = __builtin_fmaf(w.xiAf[3 * i], c, bx);
-			by = __builtin_fmaf(w.xiAf[3 * i + 1], c, by);
-			bz = __builtin_fmaf(w.xiAf[3 * i + 2], c, bz);
+		PwModeMixedConst *m = (PwModeMixedConst *)w.mm;
+		const int nm = w.nm;
+#pragma unroll 4
+		for (int i = 0; i < nm; i++) {
+			const double t = __builtin_fma(p.x, m[i].ktx, __builtin_fma(p.y, m[i].kty, __builtin_fma(p.z, m[i].ktz, m[i].bt)));
+			const float c = __builtin_amdgcn_cosf((float)__builtin_amdgcn_fract(t));
+			bx = __builtin_fmaf(m[i].ax, c, bx);
+			by = __builtin_fmaf(m[i].ay, c, by);
+			bz = __builtin_fmaf(m[i].az, c, bz);
 		}
 		return make_float3(bx, by, bz);
 	}

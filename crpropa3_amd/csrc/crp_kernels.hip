@@ -26,6 +26,7 @@
 
 #include "crp_device.hpp"
 #include "crp_propagate.hpp"
+#include "crp_composite.hpp"
 #include "crp_sde.hpp"
 #include "crp_jf12.hpp"
 #include "crp_adjoint.hpp"
@@ -54,7 +55,10 @@ static int fail(int code, const std::string &msg) {
 	} while (0)
 
 // ------------------------------------------------------------------ objects
-enum FieldKind { FK_NONE = 0, FK_UNIFORM = 1, FK_GRID = 2, FK_PLANEWAVE = 3, FK_MODULATED = 4, FK_JF12 = 5 };
+enum FieldKind { FK_NONE = 0, FK_UNIFORM = 1, FK_GRID = 2, FK_PLANEWAVE = 3, FK_MODULATED = 4, FK_JF12 = 5, FK_COMPOSITE = 6 };
+// FK_COMPOSITE decorators (crp_field.deco)
+enum DecoKind { DECO_PERIODIC = 1, DECO_LIST = 2, DECO_EVOLUTION = 3 };
+constexpr int kDecoMaxChildren = 8;
 
 struct crp_field {
 	int kind;
@@ -67,6 +71,16 @@ struct crp_field {
 	float *cells; // cell-major corner table (periodic grids), or nullptr
 	unsigned char *cells16; // FP16 corner table with brick scales (mixed path), or nullptr
 	void *owned[8];
+	// FK_COMPOSITE: a decorator over borrowed fields (the caller keeps them alive) and the device
+	// program compiled from the tree (rebuilt before each use: a child's layout may have changed)
+	int deco;
+	const crp_field *child[kDecoMaxChildren];
+	int nchild;
+	double extends[3], porigin[3];
+	int preflective;
+	double evoM;
+	CompositeDesc *dcomp; // device copy of the program
+	CompositeDesc *hcomp; // host copy of what dcomp holds
 };
 
 struct crp_propagator {
@@ -107,6 +121,7 @@ struct crp_batch {
 	double *px, *py, *pz, *dx, *dy, *dz, *E, *q, *cur, *next, *len, *time;
 	double *ppx, *ppy, *ppz, *pdx, *pdy, *pdz;
 	double *sx, *sy, *sz, *cx, *cy, *cz; // source and created positions
+	double *z;                           // redshift (read by a top-level MagneticFieldEvolution)
 	int32_t *active;
 	uint32_t *order; // queue order: particle indices, longest expected work first
 	unsigned long long *ctr;
@@ -140,6 +155,7 @@ static Soa soa_of(const crp_batch *b) {
 	s.cap = b->cap;
 	s.sdeStep = b->sdeStep;
 	s.sdeStream = b->sdeStream;
+	s.z = b->z;
 	return s;
 }
 
@@ -278,6 +294,21 @@ __global__ void k_field_eval(F field, const double *__restrict__ pos, double *__
 		return;
 	V3 p = v3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
 	V3 b = field.eval(p);
+	B[3 * i] = b.x;
+	B[3 * i + 1] = b.y;
+	B[3 * i + 2] = b.z;
+}
+
+// getField(position, z) per position: the field bound to each position's redshift
+template <class F>
+__global__ void k_field_eval_z(F field0, const double *__restrict__ pos, const double *__restrict__ z,
+                               double *__restrict__ B, size_t n) {
+	size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	F field = field0;
+	bind_redshift(field, z[i]);
+	V3 b = field.eval(v3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]));
 	B[3 * i] = b.x;
 	B[3 * i + 1] = b.y;
 	B[3 * i + 2] = b.z;
@@ -824,12 +855,124 @@ int crp_field_set_grid_storage(crp_field_t *f, int storage) {
 	return CRP_OK;
 }
 
+// ---------------------------------------------------------------- field decorators (host side)
+// Compiles a tree of PeriodicMagneticField / MagneticFieldList / MagneticFieldEvolution over
+// device fields into the program FieldComposite runs (crp_composite.hpp).
+struct CompBuilder {
+	CompositeDesc D;
+	int nl = 0, np = 0;
+	std::string err;
+
+	bool emit(int op, int a, int b, int c) {
+		if (D.nops >= kCompMaxOps) {
+			err = "more than " + std::to_string(kCompMaxOps) + " operations";
+			return false;
+		}
+		D.ops[D.nops++] = CompOp{op, a, b, c};
+		return true;
+	}
+	int leaf(const crp_field *f) {
+		if (nl >= kCompMaxLeaves) {
+			err = "more than " + std::to_string(kCompMaxLeaves) + " fields below the decorators";
+			return -1;
+		}
+		CompLeaf &L = D.leaves[nl];
+		std::memset(&L, 0, sizeof(L));
+		switch (f ? f->kind : FK_NONE) {
+		case FK_UNIFORM: L.kind = CL_UNIFORM; L.uniform = f->uniform; break;
+		case FK_GRID: L.kind = CL_GRID; L.grid = f->grid; L.cells = f->cells; L.cells16 = f->cells16; break;
+		case FK_MODULATED: L.kind = CL_MODULATED; L.grid = f->grid; L.cells = f->cells; L.mod = f->mod; break;
+		case FK_PLANEWAVE: L.kind = CL_PLANEWAVE; L.pw = f->pw; break;
+		case FK_JF12: L.kind = CL_JF12; L.jf12 = f->jf12; break;
+		default: L.kind = CL_NONE; break;
+		}
+		return nl++;
+	}
+	// code for getField(pos[ps]) of f (the one-argument call), stored into acc[as] or added to it
+	bool gen(const crp_field *f, int ps, int as, bool add) {
+		if (!f || f->kind != FK_COMPOSITE) {
+			const int l = leaf(f);
+			return l >= 0 && emit(add ? CO_LEAF_ADD : CO_LEAF, l, ps, as);
+		}
+		switch (f->deco) {
+		case DECO_PERIODIC: {
+			if (ps + 1 >= kCompSlots || np >= kCompMaxPeriodic) {
+				err = "periodic fields nested more than " + std::to_string(kCompSlots - 1) + " deep";
+				return false;
+			}
+			CompPeriodic &c = D.per[np];
+			for (int a = 0; a < 3; a++) {
+				c.origin[a] = f->porigin[a];
+				c.extends[a] = f->extends[a];
+			}
+			c.reflective = f->preflective;
+			return emit(CO_PERIODIC, np++, ps, ps + 1) && gen(f->child[0], ps + 1, as, add);
+		}
+		case DECO_LIST: {
+			// b = (0,0,0); b += field_i->getField(position): a list inside a sum is summed on its own
+			const int t = add ? as + 1 : as;
+			if (t >= kCompSlots) {
+				err = "field lists nested more than " + std::to_string(kCompSlots - 1) + " deep";
+				return false;
+			}
+			if (!emit(CO_ZERO, t, 0, 0))
+				return false;
+			for (int i = 0; i < f->nchild; i++)
+				if (!gen(f->child[i], ps, t, true))
+					return false;
+			return add ? emit(CO_ADD, t, as, 0) : true;
+		}
+		default:
+			// a MagneticFieldEvolution below the top: its one-argument getField is the base class's
+			// zero; adding +0 to a sum that starts at +0 changes nothing
+			return add ? true : emit(CO_ZERO, as, 0, 0);
+		}
+	}
+	bool build(const crp_field *top) {
+		std::memset(&D, 0, sizeof(D));
+		if (top->deco == DECO_EVOLUTION) {
+			D.evolve = 1;
+			D.evoM = top->evoM;
+			return gen(top->child[0], 0, 0, false) && emit(CO_EVOLVE, 0, 0, 0);
+		}
+		return gen(top, 0, 0, false);
+	}
+};
+
+// Compiles f's tree and uploads the program when it changed (a child's device layout may have).
+static int composite_refresh(const crp_field *cf) {
+	crp_field *f = const_cast<crp_field *>(cf);
+	CompBuilder B;
+	if (!B.build(f))
+		return fail(CRP_E_INVALID, "field decorator: " + B.err + " (not supported on the device)");
+	if (!f->dcomp || !f->hcomp)
+		return fail(CRP_E_STATE, "field decorator: no device program");
+	if (std::memcmp(f->hcomp, &B.D, sizeof(CompositeDesc)) != 0) {
+		CRP_HIP_TRY(hipMemcpy(f->dcomp, &B.D, sizeof(CompositeDesc), hipMemcpyHostToDevice));
+		std::memcpy(f->hcomp, &B.D, sizeof(CompositeDesc));
+	}
+	return CRP_OK;
+}
+
+// Before a field is used: a decorator's program is brought up to date (no allocation).
+static int field_prepare(const crp_field *f) {
+	return (f && f->kind == FK_COMPOSITE) ? composite_refresh(f) : CRP_OK;
+}
+
+// getField(position) of a decorator tree whose top is a MagneticFieldEvolution is zero: the class
+// overrides only getField(position, z)
+static bool composite_one_arg_zero(const crp_field *f) {
+	return f && f->kind == FK_COMPOSITE && f->deco == DECO_EVOLUTION;
+}
+
 // Launches the field evaluation of n device positions dpos into dB (both AoS xyz) on the null stream:
 // the FP64 evaluation (k_field_eval) or the mixed path's FP32 evaluation (k_field_evalf, FP16 storage
 // if set).  Shared by crp_field_eval, crp_field_eval_ex and crp_field_eval_noalloc.
 static hipError_t launch_field_eval(const crp_field_t *f, const double *dpos, double *dB, size_t n, int precision) {
 	const unsigned blocks = (unsigned)((n + 255) / 256);
 	const int kind = f ? f->kind : FK_NONE;
+	if (composite_one_arg_zero(f))
+		return hipMemsetAsync(dB, 0, 3 * n * sizeof(double), 0);
 	if (precision == CRP_PRECISION_FP64) {
 		switch (kind) {
 		case FK_UNIFORM: hipLaunchKernelGGL(k_field_eval<FieldUniform>, blocks, 256, 0, 0, f->uniform, dpos, dB, n); break;
@@ -850,6 +993,9 @@ static hipError_t launch_field_eval(const crp_field_t *f, const double *dpos, do
 				                   FieldModulated<FieldGrid>{FieldGrid{f->grid}, f->mod}, dpos, dB, n);
 			break;
 		case FK_JF12: hipLaunchKernelGGL(k_field_eval<FieldJF12>, blocks, 256, 0, 0, f->jf12, dpos, dB, n); break;
+		case FK_COMPOSITE:
+			hipLaunchKernelGGL(k_field_eval<FieldComposite>, blocks, 256, 0, 0, FieldComposite{f->dcomp, 1.0}, dpos, dB, n);
+			break;
 		default: hipLaunchKernelGGL(k_field_eval<FieldNone>, blocks, 256, 0, 0, FieldNone{}, dpos, dB, n); break;
 		}
 	} else {
@@ -875,6 +1021,9 @@ static hipError_t launch_field_eval(const crp_field_t *f, const double *dpos, do
 				                   FieldModulated<FieldGrid>{FieldGrid{f->grid}, f->mod}, dpos, dB, n);
 			break;
 		case FK_JF12: hipLaunchKernelGGL(k_field_evalf<FieldJF12>, blocks, 256, 0, 0, f->jf12, dpos, dB, n); break;
+		case FK_COMPOSITE:
+			hipLaunchKernelGGL(k_field_evalf<FieldComposite>, blocks, 256, 0, 0, FieldComposite{f->dcomp, 1.0}, dpos, dB, n);
+			break;
 		default: hipLaunchKernelGGL(k_field_evalf<FieldNone>, blocks, 256, 0, 0, FieldNone{}, dpos, dB, n); break;
 		}
 	}
@@ -890,6 +1039,8 @@ int crp_field_eval_ex(const crp_field_t *f, const double *pos, size_t n, double 
 		return fail(CRP_E_INVALID, "crp_field_eval_ex: NULL argument");
 	if (n == 0)
 		return CRP_OK;
+	if (int rc = field_prepare(f))
+		return rc;
 	double *dpos = nullptr, *dB = nullptr;
 	CRP_HIP_TRY(hipMalloc(&dpos, 3 * n * sizeof(double)));
 	hipError_t e = hipMalloc(&dB, 3 * n * sizeof(double));
@@ -953,6 +1104,8 @@ int crp_field_eval_noalloc(const crp_field_t *f, const double *pos, size_t n, do
 		                               " per call (split the call, or use crp_field_eval)");
 	if (n == 0)
 		return CRP_OK;
+	if (int rc = field_prepare(f))
+		return rc;
 	double *st = nullptr;
 	if (int rc = noalloc_scratch(&st))
 		return rc;
@@ -984,25 +1137,33 @@ int crp_field_create_planewave(int nm, const double *k, const double *Ak, const 
 	if (!f)
 		return fail(CRP_E_NOMEM, "crp_field_create_planewave: out of host memory");
 	// xi*Ak is the first product of B += xi[i] * Ak[i] * cos(...): formed once here, same IEEE op.
-	std::vector<double> xiA(3 * (size_t)nm);
-	for (int i = 0; i < nm; i++)
-		for (int a = 0; a < 3; a++)
-			xiA[3 * i + a] = xi[3 * i + a] * Ak[i];
-	// mixed-precision tables: phase in turns = p . (k kappa / 2pi) + beta / 2pi; amplitudes in FP32
+	// Mixed-precision records: phase in turns = p . (k kappa / 2pi) + beta / 2pi; amplitudes in FP32.
 	const double inv2pi = 1.0 / (2.0 * M_PI);
-	std::vector<double> kt(3 * (size_t)nm), bt(nm);
-	std::vector<float> xiAf(3 * (size_t)nm);
+	std::vector<PwMode64> m64(nm);
+	std::vector<PwModeMixed> mm(nm);
 	for (int i = 0; i < nm; i++) {
-		for (int a = 0; a < 3; a++) {
-			kt[3 * i + a] = k[i] * kappa[3 * i + a] * inv2pi;
-			xiAf[3 * i + a] = (float)xiA[3 * i + a];
-		}
-		bt[i] = beta[i] * inv2pi;
+		PwMode64 &a = m64[i];
+		a.k = k[i];
+		a.beta = beta[i];
+		a.kx = kappa[3 * i];
+		a.ky = kappa[3 * i + 1];
+		a.kz = kappa[3 * i + 2];
+		a.ax = xi[3 * i] * Ak[i];
+		a.ay = xi[3 * i + 1] * Ak[i];
+		a.az = xi[3 * i + 2] * Ak[i];
+		PwModeMixed &b = mm[i];
+		b.ktx = k[i] * kappa[3 * i] * inv2pi;
+		b.kty = k[i] * kappa[3 * i + 1] * inv2pi;
+		b.ktz = k[i] * kappa[3 * i + 2] * inv2pi;
+		b.bt = beta[i] * inv2pi;
+		b.ax = (float)a.ax;
+		b.ay = (float)a.ay;
+		b.az = (float)a.az;
+		b.pad = 0.f;
 	}
-	const size_t nb = (size_t)nm * sizeof(double);
-	const void *src[7] = {k, beta, kappa, xiA.data(), kt.data(), bt.data(), xiAf.data()};
-	const size_t sz[7] = {nb, nb, 3 * nb, 3 * nb, 3 * nb, nb, 3 * (size_t)nm * sizeof(float)};
-	for (int j = 0; j < 7; j++) {
+	const void *src[2] = {m64.data(), mm.data()};
+	const size_t sz[2] = {(size_t)nm * sizeof(PwMode64), (size_t)nm * sizeof(PwModeMixed)};
+	for (int j = 0; j < 2; j++) {
 		void *d = nullptr;
 		hipError_t e = hipMalloc(&d, sz[j]);
 		if (e == hipSuccess)
@@ -1018,13 +1179,8 @@ int crp_field_create_planewave(int nm, const double *k, const double *Ak, const 
 		f->owned[j] = d;
 	}
 	f->pw.nm = nm;
-	f->pw.k = (const double *)f->owned[0];
-	f->pw.beta = (const double *)f->owned[1];
-	f->pw.kappa = (const double *)f->owned[2];
-	f->pw.xiA = (const double *)f->owned[3];
-	f->pw.kt = (const double *)f->owned[4];
-	f->pw.bt = (const double *)f->owned[5];
-	f->pw.xiAf = (const float *)f->owned[6];
+	f->pw.m64 = (const PwMode64 *)f->owned[0];
+	f->pw.mm = (const PwModeMixed *)f->owned[1];
 	*out = f;
 	return CRP_OK;
 }
@@ -1039,7 +1195,136 @@ int crp_field_destroy(crp_field_t *f) {
 		(void)hipFree(f->cells);
 	if (f->cells16)
 		(void)hipFree(f->cells16);
+	if (f->dcomp)
+		(void)hipFree(f->dcomp);
+	delete f->hcomp;
 	delete f;
+	return CRP_OK;
+}
+
+// ---------------------------------------------------------------- decorators (C ABI)
+static int new_composite(int deco, crp_field_t **out, crp_field **res) {
+	crp_field *f = new_field(FK_COMPOSITE);
+	if (!f)
+		return fail(CRP_E_NOMEM, "field decorator: out of host memory");
+	f->deco = deco;
+	hipError_t e = hipMalloc((void **)&f->dcomp, sizeof(CompositeDesc));
+	f->hcomp = new (std::nothrow) CompositeDesc;
+	if (e != hipSuccess || !f->hcomp) {
+		if (e == hipSuccess)
+			(void)hipFree(f->dcomp);
+		delete f->hcomp;
+		delete f;
+		return fail(CRP_E_NOMEM, "field decorator: device or host memory");
+	}
+	std::memset(f->hcomp, 0xff, sizeof(CompositeDesc)); // differs from any program: the first use uploads
+	*res = f;
+	*out = f;
+	return CRP_OK;
+}
+
+// The tree below f must compile (depth and size limits of the device program).
+static int composite_check(crp_field *f) {
+	CompBuilder B;
+	if (!B.build(f))
+		return fail(CRP_E_INVALID, "field decorator: " + B.err + " (not supported on the device)");
+	return CRP_OK;
+}
+
+int crp_field_create_periodic(const crp_field_t *field, const double extends[3], const double origin[3], int reflective,
+                              crp_field_t **out) {
+	if (!extends || !out)
+		return fail(CRP_E_INVALID, "crp_field_create_periodic: NULL argument");
+	crp_field *f = nullptr;
+	if (int rc = new_composite(DECO_PERIODIC, out, &f))
+		return rc;
+	f->child[0] = field;
+	f->nchild = 1;
+	for (int a = 0; a < 3; a++) {
+		f->extends[a] = extends[a];
+		f->porigin[a] = origin ? origin[a] : 0.0;
+	}
+	f->preflective = reflective ? 1 : 0;
+	if (int rc = composite_check(f)) {
+		crp_field_destroy(f);
+		*out = nullptr;
+		return rc;
+	}
+	return CRP_OK;
+}
+
+int crp_field_create_list(crp_field_t **out) {
+	if (!out)
+		return fail(CRP_E_INVALID, "crp_field_create_list: NULL argument");
+	crp_field *f = nullptr;
+	return new_composite(DECO_LIST, out, &f);
+}
+
+int crp_field_list_add(crp_field_t *list, const crp_field_t *field) {
+	if (!list || list->kind != FK_COMPOSITE || list->deco != DECO_LIST)
+		return fail(CRP_E_INVALID, "crp_field_list_add: not a field list");
+	if (list->nchild >= kDecoMaxChildren)
+		return fail(CRP_E_INVALID, "crp_field_list_add: at most " + std::to_string(kDecoMaxChildren) +
+		                               " fields per list on the device");
+	list->child[list->nchild++] = field;
+	if (int rc = composite_check(list)) {
+		list->nchild--;
+		return rc;
+	}
+	return CRP_OK;
+}
+
+int crp_field_create_evolution(const crp_field_t *field, double m, crp_field_t **out) {
+	if (!out)
+		return fail(CRP_E_INVALID, "crp_field_create_evolution: NULL argument");
+	crp_field *f = nullptr;
+	if (int rc = new_composite(DECO_EVOLUTION, out, &f))
+		return rc;
+	f->child[0] = field;
+	f->nchild = 1;
+	f->evoM = m;
+	if (int rc = composite_check(f)) {
+		crp_field_destroy(f);
+		*out = nullptr;
+		return rc;
+	}
+	return CRP_OK;
+}
+
+int crp_field_eval_z(const crp_field_t *f, const double *pos, const double *z, size_t n, double *B) {
+	if (!pos || !z || !B)
+		return fail(CRP_E_INVALID, "crp_field_eval_z: NULL argument");
+	if (!(f && f->kind == FK_COMPOSITE && f->deco == DECO_EVOLUTION))
+		return crp_field_eval(f, pos, n, B); // only a top-level evolution reads z
+	if (n == 0)
+		return CRP_OK;
+	if (int rc = field_prepare(f))
+		return rc;
+	double *dpos = nullptr, *dB = nullptr, *dz = nullptr;
+	hipError_t e = hipMalloc(&dpos, 3 * n * sizeof(double));
+	if (e == hipSuccess)
+		e = hipMalloc(&dB, 3 * n * sizeof(double));
+	if (e == hipSuccess)
+		e = hipMalloc(&dz, n * sizeof(double));
+	if (e == hipSuccess)
+		e = hipMemcpy(dpos, pos, 3 * n * sizeof(double), hipMemcpyHostToDevice);
+	if (e == hipSuccess)
+		e = hipMemcpy(dz, z, n * sizeof(double), hipMemcpyHostToDevice);
+	if (e == hipSuccess) {
+		hipLaunchKernelGGL(k_field_eval_z<FieldComposite>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0,
+		                   FieldComposite{f->dcomp, 1.0}, dpos, dz, dB, n);
+		e = hipGetLastError();
+	}
+	if (e == hipSuccess)
+		e = hipMemcpy(B, dB, 3 * n * sizeof(double), hipMemcpyDeviceToHost);
+	if (dpos)
+		(void)hipFree(dpos);
+	if (dB)
+		(void)hipFree(dB);
+	if (dz)
+		(void)hipFree(dz);
+	if (e != hipSuccess)
+		return fail(CRP_E_HIP, std::string("crp_field_eval_z: ") + hipGetErrorString(e));
 	return CRP_OK;
 }
 
@@ -1048,6 +1333,8 @@ int crp_field_eval(const crp_field_t *f, const double *pos, size_t n, double *B)
 		return fail(CRP_E_INVALID, "crp_field_eval: NULL argument");
 	if (n == 0)
 		return CRP_OK;
+	if (int rc = field_prepare(f))
+		return rc;
 	double *dpos = nullptr, *dB = nullptr;
 	CRP_HIP_TRY(hipMalloc(&dpos, 3 * n * sizeof(double)));
 	hipError_t e = hipMalloc(&dB, 3 * n * sizeof(double));
@@ -1308,9 +1595,9 @@ int crp_batch_create(size_t capacity, crp_batch_t **out) {
 	std::memset(b, 0, sizeof(*b));
 	b->cap = capacity;
 	(void)hipGetDevice(&b->device);
-	// one allocation: 18 double arrays, the active flags, the queue order, then a
+	// one allocation: 25 double arrays, the active flags, the queue order, then a
 	// 256-byte aligned counter block
-	const size_t nd = 24;
+	const size_t nd = 25;
 	size_t off = nd * capacity * sizeof(double) + 2 * capacity * sizeof(int32_t) + 2 * capacity * sizeof(uint32_t) +
 	             capacity * sizeof(uint64_t);
 	off = (off + 255) & ~(size_t)255;
@@ -1320,9 +1607,9 @@ int crp_batch_create(size_t capacity, crp_batch_t **out) {
 		delete b;
 		return fail(CRP_E_NOMEM, std::string("crp_batch_create: hipMalloc: ") + hipGetErrorString(e));
 	}
-	double **arr[24] = {&b->px, &b->py, &b->pz, &b->dx, &b->dy, &b->dz, &b->E, &b->q, &b->cur,
+	double **arr[25] = {&b->px, &b->py, &b->pz, &b->dx, &b->dy, &b->dz, &b->E, &b->q, &b->cur,
 	                    &b->next, &b->len, &b->time, &b->ppx, &b->ppy, &b->ppz, &b->pdx, &b->pdy, &b->pdz,
-	                    &b->sx, &b->sy, &b->sz, &b->cx, &b->cy, &b->cz};
+	                    &b->sx, &b->sy, &b->sz, &b->cx, &b->cy, &b->cz, &b->z};
 	for (size_t i = 0; i < nd; i++)
 		*arr[i] = b->base + i * capacity;
 	b->active = (int32_t *)(b->base + nd * capacity);
@@ -1386,9 +1673,10 @@ int crp_batch_upload(crp_batch_t *b, size_t n, const crp_host_state *s) {
 			tmp[i] = src[3 * i + (c % 3)];
 		CRP_HIP_TRY(hipMemcpy(dst3[c], tmp.data(), n * sizeof(double), hipMemcpyHostToDevice));
 	}
-	const double *src1[6] = {s->energy, s->charge, s->current_step, s->next_step, s->trajectory_length, s->time};
-	double *dst1[6] = {b->E, b->q, b->cur, b->next, b->len, b->time};
-	for (int c = 0; c < 6; c++) {
+	const double *src1[7] = {s->energy, s->charge, s->current_step, s->next_step, s->trajectory_length, s->time,
+	                         s->redshift};
+	double *dst1[7] = {b->E, b->q, b->cur, b->next, b->len, b->time, b->z};
+	for (int c = 0; c < 7; c++) {
 		if (src1[c]) {
 			CRP_HIP_TRY(hipMemcpy(dst1[c], src1[c], n * sizeof(double), hipMemcpyHostToDevice));
 		} else {
@@ -1536,14 +1824,14 @@ int crp_batch_copy(crp_batch_t *dst, const crp_batch_t *src) {
 	if (n == 0)
 		return CRP_OK;
 	CRP_HIP_TRY(hipStreamSynchronize(src->stream));
-	const double *s[24] = {src->px, src->py, src->pz, src->dx, src->dy, src->dz, src->E, src->q, src->cur,
+	const double *s[25] = {src->px, src->py, src->pz, src->dx, src->dy, src->dz, src->E, src->q, src->cur,
 	                       src->next, src->len, src->time, src->ppx, src->ppy, src->ppz, src->pdx, src->pdy, src->pdz,
-	                       src->sx, src->sy, src->sz, src->cx, src->cy, src->cz};
-	double *d[24] = {dst->px, dst->py, dst->pz, dst->dx, dst->dy, dst->dz, dst->E, dst->q, dst->cur,
+	                       src->sx, src->sy, src->sz, src->cx, src->cy, src->cz, src->z};
+	double *d[25] = {dst->px, dst->py, dst->pz, dst->dx, dst->dy, dst->dz, dst->E, dst->q, dst->cur,
 	                 dst->next, dst->len, dst->time, dst->ppx, dst->ppy, dst->ppz, dst->pdx, dst->pdy, dst->pdz,
-	                 dst->sx, dst->sy, dst->sz, dst->cx, dst->cy, dst->cz};
+	                 dst->sx, dst->sy, dst->sz, dst->cx, dst->cy, dst->cz, dst->z};
 	const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
-	for (int c = 0; c < 24; c++)
+	for (int c = 0; c < 25; c++)
 		hipLaunchKernelGGL(k_copy_f64, dim3(blocks), dim3(256), 0, dst->stream, s[c], d[c], n);
 	CRP_HIP_TRY(hipGetLastError());
 	CRP_HIP_TRY(hipMemcpyAsync(dst->active, src->active, n * sizeof(int32_t), hipMemcpyDeviceToDevice, dst->stream));
@@ -1616,6 +1904,8 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 		if (rc0 != CRP_OK)
 			return rc0;
 	}
+	if (int rc0 = field_prepare(p->field)) // a decorator's program, before the timed region
+		return rc0;
 	CRP_HIP_TRY(hipEventRecord(b->ev0, b->stream));
 	int rc;
 	const crp_field *f = p->field;
@@ -1635,6 +1925,7 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 		              : dispatch_kind(FieldModulated<FieldGrid>{FieldGrid{f->grid}, f->mod}, p, P, b, &st.launches);
 		break;
 	case FK_JF12: rc = dispatch_kind(f->jf12, p, P, b, &st.launches); break;
+	case FK_COMPOSITE: rc = dispatch_kind(FieldComposite{f->dcomp, 1.0}, p, P, b, &st.launches); break;
 	default: rc = dispatch_kind(FieldNone{}, p, P, b, &st.launches); break;
 	}
 	if (rc != CRP_OK)

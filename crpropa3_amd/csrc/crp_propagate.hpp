@@ -16,6 +16,7 @@
 //   positions, CK error norm, step acceptance and step-size update.
 #pragma once
 
+#include "crp_composite.hpp"
 #include "crp_device.hpp"
 
 namespace crp {
@@ -38,6 +39,7 @@ struct Soa {
 	unsigned long long cap;   // snapshot stride
 	uint32_t *sdeStep;        // DiffusionSDE: steps taken (the random-number counter)
 	const uint64_t *sdeStream; // DiffusionSDE: the particle's random stream
+	const double *z;          // the candidate's redshift (read by fields that use it)
 };
 
 constexpr int kSnapDoubles = 9; // position x,y,z, direction x,y,z, trajectory length, time, nextStep
@@ -101,12 +103,13 @@ struct WavesHint<FieldGridCells> {
 
 template <class F, int KIND, bool MIX>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIX ? 1 : WavesHint<F>::fp64, 8)))
-k_propagate(F field, Params P, Soa S, unsigned long long n,
+k_propagate(F field0, Params P, Soa S, unsigned long long n,
                                                    unsigned long long *__restrict__ ctr) {
 	constexpr bool kCK = (KIND == PK_CK_FIXED || KIND == PK_CK_ADAPTIVE);
 	constexpr bool kAdaptive = (KIND == PK_BP_ADAPTIVE || KIND == PK_CK_ADAPTIVE);
 	constexpr unsigned long long kEvalsPerTrial = kCK ? 6 : (kAdaptive ? 3 : 1);
 	const int lane = __lane_id();
+	F field = field0; // the lane's copy: a field that reads the candidate's redshift binds it at refill
 
 	bool have = false, exhausted = false, inProc = false;
 	unsigned idx = 0;
@@ -156,6 +159,8 @@ k_propagate(F field, Params P, Soa S, unsigned long long n,
 							dirf = make_float3((float)dir.x, (float)dir.y, (float)dir.z);
 						E = S.E[idx];
 						q = S.q[idx];
+						if (UsesRedshift<F>::value)
+							bind_redshift(field, S.z[idx]);
 						*cCur = S.cur[idx];
 						next = S.next[idx];
 						*cLen = S.len[idx];

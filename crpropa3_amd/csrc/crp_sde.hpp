@@ -170,9 +170,10 @@ enum SdePhase { SDE_BEGIN = 0, SDE_TRIAL = 1, SDE_SUB = 2 };
 // MODS: the periodic box, spherical boundary and observer are compiled in (the length limit always is);
 // without them the kernel keeps fewer registers
 template <class F, bool MODS>
-__global__ void __launch_bounds__(256) k_sde(F field, Params P, Soa S, unsigned long long n,
+__global__ void __launch_bounds__(256) k_sde(F field0, Params P, Soa S, unsigned long long n,
                                              unsigned long long *__restrict__ ctr) {
 	const int lane = __lane_id();
+	F field = field0; // the lane's copy: a field that reads the candidate's redshift binds it at refill
 	const double cosHalfPi = crp_cos_cr(3.14159265358979323846 / 2.);
 	const double minTime = P.minStep / kCLight; // minStep / c_light of the trial loop's test
 
@@ -222,6 +223,8 @@ __global__ void __launch_bounds__(256) k_sde(F field, Params P, Soa S, unsigned 
 						pos = v3(S.px[idx], S.py[idx], S.pz[idx]);
 						dir = v3(S.dx[idx], S.dy[idx], S.dz[idx]);
 						q = S.q[idx];
+						if (UsesRedshift<F>::value)
+							bind_redshift(field, S.z[idx]); // DiffusionSDE.cpp:77, 245
 						next = S.next[idx];
 						*cCur = S.cur[idx];
 						*cLen = S.len[idx];

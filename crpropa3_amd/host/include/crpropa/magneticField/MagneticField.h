@@ -10,6 +10,7 @@
 #include "crpropa/Vector3.h"
 
 #include <mutex>
+#include <vector>
 
 struct crp_field;
 
@@ -36,6 +37,80 @@ private:
 	Vector3d value;
 	mutable crp_field *dev;
 	mutable std::mutex mtx;
+};
+
+/* Field decorators (include/crpropa/magneticField/MagneticField.h:47-103,
+ * src/magneticField/MagneticField.cpp:5-79), evaluated on the device: the wrapped fields must have
+ * a device implementation.  Each keeps a device decorator over its children's current device
+ * fields and rebuilds it when a child's device field or a setting changes. */
+class DecoratorField : public MagneticField {
+public:
+	~DecoratorField();
+	crp_field *deviceField() const;
+
+protected:
+	DecoratorField() : dev(nullptr) {}
+	void invalidate();
+	virtual int create(const std::vector<crp_field *> &children, crp_field **out) const = 0;
+	virtual std::vector<ref_ptr<MagneticField> > childFields() const = 0;
+
+private:
+	mutable crp_field *dev;
+	mutable std::vector<crp_field *> devChildren;
+	mutable std::mutex mtx;
+};
+
+/* PeriodicMagneticField: the field continued periodically (or reflected) outside the cell
+ * [origin, origin + extends] */
+class PeriodicMagneticField : public DecoratorField {
+	ref_ptr<MagneticField> field;
+	Vector3d origin, extends;
+	bool reflective;
+
+public:
+	PeriodicMagneticField(ref_ptr<MagneticField> field, const Vector3d &extends);
+	PeriodicMagneticField(ref_ptr<MagneticField> field, const Vector3d &extends, const Vector3d &origin,
+	                      bool reflective);
+	Vector3d &getOrigin();
+	void setOrigin(const Vector3d &origin);
+	Vector3d &getExtends();
+	void setExtends(const Vector3d &origin);
+	bool isReflective();
+	void setReflective(bool reflective);
+	Vector3d getField(const Vector3d &position) const;
+
+protected:
+	int create(const std::vector<crp_field *> &children, crp_field **out) const;
+	std::vector<ref_ptr<MagneticField> > childFields() const { return {field}; }
+};
+
+/* MagneticFieldList: the sum of its fields, in the order they were added */
+class MagneticFieldList : public DecoratorField {
+	std::vector<ref_ptr<MagneticField> > fields;
+
+public:
+	void addField(ref_ptr<MagneticField> field);
+	Vector3d getField(const Vector3d &position) const;
+
+protected:
+	int create(const std::vector<crp_field *> &children, crp_field **out) const;
+	std::vector<ref_ptr<MagneticField> > childFields() const { return fields; }
+};
+
+/* MagneticFieldEvolution: field * (1 + z)^m.  As in the reference only getField(position, z) is
+ * overridden: through a MagneticField reference the one-argument call gives zero, a direct call
+ * with one argument takes z = 0. */
+class MagneticFieldEvolution : public DecoratorField {
+	ref_ptr<MagneticField> field;
+	double m;
+
+public:
+	MagneticFieldEvolution(ref_ptr<MagneticField> field, double m);
+	Vector3d getField(const Vector3d &position, double z = 0) const;
+
+protected:
+	int create(const std::vector<crp_field *> &children, crp_field **out) const;
+	std::vector<ref_ptr<MagneticField> > childFields() const { return {field}; }
 };
 
 /* Evaluates any device-capable field at host positions on the GPU. */

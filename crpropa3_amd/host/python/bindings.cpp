@@ -216,6 +216,24 @@ PYBIND11_MODULE(_crpropa, m) {
 	py::class_<UniformMagneticField, MagneticField, ref_ptr<UniformMagneticField>>(m, "UniformMagneticField")
 	    .def(py::init<const Vector3d &>());
 
+	py::class_<DecoratorField, MagneticField, ref_ptr<DecoratorField>>(m, "DecoratorField");
+	py::class_<PeriodicMagneticField, DecoratorField, ref_ptr<PeriodicMagneticField>>(m, "PeriodicMagneticField")
+	    .def(py::init<ref_ptr<MagneticField>, const Vector3d &>(), py::arg("field"), py::arg("extends"))
+	    .def(py::init<ref_ptr<MagneticField>, const Vector3d &, const Vector3d &, bool>(), py::arg("field"),
+	         py::arg("extends"), py::arg("origin"), py::arg("reflective"))
+	    .def("getOrigin", &PeriodicMagneticField::getOrigin)
+	    .def("setOrigin", &PeriodicMagneticField::setOrigin)
+	    .def("getExtends", &PeriodicMagneticField::getExtends)
+	    .def("setExtends", &PeriodicMagneticField::setExtends)
+	    .def("isReflective", &PeriodicMagneticField::isReflective)
+	    .def("setReflective", &PeriodicMagneticField::setReflective);
+	py::class_<MagneticFieldList, DecoratorField, ref_ptr<MagneticFieldList>>(m, "MagneticFieldList")
+	    .def(py::init<>())
+	    .def("addField", &MagneticFieldList::addField);
+	py::class_<MagneticFieldEvolution, DecoratorField, ref_ptr<MagneticFieldEvolution>>(m, "MagneticFieldEvolution")
+	    .def(py::init<ref_ptr<MagneticField>, double>(), py::arg("field"), py::arg("m"))
+	    .def("getField", &MagneticFieldEvolution::getField, py::arg("position"), py::arg("z") = 0.0);
+
 	py::enum_<interpolationType>(m, "interpolationType")
 	    .value("TRILINEAR", TRILINEAR)
 	    .value("TRICUBIC", TRICUBIC)

@@ -134,6 +134,119 @@ Vector3d MagneticFieldGrid::getField(const Vector3d &position) const {
 	return Vector3d(B[0], B[1], B[2]);
 }
 
+// ------------------------------------------------------------------ decorators
+DecoratorField::~DecoratorField() {
+	if (dev)
+		crp_field_destroy(dev);
+}
+
+void DecoratorField::invalidate() {
+	std::lock_guard<std::mutex> lock(mtx);
+	if (dev)
+		crp_field_destroy(dev);
+	dev = nullptr;
+	devChildren.clear();
+}
+
+crp_field *DecoratorField::deviceField() const {
+	std::vector<crp_field *> ch;
+	for (const ref_ptr<MagneticField> &f : childFields()) {
+		crp_field *d = f.valid() ? f->deviceField() : nullptr;
+		if (f.valid() && !d)
+			return nullptr; // a child the device cannot evaluate
+		ch.push_back(d);
+	}
+	std::lock_guard<std::mutex> lock(mtx);
+	if (dev && ch == devChildren)
+		return dev;
+	if (dev)
+		crp_field_destroy(dev);
+	dev = nullptr;
+	throwIf(create(ch, &dev));
+	devChildren = ch;
+	return dev;
+}
+
+PeriodicMagneticField::PeriodicMagneticField(ref_ptr<MagneticField> field, const Vector3d &extends)
+    : field(field), origin(0, 0, 0), extends(extends), reflective(false) {}
+
+PeriodicMagneticField::PeriodicMagneticField(ref_ptr<MagneticField> field, const Vector3d &extends,
+                                             const Vector3d &origin, bool reflective)
+    : field(field), origin(origin), extends(extends), reflective(reflective) {}
+
+Vector3d &PeriodicMagneticField::getOrigin() {
+	invalidate(); // the reference returns a reference that may be written through
+	return origin;
+}
+void PeriodicMagneticField::setOrigin(const Vector3d &o) {
+	origin = o;
+	invalidate();
+}
+Vector3d &PeriodicMagneticField::getExtends() {
+	invalidate();
+	return extends;
+}
+void PeriodicMagneticField::setExtends(const Vector3d &e) {
+	extends = e;
+	invalidate();
+}
+bool PeriodicMagneticField::isReflective() { return reflective; }
+void PeriodicMagneticField::setReflective(bool r) {
+	reflective = r;
+	invalidate();
+}
+
+int PeriodicMagneticField::create(const std::vector<crp_field *> &ch, crp_field **out) const {
+	const double e[3] = {extends.x, extends.y, extends.z}, o[3] = {origin.x, origin.y, origin.z};
+	return crp_field_create_periodic(ch[0], e, o, reflective ? 1 : 0, out);
+}
+
+Vector3d PeriodicMagneticField::getField(const Vector3d &position) const {
+	double B[3];
+	const double p[3] = {position.x, position.y, position.z};
+	evaluateOnDevice(*this, p, 1, B);
+	return Vector3d(B[0], B[1], B[2]);
+}
+
+void MagneticFieldList::addField(ref_ptr<MagneticField> field) {
+	fields.push_back(field);
+	invalidate();
+}
+
+int MagneticFieldList::create(const std::vector<crp_field *> &ch, crp_field **out) const {
+	int rc = crp_field_create_list(out);
+	for (size_t i = 0; rc == CRP_OK && i < ch.size(); i++)
+		rc = crp_field_list_add(*out, ch[i]);
+	if (rc != CRP_OK && *out) {
+		crp_field_destroy(*out);
+		*out = nullptr;
+	}
+	return rc;
+}
+
+Vector3d MagneticFieldList::getField(const Vector3d &position) const {
+	double B[3];
+	const double p[3] = {position.x, position.y, position.z};
+	evaluateOnDevice(*this, p, 1, B);
+	return Vector3d(B[0], B[1], B[2]);
+}
+
+MagneticFieldEvolution::MagneticFieldEvolution(ref_ptr<MagneticField> field, double m) : field(field), m(m) {}
+
+int MagneticFieldEvolution::create(const std::vector<crp_field *> &ch, crp_field **out) const {
+	return crp_field_create_evolution(ch[0], m, out);
+}
+
+Vector3d MagneticFieldEvolution::getField(const Vector3d &position, double z) const {
+	crp_field *d = deviceField();
+	if (!d)
+		throw std::runtime_error("crpropa3_amd: this magnetic field has no device implementation");
+	double B[3];
+	const double p[3] = {position.x, position.y, position.z};
+	throwIf(crp_field_eval_z(d, p, &z, 1, B));
+	return Vector3d(B[0], B[1], B[2]);
+}
+
 // ------------------------------------------------------------------ plane waves
 // Mode table exactly as PlaneWaveTurbulence::PlaneWaveTurbulence builds it
 // (src/magneticField/turbulentField/PlaneWaveTurbulence.cpp:99-178), with the

@@ -38,7 +38,7 @@ typedef std::unique_ptr<crp_batch, BatchDeleter> BatchHandle;
 // ------------------------------------------------------------------ packing
 /* Host mirror of a candidate vector in the layout crp_batch_upload takes. */
 struct Packed {
-	std::vector<double> pos, dir, E, q, cur, next, len, time, ppos, pdir;
+	std::vector<double> pos, dir, E, q, cur, next, len, time, ppos, pdir, z;
 	std::vector<int32_t> active;
 	std::vector<int32_t> det; // observer detections per candidate (empty: none tracked)
 	std::vector<double> src, cre; // source / created positions (empty: not tracked)
@@ -47,6 +47,7 @@ struct Packed {
 	void resize(size_t n) {
 		pos.resize(3 * n); dir.resize(3 * n); ppos.resize(3 * n); pdir.resize(3 * n);
 		E.resize(n); q.resize(n); cur.resize(n); next.resize(n); len.resize(n); time.resize(n);
+		z.resize(n);
 		active.resize(n);
 	}
 	crp_host_state view() {
@@ -60,6 +61,7 @@ struct Packed {
 		s.created_pos = cre.empty() ? nullptr : cre.data();
 		s.sde_step = sdeStep.empty() ? nullptr : sdeStep.data();
 		s.sde_stream = sdeStream.empty() ? nullptr : sdeStream.data();
+		s.redshift = z.data();
 		return s;
 	}
 };
@@ -85,6 +87,7 @@ static void pack(const std::vector<Candidate *> &c, Packed &p, bool sde = false)
 		p.next[i] = c[i]->getNextStep();
 		p.len[i] = c[i]->getTrajectoryLength();
 		p.time[i] = c[i]->getTime();
+		p.z[i] = c[i]->getRedshift();
 		p.active[i] = c[i]->isActive() ? 1 : 0;
 	}
 }

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define CRP_ABI_VERSION 8
+#define CRP_ABI_VERSION 9
 
 enum crp_status {
 	CRP_OK = 0,
@@ -116,6 +116,27 @@ int crp_field_destroy(crp_field_t *f);
 
 /* B at n host positions (AoS xyz), evaluated on the device with the FP64 path. */
 int crp_field_eval(const crp_field_t *f, const double *pos, size_t n, double *B);
+
+/* Field decorators (src/magneticField/MagneticField.cpp:41-79, include/crpropa/magneticField/MagneticField.h:47-103).
+ * They borrow the fields they wrap: those must outlive the decorator.  Decorators nest (a list of
+ * periodic fields, a periodic list, ...) up to 3 levels of lists or periodic maps, at most 8 fields
+ * below a tree; deeper trees are rejected with CRP_E_INVALID.
+ *   PeriodicMagneticField(field, extends, origin, reflective): origin may be NULL (0, 0, 0).
+ *   MagneticFieldList() + addField(field): crp_field_create_list, then crp_field_list_add in order
+ *     (at most 8 fields per list).
+ *   MagneticFieldEvolution(field, m): getField(position, z) = field->getField(position) * (1+z)^m;
+ *     its one-argument getField (crp_field_eval) is zero, as in the reference; the propagators use
+ *     the candidate's redshift (crp_host_state.redshift), and pow is correctly rounded. */
+int crp_field_create_periodic(const crp_field_t *field, const double extends[3], const double origin[3], int reflective,
+                              crp_field_t **out);
+int crp_field_create_list(crp_field_t **out);
+int crp_field_list_add(crp_field_t *list, const crp_field_t *field);
+int crp_field_create_evolution(const crp_field_t *field, double m, crp_field_t **out);
+
+/* MagneticField::getField(position, z) at n host positions with their redshifts z[n] (FP64): the call
+ * PropagationBP/CK::getFieldAtPosition(pos, z) makes (src/module/PropagationBP.cpp:137-149).  Only a
+ * top-level MagneticFieldEvolution reads z; for every other field this is crp_field_eval. */
+int crp_field_eval_z(const crp_field_t *f, const double *pos, const double *z, size_t n, double *B);
 
 /* crp_field_eval_ex without any device allocation: positions and results are staged through a
  * fixed per-device buffer of the library (CRP_EVAL_NOALLOC_MAX positions, 6 MiB, held until the
@@ -216,6 +237,8 @@ typedef struct {
 	                              (may be NULL: 0 on upload) */
 	const uint64_t *sde_stream; /* [n] DiffusionSDE random stream of each particle, e.g. its serial
 	                              number (upload only; may be NULL: the particle's index) */
+	const double *redshift;    /* [n] candidate redshift, the z of getField(position, z) (upload only;
+	                              may be NULL: 0) */
 } crp_host_state;
 
 int crp_batch_create(size_t capacity, crp_batch_t **out);

@@ -548,22 +548,73 @@ static void orc_jf12_eval1(const orc_field *f, const double p[3], double B[3], i
 	B[2] = b[2];
 }
 
-/* MagneticField::getField dispatch + PropagationBP/CK::getFieldAtPosition
- * (src/module/PropagationBP.cpp:137-149): a missing field gives B = 0. */
-static inline void orc_field_eval1(const orc_field *f, const double p[3], double B[3], int libm) {
+/* MagneticField::getField(position), the one-argument call (include/crpropa/magneticField/MagneticField.h:26-28):
+ * a missing field gives B = 0 (src/module/PropagationBP.cpp:137-149). */
+static void orc_field_eval1(const orc_field *f, const double p[3], double B[3], int libm) {
 	switch (f ? f->type : ORC_FIELD_NONE) {
 	case ORC_FIELD_UNIFORM: B[0] = f->B0[0]; B[1] = f->B0[1]; B[2] = f->B0[2]; break;
 	case ORC_FIELD_GRID: orc_grid_eval1(f, p, B); break;
 	case ORC_FIELD_PLANEWAVE: orc_planewave_eval1(f, p, B, libm); break;
 	case ORC_FIELD_MODULATED: orc_modulated_eval1(f, p, B); break;
 	case ORC_FIELD_JF12: orc_jf12_eval1(f, p, B, libm); break;
+	case ORC_FIELD_PERIODIC: {
+		/* PeriodicMagneticField::getField, src/magneticField/MagneticField.cpp:41-58:
+		 * n = floor((position - origin) / extends), p = position - origin - n * extends,
+		 * reflected in the odd periods */
+		double q[3];
+		for (int a = 0; a < 3; a++) {
+			double n = floor((p[a] - f->porigin[a]) / f->extends[a]);
+			q[a] = (p[a] - f->porigin[a]) - n * f->extends[a];
+			if (f->preflective && ((long)fabs(n) % 2) == 1)
+				q[a] = f->extends[a] - q[a];
+		}
+		orc_field_eval1(f->children[0], q, B, libm);
+		break;
+	}
+	case ORC_FIELD_LIST: {
+		/* MagneticFieldList::getField, src/magneticField/MagneticField.cpp:64-69: b = 0; b += each */
+		double b[3] = {0, 0, 0};
+		for (int i = 0; i < f->nchildren; i++) {
+			double c[3];
+			orc_field_eval1(f->children[i], p, c, libm);
+			b[0] += c[0];
+			b[1] += c[1];
+			b[2] += c[2];
+		}
+		B[0] = b[0]; B[1] = b[1]; B[2] = b[2];
+		break;
+	}
+	/* MagneticFieldEvolution only overrides getField(position, z): its one-argument getField is the
+	 * base class's, zero (MagneticField.h:26-28, 102) */
 	default: B[0] = B[1] = B[2] = 0; break;
 	}
+}
+
+/* MagneticField::getField(position, z), the call of PropagationBP/CK::getFieldAtPosition
+ * (src/module/PropagationBP.cpp:137-149): the base class forwards to getField(position);
+ * MagneticFieldEvolution::getField (src/magneticField/MagneticField.cpp:76-79) scales its field's
+ * getField(position) by pow(1 + z, m). */
+static inline void orc_field_eval2(const orc_field *f, const double p[3], double z, double B[3], int libm) {
+	if (f && f->type == ORC_FIELD_EVOLUTION) {
+		double c[3];
+		orc_field_eval1(f->children[0], p, c, libm);
+		const double s = orc_pow(1 + z, f->evo_m, libm);
+		B[0] = c[0] * s;
+		B[1] = c[1] * s;
+		B[2] = c[2] * s;
+		return;
+	}
+	orc_field_eval1(f, p, B, libm);
 }
 
 void orc_field_eval(const orc_field *f, const double *pos, size_t m, double *B, int libm) {
 	for (size_t i = 0; i < m; i++)
 		orc_field_eval1(f, pos + 3 * i, B + 3 * i, libm);
+}
+
+void orc_field_eval_z(const orc_field *f, const double *pos, const double *z, size_t m, double *B, int libm) {
+	for (size_t i = 0; i < m; i++)
+		orc_field_eval2(f, pos + 3 * i, z[i], B + 3 * i, libm);
 }
 
 /* PlaneWaveTurbulence ctor, src/magneticField/turbulentField/PlaneWaveTurbulence.cpp:99-178 (seed != 0).
@@ -620,11 +671,11 @@ typedef struct {
 
 /* PropagationBP::dY, src/module/PropagationBP.cpp:19-39 */
 static inline orc_y orc_bp_dy(const orc_field *f, const double pos0[3], const double dir0[3], double step, double q,
-                              double m, int libm) {
+                              double m, double z, int libm) {
 	double pos[3], dir[3], B[3], t[3], s[3], v[3];
 	for (int a = 0; a < 3; a++)
 		pos[a] = pos0[a] + (dir0[a] * step) / 2.;
-	orc_field_eval1(f, pos, B, libm);
+	orc_field_eval2(f, pos, z, B, libm);
 	for (int a = 0; a < 3; a++)
 		t[a] = ((((B[a] * q) / 2) / m) * step) / C_LIGHT;
 	double tt = (t[0] * t[0] + t[1] * t[1]) + t[2] * t[2];
@@ -649,13 +700,13 @@ static inline orc_y orc_bp_dy(const orc_field *f, const double pos0[3], const do
 static inline double orc_norm(const double v[3]) { return sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]); }
 
 /* PropagationCK::dYdt, src/module/PropagationCK.cpp:51-61 */
-static inline orc_y orc_ck_dydt(const orc_field *f, const orc_y *y, double q, double E, int libm) {
+static inline orc_y orc_ck_dydt(const orc_field *f, const orc_y *y, double q, double E, double z, int libm) {
 	orc_y d;
 	double n = orc_norm(y->u);
 	double B[3];
 	for (int a = 0; a < 3; a++)
 		d.x[a] = (y->u[a] / n) * C_LIGHT;
-	orc_field_eval1(f, y->x, B, libm);
+	orc_field_eval2(f, y->x, z, B, libm);
 	double fac = (q * C_LIGHT) / E;
 	d.u[0] = (d.x[1] * B[2] - B[1] * d.x[2]) * fac;
 	d.u[1] = (d.x[2] * B[0] - B[2] * d.x[0]) * fac;
@@ -675,7 +726,7 @@ static const double CK_BS[6] = {2825. / 27648., 0., 18575. / 48384., 13525. / 55
 
 /* PropagationCK::tryStep, src/module/PropagationCK.cpp:28-49 */
 static inline void orc_ck_trystep(const orc_field *f, const orc_y *y, orc_y *out, orc_y *err, double h, double q,
-                                  double E, int libm) {
+                                  double E, double z, int libm) {
 	orc_y k[6];
 	*out = *y;
 	for (int a = 0; a < 3; a++)
@@ -689,7 +740,7 @@ static inline void orc_ck_trystep(const orc_field *f, const orc_y *y, orc_y *out
 				yn.u[a] += (k[j].u[a] * aij) * h;
 			}
 		}
-		k[i] = orc_ck_dydt(f, &yn, q, E, libm);
+		k[i] = orc_ck_dydt(f, &yn, q, E, z, libm);
 		double bi = CK_B[i], ei = CK_B[i] - CK_BS[i];
 		for (int a = 0; a < 3; a++) {
 			out->x[a] += (k[i].x[a] * bi) * h;
@@ -798,6 +849,7 @@ static inline void orc_process(const orc_field *f, const orc_propagator *p, orc_
 		}
 	}
 	double q = s->charge[i];
+	const double zr = s->redshift ? s->redshift[i] : 0.0; /* candidate->getRedshift() */
 	double step = p->maxStep;
 	double newStep;
 	orc_y yIn, yOut, yErr;
@@ -816,9 +868,9 @@ static inline void orc_process(const orc_field *f, const orc_propagator *p, orc_
 		if (p->minStep == p->maxStep) {
 			if (p->type == ORC_PROP_BP) {
 				double m = E / (C_LIGHT * C_LIGHT);
-				yOut = orc_bp_dy(f, yIn.x, yIn.u, step, q, m, libm);
+				yOut = orc_bp_dy(f, yIn.x, yIn.u, step, q, m, zr, libm);
 			} else {
-				orc_ck_trystep(f, &yIn, &yOut, &yErr, step / C_LIGHT, q, E, libm);
+				orc_ck_trystep(f, &yIn, &yOut, &yErr, step / C_LIGHT, q, E, zr, libm);
 			}
 		} else {
 			step = orc_clip(s->next_step[i], p->minStep, p->maxStep);
@@ -828,15 +880,15 @@ static inline void orc_process(const orc_field *f, const orc_propagator *p, orc_
 			for (;;) {
 				if (p->type == ORC_PROP_BP) {
 					/* tryStep + errorEstimation, src/module/PropagationBP.cpp:8-16,152-159 */
-					yOut = orc_bp_dy(f, yIn.x, yIn.u, step, q, m, libm);
-					orc_y half = orc_bp_dy(f, yIn.x, yIn.u, step / 2, q, m, libm);
-					orc_y cmp = orc_bp_dy(f, half.x, half.u, step / 2, q, m, libm);
+					yOut = orc_bp_dy(f, yIn.x, yIn.u, step, q, m, zr, libm);
+					orc_y half = orc_bp_dy(f, yIn.x, yIn.u, step / 2, q, m, zr, libm);
+					orc_y cmp = orc_bp_dy(f, half.x, half.u, step / 2, q, m, zr, libm);
 					double diff[3] = {yOut.x[0] - cmp.x[0], yOut.x[1] - cmp.x[1], yOut.x[2] - cmp.x[2]};
 					double S = orc_norm(diff) / (step * (1 - 1 / 4.));
 					/* Y(double) sets all three u components to S (PropagationBP.h:40-42) */
 					r = sqrt((S * S + S * S) + S * S) / p->tolerance;
 				} else {
-					orc_ck_trystep(f, &yIn, &yOut, &yErr, step / C_LIGHT, q, E, libm);
+					orc_ck_trystep(f, &yIn, &yOut, &yErr, step / C_LIGHT, q, E, zr, libm);
 					r = orc_norm(yErr.u) / p->tolerance;
 				}
 				if (r > 1) {
@@ -1330,7 +1382,8 @@ static void sde_randcone(orc_sde_rng *r, const double mean[3], double o[3]) {
 }
 
 /* DiffusionSDE::tryStep (src/module/DiffusionSDE.cpp:233-254): Cash-Karp along the unit field direction */
-static void sde_trystep(const orc_field *f, const double in[3], double out[3], double err[3], double t, int libm) {
+static void sde_trystep(const orc_field *f, const double in[3], double out[3], double err[3], double t, double z,
+                        int libm) {
 	double k[6][3];
 	for (int a = 0; a < 3; a++)
 		out[a] = in[a];
@@ -1340,7 +1393,7 @@ static void sde_trystep(const orc_field *f, const double in[3], double out[3], d
 			for (int a = 0; a < 3; a++)
 				y[a] += (k[j][a] * CK_A[i * 6 + j]) * t;
 		double B[3], ub[3];
-		orc_field_eval1(f, y, B, libm);
+		orc_field_eval2(f, y, z, B, libm);
 		sde_unit(B, ub);
 		const double bi = CK_B[i], ei = CK_B[i] - CK_BS[i];
 		for (int a = 0; a < 3; a++) {
@@ -1354,6 +1407,7 @@ static void sde_trystep(const orc_field *f, const double in[3], double out[3], d
 /* DiffusionSDE::process (src/module/DiffusionSDE.cpp:48-209) */
 static void orc_sde_process(const orc_field *f, const orc_sde *p, orc_state *s, size_t i, orc_sde_rng *r) {
 	const int k = r->kind, libm = k == ORC_SDE_MT ? ORC_LIBM_GLIBC : ORC_LIBM_CR;
+	const double zr = s->redshift ? s->redshift[i] : 0.0; /* candidate->getRedshift() (DiffusionSDE.cpp:77) */
 	double *pos = s->pos + 3 * i, *dir = s->dir + 3 * i;
 	if (s->prev_pos)
 		for (int a = 0; a < 3; a++) {
@@ -1400,7 +1454,7 @@ static void orc_sde_process(const orc_field *f, const orc_sde *p, orc_state *s, 
 	double rr;
 	do {
 		err[0] = err[1] = err[2] = 0;
-		sde_trystep(f, in, out, err, propTime, libm);
+		sde_trystep(f, in, out, err, propTime, zr, libm);
 		rr = sde_r(err) / p->tolerance;
 		propTime *= 0.5;
 		counter += 1;
@@ -1410,7 +1464,7 @@ static void orc_sde_process(const orc_field *f, const orc_sde *p, orc_state *s, 
 	double start[3] = {in[0], in[1], in[2]};
 	err[0] = err[1] = err[2] = 0;
 	for (double j = 0; j < stepNumber; j++) {
-		sde_trystep(f, start, out, err, allowedTime, libm);
+		sde_trystep(f, start, out, err, allowedTime, zr, libm);
 		start[0] = out[0];
 		start[1] = out[1];
 		start[2] = out[2];

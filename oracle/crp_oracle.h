@@ -27,12 +27,12 @@ extern "C" {
 
 enum { ORC_LIBM_GLIBC = 0, ORC_LIBM_CR = 1 };
 enum { ORC_FIELD_NONE = 0, ORC_FIELD_UNIFORM = 1, ORC_FIELD_GRID = 2, ORC_FIELD_PLANEWAVE = 3, ORC_FIELD_MODULATED = 4,
-       ORC_FIELD_JF12 = 5 };
+       ORC_FIELD_JF12 = 5, ORC_FIELD_PERIODIC = 6, ORC_FIELD_LIST = 7, ORC_FIELD_EVOLUTION = 8 };
 enum { ORC_PROP_BP = 0, ORC_PROP_CK = 1 };
 /* modules after the propagator, for orc_propagator.order */
 enum { ORC_MOD_BOX = 1, ORC_MOD_SPHERE = 2, ORC_MOD_MAXLEN = 3, ORC_MOD_OBSERVER = 4 };
 
-typedef struct {
+typedef struct orc_field_s {
 	int type;
 	double B0[3]; /* uniform */
 	/* Grid3f: float32 values, AoS (x,y,z) per point, index (ix*Ny + iy)*Nz + iz */
@@ -51,6 +51,14 @@ typedef struct {
 	/* JF12Field: the model's constants; the turbulent Grid3f is the grid above, the striated Grid1f
 	 * the modulation grid's slots */
 	const crp_jf12_params *jf12;
+	/* decorators (src/magneticField/MagneticField.cpp:41-79): PeriodicMagneticField(children[0], extends,
+	 * origin, reflective) and MagneticFieldEvolution(children[0], m) wrap one field, MagneticFieldList
+	 * sums children[0..nchildren) in order */
+	const struct orc_field_s *const *children;
+	int nchildren;
+	double extends[3], porigin[3];
+	int preflective;
+	double evo_m;
 } orc_field;
 
 /* JF12Field / PlanckJF12bField constructor values (src/magneticField/JF12Field.cpp:8-79,348-363) */
@@ -167,8 +175,10 @@ void orc_noise_grid(uint32_t seed, int nx, int ny, int nz, double amp, float *ou
 void orc_sample_particles(uint32_t seed, size_t n, double emin, double emax, const double *weights, size_t nw,
                           int32_t *species, double *energy, double *dir);
 
-/* fields */
+/* fields: MagneticField::getField(position) at m positions (AoS) */
 void orc_field_eval(const orc_field *f, const double *pos, size_t m, double *B, int libm);
+/* MagneticField::getField(position, z), the call the propagators make, with z[i] per position */
+void orc_field_eval_z(const orc_field *f, const double *pos, const double *z, size_t m, double *B, int libm);
 void orc_planewave_init(double brms, double lmin, double lmax, double lbendover, double sindex, double qindex,
                         int simple_spectrum, int nm, int seed, double *k, double *Ak, double *xi, double *kappa,
                         double *beta);

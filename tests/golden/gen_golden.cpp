@@ -193,6 +193,9 @@ static void save_particles(const std::string &dir, const std::vector<Particle> &
 }
 
 // ---------------------------------------------------------------- trajectories
+// Per-particle redshifts for the next run_traj (candidate->setRedshift), or none (z = 0)
+static const std::vector<double> *g_redshift = nullptr;
+
 // Runs `prop` (then the optional length limit) on each particle, the way
 // ModuleList::run(Candidate*) loops (src/ModuleList.cpp.in:64-67), for at most
 // maxSteps process() calls, recording the state after the listed step counts.
@@ -209,6 +212,8 @@ static void run_traj(const std::string &name, const Module &prop, const MaximumT
 	for (size_t i = 0; i < N; i++) {
 		ParticleState st(ps[i].id, ps[i].E, ps[i].pos, ps[i].dir);
 		ref_ptr<Candidate> c = new Candidate(st);
+		if (g_redshift)
+			c->setRedshift((*g_redshift)[i]);
 		size_t si = 0, step = 0;
 		while (step < maxSteps && c->isActive()) {
 			prop.process(c);
@@ -250,6 +255,8 @@ static void run_traj(const std::string &name, const Module &prop, const MaximumT
 	save_i32(dir, "active", act, {S, N});
 	save_i32(dir, "nsteps", nsteps, {N});
 	save_f64(dir, "final_previous_pos", prevpos, {N, 3});
+	if (g_redshift)
+		save_f64(dir, "redshift", *g_redshift, {N});
 	save_meta(dir, meta);
 	printf("  %-28s N=%zu steps<=%zu\n", name.c_str(), N, maxSteps);
 }
@@ -796,9 +803,129 @@ static void gen_gridtools() {
 	                 fmt(33 * kpc) + "}}");
 }
 
+// ---------------------------------------------------------------- field decorators
+// PeriodicMagneticField (periodic and reflective), MagneticFieldList and MagneticFieldEvolution
+// (src/magneticField/MagneticField.cpp:41-79) around the 32^3 noise grid of the trajectory cases:
+// field samples at 4096 positions (getField(position), and getField(position, z) with a redshift
+// per sample), and trajectories through them.
+static void gen_decorators() {
+	const int n = 32;
+	const double sp = 50 * kpc, L = n * sp;
+	ref_ptr<Grid3f> g32 = make_noise_grid(n, sp, Vector3d(0.), 2024, nG);
+	ref_ptr<MagneticField> fgrid = new MagneticFieldGrid(g32);
+	ref_ptr<MagneticField> funi = new UniformMagneticField(Vector3d(0.3 * nG, -0.2 * nG, 0.5 * nG));
+	SimpleSpectrum spec2(3 * nG, 10 * kpc, 2 * Mpc, 11. / 3);
+	ref_ptr<MagneticField> fpw = new PlaneWaveTurbulence(spec2, 2, 5); // the planewave_simple2 table
+	const Vector3d ext(1.3 * L, 0.7 * L, L), org(-0.2 * L, 0.1 * L, 0.);
+	ref_ptr<MagneticField> fper = new PeriodicMagneticField(fgrid, ext, org, false);
+	ref_ptr<MagneticField> fref = new PeriodicMagneticField(fgrid, ext, org, true);
+	ref_ptr<MagneticFieldList> flist = new MagneticFieldList();
+	flist->addField(fgrid);
+	flist->addField(funi);
+	flist->addField(fpw);
+	ref_ptr<MagneticField> fevo = new MagneticFieldEvolution(fgrid, 1.7);
+	// nested: a list of a reflective periodic grid, an evolution (its one-argument getField is the base
+	// class's zero) and a periodic list
+	ref_ptr<MagneticFieldList> inner = new MagneticFieldList();
+	inner->addField(fgrid);
+	inner->addField(funi);
+	ref_ptr<MagneticFieldList> nested = new MagneticFieldList();
+	nested->addField(fref);
+	nested->addField(new MagneticFieldEvolution(fgrid, 2.0));
+	nested->addField(new PeriodicMagneticField(inner, Vector3d(0.8 * L, 1.1 * L, 0.6 * L)));
+	ref_ptr<MagneticField> fevolist = new MagneticFieldEvolution(inner, -0.5);
+	// the propagators call getField(position, z) through MagneticField (the list's own getField hides it)
+	ref_ptr<MagneticField> fnested = nested;
+
+	Random rng(99);
+	const size_t M = 4096;
+	std::vector<double> pos(3 * M), z(M);
+	std::vector<double> Bper(3 * M), Bref(3 * M), Blist(3 * M), Bevo(3 * M), Bevo1(3 * M), Bnest(3 * M), Bevol(3 * M);
+	for (size_t i = 0; i < M; i++) {
+		Vector3d p(rng.randUniform(-2 * L, 3 * L), rng.randUniform(-2 * L, 3 * L), rng.randUniform(-2 * L, 3 * L));
+		if (i < 64) // on period boundaries
+			p = org + Vector3d(ext.x * (double)((int)i % 4 - 1), ext.y * (double)((int)i / 4 % 4 - 1), ext.z * (double)((int)i / 16 - 1));
+		z[i] = rng.randUniform(0, 3);
+		pos[3 * i] = p.x; pos[3 * i + 1] = p.y; pos[3 * i + 2] = p.z;
+		const Vector3d b[7] = {fper->getField(p), fref->getField(p), flist->getField(p), fevo->getField(p, z[i]),
+		                       fevo->getField(p), fnested->getField(p, z[i]), fevolist->getField(p, z[i])};
+		std::vector<double> *out[7] = {&Bper, &Bref, &Blist, &Bevo, &Bevo1, &Bnest, &Bevol};
+		for (int k = 0; k < 7; k++) {
+			(*out[k])[3 * i] = b[k].x;
+			(*out[k])[3 * i + 1] = b[k].y;
+			(*out[k])[3 * i + 2] = b[k].z;
+		}
+	}
+	{
+		std::string d = casedir("decorators32");
+		save_f64(d, "pos", pos, {M, 3});
+		save_f64(d, "z", z, {M});
+		save_f64(d, "B_periodic", Bper, {M, 3});
+		save_f64(d, "B_reflective", Bref, {M, 3});
+		save_f64(d, "B_list", Blist, {M, 3});
+		save_f64(d, "B_evolution", Bevo, {M, 3});
+		save_f64(d, "B_evolution_noz", Bevo1, {M, 3});
+		save_f64(d, "B_nested", Bnest, {M, 3});
+		save_f64(d, "B_evolution_list", Bevol, {M, 3});
+		save_meta(d, "{\"grid\": \"grid32\", \"uniform\": [" + fmt(0.3 * nG) + ", " + fmt(-0.2 * nG) + ", " +
+		                 fmt(0.5 * nG) + "], \"planewave\": \"planewave_simple2\", \"extends\": [" + fmt(ext.x) +
+		                 ", " + fmt(ext.y) + ", " + fmt(ext.z) + "], \"origin\": [" + fmt(org.x) + ", " + fmt(org.y) +
+		                 ", " + fmt(org.z) + "], \"evolution_m\": 1.7, \"nested_evolution_m\": 2.0, "
+		                 "\"nested_extends\": [" + fmt(0.8 * L) + ", " + fmt(1.1 * L) + ", " + fmt(0.6 * L) +
+		                 "], \"evolution_list_m\": -0.5}");
+		printf("  %-28s M=%zu\n", "decorators32", M);
+	}
+
+	const Vector3d centre(L / 2, L / 2, L / 2);
+	std::vector<int> proton = {nucleusId(60, 1)};
+	std::vector<int> species = {nucleusId(60, 1), nucleusId(60, 2), nucleusId(60, 7), nucleusId(60, 14), nucleusId(60, 26)};
+	std::vector<double> one = {1.0}, wts = {0.5, 0.2, 0.15, 0.1, 0.05};
+	{
+		ref_ptr<MagneticFieldList> gl = new MagneticFieldList();
+		gl->addField(fgrid);
+		gl->addField(new UniformMagneticField(Vector3d(1 * nG, 0, 0)));
+		std::vector<Particle> ps = make_particles(128, 60, 1 * EeV, 100 * EeV, centre, proton, one);
+		PropagationBP bp(gl, 50 * kpc);
+		run_traj("bp_fixed_list32", bp, nullptr, ps, 400, {1, 10, 100, 400},
+		         "{\"propagator\": \"BP\", \"fixedStep\": " + fmt(50 * kpc) +
+		             ", \"field\": \"list(grid32, uniform [1 nG, 0, 0])\", \"particle_seed\": 60}");
+	}
+	{
+		ref_ptr<MagneticField> pr = new PeriodicMagneticField(fgrid, Vector3d(1.2 * L, 0.9 * L, 1.1 * L),
+		                                                      Vector3d(-0.1 * L, 0.05 * L, 0.), true);
+		std::vector<Particle> ps = make_particles(128, 61, 1 * EeV, 300 * EeV, centre, species, wts);
+		PropagationCK ck(pr, 1e-4, 1 * kpc, 1 * Mpc);
+		MaximumTrajectoryLength ml(6 * Mpc);
+		run_traj("ck_reflective32", ck, &ml, ps, 3000, {1, 10, 100, 1000, 3000},
+		         "{\"propagator\": \"CK\", \"tolerance\": 1e-4, \"minStep\": " + fmt(kpc) + ", \"maxStep\": " +
+		             fmt(Mpc) + ", \"maxTrajectoryLength\": " + fmt(6 * Mpc) +
+		             ", \"field\": \"periodic(grid32) reflective\", \"extends\": [" + fmt(1.2 * L) + ", " +
+		             fmt(0.9 * L) + ", " + fmt(1.1 * L) + "], \"origin\": [" + fmt(-0.1 * L) + ", " + fmt(0.05 * L) +
+		             ", 0], \"particle_seed\": 61}");
+	}
+	{
+		ref_ptr<MagneticField> ev = new MagneticFieldEvolution(fgrid, 1.5);
+		std::vector<Particle> ps = make_particles(128, 62, 1 * EeV, 100 * EeV, centre, proton, one);
+		Random rz(63);
+		std::vector<double> zs(ps.size());
+		for (size_t i = 0; i < zs.size(); i++)
+			zs[i] = rz.randUniform(0, 2);
+		g_redshift = &zs;
+		PropagationBP bp(ev, 50 * kpc);
+		run_traj("bp_fixed_evolution32", bp, nullptr, ps, 400, {1, 10, 100, 400},
+		         "{\"propagator\": \"BP\", \"fixedStep\": " + fmt(50 * kpc) +
+		             ", \"field\": \"evolution(grid32, m=1.5)\", \"evolution_m\": 1.5, \"particle_seed\": 62}");
+		g_redshift = nullptr;
+	}
+}
+
 int main(int argc, char **argv) {
 	g_root = argc > 1 ? argv[1] : "tests/golden";
 	mkdir(g_root.c_str(), 0755);
+	if (argc > 2 && std::string(argv[2]) == "decorators") { // only the field-decorator cases
+		gen_decorators();
+		return 0;
+	}
 	printf("gen_golden: glibc %s, g++ %s\n", gnu_get_libc_version(), __VERSION__);
 	{
 		FILE *f = fopen((g_root + "/PROVENANCE.txt").c_str(), "w");
@@ -813,6 +940,7 @@ int main(int argc, char **argv) {
 	gen_sde();
 	gen_jf12();
 	gen_gridtools();
+	gen_decorators();
 
 	// Plane waves: the C3 mode table (Simple spectrum), the reference test's
 	// general spectrum (test/testPropagation.cpp:492), and a 2-mode table.

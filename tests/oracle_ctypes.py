@@ -15,6 +15,7 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 LIBM_GLIBC = 0
 LIBM_CR = 1
 FIELD_NONE, FIELD_UNIFORM, FIELD_GRID, FIELD_PLANEWAVE, FIELD_MODULATED, FIELD_JF12 = 0, 1, 2, 3, 4, 5
+FIELD_PERIODIC, FIELD_LIST, FIELD_EVOLUTION = 6, 7, 8
 PROP_BP, PROP_CK = 0, 1
 MOD_BOX, MOD_SPHERE, MOD_MAXLEN, MOD_OBSERVER = 1, 2, 3, 4  # orc_propagator.order
 
@@ -38,7 +39,10 @@ _ip = _P(C.c_int32)
 
 
 class Field(C.Structure):
-    _fields_ = [
+    pass
+
+
+Field._fields_ = [
         ("type", C.c_int),
         ("B0", C.c_double * 3),
         ("grid", _fp),
@@ -51,7 +55,10 @@ class Field(C.Structure):
         ("mod", _fp), ("mnx", C.c_int), ("mny", C.c_int), ("mnz", C.c_int),
         ("morigin", C.c_double * 3), ("mspacing", C.c_double * 3), ("mreflective", C.c_int), ("mclip", C.c_int),
         ("jf12", C.c_void_p),
-    ]
+        ("children", C.POINTER(C.POINTER(Field))), ("nchildren", C.c_int),
+        ("extends", C.c_double * 3), ("porigin", C.c_double * 3), ("preflective", C.c_int),
+        ("evo_m", C.c_double),
+]
 
 
 class JF12Params(C.Structure):
@@ -109,6 +116,7 @@ def lib():
         L.orc_propagate.argtypes = [_P(Field), _P(Propagator), _P(State), C.c_size_t, C.c_longlong, C.c_int,
                                     C.c_int, _P(C.c_longlong)]
         L.orc_field_eval.argtypes = [_P(Field), _dp, C.c_size_t, _dp, C.c_int]
+        L.orc_field_eval_z.argtypes = [_P(Field), _dp, _dp, C.c_size_t, _dp, C.c_int]
         L.orc_planewave_init.argtypes = [C.c_double] * 6 + [C.c_int, C.c_int, C.c_int] + [_dp] * 5
         L.orc_cos_batch.argtypes = [_dp, C.c_size_t, _dp, C.c_int]
         L.orc_pow_batch.argtypes = [_dp, C.c_double, C.c_size_t, _dp, C.c_int]
@@ -236,10 +244,48 @@ class OracleField:
         f.s.k, f.s.Ak, f.s.xi, f.s.kappa, f.s.beta = [ptr(a) for a in (k, Ak, xi, kappa, beta)]
         return f
 
-    def eval(self, pos, libm=LIBM_GLIBC):
+    @staticmethod
+    def _wrap(kind, children):
+        f = OracleField()
+        f._keep += list(children)
+        arr = (C.POINTER(Field) * max(1, len(children)))(*[C.pointer(c.s) for c in children])
+        f._keep.append(arr)
+        f.s.type = kind
+        f.s.children = C.cast(arr, C.POINTER(C.POINTER(Field)))
+        f.s.nchildren = len(children)
+        return f
+
+    @staticmethod
+    def periodic(child, extends, origin=(0, 0, 0), reflective=False):
+        """PeriodicMagneticField(field, extends, origin, reflective)"""
+        f = OracleField._wrap(FIELD_PERIODIC, [child])
+        for a in range(3):
+            f.s.extends[a] = extends[a]
+            f.s.porigin[a] = origin[a]
+        f.s.preflective = int(reflective)
+        return f
+
+    @staticmethod
+    def field_list(children):
+        """MagneticFieldList with addField(children[0]), addField(children[1]), ..."""
+        return OracleField._wrap(FIELD_LIST, children)
+
+    @staticmethod
+    def evolution(child, m):
+        """MagneticFieldEvolution(field, m)"""
+        f = OracleField._wrap(FIELD_EVOLUTION, [child])
+        f.s.evo_m = m
+        return f
+
+    def eval(self, pos, libm=LIBM_GLIBC, z=None):
+        """getField(position) at each position, or getField(position, z[i]) when z is given."""
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         B = np.zeros_like(pos)
-        lib().orc_field_eval(C.byref(self.s), ptr(pos), pos.shape[0], ptr(B), libm)
+        if z is None:
+            lib().orc_field_eval(C.byref(self.s), ptr(pos), pos.shape[0], ptr(B), libm)
+        else:
+            zz = np.ascontiguousarray(np.broadcast_to(np.asarray(z, dtype=np.float64), (pos.shape[0],)))
+            lib().orc_field_eval_z(C.byref(self.s), ptr(pos), ptr(zz), pos.shape[0], ptr(B), libm)
         return B
 
 
@@ -421,6 +467,15 @@ def field_for_case(case):
         gm = g["meta"]["grid"]
         return OracleField.modulated(g["grid"], gm["origin"], gm["spacing"], case["mod"], meta["mod_origin"],
                                      meta["mod_spacing"], mreflective=meta["mod_reflective"])
+    if f in ("list(grid32, uniform [1 nG, 0, 0])", "periodic(grid32) reflective", "evolution(grid32, m=1.5)"):
+        g = load_case("grid32")
+        gm = g["meta"]["grid"]
+        grid = OracleField.grid(g["grid"], gm["origin"], gm["spacing"])
+        if f.startswith("list"):
+            return OracleField.field_list([grid, OracleField.uniform([1 * nG, 0, 0])])
+        if f.startswith("periodic"):
+            return OracleField.periodic(grid, meta["extends"], meta["origin"], reflective=True)
+        return OracleField.evolution(grid, meta["evolution_m"])
     if isinstance(f, dict) and f["type"] == "planewave":
         return OracleField.planewave(planewave_tables(f))
     if isinstance(f, dict) and f["type"] == "uniform":
@@ -430,6 +485,31 @@ def field_for_case(case):
         vals = noise_grid(gm["seed"], gm["n"], gm["amp"])
         return OracleField.grid(vals, gm["origin"], gm["spacing"])
     raise ValueError("unknown field in case")
+
+
+def decorator_fields():
+    """The fields of the decorators32 samples (gen_golden.cpp gen_decorators), by fixture name."""
+    c = load_case("decorators32")
+    m = c["meta"]
+    g = load_case("grid32")
+    gm = g["meta"]["grid"]
+    grid = OracleField.grid(g["grid"], gm["origin"], gm["spacing"])
+    uni = OracleField.uniform(m["uniform"])
+    pw = OracleField.planewave(planewave_tables(load_case("planewave_simple2")["meta"]))
+    inner = OracleField.field_list([grid, uni])
+    L = gm["n"] * gm["spacing"]
+    fields = {
+        "B_periodic": OracleField.periodic(grid, m["extends"], m["origin"]),
+        "B_reflective": OracleField.periodic(grid, m["extends"], m["origin"], reflective=True),
+        "B_list": OracleField.field_list([grid, uni, pw]),
+        "B_evolution": OracleField.evolution(grid, m["evolution_m"]),
+        "B_nested": OracleField.field_list([OracleField.periodic(grid, m["extends"], m["origin"], reflective=True),
+                                            OracleField.evolution(grid, m["nested_evolution_m"]),
+                                            OracleField.periodic(inner, m["nested_extends"])]),
+        "B_evolution_list": OracleField.evolution(inner, m["evolution_list_m"]),
+    }
+    assert L > 0
+    return c, fields
 
 
 def propagator_args(meta):

@@ -15,9 +15,11 @@ import pytest
 import oracle_ctypes as O
 
 TRAJ_CASES = ["bp_fixed_grid32", "bp_adaptive_grid32", "ck_grid32_maxlen", "ck_fixed_grid32",
-              "bp_fixed_planewave32", "ck_planewave32", "bp_fixed_uniform", "ck_uniform", "bp_fixed_grid64_c1", "bp_fixed_grid32_reflective", "ck_grid32_clip"]
+              "bp_fixed_planewave32", "ck_planewave32", "bp_fixed_uniform", "ck_uniform", "bp_fixed_grid64_c1",
+              "bp_fixed_grid32_reflective", "ck_grid32_clip", "bp_fixed_list32", "ck_reflective32",
+              "bp_fixed_evolution32"]
 NO_TRANSCENDENTAL = {"bp_fixed_grid32", "ck_fixed_grid32", "bp_fixed_uniform", "bp_fixed_grid64_c1",
-                     "bp_fixed_grid32_reflective"}
+                     "bp_fixed_grid32_reflective", "bp_fixed_list32"}
 
 
 def bits_equal(a, b):
@@ -30,6 +32,8 @@ def run_case(name, libm):
     m = c["meta"]
     f = O.field_for_case(c)
     st = O.OracleState(c["init_pos"], c["init_dir"], c["energy"], c["charge"])
+    if "redshift" in c:
+        st.redshift[:] = c["redshift"]
     pa = O.propagator_args(m)
     done = 0
     snaps = []
@@ -275,6 +279,8 @@ def test_oracle_observer_bit_exact_vs_reference(name):
     m = c["meta"]
     f = O.field_for_case(c)
     st = O.OracleState(c["init_pos"], c["init_dir"], c["energy"], c["charge"])
+    if "redshift" in c:
+        st.redshift[:] = c["redshift"]
     pa = O.propagator_args(m)
     O.propagate(f, steps=int(c["nsteps"].max()), max_len=m.get("maxTrajectoryLength", 0.0), libm=O.LIBM_GLIBC,
                 state=st, observer_times=c["times"], observer_deactivate=m["observer"]["deactivate"],
@@ -302,6 +308,8 @@ def test_oracle_periodic_box_bit_exact_vs_reference(name):
     m = c["meta"]
     f = O.field_for_case(c)
     st = O.OracleState(c["init_pos"], c["init_dir"], c["energy"], c["charge"])
+    if "redshift" in c:
+        st.redshift[:] = c["redshift"]
     pa = O.propagator_args(m)
     box = (m["box"]["origin"], m["box"]["size"])
     done = 0
@@ -383,3 +391,38 @@ def test_oracle_modulated_trajectories_bit_exact_vs_reference(name):
                     libm=O.LIBM_GLIBC, state=st, **O.propagator_args(m))
         done = int(s)
         assert bits_equal(st.pos, c["pos"][si]) and bits_equal(st.dir, c["dir"][si]), (name, s)
+
+
+@pytest.mark.parametrize("name", ["B_periodic", "B_reflective", "B_list", "B_evolution", "B_nested",
+                                  "B_evolution_list"])
+def test_decorator_fields_bit_exact(name):
+    """PeriodicMagneticField (periodic and reflective), MagneticFieldList, MagneticFieldEvolution and nestings
+    (src/magneticField/MagneticField.cpp:41-79) against the reference's getField(position, z) samples."""
+    c, fields = O.decorator_fields()
+    assert bits_equal(fields[name].eval(c["pos"], z=c["z"]), c[name])
+
+
+def test_decorator_one_argument_calls():
+    """getField(position) of MagneticFieldEvolution is the base class's zero (it only overrides the two-argument
+    call); without a redshift every other decorator equals its two-argument result."""
+    c, fields = O.decorator_fields()
+    assert bits_equal(fields["B_evolution"].eval(c["pos"]), c["B_evolution_noz"])
+    assert not np.any(c["B_evolution_noz"])
+    for name in ("B_periodic", "B_reflective", "B_list"):
+        assert bits_equal(fields[name].eval(c["pos"]), c[name])
+
+
+def test_periodic_decorator_closed_forms():
+    """The periodic map repeats the base cell exactly; the reflective map mirrors odd cells (p -> extends - p)."""
+    g = np.random.default_rng(3).normal(size=(5, 4, 6, 3)).astype(np.float32)
+    base = O.OracleField.grid(g, [0, 0, 0], [1.0, 1.0, 1.0], clip=True)
+    ext = np.array([3.0, 2.5, 4.0])
+    per = O.OracleField.periodic(base, ext)
+    ref = O.OracleField.periodic(base, ext, reflective=True)
+    p = np.random.default_rng(4).uniform(0, 1, size=(500, 3)) * ext
+    shifts = np.array([[1, 0, 0], [0, -2, 0], [0, 0, 3], [-1, 1, 2]], dtype=float)
+    for s in shifts:
+        assert np.array_equal(per.eval(p + s * ext), base.eval(p))
+    mirrored = p.copy()
+    mirrored[:, 0] = 2 * ext[0] - p[:, 0]  # lies in the odd cell n_x = 1 at extends - p_x
+    assert np.allclose(ref.eval(mirrored), base.eval(p), rtol=0, atol=1e-6)
