@@ -101,6 +101,10 @@ def lib():
         L.crp_field_list_add.argtypes = [_vp, _vp]
         L.crp_field_create_evolution.argtypes = [_vp, C.c_double, C.POINTER(_vp)]
         L.crp_field_eval_z.argtypes = [_vp, _dp, _dp, C.c_size_t, _dp]
+        L.crp_bp_dy.argtypes = [_vp, C.c_size_t] + [_dp] * 8
+        L.crp_bp_try_step.argtypes = [_vp, C.c_size_t] + [_dp] * 10
+        L.crp_ck_dydt.argtypes = [_vp, C.c_size_t] + [_dp] * 7
+        L.crp_ck_try_step.argtypes = [_vp, C.c_size_t] + [_dp] * 10
         for nm in ("crp_propagator_create_bp", "crp_propagator_create_ck"):
             getattr(L, nm).argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.POINTER(_vp)]
         L.crp_propagator_create_sde.argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(_vp)]
@@ -384,6 +388,25 @@ class Propagator:
             self.set_sde(alpha, scale, epsilon, seed)
             if advection is not None:
                 self.set_sde_advection(advection)
+
+    def step(self, op, x, u, h, z, q, m):
+        """The reference's single-step functions on n phase points: op "dY" / "tryStep" (BP; m = mass,
+        h = step) or "dYdt" / "tryStep" (CK; m = energy, h = t = step / c).  Returns (out_x, out_u) or
+        (out_x, out_u, err_x, err_u) as (n, 3) arrays."""
+        x, u = _f64(x, (-1, 3)), _f64(u, (-1, 3))
+        n = x.shape[0]
+        col = [_f64(np.broadcast_to(np.asarray(a, dtype=np.float64), (n,))) for a in (h, z, q, m)]
+        outs = [np.zeros((n, 3)) for _ in range(4)]
+        L = lib()
+        if self.kind == "BP" and op == "dY":
+            check(L.crp_bp_dy(self.handle, n, _p(x), _p(u), *[_p(a) for a in col], _p(outs[0]), _p(outs[1])))
+            return outs[0], outs[1]
+        if self.kind == "CK" and op == "dYdt":
+            check(L.crp_ck_dydt(self.handle, n, _p(x), _p(u), *[_p(a) for a in col[1:]], _p(outs[0]), _p(outs[1])))
+            return outs[0], outs[1]
+        fn = L.crp_bp_try_step if self.kind == "BP" else L.crp_ck_try_step
+        check(fn(self.handle, n, _p(x), _p(u), *[_p(a) for a in col], *[_p(o) for o in outs]))
+        return tuple(outs)
 
     def set_sde(self, alpha, scale, epsilon, seed):
         check(lib().crp_propagator_set_sde(self.handle, alpha, scale, epsilon, int(seed) & 0xFFFFFFFF))

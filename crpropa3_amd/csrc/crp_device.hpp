@@ -682,10 +682,9 @@ __device__ __forceinline__ void axpy(Y &yn, const Y &k, double a, double h) {
 }
 
 // PropagationCK::tryStep (src/module/PropagationCK.cpp:28-49).  Returns the
-// new phase point in `out` and |err.u| in *errU.
+// new phase point in `out` and the error estimate in `err`.
 template <class F>
-__device__ __forceinline__ void ck_trystep(const F &field, const Y &y, double h, double qcE, Y &out, double &errU) {
-	Y err;
+__device__ __forceinline__ void ck_trystep_err(const F &field, const Y &y, double h, double qcE, Y &out, Y &err) {
 	err.x = v3(0.0, 0.0, 0.0);
 	err.u = v3(0.0, 0.0, 0.0);
 	out = y;
@@ -730,6 +729,13 @@ __device__ __forceinline__ void ck_trystep(const F &field, const Y &y, double h,
 	k5 = ck_dydt(field, yn, qcE);
 	axpy(out, k5, CK::b5, h);
 	axpy(err, k5, e5, h);
+}
+
+// the same with only |err.u|, the quantity the step control reads
+template <class F>
+__device__ __forceinline__ void ck_trystep(const F &field, const Y &y, double h, double qcE, Y &out, double &errU) {
+	Y err;
+	ck_trystep_err(field, y, h, qcE, out, err);
 	errU = norm(err.u);
 }
 

@@ -299,6 +299,50 @@ __global__ void k_field_eval(F field, const double *__restrict__ pos, double *__
 	B[3 * i + 2] = b.z;
 }
 
+// The reference's public per-step functions on n phase points (PropagationBP::dY / tryStep,
+// PropagationCK::dYdt / tryStep), with the same device code as the propagation kernels.
+enum StepOp { SO_BP_DY = 0, SO_BP_TRY = 1, SO_CK_DYDT = 2, SO_CK_TRY = 3 };
+struct StepArgs {
+	const double *x, *u, *h, *z, *q, *m; // x, u AoS [3n]; m: BP mass, CK energy
+	double *ox, *ou, *ex, *eu;            // AoS [3n]
+	size_t n;
+};
+
+template <class F, int OP>
+__global__ void k_step_api(F field0, StepArgs A) {
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= A.n)
+		return;
+	F field = field0;
+	bind_redshift(field, A.z[i]);
+	const V3 x = v3(A.x[3 * i], A.x[3 * i + 1], A.x[3 * i + 2]), u = v3(A.u[3 * i], A.u[3 * i + 1], A.u[3 * i + 2]);
+	const double q = A.q[i], m = A.m[i];
+	Y o, e;
+	if (OP == SO_BP_DY || OP == SO_BP_TRY) {
+		const double h = A.h[i], invm = 1 / m;
+		o = bp_dy(field, x, u, h, q, m, invm);
+		if (OP == SO_BP_TRY) { // error = Y(errorEstimation(out.x, dY(dY(h/2), h/2).x, h))
+			const Y half = bp_dy(field, x, u, h / 2, q, m, invm);
+			const Y cmp = bp_dy(field, half.x, half.u, h / 2, q, m, invm);
+			const double S = norm(v3(o.x.x - cmp.x.x, o.x.y - cmp.x.y, o.x.z - cmp.x.z)) / (h * (1 - 1 / 4.));
+			e.x = v3(S, S, S);
+			e.u = e.x;
+		}
+	} else if (OP == SO_CK_DYDT) {
+		o = ck_dydt(field, Y{x, u}, (q * kCLight) / m);
+	} else {
+		ck_trystep_err(field, Y{x, u}, A.h[i], (q * kCLight) / m, o, e);
+	}
+	const V3 w[4] = {o.x, o.u, e.x, e.u};
+	double *dst[4] = {A.ox, A.ou, A.ex, A.eu};
+	const int nout = (OP == SO_BP_TRY || OP == SO_CK_TRY) ? 4 : 2;
+	for (int k = 0; k < nout; k++) {
+		dst[k][3 * i] = w[k].x;
+		dst[k][3 * i + 1] = w[k].y;
+		dst[k][3 * i + 2] = w[k].z;
+	}
+}
+
 // getField(position, z) per position: the field bound to each position's redshift
 template <class F>
 __global__ void k_field_eval_z(F field0, const double *__restrict__ pos, const double *__restrict__ z,
@@ -1326,6 +1370,107 @@ int crp_field_eval_z(const crp_field_t *f, const double *pos, const double *z, s
 	if (e != hipSuccess)
 		return fail(CRP_E_HIP, std::string("crp_field_eval_z: ") + hipGetErrorString(e));
 	return CRP_OK;
+}
+
+// ---------------------------------------------------------------- per-step functions (C ABI)
+} // extern "C"
+
+// Calls fn(field) with the FP64 device field of f, as crp_propagate dispatches it.
+template <class Fn> static int with_fp64_field(const crp_field *f, Fn &&fn) {
+	switch (f ? f->kind : FK_NONE) {
+	case FK_UNIFORM: return fn(f->uniform);
+	case FK_GRID: return f->cells ? fn(FieldGridCells{f->grid, f->cells}) : fn(FieldGrid{f->grid});
+	case FK_PLANEWAVE: return fn(FieldPlaneWave{f->pw});
+	case FK_MODULATED:
+		return f->cells ? fn(FieldModulated<FieldGridCells>{FieldGridCells{f->grid, f->cells}, f->mod})
+		                : fn(FieldModulated<FieldGrid>{FieldGrid{f->grid}, f->mod});
+	case FK_JF12: return fn(f->jf12);
+	case FK_COMPOSITE: return fn(FieldComposite{f->dcomp, 1.0});
+	default: return fn(FieldNone{});
+	}
+}
+
+static int step_api(const crp_propagator *p, int op, size_t n, const double *x, const double *u, const double *h,
+                    const double *z, const double *q, const double *m, double *ox, double *ou, double *ex, double *eu,
+                    const char *who) {
+	const bool tryStep = op == SO_BP_TRY || op == SO_CK_TRY;
+	if (!p || !x || !u || (op != SO_CK_DYDT && !h) || !q || !m || !ox || !ou || (tryStep && (!ex || !eu)))
+		return fail(CRP_E_INVALID, std::string(who) + ": NULL argument");
+	if ((op == SO_BP_DY || op == SO_BP_TRY) != (p->type == 0) || p->type == 2)
+		return fail(CRP_E_INVALID, std::string(who) + ": wrong propagator type");
+	if (n == 0)
+		return CRP_OK;
+	if (int rc = field_prepare(p->field))
+		return rc;
+	double *d = nullptr;
+	CRP_HIP_TRY(hipMalloc(&d, 22 * n * sizeof(double)));
+	StepArgs A;
+	double *dx = d, *du = d + 3 * n, *dh = d + 6 * n, *dz = d + 7 * n, *dq = d + 8 * n, *dm = d + 9 * n;
+	A.x = dx; A.u = du; A.h = dh; A.z = dz; A.q = dq; A.m = dm;
+	A.ox = d + 10 * n; A.ou = d + 13 * n; A.ex = d + 16 * n; A.eu = d + 19 * n;
+	A.n = n;
+	std::vector<double> zeros(n, 0.0);
+	hipError_t e = hipMemcpy(dx, x, 3 * n * sizeof(double), hipMemcpyHostToDevice);
+	if (e == hipSuccess)
+		e = hipMemcpy(du, u, 3 * n * sizeof(double), hipMemcpyHostToDevice);
+	if (e == hipSuccess)
+		e = hipMemcpy(dh, h ? h : zeros.data(), n * sizeof(double), hipMemcpyHostToDevice);
+	if (e == hipSuccess)
+		e = hipMemcpy(dz, z ? z : zeros.data(), n * sizeof(double), hipMemcpyHostToDevice);
+	if (e == hipSuccess)
+		e = hipMemcpy(dq, q, n * sizeof(double), hipMemcpyHostToDevice);
+	if (e == hipSuccess)
+		e = hipMemcpy(dm, m, n * sizeof(double), hipMemcpyHostToDevice);
+	if (e == hipSuccess) {
+		const unsigned blocks = (unsigned)((n + 255) / 256);
+		with_fp64_field(p->field, [&](auto fld) {
+			typedef decltype(fld) F;
+			switch (op) {
+			case SO_BP_DY: hipLaunchKernelGGL((k_step_api<F, SO_BP_DY>), blocks, 256, 0, 0, fld, A); break;
+			case SO_BP_TRY: hipLaunchKernelGGL((k_step_api<F, SO_BP_TRY>), blocks, 256, 0, 0, fld, A); break;
+			case SO_CK_DYDT: hipLaunchKernelGGL((k_step_api<F, SO_CK_DYDT>), blocks, 256, 0, 0, fld, A); break;
+			default: hipLaunchKernelGGL((k_step_api<F, SO_CK_TRY>), blocks, 256, 0, 0, fld, A); break;
+			}
+			return 0;
+		});
+		e = hipGetLastError();
+	}
+	if (e == hipSuccess)
+		e = hipMemcpy(ox, A.ox, 3 * n * sizeof(double), hipMemcpyDeviceToHost);
+	if (e == hipSuccess)
+		e = hipMemcpy(ou, A.ou, 3 * n * sizeof(double), hipMemcpyDeviceToHost);
+	if (e == hipSuccess && tryStep)
+		e = hipMemcpy(ex, A.ex, 3 * n * sizeof(double), hipMemcpyDeviceToHost);
+	if (e == hipSuccess && tryStep)
+		e = hipMemcpy(eu, A.eu, 3 * n * sizeof(double), hipMemcpyDeviceToHost);
+	(void)hipFree(d);
+	if (e != hipSuccess)
+		return fail(CRP_E_HIP, std::string(who) + ": " + hipGetErrorString(e));
+	return CRP_OK;
+}
+
+extern "C" {
+
+int crp_bp_dy(const crp_propagator_t *p, size_t n, const double *pos, const double *dir, const double *step,
+              const double *z, const double *q, const double *m, double *out_pos, double *out_dir) {
+	return step_api(p, SO_BP_DY, n, pos, dir, step, z, q, m, out_pos, out_dir, nullptr, nullptr, "crp_bp_dy");
+}
+
+int crp_bp_try_step(const crp_propagator_t *p, size_t n, const double *pos, const double *dir, const double *h,
+                    const double *z, const double *q, const double *m, double *out_pos, double *out_dir,
+                    double *err_pos, double *err_dir) {
+	return step_api(p, SO_BP_TRY, n, pos, dir, h, z, q, m, out_pos, out_dir, err_pos, err_dir, "crp_bp_try_step");
+}
+
+int crp_ck_dydt(const crp_propagator_t *p, size_t n, const double *x, const double *u, const double *z,
+                const double *q, const double *E, double *dxdt, double *dudt) {
+	return step_api(p, SO_CK_DYDT, n, x, u, nullptr, z, q, E, dxdt, dudt, nullptr, nullptr, "crp_ck_dydt");
+}
+
+int crp_ck_try_step(const crp_propagator_t *p, size_t n, const double *x, const double *u, const double *t,
+                    const double *z, const double *q, const double *E, double *out_x, double *out_u, double *err_x,
+                    double *err_u) {
+	return step_api(p, SO_CK_TRY, n, x, u, t, z, q, E, out_x, out_u, err_x, err_u, "crp_ck_try_step");
 }
 
 int crp_field_eval(const crp_field_t *f, const double *pos, size_t n, double *B) {

@@ -59,6 +59,11 @@ public:
 	 * maxLength > 0 fuses MaximumTrajectoryLength. Throws std::runtime_error. */
 	crp_propagator *makeDevicePropagator(double maxLength) const;
 
+	/* getFieldAtPosition (src/module/PropagationBP.cpp:137-149, src/module/PropagationCK.cpp:146-158):
+	 * field->getField(pos, z), zero without a field; an exception of the field is reported on stderr
+	 * and gives zero. */
+	Vector3d getFieldAtPosition(Vector3d pos, double z) const;
+
 protected:
 	const char *name() const { return kind == BP ? "PropagationBP" : (kind == CK ? "PropagationCK" : "DiffusionSDE"); }
 	Kind kind;
@@ -74,12 +79,37 @@ protected:
 	mutable double lastPPL = 0;
 };
 
+class ParticleState;
+
+/* Phase point (position, direction) of the single-step functions (include/crpropa/module/PropagationBP.h:24-46) */
+class PhasePoint {
+public:
+	Vector3d x, u;
+	PhasePoint() {}
+	PhasePoint(const Vector3d &x, const Vector3d &u) : x(x), u(u) {}
+	PhasePoint(double f) : x(Vector3d(f, f, f)), u(Vector3d(f, f, f)) {}
+	PhasePoint operator*(double f) const { return PhasePoint(x * f, u * f); }
+	PhasePoint &operator+=(const PhasePoint &y) {
+		x += y.x;
+		u += y.u;
+		return *this;
+	}
+};
+
 class PropagationBP : public DevicePropagator {
 public:
+	typedef PhasePoint Y;
 	/* fixed step: tolerance 0.42, minStep = maxStep = fixedStep */
 	PropagationBP(ref_ptr<MagneticField> field = NULL, double fixedStep = 1. * kpc);
 	PropagationBP(ref_ptr<MagneticField> field, double tolerance, double minStep, double maxStep);
 	std::string getDescription() const;
+
+	/* The reference's single-step functions (include/crpropa/module/PropagationBP.h:89-116), run on the
+	 * device with this propagator's field (the kernels' FP64 code): one Boris step; the error
+	 * estimate |x1 - x2| / (0.75 step); a full step, two half steps and the error as Y(estimate). */
+	Y dY(Vector3d pos, Vector3d dir, double step, double z, double q, double m) const;
+	double errorEstimation(const Vector3d x1, const Vector3d x2, double step) const;
+	void tryStep(const Y &y, Y &out, Y &error, double h, ParticleState &p, double z, double q, double m) const;
 };
 
 } // namespace crpropa

@@ -431,15 +431,34 @@ PYBIND11_MODULE(_crpropa, m) {
 	    .def("setAutotune", &DevicePropagator::setAutotune, py::arg("on"), py::arg("probeSteps") = 32)
 	    .def("getAutotune", &DevicePropagator::getAutotune);
 
-	py::class_<PropagationBP, DevicePropagator, ref_ptr<PropagationBP>>(m, "PropagationBP")
+	py::class_<PhasePoint>(m, "PhasePoint")
+	    .def(py::init<>())
+	    .def(py::init<const Vector3d &, const Vector3d &>())
+	    .def(py::init<double>())
+	    .def_readwrite("x", &PhasePoint::x)
+	    .def_readwrite("u", &PhasePoint::u)
+	    .def("__mul__", &PhasePoint::operator*)
+	    .def("__iadd__", &PhasePoint::operator+=, py::return_value_policy::reference_internal);
+
+	auto bp = py::class_<PropagationBP, DevicePropagator, ref_ptr<PropagationBP>>(m, "PropagationBP")
 	    .def(py::init<ref_ptr<MagneticField>, double>(), py::arg("field") = ref_ptr<MagneticField>(),
 	         py::arg("fixedStep") = 1. * kpc)
 	    .def(py::init<ref_ptr<MagneticField>, double, double, double>(), py::arg("field"), py::arg("tolerance"),
-	         py::arg("minStep"), py::arg("maxStep"));
+	         py::arg("minStep"), py::arg("maxStep"))
+	    .def("dY", &PropagationBP::dY, py::arg("pos"), py::arg("dir"), py::arg("step"), py::arg("z"), py::arg("q"),
+	         py::arg("m"))
+	    .def("errorEstimation", &PropagationBP::errorEstimation)
+	    .def("tryStep", &PropagationBP::tryStep)
+	    .def("getFieldAtPosition", &PropagationBP::getFieldAtPosition);
+	bp.attr("Y") = m.attr("PhasePoint");
 
-	py::class_<PropagationCK, DevicePropagator, ref_ptr<PropagationCK>>(m, "PropagationCK")
+	auto ck = py::class_<PropagationCK, DevicePropagator, ref_ptr<PropagationCK>>(m, "PropagationCK")
 	    .def(py::init<ref_ptr<MagneticField>, double, double, double>(), py::arg("field") = ref_ptr<MagneticField>(),
-	         py::arg("tolerance") = 1e-4, py::arg("minStep") = 0.1 * kpc, py::arg("maxStep") = 1 * Gpc);
+	         py::arg("tolerance") = 1e-4, py::arg("minStep") = 0.1 * kpc, py::arg("maxStep") = 1 * Gpc)
+	    .def("dYdt", &PropagationCK::dYdt)
+	    .def("tryStep", &PropagationCK::tryStep)
+	    .def("getFieldAtPosition", &PropagationCK::getFieldAtPosition);
+	ck.attr("Y") = m.attr("PhasePoint");
 
 	// ---- JF12 (src/magneticField/JF12Field.cpp)
 	py::class_<JF12Field, MagneticField, ref_ptr<JF12Field>>(m, "JF12Field")

@@ -141,8 +141,8 @@ crp_propagator *DevicePropagator::makeDevicePropagator(double maxLength) const {
 		f = field->deviceField();
 		if (!f)
 			throw std::runtime_error(std::string(name()) +
-			                         ": the magnetic field has no GPU implementation (only uniform, "
-			                         "MagneticFieldGrid and PlaneWaveTurbulence fields run on the device)");
+			                         ": the magnetic field has no GPU implementation (fields defined in Python or "
+			                         "other host code do not run on the device)");
 	}
 	crp_propagator *p = nullptr;
 	if (kind == BP)
@@ -256,6 +256,76 @@ PropagationCK::PropagationCK(ref_ptr<MagneticField> f, double tol, double minS, 
 	setMinimumStep(minS);
 }
 
+// ------------------------------------------------------------------ single-step functions
+Vector3d DevicePropagator::getFieldAtPosition(Vector3d pos, double z) const {
+	Vector3d B(0, 0, 0);
+	try {
+		if (field.valid())
+			B = field->getField(pos, z);
+	} catch (std::exception &e) {
+		std::cerr << name() << ": Exception in " << name() << "::getFieldAtPosition.\n" << e.what() << std::endl;
+	}
+	return B;
+}
+
+namespace {
+// a device propagator of this module's kind and field for the single-step C functions
+struct StepHandle {
+	crp_propagator *p = nullptr;
+	~StepHandle() {
+		if (p)
+			crp_propagator_destroy(p);
+	}
+};
+} // namespace
+
+PropagationBP::Y PropagationBP::dY(Vector3d pos, Vector3d dir, double step, double z, double q, double m) const {
+	StepHandle h;
+	h.p = makeDevicePropagator(0);
+	const double x[3] = {pos.x, pos.y, pos.z}, u[3] = {dir.x, dir.y, dir.z};
+	double ox[3], ou[3];
+	throwIf(crp_bp_dy(h.p, 1, x, u, &step, &z, &q, &m, ox, ou));
+	return Y(Vector3d(ox[0], ox[1], ox[2]), Vector3d(ou[0], ou[1], ou[2]));
+}
+
+double PropagationBP::errorEstimation(const Vector3d x1, const Vector3d x2, double step) const {
+	// src/module/PropagationBP.cpp:152-159: |x1 - x2| / (step * (1 - 1/4))
+	Vector3d diff = (x1 - x2);
+	return diff.getR() / (step * (1 - 1 / 4.));
+}
+
+void PropagationBP::tryStep(const Y &y, Y &out, Y &error, double h, ParticleState &, double z, double q,
+                            double m) const {
+	StepHandle hd;
+	hd.p = makeDevicePropagator(0);
+	const double x[3] = {y.x.x, y.x.y, y.x.z}, u[3] = {y.u.x, y.u.y, y.u.z};
+	double ox[3], ou[3], ex[3], eu[3];
+	throwIf(crp_bp_try_step(hd.p, 1, x, u, &h, &z, &q, &m, ox, ou, ex, eu));
+	out = Y(Vector3d(ox[0], ox[1], ox[2]), Vector3d(ou[0], ou[1], ou[2]));
+	error = Y(Vector3d(ex[0], ex[1], ex[2]), Vector3d(eu[0], eu[1], eu[2]));
+}
+
+PropagationCK::Y PropagationCK::dYdt(const Y &y, ParticleState &p, double z) const {
+	StepHandle h;
+	h.p = makeDevicePropagator(0);
+	const double x[3] = {y.x.x, y.x.y, y.x.z}, u[3] = {y.u.x, y.u.y, y.u.z};
+	const double q = p.getCharge(), E = p.getEnergy();
+	double dx[3], du[3];
+	throwIf(crp_ck_dydt(h.p, 1, x, u, &z, &q, &E, dx, du));
+	return Y(Vector3d(dx[0], dx[1], dx[2]), Vector3d(du[0], du[1], du[2]));
+}
+
+void PropagationCK::tryStep(const Y &y, Y &out, Y &error, double t, ParticleState &p, double z) const {
+	StepHandle h;
+	h.p = makeDevicePropagator(0);
+	const double x[3] = {y.x.x, y.x.y, y.x.z}, u[3] = {y.u.x, y.u.y, y.u.z};
+	const double q = p.getCharge(), E = p.getEnergy();
+	double ox[3], ou[3], ex[3], eu[3];
+	throwIf(crp_ck_try_step(h.p, 1, x, u, &t, &z, &q, &E, ox, ou, ex, eu));
+	out = Y(Vector3d(ox[0], ox[1], ox[2]), Vector3d(ou[0], ou[1], ou[2]));
+	error = Y(Vector3d(ex[0], ex[1], ex[2]), Vector3d(eu[0], eu[1], eu[2]));
+}
+
 std::string PropagationCK::getDescription() const {
 	std::stringstream s;
 	s << "Propagation in magnetic fields using the Cash-Karp method.";
@@ -304,9 +374,14 @@ void DiffusionSDE::setScale(double s) {
 }
 
 Vector3d DiffusionSDE::getMagneticFieldAtPosition(Vector3d pos, double z) const {
+	// src/module/DiffusionSDE.cpp:352-366: an exception of the field is reported and gives zero
 	Vector3d B(0, 0, 0);
-	if (field.valid())
-		B = field->getField(pos, z);
+	try {
+		if (field.valid())
+			B = field->getField(pos, z);
+	} catch (std::exception &e) {
+		std::cerr << "DiffusionSDE: Exception in DiffusionSDE::getMagneticFieldAtPosition.\n" << e.what() << std::endl;
+	}
 	return B;
 }
 

@@ -213,6 +213,27 @@ int crp_propagator_set_max_trajectory_length(crp_propagator_t *p, double max_len
 int crp_propagator_set_precision(crp_propagator_t *p, int precision);
 int crp_propagator_destroy(crp_propagator_t *p);
 
+/* The reference's public per-step functions, on the device with the propagator's field (FP64, the
+ * kernels' arithmetic), for n phase points; positions/directions/outputs are AoS [3n], the other
+ * arrays [n]; z may be NULL (0).  For callers of the single-step API of
+ *   PropagationBP::dY(pos, dir, step, z, q, m)              include/crpropa/module/PropagationBP.h:89
+ *   PropagationBP::tryStep(y, out, error, h, p, z, q, m)     include/crpropa/module/PropagationBP.h:116
+ *     (error = Y(errorEstimation(out.x, x after two h/2 steps, h)): all six components equal S)
+ *   PropagationCK::dYdt(y, p, z) -> (velocity, du/dt)        include/crpropa/module/PropagationCK.h:74
+ *   PropagationCK::tryStep(y, out, error, t, p, z)           include/crpropa/module/PropagationCK.h:76
+ * (src/module/PropagationBP.cpp:8-39, src/module/PropagationCK.cpp:28-61).  m is the mass E/c^2 (BP),
+ * E the energy (CK); t = step / c. */
+int crp_bp_dy(const crp_propagator_t *p, size_t n, const double *pos, const double *dir, const double *step,
+              const double *z, const double *q, const double *m, double *out_pos, double *out_dir);
+int crp_bp_try_step(const crp_propagator_t *p, size_t n, const double *pos, const double *dir, const double *h,
+                    const double *z, const double *q, const double *m, double *out_pos, double *out_dir,
+                    double *err_pos, double *err_dir);
+int crp_ck_dydt(const crp_propagator_t *p, size_t n, const double *x, const double *u, const double *z,
+                const double *q, const double *E, double *dxdt, double *dudt);
+int crp_ck_try_step(const crp_propagator_t *p, size_t n, const double *x, const double *u, const double *t,
+                    const double *z, const double *q, const double *E, double *out_x, double *out_u, double *err_x,
+                    double *err_u);
+
 /* ---- particle batches: device-resident structure of arrays ------------ */
 /* The fields of Candidate/ParticleState that the path reads or writes
  * (include/crpropa/Candidate.h, include/crpropa/ParticleState.h). */

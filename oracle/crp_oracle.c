@@ -751,6 +751,46 @@ static inline void orc_ck_trystep(const orc_field *f, const orc_y *y, orc_y *out
 	}
 }
 
+/* The reference's single-step functions on n phase points (tests): PropagationBP::dY and tryStep
+ * (src/module/PropagationBP.cpp:8-39,152-159), PropagationCK::dYdt and tryStep
+ * (src/module/PropagationCK.cpp:28-61).  x, u and the outputs AoS [3n]; m: mass (BP) or energy (CK). */
+void orc_step_batch(const orc_field *f, int op, size_t n, const double *x, const double *u, const double *h,
+                    const double *z, const double *q, const double *m, double *ox, double *ou, double *ex, double *eu,
+                    int libm) {
+	for (size_t i = 0; i < n; i++) {
+		orc_y y, o, e;
+		for (int a = 0; a < 3; a++) {
+			y.x[a] = x[3 * i + a];
+			y.u[a] = u[3 * i + a];
+			e.x[a] = e.u[a] = 0;
+		}
+		const double zi = z ? z[i] : 0.0;
+		if (op == 0 || op == 1) {
+			o = orc_bp_dy(f, y.x, y.u, h[i], q[i], m[i], zi, libm);
+			if (op == 1) {
+				orc_y half = orc_bp_dy(f, y.x, y.u, h[i] / 2, q[i], m[i], zi, libm);
+				orc_y cmp = orc_bp_dy(f, half.x, half.u, h[i] / 2, q[i], m[i], zi, libm);
+				double diff[3] = {o.x[0] - cmp.x[0], o.x[1] - cmp.x[1], o.x[2] - cmp.x[2]};
+				double S = orc_norm(diff) / (h[i] * (1 - 1 / 4.));
+				for (int a = 0; a < 3; a++)
+					e.x[a] = e.u[a] = S; /* Y(double) */
+			}
+		} else if (op == 2) {
+			o = orc_ck_dydt(f, &y, q[i], m[i], zi, libm);
+		} else {
+			orc_ck_trystep(f, &y, &o, &e, h[i], q[i], m[i], zi, libm);
+		}
+		for (int a = 0; a < 3; a++) {
+			ox[3 * i + a] = o.x[a];
+			ou[3 * i + a] = o.u[a];
+			if (ex) {
+				ex[3 * i + a] = e.x[a];
+				eu[3 * i + a] = e.u[a];
+			}
+		}
+	}
+}
+
 static inline double orc_clip(double x, double lo, double hi) {
 	/* std::max(lower, std::min(x, upper)), include/crpropa/Common.h:29-32 */
 	double m = (hi < x) ? hi : x;

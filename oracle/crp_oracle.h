@@ -183,6 +183,11 @@ void orc_planewave_init(double brms, double lmin, double lmax, double lbendover,
                         int simple_spectrum, int nm, int seed, double *k, double *Ak, double *xi, double *kappa,
                         double *beta);
 
+/* single-step functions (op 0 BP dY, 1 BP tryStep, 2 CK dYdt, 3 CK tryStep) on n phase points */
+void orc_step_batch(const orc_field *f, int op, size_t n, const double *x, const double *u, const double *h,
+                    const double *z, const double *q, const double *m, double *ox, double *ou, double *ex, double *eu,
+                    int libm);
+
 /* propagation: maxSteps process() calls per particle (stopping early when
  * inactive), OpenMP over particles with nthreads (0: default). Returns the
  * total number of accepted steps and writes rejected trial counts to *trials. */

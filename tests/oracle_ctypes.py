@@ -117,6 +117,8 @@ def lib():
                                     C.c_int, _P(C.c_longlong)]
         L.orc_field_eval.argtypes = [_P(Field), _dp, C.c_size_t, _dp, C.c_int]
         L.orc_field_eval_z.argtypes = [_P(Field), _dp, _dp, C.c_size_t, _dp, C.c_int]
+        L.orc_step_batch.argtypes = [_P(Field), C.c_int, C.c_size_t, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                     _dp, C.c_int]
         L.orc_planewave_init.argtypes = [C.c_double] * 6 + [C.c_int, C.c_int, C.c_int] + [_dp] * 5
         L.orc_cos_batch.argtypes = [_dp, C.c_size_t, _dp, C.c_int]
         L.orc_pow_batch.argtypes = [_dp, C.c_double, C.c_size_t, _dp, C.c_int]
@@ -485,6 +487,22 @@ def field_for_case(case):
         vals = noise_grid(gm["seed"], gm["n"], gm["amp"])
         return OracleField.grid(vals, gm["origin"], gm["spacing"])
     raise ValueError("unknown field in case")
+
+
+STEP_BP_DY, STEP_BP_TRY, STEP_CK_DYDT, STEP_CK_TRY = 0, 1, 2, 3
+
+
+def step_batch(field, op, x, u, h, z, q, m, libm=LIBM_CR):
+    """The reference's single-step functions on the oracle (orc_step_batch): returns (out_x, out_u, err_x,
+    err_u) as (n, 3) arrays (the errors are zero for dY / dYdt)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = x.shape[0]
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    arr = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (n,))) for a in (h, z, q, m)]
+    outs = [np.zeros((n, 3)) for _ in range(4)]
+    lib().orc_step_batch(C.byref(field.s), op, n, ptr(x), ptr(u), *[ptr(a) for a in arr], *[ptr(o) for o in outs],
+                         libm)
+    return outs
 
 
 def decorator_fields():

@@ -245,3 +245,48 @@ def test_bench_particles_are_the_reference_draws():
     pos, dn, E2, q = bench.make_particles(w, 4096)
     assert np.array_equal(dn.view(np.uint64), O.normalise(od).view(np.uint64))
     assert set(np.unique(np.round(q / O.eplus))) == {1.0, 2.0, 7.0, 14.0, 26.0}
+
+
+def test_custom_field_at_position():
+    """test/testPythonExtension.py:70-92: getFieldAtPosition of PropagationBP / PropagationCK and
+    DiffusionSDE::getMagneticFieldAtPosition forward to the field's getField(position, z)."""
+    import crpropa3_amd as crp
+
+    class CustomMagneticField(crp.MagneticField):
+        def __init__(self, val):
+            crp.MagneticField.__init__(self)
+            self.val = val
+
+        def getField(self, position, z):
+            return crp.Vector3d(self.val)
+
+    field = CustomMagneticField(crp.gauss)
+    propBP = crp.PropagationBP(field, 1e-4, 1 * crp.Mpc, 1 * crp.Mpc)
+    propCK = crp.PropagationCK(field, 1e-4, 1 * crp.Mpc, 1 * crp.Mpc)
+    propSDE = crp.DiffusionSDE(field)
+    pos = crp.Vector3d(-1, 0, 0)
+    z = 0
+    fieldAtPos = field.getField(pos, z)
+    assert fieldAtPos == propBP.getFieldAtPosition(pos, z)
+    assert fieldAtPos == propCK.getFieldAtPosition(pos, z)
+    assert fieldAtPos == propSDE.getMagneticFieldAtPosition(pos, z)
+    # without a field: zero (src/module/PropagationBP.cpp:137-149)
+    assert crp.PropagationCK().getFieldAtPosition(pos, z) == crp.Vector3d(0.0)
+
+
+def test_single_step_api_without_device_field_is_rejected():
+    """The single-step functions run on the device: a field defined in Python is rejected with the
+    reference's exception type."""
+    import crpropa3_amd as crp
+
+    class PyField(crp.MagneticField):
+        def __init__(self):
+            crp.MagneticField.__init__(self)
+
+        def getField(self, position, z=0):
+            return crp.Vector3d(1e-13)
+
+    bp = crp.PropagationBP(PyField(), 1 * crp.kpc)
+    with pytest.raises(RuntimeError, match="no GPU implementation"):
+        bp.dY(crp.Vector3d(0.0), crp.Vector3d(1, 0, 0), 1 * crp.kpc, 0.0, crp.eplus, crp.EeV)
+    assert bp.errorEstimation(crp.Vector3d(3, 0, 0), crp.Vector3d(0, 4, 0), 1.0) == 5 / 0.75
