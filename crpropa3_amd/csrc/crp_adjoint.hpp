@@ -25,7 +25,8 @@
 namespace crp {
 
 struct AdjGrid {
-	const float *g; // (nx*ny + iy)*nz + iz, 3 floats per point
+	const float *g;     // (nx*ny + iy)*nz + iz, 3 floats per point
+	const float *cells; // null, or per cell (ix*ny + iy)*nz + iz its 8 corner values in corner order, 32 floats
 	int nx, ny, nz;
 	double gridOrigin[3], invSpacing[3], invN[3];
 	double scale;
@@ -75,16 +76,37 @@ __device__ __forceinline__ void adj_weight(const TriPoint &tp, int c, double &w,
 	dwz = wx * wy * sz;
 }
 
+// the 24 corner values of the cell, in corner order: one 128-byte line of the cell table when there is
+// one, else 8 gathers of the point grid
+__device__ __forceinline__ void adj_gather(const AdjGrid &G, const TriPoint &tp, float v[24]) {
+	if (G.cells) {
+		const size_t c = ((size_t)tp.lo[0] * G.ny + tp.lo[1]) * G.nz + tp.lo[2];
+		const float4 *q = reinterpret_cast<const float4 *>(G.cells + 32 * c);
+#pragma unroll
+		for (int k = 0; k < 6; k++) {
+			const float4 f = q[k];
+			v[4 * k] = f.x; v[4 * k + 1] = f.y; v[4 * k + 2] = f.z; v[4 * k + 3] = f.w;
+		}
+	} else {
+#pragma unroll
+		for (int k = 0; k < 8; k++) {
+			const float *gv = G.g + 3 * adj_corner(G, tp, k);
+			v[3 * k] = gv[0]; v[3 * k + 1] = gv[1]; v[3 * k + 2] = gv[2];
+		}
+	}
+}
+
 __device__ __forceinline__ V3 adj_interp(const AdjGrid &G, const TriPoint &tp) {
+	float v[24];
+	adj_gather(G, tp, v);
 	double b[3] = {0, 0, 0};
 #pragma unroll
 	for (int c = 0; c < 8; c++) {
 		double w, a, bb, cc;
 		adj_weight(tp, c, w, a, bb, cc);
-		const float *v = G.g + 3 * adj_corner(G, tp, c);
-		b[0] += w * (double)v[0];
-		b[1] += w * (double)v[1];
-		b[2] += w * (double)v[2];
+		b[0] += w * (double)v[3 * c];
+		b[1] += w * (double)v[3 * c + 1];
+		b[2] += w * (double)v[3 * c + 2];
 	}
 	return v3(b[0], b[1], b[2]);
 }
@@ -97,12 +119,9 @@ __device__ __forceinline__ V3 vcross(const V3 &a, const V3 &b) {
 	return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 
-// one forward step of the differentiable model
-__device__ __forceinline__ void adj_step(const AdjGrid &G, double h, double kq, V3 &x, V3 &u) {
-	const V3 x1 = vadd(x, vmul(u, h / 2));
-	TriPoint tp;
-	adj_cell(G, x1, tp);
-	const V3 B = vmul(adj_interp(G, tp), G.scale);
+// the step after the interpolation: T = trilinear(grid) at the half-step position x1
+__device__ __forceinline__ void adj_push(const AdjGrid &G, double h, double kq, const V3 &T, const V3 &x1, V3 &x, V3 &u) {
+	const V3 B = vmul(T, G.scale);
 	const V3 t = vmul(B, kq);
 	const double D = 1 + vdot(t, t);
 	const V3 s2 = vmul(t, 2 / D);
@@ -113,6 +132,14 @@ __device__ __forceinline__ void adj_step(const AdjGrid &G, double h, double kq, 
 	const V3 n1 = vmul(up, 1 / r1);
 	const double r2 = __builtin_sqrt(vdot(n1, n1));
 	u = vmul(n1, 1 / r2);
+}
+
+// one forward step of the differentiable model
+__device__ __forceinline__ void adj_step(const AdjGrid &G, double h, double kq, V3 &x, V3 &u) {
+	const V3 x1 = vadd(x, vmul(u, h / 2));
+	TriPoint tp;
+	adj_cell(G, x1, tp);
+	adj_push(G, h, kq, adj_interp(G, tp), x1, x, u);
 }
 
 // reverse of one step: (x, u) is the state before the step; (ax, au) the
@@ -249,6 +276,397 @@ __global__ void __launch_bounds__(256) k_adjoint_backward(AdjArgs A) {
 		gs += __shfl_xor(gs, o);
 	if ((threadIdx.x & 63) == 0)
 		unsafeAtomicAdd(A.gradScale, gs);
+}
+
+
+// ---------------------------------------------------------------- backward in blocks of segments
+// The checkpointed backward above walks every particle's segments one after the other: each
+// segment's recompute and reverse sweep wait on one gather per step at one wave per SIMD, and
+// every reverse step adds 24 FP64 atomics into the grid gradient.  The blocked backward instead
+//   1. recomputes a block of segments of every particle at once (segments x n independent lanes)
+//      and stores, for every step, what its reverse needs: the direction u before the step, the
+//      interpolated field T and its derivatives with respect to the cell fractions (15 doubles),
+//      wave by wave, and the first half of the step's deposit record (the 16^3-cell tile, the cell in
+//      it and the fractions); each workgroup also counts its records per tile;
+//   2. scans the counts into every tile's start and every particle group's place within each tile;
+//   3. runs the reverse steps of the block per particle (one lane per particle, the only sequential
+//      part: no gathers, the stored values of the next two steps loaded ahead) and writes each
+//      step's complete 32-byte record (with the adjoint aT of T: the gradient of grid value G_c is
+//      w_c aT) into its tile's range;
+//   4. accumulates each tile's records into its 17^3 grid points in LDS (FP64), one workgroup per
+//      kDepPerGroup records of a tile, and adds the points into the gradient.
+// The host runs the reverse sweeps on the caller's stream and the recompute of the next block and the
+// accumulation of the previous one on a second stream beside it.  The arithmetic of a step is that of
+// adj_step / adj_step_reverse, except that the position gradient is taken as aT . dT/df; the deposits
+// are summed in another order.
+constexpr int kTileShift = 4, kTileCells = 1 << kTileShift, kTilePts = kTileCells + 1;
+constexpr int kTilePtsAll = kTilePts * kTilePts * kTilePts;
+constexpr int kCellBits = 3 * kTileShift;
+constexpr int kAdjGroup = 256;       // particles per recompute / reverse workgroup
+constexpr int kDepPerGroup = 65536;  // records per accumulating workgroup
+constexpr int kDepTileThreads = 1024;
+constexpr int kDepBatch = 4;
+// stored per step and wave: u, T, dT/dfx, dT/dfy, dT/dfz (15 x 64 doubles)
+constexpr int kAdjStored = 15;
+constexpr int kAdjStepBytes = kAdjStored * 64 * 8;
+
+struct __align__(16) DepRec {
+	uint32_t key; // tile << kCellBits | cell in tile (x << 2 kTileShift | y << kTileShift | z)
+	float f[3];   // fractions of the half-step position in its cell
+	float a[3];   // adjoint of the interpolated field
+	uint32_t pad;
+};
+
+struct AdjBlock {
+	int seg0, nseg;      // segments of this block
+	int s0, s1;          // steps [s0, s1)
+	int first, last;     // first block of the backward (reads the output adjoints); last block
+	int npg;             // particle groups: ceil(n / kAdjGroup)
+	char *states;        // [segment - seg0][wave][step in segment] kAdjStepBytes
+	double *carry;       // [6][n]: adjoint (x, u) between blocks
+	uint4 *rec0;         // [m], record (s - s0) * n + i: key, fractions
+	DepRec *srec;        // [m], grouped by tile
+	uint32_t *counts;    // [npg][ntiles]: records per particle group and tile, then their offsets
+	uint32_t *tileStart; // [ntiles + 1]
+	uint32_t *wgStart;   // [ntiles + 1]: first accumulating workgroup of every tile
+	int ntx, nty, ntz, ntiles;
+	size_t m;            // records of the block: (s1 - s0) * n
+};
+
+__device__ __forceinline__ size_t adj_step_offset(const AdjBlock &Bk, int K, int segl, size_t i, int j) {
+	return (((size_t)segl * Bk.npg * (kAdjGroup / 64) + (i >> 6)) * K + j) * kAdjStepBytes;
+}
+
+typedef uint32_t adj_u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(kAdjGroup) k_dep_zero(AdjBlock Bk) {
+	const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (k < (size_t)Bk.npg * Bk.ntiles)
+		Bk.counts[k] = 0u;
+}
+
+// One lane per (segment, particle); a workgroup holds kAdjGroup particles of one segment.  The
+// gathers of step j + 1 are issued before the stores of step j: a wave's wait for its loads also
+// waits for the stores issued before them, and the stored values are not read again here.
+__global__ void __launch_bounds__(kAdjGroup) k_adj_recompute(AdjArgs A, AdjBlock Bk) {
+	extern __shared__ uint32_t hist[];
+	const int T = Bk.ntiles;
+	for (int k = threadIdx.x; k < T; k += blockDim.x)
+		hist[k] = 0u;
+	__syncthreads();
+	const int segl = blockIdx.x / Bk.npg, pg = blockIdx.x % Bk.npg;
+	const size_t n = A.n, i = (size_t)pg * kAdjGroup + threadIdx.x;
+	if (i < n) {
+		const int seg = Bk.seg0 + segl;
+		const int len = min(A.K, A.nsteps - seg * A.K);
+		const double *c = A.ckpt + (size_t)seg * 6 * n;
+		V3 x = v3(c[i], c[n + i], c[2 * n + i]);
+		V3 u = v3(c[3 * n + i], c[4 * n + i], c[5 * n + i]);
+		const double kq = A.kq[i];
+		char *sp = Bk.states + adj_step_offset(Bk, A.K, segl, i, 0);
+		const int lane = (int)(i & 63);
+		size_t r = (size_t)(seg * A.K - Bk.s0) * n + i;
+		V3 x1 = vadd(x, vmul(u, A.h / 2));
+		TriPoint tp;
+		adj_cell(A.G, x1, tp);
+		float cv[24];
+		adj_gather(A.G, tp, cv);
+		for (int j = 0; j < len; j++, sp += kAdjStepBytes, r += n) {
+			// T in the order of adj_interp, and its derivatives with respect to the fractions
+			double b[3] = {0, 0, 0}, jx[3] = {0, 0, 0}, jy[3] = {0, 0, 0}, jz[3] = {0, 0, 0};
+#pragma unroll
+			for (int k = 0; k < 8; k++) {
+				double w, dwx, dwy, dwz;
+				adj_weight(tp, k, w, dwx, dwy, dwz);
+				const double g0 = cv[3 * k], g1 = cv[3 * k + 1], g2 = cv[3 * k + 2];
+				b[0] += w * g0; b[1] += w * g1; b[2] += w * g2;
+				jx[0] += dwx * g0; jx[1] += dwx * g1; jx[2] += dwx * g2;
+				jy[0] += dwy * g0; jy[1] += dwy * g1; jy[2] += dwy * g2;
+				jz[0] += dwz * g0; jz[1] += dwz * g1; jz[2] += dwz * g2;
+			}
+			const double sd[kAdjStored] = {u.x, u.y, u.z, b[0], b[1], b[2], jx[0], jx[1], jx[2],
+			                               jy[0], jy[1], jy[2], jz[0], jz[1], jz[2]};
+			const int cx = tp.lo[0], cy = tp.lo[1], cz = tp.lo[2];
+			const uint32_t tile = (uint32_t)(((cx >> kTileShift) * Bk.nty + (cy >> kTileShift)) * Bk.ntz + (cz >> kTileShift));
+			const uint32_t m = kTileCells - 1;
+			adj_u32x4 rec;
+			rec.x = (tile << kCellBits) | (((uint32_t)cx & m) << (2 * kTileShift)) | (((uint32_t)cy & m) << kTileShift) |
+			        ((uint32_t)cz & m);
+			rec.y = __float_as_uint((float)tp.fx);
+			rec.z = __float_as_uint((float)tp.fy);
+			rec.w = __float_as_uint((float)tp.fz);
+			atomicAdd(&hist[tile], 1u);
+			adj_push(A.G, A.h, kq, v3(b[0], b[1], b[2]), x1, x, u);
+			// the next step's cell and corners (after the last step unused, but in bounds: adj_cell wraps)
+			x1 = vadd(x, vmul(u, A.h / 2));
+			adj_cell(A.G, x1, tp);
+			adj_gather(A.G, tp, cv);
+			double *pd = reinterpret_cast<double *>(sp) + lane;
+#pragma unroll
+			for (int k = 0; k < kAdjStored; k++)
+				__builtin_nontemporal_store(sd[k], pd + 64 * k);
+			__builtin_nontemporal_store(rec, reinterpret_cast<adj_u32x4 *>(Bk.rec0 + r));
+		}
+	}
+	__syncthreads();
+	uint32_t *cnt = Bk.counts + (size_t)pg * T;
+	for (int k = threadIdx.x; k < T; k += blockDim.x)
+		if (hist[k])
+			atomicAdd(cnt + k, hist[k]);
+}
+
+// per tile: every particle group's offset within the tile (exclusive scan over the groups, loaded
+// kScanBatch groups at a time), and the tile's total in tileStart[t + 1]
+constexpr int kScanBatch = 16;
+__global__ void __launch_bounds__(256) k_dep_scan_chunks(AdjBlock Bk) {
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= Bk.ntiles)
+		return;
+	uint32_t acc = 0;
+	for (int c0 = 0; c0 < Bk.npg; c0 += kScanBatch) {
+		uint32_t v[kScanBatch];
+#pragma unroll
+		for (int k = 0; k < kScanBatch; k++)
+			v[k] = (c0 + k < Bk.npg) ? Bk.counts[(size_t)(c0 + k) * Bk.ntiles + t] : 0u;
+#pragma unroll
+		for (int k = 0; k < kScanBatch; k++) {
+			if (c0 + k < Bk.npg)
+				Bk.counts[(size_t)(c0 + k) * Bk.ntiles + t] = acc;
+			acc += v[k];
+		}
+	}
+	Bk.tileStart[t + 1] = acc;
+}
+
+// exclusive scans of the tile totals into tile starts and of the tiles' workgroup counts into
+// wgStart (one workgroup; at most kScanPer tiles per thread).  Every total is read into registers
+// before the first barrier and every start written after it, since thread j's starts overwrite the
+// totals thread j - 1 reads.
+constexpr int kScanPer = 8;
+__global__ void __launch_bounds__(1024) k_dep_scan_tiles(AdjBlock Bk) {
+	__shared__ uint32_t part[1024], partw[1024];
+	const int T = Bk.ntiles;
+	const int per = (T + 1023) / 1024;
+	const int b = threadIdx.x * per, e = min(b + per, T);
+	uint32_t v[kScanPer];
+	uint32_t sum = 0, sumw = 0;
+#pragma unroll
+	for (int k = 0; k < kScanPer; k++) {
+		v[k] = (b + k < e) ? Bk.tileStart[b + k + 1] : 0u;
+		sum += v[k];
+		sumw += (v[k] + kDepPerGroup - 1) / kDepPerGroup;
+	}
+	part[threadIdx.x] = sum;
+	partw[threadIdx.x] = sumw;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		uint32_t acc = 0, accw = 0;
+		for (int k = 0; k < 1024; k++) {
+			const uint32_t x = part[k], y = partw[k];
+			part[k] = acc;
+			partw[k] = accw;
+			acc += x;
+			accw += y;
+		}
+	}
+	__syncthreads();
+	uint32_t acc = part[threadIdx.x], accw = partw[threadIdx.x];
+#pragma unroll
+	for (int k = 0; k < kScanPer; k++) {
+		if (b + k < e) {
+			Bk.tileStart[b + k] = acc;
+			Bk.wgStart[b + k] = accw;
+			acc += v[k];
+			accw += (v[k] + kDepPerGroup - 1) / kDepPerGroup;
+		}
+	}
+	if (e == T && b < e) {
+		Bk.tileStart[T] = acc;
+		Bk.wgStart[T] = accw;
+	}
+}
+
+// the values one reverse step reads
+struct AdjStored {
+	double d[kAdjStored]; // u, T, dT/dfx, dT/dfy, dT/dfz
+	uint4 rec;            // key, fractions
+};
+
+__device__ __forceinline__ void adj_load_stored(const AdjBlock &Bk, int K, size_t n, size_t i, int q, AdjStored &s) {
+	const char *sp = Bk.states + adj_step_offset(Bk, K, q / K, i, q % K);
+	const double *pd = reinterpret_cast<const double *>(sp) + (i & 63);
+#pragma unroll
+	for (int k = 0; k < kAdjStored; k++)
+		s.d[k] = __builtin_nontemporal_load(pd + 64 * k);
+	s.rec = Bk.rec0[(size_t)q * n + i];
+}
+
+// adj_step_reverse from the stored u, T and dT/df: (ax, au) the adjoint after the step on entry, before
+// it on return; returns aT
+__device__ __forceinline__ V3 adj_reverse_stored(const AdjGrid &G, double h, double kq, const AdjStored &d, V3 &ax,
+                                                 V3 &au, double &gradScale) {
+	const V3 u = v3(d.d[0], d.d[1], d.d[2]);
+	const V3 T = v3(d.d[3], d.d[4], d.d[5]);
+	const V3 B = vmul(T, G.scale);
+	const V3 t = vmul(B, kq);
+	const double D = 1 + vdot(t, t);
+	const V3 s2 = vmul(t, 2 / D);
+	const V3 v = vadd(u, vcross(u, t));
+	const V3 up = vadd(u, vcross(v, s2));
+	const double r1 = __builtin_sqrt(vdot(up, up));
+	const V3 n1 = vmul(up, 1 / r1);
+	const double r2 = __builtin_sqrt(vdot(n1, n1));
+	const V3 un = vmul(n1, 1 / r2);
+	V3 an1 = vmul(vsub(au, vmul(un, vdot(un, au))), 1 / r2);
+	V3 aup = vmul(vsub(an1, vmul(n1, vdot(n1, an1))), 1 / r1);
+	aup = vadd(aup, vmul(ax, h / 2));
+	V3 ax1 = ax;
+	V3 aU = aup;
+	const V3 av = vcross(s2, aup);
+	const V3 as2 = vcross(aup, v);
+	aU = vadd(aU, vadd(av, vcross(t, av)));
+	V3 at = vcross(av, u);
+	at = vadd(at, vsub(vmul(as2, 2 / D), vmul(t, 4 / (D * D) * vdot(t, as2))));
+	const V3 aB = vmul(at, kq);
+	gradScale += vdot(T, aB);
+	const V3 aT = vmul(aB, G.scale);
+	const double gx = aT.x * d.d[6] + aT.y * d.d[7] + aT.z * d.d[8];
+	const double gy = aT.x * d.d[9] + aT.y * d.d[10] + aT.z * d.d[11];
+	const double gz = aT.x * d.d[12] + aT.y * d.d[13] + aT.z * d.d[14];
+	ax1 = vadd(ax1, v3(gx * G.invSpacing[0], gy * G.invSpacing[1], gz * G.invSpacing[2]));
+	ax = ax1;
+	au = vadd(aU, vmul(ax1, h / 2));
+	return aT;
+}
+
+// One lane per particle, kAdjGroup particles per workgroup; each step's record goes to its tile's range
+// at the group's offset (counts, from the scans) plus its rank among the group's records of the tile.
+__global__ void __launch_bounds__(kAdjGroup) k_adj_reverse(AdjArgs A, AdjBlock Bk) {
+	extern __shared__ uint32_t lds[];
+	const int T = Bk.ntiles;
+	uint32_t *base = lds, *rank = lds + T;
+	const uint32_t *off = Bk.counts + (size_t)blockIdx.x * T;
+	for (int t = threadIdx.x; t < T; t += blockDim.x) {
+		base[t] = Bk.tileStart[t] + off[t];
+		rank[t] = 0u;
+	}
+	__syncthreads();
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const bool live = i < A.n;
+	const size_t n = A.n;
+	const int K = A.K;
+	double gs = 0;
+	if (live) {
+		V3 ax, au;
+		if (Bk.first) {
+			ax = A.gxout ? v3(A.gxout[3 * i], A.gxout[3 * i + 1], A.gxout[3 * i + 2]) : v3(0, 0, 0);
+			au = v3(A.guout[3 * i], A.guout[3 * i + 1], A.guout[3 * i + 2]);
+		} else {
+			ax = v3(Bk.carry[i], Bk.carry[n + i], Bk.carry[2 * n + i]);
+			au = v3(Bk.carry[3 * n + i], Bk.carry[4 * n + i], Bk.carry[5 * n + i]);
+		}
+		const double kq = A.kq[i];
+		// the stored values of step s, s - 1 and s - 2 (loaded two steps ahead)
+		AdjStored d0, d1, d2;
+		int q = Bk.s1 - 1 - Bk.s0;
+		adj_load_stored(Bk, K, n, i, q, d0);
+		if (q >= 1)
+			adj_load_stored(Bk, K, n, i, q - 1, d1);
+		for (; q >= 0; q--) {
+			if (q >= 2)
+				adj_load_stored(Bk, K, n, i, q - 2, d2);
+			const V3 aT = adj_reverse_stored(A.G, A.h, kq, d0, ax, au, gs);
+			const uint32_t t = d0.rec.x >> kCellBits;
+			const size_t dst = (size_t)base[t] + atomicAdd(&rank[t], 1u);
+			uint4 *pr = reinterpret_cast<uint4 *>(Bk.srec + dst);
+			pr[0] = d0.rec;
+			pr[1] = make_uint4(__float_as_uint((float)aT.x), __float_as_uint((float)aT.y), __float_as_uint((float)aT.z), 0u);
+			d0 = d1;
+			d1 = d2;
+		}
+		if (Bk.last) {
+			A.gradX0[3 * i] = ax.x; A.gradX0[3 * i + 1] = ax.y; A.gradX0[3 * i + 2] = ax.z;
+			A.gradU0[3 * i] = au.x; A.gradU0[3 * i + 1] = au.y; A.gradU0[3 * i + 2] = au.z;
+		} else {
+			Bk.carry[i] = ax.x; Bk.carry[n + i] = ax.y; Bk.carry[2 * n + i] = ax.z;
+			Bk.carry[3 * n + i] = au.x; Bk.carry[4 * n + i] = au.y; Bk.carry[5 * n + i] = au.z;
+		}
+	}
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1)
+		gs += __shfl_xor(gs, o);
+	if ((threadIdx.x & 63) == 0)
+		unsafeAtomicAdd(A.gradScale, gs);
+}
+
+// accumulation: kDepPerGroup records of one tile per workgroup, summed into the tile's 17^3 points in
+// LDS (FP64), then added into the grid gradient (points on tile faces are shared with the
+// neighbouring tiles, and a tile with many records with the tile's other workgroups).  The grid
+// is launched with ntiles + ceil(m / kDepPerGroup) workgroups, an upper bound of wgStart[ntiles].
+__global__ void __launch_bounds__(kDepTileThreads) k_dep_tiles(AdjArgs A, AdjBlock Bk) {
+	__shared__ double acc[kTilePtsAll * 3];
+	const uint32_t g = blockIdx.x;
+	const int T = Bk.ntiles;
+	if (g >= Bk.wgStart[T])
+		return;
+	// the tile: the last t with wgStart[t] <= g (tiles without records have wgStart[t] == wgStart[t + 1])
+	int lo = 0, hi = T - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (Bk.wgStart[mid] <= g)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	const int t = lo;
+	const uint32_t r0 = Bk.tileStart[t] + (g - Bk.wgStart[t]) * (uint32_t)kDepPerGroup;
+	const uint32_t r1 = min(r0 + (uint32_t)kDepPerGroup, Bk.tileStart[t + 1]);
+	for (int k = threadIdx.x; k < kTilePtsAll * 3; k += blockDim.x)
+		acc[k] = 0.0;
+	__syncthreads();
+	const int cx[8] = {0, 1, 0, 0, 1, 0, 1, 1}, cy[8] = {0, 0, 1, 0, 0, 1, 1, 1}, cz[8] = {0, 0, 0, 1, 1, 1, 0, 1};
+	constexpr uint32_t m = kTileCells - 1;
+	for (uint32_t rb = r0 + threadIdx.x; rb < r1; rb += kDepBatch * blockDim.x) {
+		uint4 p0[kDepBatch], p1[kDepBatch];
+#pragma unroll
+		for (int q = 0; q < kDepBatch; q++) {
+			const uint32_t r = rb + q * blockDim.x;
+			if (r < r1) {
+				const uint4 *ps = reinterpret_cast<const uint4 *>(Bk.srec + r);
+				p0[q] = ps[0];
+				p1[q] = ps[1];
+			}
+		}
+#pragma unroll
+		for (int q = 0; q < kDepBatch; q++) {
+			if (rb + q * blockDim.x >= r1)
+				continue;
+			const uint32_t k = p0[q].x;
+			const int lx = (k >> (2 * kTileShift)) & m, ly = (k >> kTileShift) & m, lz = k & m;
+			const double fx = __uint_as_float(p0[q].y), fy = __uint_as_float(p0[q].z), fz = __uint_as_float(p0[q].w);
+			const double ax = __uint_as_float(p1[q].x), ay = __uint_as_float(p1[q].y), az = __uint_as_float(p1[q].z);
+#pragma unroll
+			for (int c = 0; c < 8; c++) {
+				const double w = (cx[c] ? fx : 1 - fx) * (cy[c] ? fy : 1 - fy) * (cz[c] ? fz : 1 - fz);
+				double *a = acc + 3 * (((lx + cx[c]) * kTilePts + (ly + cy[c])) * kTilePts + (lz + cz[c]));
+				atomicAdd(a, w * ax);
+				atomicAdd(a + 1, w * ay);
+				atomicAdd(a + 2, w * az);
+			}
+		}
+	}
+	__syncthreads();
+	const int tz = t % Bk.ntz, ty = (t / Bk.ntz) % Bk.nty, tx = t / (Bk.ntz * Bk.nty);
+	for (int k = threadIdx.x; k < kTilePtsAll * 3; k += blockDim.x) {
+		const double v = acc[k];
+		if (v == 0.0)
+			continue;
+		const int p = k / 3, comp = k % 3;
+		const int lz = p % kTilePts, ly = (p / kTilePts) % kTilePts, lx = p / (kTilePts * kTilePts);
+		const int gx = (tx * kTileCells + lx) % A.G.nx, gy = (ty * kTileCells + ly) % A.G.ny,
+		          gz = (tz * kTileCells + lz) % A.G.nz;
+		unsafeAtomicAdd(A.gradG + 3 * (((size_t)gx * A.G.ny + gy) * A.G.nz + gz) + comp, v);
+	}
 }
 
 } // namespace crp

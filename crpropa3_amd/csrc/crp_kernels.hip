@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -2246,16 +2247,112 @@ static int adj_args(const crp_adjoint_config *cfg, size_t n, AdjArgs &A) {
 	return CRP_OK;
 }
 
+// Plan of the blocked backward (crp_adjoint.hpp): segments per block, deposit tiles and the scratch
+// layout.  Every per-block buffer is doubled so that the reverse sweep of one block overlaps the
+// recompute of the next and the accumulation of the previous one.  Grids with more than kAdjMaxTiles
+// tiles of 16^3 cells use the checkpointed backward with direct atomics.
+constexpr int kAdjMaxTiles = 8192;
+constexpr size_t kAdjBlockBudget = (size_t)4 << 30; // bytes of one buffer set (stored values, records)
+
+struct AdjPlan {
+	bool blocked;
+	int nseg, segPerBlock, nblocks, npg;
+	int ntx, nty, ntz, ntiles;
+	size_t maxRecords; // records of the largest block
+	// byte offsets into the scratch
+	size_t oCarry, oStates[2], oRec0[2], oSRec[2], oCounts[2], oTileStart[2], oWgStart[2], total;
+};
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+static AdjPlan adj_plan(const crp_adjoint_config *cfg, size_t n) {
+	AdjPlan P;
+	std::memset(&P, 0, sizeof(P));
+	const int K = cfg->checkpoint_every;
+	P.nseg = (cfg->nsteps + K - 1) / K;
+	P.npg = (int)((n + kAdjGroup - 1) / kAdjGroup);
+	P.ntx = (cfg->nx + kTileCells - 1) / kTileCells;
+	P.nty = (cfg->ny + kTileCells - 1) / kTileCells;
+	P.ntz = (cfg->nz + kTileCells - 1) / kTileCells;
+	const long long tiles = (long long)P.ntx * P.nty * P.ntz;
+	// per segment and buffer set: stored values (per padded lane and step) and both record copies (48 B)
+	const double perSeg = (double)K * ((double)P.npg * kAdjGroup * (kAdjStepBytes / 64) + (double)n * 48.0);
+	// CRP_ADJOINT_BLOCK_BYTES overrides the buffer-set budget (0: per-step atomics), so that tests
+	// can run the multi-block and the atomic paths on small inputs
+	double budget = (double)kAdjBlockBudget;
+	if (const char *e = std::getenv("CRP_ADJOINT_BLOCK_BYTES"))
+		budget = std::strtod(e, nullptr);
+	P.blocked = budget > 0 && cfg->nsteps > 0 && n > 0 && tiles <= kAdjMaxTiles && (double)K * n < 2e9;
+	if (!P.blocked) {
+		P.total = (size_t)K * 6 * n * sizeof(double);
+		return P;
+	}
+	P.ntiles = (int)tiles;
+	P.segPerBlock = (int)std::max(1.0, std::min((double)P.nseg, std::floor(budget / perSeg)));
+	while ((double)P.segPerBlock * K * n >= 2e9 && P.segPerBlock > 1)
+		P.segPerBlock--;
+	P.nblocks = (P.nseg + P.segPerBlock - 1) / P.segPerBlock;
+	P.segPerBlock = (P.nseg + P.nblocks - 1) / P.nblocks; // the same number of segments in every block
+	P.maxRecords = (size_t)P.segPerBlock * K * n;
+	size_t o = 0;
+	P.oCarry = o;      o += align256(6 * n * sizeof(double));
+	for (int k = 0; k < 2; k++) {
+		P.oStates[k] = o;     o += align256((size_t)P.segPerBlock * K * P.npg * (kAdjGroup / 64) * kAdjStepBytes);
+		P.oRec0[k] = o;       o += align256(P.maxRecords * sizeof(uint4));
+		P.oSRec[k] = o;       o += align256(P.maxRecords * sizeof(DepRec));
+		P.oCounts[k] = o;     o += align256((size_t)P.npg * P.ntiles * sizeof(uint32_t));
+		P.oTileStart[k] = o;  o += align256(((size_t)P.ntiles + 1) * sizeof(uint32_t));
+		P.oWgStart[k] = o;    o += align256(((size_t)P.ntiles + 1) * sizeof(uint32_t));
+	}
+	P.total = o;
+	return P;
+}
+
+// the second stream of the blocked backward, one per device, default priority
+static std::mutex g_adj_stream_mutex;
+static hipStream_t g_adj_stream[64];
+
+static hipError_t adj_side_stream(hipStream_t *out) {
+	int dev = 0;
+	hipError_t e = hipGetDevice(&dev);
+	if (e != hipSuccess)
+		return e;
+	if (dev < 0 || dev >= 64)
+		return hipErrorInvalidDevice;
+	std::lock_guard<std::mutex> lock(g_adj_stream_mutex);
+	if (!g_adj_stream[dev]) {
+		e = hipStreamCreateWithFlags(&g_adj_stream[dev], hipStreamNonBlocking);
+		if (e != hipSuccess)
+			return e;
+	}
+	*out = g_adj_stream[dev];
+	return hipSuccess;
+}
+
+// The forward keeps a cell table of the grid (k_build_cells: the 8 corner values of every cell in one
+// 128-byte line) after the checkpoints, for itself and the backward's recompute, when it takes at most
+// kAdjCellsMax bytes; larger grids are gathered point by point.
+constexpr size_t kAdjCellsMax = (size_t)8 << 30;
+
+static size_t adj_cells_bytes(const crp_adjoint_config *cfg) {
+	const size_t b = (size_t)cfg->nx * cfg->ny * cfg->nz * kCellFloats * sizeof(float);
+	return b <= kAdjCellsMax ? b : 0;
+}
+
+static size_t adj_checkpoint_bytes(const crp_adjoint_config *cfg, size_t n) {
+	const size_t nseg = ((size_t)cfg->nsteps + cfg->checkpoint_every - 1) / cfg->checkpoint_every;
+	return align256((nseg > 0 ? nseg : 1) * 6 * n * sizeof(double));
+}
+
 int crp_adjoint_workspace(const crp_adjoint_config *cfg, size_t n, size_t *checkpoint_bytes, size_t *scratch_bytes) {
 	AdjArgs A;
 	int rc = adj_args(cfg, n, A);
 	if (rc != CRP_OK)
 		return rc;
-	const size_t nseg = ((size_t)cfg->nsteps + cfg->checkpoint_every - 1) / cfg->checkpoint_every;
 	if (checkpoint_bytes)
-		*checkpoint_bytes = (nseg > 0 ? nseg : 1) * 6 * n * sizeof(double);
+		*checkpoint_bytes = adj_checkpoint_bytes(cfg, n) + adj_cells_bytes(cfg);
 	if (scratch_bytes)
-		*scratch_bytes = (size_t)cfg->checkpoint_every * 6 * n * sizeof(double);
+		*scratch_bytes = std::max<size_t>(adj_plan(cfg, n).total, 8);
 	return CRP_OK;
 }
 
@@ -2275,10 +2372,33 @@ int crp_adjoint_forward(const crp_adjoint_config *cfg, size_t n, const double *x
 	A.ckpt = checkpoints;
 	A.xout = x_out;
 	A.uout = u_out;
+	if (adj_cells_bytes(cfg)) {
+		float *cells = reinterpret_cast<float *>(reinterpret_cast<char *>(checkpoints) + adj_checkpoint_bytes(cfg, n));
+		const size_t ncell = (size_t)cfg->nx * cfg->ny * cfg->nz;
+		const unsigned blocks = (unsigned)std::min<size_t>((ncell + 255) / 256, 65535);
+		hipLaunchKernelGGL(k_build_cells, dim3(blocks), dim3(256), 0, (hipStream_t)stream, cfg->grid, cells, cfg->nx,
+		                   cfg->ny, cfg->nz);
+		A.G.cells = cells;
+	}
 	hipLaunchKernelGGL(k_adjoint_forward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, A);
 	CRP_HIP_TRY(hipGetLastError());
 	return CRP_OK;
 }
+
+// events of one blocked backward: released (after the work they mark has completed) when it returns
+struct AdjEvents {
+	std::vector<hipEvent_t> ev;
+	~AdjEvents() {
+		for (hipEvent_t e : ev)
+			(void)hipEventDestroy(e);
+	}
+	hipError_t make(hipEvent_t *out) {
+		hipError_t e = hipEventCreateWithFlags(out, hipEventDisableTiming);
+		if (e == hipSuccess)
+			ev.push_back(*out);
+		return e;
+	}
+};
 
 int crp_adjoint_backward(const crp_adjoint_config *cfg, size_t n, const double *kq, const double *checkpoints,
                          const double *grad_x_out, const double *grad_u_out, double *segment_scratch, double *grad_grid,
@@ -2300,8 +2420,108 @@ int crp_adjoint_backward(const crp_adjoint_config *cfg, size_t n, const double *
 	A.gradX0 = grad_x0;
 	A.gradU0 = grad_u0;
 	A.gradScale = grad_scale;
-	hipLaunchKernelGGL(k_adjoint_backward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, A);
+	if (adj_cells_bytes(cfg))
+		A.G.cells = reinterpret_cast<const float *>(reinterpret_cast<const char *>(checkpoints) + adj_checkpoint_bytes(cfg, n));
+	hipStream_t sa = (hipStream_t)stream;
+	const AdjPlan P = adj_plan(cfg, n);
+	if (!P.blocked) {
+		hipLaunchKernelGGL(k_adjoint_backward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, sa, A);
+		CRP_HIP_TRY(hipGetLastError());
+		return CRP_OK;
+	}
+	// stream A (the caller's) runs the reverse sweeps, block after block; stream B the recompute of
+	// the block after (in backward order) and the sort and accumulation of the block before
+	hipStream_t sb;
+	CRP_HIP_TRY(adj_side_stream(&sb));
+	AdjEvents E;
+	const int NB = P.nblocks;
+	std::vector<hipEvent_t> recomputed(NB), reversed(NB), sorted(NB);
+	for (int b = 0; b < NB; b++) {
+		CRP_HIP_TRY(E.make(&recomputed[b]));
+		CRP_HIP_TRY(E.make(&reversed[b]));
+		CRP_HIP_TRY(E.make(&sorted[b]));
+	}
+	hipEvent_t start, done;
+	CRP_HIP_TRY(E.make(&start));
+	CRP_HIP_TRY(E.make(&done));
+
+	char *base = reinterpret_cast<char *>(segment_scratch);
+	const int K = cfg->checkpoint_every;
+	auto block = [&](int b) {
+		AdjBlock Bk;
+		std::memset(&Bk, 0, sizeof(Bk));
+		const int buf = b & 1;
+		Bk.carry = reinterpret_cast<double *>(base + P.oCarry);
+		Bk.states = base + P.oStates[buf];
+		Bk.rec0 = reinterpret_cast<uint4 *>(base + P.oRec0[buf]);
+		Bk.srec = reinterpret_cast<DepRec *>(base + P.oSRec[buf]);
+		Bk.counts = reinterpret_cast<uint32_t *>(base + P.oCounts[buf]);
+		Bk.tileStart = reinterpret_cast<uint32_t *>(base + P.oTileStart[buf]);
+		Bk.wgStart = reinterpret_cast<uint32_t *>(base + P.oWgStart[buf]);
+		Bk.npg = P.npg;
+		Bk.ntx = P.ntx;
+		Bk.nty = P.nty;
+		Bk.ntz = P.ntz;
+		Bk.ntiles = P.ntiles;
+		Bk.seg0 = b * P.segPerBlock;
+		Bk.nseg = std::min(P.segPerBlock, P.nseg - Bk.seg0);
+		Bk.s0 = Bk.seg0 * K;
+		Bk.s1 = std::min(cfg->nsteps, (Bk.seg0 + Bk.nseg) * K);
+		Bk.first = b == NB - 1;
+		Bk.last = b == 0;
+		Bk.m = (size_t)(Bk.s1 - Bk.s0) * n;
+		return Bk;
+	};
+	const size_t T = (size_t)P.ntiles;
+	// recompute of block b with the first half of its records, their counts per particle group and
+	// tile, and the scans
+	auto recompute = [&](int b) {
+		const AdjBlock Bk = block(b);
+		const size_t nc = (size_t)Bk.npg * T;
+		hipLaunchKernelGGL(k_dep_zero, dim3((unsigned)((nc + kAdjGroup - 1) / kAdjGroup)), dim3(kAdjGroup), 0, sb, Bk);
+		hipLaunchKernelGGL(k_adj_recompute, dim3((unsigned)((size_t)Bk.nseg * Bk.npg)), dim3(kAdjGroup),
+		                   T * sizeof(uint32_t), sb, A, Bk);
+		hipLaunchKernelGGL(k_dep_scan_chunks, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, sb, Bk);
+		hipLaunchKernelGGL(k_dep_scan_tiles, dim3(1), dim3(1024), 0, sb, Bk);
+	};
+	auto accumulate = [&](int b) {
+		const AdjBlock Bk = block(b);
+		const size_t groups = T + (Bk.m + kDepPerGroup - 1) / kDepPerGroup;
+		hipLaunchKernelGGL(k_dep_tiles, dim3((unsigned)groups), dim3(kDepTileThreads), 0, sb, A, Bk);
+	};
+
+	// stream B starts after the work already queued on stream A (e.g. the zeroed gradients)
+	CRP_HIP_TRY(hipEventRecord(start, sa));
+	CRP_HIP_TRY(hipStreamWaitEvent(sb, start, 0));
+	recompute(NB - 1);
+	CRP_HIP_TRY(hipEventRecord(recomputed[NB - 1], sb));
+	for (int b = NB - 1; b >= 0; b--) {
+		// A: reverse sweep of block b once its stored values and counts are there and the accumulation
+		// of block b + 2 (the same record buffer) has read its records
+		CRP_HIP_TRY(hipStreamWaitEvent(sa, recomputed[b], 0));
+		if (b + 2 < NB)
+			CRP_HIP_TRY(hipStreamWaitEvent(sa, sorted[b + 2], 0));
+		{
+			const AdjBlock Bk = block(b);
+			hipLaunchKernelGGL(k_adj_reverse, dim3((unsigned)Bk.npg), dim3(kAdjGroup), 2 * T * sizeof(uint32_t), sa, A, Bk);
+		}
+		CRP_HIP_TRY(hipEventRecord(reversed[b], sa));
+		// B: recompute of block b - 1 into the buffers that the reverse sweep of block b + 1 has
+		// finished with, then the accumulation of block b
+		if (b > 0) {
+			if (b + 1 < NB)
+				CRP_HIP_TRY(hipStreamWaitEvent(sb, reversed[b + 1], 0));
+			recompute(b - 1);
+			CRP_HIP_TRY(hipEventRecord(recomputed[b - 1], sb));
+		}
+		CRP_HIP_TRY(hipStreamWaitEvent(sb, reversed[b], 0));
+		accumulate(b);
+		CRP_HIP_TRY(hipEventRecord(sorted[b], sb));
+	}
 	CRP_HIP_TRY(hipGetLastError());
+	// stream A continues once the last accumulation is in the gradient
+	CRP_HIP_TRY(hipEventRecord(done, sb));
+	CRP_HIP_TRY(hipStreamWaitEvent(sa, done, 0));
 	return CRP_OK;
 }
 

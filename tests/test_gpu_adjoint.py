@@ -165,3 +165,21 @@ def test_adjoint_rejects_bad_input():
         boris_grid_propagate(g, x0, u0, 1 * O.EeV, O.eplus, [0, 0, 0], [s["sp"]] * 3, s["h"], 10, 0)
     with pytest.raises(ValueError):
         boris_grid_propagate(g.double(), x0, u0, 1 * O.EeV, O.eplus, [0, 0, 0], [s["sp"]] * 3, s["h"], 10, 4)
+
+
+@pytest.mark.parametrize("budget", [None, "3e5", "0"])
+def test_adjoint_backward_paths(budget, monkeypatch):
+    """The backward's three schedules against the oracle on a 20^3 grid (tiles cut by the grid's end):
+    all segments in one block (default), several blocks of segments with the adjoint carried between
+    them (small per-block budget), and per-step atomics (budget 0)."""
+    if budget is not None:
+        monkeypatch.setenv("CRP_ADJOINT_BLOCK_BYTES", budget)
+    s = setup(n=96, N=20, nsteps=130, K=8, seed=9)
+    ref = oracle_grad(s)
+    got = gpu_run(s)
+    rel = np.abs(got[3] - ref[3]).max() / np.abs(ref[3]).max()
+    print("budget %s: grid-gradient max rel %.2e" % (budget, rel))
+    assert rel < 1e-6
+    assert np.allclose(got[4], ref[4], rtol=1e-7, atol=1e-8 * np.abs(ref[4]).max())
+    assert np.allclose(got[5], ref[5], rtol=1e-7, atol=1e-8 * np.abs(ref[5]).max())
+    assert got[6] == pytest.approx(ref[6], rel=1e-8)
