@@ -233,9 +233,10 @@ def cpu_baseline(w, fdata, pos, d, E, q, gpu_final=None, target_s=15.0):
 
 ADJOINT = dict(n=65536, grid_n=128, spacing=20 * kpc, lmin_cells=2, lmax_cells=128, seed=5, step=10 * kpc,
                steps=4096, K=64, emin=1 * EeV, emax=100 * EeV)
-# per reverse step: recompute gather (96 B) + reverse gather (96 B) + 24 FP64 atomic read-modify-writes (384 B)
-# + state to and from the per-particle scratch (2 x 48 B)
-ADJ_BYTES_PER_STEP = 96 + 96 + 24 * 16 + 96
+# per step of the backward: recompute (the cell's 128-byte corner line, 120 B of stored u, T and dT/df,
+# the 16-byte first half of the deposit record), reverse (those 136 B read back, the 32-byte record
+# written into its tile's range), accumulation (the record read)
+ADJ_BYTES_PER_STEP = (128 + 120 + 16) + (136 + 32) + 32
 
 
 def adjoint_inputs(a, n):
@@ -324,20 +325,8 @@ def adjoint_measure(args, steps, warmup):
     fwd = sum(e[0].elapsed_time(e[1]) for e in evs) / steps
     bwd = sum(e[1].elapsed_time(e[2]) for e in evs) / steps
     nsteps = n * a["steps"]
-    return {"workload": "adjoint_grid128", "particles": n, "propagation_steps": a["steps"], "step_m": a["step"],
-            "checkpoint_every": a["K"], "grid": "%d^3 FP32, spacing %.0f kpc" % (a["grid_n"], a["spacing"] / kpc),
-            "forward_ms": fwd, "backward_ms": bwd, "adjoint_over_forward": bwd / fwd,
-            "gradient_over_forward": (fwd + bwd) / fwd, "value": nsteps * steps / dt,
-            "unit": "particle-steps/s (loss and gradient)", "ms_per_step": dt / steps * 1e3,
-            "inputs": (a, grid_np, pos, d, E, tgt)}
-
-
-def bench_adjoint(args):
-    """The adjoint workload alone; value = differentiated particle-steps per second."""
-    r = adjoint_measure(args, args.steps, args.warmup)
-    a, grid_np, pos, d, E, tgt = r.pop("inputs")
-    n = r["particles"]
-    # the product path's forward on the same particles (FP32 trilinear, fixed-step BP), for comparison
+    # the product path's forward on the same particles (PropagationBP, FP64, fixed step, FP32 trilinear
+    # grid in the cell layout), for a second denominator
     from crpropa3_amd import capi
     f = capi.Field.grid3f(grid_np, [0, 0, 0], [a["spacing"]] * 3)
     prop = capi.Propagator("BP", f, 0.42, a["step"], a["step"], precision=capi.PRECISION_FP64)
@@ -347,12 +336,26 @@ def bench_adjoint(args):
     work.copy_from(init)
     prop.propagate(work, max_steps=a["steps"])
     work.copy_from(init)
-    st = prop.propagate(work, max_steps=a["steps"])
+    bp_ms = prop.propagate(work, max_steps=a["steps"]).kernel_ms
+    return {"workload": "adjoint_grid128", "particles": n, "propagation_steps": a["steps"], "step_m": a["step"],
+            "checkpoint_every": a["K"], "grid": "%d^3 FP32, spacing %.0f kpc" % (a["grid_n"], a["spacing"] / kpc),
+            "forward_ms": fwd, "backward_ms": bwd, "adjoint_over_forward": bwd / fwd,
+            "gradient_over_forward": (fwd + bwd) / fwd, "propagate_bp_fp64_ms": bp_ms,
+            "adjoint_over_propagate_bp": bwd / bp_ms, "value": nsteps * steps / dt,
+            "unit": "particle-steps/s (loss and gradient)", "ms_per_step": dt / steps * 1e3,
+            "inputs": (a, grid_np, pos, d, E, tgt)}
+
+
+def bench_adjoint(args):
+    """The adjoint workload alone; value = differentiated particle-steps per second."""
+    r = adjoint_measure(args, args.steps, args.warmup)
+    a, grid_np, pos, d, E, tgt = r.pop("inputs")
+    n = r["particles"]
     bytes_per_launch = n * a["steps"] * ADJ_BYTES_PER_STEP
     achieved = bytes_per_launch / (r["backward_ms"] / 1e3) / 1e9
     config = {k: r[k] for k in ("workload", "particles", "propagation_steps", "step_m", "checkpoint_every", "grid",
-                                "forward_ms", "backward_ms", "adjoint_over_forward", "gradient_over_forward")}
-    config["propagate_bp_fp64_ms"] = st.kernel_ms
+                                "forward_ms", "backward_ms", "adjoint_over_forward", "gradient_over_forward",
+                                "propagate_bp_fp64_ms", "adjoint_over_propagate_bp")}
     out = {
         "metric": "particle-steps/s (1 MI355X), FP64 vs mixed at stated error; adjoint/fwd time",
         "value": r["value"], "unit": r["unit"],
@@ -361,7 +364,8 @@ def bench_adjoint(args):
         "data": "synthetic: seeded turbulent field, particles and target directions",
         "config": config,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_adjoint_backward",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic("adjoint_grid128", "fp64"),
+                     "kernel": "backward: k_adj_recompute + k_adj_reverse + k_dep_tiles on three streams",
                      "kernel_ms_avg": r["backward_ms"], "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": None,
     }
@@ -602,7 +606,8 @@ def main():
         a = adjoint_measure(args, max(1, min(args.steps, 5)), max(1, min(args.warmup, 2)))
         out["adjoint"] = {k: a[k] for k in ("workload", "particles", "propagation_steps", "checkpoint_every",
                                             "forward_ms", "backward_ms", "adjoint_over_forward",
-                                            "gradient_over_forward", "value", "unit")}
+                                            "gradient_over_forward", "propagate_bp_fp64_ms",
+                                            "adjoint_over_propagate_bp", "value", "unit")}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, fdata, pos, d, E, q, gpu_final=final)
     print(json.dumps(out), flush=True)

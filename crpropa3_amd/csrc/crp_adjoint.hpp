@@ -286,26 +286,27 @@ __global__ void __launch_bounds__(256) k_adjoint_backward(AdjArgs A) {
 //   1. recomputes a block of segments of every particle at once (segments x n independent lanes)
 //      and stores, for every step, what its reverse needs: the direction u before the step, the
 //      interpolated field T and its derivatives with respect to the cell fractions (15 doubles),
-//      wave by wave, and the first half of the step's deposit record (the 16^3-cell tile, the cell in
+//      wave by wave, and the first half of the step's deposit record (the 8^3-cell tile, the cell in
 //      it and the fractions); each workgroup also counts its records per tile;
 //   2. scans the counts into every tile's start and every particle group's place within each tile;
 //   3. runs the reverse steps of the block per particle (one lane per particle, the only sequential
 //      part: no gathers, the stored values of the next two steps loaded ahead) and writes each
 //      step's complete 32-byte record (with the adjoint aT of T: the gradient of grid value G_c is
 //      w_c aT) into its tile's range;
-//   4. accumulates each tile's records into its 17^3 grid points in LDS (FP64), one workgroup per
+//   4. accumulates each tile's records into its 9^3 grid points in LDS (FP64), one workgroup per
 //      kDepPerGroup records of a tile, and adds the points into the gradient.
 // The host runs the reverse sweeps on the caller's stream and the recompute of the next block and the
 // accumulation of the previous one on a second stream beside it.  The arithmetic of a step is that of
 // adj_step / adj_step_reverse, except that the position gradient is taken as aT . dT/df; the deposits
 // are summed in another order.
-constexpr int kTileShift = 4, kTileCells = 1 << kTileShift, kTilePts = kTileCells + 1;
+constexpr int kTileShift = 3, kTileCells = 1 << kTileShift, kTilePts = kTileCells + 1;
 constexpr int kTilePtsAll = kTilePts * kTilePts * kTilePts;
 constexpr int kCellBits = 3 * kTileShift;
 constexpr int kAdjGroup = 256;       // particles per recompute / reverse workgroup
 constexpr int kDepPerGroup = 65536;  // records per accumulating workgroup
-constexpr int kDepTileThreads = 1024;
+constexpr int kDepTileThreads = 256;
 constexpr int kDepBatch = 4;
+constexpr int kAdjBurst = 4;         // reverse steps whose records are stored together
 // stored per step and wave: u, T, dT/dfx, dT/dfy, dT/dfz (15 x 64 doubles)
 constexpr int kAdjStored = 15;
 constexpr int kAdjStepBytes = kAdjStored * 64 * 8;
@@ -489,7 +490,7 @@ __global__ void __launch_bounds__(1024) k_dep_scan_tiles(AdjBlock Bk) {
 // the values one reverse step reads
 struct AdjStored {
 	double d[kAdjStored]; // u, T, dT/dfx, dT/dfy, dT/dfz
-	uint4 rec;            // key, fractions
+	uint32_t rec[4];      // key, fractions (plain words: a vector-type member sends copies through memory)
 };
 
 __device__ __forceinline__ void adj_load_stored(const AdjBlock &Bk, int K, size_t n, size_t i, int q, AdjStored &s) {
@@ -498,7 +499,11 @@ __device__ __forceinline__ void adj_load_stored(const AdjBlock &Bk, int K, size_
 #pragma unroll
 	for (int k = 0; k < kAdjStored; k++)
 		s.d[k] = __builtin_nontemporal_load(pd + 64 * k);
-	s.rec = Bk.rec0[(size_t)q * n + i];
+	const uint4 r = Bk.rec0[(size_t)q * n + i];
+	s.rec[0] = r.x;
+	s.rec[1] = r.y;
+	s.rec[2] = r.z;
+	s.rec[3] = r.w;
 }
 
 // adj_step_reverse from the stored u, T and dT/df: (ax, au) the adjoint after the step on entry, before
@@ -541,7 +546,8 @@ __device__ __forceinline__ V3 adj_reverse_stored(const AdjGrid &G, double h, dou
 
 // One lane per particle, kAdjGroup particles per workgroup; each step's record goes to its tile's range
 // at the group's offset (counts, from the scans) plus its rank among the group's records of the tile.
-__global__ void __launch_bounds__(kAdjGroup) k_adj_reverse(AdjArgs A, AdjBlock Bk) {
+// one wave per SIMD (n / 64 waves in all): the kernel may use the whole register file
+__global__ void __launch_bounds__(kAdjGroup) __attribute__((amdgpu_waves_per_eu(1, 1))) k_adj_reverse(AdjArgs A, AdjBlock Bk) {
 	extern __shared__ uint32_t lds[];
 	const int T = Bk.ntiles;
 	uint32_t *base = lds, *rank = lds + T;
@@ -567,22 +573,41 @@ __global__ void __launch_bounds__(kAdjGroup) k_adj_reverse(AdjArgs A, AdjBlock B
 		}
 		const double kq = A.kq[i];
 		// the stored values of step s, s - 1 and s - 2 (loaded two steps ahead)
+		// (near the block's first step the loads ahead repeat step 0: in bounds, and never used)
 		AdjStored d0, d1, d2;
 		int q = Bk.s1 - 1 - Bk.s0;
 		adj_load_stored(Bk, K, n, i, q, d0);
-		if (q >= 1)
-			adj_load_stored(Bk, K, n, i, q - 1, d1);
-		for (; q >= 0; q--) {
-			if (q >= 2)
-				adj_load_stored(Bk, K, n, i, q - 2, d2);
-			const V3 aT = adj_reverse_stored(A.G, A.h, kq, d0, ax, au, gs);
-			const uint32_t t = d0.rec.x >> kCellBits;
-			const size_t dst = (size_t)base[t] + atomicAdd(&rank[t], 1u);
-			uint4 *pr = reinterpret_cast<uint4 *>(Bk.srec + dst);
-			pr[0] = d0.rec;
-			pr[1] = make_uint4(__float_as_uint((float)aT.x), __float_as_uint((float)aT.y), __float_as_uint((float)aT.z), 0u);
-			d0 = d1;
-			d1 = d2;
+		adj_load_stored(Bk, K, n, i, max(q - 1, 0), d1);
+		// records are held kAdjBurst steps in registers and then stored together, so that each tile's
+		// run of records is filled within a short time
+		while (q >= 0) {
+			const int cnt = min(kAdjBurst, q + 1);
+			uint4 held[kAdjBurst][2];
+			uint32_t heldDst[kAdjBurst];
+#pragma unroll
+			for (int k = 0; k < kAdjBurst; k++) {
+				if (k < cnt) {
+					const int qq = q - k;
+					adj_load_stored(Bk, K, n, i, max(qq - 2, 0), d2);
+					const V3 aT = adj_reverse_stored(A.G, A.h, kq, d0, ax, au, gs);
+					const uint32_t t = d0.rec[0] >> kCellBits;
+					heldDst[k] = base[t] + atomicAdd(&rank[t], 1u);
+					held[k][0] = make_uint4(d0.rec[0], d0.rec[1], d0.rec[2], d0.rec[3]);
+					held[k][1] = make_uint4(__float_as_uint((float)aT.x), __float_as_uint((float)aT.y),
+					                        __float_as_uint((float)aT.z), 0u);
+					d0 = d1;
+					d1 = d2;
+				}
+			}
+#pragma unroll
+			for (int k = 0; k < kAdjBurst; k++) {
+				if (k < cnt) {
+					uint4 *pr = reinterpret_cast<uint4 *>(Bk.srec + heldDst[k]);
+					pr[0] = held[k][0];
+					pr[1] = held[k][1];
+				}
+			}
+			q -= cnt;
 		}
 		if (Bk.last) {
 			A.gradX0[3 * i] = ax.x; A.gradX0[3 * i + 1] = ax.y; A.gradX0[3 * i + 2] = ax.z;
@@ -599,7 +624,7 @@ __global__ void __launch_bounds__(kAdjGroup) k_adj_reverse(AdjArgs A, AdjBlock B
 		unsafeAtomicAdd(A.gradScale, gs);
 }
 
-// accumulation: kDepPerGroup records of one tile per workgroup, summed into the tile's 17^3 points in
+// accumulation: kDepPerGroup records of one tile per workgroup, summed into the tile's 9^3 points in
 // LDS (FP64), then added into the grid gradient (points on tile faces are shared with the
 // neighbouring tiles, and a tile with many records with the tile's other workgroups).  The grid
 // is launched with ntiles + ceil(m / kDepPerGroup) workgroups, an upper bound of wgStart[ntiles].

@@ -2250,7 +2250,7 @@ static int adj_args(const crp_adjoint_config *cfg, size_t n, AdjArgs &A) {
 // Plan of the blocked backward (crp_adjoint.hpp): segments per block, deposit tiles and the scratch
 // layout.  Every per-block buffer is doubled so that the reverse sweep of one block overlaps the
 // recompute of the next and the accumulation of the previous one.  Grids with more than kAdjMaxTiles
-// tiles of 16^3 cells use the checkpointed backward with direct atomics.
+// tiles of 8^3 cells use the checkpointed backward with direct atomics.
 constexpr int kAdjMaxTiles = 8192;
 constexpr size_t kAdjBlockBudget = (size_t)4 << 30; // bytes of one buffer set (stored values, records)
 
@@ -2308,11 +2308,11 @@ static AdjPlan adj_plan(const crp_adjoint_config *cfg, size_t n) {
 	return P;
 }
 
-// the second stream of the blocked backward, one per device, default priority
+// the two side streams of the blocked backward, per device, default priority
 static std::mutex g_adj_stream_mutex;
-static hipStream_t g_adj_stream[64];
+static hipStream_t g_adj_stream[64][2];
 
-static hipError_t adj_side_stream(hipStream_t *out) {
+static hipError_t adj_side_streams(hipStream_t *b, hipStream_t *c) {
 	int dev = 0;
 	hipError_t e = hipGetDevice(&dev);
 	if (e != hipSuccess)
@@ -2320,12 +2320,15 @@ static hipError_t adj_side_stream(hipStream_t *out) {
 	if (dev < 0 || dev >= 64)
 		return hipErrorInvalidDevice;
 	std::lock_guard<std::mutex> lock(g_adj_stream_mutex);
-	if (!g_adj_stream[dev]) {
-		e = hipStreamCreateWithFlags(&g_adj_stream[dev], hipStreamNonBlocking);
-		if (e != hipSuccess)
-			return e;
+	for (int k = 0; k < 2; k++) {
+		if (!g_adj_stream[dev][k]) {
+			e = hipStreamCreateWithFlags(&g_adj_stream[dev][k], hipStreamNonBlocking);
+			if (e != hipSuccess)
+				return e;
+		}
 	}
-	*out = g_adj_stream[dev];
+	*b = g_adj_stream[dev][0];
+	*c = g_adj_stream[dev][1];
 	return hipSuccess;
 }
 
@@ -2430,9 +2433,10 @@ int crp_adjoint_backward(const crp_adjoint_config *cfg, size_t n, const double *
 		return CRP_OK;
 	}
 	// stream A (the caller's) runs the reverse sweeps, block after block; stream B the recompute of
-	// the block after (in backward order) and the sort and accumulation of the block before
-	hipStream_t sb;
-	CRP_HIP_TRY(adj_side_stream(&sb));
+	// the block after (in backward order), stream C the accumulation of the block before: the three
+	// are bound by latency, memory traffic and LDS atomics respectively
+	hipStream_t sb, sc;
+	CRP_HIP_TRY(adj_side_streams(&sb, &sc));
 	AdjEvents E;
 	const int NB = P.nblocks;
 	std::vector<hipEvent_t> recomputed(NB), reversed(NB), sorted(NB);
@@ -2487,12 +2491,13 @@ int crp_adjoint_backward(const crp_adjoint_config *cfg, size_t n, const double *
 	auto accumulate = [&](int b) {
 		const AdjBlock Bk = block(b);
 		const size_t groups = T + (Bk.m + kDepPerGroup - 1) / kDepPerGroup;
-		hipLaunchKernelGGL(k_dep_tiles, dim3((unsigned)groups), dim3(kDepTileThreads), 0, sb, A, Bk);
+		hipLaunchKernelGGL(k_dep_tiles, dim3((unsigned)groups), dim3(kDepTileThreads), 0, sc, A, Bk);
 	};
 
-	// stream B starts after the work already queued on stream A (e.g. the zeroed gradients)
+	// streams B and C start after the work already queued on stream A (e.g. the zeroed gradients)
 	CRP_HIP_TRY(hipEventRecord(start, sa));
 	CRP_HIP_TRY(hipStreamWaitEvent(sb, start, 0));
+	CRP_HIP_TRY(hipStreamWaitEvent(sc, start, 0));
 	recompute(NB - 1);
 	CRP_HIP_TRY(hipEventRecord(recomputed[NB - 1], sb));
 	for (int b = NB - 1; b >= 0; b--) {
@@ -2507,20 +2512,23 @@ int crp_adjoint_backward(const crp_adjoint_config *cfg, size_t n, const double *
 		}
 		CRP_HIP_TRY(hipEventRecord(reversed[b], sa));
 		// B: recompute of block b - 1 into the buffers that the reverse sweep of block b + 1 has
-		// finished with, then the accumulation of block b
+		// finished with (its counts were last read by the accumulation of block b + 1)
 		if (b > 0) {
-			if (b + 1 < NB)
+			if (b + 1 < NB) {
 				CRP_HIP_TRY(hipStreamWaitEvent(sb, reversed[b + 1], 0));
+				CRP_HIP_TRY(hipStreamWaitEvent(sb, sorted[b + 1], 0));
+			}
 			recompute(b - 1);
 			CRP_HIP_TRY(hipEventRecord(recomputed[b - 1], sb));
 		}
-		CRP_HIP_TRY(hipStreamWaitEvent(sb, reversed[b], 0));
+		// C: accumulation of block b
+		CRP_HIP_TRY(hipStreamWaitEvent(sc, reversed[b], 0));
 		accumulate(b);
-		CRP_HIP_TRY(hipEventRecord(sorted[b], sb));
+		CRP_HIP_TRY(hipEventRecord(sorted[b], sc));
 	}
 	CRP_HIP_TRY(hipGetLastError());
 	// stream A continues once the last accumulation is in the gradient
-	CRP_HIP_TRY(hipEventRecord(done, sb));
+	CRP_HIP_TRY(hipEventRecord(done, sc));
 	CRP_HIP_TRY(hipStreamWaitEvent(sa, done, 0));
 	return CRP_OK;
 }
