@@ -482,6 +482,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision block of the default workload")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint block of the default workload")
+    ap.add_argument("--autotune", type=int, default=0, metavar="PROBE_STEPS",
+                    help="run crp_propagator_autotune with this many probe steps before timing, and report its choice")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it (got --gpus %d)"
@@ -522,6 +524,12 @@ def main():
     init.upload(pos, d, E, q)
     work = capi.Batch(n)
     max_steps = w.get("steps", -1)
+    tuned = None
+    if args.autotune > 0:
+        t0 = time.perf_counter()
+        tuned = prop.autotune(init, args.autotune)
+        tuned["autotune_s"] = time.perf_counter() - t0
+        tuned["probe_steps"] = args.autotune
 
     acc, evals, kms, launches, dt = timed_passes(capi, prop, init, work, max_steps, args.steps, args.warmup)
     final = work.download()
@@ -536,6 +544,9 @@ def main():
                        "scale": w["scale"], "field_evals_per_pass": evals // args.steps})
     else:
         config["trials_per_pass"] = (evals // per_trial) // args.steps
+    config["launch"] = prop.launch()
+    if tuned is not None:
+        config["autotuned"] = tuned
     if w["field"] == "grid":
         fp16 = args.grid_storage == "fp16"
         config.update({"grid": "%d^3 %s, spacing %.0f kpc" % (w["grid_n"], "FP16+brick scale" if fp16 else "FP32",
