@@ -2044,6 +2044,10 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 		for (int a = 0; a < 4; a++)
 			P.sdeAdvPar[a] = p->sdeAdvPar[a];
 		P.sdeSeed = p->sdeSeed;
+		// CRP_SDE_MAX_HALVINGS (1..20) lowers the sub-step limit, so that tests can reach it on small inputs
+		P.sdeMaxHalvings = kSdeMaxHalvings;
+		if (const char *e = std::getenv("CRP_SDE_MAX_HALVINGS"))
+			P.sdeMaxHalvings = std::max(1, std::min(kSdeMaxHalvings, std::atoi(e)));
 	}
 	if (p->nObs > 0) {
 		int rc0 = ensure_snapshots(b, p->nObs);
@@ -2091,7 +2095,7 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 		*stats = st;
 	if (h[CTR_SDE_LIMIT] > 0)
 		return fail(CRP_E_STATE, "DiffusionSDE: " + std::to_string(h[CTR_SDE_LIMIT]) +
-		                             " particle(s) needed more than 2^" + std::to_string(kSdeMaxHalvings) +
+		                             " particle(s) needed more than 2^" + std::to_string(P.sdeMaxHalvings) +
 		                             " sub-steps in one step (minStep far below the field's scale) and were stopped");
 	return CRP_OK;
 }

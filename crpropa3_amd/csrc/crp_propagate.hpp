@@ -67,6 +67,7 @@ struct Params {
 	double sdeScale, sdeAlpha, sdeEpsilon, sdeAdv[3], sdeAdvO[3], sdeAdvPar[4];
 	int sdeAdvection;
 	uint32_t sdeSeed;
+	int sdeMaxHalvings; // trials beyond this many halvings stop the particle (CTR_SDE_LIMIT)
 };
 
 __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; } // std::max

@@ -32,7 +32,8 @@ namespace crp {
 // include/crpropa/Units.h: au, parsec = 648000 / pi * au, kpc = 1e3 parsec
 constexpr double kKpc = 1e3 * (648000 / 3.14159265358979323846 * 149597870700.0);
 constexpr double kInvKpc = 1.0 / kKpc;
-// a step whose trials end past 2^kSdeMaxHalvings sub-steps stops the particle (CTR_SDE_LIMIT)
+// a step whose trials end past 2^kSdeMaxHalvings sub-steps stops the particle (CTR_SDE_LIMIT);
+// Params::sdeMaxHalvings holds the limit of a run (this value unless a test lowers it)
 constexpr int kSdeMaxHalvings = 20;
 
 // B / |B| * c_light (Vector3::getUnitVector, then the product)
@@ -311,8 +312,8 @@ __global__ void __launch_bounds__(256) k_sde(F field0, Params P, Soa S, unsigned
 				propTime *= 0.5;
 				counter += 1;
 				const bool more = rr > 1 && __builtin_fabs(propTime) >= minTime;
-				if (more && counter > kSdeMaxHalvings) {
-					// another trial would need more than 2^kSdeMaxHalvings sub-steps: stop the particle
+				if (more && counter > P.sdeMaxHalvings) {
+					// another trial would need more than 2^sdeMaxHalvings sub-steps: stop the particle
 					// without moving it and report it (CTR_SDE_LIMIT)
 					nLimit++;
 					alive = false;

@@ -1,7 +1,9 @@
-// AdvectionField and UniformAdvectionField with the reference's interface
-// (include/crpropa/advectionField/AdvectionField.h:20-58,
-// src/advectionField/AdvectionField.cpp:33-50).  DiffusionSDE runs on the GPU
-// with a UniformAdvectionField (or none); other advection fields are host-only.
+// AdvectionField, UniformAdvectionField, ConstantSphericalAdvectionField and
+// SphericalAdvectionField with the reference's interface
+// (include/crpropa/advectionField/AdvectionField.h:23-135,
+// src/advectionField/AdvectionField.cpp:33-170).  DiffusionSDE runs on the GPU
+// with any of these three (or none); other advection fields raise
+// std::runtime_error when a DiffusionSDE that uses them is run.
 #ifndef CRPROPA_AMD_ADVECTIONFIELD_H
 #define CRPROPA_AMD_ADVECTIONFIELD_H
 

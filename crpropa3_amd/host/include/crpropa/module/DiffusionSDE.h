@@ -11,6 +11,11 @@
 // reproducible; setSeed fixes it).  The candidate keeps its step count
 // (Candidate::getDiffusionSteps), so the numbers do not depend on how a run is
 // split into calls or batches.
+//
+// A step whose trials would need more than 2^20 sub-steps (minStep far below the
+// field's scale, where the reference's loop would run for minutes) stops that
+// candidate where it is (inactive); process() and ModuleList::run raise
+// std::runtime_error after every candidate's state has been updated.
 #ifndef CRPROPA_AMD_DIFFUSIONSDE_H
 #define CRPROPA_AMD_DIFFUSIONSDE_H
 

@@ -17,6 +17,7 @@
 #include <memory>
 #include <sstream>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 namespace crpropa {
@@ -24,6 +25,18 @@ namespace crpropa {
 static void throwIf(int rc) {
 	if (rc != CRP_OK)
 		throw std::runtime_error(crp_last_error());
+}
+
+// crp_propagate, except that CRP_E_STATE after a completed run (DiffusionSDE particles stopped at the
+// sub-step limit; the others have moved on) is returned in `deferred`: the caller downloads and unpacks
+// the batch first, so that every candidate keeps its progress, and raises afterwards.
+static void propagateKeeping(crp_propagator *p, crp_batch *b, long long steps, crp_run_stats *st,
+                             std::string &deferred) {
+	const int rc = crp_propagate(p, b, steps, st);
+	if (rc == CRP_E_STATE)
+		deferred = crp_last_error();
+	else
+		throwIf(rc);
 }
 
 struct PropagatorDeleter {
@@ -219,11 +232,14 @@ void DevicePropagator::process(Candidate *candidate) const {
 	pk.active[0] = 1;
 	crp_host_state s = pk.view();
 	throwIf(crp_batch_upload(b, 1, &s));
-	throwIf(crp_propagate(p.get(), b, 1, nullptr));
+	std::string deferred;
+	propagateKeeping(p.get(), b, 1, nullptr, deferred);
 	throwIf(crp_batch_download(b, &s));
 	unpack(one, pk, std::vector<char>(1, 1));
 	if (!pk.active[0])
 		one[0]->setActive(false);
+	if (!deferred.empty())
+		throw std::runtime_error(deferred);
 }
 
 PropagationBP::PropagationBP(ref_ptr<MagneticField> f, double fixedStep) : DevicePropagator(BP) {
@@ -623,7 +639,8 @@ bool ModuleList::runBatched(const candidate_vector_t &cands) {
 	throwIf(crp_batch_upload(b, act.size(), &s));
 	prop->tuneOn(p.get(), b);
 	crp_run_stats st;
-	throwIf(crp_propagate(p.get(), b, steps, &st));
+	std::string deferred;
+	propagateKeeping(p.get(), b, steps, &st, deferred);
 	throwIf(crp_batch_download(b, &s));
 	std::vector<double> snaps;
 	if (f.ote) {
@@ -696,6 +713,8 @@ bool ModuleList::runBatched(const candidate_vector_t &cands) {
 	}
 	lastAccepted = st.accepted_steps;
 	lastKernel = st.kernel_ms;
+	if (!deferred.empty())
+		throw std::runtime_error(deferred);
 	return true;
 }
 
@@ -724,12 +743,15 @@ void ModuleList::runStepwise(const candidate_vector_t &cands) {
 		pack(act, pk, prop->getKind() == DevicePropagator::SDE);
 		crp_host_state s = pk.view();
 		throwIf(crp_batch_upload(b, act.size(), &s));
-		throwIf(crp_propagate(p.get(), b, 1, nullptr));
+		std::string deferred;
+		propagateKeeping(p.get(), b, 1, nullptr, deferred);
 		throwIf(crp_batch_download(b, &s));
 		unpack(act, pk, std::vector<char>(act.size(), 1));
 		for (Candidate *c : act)
 			for (auto m = std::next(modules.begin()); m != modules.end(); ++m)
 				(*m)->process(c);
+		if (!deferred.empty())
+			throw std::runtime_error(deferred);
 	}
 }
 

@@ -191,7 +191,9 @@ int crp_propagator_create_ck(const crp_field_t *field, double tolerance, double 
  * numbers: Philox4x32-10 with key (seed, 0x43525053) and counter (stream low word, step, block,
  * stream high word) per particle (crp_host_state.sde_step / sde_stream).  A step whose trials
  * would need more than 2^20 sub-steps (minStep far below the field's scale) stops that particle
- * where it is and crp_propagate returns CRP_E_STATE. */
+ * where it is (inactive; its step count advances) and crp_propagate returns CRP_E_STATE after the
+ * run, with every other particle's progress in the batch.  The environment variable
+ * CRP_SDE_MAX_HALVINGS (1..20), read at each call, lowers the limit to 2^value sub-steps (tests). */
 int crp_propagator_create_sde(const crp_field_t *field, double tolerance, double min_step, double max_step,
                               double epsilon, crp_propagator_t **out);
 /* setAlpha / setScale / setEpsilon (src/module/DiffusionSDE.cpp:294-314) and the random seed */

@@ -202,3 +202,49 @@ def test_modulelist_sde_with_box_sphere_observer_fused_equals_stepwise():
                      for c in b]
     assert len(det_a) > 20 and det_a == det_b
     assert any(not c.isActive() and c.getTrajectoryLength() < 5 * crp.kpc for c in a)  # the sphere rejected some
+
+
+def test_substep_limit_keeps_every_candidates_progress(monkeypatch):
+    """A DiffusionSDE step whose trials would need more sub-steps than the limit stops its candidate
+    where it is (inactive) and the call raises, after every candidate's device state is back on the
+    host.  The limit (2^20 sub-steps in production) is lowered to 2^6 from the host
+    (CRP_SDE_MAX_HALVINGS) so that a 10 pc first step in a 1 pc white-noise grid at a 1e-8 m tolerance
+    (about 2^15-2^17 sub-steps without the limit) reaches it, while a candidate starting with 1e-12 pc
+    steps (one sub-step each) runs to its length limit untouched."""
+    monkeypatch.setenv("CRP_SDE_MAX_HALVINGS", "6")
+    rng = np.random.default_rng(4)
+    g = crp.Grid3f(crp.Vector3d(0.), 16, 1 * crp.pc)
+    g.array()[:] = (rng.normal(size=(16, 16, 16, 3)) * crp.nG).astype(np.float32)
+    d = crp.DiffusionSDE(crp.MagneticFieldGrid(g), 1e-8, 1e-20 * crp.pc, 10 * crp.pc, 0.1)
+    d.setSeed(3)
+    sim = crp.ModuleList()
+    sim.add(d)
+    sim.add(crp.MaximumTrajectoryLength(1e-9 * crp.pc))
+    start = crp.Vector3d(8 * crp.pc, 8 * crp.pc, 8 * crp.pc)
+    cands = []
+    for serial, nxt in ((101, 1e-12 * crp.pc), (102, 10 * crp.pc)):
+        c = crp.Candidate(crp.nucleusId(1, 1), 1 * crp.PeV, start, crp.Vector3d(1, 0, 0))
+        c.setSerialNumber(serial)  # names the candidate's random stream
+        c.setNextStep(nxt)
+        cands.append(c)
+    with pytest.raises(RuntimeError, match="sub-steps"):
+        sim.run(cands)
+    small, big = cands
+    print("small: length %.3e pc, %d steps; big: length %.3e pc, %d steps" % (
+        small.getTrajectoryLength() / crp.pc, small.getDiffusionSteps(), big.getTrajectoryLength() / crp.pc,
+        big.getDiffusionSteps()))
+    assert not big.isActive()
+    # stopped where it was; its one step's random numbers are counted as drawn
+    assert big.getTrajectoryLength() == 0.0 and big.getDiffusionSteps() == 1
+    p = big.current.getPosition()
+    assert (p.x, p.y, p.z) == (start.x, start.y, start.z)
+    assert not small.isActive()
+    assert small.getTrajectoryLength() == pytest.approx(1e-9 * crp.pc, rel=1e-9)
+    assert small.getDiffusionSteps() > 1
+    # one process() call of the capped case: the same error, the candidate stopped in place
+    c = crp.Candidate(crp.nucleusId(1, 1), 1 * crp.PeV, start, crp.Vector3d(1, 0, 0))
+    c.setSerialNumber(102)
+    c.setNextStep(10 * crp.pc)
+    with pytest.raises(RuntimeError, match="sub-steps"):
+        d.process(c)
+    assert not c.isActive() and c.getTrajectoryLength() == 0.0
