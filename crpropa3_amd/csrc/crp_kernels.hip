@@ -839,6 +839,24 @@ int crp_field_create_jf12(const crp_jf12_params *p, const float *striated, const
 	return CRP_OK;
 }
 
+int crp_field_create_jf12_view(const crp_field_t *base, const crp_jf12_params *p, crp_field_t **out) {
+	if (!base || !p || !out)
+		return fail(CRP_E_INVALID, "crp_field_create_jf12_view: NULL argument");
+	if (base->kind != FK_JF12)
+		return fail(CRP_E_INVALID, "crp_field_create_jf12_view: base is not a JF12 field");
+	if (p->use_striated && !base->jf12.striated.values)
+		return fail(CRP_E_INVALID, "crp_field_create_jf12_view: use_striated, but base holds no striated grid");
+	if (p->use_turbulent && !base->jf12.turbulent.values)
+		return fail(CRP_E_INVALID, "crp_field_create_jf12_view: use_turbulent, but base holds no turbulent grid");
+	crp_field *f = new_field(FK_JF12);
+	if (!f)
+		return fail(CRP_E_NOMEM, "crp_field_create_jf12_view: out of host memory");
+	f->jf12 = base->jf12; // the grid descriptors point at base's memory; nothing is owned
+	f->jf12.p = *p;
+	*out = f;
+	return CRP_OK;
+}
+
 int crp_field_set_grid_layout(crp_field_t *f, int layout) {
 	if (!f || (f->kind != FK_GRID && f->kind != FK_MODULATED))
 		return fail(CRP_E_INVALID, "crp_field_set_grid_layout: not a grid field");

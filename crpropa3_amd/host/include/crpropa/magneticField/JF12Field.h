@@ -57,9 +57,12 @@ protected:
 
 private:
 	Vector3d evalWith(const Vector3d &pos, int regular, int striated, int turbulent) const;
-	// one field per part (regular, striated, turbulent) for the part getters: each keeps its device
-	// copy while the switches and grids stay the same
-	mutable ref_ptr<JF12Field> parts[3];
+	crp_field *deviceFieldLocked() const; // under mtx
+	void releaseDevice() const;           // under mtx: the part views, then the field whose grids they read
+	// the part getters (regular, striated, turbulent) evaluate views of the device field with their
+	// switches (crp_field_create_jf12_view): no second copy of a grid
+	mutable crp_field *partDev[3] = {nullptr, nullptr, nullptr};
+	mutable crp_jf12_params partParams[3];
 	mutable crp_field *dev = nullptr;
 	mutable crp_jf12_params devParams;
 	mutable const Grid1f *devStr = nullptr;

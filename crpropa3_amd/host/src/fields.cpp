@@ -393,9 +393,17 @@ JF12Field::JF12Field(int planck) {
 		throw std::runtime_error(crp_last_error());
 }
 
-JF12Field::~JF12Field() {
+JF12Field::~JF12Field() { releaseDevice(); }
+
+void JF12Field::releaseDevice() const {
+	for (crp_field *&v : partDev) {
+		if (v)
+			crp_field_destroy(v);
+		v = nullptr;
+	}
 	if (dev)
 		crp_field_destroy(dev);
+	dev = nullptr;
 }
 
 void JF12Field::randomStriated(int seed) {
@@ -452,15 +460,18 @@ void JF12Field::setUseTurbulentField(bool use) {
 
 crp_field *JF12Field::deviceField() const {
 	std::lock_guard<std::mutex> lock(mtx);
-	const Grid1f *s = p.use_striated ? striatedGrid.get() : nullptr;
-	const Grid3f *t = p.use_turbulent ? turbulentGrid.get() : nullptr;
+	return deviceFieldLocked();
+}
+
+crp_field *JF12Field::deviceFieldLocked() const {
+	// the grids that are set go to the device whether or not their switches are on, so that the part
+	// getters can read them through views
+	const Grid1f *s = striatedGrid.valid() ? striatedGrid.get() : nullptr;
+	const Grid3f *t = turbulentGrid.valid() ? turbulentGrid.get() : nullptr;
 	if (dev && std::memcmp(&devParams, &p, sizeof(p)) == 0 && devStr == s && devTurb == t &&
 	    (!s || devStrVersion == s->getVersion()) && (!t || devTurbVersion == t->getVersion()))
 		return dev;
-	if (dev) {
-		crp_field_destroy(dev);
-		dev = nullptr;
-	}
+	releaseDevice();
 	int sn[3] = {0, 0, 0}, tn[3] = {0, 0, 0};
 	double so[3] = {0, 0, 0}, ss[3] = {1, 1, 1}, to[3] = {0, 0, 0}, ts[3] = {1, 1, 1};
 	if (s) {
@@ -491,18 +502,24 @@ crp_field *JF12Field::deviceField() const {
 
 Vector3d JF12Field::evalWith(const Vector3d &pos, int regular, int striated, int turbulent) const {
 	const int k = turbulent ? 2 : (striated ? 1 : 0);
-	// the part fields are shared by all calls on this field: one call at a time
+	// the device field and its part views are shared by all calls on this field: one call at a time
 	std::lock_guard<std::mutex> lock(mtx);
-	if (!parts[k].valid())
-		parts[k] = new JF12Field(0);
-	ref_ptr<JF12Field> f = parts[k];
-	f->p = p;
-	f->p.use_regular = regular;
-	f->p.use_striated = striated && striatedGrid.valid();
-	f->p.use_turbulent = turbulent && turbulentGrid.valid();
-	f->striatedGrid = striatedGrid;
-	f->turbulentGrid = turbulentGrid;
-	return f->getField(pos);
+	crp_field *base = deviceFieldLocked();
+	crp_jf12_params q = p;
+	q.use_regular = regular;
+	q.use_striated = striated && striatedGrid.valid();
+	q.use_turbulent = turbulent && turbulentGrid.valid();
+	if (!partDev[k] || std::memcmp(&partParams[k], &q, sizeof(q)) != 0) {
+		if (partDev[k])
+			crp_field_destroy(partDev[k]);
+		partDev[k] = nullptr;
+		throwIf(crp_field_create_jf12_view(base, &q, &partDev[k]));
+		partParams[k] = q;
+	}
+	const double x[3] = {pos.x, pos.y, pos.z};
+	double B[3];
+	throwIf(crp_field_eval(partDev[k], x, 1, B));
+	return Vector3d(B[0], B[1], B[2]);
 }
 
 Vector3d JF12Field::getRegularField(const Vector3d &pos) const { return evalWith(pos, 1, 0, 0); }

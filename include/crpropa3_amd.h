@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define CRP_ABI_VERSION 9
+#define CRP_ABI_VERSION 10
 
 enum crp_status {
 	CRP_OK = 0,
@@ -168,6 +168,10 @@ int crp_field_create_jf12(const crp_jf12_params *p, const float *striated, const
                           const float *turbulent, const int turbulent_n[3], const double turbulent_origin[3],
                           const double turbulent_spacing[3], int turbulent_reflective, int turbulent_clip,
                           crp_field_t **out);
+/* JF12Field::getRegularField, getStriatedField, getTurbulentField (JF12Field.cpp:198-341): the same
+ * model with other switches p over base's device grids (borrowed: base must outlive the view; no grid
+ * is copied).  A switch whose grid base does not hold is rejected with CRP_E_INVALID. */
+int crp_field_create_jf12_view(const crp_field_t *base, const crp_jf12_params *p, crp_field_t **out);
 
 /* ---- propagators ------------------------------------------------------ */
 

@@ -100,6 +100,21 @@ def test_python_jf12_classes():
     assert bits_equal(B, dev.eval(pos))
     Br = np.array([[v.x, v.y, v.z] for v in (f.getRegularField(crp.Vector3d(*x)) for x in pos)])
     assert bits_equal(Br, fields(c, "regular")[0].eval(pos))
+    # the other two parts (views of the same device grids), against the device variants and the oracle
+    # (correctly rounded exp/atan2/sin/cos) with the striated-only and turbulent-only switches
+    for name, getter in (("striated", f.getStriatedField), ("turbulent", f.getTurbulentField)):
+        Bpart = np.array([[v.x, v.y, v.z] for v in (getter(crp.Vector3d(*x)) for x in pos)])
+        dv, ov = fields(c, name)
+        assert bits_equal(Bpart, dv.eval(pos)), name
+        assert bits_equal(Bpart, ov.eval(pos, libm=O.LIBM_CR)), name
+    # parts still right after the grid changes (the views follow the rebuilt device field)
+    g2 = crp.Grid3f(crp.Vector3d(0.), 16, c["meta"]["turb_spacing"])
+    g2.array()[:] = 2 * c["grid"]
+    f.setTurbulentGrid(g2)
+    Bt2 = np.array([[v.x, v.y, v.z] for v in (f.getTurbulentField(crp.Vector3d(*x)) for x in pos)])
+    f.setTurbulentGrid(g)
+    Bt = np.array([[v.x, v.y, v.z] for v in (f.getTurbulentField(crp.Vector3d(*x)) for x in pos)])
+    assert bits_equal(Bt2, 2 * Bt)  # doubling every grid value doubles every product and sum exactly
     Bp = crp.PlanckJF12bField()
     Bp.randomStriated(42)
     Bp.setTurbulentGrid(g)
