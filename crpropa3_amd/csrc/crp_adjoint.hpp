@@ -299,12 +299,15 @@ __global__ void __launch_bounds__(256) k_adjoint_backward(AdjArgs A) {
 // accumulation of the previous one on a second stream beside it.  The arithmetic of a step is that of
 // adj_step / adj_step_reverse, except that the position gradient is taken as aT . dT/df; the deposits
 // are summed in another order.
-constexpr int kTileShift = 3, kTileCells = 1 << kTileShift, kTilePts = kTileCells + 1;
-constexpr int kTilePtsAll = kTilePts * kTilePts * kTilePts;
-constexpr int kCellBits = 3 * kTileShift;
+// deposit tiles of 2^S cells per axis: 8^3 (S = 3, 9^3 points = 17.5 KB of FP64 LDS per accumulating
+// workgroup of 256) or, for grids with too many of those, 16^3 (S = 4, 118 KB, workgroups of 1024)
+template <int S>
+struct Tile {
+	static constexpr int shift = S, cells = 1 << S, pts = cells + 1, ptsAll = pts * pts * pts, cellBits = 3 * S;
+	static constexpr int threads = S == 3 ? 256 : 1024;
+};
 constexpr int kAdjGroup = 256;       // particles per recompute / reverse workgroup
 constexpr int kDepPerGroup = 65536;  // records per accumulating workgroup
-constexpr int kDepTileThreads = 256;
 constexpr int kDepBatch = 4;
 constexpr int kAdjBurst = 4;         // reverse steps whose records are stored together
 // stored per step and wave: u, T, dT/dfx, dT/dfy, dT/dfz (15 x 64 doubles)
@@ -312,7 +315,7 @@ constexpr int kAdjStored = 15;
 constexpr int kAdjStepBytes = kAdjStored * 64 * 8;
 
 struct __align__(16) DepRec {
-	uint32_t key; // tile << kCellBits | cell in tile (x << 2 kTileShift | y << kTileShift | z)
+	uint32_t key; // tile << 3S | cell in tile (x << 2S | y << S | z) for tiles of 2^S cells per axis
 	float f[3];   // fractions of the half-step position in its cell
 	float a[3];   // adjoint of the interpolated field
 	uint32_t pad;
@@ -349,7 +352,9 @@ __global__ void __launch_bounds__(kAdjGroup) k_dep_zero(AdjBlock Bk) {
 // One lane per (segment, particle); a workgroup holds kAdjGroup particles of one segment.  The
 // gathers of step j + 1 are issued before the stores of step j: a wave's wait for its loads also
 // waits for the stores issued before them, and the stored values are not read again here.
+template <int S>
 __global__ void __launch_bounds__(kAdjGroup) k_adj_recompute(AdjArgs A, AdjBlock Bk) {
+	using Ti = Tile<S>;
 	extern __shared__ uint32_t hist[];
 	const int T = Bk.ntiles;
 	for (int k = threadIdx.x; k < T; k += blockDim.x)
@@ -388,10 +393,10 @@ __global__ void __launch_bounds__(kAdjGroup) k_adj_recompute(AdjArgs A, AdjBlock
 			const double sd[kAdjStored] = {u.x, u.y, u.z, b[0], b[1], b[2], jx[0], jx[1], jx[2],
 			                               jy[0], jy[1], jy[2], jz[0], jz[1], jz[2]};
 			const int cx = tp.lo[0], cy = tp.lo[1], cz = tp.lo[2];
-			const uint32_t tile = (uint32_t)(((cx >> kTileShift) * Bk.nty + (cy >> kTileShift)) * Bk.ntz + (cz >> kTileShift));
-			const uint32_t m = kTileCells - 1;
+			const uint32_t tile = (uint32_t)(((cx >> Ti::shift) * Bk.nty + (cy >> Ti::shift)) * Bk.ntz + (cz >> Ti::shift));
+			const uint32_t m = Ti::cells - 1;
 			adj_u32x4 rec;
-			rec.x = (tile << kCellBits) | (((uint32_t)cx & m) << (2 * kTileShift)) | (((uint32_t)cy & m) << kTileShift) |
+			rec.x = (tile << Ti::cellBits) | (((uint32_t)cx & m) << (2 * Ti::shift)) | (((uint32_t)cy & m) << Ti::shift) |
 			        ((uint32_t)cz & m);
 			rec.y = __float_as_uint((float)tp.fx);
 			rec.z = __float_as_uint((float)tp.fy);
@@ -547,6 +552,7 @@ __device__ __forceinline__ V3 adj_reverse_stored(const AdjGrid &G, double h, dou
 // One lane per particle, kAdjGroup particles per workgroup; each step's record goes to its tile's range
 // at the group's offset (counts, from the scans) plus its rank among the group's records of the tile.
 // one wave per SIMD (n / 64 waves in all): the kernel may use the whole register file
+template <int S>
 __global__ void __launch_bounds__(kAdjGroup) __attribute__((amdgpu_waves_per_eu(1, 1))) k_adj_reverse(AdjArgs A, AdjBlock Bk) {
 	extern __shared__ uint32_t lds[];
 	const int T = Bk.ntiles;
@@ -590,7 +596,7 @@ __global__ void __launch_bounds__(kAdjGroup) __attribute__((amdgpu_waves_per_eu(
 					const int qq = q - k;
 					adj_load_stored(Bk, K, n, i, max(qq - 2, 0), d2);
 					const V3 aT = adj_reverse_stored(A.G, A.h, kq, d0, ax, au, gs);
-					const uint32_t t = d0.rec[0] >> kCellBits;
+					const uint32_t t = d0.rec[0] >> Tile<S>::cellBits;
 					heldDst[k] = base[t] + atomicAdd(&rank[t], 1u);
 					held[k][0] = make_uint4(d0.rec[0], d0.rec[1], d0.rec[2], d0.rec[3]);
 					held[k][1] = make_uint4(__float_as_uint((float)aT.x), __float_as_uint((float)aT.y),
@@ -628,7 +634,10 @@ __global__ void __launch_bounds__(kAdjGroup) __attribute__((amdgpu_waves_per_eu(
 // LDS (FP64), then added into the grid gradient (points on tile faces are shared with the
 // neighbouring tiles, and a tile with many records with the tile's other workgroups).  The grid
 // is launched with ntiles + ceil(m / kDepPerGroup) workgroups, an upper bound of wgStart[ntiles].
-__global__ void __launch_bounds__(kDepTileThreads) k_dep_tiles(AdjArgs A, AdjBlock Bk) {
+template <int S>
+__global__ void __launch_bounds__(Tile<S>::threads) k_dep_tiles(AdjArgs A, AdjBlock Bk) {
+	using Ti = Tile<S>;
+	constexpr int kTilePts = Ti::pts, kTilePtsAll = Ti::ptsAll, kTileCells = Ti::cells, kTileShift = Ti::shift;
 	__shared__ double acc[kTilePtsAll * 3];
 	const uint32_t g = blockIdx.x;
 	const int T = Bk.ntiles;

@@ -2271,14 +2271,16 @@ static int adj_args(const crp_adjoint_config *cfg, size_t n, AdjArgs &A) {
 
 // Plan of the blocked backward (crp_adjoint.hpp): segments per block, deposit tiles and the scratch
 // layout.  Every per-block buffer is doubled so that the reverse sweep of one block overlaps the
-// recompute of the next and the accumulation of the previous one.  Grids with more than kAdjMaxTiles
-// tiles of 8^3 cells use the checkpointed backward with direct atomics.
+// recompute of the next and the accumulation of the previous one.  Deposit tiles are 8^3 cells, or 16^3
+// when the grid has more than kAdjMaxTiles of those; grids with more than kAdjMaxTiles tiles of 16^3
+// cells use the checkpointed backward with direct atomics.
 constexpr int kAdjMaxTiles = 8192;
 constexpr size_t kAdjBlockBudget = (size_t)4 << 30; // bytes of one buffer set (stored values, records)
 
 struct AdjPlan {
 	bool blocked;
 	int nseg, segPerBlock, nblocks, npg;
+	int tileShift; // 3: 8^3-cell tiles, 4: 16^3
 	int ntx, nty, ntz, ntiles;
 	size_t maxRecords; // records of the largest block
 	// byte offsets into the scratch
@@ -2293,10 +2295,19 @@ static AdjPlan adj_plan(const crp_adjoint_config *cfg, size_t n) {
 	const int K = cfg->checkpoint_every;
 	P.nseg = (cfg->nsteps + K - 1) / K;
 	P.npg = (int)((n + kAdjGroup - 1) / kAdjGroup);
-	P.ntx = (cfg->nx + kTileCells - 1) / kTileCells;
-	P.nty = (cfg->ny + kTileCells - 1) / kTileCells;
-	P.ntz = (cfg->nz + kTileCells - 1) / kTileCells;
-	const long long tiles = (long long)P.ntx * P.nty * P.ntz;
+	// CRP_ADJOINT_TILE_SHIFT=4 asks for 16^3-cell tiles on grids where 8^3 would do (tests)
+	const char *ts = std::getenv("CRP_ADJOINT_TILE_SHIFT");
+	long long tiles = 0;
+	for (int shift = (ts && std::atoi(ts) == 4) ? 4 : 3; shift <= 4; shift++) {
+		const int c = 1 << shift;
+		P.tileShift = shift;
+		P.ntx = (cfg->nx + c - 1) / c;
+		P.nty = (cfg->ny + c - 1) / c;
+		P.ntz = (cfg->nz + c - 1) / c;
+		tiles = (long long)P.ntx * P.nty * P.ntz;
+		if (tiles <= kAdjMaxTiles)
+			break;
+	}
 	// per segment and buffer set: stored values (per padded lane and step) and both record copies (48 B)
 	const double perSeg = (double)K * ((double)P.npg * kAdjGroup * (kAdjStepBytes / 64) + (double)n * 48.0);
 	// CRP_ADJOINT_BLOCK_BYTES overrides the buffer-set budget (0: per-step atomics), so that tests
@@ -2505,15 +2516,21 @@ int crp_adjoint_backward(const crp_adjoint_config *cfg, size_t n, const double *
 		const AdjBlock Bk = block(b);
 		const size_t nc = (size_t)Bk.npg * T;
 		hipLaunchKernelGGL(k_dep_zero, dim3((unsigned)((nc + kAdjGroup - 1) / kAdjGroup)), dim3(kAdjGroup), 0, sb, Bk);
-		hipLaunchKernelGGL(k_adj_recompute, dim3((unsigned)((size_t)Bk.nseg * Bk.npg)), dim3(kAdjGroup),
-		                   T * sizeof(uint32_t), sb, A, Bk);
+		const dim3 grid((unsigned)((size_t)Bk.nseg * Bk.npg));
+		if (P.tileShift == 3)
+			hipLaunchKernelGGL(k_adj_recompute<3>, grid, dim3(kAdjGroup), T * sizeof(uint32_t), sb, A, Bk);
+		else
+			hipLaunchKernelGGL(k_adj_recompute<4>, grid, dim3(kAdjGroup), T * sizeof(uint32_t), sb, A, Bk);
 		hipLaunchKernelGGL(k_dep_scan_chunks, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, sb, Bk);
 		hipLaunchKernelGGL(k_dep_scan_tiles, dim3(1), dim3(1024), 0, sb, Bk);
 	};
 	auto accumulate = [&](int b) {
 		const AdjBlock Bk = block(b);
 		const size_t groups = T + (Bk.m + kDepPerGroup - 1) / kDepPerGroup;
-		hipLaunchKernelGGL(k_dep_tiles, dim3((unsigned)groups), dim3(kDepTileThreads), 0, sc, A, Bk);
+		if (P.tileShift == 3)
+			hipLaunchKernelGGL(k_dep_tiles<3>, dim3((unsigned)groups), dim3(Tile<3>::threads), 0, sc, A, Bk);
+		else
+			hipLaunchKernelGGL(k_dep_tiles<4>, dim3((unsigned)groups), dim3(Tile<4>::threads), 0, sc, A, Bk);
 	};
 
 	// streams B and C start after the work already queued on stream A (e.g. the zeroed gradients)
@@ -2530,7 +2547,10 @@ int crp_adjoint_backward(const crp_adjoint_config *cfg, size_t n, const double *
 			CRP_HIP_TRY(hipStreamWaitEvent(sa, sorted[b + 2], 0));
 		{
 			const AdjBlock Bk = block(b);
-			hipLaunchKernelGGL(k_adj_reverse, dim3((unsigned)Bk.npg), dim3(kAdjGroup), 2 * T * sizeof(uint32_t), sa, A, Bk);
+			if (P.tileShift == 3)
+				hipLaunchKernelGGL(k_adj_reverse<3>, dim3((unsigned)Bk.npg), dim3(kAdjGroup), 2 * T * sizeof(uint32_t), sa, A, Bk);
+			else
+				hipLaunchKernelGGL(k_adj_reverse<4>, dim3((unsigned)Bk.npg), dim3(kAdjGroup), 2 * T * sizeof(uint32_t), sa, A, Bk);
 		}
 		CRP_HIP_TRY(hipEventRecord(reversed[b], sa));
 		// B: recompute of block b - 1 into the buffers that the reverse sweep of block b + 1 has

@@ -167,19 +167,35 @@ def test_adjoint_rejects_bad_input():
         boris_grid_propagate(g.double(), x0, u0, 1 * O.EeV, O.eplus, [0, 0, 0], [s["sp"]] * 3, s["h"], 10, 4)
 
 
-@pytest.mark.parametrize("budget", [None, "3e5", "0"])
-def test_adjoint_backward_paths(budget, monkeypatch):
+@pytest.mark.parametrize("budget,shift", [(None, None), ("3e5", None), ("0", None), (None, "4"), ("3e5", "4")])
+def test_adjoint_backward_paths(budget, shift, monkeypatch):
     """The backward's three schedules against the oracle on a 20^3 grid (tiles cut by the grid's end):
     all segments in one block (default), several blocks of segments with the adjoint carried between
-    them (small per-block budget), and per-step atomics (budget 0)."""
+    them (small per-block budget), and per-step atomics (budget 0); the blocked ones with 8^3-cell tiles
+    and with the 16^3-cell tiles of large grids."""
     if budget is not None:
         monkeypatch.setenv("CRP_ADJOINT_BLOCK_BYTES", budget)
+    if shift is not None:
+        monkeypatch.setenv("CRP_ADJOINT_TILE_SHIFT", shift)
     s = setup(n=96, N=20, nsteps=130, K=8, seed=9)
     ref = oracle_grad(s)
     got = gpu_run(s)
     rel = np.abs(got[3] - ref[3]).max() / np.abs(ref[3]).max()
-    print("budget %s: grid-gradient max rel %.2e" % (budget, rel))
+    print("budget %s, tile shift %s: grid-gradient max rel %.2e" % (budget, shift, rel))
     assert rel < 1e-6
     assert np.allclose(got[4], ref[4], rtol=1e-7, atol=1e-8 * np.abs(ref[4]).max())
+    assert np.allclose(got[5], ref[5], rtol=1e-7, atol=1e-8 * np.abs(ref[5]).max())
+    assert got[6] == pytest.approx(ref[6], rel=1e-8)
+
+
+def test_adjoint_large_grid_uses_16_cell_tiles():
+    """A 176^3 grid has 10648 tiles of 8^3 cells, more than the blocked backward takes: it runs with
+    16^3-cell tiles (1331) and matches the oracle."""
+    s = setup(n=64, N=176, nsteps=60, K=8, seed=12)
+    ref = oracle_grad(s)
+    got = gpu_run(s)
+    rel = np.abs(got[3] - ref[3]).max() / np.abs(ref[3]).max()
+    print("176^3: grid-gradient max rel %.2e" % rel)
+    assert rel < 1e-6
     assert np.allclose(got[5], ref[5], rtol=1e-7, atol=1e-8 * np.abs(ref[5]).max())
     assert got[6] == pytest.approx(ref[6], rel=1e-8)
