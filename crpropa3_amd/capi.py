@@ -93,6 +93,8 @@ def lib():
         L.crp_field_set_grid_layout.argtypes = [_vp, C.c_int]
         L.crp_field_get_grid_layout.argtypes = [_vp, C.POINTER(C.c_int)]
         L.crp_field_set_grid_storage.argtypes = [_vp, C.c_int]
+        L.crp_field_set_grid_interpolation.argtypes = [_vp, C.c_int, C.c_int]
+        L.crp_field_get_grid_interpolation.argtypes = [_vp, C.c_int, C.POINTER(C.c_int)]
         L.crp_field_eval_ex.argtypes = [_vp, _dp, C.c_size_t, _dp, C.c_int]
         L.crp_field_eval.argtypes = [_vp, _dp, C.c_size_t, _dp]
         L.crp_field_eval_noalloc.argtypes = [_vp, _dp, C.c_size_t, _dp, C.c_int]
@@ -308,6 +310,20 @@ class Field:
         """Grid storage read by mixed-precision runs: "fp32" or "fp16" (FP16 + per-brick scale)."""
         check(lib().crp_field_set_grid_storage(self.handle, self.STORAGE[storage]))
         return self
+
+    INTERPOLATION = {"TRILINEAR": 0, "TRICUBIC": 1, "NEAREST_NEIGHBOUR": 2}
+    GRIDS = {"values": 0, "modulation": 1}
+
+    def set_interpolation(self, interpolation, grid="values"):
+        """Grid::setInterpolationType of the Grid3f ("values") or the Grid1f ("modulation") of a grid or
+        modulated field: "TRILINEAR", "TRICUBIC" or "NEAREST_NEIGHBOUR"."""
+        check(lib().crp_field_set_grid_interpolation(self.handle, self.GRIDS[grid], self.INTERPOLATION[interpolation]))
+        return self
+
+    def interpolation(self, grid="values"):
+        v = C.c_int(-1)
+        check(lib().crp_field_get_grid_interpolation(self.handle, self.GRIDS[grid], C.byref(v)))
+        return {0: "TRILINEAR", 1: "TRICUBIC", 2: "NEAREST_NEIGHBOUR"}[v.value]
 
     def eval(self, pos, precision=PRECISION_FP64):
         pos = _f64(pos, (-1, 3))

@@ -24,7 +24,10 @@ namespace crp {
 
 constexpr int kCompMaxLeaves = 8, kCompMaxOps = 48, kCompMaxPeriodic = 8, kCompSlots = 4;
 
-enum CompLeafKind { CL_NONE = 0, CL_UNIFORM = 1, CL_GRID = 2, CL_MODULATED = 3, CL_PLANEWAVE = 4, CL_JF12 = 5 };
+// CL_GRID_ANY: a grid or modulated field whose Grid3f or Grid1f is not trilinear (FieldGridAny; `mod`
+// holds the modulation when its values are set)
+enum CompLeafKind { CL_NONE = 0, CL_UNIFORM = 1, CL_GRID = 2, CL_MODULATED = 3, CL_PLANEWAVE = 4, CL_JF12 = 5,
+                    CL_GRID_ANY = 6 };
 
 struct CompLeaf {
 	int kind;
@@ -113,6 +116,20 @@ struct FieldComposite {
 			zf = crp_pow_cr(1 + z, ld_const(&d->evoM));
 	}
 
+	// JF12 and the grids read tricubic or nearest neighbour share one out-of-line call site
+	__device__ static __attribute__((noinline)) V3 leaf_eval_call(const CompLeaf *Lp, const V3 &x) {
+		if (ld_const(&Lp->kind) == CL_JF12)
+			return ld_const(&Lp->jf12).eval(x);
+		const GridDesc mod = ld_const(&Lp->mod);
+		return FieldGridAny{ld_const(&Lp->grid), mod, mod.values != nullptr}.eval(x);
+	}
+	__device__ static __attribute__((noinline)) float3 leaf_evalf_call(const CompLeaf *Lp, const V3 &x) {
+		if (ld_const(&Lp->kind) == CL_JF12)
+			return ld_const(&Lp->jf12).evalf(x);
+		const GridDesc mod = ld_const(&Lp->mod);
+		return FieldGridAny{ld_const(&Lp->grid), mod, mod.values != nullptr}.evalf(x);
+	}
+
 	__device__ static V3 leaf_eval(const CompLeaf *Lp, const V3 &x) {
 		const int kind = ld_const(&Lp->kind);
 		switch (kind) {
@@ -130,7 +147,8 @@ struct FieldComposite {
 			return FieldModulated<FieldGrid>{FieldGrid{ld_const(&Lp->grid)}, ld_const(&Lp->mod)}.eval(x);
 		}
 		case CL_PLANEWAVE: return FieldPlaneWave{ld_const(&Lp->pw)}.eval(x);
-		case CL_JF12: return ld_const(&Lp->jf12).eval(x);
+		case CL_JF12:
+		case CL_GRID_ANY: return leaf_eval_call(Lp, x);
 		default: return v3(0.0, 0.0, 0.0);
 		}
 	}
@@ -155,7 +173,8 @@ struct FieldComposite {
 			return FieldModulated<FieldGrid>{FieldGrid{ld_const(&Lp->grid)}, ld_const(&Lp->mod)}.evalf(x);
 		}
 		case CL_PLANEWAVE: return FieldPlaneWave{ld_const(&Lp->pw)}.evalf(x);
-		case CL_JF12: return ld_const(&Lp->jf12).evalf(x);
+		case CL_JF12:
+		case CL_GRID_ANY: return leaf_evalf_call(Lp, x);
 		default: return make_float3(0.f, 0.f, 0.f);
 		}
 	}

@@ -79,7 +79,12 @@ struct GridDesc {
 	double invSpacing[3];      // RN(1/spacing), computed on the host
 	double invN[3];            // 1/N per axis, for the division-free periodic wrap
 	int fastDiv;               // spacing within [2^-500, 2^500]: div_cr may use its reciprocal path
+	int interp;                // GridInterp: Grid::setInterpolationType (Grid.h:24-28, 255-264)
 };
+
+// Interpolation types of Grid::interpolate (include/crpropa/Grid.h:24-28, 318-335), numbered as
+// the C ABI's crp_grid_interpolation
+enum GridInterp { kInterpTrilinear = 0, kInterpTricubic = 1, kInterpNearest = 2 };
 
 // Plane-wave mode tables, one record per mode, read through the constant address space so that
 // the compiler loads them with scalar (SMEM) loads into SGPRs: every lane of a wave evaluates the
@@ -189,6 +194,176 @@ __device__ __forceinline__ void reflective_clamp(double x, int n, int &lo, int &
 		hi = 0;
 	}
 	frac = x - fl;
+}
+
+// ---------------------------------------------------------------- tricubic and nearest neighbour
+// crpropa::round (include/crpropa/Grid.h:61-63): r + 0.5 is rounded before the floor (so 0.5 - 2^-54
+// goes up), unlike the C library's round
+__device__ __forceinline__ double round_sym(double r) {
+	return (r > 0.0) ? __builtin_floor(r + 0.5) : __builtin_ceil(r - 0.5);
+}
+
+// reflectiveBoundary (Grid.h:36-41) without its loop: each pass maps the index to its mirror image
+// at the nearer face, and the images repeat with period 2n, so the loop ends at i mod 2n folded
+// onto [0, n).  Every int maps into [0, n).
+__device__ __forceinline__ int reflect_index(int i, int n) {
+	const int p = 2 * n;
+	int m = i % p;
+	m += (m < 0) ? p : 0;
+	return (m < n) ? m : p - 1 - m;
+}
+
+// Grid::closestValue (include/crpropa/Grid.h:383-405): flat index of the grid point nearest to p,
+// mirrored (reflective) or wrapped (periodic) into the grid
+__device__ inline size_t grid_closest(const GridDesc &g, const V3 &p) {
+	const double rx = div_cr(p.x - g.gridOrigin[0], g.spacing[0], g.invSpacing[0], g.fastDiv);
+	const double ry = div_cr(p.y - g.gridOrigin[1], g.spacing[1], g.invSpacing[1], g.fastDiv);
+	const double rz = div_cr(p.z - g.gridOrigin[2], g.spacing[2], g.invSpacing[2], g.fastDiv);
+	int ix, iy, iz;
+	if (g.reflective) {
+		ix = reflect_index((int)round_sym(rx), g.nx);
+		iy = reflect_index((int)round_sym(ry), g.ny);
+		iz = reflect_index((int)round_sym(rz), g.nz);
+	} else {
+		// fmod(r, n) lies in (-n, n), its rounding in [-n, n]: (i + n * (i < 0)) % n is one correction each way
+		ix = (int)round_sym(__builtin_fmod(rx, (double)g.nx));
+		iy = (int)round_sym(__builtin_fmod(ry, (double)g.ny));
+		iz = (int)round_sym(__builtin_fmod(rz, (double)g.nz));
+		ix += (ix < 0) ? g.nx : 0;
+		iy += (iy < 0) ? g.ny : 0;
+		iz += (iz < 0) ? g.nz : 0;
+		ix -= (ix >= g.nx) ? g.nx : 0;
+		iy -= (iy >= g.ny) ? g.ny : 0;
+		iz -= (iz >= g.nz) ? g.nz : 0;
+	}
+	// a non-finite position converts to indices outside the grid: read point 0 instead of faulting
+	if ((unsigned)ix >= (unsigned)g.nx || (unsigned)iy >= (unsigned)g.ny || (unsigned)iz >= (unsigned)g.nz)
+		ix = iy = iz = 0;
+	return ((size_t)ix * g.ny + iy) * g.nz + iz;
+}
+
+// CubicInterpolate (include/crpropa/Grid.h:436-456), one lane of the SSE code in FP32; t, t^2 and t^3
+// are formed in double by the caller and rounded to float once each
+__device__ __forceinline__ float cubic_f32(float p0, float p1, float p2, float p3, float t, float t2, float t3) {
+	const float term = (0.5f * p2 - 0.5f * p0) * t;
+	const float term2 = ((p0 + 2.f * p2) - (2.5f * p1 + 0.5f * p3)) * t2;
+	const float term3 = ((1.5f * p1 + 0.5f * p3) - (0.5f * p0 + 1.5f * p2)) * t3;
+	return ((term3 + term2) + term) + p1;
+}
+
+// CubicInterpolateScalar (include/crpropa/Grid.h:499-501) in FP64, left to right
+__device__ __forceinline__ double cubic_f64(double p0, double p1, double p2, double p3, double t) {
+	return (((((-0.5 * p0 + 1.5 * p1) - 1.5 * p2) + 0.5 * p3) * t * t * t +
+	         (((p0 - 2.5 * p1) + p2 * 2) - 0.5 * p3) * t * t) +
+	        (-0.5 * p0 + 0.5 * p2) * t) +
+	       p1;
+}
+
+// neighbour index i of one axis through periodicGet / reflectiveGet (Grid.h:346-358)
+__device__ __forceinline__ int cubic_index(int i, int n, double invn, int reflective) {
+	return reflective ? reflect_index(i, n) : wrap_index(i, n, invn);
+}
+
+// unit-grid position of p: the reference's r = (pos - gridOrigin) / spacing, iX0 = floor(r),
+// fX = r - iX0 (Grid.h:461-472)
+__device__ __forceinline__ void cubic_origin(const GridDesc &g, const V3 &p, int i0[3], double f[3]) {
+	const double r[3] = {div_cr(p.x - g.gridOrigin[0], g.spacing[0], g.invSpacing[0], g.fastDiv),
+	                     div_cr(p.y - g.gridOrigin[1], g.spacing[1], g.invSpacing[1], g.fastDiv),
+	                     div_cr(p.z - g.gridOrigin[2], g.spacing[2], g.invSpacing[2], g.fastDiv)};
+#pragma unroll
+	for (int a = 0; a < 3; a++) {
+		i0[a] = (int)__builtin_floor(r[a]);
+		f[a] = r[a] - (double)i0[a];
+	}
+}
+
+// Grid<Vector3f>::tricubicInterpolate (include/crpropa/Grid.h:459-492): cubics along z for each of
+// the 16 (x, y) neighbours, along y for each of the 4 x neighbours, then along x; the x, y and z
+// components are independent lanes.  The result is FP32, widened to double as getField returns it.
+__device__ inline V3 grid_tricubic(const GridDesc &g, const V3 &p) {
+	int i0[3];
+	double f[3];
+	cubic_origin(g, p, i0, f);
+	float t[3], t2[3], t3[3];
+#pragma unroll
+	for (int a = 0; a < 3; a++) {
+		t[a] = (float)f[a];
+		t2[a] = (float)(f[a] * f[a]);
+		t3[a] = (float)(f[a] * f[a] * f[a]);
+	}
+	int Z[4];
+	size_t Y[4];
+#pragma unroll
+	for (int d = 0; d < 4; d++) {
+		Z[d] = cubic_index(i0[2] - 1 + d, g.nz, g.invN[2], g.reflective);
+		Y[d] = (size_t)cubic_index(i0[1] - 1 + d, g.ny, g.invN[1], g.reflective) * g.nz;
+	}
+	float vx[4][3];
+#pragma unroll
+	for (int a = 0; a < 4; a++) {
+		const size_t X = (size_t)cubic_index(i0[0] - 1 + a, g.nx, g.invN[0], g.reflective) * g.ny * g.nz;
+		float vy[4][3];
+#pragma unroll
+		for (int b = 0; b < 4; b++) {
+			const float *q0 = g.values + 3 * (X + Y[b] + Z[0]), *q1 = g.values + 3 * (X + Y[b] + Z[1]);
+			const float *q2 = g.values + 3 * (X + Y[b] + Z[2]), *q3 = g.values + 3 * (X + Y[b] + Z[3]);
+#pragma unroll
+			for (int l = 0; l < 3; l++)
+				vy[b][l] = cubic_f32(q0[l], q1[l], q2[l], q3[l], t[2], t2[2], t3[2]);
+		}
+#pragma unroll
+		for (int l = 0; l < 3; l++)
+			vx[a][l] = cubic_f32(vy[0][l], vy[1][l], vy[2][l], vy[3][l], t[1], t2[1], t3[1]);
+	}
+	float b[3];
+#pragma unroll
+	for (int l = 0; l < 3; l++)
+		b[l] = cubic_f32(vx[0][l], vx[1][l], vx[2][l], vx[3][l], t[0], t2[0], t3[0]);
+	return v3((double)b[0], (double)b[1], (double)b[2]);
+}
+
+// Grid<float>::tricubicInterpolate(double, ...) (include/crpropa/Grid.h:504-536): the same loops in
+// FP64 over the float values; Grid::interpolate narrows the result to float
+__device__ inline float scalar_grid_tricubic(const GridDesc &g, const V3 &p) {
+	int i0[3];
+	double f[3];
+	cubic_origin(g, p, i0, f);
+	int Z[4];
+	size_t Y[4];
+#pragma unroll
+	for (int d = 0; d < 4; d++) {
+		Z[d] = cubic_index(i0[2] - 1 + d, g.nz, g.invN[2], g.reflective);
+		Y[d] = (size_t)cubic_index(i0[1] - 1 + d, g.ny, g.invN[1], g.reflective) * g.nz;
+	}
+	double vx[4];
+#pragma unroll
+	for (int a = 0; a < 4; a++) {
+		const size_t X = (size_t)cubic_index(i0[0] - 1 + a, g.nx, g.invN[0], g.reflective) * g.ny * g.nz;
+		double vy[4];
+#pragma unroll
+		for (int b = 0; b < 4; b++) {
+			const float *q = g.values + X + Y[b];
+			vy[b] = cubic_f64(q[Z[0]], q[Z[1]], q[Z[2]], q[Z[3]], f[2]);
+		}
+		vx[a] = cubic_f64(vy[0], vy[1], vy[2], vy[3], f[1]);
+	}
+	return (float)cubic_f64(vx[0], vx[1], vx[2], vx[3], f[0]);
+}
+
+// Grid::interpolate (Grid.h:329-332) for the tricubic and nearest-neighbour types, after the clip
+// test; out of line, one copy for every kernel that reads such grids
+__device__ __attribute__((noinline)) V3 grid_eval_other(const GridDesc &g, const V3 &p) {
+	if (g.interp == kInterpNearest) {
+		const float *c = g.values + 3 * grid_closest(g, p);
+		return v3((double)c[0], (double)c[1], (double)c[2]);
+	}
+	return grid_tricubic(g, p);
+}
+
+__device__ __attribute__((noinline)) float scalar_grid_other(const GridDesc &g, const V3 &p) {
+	if (g.interp == kInterpNearest)
+		return g.values[grid_closest(g, p)];
+	return scalar_grid_tricubic(g, p);
 }
 
 struct FieldGrid {
@@ -499,6 +674,63 @@ template <class G> struct FieldModulated {
 	__device__ __forceinline__ float3 evalf(const V3 &p) const {
 		const float m = scalar_grid_evalf(mod, p);
 		const float3 b = grid.evalf(p);
+		return make_float3(b.x * m, b.y * m, b.z * m);
+	}
+};
+
+__device__ __forceinline__ bool in_clip_box(const GridDesc &g, const V3 &p) {
+	bool in = (p.x >= g.origin[0]) && (p.x <= g.edge[0]);
+	in &= (p.y >= g.origin[1]) && (p.y <= g.edge[1]);
+	in &= (p.z >= g.origin[2]) && (p.z <= g.edge[2]);
+	return in;
+}
+
+// Grid::interpolate (include/crpropa/Grid.h:318-335) for every interpolation type: trilinear in
+// line, tricubic and nearest neighbour out of line after the clip test
+__device__ __forceinline__ V3 grid_interpolate(const GridDesc &g, const V3 &p) {
+	if (g.interp == kInterpTrilinear)
+		return FieldGrid{g}.eval(p);
+	if (g.clip && !in_clip_box(g, p))
+		return v3(0.0, 0.0, 0.0);
+	return grid_eval_other(g, p);
+}
+
+__device__ __forceinline__ float scalar_grid_interpolate(const GridDesc &g, const V3 &p) {
+	if (g.interp == kInterpTrilinear)
+		return scalar_grid_eval(g, p);
+	if (g.clip && !in_clip_box(g, p))
+		return 0.f;
+	return scalar_grid_other(g, p);
+}
+
+// MagneticFieldGrid and ModulatedMagneticFieldGrid (src/magneticField/MagneticFieldGrid.cpp:5-55)
+// whose Grid3f or Grid1f is read tricubic or nearest-neighbour (Grid::setInterpolationType).  Its own
+// field type, so that the trilinear kernels keep their register allocation.  The mixed path reads
+// trilinear grids as FieldGrid::evalf does and the other types with the FP64 path's code, whose
+// result is a float already.
+struct FieldGridAny {
+	GridDesc g;
+	GridDesc mod; // the Grid1f modulation when `modulated`
+	int modulated;
+
+	__device__ __forceinline__ V3 eval(const V3 &p) const {
+		const V3 b = grid_interpolate(g, p);
+		if (!modulated)
+			return b;
+		const double m = (double)scalar_grid_interpolate(mod, p);
+		return v3(b.x * m, b.y * m, b.z * m);
+	}
+	__device__ __forceinline__ float3 evalf(const V3 &p) const {
+		float3 b;
+		if (g.interp == kInterpTrilinear) {
+			b = FieldGrid{g}.evalf(p);
+		} else {
+			const V3 d = grid_interpolate(g, p);
+			b = make_float3((float)d.x, (float)d.y, (float)d.z);
+		}
+		if (!modulated)
+			return b;
+		const float m = mod.interp == kInterpTrilinear ? scalar_grid_evalf(mod, p) : scalar_grid_interpolate(mod, p);
 		return make_float3(b.x * m, b.y * m, b.z * m);
 	}
 };

@@ -14,36 +14,7 @@ constexpr double kJKpc = 1e3 * (648000 / 3.14159265358979323846 * 149597870700.0
 constexpr double kJPi = 3.14159265358979323846;
 
 // Grid<float>::closestValue (include/crpropa/Grid.h:383-405): nearest point, reflective or periodic
-__device__ __forceinline__ float scalar_grid_closest(const GridDesc &g, const V3 &p) {
-	const double rx = (p.x - g.gridOrigin[0]) / g.spacing[0];
-	const double ry = (p.y - g.gridOrigin[1]) / g.spacing[1];
-	const double rz = (p.z - g.gridOrigin[2]) / g.spacing[2];
-	int ix, iy, iz;
-	const int nx = g.nx, ny = g.ny, nz = g.nz;
-	if (g.reflective) {
-		ix = (int)__builtin_round(rx);
-		iy = (int)__builtin_round(ry);
-		iz = (int)__builtin_round(rz);
-		// each pass mirrors the index at the nearer face; every pass brings it closer to [0, n)
-		for (int k = 0; k < 64 && ((ix < -0.5) || (ix > (nx - 0.5))); k++)
-			ix = 2 * nx * (ix > (nx - 0.5)) - ix - 1;
-		for (int k = 0; k < 64 && ((iy < -0.5) || (iy > (ny - 0.5))); k++)
-			iy = 2 * ny * (iy > (ny - 0.5)) - iy - 1;
-		for (int k = 0; k < 64 && ((iz < -0.5) || (iz > (nz - 0.5))); k++)
-			iz = 2 * nz * (iz > (nz - 0.5)) - iz - 1;
-	} else {
-		ix = (int)__builtin_round(__builtin_fmod(rx, (double)nx));
-		iy = (int)__builtin_round(__builtin_fmod(ry, (double)ny));
-		iz = (int)__builtin_round(__builtin_fmod(rz, (double)nz));
-		ix = (ix + nx * (ix < 0)) % nx;
-		iy = (iy + ny * (iy < 0)) % ny;
-		iz = (iz + nz * (iz < 0)) % nz;
-	}
-	// out-of-range indices (a non-finite position) read point 0 instead of faulting
-	if (ix < 0 || ix >= nx || iy < 0 || iy >= ny || iz < 0 || iz >= nz)
-		ix = iy = iz = 0;
-	return g.values[((size_t)ix * ny + iy) * nz + iz];
-}
+__device__ __forceinline__ float scalar_grid_closest(const GridDesc &g, const V3 &p) { return g.values[grid_closest(g, p)]; }
 
 struct FieldJF12 {
 	crp_jf12_params p;

@@ -598,6 +598,7 @@ static int fill_grid_desc(GridDesc &g, const float *d_values, int nx, int ny, in
 	g.nx = nx;
 	g.ny = ny;
 	g.nz = nz;
+	g.interp = kInterpTrilinear;
 	g.reflective = reflective ? 1 : 0;
 	g.clip = clip ? 1 : 0;
 	const int N[3] = {nx, ny, nz};
@@ -645,6 +646,28 @@ static int build_cells_default(crp_field *f) {
 	if (build_cells(f) != CRP_OK)
 		(void)hipGetLastError();
 	return CRP_OK;
+}
+
+// a grid or modulated field with a Grid3f or Grid1f that is not read trilinear: FieldGridAny
+static bool any_interp(const crp_field *f) {
+	if (!f)
+		return false;
+	if (f->kind == FK_GRID)
+		return f->grid.interp != kInterpTrilinear;
+	if (f->kind == FK_MODULATED)
+		return f->grid.interp != kInterpTrilinear || f->mod.interp != kInterpTrilinear;
+	return false;
+}
+
+static FieldGridAny grid_any(const crp_field *f) {
+	FieldGridAny a;
+	std::memset(&a, 0, sizeof(a));
+	a.g = f->grid;
+	if (f->kind == FK_MODULATED) {
+		a.mod = f->mod;
+		a.modulated = 1;
+	}
+	return a;
 }
 
 int crp_field_create_grid3f(const float *values, int nx, int ny, int nz, const double origin[3],
@@ -871,6 +894,8 @@ int crp_field_set_grid_layout(crp_field_t *f, int layout) {
 		return fail(CRP_E_INVALID, "crp_field_set_grid_layout: unknown layout");
 	if (f->grid.reflective)
 		return fail(CRP_E_INVALID, "crp_field_set_grid_layout: the cell table needs a periodic grid");
+	if (f->grid.interp != kInterpTrilinear)
+		return fail(CRP_E_INVALID, "crp_field_set_grid_layout: the cell table needs trilinear interpolation");
 	return f->cells ? CRP_OK : build_cells(f);
 }
 
@@ -888,6 +913,8 @@ int crp_field_set_grid_storage(crp_field_t *f, int storage) {
 		return fail(CRP_E_INVALID, "crp_field_set_grid_storage: unknown storage");
 	if (f->grid.reflective)
 		return fail(CRP_E_INVALID, "crp_field_set_grid_storage: FP16 storage needs a periodic grid");
+	if (f->grid.interp != kInterpTrilinear)
+		return fail(CRP_E_INVALID, "crp_field_set_grid_storage: FP16 storage needs trilinear interpolation");
 	if (f->cells16)
 		return CRP_OK;
 	const GridDesc &g = f->grid;
@@ -918,6 +945,54 @@ int crp_field_set_grid_storage(crp_field_t *f, int storage) {
 	return CRP_OK;
 }
 
+static_assert((int)kInterpTrilinear == (int)CRP_GRID_TRILINEAR && (int)kInterpTricubic == (int)CRP_GRID_TRICUBIC &&
+                  (int)kInterpNearest == (int)CRP_GRID_NEAREST_NEIGHBOUR,
+              "device interpolation numbers follow crp_grid_interpolation");
+
+// the grid of f that `grid` (crp_grid_select) names, or nullptr
+static GridDesc *interp_grid(crp_field *f, int grid) {
+	if (!f)
+		return nullptr;
+	if (grid == CRP_GRID_VALUES && (f->kind == FK_GRID || f->kind == FK_MODULATED))
+		return &f->grid;
+	if (grid == CRP_GRID_MODULATION && f->kind == FK_MODULATED)
+		return &f->mod;
+	return nullptr;
+}
+
+int crp_field_set_grid_interpolation(crp_field_t *f, int grid, int type) {
+	GridDesc *g = interp_grid(f, grid);
+	if (!g)
+		return fail(CRP_E_INVALID, "crp_field_set_grid_interpolation: the field holds no such grid");
+	if (type != CRP_GRID_TRILINEAR && type != CRP_GRID_TRICUBIC && type != CRP_GRID_NEAREST_NEIGHBOUR)
+		return fail(CRP_E_INVALID, "InterpolationType: unknown interpolation type");
+	if (g->interp == type)
+		return CRP_OK;
+	g->interp = type;
+	if (g != &f->grid)
+		return CRP_OK;
+	if (type == CRP_GRID_TRILINEAR)
+		return build_cells_default(f); // periodic grids get their cell table back
+	// the tables hold trilinear corners: read the points instead
+	if (f->cells) {
+		CRP_HIP_TRY(hipFree(f->cells));
+		f->cells = nullptr;
+	}
+	if (f->cells16) {
+		CRP_HIP_TRY(hipFree(f->cells16));
+		f->cells16 = nullptr;
+	}
+	return CRP_OK;
+}
+
+int crp_field_get_grid_interpolation(const crp_field_t *f, int grid, int *type) {
+	const GridDesc *g = interp_grid(const_cast<crp_field *>(f), grid);
+	if (!g || !type)
+		return fail(CRP_E_INVALID, "crp_field_get_grid_interpolation: the field holds no such grid");
+	*type = g->interp;
+	return CRP_OK;
+}
+
 // ---------------------------------------------------------------- field decorators (host side)
 // Compiles a tree of PeriodicMagneticField / MagneticFieldList / MagneticFieldEvolution over
 // device fields into the program FieldComposite runs (crp_composite.hpp).
@@ -943,8 +1018,18 @@ struct CompBuilder {
 		std::memset(&L, 0, sizeof(L));
 		switch (f ? f->kind : FK_NONE) {
 		case FK_UNIFORM: L.kind = CL_UNIFORM; L.uniform = f->uniform; break;
-		case FK_GRID: L.kind = CL_GRID; L.grid = f->grid; L.cells = f->cells; L.cells16 = f->cells16; break;
-		case FK_MODULATED: L.kind = CL_MODULATED; L.grid = f->grid; L.cells = f->cells; L.mod = f->mod; break;
+		case FK_GRID:
+			L.kind = any_interp(f) ? CL_GRID_ANY : CL_GRID;
+			L.grid = f->grid;
+			L.cells = f->cells;
+			L.cells16 = f->cells16;
+			break;
+		case FK_MODULATED:
+			L.kind = any_interp(f) ? CL_GRID_ANY : CL_MODULATED;
+			L.grid = f->grid;
+			L.cells = f->cells;
+			L.mod = f->mod;
+			break;
 		case FK_PLANEWAVE: L.kind = CL_PLANEWAVE; L.pw = f->pw; break;
 		case FK_JF12: L.kind = CL_JF12; L.jf12 = f->jf12; break;
 		default: L.kind = CL_NONE; break;
@@ -1036,6 +1121,13 @@ static hipError_t launch_field_eval(const crp_field_t *f, const double *dpos, do
 	const int kind = f ? f->kind : FK_NONE;
 	if (composite_one_arg_zero(f))
 		return hipMemsetAsync(dB, 0, 3 * n * sizeof(double), 0);
+	if (any_interp(f)) {
+		if (precision == CRP_PRECISION_FP64)
+			hipLaunchKernelGGL(k_field_eval<FieldGridAny>, blocks, 256, 0, 0, grid_any(f), dpos, dB, n);
+		else
+			hipLaunchKernelGGL(k_field_evalf<FieldGridAny>, blocks, 256, 0, 0, grid_any(f), dpos, dB, n);
+		return hipGetLastError();
+	}
 	if (precision == CRP_PRECISION_FP64) {
 		switch (kind) {
 		case FK_UNIFORM: hipLaunchKernelGGL(k_field_eval<FieldUniform>, blocks, 256, 0, 0, f->uniform, dpos, dB, n); break;
@@ -1396,6 +1488,8 @@ int crp_field_eval_z(const crp_field_t *f, const double *pos, const double *z, s
 
 // Calls fn(field) with the FP64 device field of f, as crp_propagate dispatches it.
 template <class Fn> static int with_fp64_field(const crp_field *f, Fn &&fn) {
+	if (any_interp(f))
+		return fn(grid_any(f));
 	switch (f ? f->kind : FK_NONE) {
 	case FK_UNIFORM: return fn(f->uniform);
 	case FK_GRID: return f->cells ? fn(FieldGridCells{f->grid, f->cells}) : fn(FieldGrid{f->grid});
@@ -2077,7 +2171,8 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 	CRP_HIP_TRY(hipEventRecord(b->ev0, b->stream));
 	int rc;
 	const crp_field *f = p->field;
-	switch (f ? f->kind : FK_NONE) {
+	switch (any_interp(f) ? -1 : (f ? f->kind : FK_NONE)) {
+	case -1: rc = dispatch_kind(grid_any(f), p, P, b, &st.launches); break;
 	case FK_UNIFORM: rc = dispatch_kind(f->uniform, p, P, b, &st.launches); break;
 	case FK_GRID:
 		if (p->precision == CRP_PRECISION_MIXED && f->cells16)

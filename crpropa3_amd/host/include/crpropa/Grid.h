@@ -2,7 +2,7 @@
 // Nx*Ny*Nz Vector3f values, z fastest (index ix*Ny*Nz + iy*Nz + iz), the volume
 // starting at `origin`, sample points at origin + (i + 1/2) * spacing,
 // periodic (default) or reflective continuation, optional clipVolume.
-// Interpolation runs on the GPU (trilinear, the reference's FP32 arithmetic).
+// Interpolation runs on the GPU (trilinear, tricubic or nearest neighbour, the reference's arithmetic).
 #ifndef CRPROPA_AMD_GRID_H
 #define CRPROPA_AMD_GRID_H
 
@@ -57,7 +57,7 @@ public:
 	void setSpacing(Vector3d s) { spacing = s; changed(); }
 	void setReflective(bool b) { reflective = b; changed(); }
 	void setClipVolume(bool b) { clipVolume = b; changed(); }
-	/* Only TRILINEAR runs on the GPU; the others throw. */
+	/* TRILINEAR, TRICUBIC or NEAREST_NEIGHBOUR (Grid.h:255-264); unknown values throw. */
 	void setInterpolationType(interpolationType t);
 	interpolationType getInterpolationType() const { return ipolType; }
 
@@ -79,7 +79,7 @@ public:
 	const std::vector<Vector3f> &getGrid() const { return grid; }
 	Vector3d positionFromIndex(int index) const;
 
-	/* Trilinear value at a position, evaluated on the GPU. */
+	/* Interpolated value at a position (Grid::interpolate), evaluated on the GPU. */
 	Vector3f interpolate(const Vector3d &position) const;
 
 	/* Device field for this grid (created or refreshed on demand). */
@@ -114,14 +114,24 @@ public:
 	    : Grid1f(origin, nx, ny, nz, Vector3d(spacing)) {}
 	Grid1f(Vector3d origin, size_t nx, size_t ny, size_t nz, Vector3d spacing)
 	    : grid(nx * ny * nz, 0.f), Nx(nx), Ny(ny), Nz(nz), origin(origin), spacing(spacing), clipVolume(false),
-	      reflective(false) {}
+	      reflective(false), ipolType(TRILINEAR) {}
 	Grid1f(const GridProperties &p)
 	    : grid(p.Nx * p.Ny * p.Nz, 0.f), Nx(p.Nx), Ny(p.Ny), Nz(p.Nz), origin(p.origin), spacing(p.spacing),
-	      clipVolume(p.clipVolume), reflective(p.reflective) {}
+	      clipVolume(p.clipVolume), reflective(p.reflective), ipolType(TRILINEAR) {
+		setInterpolationType(p.ipol);
+	}
 	void setOrigin(Vector3d o) { origin = o; ++version; }
 	void setSpacing(Vector3d s) { spacing = s; ++version; }
 	void setReflective(bool b) { reflective = b; ++version; }
 	void setClipVolume(bool b) { clipVolume = b; ++version; }
+	/* TRILINEAR, TRICUBIC or NEAREST_NEIGHBOUR (Grid.h:255-264); unknown values throw. */
+	void setInterpolationType(interpolationType t) {
+		if (t != TRILINEAR && t != TRICUBIC && t != NEAREST_NEIGHBOUR)
+			throw std::runtime_error("InterpolationType: unknown interpolation type");
+		ipolType = t;
+		++version;
+	}
+	interpolationType getInterpolationType() const { return ipolType; }
 	Vector3d getOrigin() const { return origin; }
 	Vector3d getSpacing() const { return spacing; }
 	size_t getNx() const { return Nx; }
@@ -143,6 +153,7 @@ private:
 	size_t Nx, Ny, Nz;
 	Vector3d origin, spacing;
 	bool clipVolume, reflective;
+	interpolationType ipolType;
 	uint64_t version = 0;
 };
 

@@ -310,6 +310,8 @@ PYBIND11_MODULE(_crpropa, m) {
 	    .def("setSpacing", &Grid1f::setSpacing)
 	    .def("setReflective", &Grid1f::setReflective)
 	    .def("setClipVolume", &Grid1f::setClipVolume)
+	    .def("setInterpolationType", &Grid1f::setInterpolationType)
+	    .def("getInterpolationType", &Grid1f::getInterpolationType)
 	    .def("getOrigin", &Grid1f::getOrigin)
 	    .def("getSpacing", &Grid1f::getSpacing)
 	    .def("getNx", &Grid1f::getNx)

@@ -71,9 +71,20 @@ void Grid3f::setGridSize(size_t nx, size_t ny, size_t nz) {
 void Grid3f::setInterpolationType(interpolationType t) {
 	if (t != TRILINEAR && t != TRICUBIC && t != NEAREST_NEIGHBOUR)
 		throw std::runtime_error("InterpolationType: unknown interpolation type");
-	if (t != TRILINEAR)
-		throw std::runtime_error("crpropa3_amd: only TRILINEAR interpolation runs on the GPU");
 	ipolType = t;
+	changed();
+}
+
+// the device field's interpolation types as the host grids hold them; destroys f on failure
+static void setDeviceInterpolation(crp_field *f, int which, interpolationType t) {
+	if (t == TRILINEAR)
+		return;
+	const int rc = crp_field_set_grid_interpolation(f, which, (int)t);
+	if (rc != CRP_OK) {
+		const std::string msg = crp_last_error();
+		crp_field_destroy(f);
+		throw std::runtime_error(msg);
+	}
 }
 
 Vector3d Grid3f::positionFromIndex(int index) const {
@@ -96,6 +107,7 @@ crp_field *Grid3f::deviceField() const {
 	crp_field *f = nullptr;
 	throwIf(crp_field_create_grid3f(reinterpret_cast<const float *>(grid.data()), (int)Nx, (int)Ny, (int)Nz, o, s,
 	                                reflective ? 1 : 0, clipVolume ? 1 : 0, &f));
+	setDeviceInterpolation(f, CRP_GRID_VALUES, ipolType);
 	dev = f;
 	dirty = false;
 	return dev;
@@ -363,6 +375,8 @@ crp_field *ModulatedMagneticFieldGrid::deviceField() const {
 	    reinterpret_cast<const float *>(g.getGrid().data()), (int)g.getNx(), (int)g.getNy(), (int)g.getNz(), o3, s3,
 	    g.isReflective() ? 1 : 0, g.getClipVolume() ? 1 : 0, m.getGrid().data(), (int)m.getNx(), (int)m.getNy(),
 	    (int)m.getNz(), o1, s1, m.isReflective() ? 1 : 0, m.getClipVolume() ? 1 : 0, &f));
+	setDeviceInterpolation(f, CRP_GRID_VALUES, g.getInterpolationType());
+	setDeviceInterpolation(f, CRP_GRID_MODULATION, m.getInterpolationType());
 	dev = f;
 	devGrid = grid.get();
 	devMod = modGrid.get();
@@ -472,6 +486,8 @@ crp_field *JF12Field::deviceFieldLocked() const {
 	    (!s || devStrVersion == s->getVersion()) && (!t || devTurbVersion == t->getVersion()))
 		return dev;
 	releaseDevice();
+	if (t && t->getInterpolationType() != TRILINEAR)
+		throw std::runtime_error("crpropa3_amd: JF12Field's turbulent grid is read TRILINEAR on the GPU");
 	int sn[3] = {0, 0, 0}, tn[3] = {0, 0, 0};
 	double so[3] = {0, 0, 0}, ss[3] = {1, 1, 1}, to[3] = {0, 0, 0}, ts[3] = {1, 1, 1};
 	if (s) {

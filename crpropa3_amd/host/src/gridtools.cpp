@@ -166,10 +166,6 @@ ref_ptr<Grid3f> loadGrid3fFromTxt(std::string filename, double c) {
 		ss >> name >> type;
 		gp.setInterpolationType(type == "TRICUBIC" ? TRICUBIC : (type == "NEAREST_NEIGHBOUR" ? NEAREST_NEIGHBOUR : TRILINEAR));
 		fin.close();
-		if (gp.ipol != TRILINEAR) {
-			// the GPU interpolates trilinearly only: keep the values but report the unsupported type
-			throw std::runtime_error("loadGrid3fFromTxt: interpolation " + type + " is not supported (TRILINEAR only)");
-		}
 		ref_ptr<Grid3f> grid = new Grid3f(gp);
 		loadGridFromTxt(grid, filename, c);
 		return grid;

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define CRP_ABI_VERSION 10
+#define CRP_ABI_VERSION 11
 
 enum crp_status {
 	CRP_OK = 0,
@@ -107,6 +107,29 @@ int crp_field_get_grid_layout(const crp_field_t *f, int *layout);
  * never uses it.  FP32 frees it.  No reference counterpart (the reference stores FP32 only). */
 enum crp_grid_storage { CRP_GRID_STORAGE_FP32 = 0, CRP_GRID_STORAGE_FP16_BRICK = 1 };
 int crp_field_set_grid_storage(crp_field_t *f, int storage);
+
+/* Grid::setInterpolationType (include/crpropa/Grid.h:24-28,255-264) and the dispatch of
+ * Grid::interpolate (Grid.h:318-335) for one grid of a field:
+ *   TRILINEAR          trilinearInterpolate (Grid.h:540-582), the default;
+ *   TRICUBIC           Grid3f: tricubicInterpolate(Vector3f) with CubicInterpolate (Grid.h:408-492), the
+ *                      Catmull-Rom cubic along z, then y, then x over the 4x4x4 neighbouring points in
+ *                      FP32, in the operation order of the reference's SIMD build (SIMD_EXTENSIONS "avx";
+ *                      its "none" build rejects tricubic Grid3f); Grid1f: tricubicInterpolate(double)
+ *                      (Grid.h:499-536) in FP64, the result narrowed to float;
+ *   NEAREST_NEIGHBOUR  closestValue (Grid.h:383-405).
+ * `grid` selects the grid: CRP_GRID_VALUES the Grid3f of a grid or modulated field; CRP_GRID_MODULATION
+ * the Grid1f of a modulated field.  (JF12's turbulent Grid3f is read trilinear only on the device, and
+ * its striated Grid1f with closestValue, as src/magneticField/JF12Field.cpp:300-301 does.)  A non-trilinear Grid3f
+ * is read from its points: its cell table and FP16 storage are released, and
+ * crp_field_set_grid_layout(CELLS) / crp_field_set_grid_storage(FP16_BRICK) fail until it is
+ * trilinear again (a periodic grid then gets its cell table back).  The mixed-precision path reads
+ * non-trilinear grids with the FP64 path's evaluation, whose result is FP32 already.  (The adjoint,
+ * crp_adjoint_*, differentiates trilinear interpolation only.)  Unknown types fail with
+ * CRP_E_INVALID, as Grid::setInterpolationType throws. */
+enum crp_grid_interpolation { CRP_GRID_TRILINEAR = 0, CRP_GRID_TRICUBIC = 1, CRP_GRID_NEAREST_NEIGHBOUR = 2 };
+enum crp_grid_select { CRP_GRID_VALUES = 0, CRP_GRID_MODULATION = 1 };
+int crp_field_set_grid_interpolation(crp_field_t *f, int grid, int type);
+int crp_field_get_grid_interpolation(const crp_field_t *f, int grid, int *type);
 
 /* B at n host positions with the given precision's field evaluation (MIXED: the FP32 evaluation
  * of the mixed path, FP16 storage if set). */

@@ -199,6 +199,126 @@ static inline void orc_reflective_clamp(double x, int n, int *lo, int *hi, doubl
 	}
 }
 
+/* periodicBoundary, include/crpropa/Grid.h:43-46 */
+int orc_periodic_boundary(int index, int n) { return ((index % n) + n) % n; }
+
+/* reflectiveBoundary, include/crpropa/Grid.h:36-41 */
+int orc_reflective_boundary(int index, int n) {
+	while ((index < -0.5) || (index > (n - 0.5)))
+		index = 2 * n * (index > (n - 0.5)) - index - 1;
+	return index;
+}
+
+/* crpropa::round, include/crpropa/Grid.h:61-63 (not the C library's round: r + 0.5 is rounded
+ * before the floor, so 0.5 - 2^-54 goes up) */
+static inline double orc_round_sym(double r) { return (r > 0.0) ? floor(r + 0.5) : ceil(r - 0.5); }
+
+/* Grid::closestValue (include/crpropa/Grid.h:383-405): flat index of the nearest grid point of a
+ * grid with n points per axis, origin and spacing */
+static size_t orc_closest_index(const int n[3], const double origin[3], const double spacing[3], int reflective,
+                                const double p[3]) {
+	int ix[3];
+	for (int a = 0; a < 3; a++) {
+		const double go = origin[a] + spacing[a] / 2;
+		const double r = (p[a] - go) / spacing[a];
+		if (reflective) {
+			int i = (int)orc_round_sym(r);
+			while ((i < -0.5) || (i > (n[a] - 0.5)))
+				i = 2 * n[a] * (i > (n[a] - 0.5)) - i - 1;
+			ix[a] = i;
+		} else {
+			int i = (int)orc_round_sym(fmod(r, n[a]));
+			ix[a] = (i + n[a] * (i < 0)) % n[a];
+		}
+	}
+	return ((size_t)ix[0] * n[1] + ix[1]) * n[2] + ix[2];
+}
+
+/* CubicInterpolate, include/crpropa/Grid.h:436-456: one lane of the SSE code, in float; position,
+ * position^2 and position^3 formed in double and rounded to float once each (_mm_set1_ps) */
+static float orc_cubic_f32(float p0, float p1, float p2, float p3, double position) {
+	const float c1 = 0.5f, c2 = 1.5f, c3 = 2.f, c4 = 2.5f;
+	const float pos = (float)position;
+	const float pos2 = (float)(position * position);
+	const float pos3 = (float)(position * position * position);
+	const float term = (c1 * p2 - c1 * p0) * pos;
+	const float term2 = ((p0 + c3 * p2) - (c4 * p1 + c1 * p3)) * pos2;
+	const float term3 = ((c2 * p1 + c1 * p3) - (c1 * p0 + c2 * p2)) * pos3;
+	return ((term3 + term2) + term) + p1;
+}
+
+/* CubicInterpolateScalar, include/crpropa/Grid.h:499-501, in double, left to right */
+static double orc_cubic_f64(double p0, double p1, double p2, double p3, double pos) {
+	return ((-0.5 * p0 + 3 / 2. * p1 - 3 / 2. * p2 + 0.5 * p3) * pos * pos * pos +
+	        (p0 - 5 / 2. * p1 + p2 * 2 - 0.5 * p3) * pos * pos + (-0.5 * p0 + 0.5 * p2) * pos + p1);
+}
+
+/* Neighbour indices iX0-1 .. iX0+2 of one axis through periodicGet / reflectiveGet (Grid.h:346-358):
+ * the int offset goes through size_t and back to int unchanged */
+static void orc_cubic_axis(int i0, int n, int reflective, int out[4]) {
+	for (int d = 0; d < 4; d++)
+		out[d] = reflective ? orc_reflective_boundary(i0 - 1 + d, n) : orc_periodic_boundary(i0 - 1 + d, n);
+}
+
+/* Grid<Vector3f>::tricubicInterpolate (include/crpropa/Grid.h:459-492, SIMD build): cubic along z
+ * for each (x, y) neighbour, then along y for each x, then along x; x, y, z lanes independent */
+static void orc_grid_tricubic3(const orc_field *f, const double p[3], double B[3]) {
+	double r[3], fr[3];
+	int i0[3];
+	for (int a = 0; a < 3; a++) {
+		const double go = f->origin[a] + f->spacing[a] / 2;
+		r[a] = (p[a] - go) / f->spacing[a];
+		i0[a] = (int)floor(r[a]);
+		fr[a] = r[a] - i0[a];
+	}
+	int X[4], Y[4], Z[4];
+	orc_cubic_axis(i0[0], f->nx, f->reflective, X);
+	orc_cubic_axis(i0[1], f->ny, f->reflective, Y);
+	orc_cubic_axis(i0[2], f->nz, f->reflective, Z);
+	float vx[4][3];
+	for (int a = 0; a < 4; a++) {
+		float vy[4][3];
+		for (int b = 0; b < 4; b++) {
+			const float *q[4];
+			for (int c = 0; c < 4; c++)
+				q[c] = f->grid + 3 * (((size_t)X[a] * f->ny + Y[b]) * f->nz + Z[c]);
+			for (int l = 0; l < 3; l++)
+				vy[b][l] = orc_cubic_f32(q[0][l], q[1][l], q[2][l], q[3][l], fr[2]);
+		}
+		for (int l = 0; l < 3; l++)
+			vx[a][l] = orc_cubic_f32(vy[0][l], vy[1][l], vy[2][l], vy[3][l], fr[1]);
+	}
+	for (int l = 0; l < 3; l++)
+		B[l] = orc_cubic_f32(vx[0][l], vx[1][l], vx[2][l], vx[3][l], fr[0]);
+}
+
+/* Grid<float>::tricubicInterpolate(double, ...) (include/crpropa/Grid.h:504-536): the same loops in
+ * double over the modulation grid's float values; Grid::interpolate narrows the result to float */
+static float orc_grid1_tricubic(const orc_field *f, const double p[3]) {
+	const int n[3] = {f->mnx, f->mny, f->mnz};
+	double fr[3];
+	int idx[3][4];
+	for (int a = 0; a < 3; a++) {
+		const double go = f->morigin[a] + f->mspacing[a] / 2;
+		const double r = (p[a] - go) / f->mspacing[a];
+		const int i0 = (int)floor(r);
+		fr[a] = r - i0;
+		orc_cubic_axis(i0, n[a], f->mreflective, idx[a]);
+	}
+	double vx[4];
+	for (int a = 0; a < 4; a++) {
+		double vy[4];
+		for (int b = 0; b < 4; b++) {
+			double vz[4];
+			for (int c = 0; c < 4; c++)
+				vz[c] = f->mod[((size_t)idx[0][a] * f->mny + idx[1][b]) * f->mnz + idx[2][c]];
+			vy[b] = orc_cubic_f64(vz[0], vz[1], vz[2], vz[3], fr[2]);
+		}
+		vx[a] = orc_cubic_f64(vy[0], vy[1], vy[2], vy[3], fr[1]);
+	}
+	return (float)orc_cubic_f64(vx[0], vx[1], vx[2], vx[3], fr[0]);
+}
+
 /* Grid<Vector3f>::interpolate + trilinearInterpolate, include/crpropa/Grid.h:318-335,540-582.
  * Weights are narrowed to float (Vector3<float>::operator*(float)), products
  * and the 8-term sum run in float, the result widens to double
@@ -215,6 +335,18 @@ static void orc_grid_eval1(const orc_field *f, const double p[3], double B[3]) {
 			B[0] = B[1] = B[2] = 0;
 			return;
 		}
+	}
+	if (f->interp == CRP_GRID_TRICUBIC) {
+		orc_grid_tricubic3(f, p, B);
+		return;
+	}
+	if (f->interp == CRP_GRID_NEAREST_NEIGHBOUR) {
+		const int n[3] = {f->nx, f->ny, f->nz};
+		const float *v = f->grid + 3 * orc_closest_index(n, f->origin, f->spacing, f->reflective, p);
+		B[0] = v[0];
+		B[1] = v[1];
+		B[2] = v[2];
+		return;
 	}
 	/* gridOrigin = origin + spacing/2 (Grid.h:225-228); r = (pos - gridOrigin)/spacing */
 	double r[3];
@@ -267,6 +399,12 @@ static float orc_grid1_eval(const orc_field *f, const double p[3]) {
 			if (!((p[a] >= f->morigin[a]) && (p[a] <= f->morigin[a] + n * f->mspacing[a])))
 				return 0.f;
 		}
+	}
+	if (f->minterp == CRP_GRID_TRICUBIC)
+		return orc_grid1_tricubic(f, p);
+	if (f->minterp == CRP_GRID_NEAREST_NEIGHBOUR) {
+		const int n[3] = {f->mnx, f->mny, f->mnz};
+		return f->mod[orc_closest_index(n, f->morigin, f->mspacing, f->mreflective, p)];
 	}
 	double r[3];
 	for (int a = 0; a < 3; a++) {
@@ -371,21 +509,7 @@ void orc_jf12_defaults(int planck, crp_jf12_params *p) {
 /* Grid<float>::closestValue (include/crpropa/Grid.h:383-405) on the modulation-grid slots */
 static float orc_grid1_closest(const orc_field *f, const double p[3]) {
 	const int n[3] = {f->mnx, f->mny, f->mnz};
-	int ix[3];
-	for (int a = 0; a < 3; a++) {
-		const double go = f->morigin[a] + f->mspacing[a] / 2;
-		const double r = (p[a] - go) / f->mspacing[a];
-		if (f->mreflective) {
-			int i = (int)round(r);
-			while ((i < -0.5) || (i > (n[a] - 0.5)))
-				i = 2 * n[a] * (i > (n[a] - 0.5)) - i - 1;
-			ix[a] = i;
-		} else {
-			int i = (int)round(fmod(r, n[a]));
-			ix[a] = (i + n[a] * (i < 0)) % n[a];
-		}
-	}
-	return f->mod[((size_t)ix[0] * f->mny + ix[1]) * f->mnz + ix[2]];
+	return f->mod[orc_closest_index(n, f->morigin, f->mspacing, f->mreflective, p)];
 }
 
 /* pow(x, 2.) as the reference's build evaluates it: GCC rewrites the call as x * x (the correctly

@@ -59,6 +59,9 @@ typedef struct orc_field_s {
 	double extends[3], porigin[3];
 	int preflective;
 	double evo_m;
+	/* Grid::setInterpolationType (include/crpropa/Grid.h:24-28,255-264) of the Grid3f (interp) and of
+	 * the Grid1f modulation (minterp): CRP_GRID_TRILINEAR, CRP_GRID_TRICUBIC, CRP_GRID_NEAREST_NEIGHBOUR */
+	int interp, minterp;
 } orc_field;
 
 /* JF12Field / PlanckJF12bField constructor values (src/magneticField/JF12Field.cpp:8-79,348-363) */
@@ -177,6 +180,10 @@ void orc_sample_particles(uint32_t seed, size_t n, double emin, double emax, con
 
 /* fields: MagneticField::getField(position) at m positions (AoS) */
 void orc_field_eval(const orc_field *f, const double *pos, size_t m, double *B, int libm);
+/* periodicBoundary / reflectiveBoundary (include/crpropa/Grid.h:36-46): the neighbour index of
+ * tricubic interpolation continued periodically or by mirroring */
+int orc_periodic_boundary(int index, int n);
+int orc_reflective_boundary(int index, int n);
 /* MagneticField::getField(position, z), the call the propagators make, with z[i] per position */
 void orc_field_eval_z(const orc_field *f, const double *pos, const double *z, size_t m, double *B, int libm);
 void orc_planewave_init(double brms, double lmin, double lmax, double lbendover, double sindex, double qindex,

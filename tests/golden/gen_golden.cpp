@@ -919,12 +919,191 @@ static void gen_decorators() {
 	}
 }
 
+// ---------------------------------------------------------------- tricubic / nearest neighbour
+#ifdef HAVE_SIMD
+// Grid::setInterpolationType (include/crpropa/Grid.h:255-264) with TRICUBIC (Grid3f: Grid.h:408-492,
+// Grid1f: :499-536) and NEAREST_NEIGHBOUR (closestValue, :383-405).  Only a build with HAVE_SIMD
+// (SIMD_EXTENSIONS "avx": -DHAVE_SIMD -msse ... -mavx, no -mfma) evaluates tricubic Grid3f; the
+// target ref-simd of oracle/Makefile builds this generator that way and runs it with "interp".
+static void fill_noise(Grid3f &g, uint32_t seed, double amp) {
+	Random rng(seed);
+	for (size_t ix = 0; ix < g.getNx(); ix++)
+		for (size_t iy = 0; iy < g.getNy(); iy++)
+			for (size_t iz = 0; iz < g.getNz(); iz++) {
+				Vector3f &b = g.get(ix, iy, iz);
+				b.x = rng.randUniform(-1, 1) * amp;
+				b.y = rng.randUniform(-1, 1) * amp;
+				b.z = rng.randUniform(-1, 1) * amp;
+			}
+}
+
+static void gen_interp() {
+	// field samples: a 9 x 8 x 7 Grid3f and a 6 x 5 x 7 Grid1f, both with spacing vectors
+	{
+		const size_t nx = 9, ny = 8, nz = 7, mx = 6, my = 5, mz = 7;
+		const Vector3d sp(0.7 * kpc, 0.9 * kpc, 1.1 * kpc), origin(-1.5 * kpc, 0.25 * kpc, 2 * kpc);
+		const Vector3d msp(1.3 * kpc, 0.8 * kpc, 1.0 * kpc), mo(0.4 * kpc, -2 * kpc, 1.5 * kpc);
+		ref_ptr<Grid3f> g = new Grid3f(origin, nx, ny, nz, sp);
+		fill_noise(*g, 808, nG);
+		ref_ptr<Grid1f> m = new Grid1f(mo, mx, my, mz, msp);
+		{
+			Random rm(809);
+			for (size_t ix = 0; ix < mx; ix++)
+				for (size_t iy = 0; iy < my; iy++)
+					for (size_t iz = 0; iz < mz; iz++)
+						m->get(ix, iy, iz) = (float)rm.randUniform(0.2, 1.8);
+		}
+		Random rng(810);
+		const size_t M = 4096;
+		std::vector<double> pos(M * 3);
+		const Vector3d L(nx * sp.x, ny * sp.y, nz * sp.z);
+		for (size_t i = 0; i < M; i++)
+			for (int a = 0; a < 3; a++) {
+				const double o = origin.data[a], l = L.data[a], h = sp.data[a];
+				const int n = (int)(l / h + 0.5);
+				double x;
+				switch (i % 8) {
+				case 0: x = o + rng.randUniform(0, l); break;                          // inside
+				case 1: x = o + rng.randUniform(-3 * l, 4 * l); break;                 // a few periods out
+				case 2: x = o + h / 2 + h * (int)rng.randUniform(-n, 2 * n); break;    // on grid points
+				case 3: x = o + h * (int)rng.randUniform(-n, 2 * n); break;            // half-way between points
+				case 4: x = o + rng.randUniform(-h, h); break;                         // near the lower edge
+				case 5: x = o + l + rng.randUniform(-h, h); break;                     // near the upper edge
+				case 6: x = o + rng.randUniform(-300 * l, 300 * l); break;             // far away
+				default: x = mo.data[a] + msp.data[a] * (0.5 + (int)rng.randUniform(-8, 16)); break; // Grid1f points
+				}
+				pos[3 * i + a] = x;
+			}
+		std::string dir = casedir("interp_grids");
+		save_grid(dir, g);
+		std::vector<float> mv(mx * my * mz);
+		for (size_t i = 0; i < mv.size(); i++)
+			mv[i] = m->getGrid()[i];
+		save_f32(dir, "mod", mv, {mx, my, mz});
+		save_f64(dir, "pos", pos, {M, 3});
+		struct Mode {
+			const char *name;
+			interpolationType t;
+			bool reflective, clip;
+		} modes[5] = {{"tricubic_periodic", TRICUBIC, false, false},
+		              {"tricubic_reflective", TRICUBIC, true, false},
+		              {"tricubic_clip", TRICUBIC, false, true},
+		              {"nearest_periodic", NEAREST_NEIGHBOUR, false, false},
+		              {"nearest_reflective", NEAREST_NEIGHBOUR, true, false}};
+		for (const Mode &md : modes) {
+			g->setInterpolationType(md.t);
+			g->setReflective(md.reflective);
+			g->setClipVolume(md.clip);
+			m->setInterpolationType(md.t);
+			m->setReflective(md.reflective);
+			m->setClipVolume(md.clip);
+			MagneticFieldGrid field(g);
+			std::vector<double> B(M * 3);
+			std::vector<float> S(M);
+			for (size_t i = 0; i < M; i++) {
+				const Vector3d x(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+				const Vector3d b = field.getField(x);
+				B[3 * i] = b.x; B[3 * i + 1] = b.y; B[3 * i + 2] = b.z;
+				S[i] = m->interpolate(x);
+			}
+			save_f64(dir, std::string("B_") + md.name, B, {M, 3});
+			save_f32(dir, std::string("m_") + md.name, S, {M});
+		}
+		save_meta(dir, "{\"shape\": [9, 8, 7], \"spacing\": [" + fmt(sp.x) + ", " + fmt(sp.y) + ", " + fmt(sp.z) +
+		                   "], \"origin\": [" + fmt(origin.x) + ", " + fmt(origin.y) + ", " + fmt(origin.z) +
+		                   "], \"mod_shape\": [6, 5, 7], \"mod_spacing\": [" + fmt(msp.x) + ", " + fmt(msp.y) + ", " +
+		                   fmt(msp.z) + "], \"mod_origin\": [" + fmt(mo.x) + ", " + fmt(mo.y) + ", " + fmt(mo.z) +
+		                   "], \"grid_seed\": 808, \"mod_seed\": 809, \"pos_seed\": 810, \"amp\": " + fmt(nG) +
+		                   ", \"build\": \"HAVE_SIMD, -mavx, no FMA\"}");
+		printf("  interp_grids M=%zu\n", M);
+	}
+
+	// trajectories through the shared 32^3 grid (tests/golden/grid32) read tricubic or nearest
+	const int n = 32;
+	const double sp = 50 * kpc;
+	ref_ptr<Grid3f> g32 = make_noise_grid(n, sp, Vector3d(0.), 2024, nG);
+	Vector3d centre(n * sp / 2, n * sp / 2, n * sp / 2);
+	std::vector<int> proton = {nucleusId(60, 1)};
+	std::vector<int> species = {nucleusId(60, 1), nucleusId(60, 2), nucleusId(60, 7), nucleusId(60, 14), nucleusId(60, 26)};
+	std::vector<double> one = {1.0}, wts = {0.5, 0.2, 0.15, 0.1, 0.05};
+	{
+		ref_ptr<Grid3f> gt = new Grid3f(*g32);
+		gt->setInterpolationType(TRICUBIC);
+		ref_ptr<MagneticField> f = new MagneticFieldGrid(gt);
+		std::vector<Particle> ps = make_particles(128, 70, 1 * EeV, 300 * EeV, centre, species, wts);
+		PropagationCK ck(f, 1e-4, 1 * kpc, 1 * Mpc);
+		MaximumTrajectoryLength ml(10 * Mpc);
+		run_traj("ck_grid32_tricubic", ck, &ml, ps, 3000, {1, 10, 100, 1000, 3000},
+		         "{\"propagator\": \"CK\", \"tolerance\": 1e-4, \"minStep\": " + fmt(kpc) + ", \"maxStep\": " +
+		             fmt(Mpc) + ", \"maxTrajectoryLength\": " + fmt(10 * Mpc) +
+		             ", \"field\": \"grid32\", \"interpolation\": \"TRICUBIC\", \"particle_seed\": 70}");
+	}
+	{
+		ref_ptr<Grid3f> gt = new Grid3f(*g32);
+		gt->setInterpolationType(TRICUBIC);
+		gt->setReflective(true);
+		ref_ptr<MagneticField> f = new MagneticFieldGrid(gt);
+		std::vector<Particle> ps = make_particles(128, 71, 1 * EeV, 100 * EeV, centre, proton, one);
+		PropagationBP bp(f, 0.1 * Mpc);
+		run_traj("bp_fixed_grid32_tricubic_reflective", bp, nullptr, ps, 600, {1, 10, 100, 600},
+		         "{\"propagator\": \"BP\", \"fixedStep\": " + fmt(0.1 * Mpc) +
+		             ", \"field\": \"grid32\", \"interpolation\": \"TRICUBIC\", \"reflective\": true, "
+		             "\"particle_seed\": 71}");
+	}
+	{
+		ref_ptr<Grid3f> gn = new Grid3f(*g32);
+		gn->setInterpolationType(NEAREST_NEIGHBOUR);
+		gn->setReflective(true);
+		ref_ptr<MagneticField> f = new MagneticFieldGrid(gn);
+		std::vector<Particle> ps = make_particles(128, 72, 1 * EeV, 100 * EeV, centre, proton, one);
+		PropagationBP bp(f, 0.1 * Mpc);
+		run_traj("bp_fixed_grid32_nearest_reflective", bp, nullptr, ps, 600, {1, 10, 100, 600},
+		         "{\"propagator\": \"BP\", \"fixedStep\": " + fmt(0.1 * Mpc) +
+		             ", \"field\": \"grid32\", \"interpolation\": \"NEAREST_NEIGHBOUR\", \"reflective\": true, "
+		             "\"particle_seed\": 72}");
+	}
+	{
+		// ModulatedMagneticFieldGrid over grid32 (periodic, tricubic) and a reflective 9x11x10 Grid1f
+		// (tricubic), as bp_fixed_grid32_modulated builds them otherwise
+		Vector3d mo(-0.3 * Mpc, 0.1 * Mpc, 0.25 * Mpc);
+		const double msp = 0.2 * Mpc;
+		ref_ptr<Grid1f> mg = make_mod_grid(9, 11, 10, msp, mo, 777);
+		ref_ptr<Grid3f> gt = new Grid3f(*g32);
+		gt->setInterpolationType(TRICUBIC);
+		mg->setInterpolationType(TRICUBIC);
+		ref_ptr<MagneticField> f = new ModulatedMagneticFieldGrid(gt, mg);
+		std::vector<Particle> ps = make_particles(128, 73, 1 * EeV, 300 * EeV, centre, species, wts);
+		PropagationCK ck(f, 1e-4, 1 * kpc, 1 * Mpc);
+		MaximumTrajectoryLength ml(8 * Mpc);
+		run_traj("ck_grid32_modulated_tricubic", ck, &ml, ps, 3000, {1, 10, 100, 1000, 3000},
+		         "{\"propagator\": \"CK\", \"tolerance\": 1e-4, \"minStep\": " + fmt(kpc) + ", \"maxStep\": " +
+		             fmt(Mpc) + ", \"maxTrajectoryLength\": " + fmt(8 * Mpc) +
+		             ", \"field\": \"grid32_modulated\", \"interpolation\": \"TRICUBIC\", \"mod_shape\": [9, 11, 10], "
+		             "\"mod_reflective\": true, \"mod_spacing\": " + fmt(msp) + ", \"mod_origin\": [" + fmt(mo.x) + ", " + fmt(mo.y) + ", " +
+		             fmt(mo.z) + "], \"mod_seed\": 777, \"particle_seed\": 73}");
+		std::vector<float> mv(9 * 11 * 10);
+		for (size_t i = 0; i < mv.size(); i++)
+			mv[i] = mg->getGrid()[i];
+		save_f32(casedir("ck_grid32_modulated_tricubic"), "mod", mv, {9, 11, 10});
+	}
+}
+#endif
+
 int main(int argc, char **argv) {
 	g_root = argc > 1 ? argv[1] : "tests/golden";
 	mkdir(g_root.c_str(), 0755);
 	if (argc > 2 && std::string(argv[2]) == "decorators") { // only the field-decorator cases
 		gen_decorators();
 		return 0;
+	}
+	if (argc > 2 && std::string(argv[2]) == "interp") { // only the tricubic / nearest-neighbour cases
+#ifdef HAVE_SIMD
+		gen_interp();
+		return 0;
+#else
+		fprintf(stderr, "gen_golden: the interp cases need a build with HAVE_SIMD (oracle/Makefile target ref-simd)\n");
+		return 1;
+#endif
 	}
 	printf("gen_golden: glibc %s, g++ %s\n", gnu_get_libc_version(), __VERSION__);
 	{

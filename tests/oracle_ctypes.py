@@ -38,6 +38,11 @@ _fp = _P(C.c_float)
 _ip = _P(C.c_int32)
 
 
+# Grid::setInterpolationType values (crp_grid_interpolation)
+GRID_TRILINEAR, GRID_TRICUBIC, GRID_NEAREST = 0, 1, 2
+GRID_INTERP = {"TRILINEAR": GRID_TRILINEAR, "TRICUBIC": GRID_TRICUBIC, "NEAREST_NEIGHBOUR": GRID_NEAREST}
+
+
 class Field(C.Structure):
     pass
 
@@ -58,6 +63,7 @@ Field._fields_ = [
         ("children", C.POINTER(C.POINTER(Field))), ("nchildren", C.c_int),
         ("extends", C.c_double * 3), ("porigin", C.c_double * 3), ("preflective", C.c_int),
         ("evo_m", C.c_double),
+        ("interp", C.c_int), ("minterp", C.c_int),
 ]
 
 
@@ -116,6 +122,8 @@ def lib():
         L.orc_propagate.argtypes = [_P(Field), _P(Propagator), _P(State), C.c_size_t, C.c_longlong, C.c_int,
                                     C.c_int, _P(C.c_longlong)]
         L.orc_field_eval.argtypes = [_P(Field), _dp, C.c_size_t, _dp, C.c_int]
+        L.orc_periodic_boundary.argtypes = [C.c_int, C.c_int]
+        L.orc_reflective_boundary.argtypes = [C.c_int, C.c_int]
         L.orc_field_eval_z.argtypes = [_P(Field), _dp, _dp, C.c_size_t, _dp, C.c_int]
         L.orc_step_batch.argtypes = [_P(Field), C.c_int, C.c_size_t, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                      _dp, C.c_int]
@@ -180,7 +188,7 @@ class OracleField:
         return f
 
     @staticmethod
-    def grid(values, origin, spacing, reflective=False, clip=False):
+    def grid(values, origin, spacing, reflective=False, clip=False, interp=0):
         f = OracleField()
         v = np.ascontiguousarray(values, dtype=np.float32)
         f._keep.append(v)
@@ -193,13 +201,14 @@ class OracleField:
             f.s.spacing[a] = sp[a]
         f.s.reflective = int(reflective)
         f.s.clip = int(clip)
+        f.s.interp = int(interp)
         return f
 
     @staticmethod
     def modulated(values, origin, spacing, mod, morigin, mspacing, reflective=False, mreflective=True,
-                  clip=False, mclip=False):
+                  clip=False, mclip=False, interp=0, minterp=0):
         """ModulatedMagneticFieldGrid: Grid3f values times a Grid1f (one float per point)."""
-        f = OracleField.grid(values, origin, spacing, reflective, clip)
+        f = OracleField.grid(values, origin, spacing, reflective, clip, interp)
         m = np.ascontiguousarray(mod, dtype=np.float32)
         f._keep.append(m)
         f.s.type = FIELD_MODULATED
@@ -211,6 +220,7 @@ class OracleField:
             f.s.mspacing[a] = msp[a]
         f.s.mreflective = int(mreflective)
         f.s.mclip = int(mclip)
+        f.s.minterp = int(minterp)
         return f
 
     @staticmethod
@@ -459,16 +469,17 @@ def field_for_case(case):
     """Build the oracle field a golden trajectory case ran in."""
     meta = case["meta"]
     f = meta["field"] if "field" in meta else None
+    ip = GRID_INTERP[meta.get("interpolation", "TRILINEAR")]
     if f == "grid32":
         g = load_case("grid32")
         gm = g["meta"]["grid"]
         return OracleField.grid(g["grid"], gm["origin"], gm["spacing"], reflective=meta.get("reflective", False),
-                                clip=meta.get("clip", False))
+                                clip=meta.get("clip", False), interp=ip)
     if f == "grid32_modulated":
         g = load_case("grid32")
         gm = g["meta"]["grid"]
         return OracleField.modulated(g["grid"], gm["origin"], gm["spacing"], case["mod"], meta["mod_origin"],
-                                     meta["mod_spacing"], mreflective=meta["mod_reflective"])
+                                     meta["mod_spacing"], mreflective=meta["mod_reflective"], interp=ip, minterp=ip)
     if f in ("list(grid32, uniform [1 nG, 0, 0])", "periodic(grid32) reflective", "evolution(grid32, m=1.5)"):
         g = load_case("grid32")
         gm = g["meta"]["grid"]
