@@ -6,6 +6,7 @@
 
 #include <sstream>
 #include <stdexcept>
+#include <string>
 
 namespace crpropa_amd {
 
@@ -22,14 +23,27 @@ crp_field_t *makeDeviceField(ref_ptr<MagneticField> field) {
 		return nullptr; // no field: B = 0 (src/module/PropagationBP.cpp:137-149)
 	if (MagneticFieldGrid *mg = dynamic_cast<MagneticFieldGrid *>(field.get())) {
 		ref_ptr<Grid3f> g = mg->getGrid();
-		if (g->getInterpolationType() != TRILINEAR)
-			throw std::runtime_error("GpuPropagation: only trilinear grids run on the device");
+		int ipol = CRP_GRID_TRILINEAR;
+		switch (g->getInterpolationType()) { // Grid::setInterpolationType (include/crpropa/Grid.h:24-28)
+		case TRILINEAR: ipol = CRP_GRID_TRILINEAR; break;
+		case TRICUBIC: ipol = CRP_GRID_TRICUBIC; break;
+		case NEAREST_NEIGHBOUR: ipol = CRP_GRID_NEAREST_NEIGHBOUR; break;
+		default: throw std::runtime_error("InterpolationType: unknown interpolation type");
+		}
 		const Vector3d o = g->getOrigin(), s = g->getSpacing();
 		const double origin[3] = {o.x, o.y, o.z}, spacing[3] = {s.x, s.y, s.z};
 		// Vector3f is three packed floats: the grid's storage is the AoS layout the C ABI takes
 		const float *values = reinterpret_cast<const float *>(g->getGrid().data());
 		check(crp_field_create_grid3f(values, (int)g->getNx(), (int)g->getNy(), (int)g->getNz(), origin, spacing,
 		                              g->isReflective() ? 1 : 0, g->getClipVolume() ? 1 : 0, &f));
+		if (ipol != CRP_GRID_TRILINEAR) {
+			const int rc = crp_field_set_grid_interpolation(f, CRP_GRID_VALUES, ipol);
+			if (rc != CRP_OK) {
+				const std::string msg = crp_last_error();
+				crp_field_destroy(f);
+				throw std::runtime_error(msg);
+			}
+		}
 		return f;
 	}
 	if (dynamic_cast<UniformMagneticField *>(field.get())) {

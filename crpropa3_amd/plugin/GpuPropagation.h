@@ -8,8 +8,9 @@
 //   sim->add(ck);                                                 // per-candidate process(), or
 //   ck->run(candidates, 100 * Mpc);                               // one batched device run
 //
-// Fields with a device counterpart: MagneticFieldGrid (trilinear Grid3f, periodic, reflective or
-// clipVolume) and UniformMagneticField; any other field is rejected with std::runtime_error.
+// Fields with a device counterpart: MagneticFieldGrid (Grid3f trilinear, tricubic or nearest
+// neighbour; periodic, reflective or clipVolume) and UniformMagneticField; any other field is rejected
+// with std::runtime_error.
 #ifndef CRPROPA3_AMD_GPUPROPAGATION_H
 #define CRPROPA3_AMD_GPUPROPAGATION_H
 
