@@ -1099,6 +1099,12 @@ int main(int argc, char **argv) {
 	if (argc > 2 && std::string(argv[2]) == "interp") { // only the tricubic / nearest-neighbour cases
 #ifdef HAVE_SIMD
 		gen_interp();
+		FILE *f = fopen((g_root + "/PROVENANCE.txt").c_str(), "a");
+		fprintf(f, "interp_grids, *_tricubic*, *_nearest*: the same generator built with the reference's SIMD_EXTENSIONS=avx "
+		           "flags (-DHAVE_SIMD -msse -msse2 -msse3 -msse4.1 -msse4.2 -mavx, no -mfma; oracle/Makefile target "
+		           "ref-simd), g++ %s\n",
+		        __VERSION__);
+		fclose(f);
 		return 0;
 #else
 		fprintf(stderr, "gen_golden: the interp cases need a build with HAVE_SIMD (oracle/Makefile target ref-simd)\n");
