@@ -118,14 +118,18 @@ struct FieldComposite {
 
 	// JF12 and the grids read tricubic or nearest neighbour share one out-of-line call site
 	__device__ static __attribute__((noinline)) V3 leaf_eval_call(const CompLeaf *Lp, const V3 &x) {
-		if (ld_const(&Lp->kind) == CL_JF12)
-			return ld_const(&Lp->jf12).eval(x);
+		if (ld_const(&Lp->kind) == CL_JF12) {
+			const FieldJF12 j = ld_const(&Lp->jf12);
+			return j.turbulent.interp == kInterpTrilinear ? j.eval(x) : j.eval_any(x);
+		}
 		const GridDesc mod = ld_const(&Lp->mod);
 		return FieldGridAny{ld_const(&Lp->grid), mod, mod.values != nullptr}.eval(x);
 	}
 	__device__ static __attribute__((noinline)) float3 leaf_evalf_call(const CompLeaf *Lp, const V3 &x) {
-		if (ld_const(&Lp->kind) == CL_JF12)
-			return ld_const(&Lp->jf12).evalf(x);
+		if (ld_const(&Lp->kind) == CL_JF12) {
+			const FieldJF12 j = ld_const(&Lp->jf12);
+			return j.turbulent.interp == kInterpTrilinear ? j.evalf(x) : FieldJF12Any{j}.evalf(x);
+		}
 		const GridDesc mod = ld_const(&Lp->mod);
 		return FieldGridAny{ld_const(&Lp->grid), mod, mod.values != nullptr}.evalf(x);
 	}

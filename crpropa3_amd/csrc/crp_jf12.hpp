@@ -168,12 +168,12 @@ struct FieldJF12 {
 	}
 
 	// JF12Field::getField (:337-346); the turbulent term is Vector3f * (float)strength in float.
-	// Out of line: one copy serves the six stages of a Cash-Karp step.
-	__device__ __attribute__((noinline)) V3 eval(const V3 &pos) const {
+	// AnyInterp: the turbulent grid may be read tricubic or nearest-neighbour (Grid::interpolate).
+	template <bool AnyInterp> __device__ __forceinline__ V3 eval_impl(const V3 &pos) const {
 		double bx = 0, by = 0, bz = 0;
 		Geo gm = geo(pos);
 		if (p.use_turbulent) {
-			const V3 g = FieldGrid{turbulent}.eval(pos);
+			const V3 g = AnyInterp ? grid_interpolate(turbulent, pos) : FieldGrid{turbulent}.eval(pos);
 			const float s = (float)turbulent_strength(pos, gm);
 			bx += (double)((float)g.x * s);
 			by += (double)((float)g.y * s);
@@ -193,8 +193,23 @@ struct FieldJF12 {
 		}
 		return v3(bx, by, bz);
 	}
+	// Out of line: one copy serves the six stages of a Cash-Karp step.
+	__device__ __attribute__((noinline)) V3 eval(const V3 &pos) const { return eval_impl<false>(pos); }
 	__device__ float3 evalf(const V3 &pos) const {
 		const V3 b = eval(pos);
+		return make_float3((float)b.x, (float)b.y, (float)b.z);
+	}
+	// the same with a turbulent grid of any interpolation type; its own copy, so that the trilinear
+	// one keeps its register allocation
+	__device__ __attribute__((noinline)) V3 eval_any(const V3 &pos) const { return eval_impl<true>(pos); }
+};
+
+// JF12Field whose turbulent Grid3f is read tricubic or nearest-neighbour
+struct FieldJF12Any {
+	FieldJF12 f;
+	__device__ __forceinline__ V3 eval(const V3 &pos) const { return f.eval_any(pos); }
+	__device__ __forceinline__ float3 evalf(const V3 &pos) const {
+		const V3 b = f.eval_any(pos);
 		return make_float3((float)b.x, (float)b.y, (float)b.z);
 	}
 };

@@ -659,6 +659,11 @@ static bool any_interp(const crp_field *f) {
 	return false;
 }
 
+// a JF12 field whose turbulent grid is not read trilinear: FieldJF12Any
+static bool jf12_any(const crp_field *f) {
+	return f && f->kind == FK_JF12 && f->jf12.turbulent.interp != kInterpTrilinear;
+}
+
 static FieldGridAny grid_any(const crp_field *f) {
 	FieldGridAny a;
 	std::memset(&a, 0, sizeof(a));
@@ -955,6 +960,8 @@ static GridDesc *interp_grid(crp_field *f, int grid) {
 		return nullptr;
 	if (grid == CRP_GRID_VALUES && (f->kind == FK_GRID || f->kind == FK_MODULATED))
 		return &f->grid;
+	if (grid == CRP_GRID_VALUES && f->kind == FK_JF12 && f->jf12.turbulent.values)
+		return &f->jf12.turbulent;
 	if (grid == CRP_GRID_MODULATION && f->kind == FK_MODULATED)
 		return &f->mod;
 	return nullptr;
@@ -1147,7 +1154,12 @@ static hipError_t launch_field_eval(const crp_field_t *f, const double *dpos, do
 				hipLaunchKernelGGL(k_field_eval<FieldModulated<FieldGrid>>, blocks, 256, 0, 0,
 				                   FieldModulated<FieldGrid>{FieldGrid{f->grid}, f->mod}, dpos, dB, n);
 			break;
-		case FK_JF12: hipLaunchKernelGGL(k_field_eval<FieldJF12>, blocks, 256, 0, 0, f->jf12, dpos, dB, n); break;
+		case FK_JF12:
+			if (jf12_any(f))
+				hipLaunchKernelGGL(k_field_eval<FieldJF12Any>, blocks, 256, 0, 0, FieldJF12Any{f->jf12}, dpos, dB, n);
+			else
+				hipLaunchKernelGGL(k_field_eval<FieldJF12>, blocks, 256, 0, 0, f->jf12, dpos, dB, n);
+			break;
 		case FK_COMPOSITE:
 			hipLaunchKernelGGL(k_field_eval<FieldComposite>, blocks, 256, 0, 0, FieldComposite{f->dcomp, 1.0}, dpos, dB, n);
 			break;
@@ -1175,7 +1187,12 @@ static hipError_t launch_field_eval(const crp_field_t *f, const double *dpos, do
 				hipLaunchKernelGGL(k_field_evalf<FieldModulated<FieldGrid>>, blocks, 256, 0, 0,
 				                   FieldModulated<FieldGrid>{FieldGrid{f->grid}, f->mod}, dpos, dB, n);
 			break;
-		case FK_JF12: hipLaunchKernelGGL(k_field_evalf<FieldJF12>, blocks, 256, 0, 0, f->jf12, dpos, dB, n); break;
+		case FK_JF12:
+			if (jf12_any(f))
+				hipLaunchKernelGGL(k_field_evalf<FieldJF12Any>, blocks, 256, 0, 0, FieldJF12Any{f->jf12}, dpos, dB, n);
+			else
+				hipLaunchKernelGGL(k_field_evalf<FieldJF12>, blocks, 256, 0, 0, f->jf12, dpos, dB, n);
+			break;
 		case FK_COMPOSITE:
 			hipLaunchKernelGGL(k_field_evalf<FieldComposite>, blocks, 256, 0, 0, FieldComposite{f->dcomp, 1.0}, dpos, dB, n);
 			break;
@@ -1497,7 +1514,7 @@ template <class Fn> static int with_fp64_field(const crp_field *f, Fn &&fn) {
 	case FK_MODULATED:
 		return f->cells ? fn(FieldModulated<FieldGridCells>{FieldGridCells{f->grid, f->cells}, f->mod})
 		                : fn(FieldModulated<FieldGrid>{FieldGrid{f->grid}, f->mod});
-	case FK_JF12: return fn(f->jf12);
+	case FK_JF12: return jf12_any(f) ? fn(FieldJF12Any{f->jf12}) : fn(f->jf12);
 	case FK_COMPOSITE: return fn(FieldComposite{f->dcomp, 1.0});
 	default: return fn(FieldNone{});
 	}
@@ -2187,7 +2204,10 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 		                              &st.launches)
 		              : dispatch_kind(FieldModulated<FieldGrid>{FieldGrid{f->grid}, f->mod}, p, P, b, &st.launches);
 		break;
-	case FK_JF12: rc = dispatch_kind(f->jf12, p, P, b, &st.launches); break;
+	case FK_JF12:
+		rc = jf12_any(f) ? dispatch_kind(FieldJF12Any{f->jf12}, p, P, b, &st.launches)
+		                 : dispatch_kind(f->jf12, p, P, b, &st.launches);
+		break;
 	case FK_COMPOSITE: rc = dispatch_kind(FieldComposite{f->dcomp, 1.0}, p, P, b, &st.launches); break;
 	default: rc = dispatch_kind(FieldNone{}, p, P, b, &st.launches); break;
 	}

@@ -486,8 +486,6 @@ crp_field *JF12Field::deviceFieldLocked() const {
 	    (!s || devStrVersion == s->getVersion()) && (!t || devTurbVersion == t->getVersion()))
 		return dev;
 	releaseDevice();
-	if (t && t->getInterpolationType() != TRILINEAR)
-		throw std::runtime_error("crpropa3_amd: JF12Field's turbulent grid is read TRILINEAR on the GPU");
 	int sn[3] = {0, 0, 0}, tn[3] = {0, 0, 0};
 	double so[3] = {0, 0, 0}, ss[3] = {1, 1, 1}, to[3] = {0, 0, 0}, ts[3] = {1, 1, 1};
 	if (s) {
@@ -507,6 +505,8 @@ crp_field *JF12Field::deviceFieldLocked() const {
 	                              s && s->isReflective() ? 1 : 0,
 	                              t ? reinterpret_cast<const float *>(t->getGrid().data()) : nullptr, t ? tn : nullptr,
 	                              to, ts, t && t->isReflective() ? 1 : 0, t && t->getClipVolume() ? 1 : 0, &f));
+	if (t)
+		setDeviceInterpolation(f, CRP_GRID_VALUES, t->getInterpolationType());
 	dev = f;
 	devParams = p;
 	devStr = s;

@@ -117,9 +117,10 @@ int crp_field_set_grid_storage(crp_field_t *f, int storage);
  *                      its "none" build rejects tricubic Grid3f); Grid1f: tricubicInterpolate(double)
  *                      (Grid.h:499-536) in FP64, the result narrowed to float;
  *   NEAREST_NEIGHBOUR  closestValue (Grid.h:383-405).
- * `grid` selects the grid: CRP_GRID_VALUES the Grid3f of a grid or modulated field; CRP_GRID_MODULATION
- * the Grid1f of a modulated field.  (JF12's turbulent Grid3f is read trilinear only on the device, and
- * its striated Grid1f with closestValue, as src/magneticField/JF12Field.cpp:300-301 does.)  A non-trilinear Grid3f
+ * `grid` selects the grid: CRP_GRID_VALUES the Grid3f of a grid or modulated field, or the turbulent
+ * Grid3f of a JF12 field (JF12Field.cpp:340); CRP_GRID_MODULATION the Grid1f of a modulated field.
+ * (JF12's striated Grid1f is always read with closestValue, as src/magneticField/JF12Field.cpp:300-301
+ * does.)  A non-trilinear Grid3f
  * is read from its points: its cell table and FP16 storage are released, and
  * crp_field_set_grid_layout(CELLS) / crp_field_set_grid_storage(FP16_BRICK) fail until it is
  * trilinear again (a periodic grid then gets its cell table back).  The mixed-precision path reads

@@ -155,8 +155,8 @@ def test_decorated_tricubic_grid_matches_oracle():
 
 
 def test_python_grid_classes():
-    """The reference's classes: Grid3f / Grid1f setInterpolationType reach the device; a JF12Field
-    turbulent grid that is not trilinear is refused."""
+    """The reference's classes: Grid3f / Grid1f setInterpolationType reach the device, for grid,
+    modulated and JF12 fields."""
     from crpropa3_amd import (NEAREST_NEIGHBOUR, TRICUBIC, TRILINEAR, Grid1f, Grid3f, JF12Field, MagneticFieldGrid,
                               ModulatedMagneticFieldGrid, Vector3d)
     c = O.load_case("interp_grids")
@@ -184,10 +184,53 @@ def test_python_grid_classes():
     want = O.OracleField.modulated(c["grid"], m["origin"], m["spacing"], c["mod"], m["mod_origin"], m["mod_spacing"],
                                    interp=O.GRID_TRICUBIC, minterp=O.GRID_TRICUBIC).eval(pos)
     assert bits_equal(sample(mf), want)
-    g.setInterpolationType(TRILINEAR)
+    # JF12Field's turbulent part (JF12Field.cpp:340) through the same Grid3f, tricubic
     j = JF12Field()
     j.setTurbulentGrid(g)
-    j.getField(Vector3d(1.0, 2.0, 3.0))
-    g.setInterpolationType(TRICUBIC)
-    with pytest.raises(RuntimeError, match="TRILINEAR"):
-        j.getField(Vector3d(1.0, 2.0, 3.0))
+    j.setUseTurbulentField(True)
+    q = O.jf12_defaults()
+    q.use_turbulent = 1
+    og = O.OracleField.grid(c["grid"], m["origin"], m["spacing"], interp=O.GRID_TRICUBIC)
+    og.s.type = O.FIELD_JF12
+    og._keep.append(q)
+    og.s.jf12 = O.C.cast(O.C.pointer(q), O.C.c_void_p)
+    assert bits_equal(sample(j), og.eval(pos, libm=O.LIBM_CR))
+
+
+@pytest.mark.parametrize("interp", ["TRICUBIC", "NEAREST_NEIGHBOUR"])
+def test_jf12_turbulent_grid_interpolation_matches_oracle(interp):
+    """JF12Field's turbulent part reads its grid with Grid::interpolate (src/magneticField/JF12Field.cpp:340):
+    tricubic and nearest neighbour on the device (its own JF12 field type), bit for bit against the oracle
+    with the correctly rounded functions, in field samples and a Cash-Karp run through the full model."""
+    c = O.load_case("jf12")
+    m = c["meta"]
+    p, q = capi.jf12_default_params(), O.jf12_defaults()
+    for x in (p, q):
+        x.use_striated = 1
+        x.use_turbulent = 1
+    striated = O.jf12_striated(m["striated_seed"])
+    dev = capi.Field.jf12(p, striated=striated, striated_spacing=0.1 * O.kpc, turbulent=c["grid"],
+                          turbulent_spacing=m["turb_spacing"])
+    dev.set_interpolation(interp)
+    assert dev.interpolation() == interp
+    orc = O.OracleField.jf12(q, striated, 0.1 * O.kpc, c["grid"], m["turb_spacing"])
+    orc.s.interp = O.GRID_INTERP[interp]
+    B = dev.eval(c["pos"])
+    assert bits_equal(B, orc.eval(c["pos"], libm=O.LIBM_CR))
+    trilinear = O.OracleField.jf12(q, striated, 0.1 * O.kpc, c["grid"], m["turb_spacing"]).eval(c["pos"],
+                                                                                           libm=O.LIBM_CR)
+    assert not np.array_equal(B, trilinear)
+    t = O.load_case("ck_jf12")
+    tm = t["meta"]
+    n = len(t["energy"])
+    prop = capi.Propagator("CK", dev, tm["tolerance"], tm["minStep"], tm["maxStep"],
+                           max_length=tm["maxTrajectoryLength"])
+    b = capi.Batch(n)
+    b.upload(t["init_pos"][:n], O.normalise(t["init_dir"][:n]), t["energy"][:n], t["charge"][:n])
+    prop.propagate(b, max_steps=40)
+    st = O.OracleState(t["init_pos"][:n], t["init_dir"][:n], t["energy"][:n], t["charge"][:n])
+    O.propagate(orc, O.PROP_CK, st, 40, tolerance=tm["tolerance"], min_step=tm["minStep"], max_step=tm["maxStep"],
+                max_len=tm["maxTrajectoryLength"], libm=O.LIBM_CR)
+    out = b.download()
+    for key in ("pos", "dir", "next_step", "trajectory_length"):
+        assert bits_equal(out[key], getattr(st, key)), key
