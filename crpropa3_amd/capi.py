@@ -112,6 +112,8 @@ def lib():
         L.crp_propagator_create_sde.argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(_vp)]
         L.crp_propagator_set_sde.argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.c_uint32]
         L.crp_propagator_set_sde_advection.argtypes = [_vp, _dp]
+        L.crp_propagator_add_sde_advection.argtypes = [_vp, C.c_int, _dp, C.c_int]
+        L.crp_propagator_clear_sde_advection.argtypes = [_vp]
         L.crp_propagator_set_sde_constant_spherical_advection.argtypes = [_vp, _dp, C.c_double]
         L.crp_propagator_set_sde_spherical_advection.argtypes = [_vp, _dp, C.c_double, C.c_double, C.c_double,
                                                                  C.c_double]
@@ -431,8 +433,14 @@ class Propagator:
     def set_sde_advection(self, v):
         """UniformAdvectionField(v) of the DiffusionSDE; None removes it; a dict {"kind": 2, "origin", "par":
         [vWind]} is a ConstantSphericalAdvectionField, {"kind": 3, "origin", "par": [radius, vMax, tau, alpha]}
-        a SphericalAdvectionField."""
-        if isinstance(v, dict):
+        a SphericalAdvectionField; {"terms": [(kind, [parameters]), ...]} any list of the
+        crp_sde_advection_kind fields, summed in order (AdvectionFieldList)."""
+        if isinstance(v, dict) and "terms" in v:
+            check(lib().crp_propagator_clear_sde_advection(self.handle))
+            for kind, par in v["terms"]:
+                p = _f64(par)
+                check(lib().crp_propagator_add_sde_advection(self.handle, int(kind), _p(p), len(p)))
+        elif isinstance(v, dict):
             o = _f64(v["origin"], (3,))
             fn = (lib().crp_propagator_set_sde_constant_spherical_advection if v["kind"] == 2
                   else lib().crp_propagator_set_sde_spherical_advection)

@@ -612,6 +612,15 @@ CRP_MATH_FN double crp_exp_rn(double x) {
 	return crp_exp_accurate(x);
 }
 
+/* tanh from the correctly rounded exp, (e - 1) / (e + 1) with e = exp(2|x|), odd in x, 1 where e
+ * overflows.  Not correctly rounded (near 0 the subtraction loses relative accuracy, down to an absolute
+ * error of about 2^-53), but the same bits on the host and the device. */
+CRP_MATH_FN double crp_tanh_det(double x) {
+	const double e = crp_exp_rn(2 * __builtin_fabs(x));
+	const double t = (e == 1.0 / 0.0) ? 1.0 : (e - 1) / (e + 1);
+	return __builtin_copysign(t, x);
+}
+
 /* a / b in double-double: three quotient digits with exact remainders (about 2^-104) */
 CRP_MATH_FN crp_dd crp_dd_div(crp_dd a, crp_dd b) {
 	double q1 = a.hi / b.hi;

@@ -108,8 +108,11 @@ struct crp_propagator {
 	// Params::postObs of the module order
 	int postObs;
 	// DiffusionSDE (type 2)
-	double sdeScale, sdeAlpha, sdeEpsilon, sdeAdv[3], sdeAdvO[3], sdeAdvPar[4];
-	int sdeAdvection;
+	double sdeScale, sdeAlpha, sdeEpsilon;
+	SdeAdvTerm sdeAdv[kSdeAdvMax];  // the advection field's terms (crp_propagator_add_sde_advection)
+	int sdeAdvection;               // their number
+	mutable SdeAdvTerm *dSdeAdv;    // device copy, made or refreshed by crp_propagate
+	mutable int sdeAdvDirty;        // the terms changed since the last copy
 	uint32_t sdeSeed;
 };
 
@@ -1674,11 +1677,10 @@ static int create_prop(int type, const char *who, const crp_field_t *field, doub
 	p->sdeScale = 1;
 	p->sdeAlpha = 1. / 3.;
 	p->sdeEpsilon = 0.1;
-	for (int a = 0; a < 3; a++)
-		p->sdeAdv[a] = p->sdeAdvO[a] = 0;
-	for (int a = 0; a < 4; a++)
-		p->sdeAdvPar[a] = 0;
+	std::memset(p->sdeAdv, 0, sizeof(p->sdeAdv));
 	p->sdeAdvection = 0;
+	p->dSdeAdv = nullptr;
+	p->sdeAdvDirty = 0;
 	p->sdeSeed = 0;
 	*out = p;
 	return CRP_OK;
@@ -1729,37 +1731,82 @@ int crp_propagator_set_sde(crp_propagator_t *p, double alpha, double scale, doub
 	return CRP_OK;
 }
 
+// the number of parameters of each crp_sde_advection_kind (index = kind)
+static const int kAdvParams[8] = {0, 3, 4, 7, 3, 5, 4, 8};
+
+// Copies changed terms to the device before a propagation reads them (the setters stay host-only).
+// Rare (the terms change between runs at most), so one lock for all propagators serves.
+static std::mutex g_adv_mutex;
+
+static int upload_sde_advection(const crp_propagator *p) {
+	std::lock_guard<std::mutex> lock(g_adv_mutex);
+	if (!p->sdeAdvDirty || p->sdeAdvection == 0)
+		return CRP_OK;
+	if (!p->dSdeAdv)
+		CRP_HIP_TRY(hipMalloc((void **)&p->dSdeAdv, sizeof(p->sdeAdv)));
+	CRP_HIP_TRY(hipDeviceSynchronize()); // no kernel may read the terms while they change
+	CRP_HIP_TRY(hipMemcpy(p->dSdeAdv, p->sdeAdv, sizeof(p->sdeAdv), hipMemcpyHostToDevice));
+	p->sdeAdvDirty = 0;
+	return CRP_OK;
+}
+
+int crp_propagator_clear_sde_advection(crp_propagator_t *p) {
+	if (!p || p->type != 2)
+		return fail(CRP_E_INVALID, "crp_propagator_clear_sde_advection: not a DiffusionSDE propagator");
+	std::memset(p->sdeAdv, 0, sizeof(p->sdeAdv));
+	p->sdeAdvection = 0;
+	p->sdeAdvDirty = 1;
+	return CRP_OK;
+}
+
+int crp_propagator_add_sde_advection(crp_propagator_t *p, int kind, const double *par, int npar) {
+	if (!p || p->type != 2)
+		return fail(CRP_E_INVALID, "crp_propagator_add_sde_advection: not a DiffusionSDE propagator");
+	if (kind < CRP_ADV_UNIFORM || kind > CRP_ADV_SPHERICAL_SHOCK)
+		return fail(CRP_E_INVALID, "crp_propagator_add_sde_advection: unknown advection field kind");
+	if (npar != kAdvParams[kind] || !par)
+		return fail(CRP_E_INVALID, "crp_propagator_add_sde_advection: kind " + std::to_string(kind) + " takes " +
+		                               std::to_string(kAdvParams[kind]) + " parameters");
+	if (p->sdeAdvection >= kSdeAdvMax)
+		return fail(CRP_E_INVALID, "crp_propagator_add_sde_advection: more than " + std::to_string(kSdeAdvMax) +
+		                               " advection fields");
+	SdeAdvTerm &T = p->sdeAdv[p->sdeAdvection];
+	std::memset(&T, 0, sizeof(T));
+	T.kind = kind;
+	for (int i = 0; i < npar; i++)
+		T.par[i] = par[i];
+	p->sdeAdvection++;
+	p->sdeAdvDirty = 1;
+	return CRP_OK;
+}
+
+static_assert((int)CRP_SDE_ADVECTION_MAX == kSdeAdvMax, "advection term count");
+
 int crp_propagator_set_sde_advection(crp_propagator_t *p, const double v[3]) {
 	if (!p || p->type != 2)
 		return fail(CRP_E_INVALID, "crp_propagator_set_sde_advection: not a DiffusionSDE propagator");
-	p->sdeAdvection = v ? 1 : 0;
-	for (int a = 0; a < 3; a++)
-		p->sdeAdv[a] = v ? v[a] : 0.0;
-	return CRP_OK;
+	if (int rc = crp_propagator_clear_sde_advection(p))
+		return rc;
+	return v ? crp_propagator_add_sde_advection(p, CRP_ADV_UNIFORM, v, 3) : CRP_OK;
 }
 
 int crp_propagator_set_sde_constant_spherical_advection(crp_propagator_t *p, const double origin[3], double v_wind) {
 	if (!p || p->type != 2 || !origin)
 		return fail(CRP_E_INVALID, "crp_propagator_set_sde_constant_spherical_advection: bad arguments");
-	p->sdeAdvection = 2;
-	for (int a = 0; a < 3; a++)
-		p->sdeAdvO[a] = origin[a];
-	p->sdeAdvPar[0] = v_wind;
-	return CRP_OK;
+	if (int rc = crp_propagator_clear_sde_advection(p))
+		return rc;
+	const double par[4] = {origin[0], origin[1], origin[2], v_wind};
+	return crp_propagator_add_sde_advection(p, CRP_ADV_CONSTANT_SPHERICAL, par, 4);
 }
 
 int crp_propagator_set_sde_spherical_advection(crp_propagator_t *p, const double origin[3], double radius,
                                                double v_max, double tau, double alpha) {
 	if (!p || p->type != 2 || !origin)
 		return fail(CRP_E_INVALID, "crp_propagator_set_sde_spherical_advection: bad arguments");
-	p->sdeAdvection = 3;
-	for (int a = 0; a < 3; a++)
-		p->sdeAdvO[a] = origin[a];
-	p->sdeAdvPar[0] = radius;
-	p->sdeAdvPar[1] = v_max;
-	p->sdeAdvPar[2] = tau;
-	p->sdeAdvPar[3] = alpha;
-	return CRP_OK;
+	if (int rc = crp_propagator_clear_sde_advection(p))
+		return rc;
+	const double par[7] = {origin[0], origin[1], origin[2], radius, v_max, tau, alpha};
+	return crp_propagator_add_sde_advection(p, CRP_ADV_SPHERICAL, par, 7);
 }
 
 int crp_propagator_set_max_trajectory_length(crp_propagator_t *p, double max_length) {
@@ -1853,9 +1900,12 @@ int crp_propagator_set_module_order(crp_propagator_t *p, const int *modules, int
 }
 
 int crp_propagator_destroy(crp_propagator_t *p) {
-	if (p && p->obsT) {
+	if (p && (p->obsT || p->dSdeAdv)) {
 		(void)hipDeviceSynchronize();
-		(void)hipFree(p->obsT);
+		if (p->obsT)
+			(void)hipFree(p->obsT);
+		if (p->dSdeAdv)
+			(void)hipFree(p->dSdeAdv);
 	}
 	delete p;
 	return CRP_OK;
@@ -2132,6 +2182,9 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 		return fail(CRP_E_INVALID, "crp_propagate: NULL argument");
 	if (max_steps < 0 && !(p->maxLength > 0 && p->minStep > 0))
 		return fail(CRP_E_UNBOUNDED, "crp_propagate: max_steps < 0 needs a maximum trajectory length and minStep > 0");
+	if (p->type == 2)
+		if (int rc = upload_sde_advection(p))
+			return rc;
 	crp_run_stats st;
 	std::memset(&st, 0, sizeof(st));
 	if (b->n == 0 || max_steps == 0) {
@@ -2166,12 +2219,7 @@ int crp_propagate(const crp_propagator_t *p, crp_batch_t *b, long long max_steps
 		P.sdeAlpha = p->sdeAlpha;
 		P.sdeEpsilon = p->sdeEpsilon;
 		P.sdeAdvection = p->sdeAdvection;
-		for (int a = 0; a < 3; a++) {
-			P.sdeAdv[a] = p->sdeAdv[a];
-			P.sdeAdvO[a] = p->sdeAdvO[a];
-		}
-		for (int a = 0; a < 4; a++)
-			P.sdeAdvPar[a] = p->sdeAdvPar[a];
+		P.sdeAdvTerms = p->dSdeAdv;
 		P.sdeSeed = p->sdeSeed;
 		// CRP_SDE_MAX_HALVINGS (1..20) lowers the sub-step limit, so that tests can reach it on small inputs
 		P.sdeMaxHalvings = kSdeMaxHalvings;

@@ -44,6 +44,14 @@ struct Soa {
 
 constexpr int kSnapDoubles = 9; // position x,y,z, direction x,y,z, trajectory length, time, nextStep
 
+// One advection field of DiffusionSDE (src/advectionField/AdvectionField.cpp); `kind` is a
+// crp_sde_advection_kind, `par` its parameters in the order the C ABI documents.
+constexpr int kSdeAdvMax = 8;
+struct SdeAdvTerm {
+	int kind, pad;
+	double par[9];
+};
+
 struct Params {
 	double tol, minStep, maxStep, maxLen;
 	long long maxSteps; // < 0: unlimited
@@ -61,10 +69,10 @@ struct Params {
 	// limit); the observer sees, and records, nextStep limited by the modules before it only
 	int postObs;
 	// DiffusionSDE (k_sde): diffusion coefficient scale * 6.1e24 * (|E/q| / 4e9)^alpha, perpendicular
-	// fraction epsilon, uniform advection velocity, Philox seed
-	// sdeAdvection: 0 none, 1 uniform (sdeAdv), 2 constant spherical wind (origin sdeAdvO, vWind
-	// sdeAdvPar[0]), 3 SphericalAdvectionField (origin, radius, vMax, tau, alpha = sdeAdvPar[0..3])
-	double sdeScale, sdeAlpha, sdeEpsilon, sdeAdv[3], sdeAdvO[3], sdeAdvPar[4];
+	// fraction epsilon, Philox seed; the advection field as sdeAdvection terms (device memory) whose
+	// velocities are summed in order (0: no advection field)
+	double sdeScale, sdeAlpha, sdeEpsilon;
+	const SdeAdvTerm *sdeAdvTerms;
 	int sdeAdvection;
 	uint32_t sdeSeed;
 	int sdeMaxHalvings; // trials beyond this many halvings stop the particle (CTR_SDE_LIMIT)

@@ -147,23 +147,92 @@ __device__ __forceinline__ V3 sde_randcone(crp_sde_rng &r, const V3 &mean, doubl
 	          mean.x * rz.x + mean.y * rz.y + mean.z * rz.z);
 }
 
-// driftStep (src/module/DiffusionSDE.cpp:212-216): lin = 0 + v(pos) h with the advection field's
-// velocity at the step's start (src/advectionField/AdvectionField.cpp:37-130)
-__device__ __forceinline__ void sde_drift(const Params &P, const V3 &pos, double h, double lin[3]) {
-	double v[3] = {P.sdeAdv[0], P.sdeAdv[1], P.sdeAdv[2]};
-	if (P.sdeAdvection >= 2) {
-		const V3 d = v3(pos.x - P.sdeAdvO[0], pos.y - P.sdeAdvO[1], pos.z - P.sdeAdvO[2]);
-		const double R = norm(d);
-		double vr = P.sdeAdvPar[0];
-		const bool outside = P.sdeAdvection == 3 && R > P.sdeAdvPar[0];
-		if (P.sdeAdvection == 3 && !outside)
-			vr = P.sdeAdvPar[1] * (1 - crp_exp_rn(-(crp_pow_cr(R, P.sdeAdvPar[3]) / P.sdeAdvPar[2])));
-		v[0] = outside ? 0.0 : (d.x / R) * vr;
-		v[1] = outside ? 0.0 : (d.y / R) * vr;
-		v[2] = outside ? 0.0 : (d.z / R) * vr;
+// The advection velocity at pos: the terms' getField (src/advectionField/AdvectionField.cpp) summed
+// in order from 0, as AdvectionFieldList::getField does (:12-17); for one term the sum is the field
+// itself (0 + v differs from v only in the sign of a zero, which the drift 0 + v h removes).  The
+// terms are read as scalar loads (every lane reads the same term).  Out of line: the transcendental
+// functions of the shock profiles stay out of the transport loop's registers.
+typedef __attribute__((address_space(4))) const SdeAdvTerm SdeAdvTermConst;
+
+__device__ __forceinline__ V3 sde_advection_velocity(const SdeAdvTerm *terms, int n, const V3 pos) {
+	SdeAdvTermConst *T = (SdeAdvTermConst *)terms;
+	double vx = 0, vy = 0, vz = 0;
+	for (int t = 0; t < n; t++) {
+		const int kind = T[t].kind;
+		double w0 = 0, w1 = 0, w2 = 0;
+		if (kind == CRP_ADV_UNIFORM) { // UniformAdvectionField (:33-35)
+			w0 = T[t].par[0];
+			w1 = T[t].par[1];
+			w2 = T[t].par[2];
+		} else if (kind == CRP_ADV_CONSTANT_SPHERICAL || kind == CRP_ADV_SPHERICAL) {
+			// ConstantSphericalAdvectionField (:54-57): unit vector of pos - origin times vWind;
+			// SphericalAdvectionField (:101-109, 120-123): vMax (1 - exp(-R^alpha / tau)), 0 beyond the radius
+			const V3 d = v3(pos.x - T[t].par[0], pos.y - T[t].par[1], pos.z - T[t].par[2]);
+			const double R = norm(d);
+			double vr = T[t].par[3];
+			const bool outside = kind == CRP_ADV_SPHERICAL && R > T[t].par[3];
+			if (kind == CRP_ADV_SPHERICAL && !outside)
+				vr = T[t].par[4] * (1 - crp_exp_rn(-(crp_pow_cr(R, T[t].par[6]) / T[t].par[5])));
+			w0 = outside ? 0.0 : (d.x / R) * vr;
+			w1 = outside ? 0.0 : (d.y / R) * vr;
+			w2 = outside ? 0.0 : (d.z / R) * vr;
+		} else if (kind == CRP_ADV_CARTESIAN_SHOCK || kind == CRP_ADV_OBLIQUE_SHOCK) {
+			// OneDimensionalCartesianShock (:188-199), ObliqueAdvectionShock (:360-372):
+			// v_x = a - b tanh(x / lShock), a, b = (vUp +- vUp / compressionRatio) / 2; v_y = vY (oblique)
+			const bool oblique = kind == CRP_ADV_OBLIQUE_SHOCK;
+			const double vUp = T[t].par[1];
+			const double vDown = vUp / T[t].par[0];
+			const double a = (vUp + vDown) * 0.5;
+			const double b = (vUp - vDown) * 0.5;
+			w0 = a - b * crp_tanh_det(pos.x / T[t].par[oblique ? 3 : 2]);
+			w1 = oblique ? T[t].par[2] : 0.0;
+		} else if (kind == CRP_ADV_SPHERICAL_SHOCK_1D) {
+			// OneDimensionalSphericalShock (:254-275): radial, a - b tanh((r - rShock) / lShock), times
+			// (rShock / r)^2 outside the shock or without upstream cooling
+			const double r = norm(pos);
+			const double rS = T[t].par[0], vUp = T[t].par[1];
+			const double vDown = vUp / T[t].par[2];
+			const double a = (vUp + vDown) * 0.5;
+			const double b = (vUp - vDown) * 0.5;
+			const double th = a - b * crp_tanh_det((r - rS) / T[t].par[3]);
+			const double v = (T[t].par[4] != 0 && r <= rS) ? th : (th * (rS / r)) * (rS / r);
+			w0 = (pos.x / r) * v;
+			w1 = (pos.y / r) * v;
+			w2 = (pos.z / r) * v;
+		} else if (kind == CRP_ADV_SPHERICAL_SHOCK) {
+			// SphericalAdvectionShock (:442-452, 463-466): v_r e_r + v_phi (r_rot / r) e_phi with
+			// v_r = v0 (1 + ((r0 / 2r)^2 - 1) g(r)), g = 1 / (1 + exp(-(r - r0) / lambda)); pow(x, 2.) is x * x
+			// e_phi = (-sin phi, cos phi, 0) with phi = atan2(y, x) (Vector3::getUnitVectorPhi) is taken as
+			// (-y, x, 0) / sqrt(x^2 + y^2): within an ulp or two of the reference's, without atan2, sin and cos
+			const V3 R = v3(pos.x - T[t].par[0], pos.y - T[t].par[1], pos.z - T[t].par[2]);
+			const double r = norm(R);
+			const double eps = 2.2250738585072014e-308; // Vector3::getPhi: numeric_limits<double>::min()
+			const bool axis = __builtin_fabs(R.x) < eps && __builtin_fabs(R.y) < eps;
+			const double rho = __builtin_sqrt(R.x * R.x + R.y * R.y);
+			const double ex = axis ? -0.0 : -(R.y / rho), ey = axis ? 1.0 : R.x / rho;
+			const double r0 = T[t].par[3], v0 = T[t].par[4], lambda = T[t].par[5];
+			const double g = 1. / (1 + crp_exp_rn(-((r - r0) / lambda)));
+			const double q = r0 / (2 * r);
+			const double vr = v0 * (1 + (q * q - 1) * g);
+			const double vp = T[t].par[7] * (T[t].par[6] / r);
+			w0 = (R.x / r) * vr + ex * vp;
+			w1 = (R.y / r) * vr + ey * vp;
+			w2 = (R.z / r) * vr + 0.0 * vp;
+		}
+		vx += w0;
+		vy += w1;
+		vz += w2;
 	}
-	for (int a = 0; a < 3; a++)
-		lin[a] = 0 + v[a] * h;
+	return v3(vx, vy, vz);
+}
+
+// driftStep (src/module/DiffusionSDE.cpp:212-216): lin = 0 + v(pos) h with the advection field's
+// velocity at the step's start
+__device__ __forceinline__ void sde_drift(const Params &P, const V3 &pos, double h, double lin[3]) {
+	const V3 v = sde_advection_velocity(P.sdeAdvTerms, P.sdeAdvection, pos);
+	lin[0] = 0 + v.x * h;
+	lin[1] = 0 + v.y * h;
+	lin[2] = 0 + v.z * h;
 }
 
 enum SdePhase { SDE_BEGIN = 0, SDE_TRIAL = 1, SDE_SUB = 2 };

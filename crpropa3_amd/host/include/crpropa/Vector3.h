@@ -6,6 +6,7 @@
 #define CRPROPA_AMD_VECTOR3_H
 
 #include <cmath>
+#include <limits>
 #include <iostream>
 
 namespace crpropa {
@@ -32,6 +33,29 @@ public:
 	T getR() const { return std::sqrt(x * x + y * y + z * z); }
 	T getR2() const { return x * x + y * y + z * z; }
 	Vector3 getUnitVector() const { return *this / getR(); }
+	/* azimuth and zenith angles and their unit vectors (include/crpropa/Vector3.h:136-169 of the
+	 * reference); the reference's build calls sin and cos of getUnitVectorPhi separately (its
+	 * SphericalAdvectionShock samples show it), and the pointers keep them separate here */
+	T getPhi() const {
+		const T eps = std::numeric_limits<T>::min();
+		if ((std::fabs(x) < eps) && (std::fabs(y) < eps))
+			return 0.0;
+		return std::atan2(y, x);
+	}
+	T getTheta() const {
+		const T eps = std::numeric_limits<T>::min();
+		if ((std::fabs(x) < eps) && (std::fabs(y) < eps) && (std::fabs(z) < eps))
+			return 0.0;
+		return std::atan2((T)std::sqrt(x * x + y * y), z);
+	}
+	Vector3 getUnitVectorTheta() const {
+		const T theta = getTheta(), phi = getPhi();
+		return Vector3(std::cos(theta) * std::cos(phi), std::cos(theta) * std::sin(phi), -std::sin(theta));
+	}
+	Vector3 getUnitVectorPhi() const {
+		T (*volatile sin_)(T) = std::sin, (*volatile cos_)(T) = std::cos;
+		return Vector3(-sin_(getPhi()), cos_(getPhi()), 0);
+	}
 	T dot(const Vector3 &v) const { return x * v.x + y * v.y + z * v.z; }
 	Vector3 cross(const Vector3 &v) const { return Vector3(y * v.z - v.y * z, z * v.x - v.z * x, x * v.y - v.x * y); }
 	T getDistanceTo(const Vector3 &p) const { return (*this - p).getR(); }

@@ -73,6 +73,10 @@ void bind_vector3(py::module_ &m, const char *name) {
 	    .def("getR", &V::getR)
 	    .def("getR2", &V::getR2)
 	    .def("getUnitVector", &V::getUnitVector)
+	    .def("getPhi", &V::getPhi)
+	    .def("getTheta", &V::getTheta)
+	    .def("getUnitVectorPhi", &V::getUnitVectorPhi)
+	    .def("getUnitVectorTheta", &V::getUnitVectorTheta)
 	    .def("getDistanceTo", &V::getDistanceTo)
 	    .def("getAngleTo", &V::getAngleTo)
 	    .def("dot", &V::dot)
@@ -518,6 +522,62 @@ PYBIND11_MODULE(_crpropa, m) {
 	    .def("getTau", &SphericalAdvectionField::getTau)
 	    .def("getAlpha", &SphericalAdvectionField::getAlpha)
 	    .def("getDescription", &SphericalAdvectionField::getDescription);
+	py::class_<AdvectionFieldList, AdvectionField, ref_ptr<AdvectionFieldList>>(m, "AdvectionFieldList")
+	    .def(py::init<>())
+	    .def("addField", &AdvectionFieldList::addField);
+	py::class_<OneDimensionalCartesianShock, AdvectionField, ref_ptr<OneDimensionalCartesianShock>>(
+	    m, "OneDimensionalCartesianShock")
+	    .def(py::init<double, double, double>(), py::arg("compressionRatio"), py::arg("vUp"), py::arg("lShock"))
+	    .def("setComp", &OneDimensionalCartesianShock::setComp)
+	    .def("setVup", &OneDimensionalCartesianShock::setVup)
+	    .def("setShockwidth", &OneDimensionalCartesianShock::setShockwidth)
+	    .def("getComp", &OneDimensionalCartesianShock::getComp)
+	    .def("getVup", &OneDimensionalCartesianShock::getVup)
+	    .def("getShockwidth", &OneDimensionalCartesianShock::getShockwidth)
+	    .def("getDescription", &OneDimensionalCartesianShock::getDescription);
+	py::class_<OneDimensionalSphericalShock, AdvectionField, ref_ptr<OneDimensionalSphericalShock>>(
+	    m, "OneDimensionalSphericalShock")
+	    .def(py::init<double, double, double, double, bool>(), py::arg("rShock"), py::arg("vUp"),
+	         py::arg("compressionRatio"), py::arg("lShock"), py::arg("coolUpstream"))
+	    .def("setComp", &OneDimensionalSphericalShock::setComp)
+	    .def("setVup", &OneDimensionalSphericalShock::setVup)
+	    .def("setShockwidth", &OneDimensionalSphericalShock::setShockwidth)
+	    .def("setShockRadius", &OneDimensionalSphericalShock::setShockRadius)
+	    .def("setCooling", &OneDimensionalSphericalShock::setCooling)
+	    .def("getComp", &OneDimensionalSphericalShock::getComp)
+	    .def("getVup", &OneDimensionalSphericalShock::getVup)
+	    .def("getShockwidth", &OneDimensionalSphericalShock::getShockwidth)
+	    .def("getShockRadius", &OneDimensionalSphericalShock::getShockRadius)
+	    .def("getCooling", &OneDimensionalSphericalShock::getCooling)
+	    .def("getDescription", &OneDimensionalSphericalShock::getDescription);
+	py::class_<ObliqueAdvectionShock, AdvectionField, ref_ptr<ObliqueAdvectionShock>>(m, "ObliqueAdvectionShock")
+	    .def(py::init<double, double, double, double>(), py::arg("compressionRatio"), py::arg("vXUp"), py::arg("vY"),
+	         py::arg("lShock"))
+	    .def("setComp", &ObliqueAdvectionShock::setComp)
+	    .def("setVup", &ObliqueAdvectionShock::setVup)
+	    .def("setVy", &ObliqueAdvectionShock::setVy)
+	    .def("setShockwidth", &ObliqueAdvectionShock::setShockwidth)
+	    .def("getComp", &ObliqueAdvectionShock::getComp)
+	    .def("getVup", &ObliqueAdvectionShock::getVup)
+	    .def("getVy", &ObliqueAdvectionShock::getVy)
+	    .def("getShockwidth", &ObliqueAdvectionShock::getShockwidth)
+	    .def("getDescription", &ObliqueAdvectionShock::getDescription);
+	py::class_<SphericalAdvectionShock, AdvectionField, ref_ptr<SphericalAdvectionShock>>(m, "SphericalAdvectionShock")
+	    .def(py::init<const Vector3d, double, double, double>(), py::arg("origin"), py::arg("r_0"), py::arg("v_0"),
+	         py::arg("l"))
+	    .def("setOrigin", &SphericalAdvectionShock::setOrigin)
+	    .def("setR0", &SphericalAdvectionShock::setR0)
+	    .def("setV0", &SphericalAdvectionShock::setV0)
+	    .def("setLambda", &SphericalAdvectionShock::setLambda)
+	    .def("setRRot", &SphericalAdvectionShock::setRRot)
+	    .def("setAzimuthalSpeed", &SphericalAdvectionShock::setAzimuthalSpeed)
+	    .def("getOrigin", &SphericalAdvectionShock::getOrigin)
+	    .def("getR0", &SphericalAdvectionShock::getR0)
+	    .def("getV0", &SphericalAdvectionShock::getV0)
+	    .def("getLambda", &SphericalAdvectionShock::getLambda)
+	    .def("getRRot", &SphericalAdvectionShock::getRRot)
+	    .def("getAzimuthalSpeed", &SphericalAdvectionShock::getAzimuthalSpeed)
+	    .def("getDescription", &SphericalAdvectionShock::getDescription);
 
 	py::class_<DiffusionSDE, DevicePropagator, ref_ptr<DiffusionSDE>>(m, "DiffusionSDE")
 	    .def(py::init<ref_ptr<MagneticField>, double, double, double, double>(), py::arg("magneticField"),

@@ -410,20 +410,12 @@ Vector3d DiffusionSDE::getAdvectionFieldAtPosition(Vector3d pos, double t) const
 
 void DiffusionSDE::configureDevice(crp_propagator *p) const {
 	throwIf(crp_propagator_set_sde(p, alpha, scale, epsilon, seed));
+	throwIf(crp_propagator_clear_sde_advection(p));
 	if (advectionField.valid()) {
-		Vector3d a;
-		double par[4] = {0, 0, 0, 0};
-		const int kind = advectionField->deviceForm(a, par);
-		const double aa[3] = {a.x, a.y, a.z};
-		if (kind == 1)
-			throwIf(crp_propagator_set_sde_advection(p, aa));
-		else if (kind == 2)
-			throwIf(crp_propagator_set_sde_constant_spherical_advection(p, aa, par[0]));
-		else if (kind == 3)
-			throwIf(crp_propagator_set_sde_spherical_advection(p, aa, par[0], par[1], par[2], par[3]));
-		else
-			throw std::runtime_error("DiffusionSDE: the advection field has no GPU implementation (UniformAdvectionField, "
-			                         "ConstantSphericalAdvectionField and SphericalAdvectionField run on the device)");
+		std::vector<DeviceAdvectionTerm> terms;
+		advectionField->deviceTerms(terms);
+		for (const DeviceAdvectionTerm &t : terms)
+			throwIf(crp_propagator_add_sde_advection(p, t.kind, t.par.data(), (int)t.par.size()));
 	}
 }
 

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define CRP_ABI_VERSION 11
+#define CRP_ABI_VERSION 12
 
 enum crp_status {
 	CRP_OK = 0,
@@ -236,6 +236,34 @@ int crp_propagator_set_sde_constant_spherical_advection(crp_propagator_t *p, con
  * x - origin inside the radius, 0 outside (AdvectionField.cpp:90-120) */
 int crp_propagator_set_sde_spherical_advection(crp_propagator_t *p, const double origin[3], double radius,
                                                double v_max, double tau, double alpha);
+/* Any advection field of src/advectionField/AdvectionField.cpp as a list of terms whose velocities are
+ * summed in order from 0 (AdvectionFieldList::getField, :12-17; a single term is the field itself).
+ * crp_propagator_add_sde_advection appends one term (at most CRP_SDE_ADVECTION_MAX); the setters above
+ * replace the list with one term; crp_propagator_clear_sde_advection empties it.  Parameters per kind:
+ *   UNIFORM              v[3]                                         (:29-39)
+ *   CONSTANT_SPHERICAL   origin[3], vWind                             (:49-62)
+ *   SPHERICAL            origin[3], radius, vMax, tau, alpha          (:93-123)
+ *   CARTESIAN_SHOCK      compressionRatio, vUp, lShock  (OneDimensionalCartesianShock, :182-199)
+ *   SPHERICAL_SHOCK_1D   rShock, vUp, compressionRatio, lShock, coolUpstream (0/1)
+ *                        (OneDimensionalSphericalShock, :246-275)
+ *   OBLIQUE_SHOCK        compressionRatio, vXUp, vY, lShock  (ObliqueAdvectionShock, :353-372)
+ *   SPHERICAL_SHOCK      origin[3], r0, v0, lambda, rRot, vPhi  (SphericalAdvectionShock with setRRot /
+ *                        setAzimuthalSpeed, :432-466)
+ * Deterministic forms where the reference calls glibc: tanh from the correctly rounded exp (not correctly
+ * rounded itself), SphericalAdvectionShock's e_phi = (-sin phi, cos phi, 0), phi = atan2(y, x), as
+ * (-y, x, 0) / sqrt(x^2 + y^2); exp and pow are correctly rounded. */
+enum crp_sde_advection_kind {
+	CRP_ADV_UNIFORM = 1,
+	CRP_ADV_CONSTANT_SPHERICAL = 2,
+	CRP_ADV_SPHERICAL = 3,
+	CRP_ADV_CARTESIAN_SHOCK = 4,
+	CRP_ADV_SPHERICAL_SHOCK_1D = 5,
+	CRP_ADV_OBLIQUE_SHOCK = 6,
+	CRP_ADV_SPHERICAL_SHOCK = 7
+};
+#define CRP_SDE_ADVECTION_MAX 8
+int crp_propagator_add_sde_advection(crp_propagator_t *p, int kind, const double *par, int npar);
+int crp_propagator_clear_sde_advection(crp_propagator_t *p);
 
 /* MaximumTrajectoryLength(maxLength) fused after each step -- src/module/BreakCondition.cpp:42-64
  * (no observer positions).  max_length <= 0 removes it. */

@@ -1568,6 +1568,99 @@ static void sde_trystep(const orc_field *f, const double in[3], double out[3], d
 	}
 }
 
+/* tanh: glibc's for the reference's stream, the deterministic one of crp_crmath.h for the device's */
+static double orc_tanh(double x, int libm) { return libm == ORC_LIBM_CR ? crp_tanh_det(x) : tanh(x); }
+
+/* One advection field's getField (src/advectionField/AdvectionField.cpp), par in the C ABI's order */
+static void orc_adv_term(int kind, const double *par, const double pos[3], int libm, double w[3]) {
+	w[0] = w[1] = w[2] = 0;
+	if (kind == CRP_ADV_UNIFORM) { /* :33-35 */
+		w[0] = par[0];
+		w[1] = par[1];
+		w[2] = par[2];
+	} else if (kind == CRP_ADV_CONSTANT_SPHERICAL || kind == CRP_ADV_SPHERICAL) { /* :54-57, :101-109 */
+		const double P[3] = {pos[0] - par[0], pos[1] - par[1], pos[2] - par[2]};
+		const double R = sde_r(P);
+		if (kind == CRP_ADV_SPHERICAL && R > par[3])
+			return;
+		double vr = par[3];
+		if (kind == CRP_ADV_SPHERICAL) /* getV (:120-123) */
+			vr = par[4] * (1 - orc_exp(-(orc_pow(R, par[6], libm) / par[5]), libm));
+		for (int a = 0; a < 3; a++)
+			w[a] = (P[a] / R) * vr;
+	} else if (kind == CRP_ADV_CARTESIAN_SHOCK || kind == CRP_ADV_OBLIQUE_SHOCK) { /* :188-199, :360-372 */
+		const int oblique = kind == CRP_ADV_OBLIQUE_SHOCK;
+		const double x = pos[0];
+		const double vUp = par[1];
+		const double vDown = vUp / par[0];
+		const double a = (vUp + vDown) * 0.5;
+		const double b = (vUp - vDown) * 0.5;
+		w[0] = a - b * orc_tanh(x / par[oblique ? 3 : 2], libm);
+		if (oblique)
+			w[1] = par[2];
+	} else if (kind == CRP_ADV_SPHERICAL_SHOCK_1D) { /* :254-275 */
+		const double r = sde_r(pos);
+		const double rShock = par[0], vUp = par[1], lShock = par[3];
+		const int coolUpstream = par[4] != 0;
+		const double vDown = vUp / par[2];
+		const double a = (vUp + vDown) * 0.5;
+		const double b = (vUp - vDown) * 0.5;
+		double v;
+		if (coolUpstream) {
+			if (r <= rShock)
+				v = a - b * orc_tanh((r - rShock) / lShock, libm);
+			else
+				v = (a - b * orc_tanh((r - rShock) / lShock, libm)) * (rShock / r) * (rShock / r);
+		} else
+			v = (a - b * orc_tanh((r - rShock) / lShock, libm)) * (rShock / r) * (rShock / r);
+		for (int k = 0; k < 3; k++)
+			w[k] = (pos[k] / r) * v;
+	} else if (kind == CRP_ADV_SPHERICAL_SHOCK) { /* :442-452 with g (:463-466) */
+		const double R[3] = {pos[0] - par[0], pos[1] - par[1], pos[2] - par[2]};
+		const double r = sde_r(R);
+		const double eps = 2.2250738585072014e-308; /* Vector3::getPhi */
+		const double phi = (fabs(R[0]) < eps && fabs(R[1]) < eps) ? 0.0 : orc_atan2(R[1], R[0], libm);
+		/* getUnitVectorPhi (include/crpropa/Vector3.h:167-169): separate sin and cos calls in the
+		 * reference's build (its samples show it); called through volatile pointers so that this
+		 * compiler does not merge them into sincos either */
+		double sphi, cphi;
+		if (libm == ORC_LIBM_CR) { /* the device's e_phi: (-y, x) / sqrt(x^2 + y^2), no atan2/sin/cos */
+			const int axis = fabs(R[0]) < eps && fabs(R[1]) < eps;
+			const double rho = sqrt(R[0] * R[0] + R[1] * R[1]);
+			sphi = axis ? 0.0 : R[1] / rho;
+			cphi = axis ? 1.0 : R[0] / rho;
+		} else {
+			double (*volatile sinf_)(double) = sin, (*volatile cosf_)(double) = cos;
+			sphi = sinf_(phi);
+			cphi = cosf_(phi);
+		}
+		const double r_0 = par[3], v_0 = par[4], lambda = par[5], r_rot = par[6], v_phi = par[7];
+		const double ga = (r - r_0) / lambda;
+		const double g = 1. / (1 + orc_exp(-ga, libm));
+		const double q = r_0 / (2 * r); /* pow(q, 2.): GCC writes q * q */
+		const double v_r = v_0 * (1 + (q * q - 1) * g);
+		const double v_p = v_phi * (r_rot / r);
+		const double e_phi[3] = {-sphi, cphi, 0};
+		for (int k = 0; k < 3; k++)
+			w[k] = (R[k] / r) * v_r + e_phi[k] * v_p;
+	}
+}
+
+/* the advection velocity of p's term list at n positions (AdvectionFieldList::getField) */
+void orc_adv_eval(const orc_sde *p, const double *pos, size_t n, double *v, int libm) {
+	for (size_t i = 0; i < n; i++) {
+		double s[3] = {0, 0, 0};
+		for (int t = 0; t < p->adv_n && t < 8; t++) {
+			double w[3];
+			orc_adv_term(p->adv_list_kind[t], p->adv_list_par[t], pos + 3 * i, libm, w);
+			for (int a = 0; a < 3; a++)
+				s[a] += w[a];
+		}
+		for (int a = 0; a < 3; a++)
+			v[3 * i + a] = s[a];
+	}
+}
+
 /* DiffusionSDE::process (src/module/DiffusionSDE.cpp:48-209) */
 static void orc_sde_process(const orc_field *f, const orc_sde *p, orc_state *s, size_t i, orc_sde_rng *r) {
 	const int k = r->kind, libm = k == ORC_SDE_MT ? ORC_LIBM_GLIBC : ORC_LIBM_CR;
@@ -1583,7 +1676,15 @@ static void orc_sde_process(const orc_field *f, const orc_sde *p, orc_state *s, 
 	if (p->advection) {
 		/* driftStep (DiffusionSDE.cpp:212-216): LinProp(0) += advField(pos) * h */
 		double v[3] = {p->adv[0], p->adv[1], p->adv[2]};
-		if (p->adv_kind == 2 || p->adv_kind == 3) {
+		if (p->adv_n > 0) { /* AdvectionFieldList::getField: b(0.); b += field_i(pos) */
+			v[0] = v[1] = v[2] = 0;
+			for (int t = 0; t < p->adv_n && t < 8; t++) {
+				double w[3];
+				orc_adv_term(p->adv_list_kind[t], p->adv_list_par[t], pos, libm, w);
+				for (int a = 0; a < 3; a++)
+					v[a] += w[a];
+			}
+		} else if (p->adv_kind == 2 || p->adv_kind == 3) {
 			const double P[3] = {pos[0] - p->adv_origin[0], pos[1] - p->adv_origin[1], pos[2] - p->adv_origin[2]};
 			const double R = sde_r(P);
 			double vr = p->adv_par[0]; /* ConstantSphericalAdvectionField: vWind */

@@ -148,7 +148,15 @@ typedef struct {
 	 * src/advectionField/AdvectionField.cpp:37-130 */
 	int adv_kind;
 	double adv_origin[3], adv_par[4];
+	/* the advection field as a list of terms (crp_sde_advection_kind, parameters in the C ABI's order),
+	 * their velocities summed in order from 0 (AdvectionFieldList::getField,
+	 * src/advectionField/AdvectionField.cpp:12-17); adv_n = 0: the single field above */
+	int adv_n;
+	int adv_list_kind[8];
+	double adv_list_par[8][9];
 } orc_sde;
+/* the advection velocity of p's term list (adv_n terms, summed from 0) at n positions */
+void orc_adv_eval(const orc_sde *p, const double *pos, size_t n, double *v, int libm);
 long long orc_sde_propagate(const orc_field *f, const orc_sde *p, orc_state *s, size_t n, long long maxSteps,
                             uint32_t *sde_step, const uint64_t *sde_stream);
 /* ModuleList::run with OpenMP: one MT19937 per thread (seed + thread number), glibc libm (CPU baseline) */

@@ -720,6 +720,93 @@ static void gen_sde() {
 	}
 }
 
+// Advection fields of src/advectionField/AdvectionField.cpp beyond the uniform and spherical winds: field
+// samples of each class and DiffusionSDE runs in each (the parameters as crp_propagator_add_sde_advection
+// orders them, in "advection_terms": [kind, [parameters]] per summed term).
+static std::string adv_terms_json(const std::vector<std::pair<int, std::vector<double>>> &terms) {
+	std::string s = "[";
+	for (size_t t = 0; t < terms.size(); t++) {
+		s += (t ? ", [" : "[") + std::to_string(terms[t].first) + ", [";
+		for (size_t i = 0; i < terms[t].second.size(); i++)
+			s += (i ? ", " : "") + fmt(terms[t].second[i]);
+		s += "]]";
+	}
+	return s + "]";
+}
+
+static void gen_advection() {
+	typedef std::vector<std::pair<int, std::vector<double>>> Terms;
+	const Vector3d o(1 * kpc, -0.5 * kpc, 0.2 * kpc);
+	ref_ptr<AdvectionField> cart = new OneDimensionalCartesianShock(4., 1e6, 0.5 * kpc);
+	ref_ptr<AdvectionField> sph1c = new OneDimensionalSphericalShock(3 * kpc, 1e6, 4., 0.5 * kpc, true);
+	ref_ptr<AdvectionField> sph1 = new OneDimensionalSphericalShock(3 * kpc, 1e6, 4., 0.5 * kpc, false);
+	ref_ptr<AdvectionField> obl = new ObliqueAdvectionShock(3., 8e5, 2e5, 0.3 * kpc);
+	ref_ptr<SphericalAdvectionShock> sas = new SphericalAdvectionShock(o, 2 * kpc, 6e5, 0.3 * kpc);
+	sas->setRRot(3 * kpc);
+	sas->setAzimuthalSpeed(2e5);
+	ref_ptr<AdvectionFieldList> lst = new AdvectionFieldList();
+	lst->addField(new UniformAdvectionField(Vector3d(1e5, -2e5, 3e4)));
+	lst->addField(sph1);
+	lst->addField(new ConstantSphericalAdvectionField(o, 3e5));
+	const Terms tCart = {{4, {4., 1e6, 0.5 * kpc}}};
+	const Terms tSph1c = {{5, {3 * kpc, 1e6, 4., 0.5 * kpc, 1}}};
+	const Terms tSph1 = {{5, {3 * kpc, 1e6, 4., 0.5 * kpc, 0}}};
+	const Terms tObl = {{6, {3., 8e5, 2e5, 0.3 * kpc}}};
+	const Terms tSas = {{7, {o.x, o.y, o.z, 2 * kpc, 6e5, 0.3 * kpc, 3 * kpc, 2e5}}};
+	const Terms tLst = {{1, {1e5, -2e5, 3e4}}, tSph1[0], {2, {o.x, o.y, o.z, 3e5}}};
+	const char *names[6] = {"cartesian_shock", "spherical_shock_1d_cool", "spherical_shock_1d", "oblique_shock",
+	                        "spherical_advection_shock", "list"};
+	const AdvectionField *fields[6] = {cart.get(), sph1c.get(), sph1.get(), obl.get(), sas.get(), lst.get()};
+	const Terms *terms[6] = {&tCart, &tSph1c, &tSph1, &tObl, &tSas, &tLst};
+	{
+		Random rng(811);
+		const size_t M = 2048;
+		std::vector<double> pos(M * 3);
+		for (size_t i = 0; i < M; i++) {
+			Vector3d p;
+			switch (i % 4) {
+			case 0: p = rng.randVector() * rng.randUniform(0, 6 * kpc); break;           // around 0
+			case 1: p = o + rng.randVector() * rng.randUniform(1.5 * kpc, 2.5 * kpc); break; // near r0
+			case 2: p = rng.randVector() * rng.randUniform(2.5 * kpc, 3.5 * kpc); break;     // near rShock
+			default: p = Vector3d(rng.randUniform(-1, 1) * kpc, rng.randUniform(-5, 5) * kpc, 0); break;
+			}
+			pos[3 * i] = p.x; pos[3 * i + 1] = p.y; pos[3 * i + 2] = p.z;
+		}
+		std::string dir = casedir("advection_fields");
+		save_f64(dir, "pos", pos, {M, 3});
+		std::string meta = "{";
+		for (int k = 0; k < 6; k++) {
+			std::vector<double> v(M * 3);
+			for (size_t i = 0; i < M; i++) {
+				const Vector3d w = fields[k]->getField(Vector3d(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]));
+				v[3 * i] = w.x; v[3 * i + 1] = w.y; v[3 * i + 2] = w.z;
+			}
+			save_f64(dir, std::string("v_") + names[k], v, {M, 3});
+			meta += std::string(k ? ", " : "") + "\"" + names[k] + "\": " + adv_terms_json(*terms[k]);
+		}
+		save_meta(dir, meta + "}");
+		printf("  advection_fields M=%zu\n", M);
+	}
+	std::vector<int> species = {nucleusId(60, 1), nucleusId(60, 2), nucleusId(60, 26), nucleusId(60, 0)};
+	std::vector<double> wts = {0.5, 0.2, 0.2, 0.1};
+	ref_ptr<MagneticField> f = new UniformMagneticField(Vector3d(0.3 * nG, 0, 1 * nG));
+	const Vector3d starts[6] = {Vector3d(0.2 * kpc, 0, 0), Vector3d(2.8 * kpc, 0.5 * kpc, 0),
+	                            Vector3d(2.8 * kpc, 0.5 * kpc, 0), Vector3d(0.1 * kpc, 0, 0),
+	                            o + Vector3d(1.9 * kpc, 0, 0), Vector3d(2.8 * kpc, 0.5 * kpc, 0)};
+	const ref_ptr<AdvectionField> refs[6] = {cart, sph1c, sph1, obl, sas, lst};
+	for (int k = 0; k < 6; k++) {
+		DiffusionSDE sde(f, refs[k], 1e-4, 1 * pc, 10 * kpc, 0.2);
+		std::vector<Particle> ps = make_particles(32, 90 + k, 1 * TeV, 1 * PeV, starts[k], species, wts);
+		const std::string name = std::string("sde_adv_") + names[k];
+		run_sde_case(name, sde, 20 * kpc, ps, 300, 4720 + k,
+		             "{\"field\": {\"type\": \"uniform\", \"B\": [" + fmt(0.3 * nG) + ", 0, " + fmt(nG) +
+		                 "]}, \"advection_terms\": " + adv_terms_json(*terms[k]) + ", \"tolerance\": 1e-4, \"minStep\": " +
+		                 fmt(pc) + ", \"maxStep\": " + fmt(10 * kpc) + ", \"epsilon\": 0.2, \"alpha\": " + fmt(1. / 3.) +
+		                 ", \"scale\": 1, \"maxTrajectoryLength\": " + fmt(20 * kpc) + ", \"seed\": " +
+		                 std::to_string(4720 + k) + "}");
+	}
+}
+
 static void gen_planewave_case(const std::string &name, const TurbulenceSpectrum &spec, int Nm, int seed,
                                double posScale, const std::string &meta) {
 	PlaneWaveTurbulence pw(spec, Nm, seed);
@@ -1096,6 +1183,10 @@ int main(int argc, char **argv) {
 		gen_decorators();
 		return 0;
 	}
+	if (argc > 2 && std::string(argv[2]) == "advection") { // only the advection-field cases
+		gen_advection();
+		return 0;
+	}
 	if (argc > 2 && std::string(argv[2]) == "interp") { // only the tricubic / nearest-neighbour cases
 #ifdef HAVE_SIMD
 		gen_interp();
@@ -1123,6 +1214,7 @@ int main(int argc, char **argv) {
 	gen_trilinear();
 	gen_modulated();
 	gen_sde();
+	gen_advection();
 	gen_jf12();
 	gen_gridtools();
 	gen_decorators();

@@ -84,7 +84,8 @@ class SDE(C.Structure):
     _fields_ = [("tolerance", C.c_double), ("minStep", C.c_double), ("maxStep", C.c_double), ("epsilon", C.c_double),
                 ("alpha", C.c_double), ("scale", C.c_double), ("advection", C.c_int), ("adv", C.c_double * 3),
                 ("maxTrajectoryLength", C.c_double), ("rng", C.c_int), ("seed", C.c_uint32), ("adv_kind", C.c_int),
-                ("adv_origin", C.c_double * 3), ("adv_par", C.c_double * 4)]
+                ("adv_origin", C.c_double * 3), ("adv_par", C.c_double * 4), ("adv_n", C.c_int),
+                ("adv_list_kind", C.c_int * 8), ("adv_list_par", (C.c_double * 9) * 8)]
 
 
 SDE_MT, SDE_PHILOX = 0, 1
@@ -122,6 +123,7 @@ def lib():
         L.orc_propagate.argtypes = [_P(Field), _P(Propagator), _P(State), C.c_size_t, C.c_longlong, C.c_int,
                                     C.c_int, _P(C.c_longlong)]
         L.orc_field_eval.argtypes = [_P(Field), _dp, C.c_size_t, _dp, C.c_int]
+        L.orc_adv_eval.argtypes = [_P(SDE), _dp, C.c_size_t, _dp, C.c_int]
         L.orc_periodic_boundary.argtypes = [C.c_int, C.c_int]
         L.orc_reflective_boundary.argtypes = [C.c_int, C.c_int]
         L.orc_field_eval_z.argtypes = [_P(Field), _dp, _dp, C.c_size_t, _dp, C.c_int]
@@ -694,11 +696,34 @@ def sde_propagate(field, state, steps, tolerance, min_step, max_step, epsilon=0.
     return lib().orc_sde_propagate(C.byref(field.s), C.byref(p), C.byref(s), state.n, steps, cnt, stream)
 
 
+# crp_sde_advection_kind
+ADV_UNIFORM, ADV_CONSTANT_SPHERICAL, ADV_SPHERICAL, ADV_CARTESIAN_SHOCK, ADV_SPHERICAL_SHOCK_1D, ADV_OBLIQUE_SHOCK, \
+    ADV_SPHERICAL_SHOCK = 1, 2, 3, 4, 5, 6, 7
+
+
+def advection_eval(terms, pos, libm=LIBM_GLIBC):
+    """the velocity of an advection-field term list [(kind, [parameters]), ...] at positions (n, 3)"""
+    p = SDE()
+    set_advection(p, {"terms": terms})
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    v = np.zeros_like(pos)
+    lib().orc_adv_eval(C.byref(p), ptr(pos), pos.shape[0], ptr(v), libm)
+    return v
+
+
 def set_advection(p, advection):
-    """advection: None, a uniform velocity (3 numbers), or a dict {"kind": 2|3, "origin": [..], "par": [..]}"""
+    """advection: None, a uniform velocity (3 numbers), a dict {"kind": 2|3, "origin": [..], "par": [..]},
+    or {"terms": [(kind, [parameters in the C ABI's order]), ...]} summed as AdvectionFieldList does"""
     if advection is None:
         return
     p.advection = 1
+    if isinstance(advection, dict) and "terms" in advection:
+        p.adv_n = len(advection["terms"])
+        for t, (kind, par) in enumerate(advection["terms"]):
+            p.adv_list_kind[t] = kind
+            for i, v in enumerate(par):
+                p.adv_list_par[t][i] = v
+        return
     if isinstance(advection, dict):
         p.adv_kind = advection["kind"]
         for a in range(3):
@@ -728,6 +753,8 @@ def sde_args(meta):
     if "advection_kind" in meta:
         kw["advection"] = {"kind": meta["advection_kind"], "origin": meta["advection_origin"],
                            "par": meta["advection_par"]}
+    if "advection_terms" in meta:
+        kw["advection"] = {"terms": [(k, par) for k, par in meta["advection_terms"]]}
     return kw
 
 

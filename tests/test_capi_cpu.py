@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert capi.lib().crp_abi_version() == 11
+    assert capi.lib().crp_abi_version() == 12
 
 
 @pytest.mark.parametrize("kind", ["BP", "CK"])

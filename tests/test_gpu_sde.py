@@ -55,8 +55,12 @@ def compare(out, st, steps):
     assert np.array_equal(out["sde_step"], steps)
 
 
+ADVECTION_CASES = ["sde_adv_cartesian_shock", "sde_adv_spherical_shock_1d_cool", "sde_adv_spherical_shock_1d",
+                   "sde_adv_oblique_shock", "sde_adv_spherical_advection_shock", "sde_adv_list"]
+
+
 @pytest.mark.parametrize("name", ["sde_uniform", "sde_grid32", "sde_advection", "sde_advection_cspherical",
-                                  "sde_advection_spherical"])
+                                  "sde_advection_spherical"] + ADVECTION_CASES)
 @pytest.mark.parametrize("split", [None, (3, 5, -1), "launch"])
 def test_sde_bit_exact_vs_oracle(name, split):
     c = O.load_case(name)

@@ -290,3 +290,36 @@ def test_single_step_api_without_device_field_is_rejected():
     with pytest.raises(RuntimeError, match="no GPU implementation"):
         bp.dY(crp.Vector3d(0.0), crp.Vector3d(1, 0, 0), 1 * crp.kpc, 0.0, crp.eplus, crp.EeV)
     assert bp.errorEstimation(crp.Vector3d(3, 0, 0), crp.Vector3d(0, 4, 0), 1.0) == 5 / 0.75
+
+
+def test_advection_field_classes_match_reference_samples():
+    """The reference's advection field classes on the host (src/advectionField/AdvectionField.cpp):
+    getField equals the reference's samples bit for bit; a DiffusionSDE accepts every one of them."""
+    from crpropa3_amd import (AdvectionFieldList, ConstantSphericalAdvectionField, DiffusionSDE, ObliqueAdvectionShock,
+                              OneDimensionalCartesianShock, OneDimensionalSphericalShock, SphericalAdvectionShock,
+                              UniformAdvectionField)
+    c = O.load_case("advection_fields")
+    m = c["meta"]
+
+    (comp, vUp, L), = [par for _, par in m["cartesian_shock"]]
+    (rS, vU1, comp1, L1, cool), = [par for _, par in m["spherical_shock_1d_cool"]]
+    (compO, vX, vY, LO), = [par for _, par in m["oblique_shock"]]
+    (ox, oy, oz, r0, v0, lam, rrot, vphi), = [par for _, par in m["spherical_advection_shock"]]
+    sas = SphericalAdvectionShock(Vector3d(ox, oy, oz), r0, v0, lam)
+    sas.setRRot(rrot)
+    sas.setAzimuthalSpeed(vphi)
+    lst = AdvectionFieldList()
+    (u,), sph, (cw,) = [par for _, par in m["list"][:1]], m["list"][1][1], [m["list"][2][1]]
+    lst.addField(UniformAdvectionField(Vector3d(*u)))
+    lst.addField(OneDimensionalSphericalShock(sph[0], sph[1], sph[2], sph[3], bool(sph[4])))
+    lst.addField(ConstantSphericalAdvectionField(Vector3d(*cw[:3]), cw[3]))
+    fields = {"cartesian_shock": OneDimensionalCartesianShock(comp, vUp, L),
+              "spherical_shock_1d_cool": OneDimensionalSphericalShock(rS, vU1, comp1, L1, True),
+              "spherical_shock_1d": OneDimensionalSphericalShock(rS, vU1, comp1, L1, False),
+              "oblique_shock": ObliqueAdvectionShock(compO, vX, vY, LO),
+              "spherical_advection_shock": sas, "list": lst}
+    for name, f in fields.items():
+        v = np.array([[w.x, w.y, w.z] for w in (f.getField(Vector3d(*p)) for p in c["pos"][:256])])
+        assert np.array_equal(v.view(np.uint64), c["v_" + name][:256].view(np.uint64)), name
+        DiffusionSDE(UniformMagneticField(Vector3d(0, 0, 1 * nG)), f)
+    assert fields["oblique_shock"].getVy() == vY and sas.getRRot() == rrot and fields["spherical_shock_1d_cool"].getCooling()

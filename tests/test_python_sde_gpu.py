@@ -82,7 +82,25 @@ ADVECTIONS = {
                    lambda: crp.ConstantSphericalAdvectionField(crp.Vector3d(0.5 * O.kpc, 0, 0), 3e5)),
     "spherical": ({"kind": 3, "origin": [0.5 * O.kpc, 0, 0], "par": [8 * O.kpc, 4e5, 2 * O.kpc, 1.5]},
                   lambda: crp.SphericalAdvectionField(crp.Vector3d(0.5 * O.kpc, 0, 0), 8 * O.kpc, 4e5, 2 * O.kpc, 1.5)),
+    "shocks": ({"terms": [(O.ADV_CARTESIAN_SHOCK, [4., 1e6, 0.3 * O.kpc]),
+                          (O.ADV_SPHERICAL_SHOCK_1D, [1 * O.kpc, 8e5, 3., 0.4 * O.kpc, 1]),
+                          (O.ADV_OBLIQUE_SHOCK, [3., 5e5, 2e5, 0.2 * O.kpc]),
+                          (O.ADV_SPHERICAL_SHOCK, [0.5 * O.kpc, 0, 0, 1 * O.kpc, 6e5, 0.3 * O.kpc, 2 * O.kpc, 2e5])]},
+               lambda: _shock_list()),
 }
+
+
+def _shock_list():
+    """AdvectionFieldList of the four shock fields (src/advectionField/AdvectionField.cpp:182-532)"""
+    lst = crp.AdvectionFieldList()
+    lst.addField(crp.OneDimensionalCartesianShock(4., 1e6, 0.3 * crp.kpc))
+    lst.addField(crp.OneDimensionalSphericalShock(1 * crp.kpc, 8e5, 3., 0.4 * crp.kpc, True))
+    lst.addField(crp.ObliqueAdvectionShock(3., 5e5, 2e5, 0.2 * crp.kpc))
+    sas = crp.SphericalAdvectionShock(crp.Vector3d(0.5 * crp.kpc, 0, 0), 1 * crp.kpc, 6e5, 0.3 * crp.kpc)
+    sas.setRRot(2 * crp.kpc)
+    sas.setAzimuthalSpeed(2e5)
+    lst.addField(sas)
+    return lst
 
 
 def sim_for(seed, adv="uniform"):
