@@ -809,16 +809,62 @@ struct YM {
 	float3 u;
 };
 
+// The field at a Cash-Karp trial's start.  A rejected trial is retried from the same state with a
+// smaller step (src/module/PropagationCK.cpp:99-126), so the retry's first stage reads the same field
+// value again; the adaptive kernels keep it for the retries instead of reading the grid (same bits).
+// CkB0Store<true> serves fields whose FP64 evaluation is an FP32 value widened (the trilinear Grid3f
+// readers): it keeps the value in three floats, three registers fewer; CkB0f is the mixed path's.
+template <class F> struct FloatValuedField {
+	static constexpr bool value = false;
+};
+template <> struct FloatValuedField<FieldGridCells> {
+	static constexpr bool value = true;
+};
+template <> struct FloatValuedField<FieldGrid> {
+	static constexpr bool value = true;
+};
+template <bool F32> struct CkB0Store;
+template <> struct CkB0Store<false> {
+	V3 B;
+	bool have;
+	__device__ __forceinline__ V3 get() const { return B; }
+	__device__ __forceinline__ void put(const V3 &b) { B = b; have = true; }
+};
+template <> struct CkB0Store<true> {
+	float3 B;
+	bool have;
+	__device__ __forceinline__ V3 get() const { return v3((double)B.x, (double)B.y, (double)B.z); }
+	__device__ __forceinline__ void put(const V3 &b) {
+		B = make_float3((float)b.x, (float)b.y, (float)b.z);
+		have = true;
+	}
+};
+struct CkB0f {
+	float3 B;
+	bool have;
+};
+
 template <class F>
-__device__ __forceinline__ void ck_dydt_mixed(const F &field, const YM &y, float Q, float3 &n, float3 &du) {
+__device__ __forceinline__ void ck_dydt_mixed(const F &field, const YM &y, float Q, float3 &n, float3 &du,
+                                              CkB0f *b0 = nullptr) {
 	n = unit3(y.u);                 // velocity / c
-	const float3 B = field.evalf(y.x);
+	float3 B;
+	if (b0 && b0->have) {
+		B = b0->B;
+	} else {
+		B = field.evalf(y.x);
+		if (b0) {
+			b0->B = B;
+			b0->have = true;
+		}
+	}
 	const float3 c = cross3(n, B);
 	du = f3(c.x * Q, c.y * Q, c.z * Q);
 }
 
 template <class F>
-__device__ __forceinline__ void ck_trystep_mixed(const F &field, const YM &y, double h, float Q, YM &out, double &errU) {
+__device__ __forceinline__ void ck_trystep_mixed(const F &field, const YM &y, double h, float Q, YM &out, double &errU,
+                                                 CkB0f *b0 = nullptr) {
 	constexpr double cA[6][5] = {{0, 0, 0, 0, 0},
 	                             {CK::a10, 0, 0, 0, 0},
 	                             {CK::a20, CK::a21, 0, 0, 0},
@@ -844,7 +890,7 @@ __device__ __forceinline__ void ck_trystep_mixed(const F &field, const YM &y, do
 			yn.u = f3(__builtin_fmaf(ku[j].x, au, yn.u.x), __builtin_fmaf(ku[j].y, au, yn.u.y),
 			          __builtin_fmaf(ku[j].z, au, yn.u.z));
 		}
-		ck_dydt_mixed(field, yn, Q, kn[i], ku[i]);
+		ck_dydt_mixed(field, yn, Q, kn[i], ku[i], i == 0 ? b0 : nullptr);
 		if (cB[i] != 0.0) {
 			const double bx = cB[i] * ch;
 			const float bu = (float)cB[i] * hf;
@@ -892,12 +938,20 @@ __device__ __forceinline__ Y bp_dy(const F &field, const V3 &pos0, const V3 &dir
 // ---------------------------------------------------------------- Cash-Karp
 // PropagationCK::dYdt (src/module/PropagationCK.cpp:51-61); qcE = (q*c)/E
 template <class F>
-__device__ __forceinline__ Y ck_dydt(const F &field, const Y &y, double qcE) {
+__device__ __forceinline__ Y ck_dydt(const F &field, const Y &y, double qcE,
+                                     CkB0Store<FloatValuedField<F>::value> *b0 = nullptr) {
 	Y d;
 	double n = norm(y.u);
 	const double invn = 1 / n;
 	d.x = v3(div_rn(y.u.x, n, invn) * kCLight, div_rn(y.u.y, n, invn) * kCLight, div_rn(y.u.z, n, invn) * kCLight);
-	V3 B = field.eval(y.x);
+	V3 B;
+	if (b0 && b0->have) {
+		B = b0->get();
+	} else {
+		B = field.eval(y.x);
+		if (b0)
+			b0->put(B);
+	}
 	d.u = v3((d.x.y * B.z - B.y * d.x.z) * qcE, (d.x.z * B.x - B.z * d.x.x) * qcE,
 	         (d.x.x * B.y - B.x * d.x.y) * qcE);
 	return d;
@@ -916,14 +970,15 @@ __device__ __forceinline__ void axpy(Y &yn, const Y &k, double a, double h) {
 // PropagationCK::tryStep (src/module/PropagationCK.cpp:28-49).  Returns the
 // new phase point in `out` and the error estimate in `err`.
 template <class F>
-__device__ __forceinline__ void ck_trystep_err(const F &field, const Y &y, double h, double qcE, Y &out, Y &err) {
+__device__ __forceinline__ void ck_trystep_err(const F &field, const Y &y, double h, double qcE, Y &out, Y &err,
+                                               CkB0Store<FloatValuedField<F>::value> *b0 = nullptr) {
 	err.x = v3(0.0, 0.0, 0.0);
 	err.u = v3(0.0, 0.0, 0.0);
 	out = y;
 	const double e0 = CK::b0 - CK::s0, e1 = CK::b1 - CK::s1, e2 = CK::b2 - CK::s2, e3 = CK::b3 - CK::s3,
 	             e4 = CK::b4 - CK::s4, e5 = CK::b5 - CK::s5;
 	Y k0, k1, k2, k3, k4, k5, yn;
-	k0 = ck_dydt(field, y, qcE);
+	k0 = ck_dydt(field, y, qcE, b0);
 	axpy(out, k0, CK::b0, h);
 	axpy(err, k0, e0, h);
 	yn = y;
@@ -965,9 +1020,10 @@ __device__ __forceinline__ void ck_trystep_err(const F &field, const Y &y, doubl
 
 // the same with only |err.u|, the quantity the step control reads
 template <class F>
-__device__ __forceinline__ void ck_trystep(const F &field, const Y &y, double h, double qcE, Y &out, double &errU) {
+__device__ __forceinline__ void ck_trystep(const F &field, const Y &y, double h, double qcE, Y &out, double &errU,
+                                           CkB0Store<FloatValuedField<F>::value> *b0 = nullptr) {
 	Y err;
-	ck_trystep_err(field, y, h, qcE, out, err);
+	ck_trystep_err(field, y, h, qcE, out, err, b0);
 	errU = norm(err.u);
 }
 

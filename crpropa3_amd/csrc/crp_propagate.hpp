@@ -121,6 +121,10 @@ k_propagate(F field0, Params P, Soa S, unsigned long long n,
 	F field = field0; // the lane's copy: a field that reads the candidate's redshift binds it at refill
 
 	bool have = false, exhausted = false, inProc = false;
+	CkB0Store<FloatValuedField<F>::value> b0; // the field at the trial sequence's start (adaptive CK), for retries
+	CkB0f b0f;
+	b0.have = false;
+	b0f.have = false;
 	unsigned idx = 0;
 	long long steps = 0;
 	V3 pos = v3(0, 0, 0), dir = v3(0, 0, 0);
@@ -245,6 +249,8 @@ k_propagate(F field0, Params P, Soa S, unsigned long long n,
 				step = dclip(next, P.minStep, P.maxStep);
 				newStep = step;
 				inProc = true;
+				b0.have = false;
+				b0f.have = false;
 			}
 		}
 		if (kAdaptive && inProc) {
@@ -253,7 +259,8 @@ k_propagate(F field0, Params P, Soa S, unsigned long long n,
 				if (kCK) {
 					double eU;
 					YM o;
-					ck_trystep_mixed(field, YM{pos, dirf}, step / kCLight, (float)((q * kCLight * kCLight) / E), o, eU);
+					ck_trystep_mixed(field, YM{pos, dirf}, step / kCLight, (float)((q * kCLight * kCLight) / E), o, eU,
+					                 &b0f);
 					oPos = o.x;
 					oDir = o.u;
 					r = eU / P.tol;
@@ -273,7 +280,7 @@ k_propagate(F field0, Params P, Soa S, unsigned long long n,
 			} else {
 				if (kCK) {
 					double eU;
-					ck_trystep(field, Y{pos, dir}, step / kCLight, (q * kCLight) / E, yOut, eU);
+					ck_trystep(field, Y{pos, dir}, step / kCLight, (q * kCLight) / E, yOut, eU, &b0);
 					r = eU / P.tol;
 				} else {
 					const double m_ = E / (kCLight * kCLight);
@@ -286,7 +293,7 @@ k_propagate(F field0, Params P, Soa S, unsigned long long n,
 					r = __builtin_sqrt((Sv * Sv + Sv * Sv) + Sv * Sv) / P.tol;
 				}
 			}
-			nEval += kEvalsPerTrial;
+			nEval += kEvalsPerTrial; // the reference's getField calls, the reused first stage included
 			if (r > 1) {
 				if (step == P.minStep) {
 					commit = true;

@@ -46,7 +46,9 @@ typedef struct crp_batch crp_batch_t;
 typedef struct {
 	unsigned long long accepted_steps; /* process() calls that moved a particle */
 	unsigned long long rejected_trials; /* adaptive trials thrown away (step shrunk) */
-	unsigned long long field_evals;     /* getField evaluations */
+	unsigned long long field_evals;     /* getField evaluations of the reference's algorithm (a rejected
+	                                       Cash-Karp trial's retry reads its first-stage value from a register,
+	                                       not from the grid, and is counted as the reference evaluates it) */
 	unsigned long long finished;        /* particles made inactive by the length limit */
 	double kernel_ms;                   /* device time of the propagation kernels (HIP events) */
 	int launches;                       /* kernel launches */
