@@ -233,10 +233,10 @@ def cpu_baseline(w, fdata, pos, d, E, q, gpu_final=None, target_s=15.0):
 
 ADJOINT = dict(n=65536, grid_n=128, spacing=20 * kpc, lmin_cells=2, lmax_cells=128, seed=5, step=10 * kpc,
                steps=4096, K=64, emin=1 * EeV, emax=100 * EeV)
-# per step of the backward: recompute (the cell's 128-byte corner line, 120 B of stored u, T and dT/df,
-# the 16-byte first half of the deposit record), reverse (those 136 B read back, the 32-byte record
-# written into its tile's range), accumulation (the record read)
-ADJ_BYTES_PER_STEP = (128 + 120 + 16) + (136 + 32) + 32
+# per step of the backward: recompute (the cell's 128-byte corner line, 48 B of sub-checkpoint per 8
+# steps written), reverse (the cell line again, the sub-checkpoint read, the 32-byte record written into
+# its tile's range), accumulation (the record read)
+ADJ_BYTES_PER_STEP = (128 + 6) + (128 + 6 + 32) + 32
 
 
 def adjoint_inputs(a, n):

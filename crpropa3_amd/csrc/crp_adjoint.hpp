@@ -283,22 +283,25 @@ __global__ void __launch_bounds__(256) k_adjoint_backward(AdjArgs A) {
 // The checkpointed backward above walks every particle's segments one after the other: each
 // segment's recompute and reverse sweep wait on one gather per step at one wave per SIMD, and
 // every reverse step adds 24 FP64 atomics into the grid gradient.  The blocked backward instead
-//   1. recomputes a block of segments of every particle at once (segments x n independent lanes)
-//      and stores, for every step, what its reverse needs: the direction u before the step, the
-//      interpolated field T and its derivatives with respect to the cell fractions (15 doubles),
-//      wave by wave, and the first half of the step's deposit record (the 8^3-cell tile, the cell in
-//      it and the fractions); each workgroup also counts its records per tile;
+//   1. recomputes a block of segments of every particle at once (segments x n independent lanes),
+//      stores the state (x, u) before every kAdjSub-th step of the block (sub-checkpoints, 6 doubles
+//      per kAdjSub steps) and counts, per workgroup, the steps whose half-step position falls in each
+//      tile of 2^S cells per axis;
 //   2. scans the counts into every tile's start and every particle group's place within each tile;
 //   3. runs the reverse steps of the block per particle (one lane per particle, the only sequential
-//      part: no gathers, the stored values of the next two steps loaded ahead) and writes each
-//      step's complete 32-byte record (with the adjoint aT of T: the gradient of grid value G_c is
-//      w_c aT) into its tile's range;
+//      part): for each run of kAdjSub steps, from the last to the first, it recomputes the run from
+//      its sub-checkpoint, keeping in registers what the reverse steps read (the direction u before
+//      the step, the interpolated field T and its derivatives with respect to the cell fractions,
+//      the cell and the fractions), then applies the reverse steps and writes each step's 32-byte
+//      record (with the adjoint aT of T: the gradient of grid value G_c is w_c aT) into its tile's range;
 //   4. accumulates each tile's records into its 9^3 grid points in LDS (FP64), one workgroup per
 //      kDepPerGroup records of a tile, and adds the points into the gradient.
-// The host runs the reverse sweeps on the caller's stream and the recompute of the next block and the
-// accumulation of the previous one on a second stream beside it.  The arithmetic of a step is that of
-// adj_step / adj_step_reverse, except that the position gradient is taken as aT . dT/df; the deposits
-// are summed in another order.
+// Per step the backward thus reads the cell line twice (recompute, reverse) and moves one record
+// through memory; the stored values of a step never leave the registers.  The host runs the reverse
+// sweeps on the caller's stream and the recompute of the next block and the accumulation of the
+// previous one on two more streams beside it.  The arithmetic of a step is that of adj_step /
+// adj_step_reverse, except that the position gradient is taken as aT . dT/df; the deposits are
+// summed in another order.
 // deposit tiles of 2^S cells per axis: 8^3 (S = 3, 9^3 points = 17.5 KB of FP64 LDS per accumulating
 // workgroup of 256) or, for grids with too many of those, 16^3 (S = 4, 118 KB, workgroups of 1024)
 template <int S>
@@ -309,10 +312,9 @@ struct Tile {
 constexpr int kAdjGroup = 256;       // particles per recompute / reverse workgroup
 constexpr int kDepPerGroup = 65536;  // records per accumulating workgroup
 constexpr int kDepBatch = 4;
-constexpr int kAdjBurst = 4;         // reverse steps whose records are stored together
-// stored per step and wave: u, T, dT/dfx, dT/dfy, dT/dfz (15 x 64 doubles)
+constexpr int kAdjSub = 8;           // steps between the sub-checkpoints of a block
+// what one reverse step reads: u, T, dT/dfx, dT/dfy, dT/dfz
 constexpr int kAdjStored = 15;
-constexpr int kAdjStepBytes = kAdjStored * 64 * 8;
 
 struct __align__(16) DepRec {
 	uint32_t key; // tile << 3S | cell in tile (x << 2S | y << S | z) for tiles of 2^S cells per axis
@@ -326,9 +328,8 @@ struct AdjBlock {
 	int s0, s1;          // steps [s0, s1)
 	int first, last;     // first block of the backward (reads the output adjoints); last block
 	int npg;             // particle groups: ceil(n / kAdjGroup)
-	char *states;        // [segment - seg0][wave][step in segment] kAdjStepBytes
+	double *sub;         // [q / kAdjSub][6][n]: x, u before step s0 + q, q a multiple of kAdjSub
 	double *carry;       // [6][n]: adjoint (x, u) between blocks
-	uint4 *rec0;         // [m], record (s - s0) * n + i: key, fractions
 	DepRec *srec;        // [m], grouped by tile
 	uint32_t *counts;    // [npg][ntiles]: records per particle group and tile, then their offsets
 	uint32_t *tileStart; // [ntiles + 1]
@@ -337,24 +338,26 @@ struct AdjBlock {
 	size_t m;            // records of the block: (s1 - s0) * n
 };
 
-__device__ __forceinline__ size_t adj_step_offset(const AdjBlock &Bk, int K, int segl, size_t i, int j) {
-	return (((size_t)segl * Bk.npg * (kAdjGroup / 64) + (i >> 6)) * K + j) * kAdjStepBytes;
-}
-
-typedef uint32_t adj_u32x4 __attribute__((ext_vector_type(4)));
-
 __global__ void __launch_bounds__(kAdjGroup) k_dep_zero(AdjBlock Bk) {
 	const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (k < (size_t)Bk.npg * Bk.ntiles)
 		Bk.counts[k] = 0u;
 }
 
-// One lane per (segment, particle); a workgroup holds kAdjGroup particles of one segment.  The
-// gathers of step j + 1 are issued before the stores of step j: a wave's wait for its loads also
-// waits for the stores issued before them, and the stored values are not read again here.
+// the record key of a cell: tile << 3S | cell in tile (x << 2S | y << S | z)
 template <int S>
+__device__ __forceinline__ uint32_t adj_key(const AdjBlock &Bk, const TriPoint &tp) {
+	const int cx = tp.lo[0], cy = tp.lo[1], cz = tp.lo[2];
+	const uint32_t tile = (uint32_t)(((cx >> S) * Bk.nty + (cy >> S)) * Bk.ntz + (cz >> S));
+	const uint32_t m = (1u << S) - 1;
+	return (tile << (3 * S)) | (((uint32_t)cx & m) << (2 * S)) | (((uint32_t)cy & m) << S) | ((uint32_t)cz & m);
+}
+
+// One lane per (segment, particle); a workgroup holds kAdjGroup particles of one segment.  Every step
+// is the forward's (adj_step); the lane stores the sub-checkpoints (every L-th step of the block) that
+// fall in its segment and counts the step's tile.
+template <int S, int L>
 __global__ void __launch_bounds__(kAdjGroup) k_adj_recompute(AdjArgs A, AdjBlock Bk) {
-	using Ti = Tile<S>;
 	extern __shared__ uint32_t hist[];
 	const int T = Bk.ntiles;
 	for (int k = threadIdx.x; k < T; k += blockDim.x)
@@ -369,49 +372,23 @@ __global__ void __launch_bounds__(kAdjGroup) k_adj_recompute(AdjArgs A, AdjBlock
 		V3 x = v3(c[i], c[n + i], c[2 * n + i]);
 		V3 u = v3(c[3 * n + i], c[4 * n + i], c[5 * n + i]);
 		const double kq = A.kq[i];
-		char *sp = Bk.states + adj_step_offset(Bk, A.K, segl, i, 0);
-		const int lane = (int)(i & 63);
-		size_t r = (size_t)(seg * A.K - Bk.s0) * n + i;
-		V3 x1 = vadd(x, vmul(u, A.h / 2));
-		TriPoint tp;
-		adj_cell(A.G, x1, tp);
-		float cv[24];
-		adj_gather(A.G, tp, cv);
-		for (int j = 0; j < len; j++, sp += kAdjStepBytes, r += n) {
-			// T in the order of adj_interp, and its derivatives with respect to the fractions
-			double b[3] = {0, 0, 0}, jx[3] = {0, 0, 0}, jy[3] = {0, 0, 0}, jz[3] = {0, 0, 0};
-#pragma unroll
-			for (int k = 0; k < 8; k++) {
-				double w, dwx, dwy, dwz;
-				adj_weight(tp, k, w, dwx, dwy, dwz);
-				const double g0 = cv[3 * k], g1 = cv[3 * k + 1], g2 = cv[3 * k + 2];
-				b[0] += w * g0; b[1] += w * g1; b[2] += w * g2;
-				jx[0] += dwx * g0; jx[1] += dwx * g1; jx[2] += dwx * g2;
-				jy[0] += dwy * g0; jy[1] += dwy * g1; jy[2] += dwy * g2;
-				jz[0] += dwz * g0; jz[1] += dwz * g1; jz[2] += dwz * g2;
+		int q = seg * A.K - Bk.s0;
+		for (int j = 0; j < len; j++, q++) {
+			if (q % L == 0) {
+				double *o = Bk.sub + (size_t)(q / L) * 6 * n + i;
+				__builtin_nontemporal_store(x.x, o);
+				__builtin_nontemporal_store(x.y, o + n);
+				__builtin_nontemporal_store(x.z, o + 2 * n);
+				__builtin_nontemporal_store(u.x, o + 3 * n);
+				__builtin_nontemporal_store(u.y, o + 4 * n);
+				__builtin_nontemporal_store(u.z, o + 5 * n);
 			}
-			const double sd[kAdjStored] = {u.x, u.y, u.z, b[0], b[1], b[2], jx[0], jx[1], jx[2],
-			                               jy[0], jy[1], jy[2], jz[0], jz[1], jz[2]};
-			const int cx = tp.lo[0], cy = tp.lo[1], cz = tp.lo[2];
-			const uint32_t tile = (uint32_t)(((cx >> Ti::shift) * Bk.nty + (cy >> Ti::shift)) * Bk.ntz + (cz >> Ti::shift));
-			const uint32_t m = Ti::cells - 1;
-			adj_u32x4 rec;
-			rec.x = (tile << Ti::cellBits) | (((uint32_t)cx & m) << (2 * Ti::shift)) | (((uint32_t)cy & m) << Ti::shift) |
-			        ((uint32_t)cz & m);
-			rec.y = __float_as_uint((float)tp.fx);
-			rec.z = __float_as_uint((float)tp.fy);
-			rec.w = __float_as_uint((float)tp.fz);
-			atomicAdd(&hist[tile], 1u);
-			adj_push(A.G, A.h, kq, v3(b[0], b[1], b[2]), x1, x, u);
-			// the next step's cell and corners (after the last step unused, but in bounds: adj_cell wraps)
-			x1 = vadd(x, vmul(u, A.h / 2));
+			const V3 x1 = vadd(x, vmul(u, A.h / 2));
+			TriPoint tp;
 			adj_cell(A.G, x1, tp);
-			adj_gather(A.G, tp, cv);
-			double *pd = reinterpret_cast<double *>(sp) + lane;
-#pragma unroll
-			for (int k = 0; k < kAdjStored; k++)
-				__builtin_nontemporal_store(sd[k], pd + 64 * k);
-			__builtin_nontemporal_store(rec, reinterpret_cast<adj_u32x4 *>(Bk.rec0 + r));
+			const V3 Tv = adj_interp(A.G, tp);
+			atomicAdd(&hist[adj_key<S>(Bk, tp) >> (3 * S)], 1u);
+			adj_push(A.G, A.h, kq, Tv, x1, x, u);
 		}
 	}
 	__syncthreads();
@@ -498,17 +475,31 @@ struct AdjStored {
 	uint32_t rec[4];      // key, fractions (plain words: a vector-type member sends copies through memory)
 };
 
-__device__ __forceinline__ void adj_load_stored(const AdjBlock &Bk, int K, size_t n, size_t i, int q, AdjStored &s) {
-	const char *sp = Bk.states + adj_step_offset(Bk, K, q / K, i, q % K);
-	const double *pd = reinterpret_cast<const double *>(sp) + (i & 63);
+// the stored values of a step from its cell, corner values cv and direction u: T in the order of
+// adj_interp, and its derivatives with respect to the fractions
+template <int S>
+__device__ __forceinline__ void adj_stored(const AdjBlock &Bk, const TriPoint &tp, const float cv[24], const V3 &u,
+                                           AdjStored &d) {
+	double b[3] = {0, 0, 0}, jx[3] = {0, 0, 0}, jy[3] = {0, 0, 0}, jz[3] = {0, 0, 0};
 #pragma unroll
-	for (int k = 0; k < kAdjStored; k++)
-		s.d[k] = __builtin_nontemporal_load(pd + 64 * k);
-	const uint4 r = Bk.rec0[(size_t)q * n + i];
-	s.rec[0] = r.x;
-	s.rec[1] = r.y;
-	s.rec[2] = r.z;
-	s.rec[3] = r.w;
+	for (int k = 0; k < 8; k++) {
+		double w, dwx, dwy, dwz;
+		adj_weight(tp, k, w, dwx, dwy, dwz);
+		const double g0 = cv[3 * k], g1 = cv[3 * k + 1], g2 = cv[3 * k + 2];
+		b[0] += w * g0; b[1] += w * g1; b[2] += w * g2;
+		jx[0] += dwx * g0; jx[1] += dwx * g1; jx[2] += dwx * g2;
+		jy[0] += dwy * g0; jy[1] += dwy * g1; jy[2] += dwy * g2;
+		jz[0] += dwz * g0; jz[1] += dwz * g1; jz[2] += dwz * g2;
+	}
+	d.d[0] = u.x; d.d[1] = u.y; d.d[2] = u.z;
+	d.d[3] = b[0]; d.d[4] = b[1]; d.d[5] = b[2];
+	d.d[6] = jx[0]; d.d[7] = jx[1]; d.d[8] = jx[2];
+	d.d[9] = jy[0]; d.d[10] = jy[1]; d.d[11] = jy[2];
+	d.d[12] = jz[0]; d.d[13] = jz[1]; d.d[14] = jz[2];
+	d.rec[0] = adj_key<S>(Bk, tp);
+	d.rec[1] = __float_as_uint((float)tp.fx);
+	d.rec[2] = __float_as_uint((float)tp.fy);
+	d.rec[3] = __float_as_uint((float)tp.fz);
 }
 
 // adj_step_reverse from the stored u, T and dT/df: (ax, au) the adjoint after the step on entry, before
@@ -551,8 +542,10 @@ __device__ __forceinline__ V3 adj_reverse_stored(const AdjGrid &G, double h, dou
 
 // One lane per particle, kAdjGroup particles per workgroup; each step's record goes to its tile's range
 // at the group's offset (counts, from the scans) plus its rank among the group's records of the tile.
-// one wave per SIMD (n / 64 waves in all): the kernel may use the whole register file
-template <int S>
+// one wave per SIMD (n / 64 waves in all): the kernel may use the whole register file, and holds a run
+// of L steps' stored values in it (the step and reverse-step arithmetic, FP64 at one wave per SIMD,
+// is what bounds it, not the run's gathers)
+template <int S, int L>
 __global__ void __launch_bounds__(kAdjGroup) __attribute__((amdgpu_waves_per_eu(1, 1))) k_adj_reverse(AdjArgs A, AdjBlock Bk) {
 	extern __shared__ uint32_t lds[];
 	const int T = Bk.ntiles;
@@ -566,7 +559,6 @@ __global__ void __launch_bounds__(kAdjGroup) __attribute__((amdgpu_waves_per_eu(
 	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	const bool live = i < A.n;
 	const size_t n = A.n;
-	const int K = A.K;
 	double gs = 0;
 	if (live) {
 		V3 ax, au;
@@ -578,42 +570,44 @@ __global__ void __launch_bounds__(kAdjGroup) __attribute__((amdgpu_waves_per_eu(
 			au = v3(Bk.carry[3 * n + i], Bk.carry[4 * n + i], Bk.carry[5 * n + i]);
 		}
 		const double kq = A.kq[i];
-		// the stored values of step s, s - 1 and s - 2 (loaded two steps ahead)
-		// (near the block's first step the loads ahead repeat step 0: in bounds, and never used)
-		AdjStored d0, d1, d2;
-		int q = Bk.s1 - 1 - Bk.s0;
-		adj_load_stored(Bk, K, n, i, q, d0);
-		adj_load_stored(Bk, K, n, i, max(q - 1, 0), d1);
-		// records are held kAdjBurst steps in registers and then stored together, so that each tile's
-		// run of records is filled within a short time
-		while (q >= 0) {
-			const int cnt = min(kAdjBurst, q + 1);
-			uint4 held[kAdjBurst][2];
-			uint32_t heldDst[kAdjBurst];
+		const int m = Bk.s1 - Bk.s0;
+		for (int sb = (m - 1) / L; sb >= 0; sb--) {
+			// steps [sb L, sb L + len) of the block, recomputed from their sub-checkpoint (the step
+			// guards, uniform branches, also keep every step in a basic block of its own: one block of
+			// L branch-free steps does not fit in the registers)
+			const int len = min(L, m - sb * L);
+			const double *c = Bk.sub + (size_t)sb * 6 * n + i;
+			V3 x = v3(c[0], c[n], c[2 * n]);
+			V3 u = v3(c[3 * n], c[4 * n], c[5 * n]);
+			AdjStored d[L];
+			V3 x1 = vadd(x, vmul(u, A.h / 2));
+			TriPoint tp;
+			adj_cell(A.G, x1, tp);
+			float cv[24];
+			adj_gather(A.G, tp, cv);
 #pragma unroll
-			for (int k = 0; k < kAdjBurst; k++) {
-				if (k < cnt) {
-					const int qq = q - k;
-					adj_load_stored(Bk, K, n, i, max(qq - 2, 0), d2);
-					const V3 aT = adj_reverse_stored(A.G, A.h, kq, d0, ax, au, gs);
-					const uint32_t t = d0.rec[0] >> Tile<S>::cellBits;
-					heldDst[k] = base[t] + atomicAdd(&rank[t], 1u);
-					held[k][0] = make_uint4(d0.rec[0], d0.rec[1], d0.rec[2], d0.rec[3]);
-					held[k][1] = make_uint4(__float_as_uint((float)aT.x), __float_as_uint((float)aT.y),
-					                        __float_as_uint((float)aT.z), 0u);
-					d0 = d1;
-					d1 = d2;
+			for (int k = 0; k < L; k++) {
+				if (k < len) {
+					adj_stored<S>(Bk, tp, cv, u, d[k]);
+					adj_push(A.G, A.h, kq, v3(d[k].d[3], d[k].d[4], d[k].d[5]), x1, x, u);
+					if (k + 1 < len) {
+						x1 = vadd(x, vmul(u, A.h / 2));
+						adj_cell(A.G, x1, tp);
+						adj_gather(A.G, tp, cv);
+					}
 				}
 			}
 #pragma unroll
-			for (int k = 0; k < kAdjBurst; k++) {
-				if (k < cnt) {
-					uint4 *pr = reinterpret_cast<uint4 *>(Bk.srec + heldDst[k]);
-					pr[0] = held[k][0];
-					pr[1] = held[k][1];
+			for (int k = L - 1; k >= 0; k--) {
+				if (k < len) {
+					const V3 aT = adj_reverse_stored(A.G, A.h, kq, d[k], ax, au, gs);
+					const uint32_t t = d[k].rec[0] >> (3 * S);
+					uint4 *pr = reinterpret_cast<uint4 *>(Bk.srec + base[t] + atomicAdd(&rank[t], 1u));
+					pr[0] = make_uint4(d[k].rec[0], d[k].rec[1], d[k].rec[2], d[k].rec[3]);
+					pr[1] = make_uint4(__float_as_uint((float)aT.x), __float_as_uint((float)aT.y),
+					                   __float_as_uint((float)aT.z), 0u);
 				}
 			}
-			q -= cnt;
 		}
 		if (Bk.last) {
 			A.gradX0[3 * i] = ax.x; A.gradX0[3 * i + 1] = ax.y; A.gradX0[3 * i + 2] = ax.z;

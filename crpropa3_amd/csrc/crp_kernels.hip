@@ -2438,7 +2438,8 @@ static int adj_args(const crp_adjoint_config *cfg, size_t n, AdjArgs &A) {
 // when the grid has more than kAdjMaxTiles of those; grids with more than kAdjMaxTiles tiles of 16^3
 // cells use the checkpointed backward with direct atomics.
 constexpr int kAdjMaxTiles = 8192;
-constexpr size_t kAdjBlockBudget = (size_t)4 << 30; // bytes of one buffer set (stored values, records)
+constexpr size_t kAdjBlockBudget = (size_t)4 << 30; // bytes of one buffer set (sub-checkpoints, records)
+constexpr int kAdjMinBlocks = 8;
 
 struct AdjPlan {
 	bool blocked;
@@ -2447,7 +2448,7 @@ struct AdjPlan {
 	int ntx, nty, ntz, ntiles;
 	size_t maxRecords; // records of the largest block
 	// byte offsets into the scratch
-	size_t oCarry, oStates[2], oRec0[2], oSRec[2], oCounts[2], oTileStart[2], oWgStart[2], total;
+	size_t oCarry, oSub[2], oSRec[2], oCounts[2], oTileStart[2], oWgStart[2], total;
 };
 
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -2471,8 +2472,8 @@ static AdjPlan adj_plan(const crp_adjoint_config *cfg, size_t n) {
 		if (tiles <= kAdjMaxTiles)
 			break;
 	}
-	// per segment and buffer set: stored values (per padded lane and step) and both record copies (48 B)
-	const double perSeg = (double)K * ((double)P.npg * kAdjGroup * (kAdjStepBytes / 64) + (double)n * 48.0);
+	// per segment and buffer set: the sub-checkpoints (48 B per kAdjSub steps) and the records (32 B a step)
+	const double perSeg = (double)K * (double)n * (48.0 / kAdjSub + 32.0);
 	// CRP_ADJOINT_BLOCK_BYTES overrides the buffer-set budget (0: per-step atomics), so that tests
 	// can run the multi-block and the atomic paths on small inputs
 	double budget = (double)kAdjBlockBudget;
@@ -2484,7 +2485,10 @@ static AdjPlan adj_plan(const crp_adjoint_config *cfg, size_t n) {
 		return P;
 	}
 	P.ntiles = (int)tiles;
-	P.segPerBlock = (int)std::max(1.0, std::min((double)P.nseg, std::floor(budget / perSeg)));
+	// at least kAdjMinBlocks blocks where there are that many segments, so that the three streams
+	// overlap over most of the backward
+	const double cap = std::ceil((double)P.nseg / kAdjMinBlocks);
+	P.segPerBlock = (int)std::max(1.0, std::min(cap, std::floor(budget / perSeg)));
 	while ((double)P.segPerBlock * K * n >= 2e9 && P.segPerBlock > 1)
 		P.segPerBlock--;
 	P.nblocks = (P.nseg + P.segPerBlock - 1) / P.segPerBlock;
@@ -2493,8 +2497,7 @@ static AdjPlan adj_plan(const crp_adjoint_config *cfg, size_t n) {
 	size_t o = 0;
 	P.oCarry = o;      o += align256(6 * n * sizeof(double));
 	for (int k = 0; k < 2; k++) {
-		P.oStates[k] = o;     o += align256((size_t)P.segPerBlock * K * P.npg * (kAdjGroup / 64) * kAdjStepBytes);
-		P.oRec0[k] = o;       o += align256(P.maxRecords * sizeof(uint4));
+		P.oSub[k] = o;        o += align256(((size_t)P.segPerBlock * K + kAdjSub - 1) / kAdjSub * 6 * n * sizeof(double));
 		P.oSRec[k] = o;       o += align256(P.maxRecords * sizeof(DepRec));
 		P.oCounts[k] = o;     o += align256((size_t)P.npg * P.ntiles * sizeof(uint32_t));
 		P.oTileStart[k] = o;  o += align256(((size_t)P.ntiles + 1) * sizeof(uint32_t));
@@ -2630,7 +2633,8 @@ int crp_adjoint_backward(const crp_adjoint_config *cfg, size_t n, const double *
 	}
 	// stream A (the caller's) runs the reverse sweeps, block after block; stream B the recompute of
 	// the block after (in backward order), stream C the accumulation of the block before: the three
-	// are bound by latency, memory traffic and LDS atomics respectively
+	// are bound by FP64 issue at one wave per SIMD, the random-line rate of the cell gathers and LDS
+	// atomics respectively
 	hipStream_t sb, sc;
 	CRP_HIP_TRY(adj_side_streams(&sb, &sc));
 	AdjEvents E;
@@ -2652,8 +2656,7 @@ int crp_adjoint_backward(const crp_adjoint_config *cfg, size_t n, const double *
 		std::memset(&Bk, 0, sizeof(Bk));
 		const int buf = b & 1;
 		Bk.carry = reinterpret_cast<double *>(base + P.oCarry);
-		Bk.states = base + P.oStates[buf];
-		Bk.rec0 = reinterpret_cast<uint4 *>(base + P.oRec0[buf]);
+		Bk.sub = reinterpret_cast<double *>(base + P.oSub[buf]);
 		Bk.srec = reinterpret_cast<DepRec *>(base + P.oSRec[buf]);
 		Bk.counts = reinterpret_cast<uint32_t *>(base + P.oCounts[buf]);
 		Bk.tileStart = reinterpret_cast<uint32_t *>(base + P.oTileStart[buf]);
@@ -2673,48 +2676,51 @@ int crp_adjoint_backward(const crp_adjoint_config *cfg, size_t n, const double *
 		return Bk;
 	};
 	const size_t T = (size_t)P.ntiles;
-	// recompute of block b with the first half of its records, their counts per particle group and
-	// tile, and the scans
-	auto recompute = [&](int b) {
+	// recompute of block b: its sub-checkpoints, the records' counts per particle group and tile, and
+	// the scans
+	auto recompute = [&](int b, hipStream_t st) {
 		const AdjBlock Bk = block(b);
 		const size_t nc = (size_t)Bk.npg * T;
-		hipLaunchKernelGGL(k_dep_zero, dim3((unsigned)((nc + kAdjGroup - 1) / kAdjGroup)), dim3(kAdjGroup), 0, sb, Bk);
+		hipLaunchKernelGGL(k_dep_zero, dim3((unsigned)((nc + kAdjGroup - 1) / kAdjGroup)), dim3(kAdjGroup), 0, st, Bk);
 		const dim3 grid((unsigned)((size_t)Bk.nseg * Bk.npg));
 		if (P.tileShift == 3)
-			hipLaunchKernelGGL(k_adj_recompute<3>, grid, dim3(kAdjGroup), T * sizeof(uint32_t), sb, A, Bk);
+			hipLaunchKernelGGL((k_adj_recompute<3, kAdjSub>), grid, dim3(kAdjGroup), T * sizeof(uint32_t), st, A, Bk);
 		else
-			hipLaunchKernelGGL(k_adj_recompute<4>, grid, dim3(kAdjGroup), T * sizeof(uint32_t), sb, A, Bk);
-		hipLaunchKernelGGL(k_dep_scan_chunks, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, sb, Bk);
-		hipLaunchKernelGGL(k_dep_scan_tiles, dim3(1), dim3(1024), 0, sb, Bk);
+			hipLaunchKernelGGL((k_adj_recompute<4, kAdjSub>), grid, dim3(kAdjGroup), T * sizeof(uint32_t), st, A, Bk);
+		hipLaunchKernelGGL(k_dep_scan_chunks, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, st, Bk);
+		hipLaunchKernelGGL(k_dep_scan_tiles, dim3(1), dim3(1024), 0, st, Bk);
 	};
-	auto accumulate = [&](int b) {
+	auto reverse = [&](int b, hipStream_t st) {
+		const AdjBlock Bk = block(b);
+		const dim3 g((unsigned)Bk.npg), w(kAdjGroup);
+		const size_t lb = 2 * T * sizeof(uint32_t);
+		if (P.tileShift == 3)
+			hipLaunchKernelGGL((k_adj_reverse<3, kAdjSub>), g, w, lb, st, A, Bk);
+		else
+			hipLaunchKernelGGL((k_adj_reverse<4, kAdjSub>), g, w, lb, st, A, Bk);
+	};
+	auto accumulate = [&](int b, hipStream_t st) {
 		const AdjBlock Bk = block(b);
 		const size_t groups = T + (Bk.m + kDepPerGroup - 1) / kDepPerGroup;
 		if (P.tileShift == 3)
-			hipLaunchKernelGGL(k_dep_tiles<3>, dim3((unsigned)groups), dim3(Tile<3>::threads), 0, sc, A, Bk);
+			hipLaunchKernelGGL(k_dep_tiles<3>, dim3((unsigned)groups), dim3(Tile<3>::threads), 0, st, A, Bk);
 		else
-			hipLaunchKernelGGL(k_dep_tiles<4>, dim3((unsigned)groups), dim3(Tile<4>::threads), 0, sc, A, Bk);
+			hipLaunchKernelGGL(k_dep_tiles<4>, dim3((unsigned)groups), dim3(Tile<4>::threads), 0, st, A, Bk);
 	};
 
 	// streams B and C start after the work already queued on stream A (e.g. the zeroed gradients)
 	CRP_HIP_TRY(hipEventRecord(start, sa));
 	CRP_HIP_TRY(hipStreamWaitEvent(sb, start, 0));
 	CRP_HIP_TRY(hipStreamWaitEvent(sc, start, 0));
-	recompute(NB - 1);
+	recompute(NB - 1, sb);
 	CRP_HIP_TRY(hipEventRecord(recomputed[NB - 1], sb));
 	for (int b = NB - 1; b >= 0; b--) {
-		// A: reverse sweep of block b once its stored values and counts are there and the accumulation
+		// A: reverse sweep of block b once its sub-checkpoints and counts are there and the accumulation
 		// of block b + 2 (the same record buffer) has read its records
 		CRP_HIP_TRY(hipStreamWaitEvent(sa, recomputed[b], 0));
 		if (b + 2 < NB)
 			CRP_HIP_TRY(hipStreamWaitEvent(sa, sorted[b + 2], 0));
-		{
-			const AdjBlock Bk = block(b);
-			if (P.tileShift == 3)
-				hipLaunchKernelGGL(k_adj_reverse<3>, dim3((unsigned)Bk.npg), dim3(kAdjGroup), 2 * T * sizeof(uint32_t), sa, A, Bk);
-			else
-				hipLaunchKernelGGL(k_adj_reverse<4>, dim3((unsigned)Bk.npg), dim3(kAdjGroup), 2 * T * sizeof(uint32_t), sa, A, Bk);
-		}
+		reverse(b, sa);
 		CRP_HIP_TRY(hipEventRecord(reversed[b], sa));
 		// B: recompute of block b - 1 into the buffers that the reverse sweep of block b + 1 has
 		// finished with (its counts were last read by the accumulation of block b + 1)
@@ -2723,12 +2729,12 @@ int crp_adjoint_backward(const crp_adjoint_config *cfg, size_t n, const double *
 				CRP_HIP_TRY(hipStreamWaitEvent(sb, reversed[b + 1], 0));
 				CRP_HIP_TRY(hipStreamWaitEvent(sb, sorted[b + 1], 0));
 			}
-			recompute(b - 1);
+			recompute(b - 1, sb);
 			CRP_HIP_TRY(hipEventRecord(recomputed[b - 1], sb));
 		}
 		// C: accumulation of block b
 		CRP_HIP_TRY(hipStreamWaitEvent(sc, reversed[b], 0));
-		accumulate(b);
+		accumulate(b, sc);
 		CRP_HIP_TRY(hipEventRecord(sorted[b], sc));
 	}
 	CRP_HIP_TRY(hipGetLastError());

@@ -417,9 +417,9 @@ typedef struct {
  * builds and the backward reads) and the backward's scratch.
  * For grids of at most 8192 tiles of 8^3 cells (or of 16^3 cells, up to about 320^3 points) the
  * backward recomputes several segments of every
- * particle at once and sorts one deposit record per step by tile before accumulating the grid
- * gradient tile by tile in LDS; its scratch (about 4 GiB at most, less for small batches) holds those
- * states and records.  Larger grids use per-step atomics and K * 6 * n doubles of scratch.  The
+ * particle at once, keeping the state every 8 steps, and sorts one deposit record per step by tile
+ * before accumulating the grid gradient tile by tile in LDS; its scratch (two buffer sets of at most
+ * 4 GiB each, less for small batches) holds those states and records.  Larger grids use per-step atomics and K * 6 * n doubles of scratch.  The
  * environment variable CRP_ADJOINT_TILE_SHIFT=4 selects 16^3-cell tiles where 8^3 would do (tests).  The
  * environment variable CRP_ADJOINT_BLOCK_BYTES, read at each call, overrides the per-block budget
  * in bytes (0 selects the per-step atomics). */

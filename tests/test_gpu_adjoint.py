@@ -167,12 +167,14 @@ def test_adjoint_rejects_bad_input():
         boris_grid_propagate(g.double(), x0, u0, 1 * O.EeV, O.eplus, [0, 0, 0], [s["sp"]] * 3, s["h"], 10, 4)
 
 
-@pytest.mark.parametrize("budget,shift", [(None, None), ("3e5", None), ("0", None), (None, "4"), ("3e5", "4")])
+@pytest.mark.parametrize("budget,shift", [(None, None), ("3e4", None), ("0", None), (None, "4"), ("3e4", "4")])
 def test_adjoint_backward_paths(budget, shift, monkeypatch):
     """The backward's three schedules against the oracle on a 20^3 grid (tiles cut by the grid's end):
-    all segments in one block (default), several blocks of segments with the adjoint carried between
-    them (small per-block budget), and per-step atomics (budget 0); the blocked ones with 8^3-cell tiles
-    and with the 16^3-cell tiles of large grids."""
+    the default blocks (17 segments of 8 steps in 6 blocks of at most 3, each recomputed in runs of 8
+    steps from sub-checkpoints), one segment per block with the adjoint carried between 17 blocks (a
+    small per-block budget), and per-step atomics (budget 0); the blocked ones with 8^3-cell tiles and
+    with the 16^3-cell tiles of large grids.  (One block in all: nsteps = K in
+    test_adjoint_checkpoint_interval_does_not_change_results.)"""
     if budget is not None:
         monkeypatch.setenv("CRP_ADJOINT_BLOCK_BYTES", budget)
     if shift is not None:
