@@ -15,6 +15,11 @@
 // Gradients flow to the grid values G (scatter-add, FP64 atomics), to the
 // initial positions and directions, and to the field scale s.
 //
+// The model's arithmetic is its own (compared with the oracle's FP64 gradient within a tolerance, not
+// bit for bit), so its functions contract a*b + c into fused multiply-adds: `#pragma clang fp
+// contract(on)` at the top of their bodies (the translation unit is built with -ffp-contract=off, which
+// the propagation kernels need).  The contraction is decided per expression in the source, so the
+// forward and every recompute produce the same bits.
 // Forward stores the state every K steps (checkpoints); backward walks the
 // segments from last to first, recomputes the segment's K states into a
 // per-particle scratch area, and applies the reverse step K times.
@@ -97,6 +102,7 @@ __device__ __forceinline__ void adj_gather(const AdjGrid &G, const TriPoint &tp,
 }
 
 __device__ __forceinline__ V3 adj_interp(const AdjGrid &G, const TriPoint &tp) {
+#pragma clang fp contract(on)
 	float v[24];
 	adj_gather(G, tp, v);
 	double b[3] = {0, 0, 0};
@@ -114,13 +120,18 @@ __device__ __forceinline__ V3 adj_interp(const AdjGrid &G, const TriPoint &tp) {
 __device__ __forceinline__ V3 vadd(const V3 &a, const V3 &b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
 __device__ __forceinline__ V3 vsub(const V3 &a, const V3 &b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
 __device__ __forceinline__ V3 vmul(const V3 &a, double s) { return v3(a.x * s, a.y * s, a.z * s); }
-__device__ __forceinline__ double vdot(const V3 &a, const V3 &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ double vdot(const V3 &a, const V3 &b) {
+#pragma clang fp contract(on)
+	return a.x * b.x + a.y * b.y + a.z * b.z;
+}
 __device__ __forceinline__ V3 vcross(const V3 &a, const V3 &b) {
+#pragma clang fp contract(on)
 	return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 
 // the step after the interpolation: T = trilinear(grid) at the half-step position x1
 __device__ __forceinline__ void adj_push(const AdjGrid &G, double h, double kq, const V3 &T, const V3 &x1, V3 &x, V3 &u) {
+#pragma clang fp contract(on)
 	const V3 B = vmul(T, G.scale);
 	const V3 t = vmul(B, kq);
 	const double D = 1 + vdot(t, t);
@@ -146,6 +157,7 @@ __device__ __forceinline__ void adj_step(const AdjGrid &G, double h, double kq, 
 // adjoint after it on entry and before it on return.
 __device__ __forceinline__ void adj_step_reverse(const AdjGrid &G, double h, double kq, const V3 &x, const V3 &u, V3 &ax,
                                                  V3 &au, double *__restrict__ gradG, double &gradScale) {
+#pragma clang fp contract(on)
 	// recompute the forward intermediates
 	const V3 x1 = vadd(x, vmul(u, h / 2));
 	TriPoint tp;
@@ -480,6 +492,7 @@ struct AdjStored {
 template <int S>
 __device__ __forceinline__ void adj_stored(const AdjBlock &Bk, const TriPoint &tp, const float cv[24], const V3 &u,
                                            AdjStored &d) {
+#pragma clang fp contract(on)
 	double b[3] = {0, 0, 0}, jx[3] = {0, 0, 0}, jy[3] = {0, 0, 0}, jz[3] = {0, 0, 0};
 #pragma unroll
 	for (int k = 0; k < 8; k++) {
@@ -506,6 +519,7 @@ __device__ __forceinline__ void adj_stored(const AdjBlock &Bk, const TriPoint &t
 // it on return; returns aT
 __device__ __forceinline__ V3 adj_reverse_stored(const AdjGrid &G, double h, double kq, const AdjStored &d, V3 &ax,
                                                  V3 &au, double &gradScale) {
+#pragma clang fp contract(on)
 	const V3 u = v3(d.d[0], d.d[1], d.d[2]);
 	const V3 T = v3(d.d[3], d.d[4], d.d[5]);
 	const V3 B = vmul(T, G.scale);
