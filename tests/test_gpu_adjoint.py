@@ -190,6 +190,21 @@ def test_adjoint_backward_paths(budget, shift, monkeypatch):
     assert got[6] == pytest.approx(ref[6], rel=1e-8)
 
 
+def test_adjoint_several_particle_groups():
+    """600 particles: two full groups of 256 and a partial third one (their tile counts, offsets and
+    carried adjoints kept apart), 9 segments of 13 steps (the last of 7) in 5 blocks: runs of 8, 8, 8
+    and 2 steps that cross the segments' ends, and one run of 7 in the last block; against the oracle."""
+    s = setup(n=600, N=16, nsteps=111, K=13, seed=21)
+    ref = oracle_grad(s)
+    got = gpu_run(s)
+    rel = np.abs(got[3] - ref[3]).max() / np.abs(ref[3]).max()
+    print("600 particles: grid-gradient max rel %.2e" % rel)
+    assert rel < 1e-6
+    assert np.allclose(got[4], ref[4], rtol=1e-7, atol=1e-8 * np.abs(ref[4]).max())
+    assert np.allclose(got[5], ref[5], rtol=1e-7, atol=1e-8 * np.abs(ref[5]).max())
+    assert got[6] == pytest.approx(ref[6], rel=1e-8)
+
+
 def test_adjoint_large_grid_uses_16_cell_tiles():
     """A 176^3 grid has 10648 tiles of 8^3 cells, more than the blocked backward takes: it runs with
     16^3-cell tiles (1331) and matches the oracle."""
