@@ -479,6 +479,8 @@ def main():
                     help="device storage of the grid (results are identical)")
     ap.add_argument("--grid-storage", default="fp32", choices=["fp32", "fp16"],
                     help="grid values read by the mixed path: FP32, or FP16 with a per-8^3-brick FP32 scale")
+    ap.add_argument("--interpolation", default="trilinear", choices=["trilinear", "tricubic", "nearest"],
+                    help="Grid::setInterpolationType of a grid workload (tricubic and nearest read the points)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision block of the default workload")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint block of the default workload")
@@ -504,7 +506,15 @@ def main():
         field = capi.Field.jf12(striated=fdata["striated"], striated_spacing=0.1 * kpc, turbulent=fdata["turbulent"],
                                 turbulent_spacing=4 * pc)
     elif w["field"] == "grid":
-        field = capi.Field.grid3f(fdata, [0, 0, 0], [w["spacing"]] * 3).set_layout(args.grid_layout)
+        field = capi.Field.grid3f(fdata, [0, 0, 0], [w["spacing"]] * 3)
+        if args.interpolation != "trilinear":
+            if args.grid_storage == "fp16":
+                sys.exit("bench.py: --interpolation %s reads the grid's FP32 points (no FP16 storage)"
+                         % args.interpolation)
+            args.grid_layout = "points"  # the cell table holds trilinear corners
+            field.set_interpolation({"tricubic": "TRICUBIC", "nearest": "NEAREST_NEIGHBOUR"}[args.interpolation])
+        else:
+            field.set_layout(args.grid_layout)
         if args.grid_storage == "fp16":
             if args.precision != "mixed":
                 sys.exit("bench.py: --grid-storage fp16 is read by the mixed path only (use --precision mixed)")
@@ -551,13 +561,18 @@ def main():
         fp16 = args.grid_storage == "fp16"
         config.update({"grid": "%d^3 %s, spacing %.0f kpc" % (w["grid_n"], "FP16+brick scale" if fp16 else "FP32",
                                                               w["spacing"] / kpc),
-                       "grid_layout": "cells16" if fp16 else args.grid_layout})
-        bytes_per_launch = evals * (BYTES_PER_EVAL_FP16 if fp16 else BYTES_PER_EVAL) / launches
+                       "grid_layout": "cells16" if fp16 else args.grid_layout, "interpolation": args.interpolation})
+        # bytes one evaluation reads: 8 corners (trilinear), 4^3 points (tricubic) or 1 point (nearest), 3 x FP32
+        per_eval = {"trilinear": BYTES_PER_EVAL_FP16 if fp16 else BYTES_PER_EVAL, "tricubic": 64 * 12,
+                    "nearest": 12}[args.interpolation]
+        bytes_per_launch = evals * per_eval / launches
         achieved = bytes_per_launch / kern_s / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(args.workload, args.precision),
+                    "frac": achieved / HBM_PEAK_GBS,
+                    "traffic": read_traffic(args.workload, args.precision) if args.interpolation == "trilinear" else None,
                     "kernel": ("k_sde<%s>" if w["kind"] == "SDE" else "k_propagate<%s, " + w["kind"] + ">") % (
-                        "FieldGridCells16" if fp16 else ("FieldGridCells" if args.grid_layout == "cells" else "FieldGrid")),
+                        "FieldGridAny" if args.interpolation != "trilinear" else "FieldGridCells16" if fp16 else
+                        ("FieldGridCells" if args.grid_layout == "cells" else "FieldGrid")),
                     "kernel_ms_avg": kern_s * 1e3, "algorithmic_bytes_per_launch": bytes_per_launch}
         if args.grid_layout == "cells":
             # every evaluation reads one random line of the cell table; the GPU's rate of such reads,
@@ -607,7 +622,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": None,
     }
-    headline = args.workload == "ck_grid256" and args.precision == "fp64" and args.grid_layout == "cells"
+    headline = (args.workload == "ck_grid256" and args.precision == "fp64" and args.grid_layout == "cells"
+                and args.interpolation == "trilinear")
     if headline and not args.no_mixed:
         # the metric's second half: the mixed path on the same particles, error against this FP64 pass
         out["mixed"] = mixed_block(capi, w, fdata, field, init, work, pos, final, acc / dt, args)
@@ -619,7 +635,7 @@ def main():
                                             "forward_ms", "backward_ms", "adjoint_over_forward",
                                             "gradient_over_forward", "propagate_bp_fp64_ms",
                                             "adjoint_over_propagate_bp", "value", "unit")}
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.interpolation == "trilinear":
         out["cpu_baseline"] = cpu_baseline(w, fdata, pos, d, E, q, gpu_final=final)
     print(json.dumps(out), flush=True)
 
