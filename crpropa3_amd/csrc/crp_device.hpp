@@ -351,19 +351,26 @@ __device__ inline float scalar_grid_tricubic(const GridDesc &g, const V3 &p) {
 }
 
 // Grid::interpolate (Grid.h:329-332) for the tricubic and nearest-neighbour types, after the clip
-// test; out of line, one copy for every kernel that reads such grids
-__device__ __attribute__((noinline)) V3 grid_eval_other(const GridDesc &g, const V3 &p) {
+// test: inline in the kernels of FieldGridAny (which read nothing else), out of line for the field
+// types that read such grids beside others (JF12's turbulent grid, decorator leaves)
+__device__ __forceinline__ V3 grid_eval_other_inline(const GridDesc &g, const V3 &p) {
 	if (g.interp == kInterpNearest) {
 		const float *c = g.values + 3 * grid_closest(g, p);
 		return v3((double)c[0], (double)c[1], (double)c[2]);
 	}
 	return grid_tricubic(g, p);
 }
+__device__ __attribute__((noinline)) V3 grid_eval_other(const GridDesc &g, const V3 &p) {
+	return grid_eval_other_inline(g, p);
+}
 
-__device__ __attribute__((noinline)) float scalar_grid_other(const GridDesc &g, const V3 &p) {
+__device__ __forceinline__ float scalar_grid_other_inline(const GridDesc &g, const V3 &p) {
 	if (g.interp == kInterpNearest)
 		return g.values[grid_closest(g, p)];
 	return scalar_grid_tricubic(g, p);
+}
+__device__ __attribute__((noinline)) float scalar_grid_other(const GridDesc &g, const V3 &p) {
+	return scalar_grid_other_inline(g, p);
 }
 
 struct FieldGrid {
@@ -686,21 +693,23 @@ __device__ __forceinline__ bool in_clip_box(const GridDesc &g, const V3 &p) {
 }
 
 // Grid::interpolate (include/crpropa/Grid.h:318-335) for every interpolation type: trilinear in
-// line, tricubic and nearest neighbour out of line after the clip test
+// line, tricubic and nearest neighbour after the clip test (in line when Inline, else out of line)
+template <bool Inline = false>
 __device__ __forceinline__ V3 grid_interpolate(const GridDesc &g, const V3 &p) {
 	if (g.interp == kInterpTrilinear)
 		return FieldGrid{g}.eval(p);
 	if (g.clip && !in_clip_box(g, p))
 		return v3(0.0, 0.0, 0.0);
-	return grid_eval_other(g, p);
+	return Inline ? grid_eval_other_inline(g, p) : grid_eval_other(g, p);
 }
 
+template <bool Inline = false>
 __device__ __forceinline__ float scalar_grid_interpolate(const GridDesc &g, const V3 &p) {
 	if (g.interp == kInterpTrilinear)
 		return scalar_grid_eval(g, p);
 	if (g.clip && !in_clip_box(g, p))
 		return 0.f;
-	return scalar_grid_other(g, p);
+	return Inline ? scalar_grid_other_inline(g, p) : scalar_grid_other(g, p);
 }
 
 // MagneticFieldGrid and ModulatedMagneticFieldGrid (src/magneticField/MagneticFieldGrid.cpp:5-55)
@@ -714,10 +723,10 @@ struct FieldGridAny {
 	int modulated;
 
 	__device__ __forceinline__ V3 eval(const V3 &p) const {
-		const V3 b = grid_interpolate(g, p);
+		const V3 b = grid_interpolate<true>(g, p);
 		if (!modulated)
 			return b;
-		const double m = (double)scalar_grid_interpolate(mod, p);
+		const double m = (double)scalar_grid_interpolate<true>(mod, p);
 		return v3(b.x * m, b.y * m, b.z * m);
 	}
 	__device__ __forceinline__ float3 evalf(const V3 &p) const {
@@ -725,12 +734,12 @@ struct FieldGridAny {
 		if (g.interp == kInterpTrilinear) {
 			b = FieldGrid{g}.evalf(p);
 		} else {
-			const V3 d = grid_interpolate(g, p);
+			const V3 d = grid_interpolate<true>(g, p);
 			b = make_float3((float)d.x, (float)d.y, (float)d.z);
 		}
 		if (!modulated)
 			return b;
-		const float m = mod.interp == kInterpTrilinear ? scalar_grid_evalf(mod, p) : scalar_grid_interpolate(mod, p);
+		const float m = mod.interp == kInterpTrilinear ? scalar_grid_evalf(mod, p) : scalar_grid_interpolate<true>(mod, p);
 		return make_float3(b.x * m, b.y * m, b.z * m);
 	}
 };
